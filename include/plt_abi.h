@@ -1,0 +1,137 @@
+/* plt_abi.h — C ABI of the MI355X-native PLT generalized-ray path-tracing hot path.
+ *
+ * Boundary: this library replaces the device side of the reference's PLT material interface
+ * (Source/Falcor/Rendering/Materials/IMaterialInstance.slang:122-184: eval / sample / pdf /
+ * lambdaPdf) and the PLTPT render pass's GPU work (Source/RenderPasses/PLTPT/PLTPT.cpp:357-499,
+ * two DXR dispatches per tile). Host callers pass plain device pointers, sizes and a
+ * hipStream_t (as void*). No C++ or torch types cross this boundary, no exceptions escape it:
+ * every entry point returns a plt_status and plt_last_error() describes the last failure of
+ * the calling thread.
+ *
+ * Units follow the reference (Source/Falcor/Rendering/PLT.slang:80-82): wavelengths in nm,
+ * wavenumbers in 1/um, beam distances in m, coherence in mm, surface heights/lengths in um.
+ */
+#ifndef PLT_ABI_H
+#define PLT_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct plt_context plt_context;
+typedef void* plt_stream; /* hipStream_t; NULL = the default (null) stream */
+
+/* Status codes. Replaces the reference's C++ exceptions / logError
+ * (PLTPT.cpp:173, :409; Core/Errors.h). */
+typedef enum plt_status {
+    PLT_OK = 0,
+    PLT_WARN_UNKNOWN_KEY = 1, /* PLTPT.cpp:173 logs an error and continues; so do we */
+    PLT_EINVAL = -1,
+    PLT_ENOMEM = -2,
+    PLT_EHIP = -3,
+    PLT_ENOTIMPL = -4,
+    PLT_ESTATE = -5
+} plt_status;
+
+/* Thread-local description of the last failure (empty string if none). */
+const char* plt_last_error(void);
+/* ABI version: major*10000 + minor*100 + patch. */
+int plt_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Context. One device, one process; not thread-safe (external synchronisation), as the
+ * reference's single render thread (PLTPT.cpp:487-495).
+ * ------------------------------------------------------------------------------------- */
+int plt_context_create(int device, plt_context** out);
+int plt_context_destroy(plt_context* ctx);
+
+/* ---------------------------------------------------------------------------------------
+ * Spectral profiles. Layout of Source/Falcor/Scene/Material/BasicMaterialData.slang:43-54
+ * (SpectralProfile, kMaxBins = 128), evaluated as Scene::evalSpectrum (Scene/Scene.slang:251-295).
+ * ------------------------------------------------------------------------------------- */
+#define PLT_SPECTRAL_PROFILE_MAX_BINS 128
+typedef struct plt_spectral_profile {
+    float rgb[3];
+    float intensity;
+    uint32_t bins;
+    float min_wavelength;
+    float max_wavelength;
+    float pdf[PLT_SPECTRAL_PROFILE_MAX_BINS];
+    float icdf[PLT_SPECTRAL_PROFILE_MAX_BINS];
+} plt_spectral_profile;
+
+/* Host-side construction of a profile from a tabulated spectrum (npts >= 2, wavelengths
+ * non-decreasing): PiecewiseLinearSpectrum -> SampledSpectrum (5 nm bins) -> genSpectralProfile
+ * (Utils/Color/Spectrum.h:88-123, Utils/Color/SampledSpectrum.h:64-74, Scene/SceneBuilder.cpp:911-960).
+ * compute_icdf != 0 normalises pdf and fills icdf (emission spectra); 0 keeps raw values (n, k). */
+int plt_spectral_profile_build(uint32_t npts, const float* wavelengths, const float* values, int compute_icdf,
+                               plt_spectral_profile* out);
+/* Uploads `count` host profiles to the device; profile ids are their indices. */
+int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profile* profiles, uint32_t count);
+
+/* ---------------------------------------------------------------------------------------
+ * Sample generator (Utils/Sampling/UniformSampleGenerator.slang:51-69: SplitMix64 seeding of
+ * xoshiro128**, Utils/Sampling/Pseudorandom/Xoshiro.slang:52-66). Writes, for stream s in
+ * [0, n_streams) with pixel = (s % width, s / width), `draws` raw uint32 values to
+ * out[d * n_streams + s]. Device pointer `out`.
+ * ------------------------------------------------------------------------------------- */
+int plt_sample_generator_stream(uint32_t n_streams, uint32_t width, uint32_t sample_number, uint32_t draws,
+                                uint32_t* out, plt_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched BSDF queries (the IMaterialInstance eval + pdf pair that next-event estimation runs,
+ * Source/RenderPasses/PLTPT/pltpt_solve.rt.slang:219-243 and :289-292).
+ *
+ * A batch of n queries is PLT_Q_PLANES structure-of-arrays planes of n 32-bit words
+ * (plane p of query i at word p*n + i). Directions are in the local shading frame (N = +z):
+ * wi = sd.V = direction towards the light (the beam arrives along -wi), wo = direction
+ * towards the eye. The incident beam carries one wavelength with Stokes (S0..S3) and an
+ * isotropic coherence matrix whose coherence area at the query wavelength equals
+ * PLT_Q_COH_AREA (um^2) at 1 mm from its source. Evaluation runs on the NonDelta lobe.
+ *
+ * PLT_Q_MAT   : bits 31..28 material type (plt_material_type); for conductors bits 13..0 the
+ *               n profile id and bits 27..14 the k profile id.
+ * PLT_Q_PARAM : two IEEE binary16 values, as the reference stores material parameters
+ *               (BasicMaterialData.slang:70-83). Dielectric: lo = IoR, hi = dispersion B
+ *               (Cauchy, PLTDielectricMaterial.cpp:87-101). Conductor: lo = exterior IoR,
+ *               hi = PSD exponent gamma (PLTConductorMaterial.cpp:54-58).
+ * PLT_Q_ROUGH_Q : RMS surface height q (um), the coherent/specular split
+ *               alpha = exp(-((|cos_i|+|cos_o|) q k)^2) (FractalSurface.slang:77-88).
+ * PLT_Q_ROUGH_L : correlation length l (um): Gaussian PSD sigma2 = l^2 (GaussianSurface.slang:97-109),
+ *               K-correlation T = min(l^2, 5000) (FractalSurface.slang:44-51,92-95).
+ *
+ * Results: PLT_R_PLANES planes of n floats: outgoing Stokes S0..S3 and pdf(wi -> wo).
+ * ------------------------------------------------------------------------------------- */
+enum {
+    PLT_Q_WI_X = 0, PLT_Q_WI_Y, PLT_Q_WI_Z,
+    PLT_Q_WO_X, PLT_Q_WO_Y, PLT_Q_WO_Z,
+    PLT_Q_LAMBDA, PLT_Q_MAT, PLT_Q_PARAM,
+    PLT_Q_ROUGH_Q, PLT_Q_ROUGH_L, PLT_Q_COH_AREA,
+    PLT_Q_S0, PLT_Q_S1, PLT_Q_S2, PLT_Q_S3,
+    PLT_Q_PLANES
+};
+enum { PLT_R_S0 = 0, PLT_R_S1, PLT_R_S2, PLT_R_S3, PLT_R_PDF, PLT_R_PLANES };
+
+typedef enum plt_material_type {
+    PLT_MAT_DIELECTRIC = 0, /* PLTDielectricBSDF.slang:43-361 (Gaussian PSD) */
+    PLT_MAT_CONDUCTOR = 1   /* PLTConductorBSDF.slang:44-242 (K-correlation PSD, tabulated n,k) */
+} plt_material_type;
+
+/* Synthetic batch: query i draws 14 numbers from UniformSampleGenerator(pixel=(i & 0xFFFF,
+ * i >> 16), sample_number=seed): type (u<0.5: dielectric), IoR U[1.3,1.8] (binary16, Abbe 100
+ * dispersion), q U[0.01,0.3] um, l U[1,20] um, coherence area U[1,100] um^2, wi and wo
+ * cosine-hemisphere (concentric), transmission side for dielectrics (u<0.5 flips wo.z),
+ * wavelength U[380,780] nm, Stokes direction uniform on the Poincare sphere with degree of
+ * polarisation U[0,1]. Conductors use profile ids 0 (n) and 1 (k). Device pointer `queries`. */
+int plt_bsdf_generate_queries(plt_context* ctx, uint32_t n, uint32_t seed, uint32_t* queries, plt_stream stream);
+
+/* Evaluates a batch (device pointers). Material-divergent batches are accepted as is. */
+int plt_bsdf_eval_queries(plt_context* ctx, uint32_t n, const uint32_t* queries, float* results, plt_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLT_ABI_H */

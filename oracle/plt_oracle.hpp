@@ -1,0 +1,930 @@
+// PLT oracle: CPU restatement of the reference's PLT material / beam / sampling code.
+//
+// TEST INFRASTRUCTURE ONLY. This header is the checker for the HIP product path in
+// pltfalcor_amd/. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+// load it (through liboracle.so). The product never links, calls or falls back to it.
+//
+// Every function restates the reference's Slang/C++ text line by line, templated on the
+// scalar type T: T=float is the FP32 parity reference (strict IEEE, no fast-math, libm
+// transcendentals), T=double is the FP64 error oracle. Citations are file:line in the
+// reference tree (Source/Falcor/...).
+//
+// Parity pinning (what checks each group, besides GPU-vs-oracle agreement):
+//   PRNG / interleave      : known-answer vectors from Tests/Utils/BitTricksTests.cpp and the
+//                            published SplitMix64 / xoshiro128** reference outputs (tests/).
+//   Fresnel / Mueller      : closed forms (normal incidence, TIR, eta=1, Born&Wolf complex
+//                            Fresnel in numpy complex128), energy Rs,Rp<=1, Mueller invariants.
+//   MuellerT               : preserves S0,S3 and S1^2+S2^2; the theta=0 quirk (S1,S2)->(-S2,S1).
+//   Gaussian/Fractal PSD   : normalisation integrals (2-D quadrature), sample/pdf consistency.
+//   Spectral profiles      : analytic piecewise-linear inputs with known values.
+//   Dielectric/Conductor   : FP32 vs FP64 instantiation, reciprocity-free closed forms
+//                            (smooth limits, unpolarised M00=(Rs+Rp)/2), Stokes physicality.
+// Nothing in the reference pins PLT-specific arithmetic with numbers (no tests exist there);
+// DESIGN.md lists which functions remain "parity unpinned" beyond the checks above.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+namespace pltoracle {
+
+// ---------------------------------------------------------------------------------------
+// Constants (Utils/Math/MathConstants.slangh:58-99). Slang literals are float; the FP64
+// instantiation uses the same decimal constants in double.
+// ---------------------------------------------------------------------------------------
+template <class T> struct K {
+    static constexpr T PI = T(3.14159265358979323846);
+    static constexpr T PI_2 = T(1.57079632679489661923);
+    static constexpr T PI_4 = T(0.785398163397448309616);
+    static constexpr T _1_PI = T(0.318309886183790671538);
+    static constexpr T _2_PI = T(0.636619772367581343076);
+    static constexpr T _2PI = T(6.28318530717958647693);
+    static constexpr T _4PI = T(12.5663706143591729539);
+    static constexpr T _1_2PI = T(0.159154943091895335769);
+    static constexpr T FLT_EPS = T(1.192092896e-07);
+};
+
+// ---------------------------------------------------------------------------------------
+// Small vector / matrix types with HLSL semantics (row-major float2x2/float4x4,
+// mul(M,v) = M·v, mul(A,B) = A·B).
+// ---------------------------------------------------------------------------------------
+template <class T> struct V2 { T x, y; };
+template <class T> struct V3 { T x, y, z; };
+template <class T> struct V4 { T x, y, z, w; };
+template <class T> struct M2 { T m[2][2]; };
+template <class T> struct M4 { T m[4][4]; };
+
+template <class T> inline T sqr(T v) { return v * v; }
+// HLSL sign(): -1, 0, +1
+template <class T> inline T sgn(T v) { return v > T(0) ? T(1) : (v < T(0) ? T(-1) : T(0)); }
+// HLSL max/min map to IEEE maxNum/minNum (the non-NaN operand wins).
+template <class T> inline T hmax(T a, T b) { return std::fmax(a, b); }
+template <class T> inline T hmin(T a, T b) { return std::fmin(a, b); }
+template <class T> inline T clampT(T x, T a, T b) { return hmin(hmax(x, a), b); }
+// HLSL lerp(a,b,t) = a + t*(b-a)
+template <class T> inline T hlerp(T a, T b, T t) { return a + t * (b - a); }
+
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> inline V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline V3<T> operator*(T s, V3<T> a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> inline V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+template <class T> inline V3<T> normalize(V3<T> a) { return a / std::sqrt(dot(a, a)); }
+template <class T> inline V2<T> operator+(V2<T> a, V2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T> inline V2<T> operator-(V2<T> a, V2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T> inline V2<T> operator*(V2<T> a, T s) { return {a.x * s, a.y * s}; }
+template <class T> inline V2<T> operator/(V2<T> a, T s) { return {a.x / s, a.y / s}; }
+template <class T> inline T dot(V2<T> a, V2<T> b) { return a.x * b.x + a.y * b.y; }
+
+template <class T> inline M2<T> m2(T a, T b, T c, T d) { M2<T> r; r.m[0][0] = a; r.m[0][1] = b; r.m[1][0] = c; r.m[1][1] = d; return r; }
+template <class T> inline M2<T> ident2(T d = T(1)) { return m2<T>(d, 0, 0, d); }
+template <class T> inline M2<T> mul(M2<T> a, M2<T> b) {
+    M2<T> r;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j];
+    return r;
+}
+template <class T> inline V2<T> mul(M2<T> a, V2<T> v) {
+    return {a.m[0][0] * v.x + a.m[0][1] * v.y, a.m[1][0] * v.x + a.m[1][1] * v.y};
+}
+template <class T> inline M2<T> transpose(M2<T> a) { return m2<T>(a.m[0][0], a.m[1][0], a.m[0][1], a.m[1][1]); }
+template <class T> inline M2<T> scale(M2<T> a, T s) { return m2<T>(a.m[0][0] * s, a.m[0][1] * s, a.m[1][0] * s, a.m[1][1] * s); }
+template <class T> inline M2<T> add(M2<T> a, M2<T> b) {
+    return m2<T>(a.m[0][0] + b.m[0][0], a.m[0][1] + b.m[0][1], a.m[1][0] + b.m[1][0], a.m[1][1] + b.m[1][1]);
+}
+template <class T> inline T determinant(M2<T> a) { return a.m[0][0] * a.m[1][1] - a.m[0][1] * a.m[1][0]; }
+
+template <class T> inline M4<T> zero4() { M4<T> r; std::memset(&r, 0, sizeof(r)); return r; }
+template <class T> inline V4<T> mul(const M4<T>& a, V4<T> v) {
+    T in[4] = {v.x, v.y, v.z, v.w}, o[4];
+    for (int i = 0; i < 4; ++i) o[i] = a.m[i][0] * in[0] + a.m[i][1] * in[1] + a.m[i][2] * in[2] + a.m[i][3] * in[3];
+    return {o[0], o[1], o[2], o[3]};
+}
+template <class T> inline V4<T> operator*(T s, V4<T> a) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+
+// ---------------------------------------------------------------------------------------
+// Complex helpers (Utils/Math/MathHelpers.slang:822-837)
+// ---------------------------------------------------------------------------------------
+template <class T> inline V2<T> c_t(T re, T im = T(0)) { return {re, im}; }
+template <class T> inline T cnorm2(V2<T> z) { return dot(z, z); }
+template <class T> inline T cabs(V2<T> z) { return std::sqrt(cnorm2(z)); }
+template <class T> inline V2<T> csqrt(V2<T> z) {  // MathHelpers.slang:825-830
+    const T r = cabs(z);
+    const T a = std::sqrt(hmax(T(0), r + z.x) / T(2));
+    const T b = std::sqrt(hmax(T(0), r - z.x) / T(2));
+    return c_t(a, sgn(z.y) * b);
+}
+template <class T> inline V2<T> cconj(V2<T> z) { return c_t(z.x, -z.y); }
+template <class T> inline V2<T> cmul(V2<T> a, V2<T> b) { return c_t(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <class T> inline V2<T> cdiv(V2<T> a, V2<T> b) { return cmul(a, cconj(b)) / cnorm2(b); }
+template <class T> inline V2<T> csqr(V2<T> z) { return cmul(z, z); }
+
+// ---------------------------------------------------------------------------------------
+// PRNG (Utils/Sampling/Pseudorandom/SplitMix64.slang:54-78, Xoshiro.slang:52-66,
+//       Utils/Math/BitTricks.slang:45-62, UniformSampleGenerator.slang:51-69,
+//       SampleGeneratorInterface.slang:46-71)
+// ---------------------------------------------------------------------------------------
+inline uint32_t interleave_32bit(uint32_t vx, uint32_t vy) {  // BitTricks.slang:45-62
+    uint32_t x = vx & 0x0000ffffu, y = vy & 0x0000ffffu;
+    x = (x | (x << 8)) & 0x00FF00FFu; x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u; x = (x | (x << 1)) & 0x55555555u;
+    y = (y | (y << 8)) & 0x00FF00FFu; y = (y | (y << 4)) & 0x0F0F0F0Fu;
+    y = (y | (y << 2)) & 0x33333333u; y = (y | (y << 1)) & 0x55555555u;
+    return x | (y << 1);
+}
+struct SplitMix64 {
+    uint64_t state;
+    uint64_t next64() {  // SplitMix64.slang:54-60
+        uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+};
+inline SplitMix64 createSplitMix64(uint32_t lo, uint32_t hi) { return {(uint64_t(hi) << 32) | uint64_t(lo)}; }
+inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+struct Xoshiro128SS {
+    uint32_t s[4];
+    uint32_t next() {  // Xoshiro.slang:52-66
+        const uint32_t result = rotl32(s[0] * 5u, 7) * 9u;
+        const uint32_t t = s[1] << 9;
+        s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3];
+        s[2] ^= t; s[3] = rotl32(s[3], 11);
+        return result;
+    }
+};
+struct UniformSampleGenerator {  // UniformSampleGenerator.slang:51-69
+    Xoshiro128SS rng;
+    UniformSampleGenerator(uint32_t px, uint32_t py, uint32_t sampleNumber) {
+        SplitMix64 sm = createSplitMix64(interleave_32bit(px, py), sampleNumber);
+        const uint64_t s0 = sm.next64(), s1 = sm.next64();
+        rng.s[0] = uint32_t(s0); rng.s[1] = uint32_t(s0 >> 32);
+        rng.s[2] = uint32_t(s1); rng.s[3] = uint32_t(s1 >> 32);
+    }
+    uint32_t next() { return rng.next(); }
+};
+// sampleNext1D: (bits>>8) * 2^-24, exact in float (SampleGeneratorInterface.slang:46-52)
+template <class T, class S> inline T sampleNext1D(S& sg) { return T(float(sg.next() >> 8) * 0x1p-24f); }
+template <class T, class S> inline V2<T> sampleNext2D(S& sg) { V2<T> r; r.x = sampleNext1D<T>(sg); r.y = sampleNext1D<T>(sg); return r; }
+
+// ---------------------------------------------------------------------------------------
+// IEEE binary16 round-to-nearest-even conversion (the reference stores material params as
+// float16_t, Scene/Material/BasicMaterialData.slang:70-83; glm/HLSL conversions are RNE).
+// ---------------------------------------------------------------------------------------
+inline uint16_t f32_to_f16_bits(float f) {
+    uint32_t x; std::memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return uint16_t(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u));
+    if (ax >= 0x477ff000u) return uint16_t(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+    if (ax < 0x38800000u) {  // result is a subnormal half (or zero)
+        if (ax < 0x33000000u) return uint16_t(sign);  // below 2^-25: rounds to zero
+        // value = mant24 * 2^(e-150); in half-subnormal units of 2^-24: mant24 >> (126-e)
+        const uint32_t e = ax >> 23;
+        const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - e;  // 14..24
+        uint32_t h = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u);
+        const uint32_t halfway = 1u << (shift - 1);
+        if (rem > halfway || (rem == halfway && (h & 1u))) ++h;
+        return uint16_t(sign | h);
+    }
+    uint32_t h = ((ax >> 13) - ((127u - 15u) << 10));
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+    return uint16_t(sign | h);
+}
+inline float f16_bits_to_f32(uint16_t h) {
+    const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        if (m == 0) x = sign;
+        else { float v = std::ldexp(float(m), -24); std::memcpy(&x, &v, 4); x |= sign; }
+    } else if (e == 31) x = sign | 0x7f800000u | (m << 13);
+    else x = sign | ((e + 112u) << 23) | (m << 13);
+    float f; std::memcpy(&f, &x, 4); return f;
+}
+inline float to_half(float f) { return f16_bits_to_f32(f32_to_f16_bits(f)); }
+
+// ---------------------------------------------------------------------------------------
+// Sampling warps (Utils/Math/MathHelpers.slang:305-337, 416-450, 609-621)
+// ---------------------------------------------------------------------------------------
+template <class T> inline V2<T> sample_disk_concentric(V2<T> u) {  // MathHelpers.slang:305-321
+    u = {T(2) * u.x - T(1), T(2) * u.y - T(1)};
+    if (u.x == T(0) && u.y == T(0)) return u;
+    T phi, r;
+    if (std::fabs(u.x) > std::fabs(u.y)) { r = u.x; phi = (u.y / u.x) * K<T>::PI_4; }
+    else { r = u.y; phi = K<T>::PI_2 - (u.x / u.y) * K<T>::PI_4; }
+    return {r * std::cos(phi), r * std::sin(phi)};
+}
+template <class T> inline V3<T> sample_cosine_hemisphere_concentric(V2<T> u, T& pdf) {  // :328-334
+    V2<T> d = sample_disk_concentric(u);
+    T z = std::sqrt(hmax(T(0), T(1) - dot(d, d)));
+    pdf = z * K<T>::_1_PI;
+    return {d.x, d.y, z};
+}
+template <class T> inline V3<T> perp_stark(V3<T> u) {  // MathHelpers.slang:609-621
+    V3<T> a = {std::fabs(u.x), std::fabs(u.y), std::fabs(u.z)};
+    uint32_t uyx = (a.x - a.y) < T(0) ? 1 : 0;
+    uint32_t uzx = (a.x - a.z) < T(0) ? 1 : 0;
+    uint32_t uzy = (a.y - a.z) < T(0) ? 1 : 0;
+    uint32_t xm = uyx & uzx;
+    uint32_t ym = (1 ^ xm) & uzy;
+    uint32_t zm = 1 ^ (xm | ym);
+    return normalize(cross(u, V3<T>{T(xm), T(ym), T(zm)}));
+}
+template <class T> inline V2<T> sample_boxmueller_truncated(V2<T> sample, V2<T> mean, T sigma2, T& pdf) {  // :416-434
+    const T l = std::sqrt(hmin(T(1), dot(mean, mean)));
+    const T coso = std::sqrt(hmax(T(0), T(1) - dot(mean, mean)));
+    const T phi_i = (mean.x != T(0) || mean.y != T(0)) ? std::atan2(mean.y, mean.x) : T(0);
+    const T s = std::exp(T(-.5) * sqr(T(1) + l) / sigma2);
+    const T x = (T(1) - s) * hmax(K<T>::FLT_EPS, sample.x) + s;
+    const T r = std::sqrt(T(-2) * sigma2 * std::log(x));
+    const T max_phi = (r < K<T>::FLT_EPS || l < K<T>::FLT_EPS) ? K<T>::PI
+                      : std::acos(clampT((sqr(r) + sqr(l) - T(1)) / (T(2) * r * l), T(-1), T(1)));
+    const T phi = phi_i + K<T>::PI + max_phi * (T(2) * sample.y - T(1));
+    const V2<T> p = {r * std::cos(phi), r * std::sin(phi)};
+    pdf = T(.5) * x / (max_phi * sigma2) * coso;
+    return p + mean;
+}
+template <class T> inline T boxmueller_truncated_pdf(V2<T> wo, V2<T> mean, T sigma2) {  // :435-450
+    const T l = std::sqrt(hmin(T(1), dot(mean, mean)));
+    const T coso = std::sqrt(hmax(T(0), T(1) - dot(mean, mean)));
+    wo = wo - mean;
+    const T r2 = dot(wo, wo);
+    const T x = std::exp(T(-.5) * r2 / sigma2);
+    const T r = std::sqrt(r2);
+    const T max_phi = (r < K<T>::FLT_EPS || l < K<T>::FLT_EPS) ? K<T>::PI
+                      : std::acos(clampT((sqr(r) + sqr(l) - T(1)) / (T(2) * r * l), T(-1), T(1)));
+    return T(.5) * x / (max_phi * sigma2) * coso;
+}
+
+// ---------------------------------------------------------------------------------------
+// Fresnel (Rendering/Materials/Fresnel.slang:30-90)
+// ---------------------------------------------------------------------------------------
+template <class T>
+inline void fresnel_dielectric(T cosThetaI_, T eta, T& rs, T& rp, T& ts, T& tp, T& cosThetaT) {  // :30-55
+    if (eta == T(1)) { cosThetaT = -cosThetaI_; rs = rp = 0; ts = tp = 1; return; }
+    eta = (cosThetaI_ > T(0)) ? eta : T(1) / eta;
+    const T cosThetaI = std::fabs(cosThetaI_);
+    const T cosThetaTSqr = T(1) - (T(1) - sqr(cosThetaI)) / sqr(eta);
+    cosThetaT = std::sqrt(hmax(T(0), cosThetaTSqr));
+    rs = (cosThetaI - eta * cosThetaT) / (cosThetaI + eta * cosThetaT);
+    rp = (eta * cosThetaI - cosThetaT) / (eta * cosThetaI + cosThetaT);
+    const T imp = std::sqrt(eta * cosThetaT / cosThetaI);
+    ts = imp * (rs + T(1));
+    tp = imp * (rp + T(1)) / eta;
+    cosThetaT *= -sgn(cosThetaI_);
+}
+template <class T> inline void fresnel_conductor(T cosThetaI, V2<T> eta, V2<T>& rs, V2<T>& rp) {  // :60-73
+    if ((eta.x == T(0) && eta.y == T(0)) || cosThetaI < T(0)) { rs = rp = c_t<T>(0, 0); return; }
+    const V2<T> t2 = c_t<T>(1) - csqr(cdiv(c_t<T>(1), eta)) * (T(1) - sqr(cosThetaI));
+    const V2<T> t = csqrt(t2);
+    const V2<T> i = c_t(cosThetaI);
+    rs = cdiv(i - cmul(eta, t), i + cmul(eta, t));
+    rp = cdiv(cmul(eta, i) - t, cmul(eta, i) + t);
+}
+template <class T> inline T evalFresnelDielectric(T eta, T cosThetaI, T& cosThetaT) {  // :75-80
+    T rs, rp, ts, tp;
+    fresnel_dielectric(cosThetaI, eta, rs, rp, ts, tp, cosThetaT);
+    return T(.5) * (sqr(rs) + sqr(rp));
+}
+template <class T> inline T evalFresnelDielectric(T eta, T cosThetaI) { T c; return evalFresnelDielectric(eta, cosThetaI, c); }
+template <class T> inline T evalFresnelConductor(V2<T> eta, T cosThetaI) {
+    V2<T> rs, rp; fresnel_conductor(cosThetaI, eta, rs, rp);
+    return T(.5) * (cnorm2(rs) + cnorm2(rp));
+}
+
+// ---------------------------------------------------------------------------------------
+// Mueller calculus (Rendering/Materials/PLT/Mueller.slang:34-144)
+// ---------------------------------------------------------------------------------------
+template <class T>
+inline M4<T> MuellerT(V3<T> n1, V3<T> t1, V3<T> /*n2*/, V3<T> t2, T& outsint, T& outcost) {  // :34-53
+    T theta = std::atan2(dot(n1, cross(t1, t2)), dot(t1, t2));
+    theta = std::isfinite(theta) ? theta : T(0);
+    const T sint = std::sin(theta), cost = std::cos(theta);
+    const T rx = T(2) * sint * cost;
+    const T ry = T(1) - T(2) * sqr(sint);
+    outsint = sint; outcost = cost;
+    M4<T> Tm = zero4<T>();
+    Tm.m[0][0] = Tm.m[3][3] = 1;
+    Tm.m[1][1] = Tm.m[2][2] = rx;
+    Tm.m[1][2] = -ry;
+    Tm.m[2][1] = ry;
+    return Tm;
+}
+template <class T> inline M4<T> MuellerPolarizer(T theta) {  // :60-74
+    const T s = std::sin(theta), c = std::cos(theta);
+    M4<T> P = zero4<T>();
+    P.m[0][0] = 1; P.m[0][1] = P.m[1][0] = c; P.m[0][2] = P.m[2][0] = s;
+    P.m[1][2] = P.m[2][1] = c * s; P.m[1][1] = c * c; P.m[2][2] = s * s; P.m[3][3] = 0;
+    for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) P.m[i][j] *= T(.5);
+    return P;
+}
+template <class T> inline M4<T> MuellerFresnel(V2<T> fs, V2<T> fp) {  // :77-88
+    const T Rp = cnorm2(fp), Rs = cnorm2(fs);
+    const T m00 = (Rp + Rs) / T(2), m01 = (Rp - Rs) / T(2);
+    const V2<T> c = cmul(fs, cconj(fp));
+    const T m22 = c.x, m23 = c.y;
+    M4<T> M = zero4<T>();
+    M.m[0][0] = m00; M.m[0][1] = m01; M.m[1][0] = m01; M.m[1][1] = m00;
+    M.m[2][2] = m22; M.m[2][3] = m23; M.m[3][2] = -m23; M.m[3][3] = m22;
+    return M;
+}
+template <class T> inline M4<T> MuellerFresnelRConductor(T cosThetaI, V2<T> eta) {  // :111-116
+    V2<T> rs, rp; fresnel_conductor(cosThetaI, eta, rs, rp);
+    return MuellerFresnel(rs, rp);
+}
+template <class T> inline M4<T> MuellerFresnelRDielectric(T cosThetaI, T eta) {  // :122-127
+    T rs, rp, ts, tp, c; fresnel_dielectric(cosThetaI, eta, rs, rp, ts, tp, c);
+    return MuellerFresnel(c_t(rs), c_t(rp));
+}
+template <class T> inline M4<T> MuellerFresnelTDielectric(T cosThetaI, T eta) {  // :134-139
+    T rs, rp, ts, tp, c; fresnel_dielectric(cosThetaI, eta, rs, rp, ts, tp, c);
+    return MuellerFresnel(c_t(ts), c_t(tp));
+}
+template <class T> inline M4<T> MuellerFresnelDielectric(T cosThetaI, T eta, bool reflection) {  // :140-144
+    return reflection ? MuellerFresnelRDielectric(cosThetaI, eta) : MuellerFresnelTDielectric(cosThetaI, eta);
+}
+
+// ---------------------------------------------------------------------------------------
+// Spectral utilities (Utils/Color/SpectrumUtils.slang:43-239, SpectrumSampler.slang:38-54,
+// ColorHelpers.slang:61-70)
+// ---------------------------------------------------------------------------------------
+template <class T> struct Spectrum {
+    static constexpr T minWavelength = T(380), maxWavelength = T(720), meanWavelength = T(550);
+    static T G(T x, T lambda, T is1, T is2) { T t = (x - lambda) * (x < lambda ? is1 : is2); return std::exp(T(-0.5) * t * t); }
+    static V3<T> wavelengthToXYZ(T l) {  // SpectrumUtils.slang:43-49
+        T x = T(0.362) * G(l, T(442.0), T(0.0624), T(0.0374)) + T(1.056) * G(l, T(599.8), T(0.0264), T(0.0323)) -
+              T(0.065) * G(l, T(501.1), T(0.0490), T(0.0382));
+        T y = T(0.821) * G(l, T(568.8), T(0.0213), T(0.0247)) + T(0.286) * G(l, T(530.9), T(0.0613), T(0.0322));
+        T z = T(1.217) * G(l, T(437.0), T(0.0845), T(0.0278)) + T(0.681) * G(l, T(459.0), T(0.0385), T(0.0725));
+        return {x, y, z};
+    }
+    static T wavelengthToD65(T lambda) {  // SpectrumUtils.slang:65-78
+        static const T D65[69] = {49.9755, 52.3118, 54.6482, 68.7015, 82.7549, 87.1204, 91.486, 92.4589, 93.4318, 90.057,
+            86.6823, 95.7736, 104.865, 110.936, 117.008, 117.41, 117.812, 116.336, 114.861, 115.392, 115.923, 112.367,
+            108.811, 109.082, 109.354, 108.578, 107.802, 106.296, 104.79, 106.239, 107.689, 106.047, 104.405, 104.225,
+            104.046, 102.023, 100., 98.1671, 96.3342, 96.0611, 95.788, 92.2368, 88.6856, 89.3459, 90.0062, 89.8026,
+            89.5991, 88.6489, 87.6987, 85.4936, 83.2886, 83.4939, 83.6992, 81.863, 80.0268, 80.1207, 80.2146, 81.2462,
+            82.2778, 80.281, 78.2842, 74.0027, 69.7213, 70.6652, 71.6091, 72.979, 74.349, 67.9765, 61.604};
+        const T r = (lambda - minWavelength) * T(0.2);
+        const int t = int(std::ceil(r) + T(.5));
+        const int b = int(std::floor(r) + T(.5));
+        const T w = r - T(b);
+        return D65[t] * w + D65[b] * (T(1) - w);
+    }
+    static V3<T> XYZtoRGB_Rec709(V3<T> c) {  // ColorHelpers.slang:61-70
+        return {T(3.240969941904522) * c.x + T(-1.537383177570094) * c.y + T(-0.4986107602930032) * c.z,
+                T(-0.9692436362808803) * c.x + T(1.875967501507721) * c.y + T(0.04155505740717569) * c.z,
+                T(0.05563007969699373) * c.x + T(-0.2039769588889765) * c.y + T(1.056971514242878) * c.z};
+    }
+    static V3<T> spectrumToRgb(const T* spectrum, const T* wavelengths, unsigned samples) {  // :92-104
+        V3<T> result = {0, 0, 0};
+        for (unsigned i = 0; i < samples; i++) {
+            V3<T> xyz = wavelengthToXYZ(wavelengths[i]) * wavelengthToD65(wavelengths[i]) * spectrum[i];
+            result = result + XYZtoRGB_Rec709(xyz);
+        }
+        return result / T(10.5670762);
+    }
+    static T rgbToSpectrum(V3<T> rgb, T wavelength) {  // SpectrumUtils.slang:106-173
+        static const T white_sd[11] = {1.0000, 1.0000, 0.9999, 0.9993, 0.9992, 0.9998, 1.0000, 1.0000, 1.0000, 1.0000, 0.0};
+        static const T cyan_sd[11] = {0.9710, 0.9426, 1.0007, 1.0007, 1.0007, 1.0007, 0.1564, 0.0000, 0.0000, 0.0000, 0.0};
+        static const T magenta_sd[11] = {1.0000, 1.0000, 0.968, 0.22295, 0.0000, 0.0458, 0.8369, 1.0000, 1.0000, 0.9959, 0.0};
+        static const T yellow_sd[11] = {0.0001, 0.0000, 0.1088, 0.6651, 1.0000, 1.0000, 0.9996, 0.9586, 0.9685, 0.9840, 0.0};
+        static const T red_sd[11] = {0.1012, 0.0515, 0.0000, 0.0000, 0.0000, 0.0000, 0.8325, 1.0149, 1.0149, 1.014, 0.0};
+        static const T green_sd[11] = {0.0000, 0.0000, 0.0273, 0.7937, 1.0000, 0.9418, 0.1719, 0.0000, 0.0000, 0.0025, 0.0};
+        static const T blue_sd[11] = {1.0000, 1.0000, 0.8916, 0.3323, 0.0000, 0.0000, 0.0003, 0.0369, 0.0483, 0.0496, 0.0};
+        // The bin index is computed in float in the reference; keep it in float for both instantiations so the
+        // FP64 oracle selects the same bin as the FP32 one (a bin switch is a discontinuity, not an error).
+        const float wf = float(wavelength);
+        const int b = int(clampT((wf - 380.0f) / (720.0f - 380.0f), 0.0f, 1.0f) * 10.f);
+        const T w = white_sd[b], c = cyan_sd[b], m = magenta_sd[b], y = yellow_sd[b], r = red_sd[b], g = green_sd[b], bl = blue_sd[b];
+        T sd = 0;
+        if (rgb.x <= rgb.y && rgb.x <= rgb.z) {
+            sd += w * rgb.x;
+            if (rgb.y <= rgb.z) { sd += c * (rgb.y - rgb.x); sd += bl * (rgb.z - rgb.y); }
+            else { sd += c * (rgb.z - rgb.x); sd += g * (rgb.y - rgb.z); }
+        } else if (rgb.y <= rgb.x && rgb.y <= rgb.z) {
+            sd += w * rgb.y;
+            if (rgb.x <= rgb.z) { sd += m * (rgb.x - rgb.y); sd += bl * (rgb.z - rgb.x); }
+            else { sd += m * (rgb.z - rgb.y); sd += r * (rgb.x - rgb.z); }
+        } else {
+            sd += w * rgb.z;
+            if (rgb.x <= rgb.y) { sd += y * (rgb.x - rgb.z); sd += g * (rgb.y - rgb.x); }
+            else { sd += y * (rgb.y - rgb.z); sd += r * (rgb.x - rgb.y); }
+        }
+        return sd;
+    }
+    static T sampleOne(T u) { return minWavelength + u * (maxWavelength - minWavelength); }  // SpectrumSampler.slang:38-41
+    static void hwssWavelengths(T hero, int n, T* out) {  // SpectrumSampler.slang:44-53
+        const T span = maxWavelength - minWavelength;
+        for (int i = 0; i < n; ++i) {
+            T h = hero + T(i) * (span / T(n));
+            h -= h > maxWavelength ? span : T(0);
+            out[i] = h;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// Spectral profiles (Scene/Material/BasicMaterialData.slang:43-54, Scene/Scene.slang:251-295,
+// Scene/SceneBuilder.cpp:911-960, Utils/Color/Spectrum.h:88-123, SampledSpectrum.h:64-74,308-317)
+// ---------------------------------------------------------------------------------------
+struct SpectralProfile {
+    static constexpr uint32_t kMaxBins = 128;
+    float rgb[3];
+    float intensity;
+    uint32_t bins;
+    float minWavelength, maxWavelength;
+    float pdf[kMaxBins];
+    float icdf[kMaxBins];
+};
+inline float clerp(float a, float b, float t) { return (1.f - t) * a + t * b; }  // Utils/Math/Common.h:54-57
+
+struct PiecewiseLinearSpectrum {  // Utils/Color/Spectrum.h:88-123
+    std::vector<float> wl, val;
+    float eval(float w) const {
+        if (wl.empty() || w < wl.front() || w > wl.back()) return 0.f;
+        auto it = std::lower_bound(wl.begin(), wl.end(), w);
+        if (it == wl.begin()) return val.front();
+        size_t i = size_t(it - wl.begin()) - 1;
+        float t = (w - wl[i]) / (wl[i + 1] - wl[i]);
+        return clerp(val[i], val[i + 1], t);
+    }
+    float leastBin() const {
+        float s = wl.back() - wl.front();
+        for (size_t i = 1; i < wl.size(); ++i) s = std::min(s, wl[i] - wl[i - 1]);
+        return s;
+    }
+};
+struct SampledSpectrumF {  // Utils/Color/SampledSpectrum.h:64-74, 308-317
+    float start, end;
+    std::vector<float> s;
+    explicit SampledSpectrumF(const PiecewiseLinearSpectrum& p, float minBin = 5.f) : start(p.wl.front()), end(p.wl.back()) {
+        const float binSize = std::max(minBin, p.leastBin() / 2.f);
+        s.resize(std::max<size_t>(1, size_t((end - start) / binSize + .5f)));
+        for (size_t i = 0; i < s.size(); ++i) s[i] = p.eval(start + (end - start) / float(s.size() - 1) * float(i));
+    }
+    float eval(float l) const {
+        if (l < start || l > end) return 0.f;
+        float x = ((l - start) / (end - start)) * (float(s.size()) - 1.0f);
+        size_t i = size_t(std::floor(x));
+        if (i + 1 >= s.size()) return s[s.size() - 1];
+        float w = x - float(i);
+        return clerp(s[i], s[i + 1], w);
+    }
+};
+// genSpectralProfile (SceneBuilder.cpp:911-960); rgb is not used on the hot path and left 0.
+inline SpectralProfile genSpectralProfile(const SampledSpectrumF& ss, bool computeICdf) {
+    SpectralProfile sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.intensity = 1.f;
+    sp.minWavelength = std::max(ss.start, 380.f);
+    sp.maxWavelength = std::min(ss.end, 720.f);
+    sp.bins = std::min<uint32_t>(SpectralProfile::kMaxBins,
+        uint32_t(float(ss.s.size()) * (sp.maxWavelength - sp.minWavelength) / (ss.end - ss.start) + .5f));
+    if (sp.bins == 0) return sp;
+    const float r = sp.bins > 1 ? 1.f / float(sp.bins - 1) : 1.f;
+    float sum = 0.f;
+    for (uint32_t i = 0; i < sp.bins; ++i) { sp.pdf[i] = ss.eval(clerp(sp.minWavelength, sp.maxWavelength, float(i) * r)); sum += sp.pdf[i]; }
+    if (sum < 1.192092896e-07f || !computeICdf) return sp;
+    for (uint32_t i = 0; i < sp.bins; ++i) sp.pdf[i] /= sum;
+    sp.intensity = sum;
+    float cdfa = 0.f, cdfb = sp.pdf[0];
+    for (uint32_t pdfi = 0, icdfi = 0; icdfi < sp.bins; ++icdfi) {
+        const float cdf = float(icdfi) * r;
+        while (cdf >= cdfb && pdfi + 1 < sp.bins) { cdfa = cdfb; ++pdfi; cdfb += sp.pdf[pdfi]; }
+        const float l = std::min(1.f, (cdf - cdfa) / (cdfb - cdfa));
+        sp.icdf[icdfi] = clerp(float(std::max<uint32_t>(1, pdfi)) - 1.f, float(pdfi), l) * r;
+    }
+    return sp;
+}
+// Scene::evalSpectrum (Scene/Scene.slang:251-269)
+template <class T> inline T evalSpectrum(const SpectralProfile* sp, T lambda) {
+    if (!sp) return T(0);
+    const uint32_t bins = sp->bins;
+    const T minl = sp->minWavelength, maxl = sp->maxWavelength;
+    const T x = (lambda - minl) / (maxl - minl) * (T(bins) - T(1));
+    const T i = std::floor(x);
+    const T w = x - i;
+    auto at = [&](T fi) { int ii = int(fi); ii = std::max(0, std::min(int(bins) - 1, ii)); return T(sp->pdf[ii]); };
+    return (lambda < minl || lambda > maxl) ? T(0) : T(sp->intensity) * hlerp(at(i), at(i + T(1)), w);
+}
+
+// ---------------------------------------------------------------------------------------
+// PLT coherence and beam (Rendering/PLT.slang:41-245). N = HWSS sample count (1..4).
+// ---------------------------------------------------------------------------------------
+template <class T> struct PLTCoherence {  // PLT.slang:41-78
+    T D[3];  // D00, D01, D11
+    T r_m;
+    M2<T> toDiffusivityMat() const { return m2<T>(D[0], D[1], D[1], D[2]); }
+    T r() const { return r_m * T(1e+3); }
+    M2<T> toInvCoherenceMat(T k) const { const T rl = k / K<T>::_2PI; return scale(toDiffusivityMat(), sqr(rl / r())); }
+    M2<T> toInvCoherenceMat() const { return scale(toDiffusivityMat(), sqr(T(1) / r())); }
+    T det(T k) const { return T(1) / determinant(toInvCoherenceMat(k)); }
+    void propagate(T dist_m) { r_m += dist_m; }
+    void transform(M2<T> U) {
+        const M2<T> nd = mul(transpose(U), mul(toDiffusivityMat(), U));
+        D[0] = nd.m[0][0]; D[1] = nd.m[0][1]; D[2] = nd.m[1][1];
+    }
+};
+
+template <class T, int N> struct PLTBeam {  // PLT.slang:83-245
+    T L[N], L1[N], L2[N], L3[N];
+    T wavelengths[N];
+    T spectralPDFs[N];
+    int hwss_comps = N;
+    PLTCoherence<T> Theta;
+    V3<T> posW, r, t;
+    bool distant = false;
+
+    V4<T> getSP(int j) const { return {L[j], L1[j], L2[j], L3[j]}; }
+    void setSP(int j, V4<T> s) { L[j] = s.x; L1[j] = s.y; L2[j] = s.z; L3[j] = s.w; }
+    void setZero() { for (int j = 0; j < N; ++j) L[j] = L1[j] = L2[j] = L3[j] = 0; }
+    M2<T> Q() const {  // PLT.slang:151-155
+        const V3<T> b = cross(t, r);
+        return transpose(m2<T>(t.x, t.y, b.x, b.y));
+    }
+    M2<T> invTheta(T k) const { return Theta.toInvCoherenceMat(k); }
+    M2<T> invTheta() const { return Theta.toInvCoherenceMat(); }
+    T mutualCoherenceAngular(V3<T> d1, V3<T> d2) const {  // PLT.slang:172-177
+        const T s4 = std::sqrt(K<T>::_4PI);
+        const V2<T> v = {T(1) / hmax(s4 * std::fabs(d1.x - d2.x), K<T>::FLT_EPS), T(1) / hmax(s4 * std::fabs(d1.y - d2.y), K<T>::FLT_EPS)};
+        const M2<T> invT = scale(invTheta(), sqr(Theta.r()));
+        const T x = T(1) / dot(v, mul(invT, v));
+        return std::exp(T(-.5) * x);
+    }
+    T coherenceArea(T k) const { return K<T>::PI * std::sqrt(Theta.det(k)); }  // PLT.slang:190-192
+    void transformCoherence(M2<T> U) { Theta.transform(U); }
+    void rotateFrame(V3<T> newt, V3<T> newr) {  // PLT.slang:203-215
+        T sint, cost;
+        M4<T> Tm = MuellerT(r, t, newr, newt, sint, cost);
+        for (int j = 0; j < hwss_comps; ++j) setSP(j, mul(Tm, getSP(j)));
+        const M2<T> R = m2<T>(cost, -sint, sint, cost);
+        transformCoherence(R);
+        r = newr; t = newt;
+    }
+    void constructLocalFrame(V3<T> newwo) {  // PLT.slang:217-223
+        V3<T> newt = cross(V3<T>{0, 0, 1}, newwo);
+        const T l2 = dot(newt, newt);
+        newt = l2 > T(1e-8) ? newt / std::sqrt(l2) : perp_stark(newwo);
+        rotateFrame(newt, newwo);
+    }
+    void propagate(V3<T> p) {  // PLT.slang:224-230
+        if (!distant) Theta.propagate(length(p - posW));
+        posW = p;
+    }
+    V3<T> toLocal(V3<T> v) const { return {dot(v, t), dot(v, cross(t, r)), dot(v, r)}; }
+};
+
+// Shading frame (Scene/ShadingData.slang:86-98)
+template <class T> struct Frame {
+    V3<T> T_, B, N;
+    V3<T> toLocal(V3<T> v) const { return {dot(v, T_), dot(v, B), dot(v, N)}; }
+    V3<T> fromLocal(V3<T> v) const { return T_ * v.x + B * v.y + N * v.z; }
+};
+
+// LobeType (Rendering/Materials/LobeType.slang:36-69)
+enum : uint32_t {
+    Lobe_None = 0x00, Lobe_DiffuseReflection = 0x01, Lobe_ScatterReflection = 0x02, Lobe_DeltaReflection = 0x04,
+    Lobe_DispersiveDeltaReflection = 0x08, Lobe_DiffuseTransmission = 0x10, Lobe_ScatterTransmission = 0x20,
+    Lobe_DeltaTransmission = 0x40, Lobe_DispersiveDeltaTransmission = 0x80, Lobe_Diffuse = 0x11, Lobe_Scatter = 0x22,
+    Lobe_NonDispersiveDelta = 0x44, Lobe_DispersiveDelta = 0x88, Lobe_Delta = 0xCC, Lobe_NonDelta = 0x33,
+    Lobe_Reflection = 0x0f, Lobe_Transmission = 0xf0, Lobe_NonDeltaReflection = 0x03, Lobe_NonDeltaTransmission = 0x30,
+    Lobe_All = 0xffff
+};
+
+// ---------------------------------------------------------------------------------------
+// Surface PSD models.
+// GaussianSurface (Rendering/Materials/PLT/GaussianSurface.slang:37-110)
+// FractalSurface  (Rendering/Materials/PLT/FractalSurface.slang:38-135)
+// Both carry q explicitly: the reference constructors derive it from a "roughness"; the
+// explicit (q, sigma2|T) form lets a caller set an independent RMS height. makeRef() is the
+// reference constructor.
+// ---------------------------------------------------------------------------------------
+template <class T> struct GaussianSurface {
+    T sigma2, q;
+    static T roughnessToSigma2(T r) { return std::pow(clampT(T(1) - T(.5) * r, T(0), T(1)), T(28)) * T(2500); }  // :42-44
+    static GaussianSurface makeRef(T PSDroughness) {  // :51-59
+        const T s2 = roughnessToSigma2(PSDroughness), s2max = roughnessToSigma2(T(0.1)), s2r0 = roughnessToSigma2(T(0));
+        GaussianSurface g;
+        g.q = hlerp(T(0), T(1), (s2r0 - s2) / (s2r0 - s2max));
+        g.sigma2 = hmin(s2max, s2);
+        return g;
+    }
+    bool isDeltaOnly() const { return q == T(0); }
+    T alpha(V3<T> wi, V3<T> wo, T k) const {  // :67-72
+        const T a = sqr((std::fabs(wi.z) + std::fabs(wo.z)) * q * k);
+        return isDeltaOnly() ? T(1) : std::exp(-a);
+    }
+    template <class S> V3<T> sample(V3<T> wi, S& sg, T k, T& pdf) const {  // :76-86
+        const T w = T(1) / sigma2;
+        const T s2 = w / sqr(k);
+        const V2<T> mean = {-wi.x, -wi.y};
+        const V2<T> wo = sample_boxmueller_truncated(sampleNext2D<T>(sg), mean, s2, pdf);
+        const T z = std::sqrt(hmax(T(0), T(1) - dot(wo, wo)));
+        return {wo.x, wo.y, sgn(wi.z) * z};
+    }
+    T pdf(V3<T> wi, V3<T> wo, T k) const {  // :88-95
+        const T w = T(1) / sigma2;
+        const T s2 = w / sqr(k);
+        return boxmueller_truncated_pdf(V2<T>{wo.x, wo.y}, V2<T>{-wi.x, -wi.y}, s2);
+    }
+    T eval(T k, M2<T> invTheta, M2<T> Q, V3<T> h) const {  // :97-109
+        invTheta = mul(Q, mul(invTheta, transpose(Q)));
+        const M2<T> S = add(scale(ident2<T>(), T(1) / sigma2), invTheta);
+        const T d = determinant(S);
+        const M2<T> invS = scale(m2<T>(S.m[1][1], -S.m[0][1], -S.m[1][0], S.m[0][0]), T(1) / d);
+        const V2<T> hx = {h.x, h.y};
+        return std::exp(T(-.5) * sqr(k) * dot(hx, mul(invS, hx))) * (sqr(k) * K<T>::_1_2PI / std::sqrt(d));
+    }
+};
+
+template <class T> struct FractalSurface {
+    T Tc, gamma, q;
+    static T meank() { return K<T>::_2PI / (T(550) * T(1e-3)); }
+    static T roughnessToT(T alpha) {  // :44-51
+        const T alpha2 = sqr(clampT(alpha, T(0), T(.75)));
+        return hmin(T(5000), (T(1) - alpha2) / (T(4) * sqr(meank()) * alpha2));
+    }
+    static T TToRoughness(T Tv) { return T(1) / std::sqrt(T(1) + T(4) * sqr(meank()) * Tv); }  // :52-56
+    static FractalSurface makeRef(T PSDroughness, T gamma = T(3)) {  // :71-75
+        FractalSurface f;
+        f.gamma = hmax(T(1.1), gamma);
+        f.Tc = roughnessToT(PSDroughness);
+        f.q = TToRoughness(f.Tc);
+        return f;
+    }
+    bool isDeltaOnly() const { return q == T(0); }
+    T alpha(V3<T> wi, V3<T> wo, T k) const {  // :77-88
+        const T a = sqr((std::fabs(wi.z) + std::fabs(wo.z)) * q * k);
+        return std::exp(-a);
+    }
+    T pGF(V2<T> z) const {  // :92-95
+        const T f = T(1) / std::pow(T(1) + Tc * dot(z, z), (gamma + T(1)) / T(2));
+        return K<T>::_1_2PI * (gamma - T(1)) * Tc * f;
+    }
+    template <class S> V3<T> sample(V3<T> wi, S& sg, T k, T& pdf) const {  // :97-124
+        const T s = std::sqrt(hmax(T(0), T(1) - sqr(wi.z)));
+        const T phi_i = s > T(0) ? std::atan2(wi.y, wi.x) : T(0);
+        const T sqrtT = std::sqrt(Tc);
+        const V2<T> u1 = sampleNext2D<T>(sg);
+        const V2<T> u2 = sampleNext2D<T>(sg);
+        (void)u2;
+        const T M = T(1) - std::pow(T(1) + sqr(k) * Tc * sqr(T(1) + s), -(gamma - T(1)) / T(2));
+        const T f = std::sqrt(std::pow(T(1) - M * u1.x, T(-2) / (gamma - T(1))) - T(1)) / sqrtT;
+        const T phi_max = (f == T(0) || s == T(0)) ? K<T>::PI
+            : std::acos(clampT((sqr(f / k) + sqr(s) - T(1)) / (T(2) * (f / k) * s), T(-1), T(1)));
+        const T phi_f = phi_i + (T(2) * u1.y - T(1)) * phi_max;
+        const V2<T> zeta = {f * std::cos(phi_f), f * std::sin(phi_f)};
+        const V2<T> wo = {zeta.x / k - wi.x, zeta.y / k - wi.y};
+        const T z = std::sqrt(hmax(T(0), T(1) - dot(wo, wo)));
+        pdf = K<T>::PI / phi_max * z * pGF(zeta);
+        return {wo.x, wo.y, sgn(wi.z) * z};
+    }
+    T pdf(V3<T> wi, V3<T> wo, T k) const { return pGF(V2<T>{k * (wi.x + wo.x), k * (wi.y + wo.y)}); }  // :126-129
+    T eval(T k, V3<T> h) const { return pGF(V2<T>{k * h.x, k * h.y}); }  // :131-134
+};
+
+// ---------------------------------------------------------------------------------------
+// PLTDielectricBxDF (Rendering/Materials/PLT/PLTDielectricBSDF.slang:43-279)
+// ---------------------------------------------------------------------------------------
+template <class T> struct DielectricBxDF {
+    V3<T> specular{1, 1, 1}, transmission{1, 1, 1};
+    GaussianSurface<T> gfs;
+    T etaInt, etaExt, dispersionB;
+
+    T indexOfRefraction(T k) const { return (etaInt + dispersionB * sqr(k * K<T>::_1_2PI)) / etaExt; }  // :53-60
+    bool isDispersive() const { return dispersionB != T(0); }
+    static V3<T> reflect(V3<T> wi) { return {-wi.x, -wi.y, wi.z}; }
+    static V3<T> refract(V3<T> wi, T cosThetaT, T eta) {  // :66-69
+        const T s = cosThetaT < T(0) ? T(1) / eta : eta;
+        return {-s * wi.x, -s * wi.y, cosThetaT};
+    }
+    static V3<T> scattered_wo(V3<T> wo, T eta) {  // :71-77
+        const T s = wo.z > T(0) ? T(1) / eta : eta;
+        const V2<T> xy = {wo.x * s, wo.y * s};
+        const T l2 = dot(xy, xy);
+        return l2 > T(1) ? V3<T>{0, 0, 0} : V3<T>{xy.x, xy.y, -sgn(wo.z) * std::sqrt(hmax(T(0), T(1) - l2))};
+    }
+    template <int N> void eval(V3<T> toLight, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) const {  // :79-137
+        const bool isReflection = toLight.z * toEye.z > T(0);
+        const bool isScatter = (lobe & Lobe_NonDelta) != 0;
+        const bool isDirect = (lobe & Lobe_Delta) != 0;
+        const bool hasTransmission = (lobe & Lobe_Transmission) != 0;
+        const bool hasReflection = (lobe & Lobe_Reflection) != 0;
+        if (toLight.z * toEye.z == T(0) || (!isScatter && !isDirect) || (!hasTransmission && !isReflection) || (!hasReflection && isReflection)) {
+            beam.setZero(); return;
+        }
+        T k[N], m00[N], eta[N], relEta2[N], a[N];
+        for (int j = 0; j < N; ++j) {
+            k[j] = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            m00[j] = Spectrum<T>::rgbToSpectrum(isReflection ? specular : transmission, beam.wavelengths[j]);
+            eta[j] = indexOfRefraction(k[j]);
+            relEta2[j] = sqr(toEye.z > T(0) ? eta[j] : T(1) / eta[j]);
+        }
+        const V3<T> toEyeUp = isReflection ? toEye : scattered_wo(toEye, eta[0]);
+        const V3<T> toLightBeam = beam.toLocal(toLight);
+        for (int j = 0; j < N; ++j) a[j] = gfs.alpha(toLight, toEyeUp, k[j]);
+        if (isScatter) {
+            const V3<T> h = toLight + toEyeUp;
+            const V3<T> m = normalize(h);
+            const T LdotH = hmax(T(0), dot(toLight, m));
+            const M2<T> Q = beam.Q();
+            const M2<T> invTheta = beam.invTheta(T(1));
+            for (int j = 0; j < beam.hwss_comps; ++j) {
+                const V4<T> SP = beam.getSP(j);
+                const T J = isReflection ? T(1) : relEta2[j];
+                const T D = gfs.eval(k[j], invTheta, Q, h);
+                const M4<T> M1 = MuellerFresnelDielectric(sgn(toLight.z) * LdotH, eta[j], isReflection);
+                beam.setSP(j, (m00[j] * (T(1) - a[j]) * J * std::fabs(toLight.z) * D) * mul(M1, SP));
+            }
+        } else {
+            for (int j = 0; j < beam.hwss_comps; ++j) {
+                const V4<T> SP = beam.getSP(j);
+                const T J = isReflection ? T(1) : relEta2[j];
+                const M4<T> M0 = MuellerFresnelDielectric(toLight.z, eta[j], isReflection);
+                T coh = T(1);
+                if (!isReflection) {
+                    T cosThetaT;
+                    (void)evalFresnelDielectric(eta[j], toEye.z, cosThetaT);
+                    const V3<T> lobeWo = beam.toLocal(refract(toEye, cosThetaT, eta[j]));
+                    coh = j == 0 ? T(1) : beam.mutualCoherenceAngular(lobeWo, toLightBeam);
+                }
+                beam.setSP(j, (m00[j] * a[j] * coh * J) * mul(M0, SP));
+            }
+        }
+    }
+    template <class S>
+    bool sample(V3<T> wiLocal, V3<T>& wo, T& pdf, T& weight, uint32_t& lobe, S& sg, T wavelength, uint32_t lobeMask) const {  // :159-211
+        const T k = K<T>::_2PI / (T(1e-3) * wavelength);
+        const T a = gfs.alpha(wiLocal, wiLocal, k);
+        const T eta = indexOfRefraction(k);
+        const bool hasReflection = (lobeMask & Lobe_Reflection) != 0, hasTransmission = (lobeMask & Lobe_Transmission) != 0;
+        const bool hasDirect = (lobeMask & Lobe_Delta) != 0, hasScatter = (lobeMask & Lobe_NonDelta) != 0;
+        if ((!hasReflection && !hasTransmission) || (!hasDirect && !hasScatter)) return false;
+        const T pdfDirect = hasDirect && hasScatter ? a : hasDirect ? T(1) : T(0);
+        bool isReflection;
+        const bool isDirect = pdfDirect == T(1) || sampleNext1D<T>(sg) < pdfDirect;
+        pdf = (hasDirect && hasScatter) ? (isDirect ? pdfDirect : T(1) - pdfDirect) : T(1);
+        if (isDirect) {
+            T cosThetaT;
+            const T Fd = evalFresnelDielectric(eta, wiLocal.z, cosThetaT);
+            isReflection = hasReflection ? (Fd > T(0) && sampleNext1D<T>(sg) <= Fd) : false;
+            wo = isReflection ? reflect(wiLocal) : refract(wiLocal, cosThetaT, eta);
+            pdf *= (hasReflection && hasTransmission) ? (isReflection ? Fd : T(1) - Fd) : T(1);
+            lobe = isReflection ? Lobe_DeltaReflection : (isDispersive() ? Lobe_DispersiveDeltaTransmission : Lobe_DeltaTransmission);
+        } else {
+            T surfPdf;
+            wo = gfs.sample(wiLocal, sg, k, surfPdf);
+            pdf *= surfPdf;
+            const V3<T> m = normalize(wiLocal + wo);
+            const T F = evalFresnelDielectric(eta, sgn(wiLocal.z) * dot(wiLocal, m));
+            isReflection = hasReflection ? (F > T(0) && sampleNext1D<T>(sg) <= F) : false;
+            if (!isReflection) wo = scattered_wo(wo, eta);
+            pdf *= (hasReflection && hasTransmission) ? (isReflection ? F : T(1) - F) : T(1);
+            lobe = isReflection ? Lobe_ScatterReflection : Lobe_ScatterTransmission;
+        }
+        weight = Spectrum<T>::rgbToSpectrum(isReflection ? specular : transmission, wavelength);
+        return (isReflection && wiLocal.z * wo.z <= T(0)) ? false : (!isReflection && wiLocal.z * wo.z >= T(0)) ? false : true;
+    }
+    T pdf(V3<T> wiLocal, V3<T> woLocal, T wavelength) const {  // :213-229
+        const T k = K<T>::_2PI / (T(1e-3) * wavelength);
+        const T a = gfs.alpha(wiLocal, wiLocal, k);
+        const T eta = indexOfRefraction(k);
+        const bool isReflection = wiLocal.z * woLocal.z > T(0);
+        const V3<T> wo = isReflection ? woLocal : scattered_wo(woLocal, eta);
+        const V3<T> m = normalize(wiLocal + wo);
+        const T F = evalFresnelDielectric(eta, sgn(wiLocal.z) * dot(wiLocal, m));
+        const T pdfF = isReflection ? F : T(1) - F;
+        return (T(1) - a) * pdfF * gfs.pdf(wiLocal, wo, k);
+    }
+    template <int N> void lambdaPdf(V3<T> toLight, V3<T> toEye, uint32_t lobe, const PLTBeam<T, N>& beam, T* ret) const {  // :238-270
+        const bool isReflection = toLight.z * toEye.z > T(0);
+        const bool isScatter = (lobe & Lobe_NonDelta) != 0, isDirect = (lobe & Lobe_Delta) != 0;
+        const V3<T> toLightBeam = beam.toLocal(toLight);
+        for (int j = 0; j < N; ++j) {
+            const T k = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            const T eta = indexOfRefraction(k);
+            if (isDirect && !isReflection) {
+                T cosThetaT;
+                (void)evalFresnelDielectric(eta, toEye.z, cosThetaT);
+                const V3<T> lobeWo = beam.toLocal(refract(toEye, cosThetaT, eta));
+                ret[j] = j == 0 ? T(1) : beam.mutualCoherenceAngular(lobeWo, toLightBeam);
+            } else if (isScatter) {
+                const V3<T> wo = isReflection ? toLight : scattered_wo(toLight, eta);
+                ret[j] = gfs.pdf(toEye, wo, k);
+            } else ret[j] = T(1);
+        }
+    }
+    M2<T> coherenceTransform(V3<T> /*wi*/, V3<T> wo, uint32_t lobe) const {  // :272-278
+        const bool isDirect = (lobe & Lobe_Delta) != 0;
+        const T c = std::sqrt(hmax(T(1e-2), std::fabs(wo.z)));
+        return isDirect ? ident2<T>() : m2<T>(c, 0, 0, T(1) / c);
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// PLTConductorBxDF (Rendering/Materials/PLT/PLTConductorBSDF.slang:44-173). n,k are spectral
+// profiles, evaluated with Scene::evalSpectrum.
+// ---------------------------------------------------------------------------------------
+template <class T> struct ConductorBxDF {
+    V3<T> specular{1, 1, 1};
+    FractalSurface<T> gfs;
+    const SpectralProfile* etaN = nullptr;
+    const SpectralProfile* etaK = nullptr;
+    T extEta = T(1);
+
+    template <int N> void eval(V3<T> toLight, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) const {  // :57-94
+        const bool isScatter = (lobe & Lobe_ScatterReflection) != 0;
+        const bool isDirect = (lobe & Lobe_DeltaReflection) != 0;
+        const bool hasReflection = (lobe & Lobe_Reflection) != 0;
+        if ((!isScatter && !isDirect) || !hasReflection) { beam.setZero(); return; }
+        const V3<T> h = toLight + toEye;
+        const V3<T> m = normalize(h);
+        const T LdotH = hmax(T(0), dot(toLight, m));
+        for (int j = 0; j < beam.hwss_comps; ++j) {
+            const T k = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            const T a = gfs.alpha(toLight, toEye, k);
+            const T m00 = Spectrum<T>::rgbToSpectrum(specular, beam.wavelengths[j]);
+            const T iorN = evalSpectrum<T>(etaN, beam.wavelengths[j]);
+            const T iorK = evalSpectrum<T>(etaK, beam.wavelengths[j]);
+            const V4<T> SP = beam.getSP(j);
+            V4<T> newSP = {0, 0, 0, 0};
+            const V2<T> ior = {iorN / extEta, iorK / extEta};
+            if (isScatter) {
+                const M4<T> M1 = MuellerFresnelRConductor(LdotH, ior);
+                const T D = gfs.eval(k, h);
+                newSP = ((T(1) - a) * std::fabs(toLight.z) * D) * mul(M1, SP);
+            } else {
+                const M4<T> M0 = MuellerFresnelRConductor(toLight.z, ior);
+                newSP = a * mul(M0, SP);
+            }
+            beam.setSP(j, m00 * newSP);
+        }
+    }
+    template <class S>
+    bool sample(V3<T> wiLocal, V3<T>& wo, T& pdf, T& weight, uint32_t& lobe, S& sg, T wavelength, uint32_t lobeMask) const {  // :101-133
+        const bool hasReflection = (lobeMask & Lobe_Reflection) != 0;
+        const bool hasDirect = (lobeMask & Lobe_Delta) != 0, hasScatter = (lobeMask & Lobe_NonDelta) != 0;
+        if (!hasReflection || (!hasDirect && !hasScatter)) return false;
+        const T k = K<T>::_2PI / (T(1e-3) * wavelength);
+        const T a = gfs.alpha(wiLocal, wiLocal, k);
+        const T m00 = Spectrum<T>::rgbToSpectrum(specular, wavelength);
+        pdf = T(1);
+        bool isDirect = hasDirect;
+        if (hasDirect && hasScatter) {
+            const T pdfDirect = a;
+            isDirect = pdfDirect == T(1) || sampleNext1D<T>(sg) < pdfDirect;
+            pdf = isDirect ? pdfDirect : T(1) - pdfDirect;
+        }
+        weight = m00;
+        if (isDirect) { wo = V3<T>{-wiLocal.x, -wiLocal.y, wiLocal.z}; lobe = Lobe_DeltaReflection; }
+        else { T surfPdf; wo = gfs.sample(wiLocal, sg, k, surfPdf); pdf *= surfPdf; lobe = Lobe_ScatterReflection; }
+        return true;
+    }
+    T pdf(V3<T> wiLocal, V3<T> woLocal, T wavelength) const {  // :135-142
+        const T k = K<T>::_2PI / (T(1e-3) * wavelength);
+        const T a = gfs.alpha(wiLocal, wiLocal, k);
+        return (T(1) - a) * gfs.pdf(wiLocal, woLocal, k);
+    }
+    template <int N> void lambdaPdf(V3<T> toLight, V3<T> toEye, uint32_t lobe, const PLTBeam<T, N>& beam, T* ret) const {  // :149-163
+        const bool isScatter = (lobe & Lobe_NonDelta) != 0;
+        for (int j = 0; j < N; ++j) {
+            const T k = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            ret[j] = isScatter ? gfs.pdf(toEye, toLight, k) : T(1);
+        }
+    }
+    M2<T> coherenceTransform(V3<T> /*wi*/, V3<T> wo, uint32_t lobe) const {  // :165-171
+        const bool isDirect = (lobe & Lobe_Delta) != 0;
+        const T c = std::sqrt(hmax(T(1e-2), std::fabs(wo.z)));
+        return isDirect ? ident2<T>() : m2<T>(c, 0, 0, T(1) / c);
+    }
+};
+
+// BSDF wrappers: the IMaterialInstance-level eval (PLTDielectricBSDF.slang:283-299,
+// PLTConductorBSDF.slang:177-193): frame change, constructLocalFrame, bxdf.eval,
+// coherence transform, back to world.
+template <class T, int N, class BxDF>
+inline void bsdfEval(const BxDF& bxdf, const Frame<T>& sd, V3<T> sdV, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) {
+    beam.r = sd.toLocal(beam.r);
+    beam.t = sd.toLocal(beam.t);
+    const V3<T> wiLocal = sd.toLocal(sdV), woLocal = sd.toLocal(toEye);
+    beam.constructLocalFrame(woLocal);
+    bxdf.eval(wiLocal, woLocal, lobe, beam);
+    beam.transformCoherence(bxdf.coherenceTransform(wiLocal, woLocal, lobe));
+    beam.r = sd.fromLocal(beam.r);
+    beam.t = sd.fromLocal(beam.t);
+}
+
+}  // namespace pltoracle

@@ -1,0 +1,79 @@
+"""ctypes binding of libplt_amd.so (the C ABI declared in include/plt_abi.h).
+
+The library is the product: if it is missing this module raises instead of falling back to any
+host implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libplt_amd.so"
+
+# Plane indices and material types (include/plt_abi.h).
+Q_WI_X, Q_WI_Y, Q_WI_Z, Q_WO_X, Q_WO_Y, Q_WO_Z, Q_LAMBDA, Q_MAT, Q_PARAM, Q_ROUGH_Q, Q_ROUGH_L, Q_COH_AREA, \
+    Q_S0, Q_S1, Q_S2, Q_S3, Q_PLANES = range(17)
+R_S0, R_S1, R_S2, R_S3, R_PDF, R_PLANES = range(6)
+MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
+
+PLT_OK = 0
+PLT_WARN_UNKNOWN_KEY = 1
+STATUS_NAMES = {0: "PLT_OK", 1: "PLT_WARN_UNKNOWN_KEY", -1: "PLT_EINVAL", -2: "PLT_ENOMEM", -3: "PLT_EHIP",
+                -4: "PLT_ENOTIMPL", -5: "PLT_ESTATE"}
+
+
+class PLTError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class SpectralProfile(C.Structure):
+    """plt_spectral_profile (Scene/Material/BasicMaterialData.slang:43-54)."""
+    _fields_ = [("rgb", C.c_float * 3), ("intensity", C.c_float), ("bins", C.c_uint32),
+                ("min_wavelength", C.c_float), ("max_wavelength", C.c_float),
+                ("pdf", C.c_float * 128), ("icdf", C.c_float * 128)]
+
+
+# (name, restype, argtypes) for every exported symbol of include/plt_abi.h.
+SIGNATURES = [
+    ("plt_last_error", C.c_char_p, []),
+    ("plt_abi_version", C.c_int, []),
+    ("plt_context_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("plt_context_destroy", C.c_int, [C.c_void_p]),
+    ("plt_spectral_profile_build", C.c_int,
+     [C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int, C.POINTER(SpectralProfile)]),
+    ("plt_context_set_spectral_profiles", C.c_int, [C.c_void_p, C.POINTER(SpectralProfile), C.c_uint32]),
+    ("plt_sample_generator_stream", C.c_int,
+     [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("plt_bsdf_generate_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("plt_bsdf_eval_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads libplt_amd.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pltfalcor_amd.build` "
+                           "(the HIP library is required; there is no host fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> int:
+    """Raises PLTError for negative statuses; returns warnings (positive) unchanged."""
+    if status < 0:
+        msg = load().plt_last_error().decode(errors="replace")
+        raise PLTError(status, msg)
+    return status

@@ -1,0 +1,103 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) — the checker for the HIP path."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_U32 = C.POINTER(C.c_uint32)
+_U64 = C.POINTER(C.c_uint64)
+
+_SIGS = {
+    "oracle_interleave_32bit": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "oracle_splitmix64_seq": (None, [C.c_uint64, C.c_uint32, _U64]),
+    "oracle_xoshiro128ss_seq": (None, [_U32, C.c_uint32, _U32]),
+    "oracle_usg_seq": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _U32]),
+    "oracle_f32_to_f16": (C.c_uint16, [C.c_float]),
+    "oracle_f16_to_f32": (C.c_float, [C.c_uint16]),
+    "oracle_spectral_profile_size": (C.c_uint32, []),
+    "oracle_build_profile": (C.c_int, [C.c_uint32, _F, _F, C.c_int, C.c_void_p]),
+    "oracle_eval_spectrum_f32": (C.c_float, [C.c_void_p, C.c_float]),
+    "oracle_c0_generate": (None, [C.c_uint32, C.c_uint32, _F, C.c_int]),
+    "oracle_c0_eval_f32": (None, [C.c_uint32, _F, C.c_void_p, C.c_uint32, _F, C.c_int]),
+    "oracle_c0_eval_f64": (None, [C.c_uint32, _F, C.c_void_p, C.c_uint32, _D, C.c_int]),
+    "oracle_fresnel_dielectric_f64": (None, [C.c_double, C.c_double, _D]),
+    "oracle_fresnel_conductor_f64": (None, [C.c_double, C.c_double, C.c_double, _D]),
+    "oracle_mueller_rotate_f64": (None, [_D, _D, _D, _D, _D]),
+    "oracle_fractal_pgf_f64": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
+    "oracle_gaussian_eval_f64": (C.c_double, [C.c_double, C.c_double, _D, _D, C.c_double, C.c_double]),
+    "oracle_boxmueller_trunc_pdf_f64": (C.c_double, [C.c_double] * 5),
+    "oracle_rgb_to_spectrum_f64": (C.c_double, [C.c_double] * 4),
+    "oracle_wavelength_to_xyz_f64": (None, [C.c_double, _D]),
+    "oracle_hwss_wavelengths_f32": (None, [C.c_float, C.c_int, _F]),
+    "oracle_surface_sample_f64": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _D, C.c_double, C.c_uint32,
+                                         C.c_uint32, _D, _D]),
+    "oracle_surface_pdf_f64": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, _D, _D, C.c_double]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
+        l = C.CDLL(str(ORACLE_SO))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def usg_seq(px: int, py: int, sn: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_usg_seq(px, py, sn, n, ptr(out, C.c_uint32))
+    return out
+
+
+def build_profile(wl, val, compute_icdf: bool) -> bytes:
+    size = lib().oracle_spectral_profile_size()
+    buf = C.create_string_buffer(size)
+    wl = np.ascontiguousarray(wl, dtype=np.float32)
+    val = np.ascontiguousarray(val, dtype=np.float32)
+    rc = lib().oracle_build_profile(len(wl), ptr(wl, C.c_float), ptr(val, C.c_float), int(compute_icdf), buf)
+    assert rc == 0
+    return buf.raw
+
+
+def c0_generate(n: int, seed: int, threads: int = 0) -> np.ndarray:
+    q = np.zeros((16, n), dtype=np.float32)
+    lib().oracle_c0_generate(n, seed, ptr(q, C.c_float), threads)
+    return q
+
+
+def c0_eval(queries: np.ndarray, profiles: bytes, nprof: int, fp64: bool = False, threads: int = 0) -> np.ndarray:
+    """queries: [16, n] float32 view (integer planes as raw bits). Returns [5, n] results."""
+    q = np.ascontiguousarray(queries).view(np.float32)
+    n = q.shape[1]
+    pbuf = C.create_string_buffer(profiles, len(profiles))
+    if fp64:
+        out = np.zeros((5, n), dtype=np.float64)
+        lib().oracle_c0_eval_f64(n, ptr(q, C.c_float), pbuf, nprof, ptr(out, C.c_double), threads)
+    else:
+        out = np.zeros((5, n), dtype=np.float32)
+        lib().oracle_c0_eval_f32(n, ptr(q, C.c_float), pbuf, nprof, ptr(out, C.c_float), threads)
+    return out
+
+
+def dvec(*v) -> np.ndarray:
+    return np.array(v, dtype=np.float64)
