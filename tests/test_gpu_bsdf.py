@@ -64,9 +64,12 @@ def test_query_generator_matches_oracle(gpu):
     for p in exact:
         assert np.array_equal(g[p], ref[p]), f"plane {p}"
     gf, rf = g.view(np.float32), ref.view(np.float32)
-    for p in (0, 1, 2, 3, 4, 5, 13, 14):
+    for p in (0, 1, 3, 4, 13, 14):  # cos/sin of the same argument: a few ulp
         ulp = np.spacing(np.maximum(np.abs(rf[p]), 1e-30).astype(np.float32))
         assert np.all(np.abs(gf[p] - rf[p]) <= 4 * np.maximum(ulp, 1e-7)), f"plane {p}"
+    for p in (2, 5):  # z = sqrt(1 - x^2 - y^2): error of x,y amplified by |x|/|z| near grazing
+        tol = 4e-7 * (1.0 + 1.0 / np.maximum(np.abs(rf[p]), 1e-4))
+        assert np.all(np.abs(gf[p] - rf[p]) <= tol), f"plane {p}"
 
 
 @pytest.mark.parametrize("seed", [0, 3])
@@ -92,8 +95,10 @@ def test_eval_parity_vs_oracle(gpu, seed):
     print(f"seed {seed}: gpu-vs-fp64 Stokes max {e_gs[live].max():.3e} p99.9 {np.quantile(e_gs[live], .999):.3e}; "
           f"fp32 oracle max {e_os[live].max():.3e}; pdf max {e_gp.max():.3e} (fp32 oracle {e_op.max():.3e})")
     assert bad.size == 0, f"{bad.size} queries outside tolerance, first {bad[:8]}"
-    # GPU vs FP32 oracle agreement for the bulk.
-    assert np.quantile(np.max(np.abs(res[:4] - o32[:4]), axis=0) / np.maximum(np.abs(o32[0]), 1e-30), 0.99) < TOL
+    # The envelope clause is a fallback only: in the bulk the GPU meets 1e-4 against FP64 outright
+    # and is no less accurate than the reference's own FP32 arithmetic.
+    assert np.mean(e_gs[live] <= TOL) >= 0.9999 and np.mean(e_gp <= TOL) >= 0.9999
+    assert np.quantile(e_gs[live], 0.99) <= max(np.quantile(e_os[live], 0.99), 1e-6)
 
 
 def test_eval_full_size_properties(gpu):
@@ -108,10 +113,12 @@ def test_eval_full_size_properties(gpu):
     lam = b.queries.cpu().numpy().view(np.float32)[6]
     assert np.all(np.isfinite(res))
     assert np.all(res[:4, lam >= 720.0] == 0.0)
-    s0 = res[0]
-    pol = np.sqrt(res[1] ** 2 + res[2] ** 2 + res[3] ** 2)
-    assert np.all(s0 >= -1e-30)
-    assert np.all(pol <= s0 * (1 + 1e-4) + 1e-30)
+    s0 = res[0].astype(np.float64)
+    assert np.all(s0 >= 0.0)
+    # degree of polarisation <= 1 (ratios in float64: squaring tiny float32 values would underflow)
+    nz = s0 > 1e-30
+    dop = np.sqrt(sum((res[c, nz].astype(np.float64) / s0[nz]) ** 2 for c in (1, 2, 3)))
+    assert np.all(dop <= 1 + 1e-3), f"max DoP {dop.max()}"
     assert np.all(res[4] >= 0.0)
     # Deterministic: a second evaluation is bit-identical.
     res2 = b.eval().cpu().numpy()

@@ -1,0 +1,201 @@
+"""Pins the oracle's optics against closed forms and invariants (CPU only).
+
+Each check is chosen so that a plausible restatement mistake (swapped s/p, a dropped conjugate,
+a wrong exponent or normalisation, a transposed rotation) fails it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle_lib as O
+
+D = C.c_double
+
+
+def fresnel_d(cos_i, eta):
+    out = np.zeros(5)
+    O.lib().oracle_fresnel_dielectric_f64(cos_i, eta, O.ptr(out, D))
+    return out  # rs, rp, ts, tp, cosT
+
+
+def fresnel_c(cos_i, n, k):
+    out = np.zeros(4)
+    O.lib().oracle_fresnel_conductor_f64(cos_i, n, k, O.ptr(out, D))
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
+def born_wolf(cos_i, n2):
+    """Textbook amplitude coefficients for n1 = 1 -> n2 (complex), principal-branch cos_t."""
+    n2 = complex(n2)
+    sin2 = 1 - cos_i ** 2
+    cos_t = np.sqrt(1 - sin2 / n2 ** 2 + 0j)
+    rs = (cos_i - n2 * cos_t) / (cos_i + n2 * cos_t)
+    rp = (n2 * cos_i - cos_t) / (n2 * cos_i + cos_t)
+    return rs, rp, cos_t
+
+
+@pytest.mark.parametrize("eta", [1.3, 1.5, 1.8, 2.4])
+def test_dielectric_normal_incidence(eta):
+    rs, rp, ts, tp, ct = fresnel_d(1.0, eta)
+    R = ((eta - 1) / (eta + 1)) ** 2
+    assert rs ** 2 == pytest.approx(R, rel=1e-12) and rp ** 2 == pytest.approx(R, rel=1e-12)
+    assert ct == pytest.approx(-1.0)
+
+
+def test_dielectric_vs_born_wolf_and_energy():
+    for eta in (1.33, 1.5, 1.9):
+        for ci in np.linspace(0.02, 1.0, 50):
+            rs, rp, ts, tp, ct = fresnel_d(ci, eta)
+            brs, brp, bct = born_wolf(ci, eta)
+            assert rs == pytest.approx(brs.real, abs=1e-12)
+            assert rp == pytest.approx(brp.real, abs=1e-12)
+            assert -ct == pytest.approx(bct.real, abs=1e-12)
+            assert rs ** 2 + ts ** 2 == pytest.approx(1.0, abs=1e-12)  # s energy
+            assert rp ** 2 + tp ** 2 == pytest.approx(1.0, abs=1e-12)  # p energy
+
+
+def test_dielectric_tir_and_index_matched():
+    rs, rp, ts, tp, ct = fresnel_d(-0.2, 1.5)  # from inside, beyond the critical angle
+    assert rs == pytest.approx(1.0) and rp == pytest.approx(1.0) and ct == 0.0
+    rs, rp, ts, tp, ct = fresnel_d(0.7, 1.0)
+    assert (rs, rp, ts, tp) == (0.0, 0.0, 1.0, 1.0) and ct == -0.7
+
+
+def test_conductor_vs_born_wolf():
+    for (n, k) in [(0.05, 3.0), (0.2, 2.0), (1.5, 0.5), (2.8, 3.9)]:
+        for ci in np.linspace(0.01, 1.0, 40):
+            rs, rp = fresnel_c(ci, n, k)
+            brs, brp, _ = born_wolf(ci, complex(n, k))
+            assert abs(rs - brs) < 1e-10 and abs(rp - brp) < 1e-10, (n, k, ci)
+    assert fresnel_c(-0.1, 0.2, 3.0) == (0j, 0j)
+    assert fresnel_c(0.5, 0.0, 0.0) == (0j, 0j)
+
+
+def rotate(n1, t1, t2, s):
+    out = np.zeros(4)
+    O.lib().oracle_mueller_rotate_f64(O.ptr(np.array(n1, float), D), O.ptr(np.array(t1, float), D),
+                                      O.ptr(np.array(t2, float), D), O.ptr(np.array(s, float), D), O.ptr(out, D))
+    return out
+
+
+def test_mueller_rotation_invariants_and_quirk():
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        n1 = rng.normal(size=3); n1 /= np.linalg.norm(n1)
+        t1 = np.cross(n1, rng.normal(size=3)); t1 /= np.linalg.norm(t1)
+        t2 = np.cross(n1, rng.normal(size=3)); t2 /= np.linalg.norm(t2)
+        s = np.array([1.0, *rng.uniform(-0.5, 0.5, 3)])
+        o = rotate(n1, t1, t2, s)
+        assert o[0] == s[0] and o[3] == s[3]
+        assert o[1] ** 2 + o[2] ** 2 == pytest.approx(s[1] ** 2 + s[2] ** 2, rel=1e-12)
+        # rotation by theta: linear-polarisation angle (S1,S2) advances by 2 theta + pi/2 (quirk offset)
+        theta = np.arctan2(np.dot(n1, np.cross(t1, t2)), np.dot(t1, t2))
+        ang_in = np.arctan2(s[2], s[1])
+        ang_out = np.arctan2(o[2], o[1])
+        d = (ang_out - ang_in - (np.pi / 2 - 2 * theta)) % (2 * np.pi)
+        assert min(d, 2 * np.pi - d) < 1e-9
+    # theta = 0: (S1, S2) -> (-S2, S1) (Mueller.slang:41-51: rx = 0, ry = 1)
+    o = rotate([0, 0, 1], [1, 0, 0], [1, 0, 0], [1, 0.3, -0.2, 0.1])
+    assert np.allclose(o, [1, 0.2, 0.3, 0.1], atol=1e-15)
+
+
+@pytest.mark.parametrize("T,gamma", [(1.0, 3.0), (50.0, 3.0), (400.0, 2.2), (9.0, 4.5)])
+def test_fractal_psd_normalised(T, gamma):
+    """K-correlation PSD integrates to 1 over the plane: int pGF(r) 2 pi r dr = 1."""
+    f = lambda r: O.lib().oracle_fractal_pgf_f64(T, gamma, r, 0.0) * 2 * np.pi * r  # noqa: E731
+    val, _ = integrate.quad(f, 0, np.inf, limit=400)
+    assert val == pytest.approx(1.0, rel=1e-6)
+    # isotropic
+    assert O.lib().oracle_fractal_pgf_f64(T, gamma, 0.3, 0.4) == pytest.approx(
+        O.lib().oracle_fractal_pgf_f64(T, gamma, 0.5, 0.0), rel=1e-14)
+
+
+def test_gaussian_psd_normalised_and_anisotropic():
+    """The coherence-convolved Gaussian lobe integrates to 1 over h (any S, Q)."""
+    k, sigma2 = 12.0, 30.0
+    inv = np.array([0.02, 0.005, 0.005, 0.01])
+    th = 0.4
+    Q = np.array([np.cos(th), -np.sin(th), np.sin(th), np.cos(th)])
+    f = lambda hy, hx: O.lib().oracle_gaussian_eval_f64(sigma2, k, O.ptr(inv, D), O.ptr(Q, D), hx, hy)  # noqa
+    val, _ = integrate.dblquad(f, -0.3, 0.3, -0.3, 0.3, epsabs=1e-10)
+    assert val == pytest.approx(1.0, rel=1e-5)
+    # a less coherent beam (larger inverse coherence) widens the lobe: the peak k^2/(2 pi sqrt(det S)) drops
+    inv2 = inv * 10
+    p1 = O.lib().oracle_gaussian_eval_f64(sigma2, k, O.ptr(inv, D), O.ptr(Q, D), 0, 0)
+    p2 = O.lib().oracle_gaussian_eval_f64(sigma2, k, O.ptr(inv2, D), O.ptr(Q, D), 0, 0)
+    assert p2 < p1
+    qm = Q.reshape(2, 2)
+    S = np.eye(2) / sigma2 + qm @ inv.reshape(2, 2) @ qm.T
+    assert p1 == pytest.approx(k * k / (2 * np.pi * np.sqrt(np.linalg.det(S))), rel=1e-12)
+
+
+def test_truncated_boxmueller_pdf_matches_sampler():
+    """sample_boxmueller_truncated's returned pdf equals boxmueller_truncated_pdf at the sample."""
+    for (sigma2, wi) in [(60.0, (0.3, -0.2, 0.93)), (5.0, (0.0, 0.0, 1.0)), (120.0, (-0.7, 0.1, 0.7))]:
+        wi3 = np.array(wi, float); wi3 /= np.linalg.norm(wi3)
+        k = 10.0
+        cnt = 500
+        wo = np.zeros(3 * cnt); pdf = np.zeros(cnt)
+        O.lib().oracle_surface_sample_f64(0, sigma2, 0.0, 0.1, O.ptr(wi3, D), k, 5, cnt, O.ptr(wo, D), O.ptr(pdf, D))
+        wo = wo.reshape(cnt, 3)
+        for i in range(cnt):
+            if np.dot(wo[i, :2], wo[i, :2]) > 1:
+                continue
+            p = O.lib().oracle_surface_pdf_f64(0, sigma2, 0.0, 0.1, O.ptr(wi3, D), O.ptr(np.ascontiguousarray(wo[i]), D), k)
+            assert p == pytest.approx(pdf[i], rel=1e-9)
+
+
+def test_fractal_sampler_pdf_relation():
+    """FractalSurface.sample returns pi/phi_max * z * pGF(zeta), i.e. its pdf() times pi/phi_max * cos_o
+    (FractalSurface.slang:97-129): the sampled zeta equals k (wi + wo).xy."""
+    wi3 = np.array([0.4, 0.1, np.sqrt(1 - 0.17)])
+    T, gamma, k = 40.0, 3.0, 11.0
+    cnt = 300
+    wo = np.zeros(3 * cnt); pdf = np.zeros(cnt)
+    O.lib().oracle_surface_sample_f64(1, T, gamma, 0.05, O.ptr(wi3, D), k, 9, cnt, O.ptr(wo, D), O.ptr(pdf, D))
+    wo = wo.reshape(cnt, 3)
+    ratios = []
+    for i in range(cnt):
+        p = O.lib().oracle_surface_pdf_f64(1, T, gamma, 0.05, O.ptr(wi3, D), O.ptr(np.ascontiguousarray(wo[i]), D), k)
+        if p > 0 and wo[i, 2] > 1e-3:
+            ratios.append(pdf[i] / (p * wo[i, 2]))
+    r = np.array(ratios)
+    assert np.all(r >= 1.0 - 1e-9)  # pi / phi_max >= 1
+    assert np.all(r <= 1e6)
+
+
+def test_rgb_to_spectrum_and_xyz():
+    L = O.lib()
+    # white is the white basis; beyond 720 nm everything is zero (bin 10)
+    assert L.oracle_rgb_to_spectrum_f64(1, 1, 1, 500) == pytest.approx(0.9993)
+    assert L.oracle_rgb_to_spectrum_f64(1, 1, 1, 720) == 0.0
+    assert L.oracle_rgb_to_spectrum_f64(0.2, 0.5, 0.9, 780) == 0.0
+    # pure red at 650 nm: red basis bin 7
+    assert L.oracle_rgb_to_spectrum_f64(1, 0, 0, 650) == pytest.approx(1.0149)
+    # Wyman 2013 fit: y peaks near 555-560 nm with value ~1
+    xyz = np.zeros(3)
+    L.oracle_wavelength_to_xyz_f64(556.0, O.ptr(xyz, D))
+    assert xyz[1] == pytest.approx(1.0, abs=0.02)
+
+
+def test_hwss_wavelengths_wrap():
+    out = np.zeros(4, np.float32)
+    O.lib().oracle_hwss_wavelengths_f32(700.0, 4, O.ptr(out, C.c_float))
+    assert np.allclose(out, [700, 445, 530, 615])
+    for hero in np.linspace(380, 719.9, 50, dtype=np.float32):
+        O.lib().oracle_hwss_wavelengths_f32(float(hero), 4, O.ptr(out, C.c_float))
+        assert np.all(out >= 380) and np.all(out <= 720)
+
+
+def test_fp16_conversion_vs_numpy():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-70000, 70000, 20000), rng.uniform(-2, 2, 20000),
+                        rng.uniform(-1e-4, 1e-4, 20000), 2.0 ** -np.arange(10, 30),
+                        np.array([0.0, -0.0, 1.5, 65504, 65519.9, 65520, np.inf, -np.inf])]).astype(np.float32)
+    ref = x.astype(np.float16).view(np.uint16)
+    got = np.array([O.lib().oracle_f32_to_f16(float(v)) for v in x], dtype=np.uint16)
+    assert np.array_equal(got, ref)
+    back = np.array([O.lib().oracle_f16_to_f32(int(h)) for h in ref[:2000]], dtype=np.float32)
+    assert np.array_equal(back, ref[:2000].view(np.float16).astype(np.float32))
