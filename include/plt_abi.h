@@ -116,8 +116,12 @@ enum {
 enum { PLT_R_S0 = 0, PLT_R_S1, PLT_R_S2, PLT_R_S3, PLT_R_PDF, PLT_R_PLANES };
 
 typedef enum plt_material_type {
-    PLT_MAT_DIELECTRIC = 0, /* PLTDielectricBSDF.slang:43-361 (Gaussian PSD) */
-    PLT_MAT_CONDUCTOR = 1   /* PLTConductorBSDF.slang:44-242 (K-correlation PSD, tabulated n,k) */
+    PLT_MAT_DIELECTRIC = 0,               /* PLTDielectricBSDF.slang:43-361 (Gaussian PSD) */
+    PLT_MAT_CONDUCTOR = 1,                /* PLTConductorBSDF.slang:44-242 (K-correlation PSD, tabulated n,k) */
+    PLT_MAT_DIFFUSE = 2,                  /* PLTDiffuseBSDF.slang:39-155 (Lambertian; the emitter material) */
+    PLT_MAT_OPAQUE_DIELECTRIC = 3,        /* PLTOpaqueDielectricBSDF.slang:43-212 */
+    PLT_MAT_COATED_OPAQUE_DIELECTRIC = 4, /* PLTCoatedOpaqueDielectricBSDF.slang:45-312 */
+    PLT_MAT_GRATED_CONDUCTOR = 5          /* PLTDiffractionGratedConductorBSDF.slang:45-290 */
 } plt_material_type;
 
 /* Synthetic batch: query i draws 14 numbers from UniformSampleGenerator(pixel=(i & 0xFFFF,
@@ -130,6 +134,96 @@ int plt_bsdf_generate_queries(plt_context* ctx, uint32_t n, uint32_t seed, uint3
 
 /* Evaluates a batch (device pointers). Material-divergent batches are accepted as is. */
 int plt_bsdf_eval_queries(plt_context* ctx, uint32_t n, const uint32_t* queries, float* results, plt_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Materials. The reference's BasicMaterialData parameter block (Scene/Material/
+ * BasicMaterialData.slang:63-107): binary16 fields stored as raw bits, spectral-profile ids
+ * (0xFFFFFFFF = none). Field use per type follows each *Material.slang setupMaterialInstance:
+ *   dielectric   : specular = (dispersion B, roughness, -, ext IoR), ior, transmission
+ *   conductor    : specular = (gamma, roughness, -, -), ior = ext IoR, n/k profiles
+ *   diffuse      : base_color, emission profile
+ *   opaque diel. : base_color, specular.y = roughness
+ *   coated opaque: base_color, specular = (-, roughness, thickness scale, -), data1 = (thickness,
+ *                  -, -, coat IoR), ior = ext IoR
+ *   grated cond. : base_color, specular = (height scale, power mult), data1 = (pitch x, pitch y,
+ *                  direction, type|lobes bits), data2 = (height, roughness), ior, n/k profiles
+ * ------------------------------------------------------------------------------------- */
+#define PLT_NO_PROFILE 0xFFFFFFFFu
+typedef struct plt_material {
+    uint32_t type; /* plt_material_type */
+    uint16_t base_color[4];
+    uint16_t specular[4];
+    uint16_t data1[4];
+    uint16_t data2[4];
+    uint16_t transmission[3];
+    uint16_t ior;
+    uint32_t emission_profile;
+    uint32_t ior_n_profile;
+    uint32_t ior_k_profile;
+} plt_material;
+
+/* Pinhole camera (Scene/Camera/Camera.slang:46-78, Camera.cpp:187-193). */
+typedef struct plt_camera {
+    float pos[3];
+    float target[3];
+    float up[3];
+    float fov_y;          /* radians */
+    float aspect;         /* width / height */
+    float focal_distance;
+    float jitter[2];      /* +-0.5 px sample offset (data.jitterX/Y); 0 = pixel centre */
+} plt_camera;
+
+/* ---------------------------------------------------------------------------------------
+ * Scenes. A scene is triangle meshes (world space, one material per mesh), spectral profiles,
+ * materials and a camera, with a software BVH built natively on the host (binned SAH) and kept
+ * in HBM. Replaces the reference's DXR BLAS/TLAS build (Scene/Scene.cpp:3092, :3624).
+ * ------------------------------------------------------------------------------------- */
+typedef struct plt_scene plt_scene;
+
+/* Procedural workload scene: a ground quad (PLTDiffuse), `sphere_count` icospheres of
+ * subdivision `subdivision` (20*4^s triangles each) at uniform random positions on the quad,
+ * resting on it, radius U[0.2, 1.0], materials drawn uniformly from the first
+ * `material_kinds` of {dielectric, conductor, coated opaque dielectric, grated conductor}
+ * (grating pitch U[0.5, 2] um), one quad area light (PLTDiffuse emitter, 6500 K blackbody
+ * tabulated in 5 nm bins), and a pinhole camera of the given aspect. Deterministic in `seed`. */
+typedef struct plt_procedural_scene_desc {
+    uint32_t seed;
+    uint32_t sphere_count;
+    uint32_t subdivision;
+    uint32_t material_kinds;
+    float ground_size;
+    float aspect;
+} plt_procedural_scene_desc;
+
+/* ctx == NULL builds a host-only scene (arrays and BVH; plt_scene_trace then returns PLT_ESTATE). */
+int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_desc* desc, plt_scene** out);
+int plt_scene_destroy(plt_scene* scene);
+
+/* Host view of a scene's arrays (valid until plt_scene_destroy). */
+typedef struct plt_scene_arrays {
+    uint32_t n_vertices, n_triangles, n_meshes, n_materials, n_profiles, n_bvh_nodes;
+    const float* positions;          /* 3 per vertex, world space */
+    const float* normals;            /* 3 per vertex, binary16-rounded (SceneTypes.slang:240-241) */
+    const uint32_t* indices;         /* 3 per triangle */
+    const uint32_t* tri_mesh;        /* mesh id per triangle */
+    const uint32_t* mesh_first_tri;  /* first triangle of each mesh */
+    const uint32_t* mesh_material;   /* material id per mesh */
+    const plt_material* materials;
+    const plt_spectral_profile* profiles;
+    plt_camera camera;
+} plt_scene_arrays;
+int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
+
+/* ---------------------------------------------------------------------------------------
+ * Ray casting against a scene (device pointers). Rays: 8 planes of n floats (origin xyz,
+ * direction xyz, tmin, tmax). Closest hit (mode 0, the reference's TraceRay with a
+ * closest-hit shader) writes 4 planes of n 32-bit words: triangle id (0xFFFFFFFF on miss),
+ * barycentrics b1, b2 (the weights of vertices 1 and 2, DXR convention) and t. Any hit
+ * (mode 1, RAY_FLAG_ACCEPT_FIRST_HIT_AND_END_SEARCH, ShadowRays.slangh:65-88) writes one
+ * plane: 1 if any triangle is hit with tmin <= t <= tmax, else 0. Intersection is watertight
+ * and deterministic; among equal distances the smallest triangle id wins.
+ * ------------------------------------------------------------------------------------- */
+int plt_scene_trace(plt_scene* scene, uint32_t n, const float* rays, uint32_t* hits, int mode, plt_stream stream);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@
 // The BSDF batch layout mirrors the product's documented SoA contract: 16 planes of n
 // 32-bit words per query batch, 5 planes of n floats per result batch.
 #include "plt_oracle.hpp"
+#include "plt_oracle_geom.hpp"
 #include <omp.h>
 
 using namespace pltoracle;
@@ -246,5 +247,45 @@ double oracle_surface_pdf_f64(int type, double p0, double p1, double q, const do
     if (type == 1) { FractalSurface<double> f; f.Tc = p0; f.gamma = p1; f.q = q; return f.pdf(wi, wo, k); }
     GaussianSurface<double> g; g.sigma2 = p0; g.q = q; return g.pdf(wi, wo, k);
 }
+
+// Ray casting: rays are 8 planes of n floats (o, d, tmin, tmax). mode 0 writes 4 planes (tri id,
+// b1, b2, t as raw 32-bit words), mode 1 writes one plane (any hit 1/0). brute != 0 tests every
+// triangle; otherwise the oracle's own median-split BVH is used.
+int oracle_trace(const float* positions, uint32_t ntri, const uint32_t* indices, uint32_t n, const float* rays,
+                 uint32_t* hits, int mode, int brute, int threads) {
+    OMesh m;
+    m.pos = positions; m.idx = indices; m.ntri = ntri;
+    OBvh bvh;
+    if (!brute) bvh.build(m);
+    if (threads > 0) omp_set_num_threads(threads);
+    const size_t N = n;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(n); ++i) {
+        ORay r;
+        for (int a = 0; a < 3; ++a) { r.o[a] = rays[a * N + size_t(i)]; r.d[a] = rays[(3 + a) * N + size_t(i)]; }
+        r.tmin = rays[6 * N + size_t(i)];
+        r.tmax = rays[7 * N + size_t(i)];
+        OHit h;
+        if (brute) {
+            if (mode == 1) {
+                const WoopPrep p = woop_prep(r);
+                for (uint32_t t = 0; t < ntri; ++t) {
+                    float tt, b1, b2;
+                    if (woop_intersect(r, p, m.vtx(t, 0), m.vtx(t, 1), m.vtx(t, 2), r.tmin, r.tmax, tt, b1, b2)) { h.tri = t; break; }
+                }
+            } else h = closest_brute(m, r);
+        } else h = bvh.closest(m, r, mode == 1);
+        if (mode == 1) hits[size_t(i)] = h.tri != 0xFFFFFFFFu ? 1u : 0u;
+        else {
+            hits[0 * N + size_t(i)] = h.tri;
+            std::memcpy(&hits[1 * N + size_t(i)], &h.b1, 4);
+            std::memcpy(&hits[2 * N + size_t(i)], &h.b2, 4);
+            std::memcpy(&hits[3 * N + size_t(i)], &h.t, 4);
+        }
+    }
+    return 0;
+}
+
+void oracle_compute_ray_origin(const float* pos, const float* n, float* out) { compute_ray_origin(pos, n, out); }
 
 }  // extern "C"

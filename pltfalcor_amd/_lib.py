@@ -36,6 +36,34 @@ class SpectralProfile(C.Structure):
                 ("pdf", C.c_float * 128), ("icdf", C.c_float * 128)]
 
 
+class Material(C.Structure):
+    """plt_material: the reference's BasicMaterialData parameter block (binary16 fields as bits)."""
+    _fields_ = [("type", C.c_uint32), ("base_color", C.c_uint16 * 4), ("specular", C.c_uint16 * 4),
+                ("data1", C.c_uint16 * 4), ("data2", C.c_uint16 * 4), ("transmission", C.c_uint16 * 3),
+                ("ior", C.c_uint16), ("emission_profile", C.c_uint32), ("ior_n_profile", C.c_uint32),
+                ("ior_k_profile", C.c_uint32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov_y", C.c_float),
+                ("aspect", C.c_float), ("focal_distance", C.c_float), ("jitter", C.c_float * 2)]
+
+
+class ProceduralSceneDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("sphere_count", C.c_uint32), ("subdivision", C.c_uint32),
+                ("material_kinds", C.c_uint32), ("ground_size", C.c_float), ("aspect", C.c_float)]
+
+
+class SceneArrays(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32), ("n_meshes", C.c_uint32),
+                ("n_materials", C.c_uint32), ("n_profiles", C.c_uint32), ("n_bvh_nodes", C.c_uint32),
+                ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("tri_mesh", C.POINTER(C.c_uint32)),
+                ("mesh_first_tri", C.POINTER(C.c_uint32)), ("mesh_material", C.POINTER(C.c_uint32)),
+                ("materials", C.POINTER(Material)), ("profiles", C.POINTER(SpectralProfile)),
+                ("camera", Camera)]
+
+
 # (name, restype, argtypes) for every exported symbol of include/plt_abi.h.
 SIGNATURES = [
     ("plt_last_error", C.c_char_p, []),
@@ -49,6 +77,10 @@ SIGNATURES = [
      [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("plt_bsdf_generate_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("plt_bsdf_eval_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),
+    ("plt_scene_destroy", C.c_int, [C.c_void_p]),
+    ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
+    ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
 ]
 
 _lib = None

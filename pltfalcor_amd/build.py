@@ -24,6 +24,7 @@ HIPCC_FLAGS = [
     "-shared",
     "-Wall",
     "-Wno-unused-function",
+    "-pthread",
     # Packed FP32 (v_pk_*_f32) stays off: hipcc would otherwise form it from scalar float code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
@@ -37,7 +38,7 @@ def _hipcc() -> str:
 
 
 def build_library(verbose: bool = True) -> Path:
-    sources = sorted(str(p) for p in (PKG / "csrc").glob("*.hip"))
+    sources = sorted(str(p) for p in (PKG / "csrc").glob("*.hip")) + sorted(str(p) for p in (PKG / "csrc").glob("*.cpp"))
     if not sources:
         raise RuntimeError("no HIP sources under pltfalcor_amd/csrc")
     tmp = LIB.with_suffix(".so.tmp")

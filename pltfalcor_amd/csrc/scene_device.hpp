@@ -1,0 +1,26 @@
+// Device-resident scene buffers (HBM). Owned by plt_scene; raw pointers are passed to kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+#include "../../include/plt_abi.h"
+
+namespace plt {
+
+// Deepest BVH the traversal kernels accept: the LDS stack holds one sibling per level.
+constexpr int kTraversalStackDepth = 64;
+
+struct DeviceSceneBuffers {
+    float4* nodes = nullptr;          // 4 x float4 per BVH node (64 B)
+    float4* tris = nullptr;           // 3 x float4 per triangle, leaf order (48 B)
+    float* positions = nullptr;       // 3 floats per vertex
+    float* normals = nullptr;         // 3 floats per vertex
+    uint32_t* indices = nullptr;      // 3 per triangle
+    uint32_t* tri_mesh = nullptr;     // mesh per triangle
+    uint32_t* mesh_material = nullptr;
+    plt_spectral_profile* profiles = nullptr;
+    uint32_t n_tris = 0, n_nodes = 0;
+    std::vector<void*> owned;
+};
+
+}  // namespace plt

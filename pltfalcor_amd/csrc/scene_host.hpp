@@ -1,0 +1,65 @@
+// Host-side scene representation shared by the scene builder, the BVH builder and the device
+// upload. Device layouts are documented next to the GPU structs in scene_device.hpp.
+#pragma once
+#include <cstdint>
+#include <vector>
+#include "../../include/plt_abi.h"
+
+namespace plt {
+
+struct Aabb {
+    float lo[3] = {3.4e38f, 3.4e38f, 3.4e38f};
+    float hi[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+    void grow(const float* p) {
+        for (int a = 0; a < 3; ++a) { lo[a] = lo[a] < p[a] ? lo[a] : p[a]; hi[a] = hi[a] > p[a] ? hi[a] : p[a]; }
+    }
+    void grow(const Aabb& b) {
+        for (int a = 0; a < 3; ++a) { lo[a] = lo[a] < b.lo[a] ? lo[a] : b.lo[a]; hi[a] = hi[a] > b.hi[a] ? hi[a] : b.hi[a]; }
+    }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return (dx < 0.f || dy < 0.f || dz < 0.f) ? 0.f : 2.f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+// BVH2 node as stored in HBM: both children's boxes live in the parent so one 64 B fetch tests
+// two boxes. Child reference >= 0: inner node index; < 0: leaf, ~ref = (first << 3) | (count - 1).
+struct BvhNode {
+    float b0x[2], b0y[2];   // child 0 lo/hi x, lo/hi y
+    float b1x[2], b1y[2];   // child 1 lo/hi x, lo/hi y
+    float bz[4];            // child 0 lo/hi z, child 1 lo/hi z
+    int32_t child[2];
+    int32_t pad[2];
+};
+static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 B");
+
+struct HostBvh {
+    std::vector<BvhNode> nodes;
+    std::vector<uint32_t> tri_order;  // triangle ids in leaf order
+    int max_depth = 0;
+};
+
+// Builds a binned-SAH BVH over triangles (positions as 3 floats per vertex, indices 3 per tri).
+HostBvh build_bvh(const std::vector<float>& pos, const std::vector<uint32_t>& idx, int max_leaf = 4);
+
+struct HostScene {
+    std::vector<float> positions;       // 3 per vertex
+    std::vector<float> normals;         // 3 per vertex (binary16-rounded)
+    std::vector<uint32_t> indices;      // 3 per triangle
+    std::vector<uint32_t> tri_mesh;
+    std::vector<uint32_t> mesh_first_tri;
+    std::vector<uint32_t> mesh_material;
+    std::vector<plt_material> materials;
+    std::vector<plt_spectral_profile> profiles;
+    plt_camera camera{};
+    HostBvh bvh;
+};
+
+// Procedural workload scene (plt_procedural_scene_desc).
+int build_procedural_scene(const plt_procedural_scene_desc& d, HostScene& out);
+
+// binary16 round-trip helpers for host-side parameter storage.
+uint16_t f32_to_h(float f);
+float h_to_f32(uint16_t h);
+
+}  // namespace plt

@@ -1,0 +1,160 @@
+// Software ray traversal for CDNA4: BVH2 with both child boxes in the parent (one 64 B node
+// fetch = four 16 B loads tests two boxes), triangles re-ordered into leaf order as 48 B records
+// (v0 + triangle id, v1, v2), and a per-lane traversal stack in LDS laid out [depth][lane] so a
+// wave's pushes/pops hit 64 distinct banks.
+//
+// Hit results are independent of the tree: the watertight test (Woop, Benthin, Wald 2013,
+// "Watertight Ray/Triangle Intersection", JCGT 2(1)) runs with FP contraction off and a double
+// fallback on zero edge functions, box tests are conservative (far distance widened by
+// 1 + 2*gamma(3), Ize 2013 "Robust BVH Ray Traversal"), and equal distances resolve to the
+// smallest triangle id. The CPU oracle uses the same arithmetic, so hit ids are bit-exact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace plt {
+
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+};
+struct Hit {
+    uint32_t tri;  // 0xFFFFFFFF = miss
+    float b1, b2, t;
+};
+
+struct TraceScene {
+    const float4* __restrict__ nodes;  // 4 per node
+    const float4* __restrict__ tris;   // 3 per triangle in leaf order
+};
+
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+
+// Watertight ray/triangle state: axis permutation and shear constants.
+struct RayPrep {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float o[3];
+};
+
+__device__ __forceinline__ float pick3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ RayPrep prep_ray(const Ray& r) {
+#pragma clang fp contract(off)
+    RayPrep p;
+    const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+    p.kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    p.kx = p.kz == 2 ? 0 : p.kz + 1;
+    p.ky = p.kx == 2 ? 0 : p.kx + 1;
+    const float dkz = pick3(r.dx, r.dy, r.dz, p.kz);
+    if (dkz < 0.f) { const int t = p.kx; p.kx = p.ky; p.ky = t; }
+    p.Sx = pick3(r.dx, r.dy, r.dz, p.kx) / dkz;
+    p.Sy = pick3(r.dx, r.dy, r.dz, p.ky) / dkz;
+    p.Sz = 1.f / dkz;
+    p.o[0] = r.ox; p.o[1] = r.oy; p.o[2] = r.oz;
+    return p;
+}
+
+// Returns true and (t, b1, b2) when the ray hits the triangle with tmin <= t <= tmax.
+__device__ __forceinline__ bool intersect_tri(const RayPrep& p, float4 v0, float4 v1, float4 v2, float tmin, float tmax,
+                                              float& t_out, float& b1, float& b2) {
+#pragma clang fp contract(off)
+    const float Ak = pick3(v0.x, v0.y, v0.z, p.kx) - pick3(p.o[0], p.o[1], p.o[2], p.kx);
+    const float Al = pick3(v0.x, v0.y, v0.z, p.ky) - pick3(p.o[0], p.o[1], p.o[2], p.ky);
+    const float Az0 = pick3(v0.x, v0.y, v0.z, p.kz) - pick3(p.o[0], p.o[1], p.o[2], p.kz);
+    const float Bk = pick3(v1.x, v1.y, v1.z, p.kx) - pick3(p.o[0], p.o[1], p.o[2], p.kx);
+    const float Bl = pick3(v1.x, v1.y, v1.z, p.ky) - pick3(p.o[0], p.o[1], p.o[2], p.ky);
+    const float Bz0 = pick3(v1.x, v1.y, v1.z, p.kz) - pick3(p.o[0], p.o[1], p.o[2], p.kz);
+    const float Ck = pick3(v2.x, v2.y, v2.z, p.kx) - pick3(p.o[0], p.o[1], p.o[2], p.kx);
+    const float Cl = pick3(v2.x, v2.y, v2.z, p.ky) - pick3(p.o[0], p.o[1], p.o[2], p.ky);
+    const float Cz0 = pick3(v2.x, v2.y, v2.z, p.kz) - pick3(p.o[0], p.o[1], p.o[2], p.kz);
+    const float Ax = Ak - p.Sx * Az0, Ay = Al - p.Sy * Az0;
+    const float Bx = Bk - p.Sx * Bz0, By = Bl - p.Sy * Bz0;
+    const float Cx = Ck - p.Sx * Cz0, Cy = Cl - p.Sy * Cz0;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    const float det = U + V + W;
+    if (det == 0.f) return false;
+    const float Az = p.Sz * Az0, Bz = p.Sz * Bz0, Cz = p.Sz * Cz0;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    t_out = t;
+    b1 = V / det;
+    b2 = W / det;
+    return true;
+}
+
+// Conservative slab test of one child box; returns the entry distance or +inf on a miss.
+__device__ __forceinline__ float box_entry(float lox, float hix, float loy, float hiy, float loz, float hiz, float ix,
+                                           float iy, float iz, float oix, float oiy, float oiz, float tmin, float tmax) {
+    const float tx0 = fmaf(lox, ix, -oix), tx1 = fmaf(hix, ix, -oix);
+    const float ty0 = fmaf(loy, iy, -oiy), ty1 = fmaf(hiy, iy, -oiy);
+    const float tz0 = fmaf(loz, iz, -oiz), tz1 = fmaf(hiz, iz, -oiz);
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    return tn <= tf * 1.0000005f ? tn : __builtin_inff();
+}
+
+// Traverses the BVH. kAnyHit: stop at the first hit in [tmin, tmax] (visibility rays).
+// `stack` points at this lane's first LDS slot; consecutive slots are `stride` ints apart.
+template <bool kAnyHit, int kDepth>
+__device__ __forceinline__ Hit traverse(const TraceScene& sc, const Ray& r, int* stack, int stride) {
+    Hit h{kNoHit, 0.f, 0.f, r.tmax};
+    const RayPrep p = prep_ray(r);
+    const float dx = fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx);
+    const float dy = fabsf(r.dy) > 1e-30f ? r.dy : copysignf(1e-30f, r.dy);
+    const float dz = fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz);
+    const float ix = 1.f / dx, iy = 1.f / dy, iz = 1.f / dz;
+    const float oix = r.ox * ix, oiy = r.oy * iy, oiz = r.oz * iz;
+    int sp = 0;
+    int node = 0;
+    float best = r.tmax;
+    while (true) {
+        if (node >= 0) {
+            const float4 n0 = sc.nodes[4 * node + 0];
+            const float4 n1 = sc.nodes[4 * node + 1];
+            const float4 n2 = sc.nodes[4 * node + 2];
+            const float4 n3 = sc.nodes[4 * node + 3];
+            const float e0 = box_entry(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            const float e1 = box_entry(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            const int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+            const bool h0 = e0 != __builtin_inff(), h1 = e1 != __builtin_inff();
+            if (h0 && h1) {
+                const bool first0 = e0 <= e1;
+                node = first0 ? c0 : c1;
+                if (sp < kDepth) stack[(sp++) * stride] = first0 ? c1 : c0;
+                continue;
+            }
+            if (h0 || h1) { node = h0 ? c0 : c1; continue; }
+        } else {
+            const uint32_t leaf = uint32_t(~node);
+            const uint32_t start = leaf >> 3, cnt = (leaf & 7u) + 1u;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const float4 v0 = sc.tris[3 * (start + i) + 0];
+                const float4 v1 = sc.tris[3 * (start + i) + 1];
+                const float4 v2 = sc.tris[3 * (start + i) + 2];
+                float t, b1, b2;
+                if (intersect_tri(p, v0, v1, v2, r.tmin, best, t, b1, b2)) {
+                    const uint32_t id = __float_as_uint(v0.w);
+                    if (t < best || id < h.tri) {
+                        best = t;
+                        h.tri = id; h.b1 = b1; h.b2 = b2; h.t = t;
+                        if (kAnyHit) return h;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = stack[(--sp) * stride];
+    }
+    return h;
+}
+
+}  // namespace plt

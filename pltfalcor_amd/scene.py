@@ -1,0 +1,70 @@
+"""Scenes (geometry + BVH + materials + camera in HBM) and batched ray casting over the C ABI."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+from ._lib import Camera, Material, ProceduralSceneDesc, SceneArrays  # noqa: F401
+
+
+# Workload scenes: sphere count / subdivision / material kinds
+PATH_1 = dict(seed=1, sphere_count=64, subdivision=5, material_kinds=4)   # 512x512, ~1.31M triangles
+PATH_2 = dict(seed=1, sphere_count=256, subdivision=5, material_kinds=4)  # 1920x1080, ~5.24M triangles
+
+
+class Scene:
+    """A scene on one GPU (or host-only when ctx is None)."""
+
+    def __init__(self, ctx=None, seed=1, sphere_count=64, subdivision=5, material_kinds=4, ground_size=20.0,
+                 aspect=1.0):
+        self.lib = _lib.load()
+        self.ctx = ctx
+        d = ProceduralSceneDesc(seed, sphere_count, subdivision, material_kinds, ground_size, aspect)
+        h = C.c_void_p()
+        _lib.check(self.lib.plt_scene_create_procedural(ctx.handle if ctx is not None else None, C.byref(d), C.byref(h)))
+        self.handle = h
+        a = SceneArrays()
+        _lib.check(self.lib.plt_scene_get_arrays(h, C.byref(a)))
+        self.arrays = a
+
+    def numpy(self):
+        """Host copies of the scene arrays."""
+        a = self.arrays
+        cp = lambda p, n, dt: np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy()  # noqa: E731
+        return dict(
+            positions=cp(a.positions, 3 * a.n_vertices, np.float32).reshape(-1, 3),
+            normals=cp(a.normals, 3 * a.n_vertices, np.float32).reshape(-1, 3),
+            indices=cp(a.indices, 3 * a.n_triangles, np.uint32).reshape(-1, 3),
+            tri_mesh=cp(a.tri_mesh, a.n_triangles, np.uint32),
+            mesh_first_tri=cp(a.mesh_first_tri, a.n_meshes, np.uint32),
+            mesh_material=cp(a.mesh_material, a.n_meshes, np.uint32),
+        )
+
+    def trace(self, rays: torch.Tensor, any_hit: bool = False, stream=None) -> torch.Tensor:
+        """rays: [8, n] float32 on the GPU. Returns [4, n] int32 (id, b1, b2, t bits) or [n] int32 (0/1)."""
+        if rays.dtype != torch.float32 or rays.dim() != 2 or rays.shape[0] != 8 or not rays.is_cuda:
+            raise ValueError("rays must be a float32 CUDA tensor of shape [8, n]")
+        rays = rays.contiguous()
+        n = rays.shape[1]
+        out = torch.empty((1 if any_hit else 4, n), dtype=torch.int32, device=rays.device)
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        _lib.check(self.lib.plt_scene_trace(self.handle, n, C.c_void_p(rays.data_ptr()), C.c_void_p(out.data_ptr()),
+                                            1 if any_hit else 0, C.c_void_p(stream.cuda_stream)))
+        return out[0] if any_hit else out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.plt_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -101,3 +101,38 @@ def c0_eval(queries: np.ndarray, profiles: bytes, nprof: int, fp64: bool = False
 
 def dvec(*v) -> np.ndarray:
     return np.array(v, dtype=np.float64)
+
+
+_SIGS_GEOM = {
+    "oracle_trace": (C.c_int, [_F, C.c_uint32, _U32, C.c_uint32, _F, _U32, C.c_int, C.c_int, C.c_int]),
+    "oracle_compute_ray_origin": (None, [_F, _F, _F]),
+}
+
+
+def _geom():
+    l = lib()
+    for name, (res, args) in _SIGS_GEOM.items():
+        fn = getattr(l, name)
+        fn.restype = res
+        fn.argtypes = args
+    return l
+
+
+def trace(positions: np.ndarray, indices: np.ndarray, rays: np.ndarray, any_hit=False, brute=False, threads=0):
+    """positions [V,3] f32, indices [T,3] u32, rays [8,n] f32 -> [4,n] u32 or [n] u32."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    idx = np.ascontiguousarray(indices, dtype=np.uint32)
+    r = np.ascontiguousarray(rays, dtype=np.float32)
+    n = r.shape[1]
+    out = np.zeros((1 if any_hit else 4, n), dtype=np.uint32)
+    _geom().oracle_trace(ptr(pos, C.c_float), idx.shape[0], ptr(idx, C.c_uint32), n, ptr(r, C.c_float),
+                         ptr(out, C.c_uint32), 1 if any_hit else 0, 1 if brute else 0, threads)
+    return out[0] if any_hit else out
+
+
+def compute_ray_origin(pos, n):
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    nn = np.ascontiguousarray(n, dtype=np.float32)
+    out = np.zeros(3, np.float32)
+    _geom().oracle_compute_ray_origin(ptr(p, C.c_float), ptr(nn, C.c_float), ptr(out, C.c_float))
+    return out
