@@ -225,6 +225,51 @@ int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
  * ------------------------------------------------------------------------------------- */
 int plt_scene_trace(plt_scene* scene, uint32_t n, const float* rays, uint32_t* hits, int mode, plt_stream stream);
 
+/* ---------------------------------------------------------------------------------------
+ * The PLTPT render pass (Source/RenderPasses/PLTPT/PLTPT.h:42-121). Replaces the reference's
+ * plugin class (registerPlugin / RenderPass::create/execute/setScene, PLTPT.cpp:37-46,139-573).
+ * Options are the reference's dictionary keys with numeric values (booleans as 0/1, enums as
+ * their integer value) and the reference's defaults; an unknown key returns
+ * PLT_WARN_UNKNOWN_KEY and changes nothing (PLTPT.cpp:173 logs and continues).
+ * ------------------------------------------------------------------------------------- */
+typedef struct plt_pass plt_pass;
+
+/* type must be "PLTPT". ctx may be NULL for a pass that only holds options (set_scene and
+ * execute then return PLT_ESTATE). */
+int plt_pass_create(plt_context* ctx, const char* type, plt_pass** out);
+int plt_pass_destroy(plt_pass* pass);
+int plt_pass_set_option(plt_pass* pass, const char* key, double value);
+int plt_pass_get_option(const plt_pass* pass, const char* key, double* value);
+/* i-th option key (NULL past the end), for getScriptingDictionary-style enumeration. */
+const char* plt_pass_option_name(uint32_t i);
+/* Binds a scene (NULL unbinds) and resets the frame counter to 0 (PLTPT.cpp:501-552). */
+int plt_pass_set_scene(plt_pass* pass, plt_scene* scene);
+/* Frame counter: seeds UniformSampleGenerator(pixel, frame) of the next execute. */
+int plt_pass_set_frame(plt_pass* pass, uint32_t frame);
+
+/* I/O of one execute (PLTPT.cpp:65-78): caller-owned device buffers of width*height float4
+ * (RGBA32F): color (spectral radiance -> linear Rec.709), albedo and shading normal of the first
+ * non-delta vertex (world space). vbuffer is accepted and ignored, as in the reference. */
+typedef struct plt_render_data {
+    uint32_t width, height;
+    float* color;
+    float* albedo;
+    float* normal;
+    const void* vbuffer;
+} plt_render_data;
+
+/* Renders one frame (sample pass + solve pass) on `stream`, then increments the frame counter.
+ * Options not built yet (doMNEE=1, NEEUsePerTileSG=1, non-power emissive sampler, debug views,
+ * HWSS other than 1 or 4) return PLT_ENOTIMPL. */
+int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream stream);
+
+/* Copies the bounce records of the last execute to host memory `dst` (synchronous). Layout:
+ * (maxBounces+1) x 9 planes of width*height 32-bit words, plane (bounce*9 + field), fields:
+ * triangle id (0xFFFFFFFF = miss), b1, b2, octahedral wi.x, wi.y, sampled lobe, path flags,
+ * path contribution 1/(pdf*thp), last non-delta pdf (the reference's BounceData, Path.slang:113-142).
+ * Returns PLT_EINVAL if `bytes` is smaller than the record buffer. */
+int plt_pass_copy_bounce_records(plt_pass* pass, void* dst, size_t bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,535 @@
+// PLT oracle: the PLTPT sample and solve passes on the CPU (C entry point oracle_render).
+//
+// TEST INFRASTRUCTURE ONLY (see plt_oracle.hpp). Restates, per pixel:
+//   sample pass  Source/RenderPasses/PLTPT/pltpt_sample.rt.slang:36-185, Path.slang:146-196,
+//                pltpt.slangh:132-176
+//   solve pass   Source/RenderPasses/PLTPT/pltpt_solve.rt.slang:37-476 (manifold NEE off,
+//                no environment or analytic lights), LightSampling.slangh:40-206,
+//                Rendering/Lights/EmissivePowerSampler.slang:62-121,
+//                EmissiveLightSamplerHelpers.slang:56-150, Scene/Scene.slang:251-327.
+// Camera, shading frames and emissive tables are built here from the scene arrays, independently
+// of the product.
+#include <omp.h>
+#include <vector>
+#include "plt_oracle_pt.hpp"
+
+using namespace pltoracle;
+
+namespace {
+
+using F = float;
+constexpr int HW = 4;
+using Beam = PLTBeam<F, HW>;
+constexpr float kFLT_MIN = 1.175494351e-38f;
+
+struct OScene {
+    const float* pos;
+    const float* nrm;
+    const uint32_t* idx;
+    const uint32_t* triMesh;
+    const uint32_t* meshMat;
+    const MatData* mats;
+    const SpectralProfile* prof;
+    uint32_t nprof, ntri;
+    OMesh mesh;
+    OBvh bvh;
+    // camera (Camera.cpp:187-193)
+    V3<F> camPos, camU, camV, camW;
+    float jitterX, jitterY;
+    // emissive triangles
+    std::vector<uint32_t> lightTri;
+    std::vector<float> lightArea, lightPdf;
+    std::vector<V3<F>> lightN;
+    std::vector<uint32_t> triToLight;
+    std::vector<uint32_t> aliasX, aliasY;
+};
+
+struct Options {
+    uint32_t width, height, frame, maxBounces;
+    int doNEE, doMIS, doRR, hwssMIS, isEmitters, directLights, emissiveLights;
+    float sourcingArea, maxOmega;
+};
+
+// ndir_to_oct_equal_area_snorm / oct_to_ndir_equal_area_snorm (MathHelpers.slang:199-240)
+V2<F> ndirToOct(V3<F> n) {
+    const F r = std::sqrt(F(1) - std::fabs(n.z));
+    const F phi = std::atan2(std::fabs(n.y), std::fabs(n.x));
+    V2<F> p;
+    p.y = r * phi * K<F>::_2_PI;
+    p.x = r - p.y;
+    if (n.z < F(0)) p = V2<F>{F(1) - p.y, F(1) - p.x};
+    return V2<F>{p.x * sgn(n.x), p.y * sgn(n.y)};
+}
+V3<F> octToNdir(V2<F> p) {
+    const F d = F(1) - (std::fabs(p.x) + std::fabs(p.y));
+    const F r = F(1) - std::fabs(d);
+    const F phi = r > F(0) ? ((std::fabs(p.y) - std::fabs(p.x)) / r + F(1)) * K<F>::PI_4 : F(0);
+    const F f = r * std::sqrt(F(2) - r * r);
+    return V3<F>{f * sgn(p.x) * std::cos(phi), f * sgn(p.y) * std::sin(phi), sgn(d) * (F(1) - r * r)};
+}
+
+struct Bounce {  // BounceData (Path.slang:113-142)
+    uint32_t tri = 0xFFFFFFFFu;
+    float b1 = 0, b2 = 0;
+    V2<F> encWi{0, 0};
+    uint32_t lobe = 0, flags = 0;
+    float pathContribution = 0, lastNonDeltaPdf = 0;
+    V3<F> wi() const { return octToNdir(encWi); }
+};
+enum : uint32_t { RF_LastDelta = 0x1, RF_Dispersive = 0x2, RF_OnlyDelta = 0x4, RF_Miss = 0x10, RF_Terminated = 0x20 };
+
+// getVertexData + prepareShadingData (Scene.slang:530-567, MaterialFactory.slang:79-130,
+// ShadingUtils.slang:126-153). No tangents -> perp_stark frame.
+SD<F> shadingData(const OScene& s, uint32_t tri, float b1, float b2, V3<F> viewDir) {
+    const V3<F> bary = {F(1) - b1 - b2, b1, b2};
+    V3<F> p[3], n[3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = s.idx[3 * size_t(tri) + k];
+        p[k] = V3<F>{s.pos[3 * size_t(v)], s.pos[3 * size_t(v) + 1], s.pos[3 * size_t(v) + 2]};
+        n[k] = V3<F>{s.nrm[3 * size_t(v)], s.nrm[3 * size_t(v) + 1], s.nrm[3 * size_t(v) + 2]};
+    }
+    SD<F> sd;
+    sd.posW = p[0] * bary.x;
+    sd.posW = sd.posW + p[1] * bary.y;
+    sd.posW = sd.posW + p[2] * bary.z;
+    V3<F> nw = n[0] * bary.x;
+    nw = nw + n[1] * bary.y;
+    nw = nw + n[2] * bary.z;
+    sd.f.N = normalize(nw);
+    sd.faceN = normalize(cross(p[1] - p[0], p[2] - p[0]));
+    sd.f.T_ = perp_stark(sd.f.N);
+    sd.f.B = cross(sd.f.N, sd.f.T_);
+    sd.V = viewDir;
+    sd.frontFacing = dot(sd.V, sd.faceN) >= F(0);
+    sd.materialID = s.meshMat[s.triMesh[tri]];
+    return sd;
+}
+V3<F> rayOrigin(V3<F> p, V3<F> n) {
+    const float pp[3] = {p.x, p.y, p.z}, nn[3] = {n.x, n.y, n.z};
+    float o[3];
+    compute_ray_origin(pp, nn, o);
+    return V3<F>{o[0], o[1], o[2]};
+}
+Instance<F> instanceAt(const OScene& s, uint32_t m) { return setupMaterialInstance<F>(s.mats[m], s.prof, s.nprof); }
+
+// Scene::sampleSpectrum (Scene.slang:296-327) with HW = 4 HWSS samples.
+template <class S> void sampleSpectrum(const SpectralProfile* sp, float hero, S& sg, bool importance, float* Le, float* lam) {
+    if (!sp) { for (int j = 0; j < HW; ++j) { Le[j] = 0; lam[j] = hero; } return; }
+    if (importance) {
+        lam[0] = hero;
+        Le[0] = evalSpectrum<float>(sp, hero);
+        float x[3];
+        for (int k = 0; k < 3; ++k) x[k] = float(sp->bins) * sampleNext1D<float>(sg);
+        for (int j = 1; j < HW; ++j) {
+            const uint32_t id = uint32_t(x[j - 1]);
+            const float a = id < sp->bins ? sp->icdf[id] : 1.f;
+            const float b = id + 1 < sp->bins ? sp->icdf[id + 1] : 1.f;
+            const float cdf = hlerp(a, b, x[j - 1] - std::floor(x[j - 1]));
+            lam[j] = hlerp(sp->minWavelength, sp->maxWavelength, cdf);
+            Le[j] = sp->intensity / float(sp->bins);
+        }
+    } else {
+        Spectrum<float>::hwssWavelengths(hero, HW, lam);
+        for (int j = 0; j < HW; ++j) Le[j] = evalSpectrum<float>(sp, lam[j]);
+    }
+}
+// pltSourceBeamArea (PLT.slang:275-290)
+Beam sourceBeamArea(V3<F> posW, V3<F> dir, float A, float dist, const float* Le, const float* lam, float maxOmega, int comps) {
+    const float r2 = sqr(dist * 1e+3f);
+    A = std::fmin(A, maxOmega * r2);
+    Beam b;
+    for (int j = 0; j < HW; ++j) { b.L[j] = Le[j]; b.L1[j] = b.L2[j] = b.L3[j] = 0; b.wavelengths[j] = lam[j]; b.spectralPDFs[j] = 1; }
+    b.hwss_comps = comps;
+    b.Theta.D[0] = A; b.Theta.D[1] = 0; b.Theta.D[2] = A;
+    b.Theta.r_m = 0;
+    b.posW = posW;
+    b.r = dir;
+    b.t = perp_stark(dir);
+    b.distant = false;
+    return b;
+}
+
+// ---------------------------------------------------------------- sample pass
+void samplePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, std::vector<Bounce>& rec) {
+    UniformSampleGenerator sg(px, py, o.frame);
+    const float lambda = Spectrum<float>::sampleOne(sampleNext1D<float>(sg));
+    // Camera.computeRayPinhole (Camera.slang:46-78)
+    float sx = (float(px) + 0.5f) / float(o.width), sy = (float(py) + 0.5f) / float(o.height);
+    sx += -s.jitterX;
+    sy += s.jitterY;
+    const float nx = 2.f * sx + -1.f, ny = -2.f * sy + 1.f;
+    V3<F> org = s.camPos;
+    V3<F> dir = normalize(s.camU * nx + s.camV * ny + s.camW);
+    float weight = 1, pdf = 1, thp = 1, lastNonDeltaPdf = 0;
+    uint32_t flags = RF_OnlyDelta;
+    for (uint32_t len = 0; len < o.maxBounces + 1;) {
+        ORay ray;
+        ray.o[0] = org.x; ray.o[1] = org.y; ray.o[2] = org.z;
+        ray.d[0] = dir.x; ray.d[1] = dir.y; ray.d[2] = dir.z;
+        ray.tmin = 0.f; ray.tmax = 3.402823466e+38f;
+        const OHit h = s.bvh.closest(s.mesh, ray, false);
+        const V3<F> wi = -dir;
+        Bounce& b = rec[len];
+        if (h.tri == 0xFFFFFFFFu) {
+            flags |= RF_Miss;
+            b = Bounce{};
+            b.flags = flags; b.encWi = ndirToOct(wi); b.lastNonDeltaPdf = lastNonDeltaPdf; b.pathContribution = 1.f / (pdf * thp);
+            break;
+        }
+        const SD<F> sd = shadingData(s, h.tri, h.b1, h.b2, wi);
+        const Instance<F> in = instanceAt(s, sd.materialID);
+        const uint32_t props = getPropertyFlags(in);
+        float rcp = 1.f;
+        if (o.doRR) {
+            const float u = sampleNext1D<float>(sg);
+            const float p = std::pow(1.f - weight, 2.f);
+            rcp = u < p ? 0.f : 1.f - p;
+        }
+        bool end = len == o.maxBounces || (rcp == 0.f && len > 0);
+        BSDFSample bs;
+        if (!end) {
+            end = !materialSample(in, sd, sg, bs, lambda) || bs.pdf == 0.f;
+            end |= pdf * bs.pdf * thp * rcp < 1e-6f;
+        }
+        const bool isDelta = (bs.lobe & Lobe_Delta) != 0, isDisp = (bs.lobe & Lobe_DispersiveDelta) != 0, isRefl = (bs.lobe & Lobe_Reflection) != 0;
+        const bool isMS = (isRefl && (props & Prop_MWR)) || (!isRefl && (props & Prop_MWT));
+        b.tri = h.tri; b.b1 = h.b1; b.b2 = h.b2; b.encWi = ndirToOct(wi); b.lastNonDeltaPdf = lastNonDeltaPdf;
+        if (end) {
+            flags |= RF_Terminated;
+            b.flags = flags; b.lobe = 0; b.pathContribution = 1.f / (pdf * thp);
+            break;
+        }
+        weight *= bs.weight;
+        thp *= rcp;
+        pdf *= bs.pdf;
+        if (isDisp) flags |= RF_Dispersive;
+        b.flags = flags; b.lobe = bs.lobe; b.pathContribution = 1.f / (pdf * thp);
+        ++len;
+        lastNonDeltaPdf = !isDelta ? bs.pdf : lastNonDeltaPdf;
+        const uint32_t msc = isMS ? (flags >> 16) + 1u : 0u;
+        flags = (flags & 0xffffu) | (msc << 16);
+        flags = isDelta ? (flags | RF_LastDelta) : (flags & ~RF_LastDelta);
+        if (!isDelta) flags &= ~RF_OnlyDelta;
+        org = rayOrigin(sd.posW, (sd.frontFacing == isRefl) ? sd.faceN : -sd.faceN);
+        dir = bs.wo;
+    }
+}
+
+// ---------------------------------------------------------------- solve pass
+struct SolveData { V3<F> Lrgb{0, 0, 0}; V3<F> pathOrigin, pathNormal; float wavelength; };
+
+void forwardUnwind(const OScene& s, const Options& o, const std::vector<Bounce>& rec, int bounceIdx, Beam& beam) {
+    for (int p = bounceIdx - 1; p >= 0; --p) {
+        const Bounce& b = rec[size_t(p)];
+        const V3<F> wo = b.wi();
+        const SD<F> sd = shadingData(s, b.tri, b.b1, b.b2, -beam.r);
+        beam.propagate(sd.posW);
+        const Instance<F> in = instanceAt(s, sd.materialID);
+        materialEval<F, HW>(in, sd, wo, b.lobe, beam);
+        if (o.hwssMIS) {
+            float pd[HW];
+            materialLambdaPdf<F, HW>(in, sd, wo, b.lobe, beam, pd);
+            for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+        }
+    }
+}
+void measure(const OScene& s, const Options& o, SolveData& sv, int bounceIdx, float pathContribution, Beam& beam) {
+    if (!o.directLights && bounceIdx <= 1) return;
+    beam.propagate(s.camPos);
+    if (o.hwssMIS) {
+        float sum = 0;
+        for (int j = 0; j < HW; ++j) sum += beam.spectralPDFs[j];
+        pathContribution *= sum > 1.192092896e-07f ? beam.spectralPDFs[0] / sum : 0.f;
+    } else pathContribution *= 1.f / float(beam.hwss_comps);
+    const V3<F> rgb = Spectrum<float>::spectrumToRgb(beam.L, beam.wavelengths, unsigned(beam.hwss_comps)) * pathContribution;
+    const V3<F> L = {std::fmax(0.f, rgb.x), std::fmax(0.f, rgb.y), std::fmax(0.f, rgb.z)};
+    if (std::isfinite(L.x) && std::isfinite(L.y) && std::isfinite(L.z)) sv.Lrgb = sv.Lrgb + L;
+}
+
+template <class S>
+void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, const std::vector<Bounce>& rec, float* out) {
+    UniformSampleGenerator sg(px, py, o.frame);
+    SolveData sv;
+    sv.wavelength = Spectrum<float>::sampleOne(sampleNext1D<float>(sg));
+    sv.pathOrigin = s.camPos;
+    sv.pathNormal = normalize(s.camW);
+    const bool useEmissive = o.emissiveLights && !s.lightTri.empty();
+    const bool doNEE = o.doNEE != 0, doMIS = doNEE && o.doMIS;
+    for (int p = 0;; ++p) {
+        const Bounce& bounce = rec[size_t(p)];
+        if (bounce.flags & RF_Miss) break;
+        const V3<F> bwi = bounce.wi();
+        const SD<F> sd = shadingData(s, bounce.tri, bounce.b1, bounce.b2, bwi);
+        const Instance<F> bsdf = instanceAt(s, sd.materialID);
+        const uint32_t props = getPropertyFlags(bsdf);
+        if (doNEE) {
+            const uint32_t lobes = getLobes(bsdf);
+            const bool sampleRefl = (lobes & Lobe_NonDeltaReflection) != 0, sampleTrans = (lobes & Lobe_NonDeltaTransmission) != 0;
+            if (sampleRefl || sampleTrans) {
+                // generateLightSample: light type selection (only emissive lights present)
+                const float pType = useEmissive ? 1.f : 0.f;
+                float u = sampleNext1D<float>(sg);
+                bool valid = false;
+                float lsPdf = 0, lsDist = 0;
+                V3<F> lsDir{0, 0, 0}, lsOrigin{0, 0, 0}, tPos{0, 0, 0};
+                float Le[HW], lam[HW];
+                for (int j = 0; j < HW; ++j) { Le[j] = 0; lam[j] = sv.wavelength; }
+                if (u < pType) {
+                    // EmissivePowerSampler.sampleLight
+                    const uint32_t nl = uint32_t(s.lightTri.size());
+                    const float uL = sampleNext1D<float>(sg);
+                    uint32_t ti = std::min(uint32_t(uL * float(nl)), nl - 1);
+                    const uint32_t ax = s.aliasX[ti], ay = s.aliasY[ti];
+                    const float threshold = f16_bits_to_f32(uint16_t(ax >> 16u));
+                    const uint32_t above = ((ax & 0xFFFFu) << 8u) | ((ay >> 24u) & 0xFFu), below = ay & 0xFFFFFFu;
+                    ti = sampleNext1D<float>(sg) >= threshold ? above : below;
+                    const float selPdf = s.lightPdf[ti];
+                    // sampleTriangle
+                    const V2<F> u2 = sampleNext2D<float>(sg);
+                    const float su = std::sqrt(u2.x);
+                    const V3<F> bary = {1.f - (1.f - su) - u2.y * su, 1.f - su, u2.y * su};
+                    const uint32_t t = s.lightTri[ti];
+                    V3<F> pv[3];
+                    for (int k = 0; k < 3; ++k) {
+                        const uint32_t v = s.idx[3 * size_t(t) + k];
+                        pv[k] = V3<F>{s.pos[3 * size_t(v)], s.pos[3 * size_t(v) + 1], s.pos[3 * size_t(v) + 2]};
+                    }
+                    tPos = pv[0] * bary.x + pv[1] * bary.y + pv[2] * bary.z;
+                    const V3<F> toL = tPos - sd.posW;
+                    const float d2 = std::fmax(kFLT_MIN, dot(toL, toL));
+                    const float dist = std::sqrt(d2);
+                    const V3<F> ldir = toL / dist;
+                    const V3<F> ln = s.lightN[ti];
+                    const float cosT = dot(ln, -ldir);
+                    if (cosT > 0.f) {
+                        const MatData& em = s.mats[s.meshMat[s.triMesh[t]]];
+                        const SpectralProfile* ep = em.emission < s.nprof ? s.prof + em.emission : nullptr;
+                        sampleSpectrum(ep, sv.wavelength, sg, o.isEmitters != 0, Le, lam);
+                        for (int j = 0; j < HW; ++j) Le[j] *= Spectrum<float>::rgbToSpectrum(V3<F>{1, 1, 1}, lam[j]);
+                        float tpdf = d2 / std::fmax(kFLT_MIN, cosT * s.lightArea[ti]);
+                        tpdf *= selPdf;
+                        for (int j = 0; j < HW; ++j) Le[j] = tpdf > 0.f ? Le[j] / tpdf : 0.f;
+                        lsPdf = tpdf;
+                        const V3<F> lightPos = rayOrigin(tPos, ln);
+                        const V3<F> rd = lightPos - sd.posW;
+                        lsOrigin = rayOrigin(sd.posW, dot(sd.faceN, rd) >= 0.f ? sd.faceN : -sd.faceN);
+                        const V3<F> tl = lightPos - lsOrigin;
+                        lsDist = length(tl);
+                        lsDir = normalize(tl);
+                        valid = false;
+                        for (int j = 0; j < HW; ++j) valid |= Le[j] > 0.f;
+                    }
+                }
+                const bool isRefl = dot(sd.faceN, lsDir) * dot(sd.faceN, sd.V) > 0.f;
+                valid = valid && lsPdf > 1e-6f && !(!sampleRefl && isRefl) && !(!sampleTrans && !isRefl);
+                bool anyL = false;
+                for (int j = 0; j < HW; ++j) anyL |= Le[j] > 0.f;
+                if (valid && anyL) {
+                    const int comps = (!o.hwssMIS && (bounce.flags & RF_Dispersive)) ? 1 : HW;
+                    ORay vr;
+                    vr.o[0] = lsOrigin.x; vr.o[1] = lsOrigin.y; vr.o[2] = lsOrigin.z;
+                    vr.d[0] = lsDir.x; vr.d[1] = lsDir.y; vr.d[2] = lsDir.z;
+                    vr.tmin = 0.f; vr.tmax = lsDist;
+                    const OHit vh = s.bvh.closest(s.mesh, vr, true);
+                    if (vh.tri == 0xFFFFFFFFu) {
+                        Beam beam = sourceBeamArea(tPos, -lsDir, o.sourcingArea, lsDist, Le, lam, o.maxOmega, comps);
+                        if (doMIS) {
+                            const float bp = materialPdf(bsdf, sd, lsDir, beam.wavelengths[0]);
+                            beam.scaleAll(lsPdf / (lsPdf + bp));
+                        }
+                        // evalNEE
+                        const SD<F> fsd = shadingData(s, bounce.tri, bounce.b1, bounce.b2, -beam.r);
+                        beam.propagate(fsd.posW);
+                        materialEval<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam);
+                        if (o.hwssMIS) {
+                            float pd[HW];
+                            materialLambdaPdf<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam, pd);
+                            for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+                        }
+                        forwardUnwind(s, o, rec, p, beam);
+                        measure(s, o, sv, p, bounce.pathContribution, beam);
+                    }
+                }
+            }
+        }
+        // evalEmissive
+        const bool didNEE = doNEE && p > 0 && !(bounce.flags & RF_LastDelta);
+        if (useEmissive && (props & Prop_Emissive) && (doMIS || !didNEE)) {
+            const bool needMIS = doMIS && didNEE;
+            float lightPdf = 0.f;
+            const uint32_t lt = s.triToLight[bounce.tri];
+            if (lt != 0xFFFFFFFFu) {
+                const V3<F> hn = sd.frontFacing ? sd.faceN : -sd.faceN;
+                const V3<F> toL = sd.posW - sv.pathOrigin;
+                const float d2 = dot(toL, toL);
+                if (d2 > kFLT_MIN) {
+                    const V3<F> L = toL / std::sqrt(d2);
+                    const float ct = dot(hn, -L);
+                    if (ct > 0.f) lightPdf = s.lightPdf[lt] * (d2 / std::fmax(kFLT_MIN, ct * s.lightArea[lt]));
+                }
+            }
+            const float mis = needMIS ? bounce.lastNonDeltaPdf / (bounce.lastNonDeltaPdf + lightPdf) : 1.f;
+            const SpectralProfile* ep = bsdf.emission < s.nprof ? s.prof + bsdf.emission : nullptr;
+            float Le[HW], lam[HW];
+            sampleSpectrum(ep, sv.wavelength, sg, o.isEmitters != 0, Le, lam);
+            bool anyL = false;
+            for (int j = 0; j < HW; ++j) { Le[j] *= mis * Spectrum<float>::rgbToSpectrum(V3<F>{1, 1, 1}, lam[j]); anyL |= Le[j] > 0.f; }
+            if (anyL) {
+                const float dist = length(sv.pathOrigin - sd.posW);
+                const int comps = (!o.hwssMIS && (bounce.flags & RF_Dispersive)) ? 1 : HW;
+                Beam beam = sourceBeamArea(sd.posW, bwi, o.sourcingArea, dist, Le, lam, o.maxOmega, comps);
+                forwardUnwind(s, o, rec, p, beam);
+                measure(s, o, sv, p, bounce.pathContribution, beam);
+            }
+        }
+        if ((bounce.flags & RF_Terminated) || p == int(o.maxBounces)) break;
+        sv.pathOrigin = sd.posW;
+        sv.pathNormal = sd.f.N;
+    }
+    out[0] = sv.Lrgb.x; out[1] = sv.Lrgb.y; out[2] = sv.Lrgb.z; out[3] = 1.f;
+}
+
+void buildLights(OScene& s) {
+    s.triToLight.assign(s.ntri, 0xFFFFFFFFu);
+    std::vector<float> flux;
+    for (uint32_t t = 0; t < s.ntri; ++t) {
+        const MatData& m = s.mats[s.meshMat[s.triMesh[t]]];
+        if (m.type != MT_DIFFUSE || m.emission == kNoProfile) continue;
+        V3<F> p[3];
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = s.idx[3 * size_t(t) + k];
+            p[k] = V3<F>{s.pos[3 * size_t(v)], s.pos[3 * size_t(v) + 1], s.pos[3 * size_t(v) + 2]};
+        }
+        const V3<F> c = cross(p[1] - p[0], p[2] - p[0]);
+        const float len = length(c);
+        s.triToLight[t] = uint32_t(s.lightTri.size());
+        s.lightTri.push_back(t);
+        s.lightArea.push_back(0.5f * len);
+        s.lightN.push_back(c / len);
+        flux.push_back(0.5f * len * float(M_PI));  // luminance x area x pi; one emissive profile -> luminance cancels
+    }
+    const uint32_t N = uint32_t(flux.size());
+    if (!N) return;
+    // EmissivePowerSampler::generateAliasTable (EmissivePowerSampler.cpp:86-176)
+    double sum = 0;
+    for (float f : flux) sum += f;
+    const float inv = 1.f / float(sum);
+    for (float f : flux) s.lightPdf.push_back(f * inv);
+    std::vector<float> w = flux;
+    for (float& f : w) f *= float(N) / float(sum);
+    std::vector<uint32_t> perm(N), redirect(N);
+    std::vector<float> thr(N);
+    for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+    std::sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return w[a] < w[b]; });
+    uint32_t head = 0, tail = N - 1;
+    if (N == 1) { thr[0] = 1.f; redirect[0] = 0; }
+    while (head != tail) {
+        const uint32_t i = perm[head], j = perm[tail];
+        thr[i] = w[i];
+        redirect[i] = j;
+        w[j] -= 1.0f - w[i];
+        if (head == tail - 1) { thr[j] = 1.0f; redirect[j] = j; break; }
+        else if (w[j] < 1.0f) { std::swap(perm[head], perm[tail]); tail--; }
+        else head++;
+    }
+    for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+    std::mt19937 rng;
+    std::uniform_int_distribution<uint32_t> dist;
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t dst = i + (dist(rng) % (N - i));
+        std::swap(thr[i], thr[dst]); std::swap(redirect[i], redirect[dst]); std::swap(perm[i], perm[dst]);
+    }
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t prob = uint32_t(f32_to_f16_bits(thr[i])) << 16u;
+        const uint32_t lx = redirect[i] & 0xFFFFFFu, ly = perm[i] & 0xFFFFFFu;
+        s.aliasX.push_back(prob | ((lx >> 8u) & 0xFFFFu));
+        s.aliasY.push_back(((lx & 0xFFu) << 24u) | ly);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// Camera: pos[3], target[3], up[3], fov_y, aspect, focal_distance, jitter[2] (14 floats).
+// options: width, height, frame, maxBounces, doNEE, doMIS, doRR, HWSSDoMIS, doImportanceSampleEmitters,
+// useDirectLights, useEmissiveLights (11 ints); fopts: sourcingArea, maxOmega.
+// Renders pixels [first, first + count) of the row-major frame; color: 4 floats per rendered pixel;
+// records (optional): (maxBounces+1) x 9 planes of `count` words, as the product's bounce records.
+int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh,
+                  const uint32_t* meshMat, const void* mats, const void* prof, uint32_t nprof, const float* cam,
+                  const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count, float* color, uint32_t* records,
+                  int threads) {
+    OScene s;
+    s.pos = pos; s.nrm = nrm; s.idx = idx; s.triMesh = triMesh; s.meshMat = meshMat;
+    s.mats = static_cast<const MatData*>(mats);
+    s.prof = static_cast<const SpectralProfile*>(prof);
+    s.nprof = nprof; s.ntri = ntri;
+    s.mesh.pos = pos; s.mesh.idx = idx; s.mesh.ntri = ntri;
+    s.bvh.build(s.mesh);
+    // Camera.cpp:187-193 in float
+    {
+        const V3<F> P{cam[0], cam[1], cam[2]}, Tg{cam[3], cam[4], cam[5]}, Up{cam[6], cam[7], cam[8]};
+        const float fovY = cam[9], aspect = cam[10], fd = cam[11];
+        V3<F> W = normalize(Tg - P) * fd;
+        V3<F> U = normalize(cross(W, Up));
+        V3<F> V = normalize(cross(U, W));
+        const float tn = std::tan(fovY * 0.5f);
+        U = U * (fd * tn * aspect);
+        V = V * (fd * tn);
+        s.camPos = P; s.camU = U; s.camV = V; s.camW = W;
+        s.jitterX = cam[12]; s.jitterY = cam[13];
+    }
+    buildLights(s);
+    Options o;
+    o.width = uint32_t(iopt[0]); o.height = uint32_t(iopt[1]); o.frame = uint32_t(iopt[2]); o.maxBounces = uint32_t(iopt[3]);
+    o.doNEE = iopt[4]; o.doMIS = iopt[5]; o.doRR = iopt[6]; o.hwssMIS = iopt[7]; o.isEmitters = iopt[8];
+    o.directLights = iopt[9]; o.emissiveLights = iopt[10];
+    o.sourcingArea = fopt[0]; o.maxOmega = fopt[1];
+    if (threads > 0) omp_set_num_threads(threads);
+    const size_t n = count;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < int64_t(count); ++i) {
+        const uint32_t pid = first + uint32_t(i);
+        const uint32_t px = pid % o.width, py = pid / o.width;
+        std::vector<Bounce> rec(o.maxBounces + 2);
+        samplePass(s, o, px, py, rec);
+        solvePass<UniformSampleGenerator>(s, o, px, py, rec, color + 4 * size_t(i));
+        if (records) {
+            for (uint32_t p = 0; p <= o.maxBounces; ++p) {
+                const Bounce& b = rec[p];
+                uint32_t* r = records + size_t(p) * 9 * n + size_t(i);
+                r[0 * n] = b.tri;
+                std::memcpy(&r[1 * n], &b.b1, 4);
+                std::memcpy(&r[2 * n], &b.b2, 4);
+                std::memcpy(&r[3 * n], &b.encWi.x, 4);
+                std::memcpy(&r[4 * n], &b.encWi.y, 4);
+                r[5 * n] = b.lobe;
+                r[6 * n] = b.flags;
+                std::memcpy(&r[7 * n], &b.pathContribution, 4);
+                std::memcpy(&r[8 * n], &b.lastNonDeltaPdf, 4);
+            }
+        }
+    }
+    return 0;
+}
+
+// The power sampler's alias table, decoded: per entry the binary16 threshold (as float), the
+// redirect ("above") and permuted ("below") light indices, and the per-light selection pdf.
+// Returns the light count (entries beyond `cap` are not written).
+int oracle_alias_table(const float* pos, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh, const uint32_t* meshMat,
+                       const void* mats, float* thr, uint32_t* redirect, uint32_t* perm, float* pdf, uint32_t cap) {
+    OScene s;
+    s.pos = pos; s.idx = idx; s.triMesh = triMesh; s.meshMat = meshMat; s.ntri = ntri;
+    s.mats = static_cast<const MatData*>(mats);
+    buildLights(s);
+    const uint32_t n = uint32_t(s.lightPdf.size());
+    for (uint32_t i = 0; i < n && i < cap; ++i) {
+        thr[i] = f16_bits_to_f32(uint16_t(s.aliasX[i] >> 16u));
+        redirect[i] = ((s.aliasX[i] & 0xFFFFu) << 8u) | (s.aliasY[i] >> 24u);
+        perm[i] = s.aliasY[i] & 0xFFFFFFu;
+        pdf[i] = s.lightPdf[i];
+    }
+    return int(n);
+}
+
+}  // extern "C"

@@ -378,8 +378,10 @@ template <class T> struct Spectrum {
             89.5991, 88.6489, 87.6987, 85.4936, 83.2886, 83.4939, 83.6992, 81.863, 80.0268, 80.1207, 80.2146, 81.2462,
             82.2778, 80.281, 78.2842, 74.0027, 69.7213, 70.6652, 71.6091, 72.979, 74.349, 67.9765, 61.604};
         const T r = (lambda - minWavelength) * T(0.2);
-        const int t = int(std::ceil(r) + T(.5));
-        const int b = int(std::floor(r) + T(.5));
+        // Indices clamped to the table (the reference indexes past it for wavelengths above 720 nm, where
+        // the spectra it multiplies are zero).
+        const int t = std::min(68, std::max(0, int(std::ceil(r) + T(.5))));
+        const int b = std::min(68, std::max(0, int(std::floor(r) + T(.5))));
         const T w = r - T(b);
         return D65[t] * w + D65[b] * (T(1) - w);
     }
@@ -585,6 +587,15 @@ template <class T, int N> struct PLTBeam {  // PLT.slang:83-245
         posW = p;
     }
     V3<T> toLocal(V3<T> v) const { return {dot(v, t), dot(v, cross(t, r)), dot(v, r)}; }
+    // mutualCoherence(hwss k, v) (PLT.slang:165-170)
+    void mutualCoherence(const T* k, V3<T> v, T* out) const {
+        const M2<T> invT = invTheta();
+        const V2<T> vv = {v.x, v.y};
+        const T q = dot(vv, mul(invT, vv));
+        for (int j = 0; j < N; ++j) out[j] = std::exp(T(-.5) * (sqr(k[j] / K<T>::_2PI) * q));
+    }
+    void scaleAll(T s) { for (int j = 0; j < N; ++j) { L[j] *= s; L1[j] *= s; L2[j] *= s; L3[j] *= s; } }
+    void scaleEach(const T* s) { for (int j = 0; j < N; ++j) { L[j] *= s[j]; L1[j] *= s[j]; L2[j] *= s[j]; L3[j] *= s[j]; } }
 };
 
 // Shading frame (Scene/ShadingData.slang:86-98)

@@ -81,6 +81,15 @@ SIGNATURES = [
     ("plt_scene_destroy", C.c_int, [C.c_void_p]),
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
     ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    ("plt_pass_create", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("plt_pass_destroy", C.c_int, [C.c_void_p]),
+    ("plt_pass_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("plt_pass_get_option", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("plt_pass_option_name", C.c_char_p, [C.c_uint32]),
+    ("plt_pass_set_scene", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("plt_pass_set_frame", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("plt_pass_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("plt_pass_copy_bounce_records", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
 ]
 
 _lib = None
@@ -100,6 +109,16 @@ def load() -> C.CDLL:
         fn.restype = res
         fn.argtypes = args
     _lib = lib
+    return lib
+
+
+def reload_signatures() -> C.CDLL:
+    """Applies signatures registered after the first load (modules that extend SIGNATURES)."""
+    lib = load()
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
     return lib
 
 

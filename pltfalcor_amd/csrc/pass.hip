@@ -1,0 +1,349 @@
+// The PLTPT render pass over the C ABI (Source/RenderPasses/PLTPT/PLTPT.{h,cpp}): option
+// dictionary with the reference's keys and defaults (PLTPT.h:74-108, PLTPT.cpp:102-204),
+// scene binding with emissive-light tables (LightCollection + EmissivePowerSampler), and execute =
+// sample pass + solve pass over the whole frame.
+//
+// The reference splits the frame into 512^2 tiles to bound its bounce buffer; here one dispatch
+// covers the frame (the 36 B x (maxBounces+1) records of a 1080p frame are < 1 GB of HBM), so
+// `tileSize` is accepted and reported but does not change the result or the launch shape.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+#include "pathtracer.hpp"
+#include "plt_internal.hpp"
+#include "scene_device.hpp"
+#include "scene_host.hpp"
+
+using plt::set_error;
+
+namespace plt {
+namespace {
+
+struct OptionSpec {
+    const char* key;
+    double def;
+};
+// Keys and defaults of PLTPT (PLTPT.h:74-108). sampleGenerator 0 = uniform (xoshiro128**),
+// emissiveSampler 1 = power (EmissiveLightSamplerType::Power).
+const OptionSpec kOptions[] = {
+    {"maxBounces", 32}, {"tileSize", 512}, {"HWSS", 4}, {"HWSSDoMIS", 1}, {"sampleGenerator", 0},
+    {"debugView", 0}, {"debugViewIntensity", 1}, {"sourcingAreaFromEmissiveGeometry", 1},
+    {"sourcingMaxBeamOmega", .025}, {"useDirectLights", 1}, {"useEnvLights", 1}, {"useEmissiveLights", 1},
+    {"useAnalyticLights", 1}, {"alphaMasking", 1}, {"doNEE", 1}, {"doMIS", 1}, {"emissiveSampler", 1},
+    {"doRussianRoulette", 1}, {"NEEUsePerTileSG", 0}, {"doImportanceSampleEmitters", 1}, {"doMNEE", 1},
+    {"MNEEMaxOccluders", 2}, {"MNEEMaxIterations", 60}, {"MNEESolverThreshold", 5e-5},
+};
+
+struct LightTables {
+    std::vector<float4> lights;
+    std::vector<uint2> alias;
+    std::vector<float> pdf;
+    std::vector<uint32_t> tri_light;
+};
+
+// Emissive triangles in mesh order (LightCollection), flux = pi * area (the single emissive
+// profile's luminance cancels in the selection probabilities), alias table as
+// EmissivePowerSampler::generateAliasTable (EmissivePowerSampler.cpp:86-176) with its
+// default-seeded std::mt19937 shuffle.
+LightTables build_lights(const HostScene& s) {
+    LightTables t;
+    const uint32_t nt = uint32_t(s.indices.size() / 3);
+    t.tri_light.assign(nt, 0xFFFFFFFFu);
+    std::vector<float> flux;
+    for (uint32_t tri = 0; tri < nt; ++tri) {
+        const uint32_t mat = s.mesh_material[s.tri_mesh[tri]];
+        const plt_material& m = s.materials[mat];
+        if (m.type != PLT_MAT_DIFFUSE || m.emission_profile == PLT_NO_PROFILE) continue;
+        const float* p0 = &s.positions[3 * size_t(s.indices[3 * size_t(tri)])];
+        const float* p1 = &s.positions[3 * size_t(s.indices[3 * size_t(tri) + 1])];
+        const float* p2 = &s.positions[3 * size_t(s.indices[3 * size_t(tri) + 2])];
+        const float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        const float c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        const float area = 0.5f * len;
+        t.tri_light[tri] = uint32_t(t.lights.size() / 4);
+        float mbits;
+        std::memcpy(&mbits, &mat, 4);
+        t.lights.push_back(make_float4(p0[0], p0[1], p0[2], area));
+        t.lights.push_back(make_float4(p1[0], p1[1], p1[2], mbits));
+        t.lights.push_back(make_float4(p2[0], p2[1], p2[2], 0.f));
+        t.lights.push_back(make_float4(c[0] / len, c[1] / len, c[2] / len, 0.f));
+        flux.push_back(area * float(M_PI));
+    }
+    const uint32_t N = uint32_t(flux.size());
+    if (N == 0) return t;
+    double sum = 0.0;
+    for (float f : flux) sum += f;
+    const float invSum = 1.f / float(sum);
+    for (float f : flux) t.pdf.push_back(f * invSum);
+    std::vector<float> w = flux;
+    for (float& f : w) f *= float(N) / float(sum);
+    std::vector<uint32_t> perm(N), redirect(N);
+    std::vector<float> thr(N);
+    for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+    std::sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return w[a] < w[b]; });
+    uint32_t head = 0, tail = N - 1;
+    if (N == 1) { thr[0] = 1.f; redirect[0] = 0; }
+    while (head != tail) {
+        const uint32_t i = perm[head], j = perm[tail];
+        thr[i] = w[i];
+        redirect[i] = j;
+        w[j] -= 1.0f - w[i];
+        if (head == tail - 1) { thr[j] = 1.0f; redirect[j] = j; break; }
+        else if (w[j] < 1.0f) { std::swap(perm[head], perm[tail]); tail--; }
+        else head++;
+    }
+    for (uint32_t i = 0; i < N; ++i) perm[i] = i;
+    std::mt19937 rng;
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t dst = i + (uint32_t(rng()) % (N - i));
+        std::swap(thr[i], thr[dst]);
+        std::swap(redirect[i], redirect[dst]);
+        std::swap(perm[i], perm[dst]);
+    }
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t prob = uint32_t(f32_to_h(thr[i])) << 16u;
+        const uint32_t lx = redirect[i] & 0xFFFFFFu, ly = perm[i] & 0xFFFFFFu;
+        t.alias.push_back(make_uint2(prob | ((lx >> 8u) & 0xFFFFu), ((lx & 0xFFu) << 24u) | ly));
+    }
+    return t;
+}
+
+template <typename T>
+int upload_vec(const std::vector<T>& v, T** d, std::vector<void*>& owned) {
+    *d = nullptr;
+    if (v.empty()) return PLT_OK;
+    void* p = nullptr;
+    if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return set_error(PLT_ENOMEM, "pass: hipMalloc failed");
+    owned.push_back(p);
+    if (hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return set_error(PLT_EHIP, "pass: hipMemcpy failed");
+    *d = static_cast<T*>(p);
+    return PLT_OK;
+}
+}  // namespace
+}  // namespace plt
+
+struct plt_pass {
+    plt_context* ctx = nullptr;
+    std::map<std::string, double> opt;
+    plt_scene* scene = nullptr;
+    uint32_t frame = 0;
+    // device state
+    std::vector<void*> owned;  // scene-dependent buffers
+    plt_material* d_materials = nullptr;
+    float4* d_lights = nullptr;
+    uint2* d_alias = nullptr;
+    float* d_light_pdf = nullptr;
+    uint32_t* d_tri_light = nullptr;
+    uint32_t n_lights = 0;
+    uint32_t* d_records = nullptr;
+    size_t records_words = 0;
+    size_t last_words = 0;  // record words written by the last execute
+    float cam[12] = {};
+};
+
+namespace plt {
+namespace {
+void free_scene_state(plt_pass* p) {
+    for (void* q : p->owned) (void)hipFree(q);
+    p->owned.clear();
+    p->d_materials = nullptr;
+    p->d_lights = nullptr;
+    p->d_alias = nullptr;
+    p->d_light_pdf = nullptr;
+    p->d_tri_light = nullptr;
+    p->n_lights = 0;
+}
+// Camera frame (Camera.cpp:187-193).
+void camera_frame(const plt_camera& c, float out[12]) {
+    auto norm = [](float v[3]) { const float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); v[0] /= l; v[1] /= l; v[2] /= l; };
+    float W[3] = {c.target[0] - c.pos[0], c.target[1] - c.pos[1], c.target[2] - c.pos[2]};
+    norm(W);
+    for (float& x : W) x *= c.focal_distance;
+    float U[3] = {W[1] * c.up[2] - W[2] * c.up[1], W[2] * c.up[0] - W[0] * c.up[2], W[0] * c.up[1] - W[1] * c.up[0]};
+    norm(U);
+    float V[3] = {U[1] * W[2] - U[2] * W[1], U[2] * W[0] - U[0] * W[2], U[0] * W[1] - U[1] * W[0]};
+    norm(V);
+    const float tn = std::tan(c.fov_y * 0.5f);
+    const float ulen = c.focal_distance * tn * c.aspect, vlen = c.focal_distance * tn;
+    for (int a = 0; a < 3; ++a) {
+        out[a] = c.pos[a];
+        out[3 + a] = U[a] * ulen;
+        out[6 + a] = V[a] * vlen;
+        out[9 + a] = W[a];
+    }
+}
+}  // namespace
+}  // namespace plt
+
+extern "C" {
+
+int plt_pass_create(plt_context* ctx, const char* type, plt_pass** out) {
+    if (!out) return set_error(PLT_EINVAL, "plt_pass_create: bad arguments");
+    if (!type || std::strcmp(type, "PLTPT") != 0) return set_error(PLT_EINVAL, "plt_pass_create: unknown pass type (only \"PLTPT\")");
+    plt_pass* p = new (std::nothrow) plt_pass();
+    if (!p) return set_error(PLT_ENOMEM, "plt_pass_create: out of host memory");
+    p->ctx = ctx;
+    for (const auto& o : plt::kOptions) p->opt[o.key] = o.def;
+    *out = p;
+    return PLT_OK;
+}
+
+int plt_pass_destroy(plt_pass* p) {
+    if (!p) return PLT_OK;
+    plt::free_scene_state(p);
+    if (p->d_records) (void)hipFree(p->d_records);
+    delete p;
+    return PLT_OK;
+}
+
+int plt_pass_set_option(plt_pass* p, const char* key, double value) {
+    if (!p || !key) return set_error(PLT_EINVAL, "plt_pass_set_option: bad arguments");
+    auto it = p->opt.find(key);
+    if (it == p->opt.end()) return set_error(PLT_WARN_UNKNOWN_KEY, std::string("Unknown field '") + key + "' in PLTPathTracer dictionary.");
+    if (!std::isfinite(value)) return set_error(PLT_EINVAL, std::string("plt_pass_set_option: non-finite value for ") + key);
+    const std::string k = key;
+    if (k == "HWSS") value = std::max(1.0, std::min(4.0, std::floor(value)));  // PLTPT.cpp:291-292
+    it->second = value;
+    return PLT_OK;
+}
+
+int plt_pass_get_option(const plt_pass* p, const char* key, double* value) {
+    if (!p || !key || !value) return set_error(PLT_EINVAL, "plt_pass_get_option: bad arguments");
+    auto it = p->opt.find(key);
+    if (it == p->opt.end()) return set_error(PLT_WARN_UNKNOWN_KEY, std::string("Unknown field '") + key + "'");
+    *value = it->second;
+    return PLT_OK;
+}
+
+const char* plt_pass_option_name(uint32_t i) {
+    return i < sizeof(plt::kOptions) / sizeof(plt::kOptions[0]) ? plt::kOptions[i].key : nullptr;
+}
+
+int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
+    if (!p) return set_error(PLT_EINVAL, "plt_pass_set_scene: bad arguments");
+    plt::free_scene_state(p);
+    p->scene = scene;
+    p->frame = 0;  // PLTPT.cpp:510
+    if (!scene) return PLT_OK;
+    if (!scene->ctx || !p->ctx) return set_error(PLT_ESTATE, "plt_pass_set_scene: host-only scene or pass");
+    const plt::LightTables lt = plt::build_lights(scene->host);
+    int rc;
+    if ((rc = plt::upload_vec(scene->host.materials, &p->d_materials, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(lt.lights, &p->d_lights, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(lt.alias, &p->d_alias, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(lt.pdf, &p->d_light_pdf, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(lt.tri_light, &p->d_tri_light, p->owned)) < 0) return rc;
+    p->n_lights = uint32_t(lt.pdf.size());
+    plt::camera_frame(scene->host.camera, p->cam);
+    return PLT_OK;
+}
+
+int plt_pass_set_frame(plt_pass* p, uint32_t frame) {
+    if (!p) return set_error(PLT_EINVAL, "plt_pass_set_frame: bad arguments");
+    p->frame = frame;
+    return PLT_OK;
+}
+
+int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) {
+    if (!p || !rd) return set_error(PLT_EINVAL, "plt_pass_execute: bad arguments");
+    if (!p->ctx) return set_error(PLT_ESTATE, "plt_pass_execute: pass created without a device context");
+    if (!p->scene) return set_error(PLT_ESTATE, "plt_pass_execute: no scene (the reference clears its outputs, PLTPT.cpp:375-384)");
+    if (!rd->color || !rd->albedo || !rd->normal || rd->width == 0 || rd->height == 0)
+        return set_error(PLT_EINVAL, "plt_pass_execute: missing output buffers or empty frame");
+    auto o = [&](const char* k) { return p->opt[k]; };
+    if (o("doMNEE") != 0) return set_error(PLT_ENOTIMPL, "doMNEE: manifold next-event estimation is not built yet; set doMNEE=0");
+    if (o("NEEUsePerTileSG") != 0) return set_error(PLT_ENOTIMPL, "NEEUsePerTileSG=1 is not built");
+    if (o("emissiveSampler") != 1) return set_error(PLT_ENOTIMPL, "only the power emissive sampler (1) is built");
+    if (o("sampleGenerator") != 0) return set_error(PLT_ENOTIMPL, "only the uniform sample generator (0) is built");
+    if (o("debugView") != 0) return set_error(PLT_ENOTIMPL, "debug views are not built");
+    const uint32_t hwss = uint32_t(o("HWSS"));
+    if (hwss != 1 && hwss != 4) return set_error(PLT_ENOTIMPL, "HWSS must be 1 or 4");
+    const uint32_t maxb = uint32_t(o("maxBounces"));
+    if (maxb > 64) return set_error(PLT_EINVAL, "maxBounces > 64");
+    const size_t npaths = size_t(rd->width) * rd->height;
+    if (npaths >= (size_t(1) << 31)) return set_error(PLT_EINVAL, "frame too large");
+    const size_t words = size_t(maxb + 1) * plt::kRecWords * npaths;
+    if (words > p->records_words) {
+        if (p->d_records) (void)hipFree(p->d_records);
+        p->d_records = nullptr;
+        p->records_words = 0;
+        void* q = nullptr;
+        if (hipMalloc(&q, words * 4) != hipSuccess) return set_error(PLT_ENOMEM, "plt_pass_execute: bounce records allocation failed");
+        p->d_records = static_cast<uint32_t*>(q);
+        p->records_words = words;
+    }
+    const plt_scene* s = p->scene;
+    plt::PtScene sc{};
+    sc.tsc = plt::TraceScene{s->dev.nodes, s->dev.tris};
+    sc.positions = s->dev.positions;
+    sc.normals = s->dev.normals;
+    sc.indices = s->dev.indices;
+    sc.tri_mesh = s->dev.tri_mesh;
+    sc.mesh_material = s->dev.mesh_material;
+    sc.materials = p->d_materials;
+    sc.profiles = s->dev.profiles;
+    sc.n_profiles = uint32_t(s->host.profiles.size());
+    for (int a = 0; a < 3; ++a) {
+        sc.cam_pos[a] = p->cam[a];
+        sc.cam_u[a] = p->cam[3 + a];
+        sc.cam_v[a] = p->cam[6 + a];
+        sc.cam_w[a] = p->cam[9 + a];
+    }
+    sc.jitter[0] = s->host.camera.jitter[0];
+    sc.jitter[1] = s->host.camera.jitter[1];
+    sc.lights = p->d_lights;
+    sc.alias = p->d_alias;
+    sc.light_pdf = p->d_light_pdf;
+    sc.tri_light = p->d_tri_light;
+    sc.n_lights = p->n_lights;
+
+    plt::PtFrame fr{};
+    fr.width = rd->width;
+    fr.height = rd->height;
+    fr.frame = p->frame;
+    fr.max_bounces = maxb;
+    fr.hwss = hwss;
+    uint32_t opts = 0;
+    if (o("doNEE") != 0) opts |= plt::kOptNEE;
+    if (o("doMIS") != 0) opts |= plt::kOptMIS;
+    if (o("doRussianRoulette") != 0) opts |= plt::kOptRR;
+    if (o("HWSSDoMIS") != 0) opts |= plt::kOptHwssMis;
+    if (o("doImportanceSampleEmitters") != 0) opts |= plt::kOptISEmitters;
+    if (o("useDirectLights") != 0) opts |= plt::kOptDirectLights;
+    if (o("useEmissiveLights") != 0) opts |= plt::kOptEmissiveLights;
+    fr.options = opts;
+    fr.path_base = 0;
+    fr.n_paths = uint32_t(npaths);
+    fr.sourcing_area = float(o("sourcingAreaFromEmissiveGeometry"));
+    fr.max_omega = float(o("sourcingMaxBeamOmega"));
+    fr.records = p->d_records;
+    fr.color = reinterpret_cast<float4*>(rd->color);
+    fr.albedo = reinterpret_cast<float4*>(rd->albedo);
+    fr.normal = reinterpret_cast<float4*>(rd->normal);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = plt::launch_sample_pass(sc, fr, st);
+    if (rc < 0) return rc;
+    rc = plt::launch_solve_pass(sc, fr, st);
+    if (rc < 0) return rc;
+    ++p->frame;  // PLTPT.cpp:498
+    p->last_words = words;
+    return PLT_OK;
+}
+
+int plt_pass_copy_bounce_records(plt_pass* p, void* dst, size_t bytes) {
+    if (!p || !dst) return set_error(PLT_EINVAL, "plt_pass_copy_bounce_records: bad arguments");
+    if (!p->d_records || p->last_words == 0) return set_error(PLT_ESTATE, "plt_pass_copy_bounce_records: nothing rendered yet");
+    if (bytes < p->last_words * 4) return set_error(PLT_EINVAL, "plt_pass_copy_bounce_records: destination too small");
+    if (hipMemcpy(dst, p->d_records, p->last_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_copy_bounce_records: hipMemcpy failed");
+    return PLT_OK;
+}
+
+}  // extern "C"

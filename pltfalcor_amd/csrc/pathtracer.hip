@@ -1,0 +1,537 @@
+// Two-pass PLT path tracer on MI355X (Source/RenderPasses/PLTPT/pltpt_sample.rt.slang and
+// pltpt_solve.rt.slang): one lane per path.
+//
+// Sample pass: camera ray, then per bounce a closest-hit traversal (LDS stack), shading frame,
+// Russian roulette, BSDF sampling in the hero wavelength, and a 36 B bounce record written as
+// SoA planes. Solve pass: replays the records; per vertex it samples an emissive triangle
+// (power alias table, spectral importance sampling), casts an any-hit shadow ray, sources a PLT
+// beam and carries it forward through the stored vertices (forwardUnwind: eval + spectral MIS
+// per vertex), measures it at the camera, and does the same for emissive hits.
+#include "pathtracer.hpp"
+#include "plt_internal.hpp"
+#include "plt_materials.hpp"
+#include "scene_device.hpp"
+
+namespace plt {
+namespace {
+
+constexpr int kPtBlock = 64;
+constexpr float kFltMax = 3.402823466e+38f;
+constexpr float kFltMin = 1.175494351e-38f;
+
+// ---------------------------------------------------------------------------------------
+// Helpers: octahedral equal-area direction coding (MathHelpers.slang:199-240), Wachter-Binder
+// ray offset (GeometryHelpers.slang:47-61).
+// ---------------------------------------------------------------------------------------
+PLT_DEV void oct_encode(V3 n, float& u, float& v) {
+    const float r = sqrtf(1.f - fabsf(n.z));
+    const float phi = atan2f(fabsf(n.y), fabsf(n.x));
+    float py = r * phi * 0.636619772367581343076f;
+    float px = r - py;
+    if (n.z < 0.f) { const float t = px; px = 1.f - py; py = 1.f - t; }
+    u = px * sgnf(n.x);
+    v = py * sgnf(n.y);
+}
+PLT_DEV V3 oct_decode(float px, float py) {
+    const float d = 1.f - (fabsf(px) + fabsf(py));
+    const float r = 1.f - fabsf(d);
+    const float phi = r > 0.f ? ((fabsf(py) - fabsf(px)) / r + 1.f) * kPi_4 : 0.f;
+    const float f = r * sqrtf(2.f - r * r);
+    return v3(f * sgnf(px) * cosf(phi), f * sgnf(py) * sinf(phi), sgnf(d) * (1.f - r * r));
+}
+PLT_DEV float offset_component(float p, float n) {
+    const int io = int(n * (3.f * 256.f));
+    const float ip = __int_as_float(__float_as_int(p) + (p < 0.f ? -io : io));
+    return fabsf(p) < (1.f / 16.f) ? p + n * (3.f / 65536.f) : ip;
+}
+PLT_DEV V3 ray_origin(V3 p, V3 n) { return v3(offset_component(p.x, n.x), offset_component(p.y, n.y), offset_component(p.z, n.z)); }
+
+// ---------------------------------------------------------------------------------------
+// Hit -> shading data (Scene.slang:530-567, MaterialFactory.slang:79-130): barycentric
+// interpolation, invented tangent frame (perp_stark; the workload meshes carry no tangents).
+// ---------------------------------------------------------------------------------------
+struct Surf {
+    V3 pos, faceN;
+    Frame f;
+    uint32_t material;
+    bool front;
+};
+PLT_DEV Surf surface_at(const PtScene& sc, uint32_t tri, float b1, float b2, V3 viewDir) {
+    const uint32_t i0 = sc.indices[3 * tri], i1 = sc.indices[3 * tri + 1], i2 = sc.indices[3 * tri + 2];
+    const float w0 = 1.f - b1 - b2;
+    const V3 p0 = v3(sc.positions[3 * i0], sc.positions[3 * i0 + 1], sc.positions[3 * i0 + 2]);
+    const V3 p1 = v3(sc.positions[3 * i1], sc.positions[3 * i1 + 1], sc.positions[3 * i1 + 2]);
+    const V3 p2 = v3(sc.positions[3 * i2], sc.positions[3 * i2 + 1], sc.positions[3 * i2 + 2]);
+    const V3 n0 = v3(sc.normals[3 * i0], sc.normals[3 * i0 + 1], sc.normals[3 * i0 + 2]);
+    const V3 n1 = v3(sc.normals[3 * i1], sc.normals[3 * i1 + 1], sc.normals[3 * i1 + 2]);
+    const V3 n2 = v3(sc.normals[3 * i2], sc.normals[3 * i2 + 1], sc.normals[3 * i2 + 2]);
+    Surf s;
+    s.pos = p0 * w0 + p1 * b1 + p2 * b2;
+    const V3 N = normalize(n0 * w0 + n1 * b1 + n2 * b2);
+    s.faceN = normalize(cross(p1 - p0, p2 - p0));
+    s.f.N = N;
+    s.f.T = perp_stark(N);
+    s.f.B = cross(N, s.f.T);
+    s.front = dot(viewDir, s.faceN) >= 0.f;
+    s.material = sc.mesh_material[sc.tri_mesh[tri]];
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Spectra: CIE fit (Wyman 2013), D65, Rec.709 (SpectrumUtils.slang:43-104).
+// ---------------------------------------------------------------------------------------
+__constant__ const float kD65[69] = {49.9755f, 52.3118f, 54.6482f, 68.7015f, 82.7549f, 87.1204f, 91.486f, 92.4589f, 93.4318f,
+    90.057f, 86.6823f, 95.7736f, 104.865f, 110.936f, 117.008f, 117.41f, 117.812f, 116.336f, 114.861f, 115.392f, 115.923f,
+    112.367f, 108.811f, 109.082f, 109.354f, 108.578f, 107.802f, 106.296f, 104.79f, 106.239f, 107.689f, 106.047f, 104.405f,
+    104.225f, 104.046f, 102.023f, 100.f, 98.1671f, 96.3342f, 96.0611f, 95.788f, 92.2368f, 88.6856f, 89.3459f, 90.0062f,
+    89.8026f, 89.5991f, 88.6489f, 87.6987f, 85.4936f, 83.2886f, 83.4939f, 83.6992f, 81.863f, 80.0268f, 80.1207f, 80.2146f,
+    81.2462f, 82.2778f, 80.281f, 78.2842f, 74.0027f, 69.7213f, 70.6652f, 71.6091f, 72.979f, 74.349f, 67.9765f, 61.604f};
+PLT_DEV float gl(float x, float l, float a, float b) { const float t = (x - l) * (x < l ? a : b); return expf(-0.5f * t * t); }
+PLT_DEV void spectrum_to_rgb_add(float val, float lam, float& r, float& g, float& b) {
+    const float X = 0.362f * gl(lam, 442.0f, 0.0624f, 0.0374f) + 1.056f * gl(lam, 599.8f, 0.0264f, 0.0323f) -
+                    0.065f * gl(lam, 501.1f, 0.0490f, 0.0382f);
+    const float Y = 0.821f * gl(lam, 568.8f, 0.0213f, 0.0247f) + 0.286f * gl(lam, 530.9f, 0.0613f, 0.0322f);
+    const float Z = 1.217f * gl(lam, 437.0f, 0.0845f, 0.0278f) + 0.681f * gl(lam, 459.0f, 0.0385f, 0.0725f);
+    const float rr = (lam - kMinWavelength) * 0.2f;
+    // The table covers 380-720 nm; importance-sampled emitter wavelengths can lie beyond it (where the
+    // white basis, and so every spectrum here, is exactly zero), so the indices are clamped to the table.
+    const int t = min(68, max(0, int(ceilf(rr) + .5f))), bb = min(68, max(0, int(floorf(rr) + .5f)));
+    const float w = rr - float(bb);
+    const float d65 = kD65[t] * w + kD65[bb] * (1.f - w);
+    const float s = d65 * val;
+    const float x = X * s, y = Y * s, z = Z * s;
+    r += 3.240969941904522f * x + -1.537383177570094f * y + -0.4986107602930032f * z;
+    g += -0.9692436362808803f * x + 1.875967501507721f * y + 0.04155505740717569f * z;
+    b += 0.05563007969699373f * x + -0.2039769588889765f * y + 1.056971514242878f * z;
+}
+
+// Scene::sampleSpectrum (Scene.slang:296-327).
+template <int N>
+PLT_DEV void sample_spectrum(const plt_spectral_profile* sp, float hero, Rng& sg, bool importance, float* Le, float* lam) {
+    if (!sp) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) { Le[j] = 0.f; lam[j] = hero; }
+        return;
+    }
+    if (importance) {
+        lam[0] = hero;
+        Le[0] = eval_spectrum(sp, hero);
+        const float bins = float(sp->bins);
+        const float x0 = bins * sg.next1D(), x1 = bins * sg.next1D(), x2 = bins * sg.next1D();
+        const float xs[3] = {x0, x1, x2};
+#pragma unroll
+        for (int j = 1; j < N; ++j) {
+            const float xv = xs[j - 1];
+            const uint32_t idx = uint32_t(xv);
+            const float a = idx < sp->bins ? sp->icdf[idx] : 1.f;
+            const float b = idx + 1 < sp->bins ? sp->icdf[idx + 1] : 1.f;
+            const float fr = xv - floorf(xv);
+            const float cdf = a + fr * (b - a);
+            lam[j] = sp->min_wavelength + cdf * (sp->max_wavelength - sp->min_wavelength);
+            Le[j] = sp->intensity / bins;
+        }
+    } else {
+        const float span = kMaxWavelength - kMinWavelength;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            float h = hero + float(j) * (span / float(N));
+            h -= h > kMaxWavelength ? span : 0.f;
+            lam[j] = h;
+            Le[j] = eval_spectrum(sp, h);
+        }
+    }
+}
+
+// pltSourceBeamArea (PLT.slang:275-290).
+template <int N>
+PLT_DEV Beam<N> source_beam_area(V3 pos, V3 dir, float A, float dist, const float* Le, const float* lam, float maxOmega, int comps) {
+    Beam<N> b;
+    const float r2 = (dist * 1e+3f) * (dist * 1e+3f);
+    A = fminf(A, maxOmega * r2);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        b.S[j][0] = Le[j]; b.S[j][1] = b.S[j][2] = b.S[j][3] = 0.f;
+        b.lambda[j] = lam[j];
+        b.spdf[j] = 1.f;
+    }
+    b.comps = comps;
+    b.D00 = A; b.D01 = 0.f; b.D11 = A;
+    b.r_m = 0.f;
+    b.pos = pos;
+    b.r = dir;
+    b.t = perp_stark(dir);
+    b.distant = false;
+    return b;
+}
+
+struct Rec {
+    uint32_t tri;
+    float b1, b2;
+    V3 wi;
+    uint32_t lobe, flags;
+    float contrib, lastPdf;
+};
+PLT_DEV Rec read_rec(const uint32_t* rec, size_t n, size_t i, int p) {
+    const uint32_t* b = rec + size_t(p) * kRecWords * n + i;
+    Rec r;
+    r.tri = b[kRecTri * n];
+    r.b1 = __uint_as_float(b[kRecB1 * n]);
+    r.b2 = __uint_as_float(b[kRecB2 * n]);
+    r.wi = oct_decode(__uint_as_float(b[kRecWiX * n]), __uint_as_float(b[kRecWiY * n]));
+    r.lobe = b[kRecLobe * n];
+    r.flags = b[kRecFlags * n];
+    r.contrib = __uint_as_float(b[kRecContrib * n]);
+    r.lastPdf = __uint_as_float(b[kRecLastPdf * n]);
+    return r;
+}
+PLT_DEV void write_rec(uint32_t* rec, size_t n, size_t i, int p, uint32_t tri, float b1, float b2, V3 wi, uint32_t lobe,
+                       uint32_t flags, float contrib, float lastPdf) {
+    uint32_t* b = rec + size_t(p) * kRecWords * n + i;
+    float ex, ey;
+    oct_encode(wi, ex, ey);
+    b[kRecTri * n] = tri;
+    b[kRecB1 * n] = __float_as_uint(b1);
+    b[kRecB2 * n] = __float_as_uint(b2);
+    b[kRecWiX * n] = __float_as_uint(ex);
+    b[kRecWiY * n] = __float_as_uint(ey);
+    b[kRecLobe * n] = lobe;
+    b[kRecFlags * n] = flags;
+    b[kRecContrib * n] = __float_as_uint(contrib);
+    b[kRecLastPdf * n] = __float_as_uint(lastPdf);
+}
+
+PLT_DEV Mat material_at(const PtScene& sc, uint32_t m) { return setup_material(sc.materials[m], sc.profiles, sc.n_profiles); }
+
+// Camera.computeRayPinhole (Camera.slang:46-78). Uncontracted, correctly rounded arithmetic so
+// primary rays (and hence primary hits) are bit-identical to a strict-IEEE host computation.
+PLT_DEV V3 camera_dir(const PtScene& sc, const PtFrame& fr, uint32_t px, uint32_t py) {
+#pragma clang fp contract(off)
+    const float sx = (float(px) + 0.5f) / float(fr.width) + -sc.jitter[0];
+    const float sy = (float(py) + 0.5f) / float(fr.height) + sc.jitter[1];
+    const float nx = 2.f * sx + -1.f, ny = -2.f * sy + 1.f;
+    const float x = nx * sc.cam_u[0] + ny * sc.cam_v[0] + sc.cam_w[0];
+    const float y = nx * sc.cam_u[1] + ny * sc.cam_v[1] + sc.cam_w[1];
+    const float z = nx * sc.cam_u[2] + ny * sc.cam_v[2] + sc.cam_w[2];
+    const float l = sqrtf(x * x + y * y + z * z);
+    return v3(x / l, y / l, z / l);
+}
+
+// ---------------------------------------------------------------------------------------
+// Sample pass (pltpt_sample.rt.slang:44-185).
+// ---------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(kPtBlock) sample_kernel(PtScene sc, PtFrame fr) {
+    __shared__ int stack[kTraversalStackDepth * kPtBlock];
+    const uint32_t li = blockIdx.x * kPtBlock + threadIdx.x;
+    if (li >= fr.n_paths) return;
+    const uint32_t pid = fr.path_base + li;
+    const uint32_t px = pid % fr.width, py = pid / fr.width;
+    const size_t n = fr.n_paths;
+    int* stk = stack + threadIdx.x;
+
+    Rng sg = Rng::make(px, py, fr.frame);
+    const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
+    fr.albedo[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fr.normal[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    V3 dir = camera_dir(sc, fr, px, py);
+    V3 org = v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]);
+    float weight = 1.f, pdf = 1.f, thp = 1.f, lastNonDeltaPdf = 0.f;
+    uint32_t flags = kFlagOnlyDelta;
+    const bool rr = (fr.options & kOptRR) != 0;
+
+    for (int len = 0; len < int(fr.max_bounces) + 1;) {
+        Ray ray{org.x, org.y, org.z, dir.x, dir.y, dir.z, 0.f, kFltMax};
+        const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, ray, stk, kPtBlock);
+        const V3 wi = -dir;
+        if (h.tri == kNoHit) {  // scatterMiss
+            flags |= kFlagMiss;
+            write_rec(fr.records, n, li, len, kNoHit, 0.f, 0.f, wi, 0u, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
+            break;
+        }
+        const Surf s = surface_at(sc, h.tri, h.b1, h.b2, wi);
+        const Mat m = material_at(sc, s.material);
+        const uint32_t props = mat_props(m);
+        // Russian roulette: RR(1 - weight, 2, u) (pltpt.slangh:170-176)
+        float rcp_thp = 1.f;
+        if (rr) {
+            const float u = sg.next1D();
+            const float pr = powf(1.f - weight, 2.f);
+            rcp_thp = u < pr ? 0.f : 1.f - pr;
+        }
+        bool end = len == int(fr.max_bounces) || (rcp_thp == 0.f && len > 0);
+        BsdfSample bs{v3(0.f, 0.f, 0.f), 0.f, 0.f, 0u};
+        if (!end) {
+            const bool valid = mat_sample(m, s.f, wi, sg, lambda, bs);
+            end = !valid || bs.pdf == 0.f;
+            end |= pdf * bs.pdf * thp * rcp_thp < 1e-6f;
+        }
+        const bool isDelta = (bs.lobe & kLobeDelta) != 0;
+        const bool isDisp = (bs.lobe & kLobeDispersiveDelta) != 0;
+        const bool isRefl = (bs.lobe & kLobeReflection) != 0;
+        const bool isMS = (isRefl && (props & kPropMWReflective)) || (!isRefl && (props & kPropMWTransmissive));
+        if (!isDelta && (flags & kFlagOnlyDelta)) {
+            fr.albedo[pid] = make_float4(m.col[0], m.col[1], m.col[2], 1.f);
+            fr.normal[pid] = make_float4(s.f.N.x, s.f.N.y, s.f.N.z, 1.f);
+        }
+        if (end) {
+            flags |= kFlagTerminated;
+            write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, 0u, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
+            break;
+        }
+        weight *= bs.weight;
+        thp *= rcp_thp;
+        pdf *= bs.pdf;
+        if (isDisp) flags |= kFlagDispersive;
+        write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, bs.lobe, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
+        ++len;
+        lastNonDeltaPdf = !isDelta ? bs.pdf : lastNonDeltaPdf;
+        const uint32_t msc = isMS ? (flags >> 16) + 1u : 0u;
+        flags = (flags & 0xffffu) | (msc << 16);
+        flags = isDelta ? (flags | kFlagLastDelta) : (flags & ~kFlagLastDelta);
+        if (!isDelta) flags &= ~kFlagOnlyDelta;
+        // ShadingData.computeNewRayOrigin(viewside = isReflection) (ShadingData.slang:77-80)
+        org = ray_origin(s.pos, (s.front == isRefl) ? s.faceN : -s.faceN);
+        dir = bs.wo;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Solve pass (pltpt_solve.rt.slang:37-476).
+// ---------------------------------------------------------------------------------------
+struct Solve {
+    float r, g, b;
+    V3 origin, normal;
+};
+
+template <int N>
+PLT_DEV void measure(const PtScene& sc, const PtFrame& fr, Solve& sv, int bounce, float contrib, Beam<N>& beam) {
+    if ((fr.options & kOptDirectLights) == 0 && bounce <= 1) return;
+    beam.propagate(v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]));
+    if (fr.options & kOptHwssMis) {
+        float sum = 0.f;
+#pragma unroll
+        for (int j = 0; j < N; ++j) sum += beam.spdf[j];
+        contrib *= sum > kFltEps ? beam.spdf[0] / sum : 0.f;
+    } else {
+        contrib *= 1.f / float(beam.comps);
+    }
+    float r = 0.f, g = 0.f, b = 0.f;
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < beam.comps) spectrum_to_rgb_add(beam.S[j][0], beam.lambda[j], r, g, b);
+    const float inv = 1.f / 10.5670762f;
+    r = fmaxf(0.f, r * inv * contrib);
+    g = fmaxf(0.f, g * inv * contrib);
+    b = fmaxf(0.f, b * inv * contrib);
+    if (isfinite(r) && isfinite(g) && isfinite(b)) { sv.r += r; sv.g += g; sv.b += b; }
+}
+
+// One forward step through a stored vertex (forwardUnwindStep, pltpt_solve.rt.slang:48-66).
+template <int N>
+PLT_DEV void unwind_step(const PtScene& sc, const PtFrame& fr, uint32_t tri, float b1, float b2, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    const V3 V = -beam.r;
+    const Surf s = surface_at(sc, tri, b1, b2, V);
+    beam.propagate(s.pos);
+    const Mat m = material_at(sc, s.material);
+    mat_eval<N>(m, s.f, V, wo, lobe, beam);
+    if (fr.options & kOptHwssMis) {
+        float p[N];
+        mat_lambda_pdf<N>(m, s.f, V, wo, lobe, beam, p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) beam.spdf[j] *= p[j];
+    }
+}
+template <int N>
+PLT_DEV void forward_unwind(const PtScene& sc, const PtFrame& fr, size_t n, size_t i, int bounce, Beam<N>& beam) {
+    for (int p = bounce - 1; p >= 0; --p) {
+        const Rec r = read_rec(fr.records, n, i, p);
+        unwind_step<N>(sc, fr, r.tri, r.b1, r.b2, r.wi, r.lobe, beam);
+    }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr) {
+    __shared__ int stack[kTraversalStackDepth * kPtBlock];
+    const uint32_t li = blockIdx.x * kPtBlock + threadIdx.x;
+    if (li >= fr.n_paths) return;
+    const uint32_t pid = fr.path_base + li;
+    const uint32_t px = pid % fr.width, py = pid / fr.width;
+    const size_t n = fr.n_paths;
+    int* stk = stack + threadIdx.x;
+    const bool doNEE = (fr.options & kOptNEE) != 0, doMIS = doNEE && (fr.options & kOptMIS) != 0;
+    const bool useEmissive = (fr.options & kOptEmissiveLights) != 0 && sc.n_lights > 0;
+    const bool importance = (fr.options & kOptISEmitters) != 0;
+
+    Rng sg = Rng::make(px, py, fr.frame);
+    const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
+    Solve sv;
+    sv.r = sv.g = sv.b = 0.f;
+    sv.origin = v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]);
+    sv.normal = normalize(v3(sc.cam_w[0], sc.cam_w[1], sc.cam_w[2]));
+
+    for (int p = 0;; ++p) {
+        const Rec bounce = read_rec(fr.records, n, li, p);
+        if (bounce.flags & kFlagMiss) break;  // onMiss: no environment light in these scenes
+        const Surf s = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, bounce.wi);
+        const Mat m = material_at(sc, s.material);
+        const uint32_t props = mat_props(m);
+
+        // ---- NEE (pltpt_solve.rt.slang:245-366; manifold walks off) ----
+        if (doNEE) {
+            const uint32_t lobes = mat_lobes(m);
+            const bool sampleRefl = (lobes & kLobeNonDeltaReflection) != 0;
+            const bool sampleTrans = (lobes & kLobeNonDeltaTransmission) != 0;
+            if (sampleRefl || sampleTrans) {
+                // selectLightType: only emissive geometry is present -> one draw, p = 1
+                const float uType = sg.next1D();
+                bool valid = useEmissive && uType < 1.f;
+                float lsPdf = 0.f, lsDist = 0.f;
+                V3 lsDir = v3(0.f, 0.f, 0.f), lsOrigin = v3(0.f, 0.f, 0.f), lightPos = v3(0.f, 0.f, 0.f), tPos = v3(0.f, 0.f, 0.f);
+                float Le[N], lam[N];
+#pragma unroll
+                for (int j = 0; j < N; ++j) { Le[j] = 0.f; lam[j] = lambda; }
+                if (valid) {
+                    // EmissivePowerSampler.sampleLight (EmissivePowerSampler.slang:62-83)
+                    const float uL = sg.next1D();
+                    uint32_t ti = min(uint32_t(uL * float(sc.n_lights)), sc.n_lights - 1);
+                    const uint2 pk = sc.alias[ti];
+                    const float thr = float(__builtin_bit_cast(_Float16, uint16_t(pk.x >> 16)));
+                    const uint32_t above = ((pk.x & 0xFFFFu) << 8) | ((pk.y >> 24) & 0xFFu);
+                    const uint32_t below = pk.y & 0xFFFFFFu;
+                    ti = sg.next1D() >= thr ? above : below;
+                    const float selPdf = sc.light_pdf[ti];
+                    // sampleTriangle (EmissiveLightSamplerHelpers.slang:76-125)
+                    const float4 q0 = sc.lights[4 * ti], q1 = sc.lights[4 * ti + 1], q2 = sc.lights[4 * ti + 2], qn = sc.lights[4 * ti + 3];
+                    const float u0 = sg.next1D(), u1 = sg.next1D();
+                    const float su = sqrtf(u0);
+                    const float bb0 = 1.f - su, bb1 = u1 * su;
+                    const float w0 = 1.f - bb0 - bb1;
+                    tPos = v3(q0.x * w0 + q1.x * bb0 + q2.x * bb1, q0.y * w0 + q1.y * bb0 + q2.y * bb1, q0.z * w0 + q1.z * bb0 + q2.z * bb1);
+                    const V3 toL = tPos - s.pos;
+                    const float d2 = fmaxf(kFltMin, dot(toL, toL));
+                    const float dist = sqrtf(d2);
+                    const V3 ldir = toL * (1.f / dist);
+                    const V3 ln = v3(qn.x, qn.y, qn.z);
+                    const float cosT = dot(ln, -ldir);
+                    if (cosT <= 0.f) {
+                        valid = false;
+                    } else {
+                        const uint32_t lm = __float_as_uint(q1.w);
+                        const plt_material& em = sc.materials[lm];
+                        const plt_spectral_profile* ep = em.emission_profile < sc.n_profiles ? sc.profiles + em.emission_profile : nullptr;
+                        sample_spectrum<N>(ep, lambda, sg, importance, Le, lam);
+#pragma unroll
+                        for (int j = 0; j < N; ++j) Le[j] *= rgb_to_spectrum(1.f, 1.f, 1.f, lam[j]);
+                        const float tpdf = d2 / fmaxf(kFltMin, cosT * q0.w) * selPdf;
+                        lsPdf = tpdf;
+#pragma unroll
+                        for (int j = 0; j < N; ++j) Le[j] = tpdf > 0.f ? Le[j] / tpdf : 0.f;
+                        lightPos = ray_origin(tPos, ln);
+                        // computeNewRayOrigin(sd, dir) (ShadowRays.slangh:50-52)
+                        const V3 rd = lightPos - s.pos;
+                        lsOrigin = ray_origin(s.pos, dot(s.faceN, rd) >= 0.f ? s.faceN : -s.faceN);
+                        const V3 tl = lightPos - lsOrigin;
+                        lsDist = sqrtf(dot(tl, tl));
+                        lsDir = normalize(tl);
+                        bool anyL = false;
+#pragma unroll
+                        for (int j = 0; j < N; ++j) anyL |= Le[j] > 0.f;
+                        valid = anyL;
+                    }
+                }
+                // generateLightSample's acceptance (LightSampling.slangh:196-205)
+                const bool isRefl = dot(s.faceN, lsDir) * dot(s.faceN, bounce.wi) > 0.f;
+                valid = valid && lsPdf > 1e-6f && !(!sampleRefl && isRefl) && !(!sampleTrans && !isRefl);
+                bool anyLe = false;
+#pragma unroll
+                for (int j = 0; j < N; ++j) anyLe |= Le[j] > 0.f;
+                if (valid && anyLe) {
+                    const int comps = (!(fr.options & kOptHwssMis) && (bounce.flags & kFlagDispersive)) ? 1 : N;
+                    Ray vr{lsOrigin.x, lsOrigin.y, lsOrigin.z, lsDir.x, lsDir.y, lsDir.z, 0.f, lsDist};
+                    const Hit vh = traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock);
+                    if (vh.tri == kNoHit) {
+                        Beam<N> beam = source_beam_area<N>(tPos, -lsDir, fr.sourcing_area, lsDist, Le, lam, fr.max_omega, comps);
+                        if (doMIS) {
+                            const float bp = mat_pdf(m, s.f, bounce.wi, lsDir, lam[0]);
+                            beam.scale(lsPdf / (lsPdf + bp));
+                        }
+                        // evalNEE (pltpt_solve.rt.slang:219-243)
+                        const V3 V = -beam.r;
+                        const Surf fs = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, V);
+                        beam.propagate(fs.pos);
+                        mat_eval<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam);
+                        if (fr.options & kOptHwssMis) {
+                            float pp[N];
+                            mat_lambda_pdf<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam, pp);
+#pragma unroll
+                            for (int j = 0; j < N; ++j) beam.spdf[j] *= pp[j];
+                        }
+                        forward_unwind<N>(sc, fr, n, li, p, beam);
+                        measure<N>(sc, fr, sv, p, bounce.contrib, beam);
+                    }
+                }
+            }
+        }
+
+        // ---- emissive hit (evalEmissive, pltpt_solve.rt.slang:152-200) ----
+        const bool didNEE = doNEE && p > 0 && !(bounce.flags & kFlagLastDelta);
+        if (useEmissive && (props & kPropEmissive) && (doMIS || !didNEE)) {
+            const bool needMIS = doMIS && didNEE;
+            float lightPdf = 0.f;
+            const uint32_t lt = sc.tri_light[bounce.tri];
+            if (lt != kNoHit) {
+                // evalTrianglePdf (EmissiveLightSamplerHelpers.slang:134-150)
+                const V3 hn = s.front ? s.faceN : -s.faceN;
+                const V3 toL = s.pos - sv.origin;
+                const float d2 = dot(toL, toL);
+                if (d2 > kFltMin) {
+                    const V3 L = toL * (1.f / sqrtf(d2));
+                    const float ct = dot(hn, -L);
+                    if (ct > 0.f) lightPdf = sc.light_pdf[lt] * (d2 / fmaxf(kFltMin, ct * sc.lights[4 * lt].w));
+                }
+            }
+            const float mis = needMIS ? bounce.lastPdf / (bounce.lastPdf + lightPdf) : 1.f;
+            const plt_material& em = sc.materials[s.material];
+            const plt_spectral_profile* ep = em.emission_profile < sc.n_profiles ? sc.profiles + em.emission_profile : nullptr;
+            float Le[N], lam[N];
+            sample_spectrum<N>(ep, lambda, sg, importance, Le, lam);
+            bool anyLe = false;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                Le[j] *= mis * rgb_to_spectrum(1.f, 1.f, 1.f, lam[j]);
+                anyLe |= Le[j] > 0.f;
+            }
+            if (anyLe) {
+                const V3 dd = sv.origin - s.pos;
+                const float dist = sqrtf(dot(dd, dd));
+                const int comps = (!(fr.options & kOptHwssMis) && (bounce.flags & kFlagDispersive)) ? 1 : N;
+                Beam<N> beam = source_beam_area<N>(s.pos, bounce.wi, fr.sourcing_area, dist, Le, lam, fr.max_omega, comps);
+                forward_unwind<N>(sc, fr, n, li, p, beam);
+                measure<N>(sc, fr, sv, p, bounce.contrib, beam);
+            }
+        }
+
+        if ((bounce.flags & kFlagTerminated) || p == int(fr.max_bounces)) break;
+        sv.origin = s.pos;
+        sv.normal = s.f.N;
+    }
+    fr.color[pid] = make_float4(sv.r, sv.g, sv.b, 1.f);
+}
+
+}  // namespace
+
+int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
+    if (fr.n_paths == 0) return PLT_OK;
+    hipLaunchKernelGGL(sample_kernel, dim3((fr.n_paths + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, sc, fr);
+    return check_launch("sample_kernel");
+}
+int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
+    if (fr.n_paths == 0) return PLT_OK;
+    const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);
+    if (fr.hwss == 4) hipLaunchKernelGGL(solve_kernel<4>, g, b, 0, st, sc, fr);
+    else if (fr.hwss == 1) hipLaunchKernelGGL(solve_kernel<1>, g, b, 0, st, sc, fr);
+    else return set_error(PLT_ENOTIMPL, "HWSS sample counts other than 1 and 4 are not built");
+    return check_launch("solve_kernel");
+}
+
+}  // namespace plt

@@ -1,0 +1,60 @@
+// Path-tracer launch parameters and bounce-record layout shared by host and device code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/plt_abi.h"
+#include "traverse.hpp"
+
+namespace plt {
+
+// Bounce record (the reference's 40 B BounceData, Path.slang:113-142, with the hit packed as
+// triangle id + barycentrics): 9 words per (bounce, path), stored as planes
+// rec[(bounce * kRecWords + field) * n_paths + path] so every field access of a wave is one
+// contiguous 256 B segment.
+enum RecField { kRecTri = 0, kRecB1, kRecB2, kRecWiX, kRecWiY, kRecLobe, kRecFlags, kRecContrib, kRecLastPdf, kRecWords };
+
+// RayFlags (Path.slang:47-109)
+enum : uint32_t {
+    kFlagLastDelta = 0x1, kFlagDispersive = 0x2, kFlagOnlyDelta = 0x4, kFlagMiss = 0x10, kFlagTerminated = 0x20
+};
+
+enum : uint32_t {  // option bits
+    kOptNEE = 0x1, kOptMIS = 0x2, kOptRR = 0x4, kOptHwssMis = 0x8, kOptISEmitters = 0x10, kOptDirectLights = 0x20,
+    kOptEmissiveLights = 0x40
+};
+
+struct PtScene {
+    TraceScene tsc;
+    const float* positions;
+    const float* normals;
+    const uint32_t* indices;
+    const uint32_t* tri_mesh;
+    const uint32_t* mesh_material;
+    const plt_material* materials;
+    const plt_spectral_profile* profiles;
+    uint32_t n_profiles;
+    // camera (Camera.cpp:187-193)
+    float cam_pos[3], cam_u[3], cam_v[3], cam_w[3], jitter[2];
+    // emissive triangles: 4 x float4 per light (p0+area, p1+material, p2, normal), alias table,
+    // per-light selection pdf, triangle -> light index
+    const float4* lights;
+    const uint2* alias;
+    const float* light_pdf;
+    const uint32_t* tri_light;
+    uint32_t n_lights;
+};
+
+struct PtFrame {
+    uint32_t width, height, frame, max_bounces, hwss, options;
+    uint32_t path_base, n_paths;  // paths [path_base, path_base + n_paths) of this launch (row-major pixels)
+    float sourcing_area, max_omega;
+    uint32_t* records;
+    float4* color;
+    float4* albedo;
+    float4* normal;
+};
+
+int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+
+}  // namespace plt

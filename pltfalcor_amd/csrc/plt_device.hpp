@@ -327,6 +327,34 @@ template <int N> struct Beam {
         D01 = c * (D01 * ic);
         D11 = ic * (D11 * ic);
     }
+    // Free-space propagation to p (PLT.slang:224-230): distance accumulates unless distant.
+    PLT_DEV void propagate(V3 p) {
+        if (!distant) {
+            const V3 d = p - pos;
+            r_m += sqrtf(dot(d, d));
+        }
+        pos = p;
+    }
+    // Spatial mutual coherence per HWSS component for a transverse offset v (um)
+    // (PLT.slang:165-170): exp(-(k/2pi)^2 v^T (D/r^2) v / 2).
+    PLT_DEV void mutual_coherence(const float* k, V3 v, float* out) const {
+        const float ir = 1.f / r_mm();
+        const float s = ir * ir;
+        const float q = v.x * (s * D00 * v.x + s * D01 * v.y) + v.y * (s * D01 * v.x + s * D11 * v.y);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const float rl = k[j] * k1_2Pi;
+            out[j] = expf(-0.5f * (rl * rl * q));
+        }
+    }
+    PLT_DEV void scale(float s) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) { S[j][0] *= s; S[j][1] *= s; S[j][2] *= s; S[j][3] *= s; }
+    }
+    PLT_DEV void scale_each(const float* s) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) { S[j][0] *= s[j]; S[j][1] *= s[j]; S[j][2] *= s[j]; S[j][3] *= s[j]; }
+    }
 };
 
 // ---------------------------------------------------------------------------------------

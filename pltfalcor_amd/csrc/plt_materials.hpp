@@ -1,0 +1,768 @@
+// Device material layer of the path tracer: decodes a plt_material (the reference's binary16
+// BasicMaterialData block) into a register-resident instance at each hit, and implements the PLT
+// material interface (Rendering/Materials/IMaterialInstance.slang:122-184) for the five materials
+// of the path workloads: eval on a PLT beam with N HWSS wavelengths, sample, pdf, lambdaPdf,
+// getLobes and the properties the transport reads.
+//
+// Dispatch is a switch on the type tag in registers; the wavefront scheduler sorts hits by
+// material so that a wave mostly executes one case.
+#pragma once
+#include "plt_device.hpp"
+
+namespace plt {
+
+enum : uint32_t {
+    kLobeDiffuseReflection = 0x01, kLobeDispersiveDeltaReflection = 0x08, kLobeNonDeltaReflection = 0x03,
+    kLobeNonDeltaTransmission = 0x30, kLobeDispersiveDelta = 0x88, kLobeAll = 0xffff, kLobeMaterialMask = 0xff00
+};
+enum : uint32_t { kPropTransmissive = 0x1, kPropEmissive = 0x2, kPropMWReflective = 0x4, kPropMWTransmissive = 0x8 };
+
+constexpr float kMeanK = k2Pi / (550.f * 1e-3f);   // FractalSurface / opaque dielectric mean wavenumber
+constexpr float kQcutoffForMS = .15f;               // IMaterialInstance.slang:44
+
+struct Frame {
+    V3 T, B, N;
+    PLT_DEV V3 to_local(V3 v) const { return v3(dot(v, T), dot(v, B), dot(v, N)); }
+    PLT_DEV V3 from_local(V3 v) const { return T * v.x + B * v.y + N * v.z; }
+};
+
+struct BsdfSample {
+    V3 wo;
+    float pdf, weight;
+    uint32_t lobe;
+};
+
+// K-correlation surface (FractalSurface.slang:38-135).
+struct Fractal {
+    float T, gamma, q;
+    PLT_DEV static Fractal from_roughness(float rough, float gamma) {
+        Fractal f;
+        f.gamma = fmaxf(1.1f, gamma);
+        const float a = fminf(fmaxf(rough, 0.f), .75f);
+        const float a2 = a * a;
+        f.T = fminf(5000.f, (1.f - a2) / (4.f * (kMeanK * kMeanK) * a2));
+        f.q = 1.f / sqrtf(1.f + 4.f * (kMeanK * kMeanK) * f.T);
+        return f;
+    }
+    PLT_DEV float pgf(float zx, float zy) const { return fractal_pgf(T, gamma, zx, zy); }
+    PLT_DEV float alpha(float ci, float co, float k) const { return surface_alpha(ci, co, q, k); }
+    PLT_DEV float one_minus_alpha(float ci, float co, float k) const { return surface_one_minus_alpha(ci, co, q, k); }
+    // Sampling after Holzschuch & Pacanowski 2017 (FractalSurface.slang:97-124).
+    PLT_DEV V3 sample(V3 wi, Rng& sg, float k, float& pdf) const {
+        const float s = sqrtf(fmaxf(0.f, 1.f - wi.z * wi.z));
+        const float phi_i = s > 0.f ? atan2f(wi.y, wi.x) : 0.f;
+        const float sqrtT = sqrtf(T);
+        const float u0 = sg.next1D(), u1 = sg.next1D();
+        (void)sg.next1D();
+        (void)sg.next1D();
+        const float M = 1.f - powf(1.f + k * k * T * ((1.f + s) * (1.f + s)), -(gamma - 1.f) * 0.5f);
+        const float f = sqrtf(powf(1.f - M * u0, -2.f / (gamma - 1.f)) - 1.f) / sqrtT;
+        const float fk = f / k;
+        const float phi_max = (f == 0.f || s == 0.f) ? kPi
+                              : acosf(fminf(fmaxf((fk * fk + s * s - 1.f) / (2.f * fk * s), -1.f), 1.f));
+        const float phi_f = phi_i + (2.f * u1 - 1.f) * phi_max;
+        float sp, cp;
+        sincosf(phi_f, &sp, &cp);
+        const float zx = f * cp, zy = f * sp;
+        const float wx = zx / k - wi.x, wy = zy / k - wi.y;
+        const float z = sqrtf(fmaxf(0.f, 1.f - (wx * wx + wy * wy)));
+        pdf = kPi / phi_max * z * pgf(zx, zy);
+        return v3(wx, wy, sgnf(wi.z) * z);
+    }
+    PLT_DEV float pdf(V3 wi, V3 wo, float k) const { return pgf(k * (wi.x + wo.x), k * (wi.y + wo.y)); }
+};
+
+// Gaussian surface (GaussianSurface.slang:37-110).
+struct Gauss {
+    float sigma2, q;
+    PLT_DEV static float rough_to_sigma2(float r) { return powf(fminf(fmaxf(1.f - .5f * r, 0.f), 1.f), 28.f) * 2500.f; }
+    PLT_DEV static Gauss from_roughness(float r) {
+        Gauss g;
+        const float s2 = rough_to_sigma2(r), s2max = rough_to_sigma2(.1f), s2r0 = rough_to_sigma2(0.f);
+        const float t = (s2r0 - s2) / (s2r0 - s2max);
+        g.q = 0.f + t * (1.f - 0.f);
+        g.sigma2 = fminf(s2max, s2);
+        return g;
+    }
+    PLT_DEV bool delta_only() const { return q == 0.f; }
+    // Truncated Box-Muller sample around -wi.xy (MathHelpers.slang:416-434).
+    PLT_DEV V3 sample(V3 wi, Rng& sg, float k, float& pdf) const {
+        const float s2 = (1.f / sigma2) / (k * k);
+        const float mx = -wi.x, my = -wi.y;
+        const float u0 = sg.next1D(), u1 = sg.next1D();
+        const float mm = mx * mx + my * my;
+        const float l = sqrtf(fminf(1.f, mm));
+        const float coso = sqrtf(fmaxf(0.f, 1.f - mm));
+        const float phi_i = (mx != 0.f || my != 0.f) ? atan2f(my, mx) : 0.f;
+        const float s = expf(-.5f * (1.f + l) * (1.f + l) / s2);
+        const float x = (1.f - s) * fmaxf(kFltEps, u0) + s;
+        const float r = sqrtf(-2.f * s2 * logf(x));
+        const float max_phi = (r < kFltEps || l < kFltEps) ? kPi
+                              : acosf(fminf(fmaxf((r * r + l * l - 1.f) / (2.f * r * l), -1.f), 1.f));
+        const float phi = phi_i + kPi + max_phi * (2.f * u1 - 1.f);
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        pdf = .5f * x / (max_phi * s2) * coso;
+        const float wx = r * cp + mx, wy = r * sp + my;
+        const float z = sqrtf(fmaxf(0.f, 1.f - (wx * wx + wy * wy)));
+        return v3(wx, wy, sgnf(wi.z) * z);
+    }
+    PLT_DEV float pdf(V3 wi, V3 wo, float k) const {
+        return boxmueller_trunc_pdf(wo.x, wo.y, -wi.x, -wi.y, (1.f / sigma2) / (k * k));
+    }
+};
+
+// Bessel J_nu for |nu| < 7 by blending a rational small-argument form and the asymptotic
+// form (MathHelpers.slang:851-876).
+PLT_DEV float bessel_j(int nu, float x) {
+    const float sgn = (nu > 0 || ((-nu) % 2) == 0) ? 1.f : -1.f;
+    nu = abs(nu);
+    const float cutoff = float(nu) + 1.f, interp = cutoff / 10.f;
+    const float laa = sqrtf(2.f / kPi / x) * cosf(x - float(nu) * kPi / 2.f - kPi / 4.f);
+    float saa = __builtin_nanf("");
+    if (nu < 7) {
+        const uint32_t fact[13] = {1u, 1u, 2u, 6u, 24u, 120u, 720u, 5040u, 40320u, 362880u, 3628800u, 39916800u, 479001600u};
+        const float x2 = x * x, x4 = x2 * x2, x6 = x2 * x4, x8 = x4 * x4;
+        const float s = 6.f * powf(x / 2.f, float(nu));
+        const float a = 1.f / (6.f * float(fact[nu]));
+        const float b = -x2 / (24.f * float(fact[nu + 1]));
+        const float c1 = 11.f * x8 - 864.f * float(6 + nu) *
+                         (x6 - 12.f * (5.f + float(nu)) * (3.f * x4 + 32.f * float(4 + nu) * (-2.f * x2 + 27.f * float(3 + nu))));
+        const float c = c1 / float(fact[6 + nu]) * x4 * 5.652695401144601e-10f;
+        saa = s * (a + b + c);
+    }
+    if (nu >= 7 || fabsf(x) >= cutoff + interp) return laa * sgn;
+    const float w = fmaxf(0.f, fminf(1.f, .5f * (x - cutoff - interp) / interp));
+    return (saa + w * (laa - saa)) * sgn;
+}
+
+// One-dimensional or 2-D sinusoidal phase grating (DiffractionGrating.slang:41-186).
+struct Grating {
+    float gx, gy;      // grating direction in the shading tangent plane
+    float ipx, ipy;    // 1/pitch (1/um), 0 = no grating along that axis
+    float q;           // height (um)
+    uint32_t lobes, type;
+    float mult;
+    PLT_DEV bool is1d() const { return ipy == 0.f; }
+    PLT_DEV float alpha(float wiz, float k) const { const float a = wiz * q * k; return expf(-(a * a)); }
+    PLT_DEV float lobe_intensity(int lx, int ly, float wiz, float lambda_um) const {
+        const uint32_t t = type & 0xFu;
+        const float a = 4.f * kPi * q / (lambda_um * fabsf(wiz));
+        float ix, iy;
+        if (t == 0) {
+            ix = iy = lx == 0 ? 1.f : bessel_j(lx, a);
+            if (!is1d()) iy = ly == 0 ? 1.f : bessel_j(ly, a);
+        } else if (t == 1) {
+            const float s = sinf(a / 2.f);
+            const float ax = kPi * float(lx) / 2.f;
+            ix = iy = lx == 0 ? 1.f : s * (sinf(ax) / ax);
+            if (!is1d()) { const float ay = kPi * float(ly) / 2.f; iy = ly == 0 ? 1.f : s * (sinf(ay) / ay); }
+        } else {
+            ix = iy = lx == 0 ? 1.f : 1.f / sqrtf(fabsf(float(lx)));
+            if (!is1d()) iy = ly == 0 ? 1.f : 1.f / sqrtf(fabsf(float(ly)));
+        }
+        return mult * ix * iy;
+    }
+    // Grating equation for order (lx, ly) (DiffractionGrating.slang:142-156).
+    PLT_DEV V3 diffract(V3 wi, int lx, int ly, float lambda_um) const {
+        const float px = sqrtf(wi.x * wi.x + wi.z * wi.z), py = sqrtf(wi.y * wi.y + wi.z * wi.z);
+        const float sx = px > kFltEps ? wi.x / px : 0.f, sy = py > kFltEps ? wi.y / py : 0.f;
+        const float a = lambda_um * float(lx) * ipx - sx;
+        const float b = lambda_um * float(ly) * ipy - sy;
+        const float m = (a * a - 1.f) / ((a * b) * (a * b) - 1.f);
+        const float qq = 1.f - b * b * m;
+        if (!(fabsf(a) <= 1.f && fabsf(b) <= 1.f)) return v3(0.f, 0.f, 0.f);
+        return v3(a * sqrtf(fmaxf(0.f, qq)), b * sqrtf(m), sqrtf(fmaxf(0.f, 1.f - a * a * qq - b * b * m)));
+    }
+    // Samples a diffraction order proportionally to its efficiency (DiffractionGrating.slang:109-140).
+    PLT_DEV void sample_lobe(V3 wi, float lambda_um, Rng& sg, int& lx, int& ly, float& px, float& py) const {
+        float inten[8];
+        float total = 0.f;
+        for (uint32_t l = 0; l < lobes; ++l) {
+            const float li = lobe_intensity(int(l) + 1, 0, wi.z, lambda_um);
+            total += li;
+            inten[l] = li;
+        }
+        const float rx = (sg.next1D() - .5f) * 2.f, ry = (sg.next1D() - .5f) * 2.f;
+        float cdf = 0.f;
+        lx = ly = 0;
+        px = py = 0.f;
+        for (uint32_t l = 0; l < lobes; ++l) {
+            const float p = inten[l] / total;
+            cdf += p;
+            const bool sx = fabsf(rx) < cdf && lx == 0, sy = fabsf(ry) < cdf && ly == 0;
+            if (sx) { px = p; lx = int(l) + 1; }
+            if (sy) { py = p; ly = int(l) + 1; }
+        }
+        px /= 2.f;
+        py /= 2.f;
+        lx = rx < 0.f ? -lx : lx;
+        ly = ry < 0.f ? -ly : ly;
+        if (is1d()) { ly = 0; py = 1.f; }
+    }
+};
+// Diffraction order packed in the material-specific lobe bits (PLTDiffractionGratedConductorBSDF.slang:54-66).
+PLT_DEV uint32_t encode_order(int lx, int ly, uint32_t lobe) {
+    const int sx = lx < 0 ? 0 : 1, sy = ly < 0 ? 0 : 1;
+    const int ax = min(abs(lx), 7), ay = min(abs(ly), 7);
+    lobe &= ~kLobeMaterialMask;
+    lobe += uint32_t((ay << 3) + ax + (sy << 7) + (sx << 6)) << 8;
+    return lobe;
+}
+PLT_DEV void decode_order(uint32_t lobe, int& lx, int& ly) {
+    const int x = int(lobe) >> 8;
+    lx = x & 0x7;
+    ly = (x >> 3) & 0x7;
+    lx *= 2 * ((x >> 6) & 0x1) - 1;
+    ly *= 2 * ((x >> 7) & 0x1) - 1;
+}
+
+// Register-resident material instance.
+struct Mat {
+    uint32_t type;
+    float col[3];      // base colour / specular / albedo
+    float trans[3];    // dielectric transmission colour
+    // dielectric
+    float etaInt, etaExt, dispB;
+    Gauss gs;
+    // K-correlation surfaces and conductors
+    Fractal fs;
+    const plt_spectral_profile* pn;
+    const plt_spectral_profile* pk;
+    float extEta;
+    // coated
+    float coatIor, tau;
+    // grating
+    Grating gr;
+    // emission
+    uint32_t emission;
+};
+
+PLT_DEV float h16(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
+
+// setupMaterialInstance of each *Material.slang.
+PLT_DEV Mat setup_material(const plt_material& m, const plt_spectral_profile* prof, uint32_t nprof) {
+    Mat r;
+    r.type = m.type;
+    r.col[0] = h16(m.base_color[0]); r.col[1] = h16(m.base_color[1]); r.col[2] = h16(m.base_color[2]);
+    r.trans[0] = h16(m.transmission[0]); r.trans[1] = h16(m.transmission[1]); r.trans[2] = h16(m.transmission[2]);
+    r.pn = m.ior_n_profile < nprof ? prof + m.ior_n_profile : nullptr;
+    r.pk = m.ior_k_profile < nprof ? prof + m.ior_k_profile : nullptr;
+    r.emission = m.emission_profile;
+    r.extEta = h16(m.ior);
+    const float sx = h16(m.specular[0]), sy = h16(m.specular[1]), sz = h16(m.specular[2]), sw = h16(m.specular[3]);
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {  // PLTDielectricMaterial.slang:40-56
+            r.etaExt = sw;
+            r.dispB = (r.trans[0] > 0.f || r.trans[1] > 0.f || r.trans[2] > 0.f) ? sx : 0.f;
+            r.etaInt = fmaxf(1.f, h16(m.ior) - 2.897f * r.dispB);
+            r.gs = Gauss::from_roughness(sy);
+            break;
+        }
+        case PLT_MAT_CONDUCTOR:  // PLTConductorMaterial.slang:40-57
+            r.fs = Fractal::from_roughness(sy, sx);
+            break;
+        case PLT_MAT_OPAQUE_DIELECTRIC:  // PLTOpaqueDielectricMaterial.slang:40-54
+            r.fs = Fractal::from_roughness(sy, 3.f);
+            break;
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC:  // PLTCoatedOpaqueDielectricMaterial.slang:40-60
+            r.fs = Fractal::from_roughness(sy, 3.f);
+            r.tau = h16(m.data1[0]) * sz;
+            r.coatIor = h16(m.data1[3]);
+            break;
+        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorMaterial.slang:40-70
+            r.fs = Fractal::from_roughness(h16(m.data2[1]), 3.f);
+            const float ptx = h16(m.data1[0]), pty = h16(m.data1[1]);
+            r.gr.ipx = ptx > 0.f ? 1.f / ptx : 0.f;
+            r.gr.ipy = pty > 0.f ? 1.f / pty : 0.f;
+            r.gr.q = h16(m.data2[0]) * sx;
+            r.gr.mult = sy;
+            const uint32_t tl = m.data1[3];
+            r.gr.lobes = min(max(tl & 0xFFu, 1u), 8u);
+            r.gr.type = tl >> 8;
+            const float dir = h16(m.data1[2]);
+            r.gr.gx = cosf(dir);
+            r.gr.gy = sinf(dir);
+            break;
+        }
+        default:
+            break;
+    }
+    return r;
+}
+
+PLT_DEV uint32_t mat_lobes(const Mat& m) {
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {
+            const uint32_t dt = m.dispB != 0.f ? kLobeDispersiveDeltaTransmission : kLobeDeltaTransmission;
+            return m.gs.delta_only() ? (kLobeDeltaReflection | dt)
+                                     : (kLobeDeltaReflection | dt | kLobeScatterReflection | kLobeScatterTransmission);
+        }
+        case PLT_MAT_DIFFUSE: return kLobeDiffuseReflection;
+        case PLT_MAT_GRATED_CONDUCTOR:
+            return (m.fs.q == 0.f ? kLobeDeltaReflection : (kLobeDeltaReflection | kLobeScatterReflection)) |
+                   kLobeDispersiveDeltaReflection;
+        default:
+            return m.fs.q == 0.f ? kLobeDeltaReflection : (kLobeDeltaReflection | kLobeScatterReflection);
+    }
+}
+PLT_DEV uint32_t mat_props(const Mat& m) {
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: return kPropTransmissive | (m.gs.q < kQcutoffForMS ? kPropMWTransmissive : 0u);
+        case PLT_MAT_DIFFUSE: return m.emission != PLT_NO_PROFILE ? kPropEmissive : 0u;
+        case PLT_MAT_GRATED_CONDUCTOR: return kPropMWReflective;
+        default: return m.fs.q < kQcutoffForMS ? kPropMWReflective : 0u;
+    }
+}
+
+// Shading frame used for eval / sample / pdf: the grated conductor evaluates in a frame whose
+// tangent follows the grating direction (DiffractionGrating.slang:72-79).
+PLT_DEV Frame mat_frame(const Mat& m, const Frame& sd) {
+    if (m.type != PLT_MAT_GRATED_CONDUCTOR) return sd;
+    Frame g;
+    g.T = sd.from_local(v3(m.gr.gx, m.gr.gy, 0.f));
+    g.N = sd.N;
+    g.B = cross(g.N, g.T);
+    return g;
+}
+
+// ---------------------------------------------------------------------------------------
+// eval on the local frame (beam already rotated to the exitant frame).
+// ---------------------------------------------------------------------------------------
+template <int N>
+PLT_DEV void conductor_like_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    Conductor c;
+    c.etaN = m.pn; c.etaK = m.pk; c.extEta = m.extEta; c.T = m.fs.T; c.gamma = m.fs.gamma; c.q = m.fs.q;
+    c.spec[0] = m.col[0]; c.spec[1] = m.col[1]; c.spec[2] = m.col[2];
+    conductor_eval<N>(c, wi, wo, lobe, beam);
+}
+
+template <int N>
+PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {
+            Dielectric d;
+            d.etaInt = m.etaInt; d.etaExt = m.etaExt; d.B = m.dispB; d.sigma2 = m.gs.sigma2; d.q = m.gs.q;
+            for (int c = 0; c < 3; ++c) { d.spec[c] = m.col[c]; d.trans[c] = m.trans[c]; }
+            dielectric_eval<N>(d, wi, wo, lobe, beam);
+            break;
+        }
+        case PLT_MAT_CONDUCTOR:
+            conductor_like_eval<N>(m, wi, wo, lobe, beam);
+            break;
+        case PLT_MAT_DIFFUSE: {  // PLTDiffuseBSDF.slang:43-55
+            const bool valid = (lobe & kLobeDiffuseReflection) != 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const float w = valid ? fabsf(wi.z) * k1_Pi * rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]) : 0.f;
+                beam.S[j][0] *= w;
+                beam.S[j][1] = beam.S[j][2] = beam.S[j][3] = 0.f;
+            }
+            break;
+        }
+        case PLT_MAT_OPAQUE_DIELECTRIC: {  // PLTOpaqueDielectricBSDF.slang:57-82
+            const bool isDirect = (lobe & kLobeDelta) != 0, isScatter = (lobe & kLobeNonDelta) != 0;
+            if ((!isScatter && !isDirect) || (lobe & kLobeReflection) == 0) { beam.set_zero(); break; }
+            float M = m.fs.alpha(wi.z, wo.z, kMeanK);
+            if (isScatter) {
+                const V3 h = wi + wo;
+                M = m.fs.one_minus_alpha(wi.z, wo.z, kMeanK) * fabsf(wi.z) * m.fs.pgf(kMeanK * h.x, kMeanK * h.y);
+            }
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (j < beam.comps) {
+                    const float w = M * rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]);
+                    beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
+                }
+            }
+            break;
+        }
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {  // PLTCoatedOpaqueDielectricBSDF.slang:69-124
+            const bool isRefl = wi.z * wo.z > 0.f;
+            const bool isScatter = (lobe & kLobeScatterReflection) != 0, isDirect = (lobe & kLobeDeltaReflection) != 0;
+            const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
+            const float eta1 = coatior / m.extEta;
+            float cT;
+            (void)fresnel_dielectric_F(eta1, wi.z, cT);
+            const V3 wiIn = -refract_dir(wi, cT, eta1);
+            V3 woIn = v3(-wiIn.x, -wiIn.y, wiIn.z);
+            if (!isDirect) {
+                float cT2;
+                (void)fresnel_dielectric_F(eta1, wo.z, cT2);
+                woIn = -refract_dir(wo, cT2, eta1);
+            }
+            if (!isRefl || (!isScatter && !isDirect) || wiIn.z <= 0.f || woIn.z <= 0.f) { beam.set_zero(); break; }
+            float k[N], coh[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) k[j] = k2Pi / (1e-3f * beam.lambda[j]);
+            const V3 h = wiIn + woIn;
+            const V3 mh = normalize(h);
+            const float LdotH = fmaxf(0.f, dot(wiIn, mh));
+            const V3 Z = beam.to_local(v3(0.f, 0.f, 1.f));
+            beam.mutual_coherence(k, Z * m.tau, coh);
+            const float dist = 2.f * m.tau / wiIn.z;
+            // M = MiT * MoT: products of two transmission Mueller blocks
+            const Fres Ti = fresnel_dielectric_mueller(wi.z, eta1, false);
+            const Fres To = fresnel_dielectric_mueller(-woIn.z, eta1, false);
+            const Fres M{Ti.a * To.a + Ti.b * To.b, Ti.a * To.b + Ti.b * To.a, Ti.c * To.c - Ti.d * To.d, Ti.c * To.d + Ti.d * To.c};
+            const Fres Rl = fresnel_dielectric_mueller(isScatter ? LdotH : wi.z, eta1, true);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                if (j < beam.comps) {
+                    const float m00 = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]);
+                    float w;
+                    if (isScatter) {
+                        w = m.fs.one_minus_alpha(wiIn.z, woIn.z, k[j]) * fabsf(wi.z) * m.fs.pgf(k[j] * h.x, k[j] * h.y);
+                    } else {
+                        w = m.fs.alpha(wiIn.z, woIn.z, k[j]);
+                    }
+                    float a0 = beam.S[j][0], a1 = beam.S[j][1], a2 = beam.S[j][2], a3 = beam.S[j][3];
+                    float b0 = a0, b1 = a1, b2 = a2, b3 = a3;
+                    apply(M, a0, a1, a2, a3);
+                    const float s1 = m00 * w;
+                    a0 *= s1; a1 *= s1; a2 *= s1; a3 *= s1;
+                    apply(Rl, b0, b1, b2, b3);
+                    b0 *= w; b1 *= w; b2 *= w; b3 *= w;
+                    // interfere (Interference.slang:45-57) with phasor sin(dist k eta)
+                    const float ph = sinf(dist * k[j] * coatior);
+                    const float S1 = (a0 + a1) / 2.f, P1 = (a0 - a1) / 2.f, S2 = (b0 + b1) / 2.f, P2 = (b0 - b1) / 2.f;
+                    const float S = S1 + S2 + 2.f * sqrtf(fmaxf(0.f, S1 * S2)) * ph * coh[j];
+                    const float P = P1 + P2 + 2.f * sqrtf(fmaxf(0.f, P1 * P2)) * ph * coh[j];
+                    const float nrm = a0 + b0 > 1e-6f ? (S + P) / (a0 + b0) : 0.f;
+                    beam.S[j][0] = S + P;
+                    beam.S[j][1] = S - P;
+                    beam.S[j][2] = (a2 + b2) * nrm;
+                    beam.S[j][3] = (a3 + b3) * nrm;
+                }
+            }
+            break;
+        }
+        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:68-107
+            const bool isDirect = (lobe & kLobeDelta) != 0;
+            float a[N], lam[N];
+            bool anyPos = false, anyBelow1 = false;
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                lam[j] = 1e-3f * beam.lambda[j];
+                a[j] = m.gr.alpha(wi.z, k2Pi / lam[j]);
+                anyPos |= a[j] > 0.f;
+                anyBelow1 |= a[j] < 1.f;
+            }
+            int lx, ly;
+            decode_order(lobe, lx, ly);
+            if (lx == 0 && ly == 0 && anyPos) {
+                conductor_like_eval<N>(m, wi, wo, lobe, beam);
+                beam.scale_each(a);
+            } else if ((lx != 0 || ly != 0) && anyBelow1 && isDirect) {
+                const V3 wiBeam = beam.to_local(wi);
+                const float iext = 1.f / m.extEta;
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    if (j < beam.comps) {
+                        const float n = (m.pn ? eval_spectrum(m.pn, beam.lambda[j]) : 0.f) * iext;
+                        const float kk = (m.pk ? eval_spectrum(m.pk, beam.lambda[j]) : 0.f) * iext;
+                        const Fres F = fresnel_conductor_mueller(wi.z, n, kk);
+                        float coh = 1.f;
+                        if (j != 0) coh = beam.mutual_coherence_angular(beam.to_local(m.gr.diffract(wo, lx, ly, lam[j])), wiBeam);
+                        const float inten = m.gr.lobe_intensity(lx, ly, wi.z, lam[j]);
+                        const float w = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]) * (1.f - a[j]) * coh * inten;
+                        apply(F, beam.S[j][0], beam.S[j][1], beam.S[j][2], beam.S[j][3]);
+                        beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
+                    }
+                }
+            } else {
+                beam.set_zero();
+            }
+            break;
+        }
+        default:
+            beam.set_zero();
+    }
+}
+
+// Coherence transform after an interaction (coherenceTransform of each BxDF).
+template <int N>
+PLT_DEV void mat_coherence_transform(const Mat& m, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    bool scatter;
+    switch (m.type) {
+        case PLT_MAT_DIFFUSE: scatter = true; break;
+        case PLT_MAT_GRATED_CONDUCTOR: {
+            int lx, ly;
+            decode_order(lobe, lx, ly);
+            scatter = (lx == 0 && ly == 0) && (lobe & kLobeDelta) == 0;
+            break;
+        }
+        default: scatter = (lobe & kLobeDelta) == 0; break;
+    }
+    if (scatter) beam.scatter_coherence(wo.z);
+}
+
+// IMaterialInstance::eval: world <-> local frame change around the local eval
+// (e.g. PLTConductorBSDF.slang:177-193).
+template <int N>
+PLT_DEV void mat_eval(const Mat& m, const Frame& sd, V3 sdV, V3 toEye, uint32_t lobe, Beam<N>& beam) {
+    const Frame f = mat_frame(m, sd);
+    beam.r = f.to_local(beam.r);
+    beam.t = f.to_local(beam.t);
+    const V3 wi = f.to_local(sdV), wo = f.to_local(toEye);
+    beam.construct_local_frame(wo);
+    mat_eval_local<N>(m, wi, wo, lobe, beam);
+    mat_coherence_transform<N>(m, wo, lobe, beam);
+    beam.r = f.from_local(beam.r);
+    beam.t = f.from_local(beam.t);
+}
+
+// ---------------------------------------------------------------------------------------
+// lambdaPdf (spectral MIS); frame per material as in the reference (the grated conductor uses
+// the plain shading frame here).
+// ---------------------------------------------------------------------------------------
+template <int N>
+PLT_DEV void mat_lambda_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 woW, uint32_t lobe, const Beam<N>& beamW, float* ret) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) ret[j] = 1.f;
+    Beam<N> beam = beamW;
+    beam.r = sd.to_local(beam.r);
+    beam.t = sd.to_local(beam.t);
+    const V3 toLight = sd.to_local(woW);  // bxdf.lambdaPdf(toLocal(sd.V), toLocal(wo)) names them (toLight, toEye)
+    const V3 toEye = sd.to_local(sdV);
+    (void)toLight;
+    const V3 a0 = sd.to_local(sdV), a1 = sd.to_local(woW);  // argument order of the call: (sd.V, wo)
+    const bool isScatter = (lobe & kLobeNonDelta) != 0, isDirect = (lobe & kLobeDelta) != 0;
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {  // PLTDielectricBSDF.slang:238-270 (toLight = a0, toEye = a1)
+            const bool isRefl = a0.z * a1.z > 0.f;
+            const V3 lb = beam.to_local(a0);
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                const float k = k2Pi / (1e-3f * beam.lambda[j]);
+                const float u = k * k1_2Pi;
+                const float eta = (m.etaInt + m.dispB * (u * u)) / m.etaExt;
+                if (isDirect && !isRefl) {
+                    float cT;
+                    (void)fresnel_dielectric_F(eta, a1.z, cT);
+                    ret[j] = j == 0 ? 1.f : beam.mutual_coherence_angular(beam.to_local(refract_dir(a1, cT, eta)), lb);
+                } else if (isScatter) {
+                    const V3 w = isRefl ? a0 : scattered_wo(a0, eta);
+                    ret[j] = m.gs.pdf(a1, w, k);
+                }
+            }
+            break;
+        }
+        case PLT_MAT_CONDUCTOR:  // PLTConductorBSDF.slang:149-163
+            if (isScatter) {
+#pragma unroll
+                for (int j = 0; j < N; ++j) ret[j] = m.fs.pdf(a1, a0, k2Pi / (1e-3f * beam.lambda[j]));
+            }
+            break;
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC:  // PLTCoatedOpaqueDielectricBSDF.slang:190-213
+            if (isScatter) {
+                const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
+                const float eta1 = coatior / m.extEta;
+                float cT, cT1;
+                (void)fresnel_dielectric_F(eta1, a1.z, cT);
+                (void)fresnel_dielectric_F(eta1, a0.z, cT1);
+                const V3 wiIn = -refract_dir(a1, cT, eta1);
+                const V3 woIn = -refract_dir(a0, cT1, eta1);
+#pragma unroll
+                for (int j = 0; j < N; ++j) ret[j] = m.fs.pdf(wiIn, woIn, k2Pi / (1e-3f * beam.lambda[j]));
+            }
+            break;
+        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:167-190
+            int lx, ly;
+            decode_order(lobe, lx, ly);
+            bool anyBelow1 = false;
+            float lam[N];
+#pragma unroll
+            for (int j = 0; j < N; ++j) {
+                lam[j] = 1e-3f * beam.lambda[j];
+                anyBelow1 |= m.gr.alpha(a0.z, k2Pi / lam[j]) < 1.f;
+            }
+            if ((lx != 0 || ly != 0) && anyBelow1 && isDirect) {
+                const V3 lb = beam.to_local(a0);
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                    ret[j] = j == 0 ? 1.f : beam.mutual_coherence_angular(beam.to_local(m.gr.diffract(a1, lx, ly, lam[j])), lb);
+            }
+            break;
+        }
+        default:
+            break;
+    }
+    (void)toEye;
+}
+
+// ---------------------------------------------------------------------------------------
+// pdf(sd, toLight): bxdf.pdf(toLocal(toLight), toLocal(sd.V)).
+// ---------------------------------------------------------------------------------------
+PLT_DEV float mat_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 toLightW, float lambda) {
+    const Frame f = mat_frame(m, sd);
+    const V3 wi = f.to_local(toLightW), wo = f.to_local(sdV);
+    const float k = k2Pi / (1e-3f * lambda);
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {
+            Dielectric d;
+            d.etaInt = m.etaInt; d.etaExt = m.etaExt; d.B = m.dispB; d.sigma2 = m.gs.sigma2; d.q = m.gs.q;
+            return dielectric_pdf(d, wi, wo, lambda);
+        }
+        case PLT_MAT_CONDUCTOR:
+            return m.fs.one_minus_alpha(wi.z, wi.z, k) * m.fs.pdf(wi, wo, k);
+        case PLT_MAT_DIFFUSE:
+            return wo.z * k1_Pi;  // cosine_hemisphere_concentric_pdf(woLocal), woLocal = toLocal(sd.V)
+        case PLT_MAT_OPAQUE_DIELECTRIC:
+            return m.fs.one_minus_alpha(wi.z, wi.z, kMeanK) * m.fs.pdf(wi, wo, kMeanK);
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {
+            if (wi.z <= 0.f || wo.z <= 0.f) return 0.f;
+            const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
+            const float eta1 = coatior / m.extEta;
+            float cT, cT1;
+            const float Fd = fresnel_dielectric_F(eta1, wi.z, cT);
+            (void)fresnel_dielectric_F(eta1, wo.z, cT1);
+            const V3 wiIn = -refract_dir(wi, cT, eta1), woIn = -refract_dir(wo, cT1, eta1);
+            return wiIn.z * woIn.z == 0.f ? 0.f : m.fs.one_minus_alpha(wiIn.z, wiIn.z, k) * (1.f - Fd) * m.fs.pdf(wiIn, woIn, k);
+        }
+        case PLT_MAT_GRATED_CONDUCTOR:
+            return m.fs.one_minus_alpha(wi.z, wi.z, k) * m.fs.pdf(wi, wo, k) * m.gr.alpha(wi.z, k);
+        default:
+            return 0.f;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// sample(sd, sg, lambda): BSDF importance sampling in the hero wavelength.
+// ---------------------------------------------------------------------------------------
+PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float lambda, BsdfSample& s) {
+    const Frame f = mat_frame(m, sd);
+    const V3 wi = f.to_local(sdV);
+    const float k = k2Pi / (1e-3f * lambda);
+    V3 wo = v3(0.f, 0.f, 0.f);
+    float pdf = 0.f, weight = 0.f;
+    uint32_t lobe = 0;
+    bool valid = false;
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {  // PLTDielectricBSDF.slang:159-211 (lobe mask All)
+            const float a = m.gs.delta_only() ? 1.f : surface_alpha(wi.z, wi.z, m.gs.q, k);
+            const float u = k * k1_2Pi;
+            const float eta = (m.etaInt + m.dispB * (u * u)) / m.etaExt;
+            const float pdfDirect = a;
+            bool isRefl;
+            const bool isDirect = pdfDirect == 1.f || sg.next1D() < pdfDirect;
+            pdf = isDirect ? pdfDirect : 1.f - pdfDirect;
+            if (isDirect) {
+                float cT;
+                const float Fd = fresnel_dielectric_F(eta, wi.z, cT);
+                isRefl = Fd > 0.f && sg.next1D() <= Fd;
+                wo = isRefl ? v3(-wi.x, -wi.y, wi.z) : refract_dir(wi, cT, eta);
+                pdf *= isRefl ? Fd : 1.f - Fd;
+                lobe = isRefl ? kLobeDeltaReflection : (m.dispB != 0.f ? kLobeDispersiveDeltaTransmission : kLobeDeltaTransmission);
+            } else {
+                float sp;
+                wo = m.gs.sample(wi, sg, k, sp);
+                pdf *= sp;
+                const V3 mh = normalize(wi + wo);
+                float cT;
+                const float F = fresnel_dielectric_F(eta, sgnf(wi.z) * dot(wi, mh), cT);
+                isRefl = F > 0.f && sg.next1D() <= F;
+                if (!isRefl) wo = scattered_wo(wo, eta);
+                pdf *= isRefl ? F : 1.f - F;
+                lobe = isRefl ? kLobeScatterReflection : kLobeScatterTransmission;
+            }
+            weight = rgb_to_spectrum(isRefl ? m.col[0] : m.trans[0], isRefl ? m.col[1] : m.trans[1],
+                                     isRefl ? m.col[2] : m.trans[2], lambda);
+            valid = !((isRefl && wi.z * wo.z <= 0.f) || (!isRefl && wi.z * wo.z >= 0.f));
+            break;
+        }
+        case PLT_MAT_CONDUCTOR:
+        case PLT_MAT_OPAQUE_DIELECTRIC: {  // PLTConductorBSDF.slang:101-133, PLTOpaqueDielectricBSDF.slang:89-123
+            const float kk = m.type == PLT_MAT_CONDUCTOR ? k : kMeanK;
+            const float a = m.fs.alpha(wi.z, wi.z, kk);
+            weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+            const bool isDirect = a == 1.f || sg.next1D() < a;
+            pdf = isDirect ? a : 1.f - a;
+            if (isDirect) { wo = v3(-wi.x, -wi.y, wi.z); lobe = kLobeDeltaReflection; }
+            else { float sp; wo = m.fs.sample(wi, sg, kk, sp); pdf *= sp; lobe = kLobeScatterReflection; }
+            valid = true;
+            break;
+        }
+        case PLT_MAT_DIFFUSE: {  // PLTDiffuseBSDF.slang:57-65
+            weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+            const float u0 = sg.next1D(), u1 = sg.next1D();
+            wo = cosine_hemisphere(u0, u1);
+            pdf = wo.z * k1_Pi;
+            lobe = kLobeDiffuseReflection;
+            valid = true;
+            break;
+        }
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {  // PLTCoatedOpaqueDielectricBSDF.slang:131-180
+            if (wi.z <= 0.f) break;
+            const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
+            const float eta1 = coatior / m.extEta;
+            float cT;
+            const float Fd = fresnel_dielectric_F(eta1, wi.z, cT);
+            const bool isReflCoat = Fd == .1f || sg.next1D() < Fd;
+            weight = 1.f;
+            pdf = isReflCoat ? Fd : 1.f - Fd;
+            if (isReflCoat) {
+                wo = v3(-wi.x, -wi.y, wi.z);
+                lobe = kLobeDeltaReflection;
+                valid = true;
+                break;
+            }
+            const V3 wiIn = -refract_dir(wi, cT, eta1);
+            if (wiIn.z == 0.f) break;
+            weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+            const float a = m.fs.alpha(wiIn.z, wiIn.z, k);
+            const bool isDirect = a == 1.f || sg.next1D() < a;
+            pdf *= isDirect ? a : 1.f - a;
+            if (isDirect) {
+                wo = v3(-wi.x, -wi.y, wi.z);
+                lobe = kLobeDeltaReflection;
+            } else {
+                float sp;
+                const V3 woIn = m.fs.sample(wiIn, sg, k, sp);
+                pdf *= sp;
+                float cT1;
+                (void)fresnel_dielectric_F(eta1, -woIn.z, cT1);
+                wo = refract_dir(-woIn, cT1, eta1);
+                lobe = kLobeScatterReflection;
+            }
+            valid = wi.z * wo.z > 0.f;
+            break;
+        }
+        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:115-153 (no hint)
+            const float lam = 1e-3f * lambda;
+            const float a = m.gr.alpha(wi.z, k2Pi / lam);
+            const bool isDirect = a == 1.f || sg.next1D() < a;
+            pdf = isDirect ? a : 1.f - a;
+            if (isDirect) {
+                const float ab = m.fs.alpha(wi.z, wi.z, k);
+                weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+                float bp = 1.f;
+                const bool bd = ab == 1.f || sg.next1D() < ab;
+                bp = bd ? ab : 1.f - ab;
+                if (bd) { wo = v3(-wi.x, -wi.y, wi.z); lobe = kLobeDeltaReflection; }
+                else { float sp; wo = m.fs.sample(wi, sg, k, sp); bp *= sp; lobe = kLobeScatterReflection; }
+                pdf *= bp;
+                valid = true;
+                break;
+            }
+            weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+            int lx, ly;
+            float px, py;
+            m.gr.sample_lobe(wi, lam, sg, lx, ly, px, py);
+            wo = m.gr.diffract(wi, lx, ly, lam);
+            lobe = encode_order(lx, ly, kLobeDispersiveDeltaReflection);
+            pdf *= px * py;
+            valid = true;
+            break;
+        }
+        default:
+            break;
+    }
+    s.wo = f.from_local(wo);
+    s.pdf = pdf;
+    s.weight = weight;
+    s.lobe = lobe;
+    return valid;
+}
+
+}  // namespace plt

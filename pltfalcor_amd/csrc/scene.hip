@@ -12,12 +12,6 @@
 
 using plt::set_error;
 
-struct plt_scene {
-    plt_context* ctx = nullptr;
-    plt::HostScene host;
-    plt::DeviceSceneBuffers dev;
-};
-
 namespace plt {
 namespace {
 

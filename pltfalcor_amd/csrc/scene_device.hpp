@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <vector>
 #include "../../include/plt_abi.h"
+#include "scene_host.hpp"
 
 namespace plt {
 
@@ -24,3 +25,10 @@ struct DeviceSceneBuffers {
 };
 
 }  // namespace plt
+
+// Definition of the ABI's opaque scene handle.
+struct plt_scene {
+    plt_context* ctx = nullptr;
+    plt::HostScene host;
+    plt::DeviceSceneBuffers dev;
+};

@@ -1,0 +1,108 @@
+"""Host mirror of the reference's PLTPT render pass (Source/RenderPasses/PLTPT/PLTPT.{h,cpp}).
+
+`PLTPT(ctx, dict)` accepts the reference's dictionary keys (PLTPT.cpp:102-134); unknown keys
+warn and are ignored as the reference's parseDictionary does (PLTPT.cpp:173). `execute` renders
+one frame into caller-owned RGBA32F device buffers and advances the frame counter.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import torch
+
+from . import _lib
+
+
+class RenderData(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("color", C.c_void_p), ("albedo", C.c_void_p),
+                ("normal", C.c_void_p), ("vbuffer", C.c_void_p)]
+
+
+class PLTPT:
+    """The PLT path tracer pass on one GPU (ctx None: options only, execute raises)."""
+
+    def __init__(self, ctx, options: dict | None = None):
+        self.lib = _lib.load()
+        self.ctx = ctx
+        h = C.c_void_p()
+        _lib.check(self.lib.plt_pass_create(ctx.handle if ctx is not None else None, b"PLTPT", C.byref(h)))
+        self.handle = h
+        self.scene = None
+        for k, v in (options or {}).items():
+            self.set_option(k, v)
+
+    def set_option(self, key: str, value) -> bool:
+        rc = self.lib.plt_pass_set_option(self.handle, key.encode(), float(value))
+        if rc == _lib.PLT_WARN_UNKNOWN_KEY:
+            warnings.warn(f"Unknown field '{key}' in PLTPathTracer dictionary.")
+            return False
+        _lib.check(rc)
+        return True
+
+    def get_option(self, key: str) -> float:
+        v = C.c_double()
+        _lib.check(self.lib.plt_pass_get_option(self.handle, key.encode(), C.byref(v)))
+        return v.value
+
+    def get_scripting_dictionary(self) -> dict:
+        out, i = {}, 0
+        while True:
+            name = self.lib.plt_pass_option_name(i)
+            if not name:
+                return out
+            out[name.decode()] = self.get_option(name.decode())
+            i += 1
+
+    def set_scene(self, scene):
+        _lib.check(self.lib.plt_pass_set_scene(self.handle, scene.handle if scene is not None else None))
+        self.scene = scene
+
+    def set_frame(self, frame: int):
+        _lib.check(self.lib.plt_pass_set_frame(self.handle, frame))
+
+    def execute(self, width: int, height: int, color: torch.Tensor, albedo: torch.Tensor, normal: torch.Tensor,
+                stream=None):
+        for t in (color, albedo, normal):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != 4 * width * height:
+                raise ValueError("outputs must be contiguous float32 CUDA tensors of width*height*4 elements")
+        rd = RenderData(width, height, color.data_ptr(), albedo.data_ptr(), normal.data_ptr(), None)
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        _lib.check(self.lib.plt_pass_execute(self.handle, C.byref(rd), C.c_void_p(stream.cuda_stream)))
+
+    def render(self, width: int, height: int, frames: int = 1, first_frame: int = 0):
+        """Renders `frames` frames and returns their mean colour (an FP64 accumulation of the outputs) and the
+        last frame's albedo/normal, all as CPU tensors [height, width, 4]."""
+        dev = torch.device("cuda")
+        color = torch.empty((height, width, 4), dtype=torch.float32, device=dev)
+        albedo = torch.empty_like(color)
+        normal = torch.empty_like(color)
+        acc = torch.zeros((height, width, 4), dtype=torch.float64, device=dev)
+        self.set_frame(first_frame)
+        for _ in range(frames):
+            self.execute(width, height, color, albedo, normal)
+            acc += color.double()
+        torch.cuda.synchronize()
+        return (acc / frames).cpu(), albedo.cpu(), normal.cpu()
+
+    def bounce_records(self, width: int, height: int):
+        """The last frame's bounce records as uint32 [maxBounces+1, 9, width*height]: per bounce the hit
+        triangle, barycentrics, octahedral wi, lobe, flags, path contribution and last non-delta pdf."""
+        import numpy as np
+        nb = int(self.get_option("maxBounces")) + 1
+        out = np.empty((nb, 9, width * height), np.uint32)
+        torch.cuda.synchronize()
+        _lib.check(self.lib.plt_pass_copy_bounce_records(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.plt_pass_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -43,6 +43,12 @@ class Scene:
             tri_mesh=cp(a.tri_mesh, a.n_triangles, np.uint32),
             mesh_first_tri=cp(a.mesh_first_tri, a.n_meshes, np.uint32),
             mesh_material=cp(a.mesh_material, a.n_meshes, np.uint32),
+            materials=C.string_at(a.materials, a.n_materials * C.sizeof(Material)),
+            profiles=C.string_at(a.profiles, a.n_profiles * C.sizeof(_lib.SpectralProfile)),
+            n_profiles=a.n_profiles,
+            camera=np.array(list(a.camera.pos) + list(a.camera.target) + list(a.camera.up) +
+                            [a.camera.fov_y, a.camera.aspect, a.camera.focal_distance] + list(a.camera.jitter),
+                            dtype=np.float32),
         )
 
     def trace(self, rays: torch.Tensor, any_hit: bool = False, stream=None) -> torch.Tensor:

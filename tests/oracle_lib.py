@@ -136,3 +136,58 @@ def compute_ray_origin(pos, n):
     out = np.zeros(3, np.float32)
     _geom().oracle_compute_ray_origin(ptr(p, C.c_float), ptr(nn, C.c_float), ptr(out, C.c_float))
     return out
+
+
+REC_WORDS = 9
+
+
+def render(scene: dict, width: int, height: int, frame: int = 0, max_bounces: int = 6, first: int = 0, count=None,
+           do_nee=1, do_mis=1, do_rr=1, hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0,
+           max_omega=0.025, threads=0, records=False):
+    """Oracle render of pixels [first, first+count) (row-major). Returns color [count, 4] and optionally
+    the bounce records [(max_bounces+1), 9, count] (uint32)."""
+    l = lib()
+    fn = l.oracle_render
+    fn.restype = C.c_int
+    fn.argtypes = [_F, _F, C.c_uint32, _U32, _U32, _U32, C.c_char_p, C.c_char_p, C.c_uint32, _F,
+                   C.POINTER(C.c_int32), _F, C.c_uint32, C.c_uint32, _F, _U32, C.c_int]
+    if count is None:
+        count = width * height - first
+    pos = np.ascontiguousarray(scene["positions"], np.float32)
+    nrm = np.ascontiguousarray(scene["normals"], np.float32)
+    idx = np.ascontiguousarray(scene["indices"], np.uint32)
+    tm = np.ascontiguousarray(scene["tri_mesh"], np.uint32)
+    mm = np.ascontiguousarray(scene["mesh_material"], np.uint32)
+    cam = np.ascontiguousarray(scene["camera"], np.float32)
+    iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct, emissive],
+                    np.int32)
+    fopt = np.array([sourcing_area, max_omega], np.float32)
+    color = np.zeros((count, 4), np.float32)
+    rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
+    fn(ptr(pos, C.c_float), ptr(nrm, C.c_float), idx.shape[0], ptr(idx, C.c_uint32), ptr(tm, C.c_uint32),
+       ptr(mm, C.c_uint32), scene["materials"], scene["profiles"], scene["n_profiles"], ptr(cam, C.c_float),
+       iopt.ctypes.data_as(C.POINTER(C.c_int32)), ptr(fopt, C.c_float), first, count, ptr(color, C.c_float),
+       ptr(rec, C.c_uint32) if records else None, threads)
+    if records:
+        return color, rec.reshape(max_bounces + 1, REC_WORDS, count)
+    return color
+
+
+def alias_table(scene: dict):
+    """The oracle's emissive power alias table: (threshold, redirect, perm, pdf) per light."""
+    l = lib()
+    fn = l.oracle_alias_table
+    fn.restype = C.c_int
+    fn.argtypes = [_F, C.c_uint32, _U32, _U32, _U32, C.c_char_p, _F, _U32, _U32, _F, C.c_uint32]
+    pos = np.ascontiguousarray(scene["positions"], np.float32)
+    idx = np.ascontiguousarray(scene["indices"], np.uint32)
+    tm = np.ascontiguousarray(scene["tri_mesh"], np.uint32)
+    mm = np.ascontiguousarray(scene["mesh_material"], np.uint32)
+    cap = idx.shape[0]
+    thr = np.zeros(cap, np.float32)
+    red = np.zeros(cap, np.uint32)
+    perm = np.zeros(cap, np.uint32)
+    pdf = np.zeros(cap, np.float32)
+    n = fn(ptr(pos, C.c_float), idx.shape[0], ptr(idx, C.c_uint32), ptr(tm, C.c_uint32), ptr(mm, C.c_uint32),
+           scene["materials"], ptr(thr, C.c_float), ptr(red, C.c_uint32), ptr(perm, C.c_uint32), ptr(pdf, C.c_float), cap)
+    return thr[:n], red[:n], perm[:n], pdf[:n]
