@@ -40,6 +40,7 @@ PLT_DEV V3 oct_decode(float px, float py) {
     return v3(f * sgnf(px) * cosf(phi), f * sgnf(py) * sinf(phi), sgnf(d) * (1.f - r * r));
 }
 PLT_DEV float offset_component(float p, float n) {
+#pragma clang fp contract(off)
     const int io = int(n * (3.f * 256.f));
     const float ip = __int_as_float(__float_as_int(p) + (p < 0.f ? -io : io));
     return fabsf(p) < (1.f / 16.f) ? p + n * (3.f / 65536.f) : ip;
@@ -57,6 +58,9 @@ struct Surf {
     bool front;
 };
 PLT_DEV Surf surface_at(const PtScene& sc, uint32_t tri, float b1, float b2, V3 viewDir) {
+    // Uncontracted: the shading point must round as a strict-IEEE evaluation does, because light
+    // samples taken from an emitter's own plane decide on the sign of a near-zero cosine.
+#pragma clang fp contract(off)
     const uint32_t i0 = sc.indices[3 * tri], i1 = sc.indices[3 * tri + 1], i2 = sc.indices[3 * tri + 2];
     const float w0 = 1.f - b1 - b2;
     const V3 p0 = v3(sc.positions[3 * i0], sc.positions[3 * i0 + 1], sc.positions[3 * i0 + 2]);
@@ -66,9 +70,12 @@ PLT_DEV Surf surface_at(const PtScene& sc, uint32_t tri, float b1, float b2, V3 
     const V3 n1 = v3(sc.normals[3 * i1], sc.normals[3 * i1 + 1], sc.normals[3 * i1 + 2]);
     const V3 n2 = v3(sc.normals[3 * i2], sc.normals[3 * i2 + 1], sc.normals[3 * i2 + 2]);
     Surf s;
-    s.pos = p0 * w0 + p1 * b1 + p2 * b2;
+    s.pos = v3(p0.x * w0 + p1.x * b1 + p2.x * b2, p0.y * w0 + p1.y * b1 + p2.y * b2, p0.z * w0 + p1.z * b1 + p2.z * b2);
     const V3 N = normalize(n0 * w0 + n1 * b1 + n2 * b2);
-    s.faceN = normalize(cross(p1 - p0, p2 - p0));
+    const V3 e1 = p1 - p0, e2 = p2 - p0;
+    const V3 c = v3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    const float cl = sqrtf(c.x * c.x + c.y * c.y + c.z * c.z);
+    s.faceN = v3(c.x / cl, c.y / cl, c.z / cl);
     s.f.N = N;
     s.f.T = perp_stark(N);
     s.f.B = cross(N, s.f.T);
@@ -391,6 +398,9 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
 #pragma unroll
                 for (int j = 0; j < N; ++j) { Le[j] = 0.f; lam[j] = lambda; }
                 if (valid) {
+                    // Uncontracted, division-normalised geometry: the cosine test and the shadow ray
+                    // round exactly as a strict-IEEE evaluation of the same expressions.
+#pragma clang fp contract(off)
                     // EmissivePowerSampler.sampleLight (EmissivePowerSampler.slang:62-83)
                     const float uL = sg.next1D();
                     uint32_t ti = min(uint32_t(uL * float(sc.n_lights)), sc.n_lights - 1);
@@ -408,11 +418,11 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
                     const float w0 = 1.f - bb0 - bb1;
                     tPos = v3(q0.x * w0 + q1.x * bb0 + q2.x * bb1, q0.y * w0 + q1.y * bb0 + q2.y * bb1, q0.z * w0 + q1.z * bb0 + q2.z * bb1);
                     const V3 toL = tPos - s.pos;
-                    const float d2 = fmaxf(kFltMin, dot(toL, toL));
+                    const float d2 = fmaxf(kFltMin, toL.x * toL.x + toL.y * toL.y + toL.z * toL.z);
                     const float dist = sqrtf(d2);
-                    const V3 ldir = toL * (1.f / dist);
+                    const V3 ldir = v3(toL.x / dist, toL.y / dist, toL.z / dist);
                     const V3 ln = v3(qn.x, qn.y, qn.z);
-                    const float cosT = dot(ln, -ldir);
+                    const float cosT = ln.x * -ldir.x + ln.y * -ldir.y + ln.z * -ldir.z;
                     if (cosT <= 0.f) {
                         valid = false;
                     } else {
@@ -429,10 +439,11 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
                         lightPos = ray_origin(tPos, ln);
                         // computeNewRayOrigin(sd, dir) (ShadowRays.slangh:50-52)
                         const V3 rd = lightPos - s.pos;
-                        lsOrigin = ray_origin(s.pos, dot(s.faceN, rd) >= 0.f ? s.faceN : -s.faceN);
+                        const float side = s.faceN.x * rd.x + s.faceN.y * rd.y + s.faceN.z * rd.z;
+                        lsOrigin = ray_origin(s.pos, side >= 0.f ? s.faceN : -s.faceN);
                         const V3 tl = lightPos - lsOrigin;
-                        lsDist = sqrtf(dot(tl, tl));
-                        lsDir = normalize(tl);
+                        lsDist = sqrtf(tl.x * tl.x + tl.y * tl.y + tl.z * tl.z);
+                        lsDir = v3(tl.x / lsDist, tl.y / lsDist, tl.z / lsDist);
                         bool anyL = false;
 #pragma unroll
                         for (int j = 0; j < N; ++j) anyL |= Le[j] > 0.f;

@@ -80,6 +80,16 @@ struct Rng {
     }
     // (bits >> 8) * 2^-24: exact in float (SampleGeneratorInterface.slang:46-52)
     PLT_DEV float next1D() { return float(next() >> 8) * 0x1p-24f; }
+    // A draw taken only when `take` (the state is unchanged otherwise; the value is then 0), as
+    // straight-line selects: sampling decisions of the form `p == 1 || u < p` stay branch-free,
+    // a shape that hipcc -O2/-O3 for gfx950 miscompiles when `u` is drawn on a divergent branch
+    // (a value hoisted above the branch is clobbered inside it for lanes that rejoin later).
+    PLT_DEV float next1D_if(bool take) {
+        Rng n = *this;
+        const float u = n.next1D();
+        s0 = take ? n.s0 : s0; s1 = take ? n.s1 : s1; s2 = take ? n.s2 : s2; s3 = take ? n.s3 : s3;
+        return take ? u : 0.f;
+    }
 };
 
 // ---------------------------------------------------------------------------------------

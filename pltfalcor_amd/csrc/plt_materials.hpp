@@ -645,12 +645,14 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             const float eta = (m.etaInt + m.dispB * (u * u)) / m.etaExt;
             const float pdfDirect = a;
             bool isRefl;
-            const bool isDirect = pdfDirect == 1.f || sg.next1D() < pdfDirect;
+            const float uD = sg.next1D_if(pdfDirect != 1.f);
+            const bool isDirect = pdfDirect == 1.f || uD < pdfDirect;
             pdf = isDirect ? pdfDirect : 1.f - pdfDirect;
             if (isDirect) {
                 float cT;
                 const float Fd = fresnel_dielectric_F(eta, wi.z, cT);
-                isRefl = Fd > 0.f && sg.next1D() <= Fd;
+                const float uR = sg.next1D_if(Fd > 0.f);
+                isRefl = Fd > 0.f && uR <= Fd;
                 wo = isRefl ? v3(-wi.x, -wi.y, wi.z) : refract_dir(wi, cT, eta);
                 pdf *= isRefl ? Fd : 1.f - Fd;
                 lobe = isRefl ? kLobeDeltaReflection : (m.dispB != 0.f ? kLobeDispersiveDeltaTransmission : kLobeDeltaTransmission);
@@ -661,7 +663,8 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
                 const V3 mh = normalize(wi + wo);
                 float cT;
                 const float F = fresnel_dielectric_F(eta, sgnf(wi.z) * dot(wi, mh), cT);
-                isRefl = F > 0.f && sg.next1D() <= F;
+                const float uR = sg.next1D_if(F > 0.f);
+                isRefl = F > 0.f && uR <= F;
                 if (!isRefl) wo = scattered_wo(wo, eta);
                 pdf *= isRefl ? F : 1.f - F;
                 lobe = isRefl ? kLobeScatterReflection : kLobeScatterTransmission;
@@ -676,7 +679,8 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             const float kk = m.type == PLT_MAT_CONDUCTOR ? k : kMeanK;
             const float a = m.fs.alpha(wi.z, wi.z, kk);
             weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
-            const bool isDirect = a == 1.f || sg.next1D() < a;
+            const float uD = sg.next1D_if(a != 1.f);
+            const bool isDirect = a == 1.f || uD < a;
             pdf = isDirect ? a : 1.f - a;
             if (isDirect) { wo = v3(-wi.x, -wi.y, wi.z); lobe = kLobeDeltaReflection; }
             else { float sp; wo = m.fs.sample(wi, sg, kk, sp); pdf *= sp; lobe = kLobeScatterReflection; }
@@ -698,7 +702,8 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             const float eta1 = coatior / m.extEta;
             float cT;
             const float Fd = fresnel_dielectric_F(eta1, wi.z, cT);
-            const bool isReflCoat = Fd == .1f || sg.next1D() < Fd;
+            const float uC = sg.next1D_if(Fd != .1f);
+            const bool isReflCoat = Fd == .1f || uC < Fd;
             weight = 1.f;
             pdf = isReflCoat ? Fd : 1.f - Fd;
             if (isReflCoat) {
@@ -711,7 +716,8 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             if (wiIn.z == 0.f) break;
             weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
             const float a = m.fs.alpha(wiIn.z, wiIn.z, k);
-            const bool isDirect = a == 1.f || sg.next1D() < a;
+            const float uD = sg.next1D_if(a != 1.f);
+            const bool isDirect = a == 1.f || uD < a;
             pdf *= isDirect ? a : 1.f - a;
             if (isDirect) {
                 wo = v3(-wi.x, -wi.y, wi.z);
@@ -731,13 +737,15 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
         case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:115-153 (no hint)
             const float lam = 1e-3f * lambda;
             const float a = m.gr.alpha(wi.z, k2Pi / lam);
-            const bool isDirect = a == 1.f || sg.next1D() < a;
+            const float uD = sg.next1D_if(a != 1.f);
+            const bool isDirect = a == 1.f || uD < a;
             pdf = isDirect ? a : 1.f - a;
             if (isDirect) {
                 const float ab = m.fs.alpha(wi.z, wi.z, k);
                 weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
                 float bp = 1.f;
-                const bool bd = ab == 1.f || sg.next1D() < ab;
+                const float uB = sg.next1D_if(ab != 1.f);
+                const bool bd = ab == 1.f || uB < ab;
                 bp = bd ? ab : 1.f - ab;
                 if (bd) { wo = v3(-wi.x, -wi.y, wi.z); lobe = kLobeDeltaReflection; }
                 else { float sp; wo = m.fs.sample(wi, sg, k, sp); bp *= sp; lobe = kLobeScatterReflection; }

@@ -4,10 +4,11 @@ two-pass path tracer against the oracle's render of the same scene, frame and pi
 GPU parity bar:
   * primary (bounce 0) records: hit triangle ids identical for every pixel, barycentrics
     bit-identical (camera rays and traversal are exact-IEEE on both sides);
-  * whole paths: every record field identical for >= 97 % of the paths (the remainder diverge after
-    a sampled direction differs in its last bits: GPU transcendental functions are not correctly
-    rounded, so a later ray can graze a different triangle);
-  * colour: for paths whose records are identical, |GPU - oracle| <= 2e-3 * max(1, |oracle|) per
+  * whole paths: hit triangles, sampled lobes and flags identical at every bounce for >= 99 % of
+    the paths (the remainder diverge after a sampled direction differs in its last bits: GPU
+    transcendental functions are not correctly rounded, so a later ray can graze a different
+    triangle); on those paths the float record fields agree to 1e-3 for >= 99.9 %;
+  * colour: on topologically identical paths |GPU - oracle| <= 2e-3 * max(1, |oracle|) per
     channel for >= 99.5 % of them (float32 operation-order differences through up to seven
     BSDF evaluations), and the image mean over all pixels within 2 %.
 """
@@ -250,15 +251,28 @@ def _path_len(rec):
     return first_end + 1
 
 
-def _records_equal(g, o):
+def _topology_equal(g, o):
+    """Paths whose integer record fields (hit triangle, sampled lobe, flags) agree at every bounce."""
     n = g.shape[2]
     L = _path_len(o)
     eq = np.ones(n, bool)
     for p in range(g.shape[0]):
         m = p < L
-        eq &= ~m | np.all(g[p] == o[p], axis=0)
-    eq &= _path_len(g) == L
-    return eq
+        for f in (0, 5, 6):
+            eq &= ~m | (g[p, f] == o[p, f])
+    return eq & (_path_len(g) == L)
+
+
+def _float_fields_close(g, o, topo, rel=1e-3):
+    """Per path: barycentrics, octahedral wi, contribution and last pdf within `rel` (of max(1, |x|))."""
+    L = _path_len(o)
+    ok = np.ones(g.shape[2], bool)
+    for p in range(g.shape[0]):
+        m = (p < L) & topo
+        for f in (1, 2, 3, 4, 7, 8):
+            gv, ov = g[p, f].view(np.float32).astype(np.float64), o[p, f].view(np.float32).astype(np.float64)
+            ok &= ~m | (np.abs(gv - ov) <= rel * np.maximum(1.0, np.abs(ov)))
+    return ok
 
 
 BASE_OPTS = {"maxBounces": 6, "doMNEE": 0, "HWSS": 4}
@@ -284,23 +298,24 @@ def test_gpu_pltpt_matches_oracle(gpu, case):
     opts = dict(BASE_OPTS, **case["opts"])
     w, h = 64, 48
     g_col, g_rec, o_col, o_rec, _ = _render_pair(s, a, w, h, frame=1, opts=opts)
-    n = w * h
     # primary hits: exact
     assert np.array_equal(g_rec[0, 0], o_rec[0, 0])
     hit0 = o_rec[0, 0] != 0xFFFFFFFF
     assert np.array_equal(g_rec[0, 1:3, hit0], o_rec[0, 1:3, hit0])
-    eq = _records_equal(g_rec, o_rec)
-    print(f"[{case['name']}] identical paths {eq.mean():.5f}")
-    assert eq.mean() >= 0.97
+    topo = _topology_equal(g_rec, o_rec)
+    print(f"[{case['name']}] topologically identical paths {topo.mean():.5f}")
+    assert topo.mean() >= 0.99
+    fclose = _float_fields_close(g_rec, o_rec, topo)
+    print(f"[{case['name']}] float record fields close {fclose[topo].mean():.5f}")
+    assert fclose[topo].mean() >= 0.999
     tol = 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3]))
     close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= tol, axis=1)
-    print(f"[{case['name']}] colour within tol on identical paths {close[eq].mean():.5f}")
-    assert close[eq].mean() >= 0.995
+    print(f"[{case['name']}] colour within tol on identical paths {close[topo].mean():.5f}")
+    assert close[topo].mean() >= 0.995
     assert np.isfinite(g_col).all() and (g_col[:, 3] == 1).all()
     gm, om = g_col[:, :3].mean(0), o_col[:, :3].mean(0)
     print(f"[{case['name']}] mean gpu {gm} oracle {om}")
     assert np.all(np.abs(gm - om) <= 0.02 * np.maximum(om, 1e-3) + 1e-4)
-    del n
 
 
 @pytest.mark.gpu
@@ -325,8 +340,10 @@ def test_gpu_pltpt_frames_accumulate_and_options(gpu):
         p.execute(w, h, color, albedo, normal)
         frames.append(color.cpu().numpy().reshape(-1, 4).copy())
     for f in range(3):
-        o = O.render(a, w, h, frame=f, max_bounces=6)
-        assert abs(frames[f][:, :3].mean() - o[:, :3].mean()) <= 0.02 * o[:, :3].mean()
+        o = O.render(a, w, h, frame=f, max_bounces=6)[:, :3]
+        close = np.all(np.abs(frames[f][:, :3] - o) <= 2e-3 * np.maximum(1.0, np.abs(o)), axis=1)
+        assert close.mean() >= 0.99, (f, close.mean())
+    assert not np.array_equal(frames[0], frames[1])
     p.set_scene(s)  # resets the frame counter (PLTPT.cpp:510)
     p.execute(w, h, color, albedo, normal)
     assert np.array_equal(color.cpu().numpy().reshape(-1, 4), frames[0])
