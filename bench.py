@@ -1,12 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark of the PLT hot path on one MI355X.
+"""Benchmark of the PLT path-tracing hot path on one MI355X. Prints one JSON line.
 
-Workload (one "step"): one eval + pdf pass of the HIP BSDF kernel over a resident batch of
-2^20 synthetic PLT BSDF queries (seed 0; 50/50 rough dielectric / rough Drude conductor,
-K-correlation / Gaussian PSD roughness, coherence-aware beams, FP32). Prints one JSON line.
+Default workload (one "step"): the PLTPT pass renders 4 frames (4 spp) of a 1920x1080 image at
+max depth 8 of the procedural scene with 256 subdivision-5 icospheres (5,242,884 triangles) on a
+20 x 20 ground quad under one 3 x 3 quad area light (6500 K blackbody emission profile), spectral
+MIS over 4 hero wavelengths, NEE + MIS, Russian roulette. Scene, BVH, materials and bounce records
+are resident in HBM before timing starts. `value` is path samples per second (pixels x spp / s).
 
-The CPU baseline is the oracle restatement (oracle/liboracle.so, OpenMP over the host cores)
-timed on a bounded sample of the same batch.
+The roofline figure is for the kernel that takes the most time (sample or solve), from HIP
+events recorded around each kernel launch on the pass's stream during the timed region, with
+algorithmic bytes counted from the bounce records of the timed frames:
+  sample kernel: 84 B per traced path segment (ray 32 + hit 16 + 36 B bounce record written)
+                 + 32 B per pixel (albedo and normal written);
+  solve kernel:  36 B per path segment (its bounce record read once) + 16 B per pixel (colour).
+Shadow rays, BVH nodes and triangles are not algorithmic bytes; rocprofv3's FETCH_SIZE /
+WRITE_SIZE give the actual traffic (profiles/).
+
+`secondary.bsdf_eval_batch` is the 2^20-query BSDF eval + pdf batch (BSDF evals/s), the second
+part of the metric. The CPU baseline is the oracle restatement (C++, OpenMP over the host cores)
+rendering a bounded sample of the same frame.
+
+`--workload bsdf` makes the BSDF batch the primary line instead.
 """
 from __future__ import annotations
 
@@ -20,34 +34,83 @@ from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 METRIC = "path samples/s @1080p depth 8; BSDF evals/s; surrogate vs exact at ≤1% rel err"
-BYTES_PER_QUERY = 84  # 16 planes x 4 B in + 5 planes x 4 B out
 HBM_PEAK_GBS = 8000.0
+BSDF_BYTES_PER_QUERY = 84  # 16 planes x 4 B in + 5 planes x 4 B out
 N_QUERIES = 1 << 20
+WIDTH, HEIGHT, SPP, MAX_BOUNCES = 1920, 1080, 4, 8
+SAMPLE_BYTES_PER_SEGMENT, SAMPLE_BYTES_PER_PIXEL = 84, 32
+SOLVE_BYTES_PER_SEGMENT, SOLVE_BYTES_PER_PIXEL = 36, 16
 
 
 def parse_args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("path", "bsdf"), default="path")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the BSDF-batch secondary measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed region, run 5 plain device copies of MIB MiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes)")
     return ap.parse_args()
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the eval kernel from the committed rocprofv3 PMC summary, or None."""
-    p = ROOT / "profiles" / "bsdf_eval_2p20_pmc.md"
+def traffic_from_profile(name: str):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 PMC summary, or None."""
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     m = re.search(r"hbm_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(queries_np, budget_s: float):
+def copy_calibration(mib: int):
+    import torch
+    nel = mib * (1 << 20) // 4
+    src = torch.ones(nel, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return nel * 4
+
+
+# ------------------------------------------------------------------------------------ BSDF batch
+
+
+def bsdf_batch(steps: int, warmup: int):
+    import numpy as np
+    import torch
+    from pltfalcor_amd.bsdf import BSDFBatch, default_context
+    batch = BSDFBatch(default_context(0), N_QUERIES).generate(seed=0)
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        batch.eval(stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(stream)
+        batch.eval(stream)
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    achieved = BSDF_BYTES_PER_QUERY * N_QUERIES / (kern_ms * 1e-3) / 1e9
+    return batch, {
+        "value": N_QUERIES * steps / wall, "unit": "BSDF evals/s", "ms_per_step": wall * 1e3 / steps,
+        "workload": "PLT BSDF eval+pdf batch, 2^20 queries from seed 0, FP32 (50/50 rough dielectric IOR "
+                    "U[1.3,1.8] / rough Drude conductor 64-bin n,k; q U[0.01,0.3] um, l U[1,20] um, coherence "
+                    "area U[1,100] um^2, cosine wi/wo, lambda U[380,780] nm, random Stokes)",
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile("bsdf_eval_2p20_pmc.md"),
+                     "kernel": "eval_queries_kernel", "kernel_ms": kern_ms,
+                     "algorithmic_bytes_per_launch": BSDF_BYTES_PER_QUERY * N_QUERIES}}
+
+
+def bsdf_cpu_baseline(queries_np, budget_s: float):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the oracle is the CPU baseline (test infrastructure)
     from pltfalcor_amd.spectra import drude_profiles
@@ -65,50 +128,133 @@ def cpu_baseline(queries_np, budget_s: float):
         best = min(best, dt)
         total += dt
         passes += 1
-        if passes >= 20000:
-            break
     return {"value": n / best, "unit": "BSDF evals/s", "cores": threads, "kind": "port",
             "sample": f"the same {n}-query batch, FP32 oracle, best of {passes} passes "
                       f"({total:.1f} s wall on {threads} threads)"}
+
+
+# ------------------------------------------------------------------------------------ path tracing
+
+
+def segment_count(pltpt, frames):
+    """Traced path segments (bounce records written) per frame, read back from the pass."""
+    import numpy as np
+    counts = []
+    for f in frames:
+        pltpt.set_frame(f)
+        pltpt.execute_frame()
+        rec = pltpt.bounce_records(WIDTH, HEIGHT)
+        flags = rec[:, 6, :]
+        end = (flags & 0x30) != 0
+        counts.append(int((np.argmax(end, axis=0) + 1).sum()))
+        del rec, flags, end
+    return counts
+
+
+def path_cpu_baseline(scene_np, budget_s: float, opts):
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O  # the oracle is the CPU baseline (test infrastructure)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    osc = O.OracleScene(scene_np)
+    kw = dict(max_bounces=MAX_BOUNCES, threads=threads, do_rr=opts["doRussianRoulette"])
+    # chunks of 64 rows in frame order, frames 0..SPP-1 in turn, until the budget is spent
+    osc.render(WIDTH, HEIGHT, frame=0, first=WIDTH * (HEIGHT // 2), count=WIDTH, **kw)  # warm-up
+    chunk = 64 * WIDTH
+    total, paths, frame, first = 0.0, 0, 0, 0
+    while total < budget_s:
+        n = min(chunk, WIDTH * HEIGHT - first)
+        t0 = time.perf_counter()
+        osc.render(WIDTH, HEIGHT, frame=frame, first=first, count=n, **kw)
+        total += time.perf_counter() - t0
+        paths += n
+        first += n
+        if first >= WIDTH * HEIGHT:
+            first, frame = 0, (frame + 1) % SPP
+    osc.close()
+    return {"value": paths / total, "unit": "path samples/s", "cores": threads, "kind": "port",
+            "sample": f"{paths} paths of the same workload rendered in 64-row chunks in frame order from frame 0 "
+                      f"({paths / (WIDTH * HEIGHT):.2f} frames, {total:.1f} s on {threads} threads; FP32 oracle, "
+                      "BVH prebuilt)"}
 
 
 def main():
     args = parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N>1 is not supported "
-                          "(the path shards by pixels/queries, see DESIGN.md)", "n_gpus": args.gpus}))
+                          "(paths and queries shard by pixel/query range across GPUs; see DESIGN.md)",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
-    from pltfalcor_amd.bsdf import BSDFBatch, default_context
-
     torch.cuda.set_device(0)
-    ctx = default_context(0)
-    batch = BSDFBatch(ctx, N_QUERIES).generate(seed=0)
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        batch.eval(stream)
-    torch.cuda.synchronize()
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    if args.workload == "bsdf":
+        batch, r = bsdf_batch(args.steps, args.warmup)
+        out = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": r["workload"], "queries": N_QUERIES}, "roofline": r["roofline"],
+               "cpu_baseline": None}
+        if args.copy_calibration > 0:
+            out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = bsdf_cpu_baseline(batch.queries.cpu().numpy(), args.cpu_seconds)
+        print(json.dumps(out), flush=True)
+        return
+
+    from pltfalcor_amd.bsdf import default_context
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import PATH_2, Scene
+
+    ctx = default_context(0)
+    t_build = time.perf_counter()
+    scene = Scene(ctx, aspect=WIDTH / HEIGHT, **PATH_2)
+    t_build = time.perf_counter() - t_build
+    opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4, "doMNEE": 0}
+    pltpt = PLTPT(ctx, opts)
+    pltpt.set_scene(scene)
+    dev = torch.device("cuda")
+    color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device=dev)
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    pltpt.execute_frame = lambda: pltpt.execute(WIDTH, HEIGHT, color, albedo, normal)  # noqa: E731
+
+    def step():
+        pltpt.set_frame(0)  # one step = frames 0..SPP-1 (SPP samples per pixel)
+        for _ in range(SPP):
+            pltpt.execute_frame()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    pltpt.set_profiling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s, e in ev:
-        s.record(stream)
-        batch.eval(stream)
-        e.record(stream)
+    for _ in range(args.steps):
+        step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in ev]))
+    sample_ms, solve_ms, executes = pltpt.kernel_times()
+    pltpt.set_profiling(False)
 
-    value = N_QUERIES * args.steps / wall
-    achieved = BYTES_PER_QUERY * N_QUERIES / (kern_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile()
+    segs = segment_count(pltpt, range(SPP))  # outside the timed region
+    pixels = WIDTH * HEIGHT
+    seg_per_exec = sum(segs) / SPP
+    sample_avg, solve_avg = sample_ms / executes, solve_ms / executes
+    if solve_avg >= sample_avg:
+        kname, kms = "solve_kernel<4>", solve_avg
+        kbytes = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
+        traffic = traffic_from_profile("pltpt_solve_1080p_d8_pmc.md")
+    else:
+        kname, kms = "sample_kernel", sample_avg
+        kbytes = SAMPLE_BYTES_PER_SEGMENT * seg_per_exec + SAMPLE_BYTES_PER_PIXEL * pixels
+        traffic = traffic_from_profile("pltpt_sample_1080p_d8_pmc.md")
+    achieved = kbytes / (kms * 1e-3) / 1e9
+
+    bsdf = None if args.no_secondary else bsdf_batch(500, 20)[1]
     out = {
         "metric": METRIC,
-        "value": value,
-        "unit": "BSDF evals/s",
+        "value": pixels * SPP * args.steps / wall,
+        "unit": "path samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -118,27 +264,26 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "PLT BSDF eval+pdf batch, 2^20 queries from seed 0, FP32 "
-                               "(50/50 rough dielectric IOR U[1.3,1.8] / rough Drude conductor 64-bin n,k; "
-                               "q U[0.01,0.3] um, l U[1,20] um, coherence area U[1,100] um^2, "
-                               "cosine wi/wo, lambda U[380,780] nm, random Stokes)",
-                   "queries": N_QUERIES, "kernel": "eval_queries_kernel"},
+        "config": {"workload": f"PLTPT {WIDTH}x{HEIGHT}, {SPP} spp (frames 0-{SPP - 1}) per step, max depth "
+                               f"{MAX_BOUNCES}, procedural scene seed 1: 20x20 ground + 256 icospheres subdiv 5 "
+                               "(5,242,884 triangles), one 3x3 quad light (6500 K blackbody), materials uniform "
+                               "over dielectric / Drude conductor / coated opaque dielectric / grated conductor; "
+                               "HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE off",
+                   "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
+                   "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
+                   "segments_per_frame": seg_per_exec},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": kern_ms, "algorithmic_bytes_per_launch": BYTES_PER_QUERY * N_QUERIES},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "kernel_ms": kms,
+                     "algorithmic_bytes_per_launch": kbytes,
+                     "sample_kernel_ms": sample_avg, "solve_kernel_ms": solve_avg},
         "cpu_baseline": None,
+        "secondary": {"bsdf_eval_batch": {k: bsdf[k] for k in ("value", "unit", "roofline")}} if bsdf else None,
     }
     if args.copy_calibration > 0:
-        nel = args.copy_calibration * (1 << 20) // 4
-        src = torch.ones(nel, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = nel * 4
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu_baseline:
-        q = batch.queries.cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(q, args.cpu_seconds)
+        out["cpu_baseline"] = path_cpu_baseline(scene.numpy(), args.cpu_seconds, {"doRussianRoulette": 1})
+    pltpt.close()
     print(json.dumps(out), flush=True)
 
 

@@ -270,6 +270,15 @@ int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream str
  * Returns PLT_EINVAL if `bytes` is smaller than the record buffer. */
 int plt_pass_copy_bounce_records(plt_pass* pass, void* dst, size_t bytes);
 
+/* Kernel timing, the counterpart of the reference's profiler scopes around its sample and solve
+ * dispatches (FALCOR_PROFILE in Falcor passes; PLTPT.cpp:487-495 issues the two dispatches):
+ * while enabled, every execute records HIP events on its stream around the sample and solve
+ * kernels. plt_pass_get_kernel_times waits for them and returns the summed milliseconds of each
+ * kernel and the number of executes since the previous call (or since enabling), then restarts
+ * the sums. Enabling or disabling restarts them too. */
+int plt_pass_set_profiling(plt_pass* pass, int enable);
+int plt_pass_get_kernel_times(plt_pass* pass, double* sample_ms, double* solve_ms, uint32_t* executes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -455,17 +455,25 @@ extern "C" {
 // useDirectLights, useEmissiveLights (11 ints); fopts: sourcingArea, maxOmega.
 // Renders pixels [first, first + count) of the row-major frame; color: 4 floats per rendered pixel;
 // records (optional): (maxBounces+1) x 9 planes of `count` words, as the product's bounce records.
-int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh,
-                  const uint32_t* meshMat, const void* mats, const void* prof, uint32_t nprof, const float* cam,
-                  const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count, float* color, uint32_t* records,
-                  int threads) {
-    OScene s;
-    s.pos = pos; s.nrm = nrm; s.idx = idx; s.triMesh = triMesh; s.meshMat = meshMat;
-    s.mats = static_cast<const MatData*>(mats);
-    s.prof = static_cast<const SpectralProfile*>(prof);
-    s.nprof = nprof; s.ntri = ntri;
-    s.mesh.pos = pos; s.mesh.idx = idx; s.mesh.ntri = ntri;
-    s.bvh.build(s.mesh);
+void* oracle_scene_create(const float* pos, const float* nrm, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh,
+                          const uint32_t* meshMat, const void* mats, const void* prof, uint32_t nprof) {
+    OScene* s = new OScene();
+    s->pos = pos; s->nrm = nrm; s->idx = idx; s->triMesh = triMesh; s->meshMat = meshMat;
+    s->mats = static_cast<const MatData*>(mats);
+    s->prof = static_cast<const SpectralProfile*>(prof);
+    s->nprof = nprof; s->ntri = ntri;
+    s->mesh.pos = pos; s->mesh.idx = idx; s->mesh.ntri = ntri;
+    s->bvh.build(s->mesh);
+    buildLights(*s);
+    return s;
+}
+
+void oracle_scene_destroy(void* h) { delete static_cast<OScene*>(h); }
+
+// Renders with a scene made by oracle_scene_create (whose arrays the caller keeps alive).
+int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count,
+                        float* color, uint32_t* records, int threads) {
+    OScene& s = *static_cast<OScene*>(h);
     // Camera.cpp:187-193 in float
     {
         const V3<F> P{cam[0], cam[1], cam[2]}, Tg{cam[3], cam[4], cam[5]}, Up{cam[6], cam[7], cam[8]};
@@ -479,7 +487,6 @@ int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint3
         s.camPos = P; s.camU = U; s.camV = V; s.camW = W;
         s.jitterX = cam[12]; s.jitterY = cam[13];
     }
-    buildLights(s);
     Options o;
     o.width = uint32_t(iopt[0]); o.height = uint32_t(iopt[1]); o.frame = uint32_t(iopt[2]); o.maxBounces = uint32_t(iopt[3]);
     o.doNEE = iopt[4]; o.doMIS = iopt[5]; o.doRR = iopt[6]; o.hwssMIS = iopt[7]; o.isEmitters = iopt[8];
@@ -511,6 +518,17 @@ int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint3
         }
     }
     return 0;
+}
+
+// One-shot render: builds the scene, renders, frees it.
+int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh,
+                  const uint32_t* meshMat, const void* mats, const void* prof, uint32_t nprof, const float* cam,
+                  const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count, float* color, uint32_t* records,
+                  int threads) {
+    void* h = oracle_scene_create(pos, nrm, ntri, idx, triMesh, meshMat, mats, prof, nprof);
+    const int rc = oracle_scene_render(h, cam, iopt, fopt, first, count, color, records, threads);
+    oracle_scene_destroy(h);
+    return rc;
 }
 
 // The power sampler's alias table, decoded: per entry the binary16 threshold (as float), the

@@ -90,6 +90,9 @@ SIGNATURES = [
     ("plt_pass_set_frame", C.c_int, [C.c_void_p, C.c_uint32]),
     ("plt_pass_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     ("plt_pass_copy_bounce_records", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("plt_pass_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("plt_pass_get_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_uint32)]),
 ]
 
 _lib = None

@@ -146,6 +146,10 @@ struct plt_pass {
     size_t records_words = 0;
     size_t last_words = 0;  // record words written by the last execute
     float cam[12] = {};
+    // kernel timing: three events per execute (before sample, between, after solve)
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    size_t events_used = 0;
 };
 
 namespace plt {
@@ -199,6 +203,7 @@ int plt_pass_destroy(plt_pass* p) {
     if (!p) return PLT_OK;
     plt::free_scene_state(p);
     if (p->d_records) (void)hipFree(p->d_records);
+    for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     delete p;
     return PLT_OK;
 }
@@ -328,10 +333,23 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     fr.albedo = reinterpret_cast<float4*>(rd->albedo);
     fr.normal = reinterpret_cast<float4*>(rd->normal);
     const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (p->profiling) {
+        while (p->events.size() < p->events_used + 3) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_execute: hipEventCreate failed");
+            p->events.push_back(e);
+        }
+        for (int i = 0; i < 3; ++i) ev[i] = p->events[p->events_used + i];
+        p->events_used += 3;
+        (void)hipEventRecord(ev[0], st);
+    }
     int rc = plt::launch_sample_pass(sc, fr, st);
     if (rc < 0) return rc;
+    if (ev[1]) (void)hipEventRecord(ev[1], st);
     rc = plt::launch_solve_pass(sc, fr, st);
     if (rc < 0) return rc;
+    if (ev[2]) (void)hipEventRecord(ev[2], st);
     ++p->frame;  // PLTPT.cpp:498
     p->last_words = words;
     return PLT_OK;
@@ -343,6 +361,33 @@ int plt_pass_copy_bounce_records(plt_pass* p, void* dst, size_t bytes) {
     if (bytes < p->last_words * 4) return set_error(PLT_EINVAL, "plt_pass_copy_bounce_records: destination too small");
     if (hipMemcpy(dst, p->d_records, p->last_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
         return set_error(PLT_EHIP, "plt_pass_copy_bounce_records: hipMemcpy failed");
+    return PLT_OK;
+}
+
+int plt_pass_set_profiling(plt_pass* p, int enable) {
+    if (!p) return set_error(PLT_EINVAL, "plt_pass_set_profiling: bad arguments");
+    p->profiling = enable != 0;
+    p->events_used = 0;
+    return PLT_OK;
+}
+
+int plt_pass_get_kernel_times(plt_pass* p, double* sample_ms, double* solve_ms, uint32_t* executes) {
+    if (!p || !sample_ms || !solve_ms || !executes) return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
+    double a = 0.0, b = 0.0;
+    const size_t n = p->events_used / 3;
+    for (size_t i = 0; i < n; ++i) {
+        hipEvent_t* e = &p->events[3 * i];
+        if (hipEventSynchronize(e[2]) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventSynchronize failed");
+        float t0 = 0.f, t1 = 0.f;
+        if (hipEventElapsedTime(&t0, e[0], e[1]) != hipSuccess || hipEventElapsedTime(&t1, e[1], e[2]) != hipSuccess)
+            return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventElapsedTime failed");
+        a += t0;
+        b += t1;
+    }
+    *sample_ms = a;
+    *solve_ms = b;
+    *executes = uint32_t(n);
+    p->events_used = 0;
     return PLT_OK;
 }
 

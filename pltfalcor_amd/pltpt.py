@@ -96,6 +96,15 @@ class PLTPT:
         _lib.check(self.lib.plt_pass_copy_bounce_records(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    def set_profiling(self, enable: bool):
+        _lib.check(self.lib.plt_pass_set_profiling(self.handle, int(bool(enable))))
+
+    def kernel_times(self):
+        """(sample_ms, solve_ms, executes) summed since the previous call; waits for the recorded events."""
+        a, b, n = C.c_double(), C.c_double(), C.c_uint32()
+        _lib.check(self.lib.plt_pass_get_kernel_times(self.handle, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.plt_pass_destroy(self.handle)

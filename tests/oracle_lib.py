@@ -141,36 +141,71 @@ def compute_ray_origin(pos, n):
 REC_WORDS = 9
 
 
-def render(scene: dict, width: int, height: int, frame: int = 0, max_bounces: int = 6, first: int = 0, count=None,
-           do_nee=1, do_mis=1, do_rr=1, hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0,
-           max_omega=0.025, threads=0, records=False):
-    """Oracle render of pixels [first, first+count) (row-major). Returns color [count, 4] and optionally
-    the bounce records [(max_bounces+1), 9, count] (uint32)."""
-    l = lib()
-    fn = l.oracle_render
-    fn.restype = C.c_int
-    fn.argtypes = [_F, _F, C.c_uint32, _U32, _U32, _U32, C.c_char_p, C.c_char_p, C.c_uint32, _F,
-                   C.POINTER(C.c_int32), _F, C.c_uint32, C.c_uint32, _F, _U32, C.c_int]
-    if count is None:
-        count = width * height - first
-    pos = np.ascontiguousarray(scene["positions"], np.float32)
-    nrm = np.ascontiguousarray(scene["normals"], np.float32)
-    idx = np.ascontiguousarray(scene["indices"], np.uint32)
-    tm = np.ascontiguousarray(scene["tri_mesh"], np.uint32)
-    mm = np.ascontiguousarray(scene["mesh_material"], np.uint32)
-    cam = np.ascontiguousarray(scene["camera"], np.float32)
-    iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct, emissive],
-                    np.int32)
-    fopt = np.array([sourcing_area, max_omega], np.float32)
-    color = np.zeros((count, 4), np.float32)
-    rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
-    fn(ptr(pos, C.c_float), ptr(nrm, C.c_float), idx.shape[0], ptr(idx, C.c_uint32), ptr(tm, C.c_uint32),
-       ptr(mm, C.c_uint32), scene["materials"], scene["profiles"], scene["n_profiles"], ptr(cam, C.c_float),
-       iopt.ctypes.data_as(C.POINTER(C.c_int32)), ptr(fopt, C.c_float), first, count, ptr(color, C.c_float),
-       ptr(rec, C.c_uint32) if records else None, threads)
-    if records:
-        return color, rec.reshape(max_bounces + 1, REC_WORDS, count)
-    return color
+class OracleScene:
+    """A scene built once in the oracle (BVH and light tables); keeps its arrays alive."""
+
+    def __init__(self, scene: dict):
+        l = lib()
+        l.oracle_scene_create.restype = C.c_void_p
+        l.oracle_scene_create.argtypes = [_F, _F, C.c_uint32, _U32, _U32, _U32, C.c_char_p, C.c_char_p, C.c_uint32]
+        l.oracle_scene_destroy.restype = None
+        l.oracle_scene_destroy.argtypes = [C.c_void_p]
+        l.oracle_scene_render.restype = C.c_int
+        l.oracle_scene_render.argtypes = [C.c_void_p, _F, C.POINTER(C.c_int32), _F, C.c_uint32, C.c_uint32, _F, _U32,
+                                          C.c_int]
+        self.lib = l
+        self.arrays = dict(
+            pos=np.ascontiguousarray(scene["positions"], np.float32),
+            nrm=np.ascontiguousarray(scene["normals"], np.float32),
+            idx=np.ascontiguousarray(scene["indices"], np.uint32),
+            tm=np.ascontiguousarray(scene["tri_mesh"], np.uint32),
+            mm=np.ascontiguousarray(scene["mesh_material"], np.uint32),
+            mats=scene["materials"], prof=scene["profiles"])
+        a = self.arrays
+        self.camera = np.ascontiguousarray(scene["camera"], np.float32)
+        self.handle = l.oracle_scene_create(ptr(a["pos"], C.c_float), ptr(a["nrm"], C.c_float), a["idx"].shape[0],
+                                            ptr(a["idx"], C.c_uint32), ptr(a["tm"], C.c_uint32), ptr(a["mm"], C.c_uint32),
+                                            a["mats"], a["prof"], scene["n_profiles"])
+
+    def render(self, width, height, frame=0, max_bounces=6, first=0, count=None, do_nee=1, do_mis=1, do_rr=1,
+               hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0, max_omega=0.025, threads=0,
+               records=False, camera=None):
+        """Renders pixels [first, first+count) (row-major). Returns color [count, 4] and optionally the
+        bounce records [(max_bounces+1), 9, count] (uint32)."""
+        if count is None:
+            count = width * height - first
+        cam = self.camera if camera is None else np.ascontiguousarray(camera, np.float32)
+        iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct,
+                         emissive], np.int32)
+        fopt = np.array([sourcing_area, max_omega], np.float32)
+        color = np.zeros((count, 4), np.float32)
+        rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
+        self.lib.oracle_scene_render(self.handle, ptr(cam, C.c_float), iopt.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     ptr(fopt, C.c_float), first, count, ptr(color, C.c_float),
+                                     ptr(rec, C.c_uint32) if records else None, threads)
+        if records:
+            return color, rec.reshape(max_bounces + 1, REC_WORDS, count)
+        return color
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.oracle_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(scene: dict, width: int, height: int, **kw):
+    """One-shot oracle render (builds the scene's BVH each call); see OracleScene.render."""
+    s = OracleScene(scene)
+    try:
+        return s.render(width, height, **kw)
+    finally:
+        s.close()
 
 
 def alias_table(scene: dict):
