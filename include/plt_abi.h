@@ -197,6 +197,26 @@ typedef struct plt_procedural_scene_desc {
 
 /* ctx == NULL builds a host-only scene (arrays and BVH; plt_scene_trace then returns PLT_ESTATE). */
 int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_desc* desc, plt_scene** out);
+
+/* A scene from caller arrays: what the reference's SceneBuilder hands the renderer for triangle
+ * meshes (Scene/SceneBuilder.cpp: vertex positions and normals, 32-bit indices, per-mesh material
+ * ids, the material table and the spectral profiles built by genSpectralProfile,
+ * SceneBuilder.cpp:911-960) plus the camera. Triangles are in mesh order: tri_mesh[t] names the
+ * mesh of triangle t. Normals are stored as binary16, as Falcor's packed vertex data stores them
+ * (Scene/SceneTypes.slang:219-242). Arrays are copied; the BVH is built here (binned SAH).
+ * Returns PLT_EINVAL for an index, mesh, material or profile id out of range. */
+typedef struct plt_scene_desc {
+    uint32_t n_vertices, n_triangles, n_meshes, n_materials, n_profiles;
+    const float* positions;       /* 3 x n_vertices */
+    const float* normals;         /* 3 x n_vertices */
+    const uint32_t* indices;      /* 3 x n_triangles */
+    const uint32_t* tri_mesh;     /* n_triangles */
+    const uint32_t* mesh_material;/* n_meshes */
+    const plt_material* materials;
+    const plt_spectral_profile* profiles;
+    plt_camera camera;
+} plt_scene_desc;
+int plt_scene_create(plt_context* ctx, const plt_scene_desc* desc, plt_scene** out);
 int plt_scene_destroy(plt_scene* scene);
 
 /* Host view of a scene's arrays (valid until plt_scene_destroy). */

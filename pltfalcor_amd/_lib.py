@@ -64,6 +64,15 @@ class SceneArrays(C.Structure):
                 ("camera", Camera)]
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32), ("n_meshes", C.c_uint32),
+                ("n_materials", C.c_uint32), ("n_profiles", C.c_uint32),
+                ("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("tri_mesh", C.POINTER(C.c_uint32)),
+                ("mesh_material", C.POINTER(C.c_uint32)), ("materials", C.POINTER(Material)),
+                ("profiles", C.POINTER(SpectralProfile)), ("camera", Camera)]
+
+
 # (name, restype, argtypes) for every exported symbol of include/plt_abi.h.
 SIGNATURES = [
     ("plt_last_error", C.c_char_p, []),
@@ -78,6 +87,7 @@ SIGNATURES = [
     ("plt_bsdf_generate_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("plt_bsdf_eval_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),
+    ("plt_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     ("plt_scene_destroy", C.c_int, [C.c_void_p]),
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
     ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

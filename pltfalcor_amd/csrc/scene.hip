@@ -117,6 +117,56 @@ int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_des
     return PLT_OK;
 }
 
+int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out) {
+    if (!d || !out) return set_error(PLT_EINVAL, "plt_scene_create: bad arguments");
+    if (d->n_vertices == 0 || d->n_triangles == 0 || d->n_meshes == 0 || d->n_materials == 0 || !d->positions ||
+        !d->normals || !d->indices || !d->tri_mesh || !d->mesh_material || !d->materials || (d->n_profiles && !d->profiles))
+        return set_error(PLT_EINVAL, "plt_scene_create: empty scene or missing array");
+    for (size_t i = 0; i < size_t(d->n_triangles) * 3; ++i)
+        if (d->indices[i] >= d->n_vertices) return set_error(PLT_EINVAL, "plt_scene_create: vertex index out of range");
+    for (uint32_t t = 0; t < d->n_triangles; ++t)
+        if (d->tri_mesh[t] >= d->n_meshes) return set_error(PLT_EINVAL, "plt_scene_create: mesh id out of range");
+    for (uint32_t m = 0; m < d->n_meshes; ++m)
+        if (d->mesh_material[m] >= d->n_materials) return set_error(PLT_EINVAL, "plt_scene_create: material id out of range");
+    for (uint32_t m = 0; m < d->n_materials; ++m) {
+        const plt_material& mt = d->materials[m];
+        if (mt.type > PLT_MAT_GRATED_CONDUCTOR) return set_error(PLT_EINVAL, "plt_scene_create: unknown material type");
+        for (uint32_t id : {mt.emission_profile, mt.ior_n_profile, mt.ior_k_profile})
+            if (id != PLT_NO_PROFILE && id >= d->n_profiles) return set_error(PLT_EINVAL, "plt_scene_create: profile id out of range");
+    }
+    for (uint32_t p = 0; p < d->n_profiles; ++p)
+        if (d->profiles[p].bins < 1 || d->profiles[p].bins > PLT_SPECTRAL_PROFILE_MAX_BINS)
+            return set_error(PLT_EINVAL, "plt_scene_create: profile bin count out of range");
+    plt_scene* s = new (std::nothrow) plt_scene();
+    if (!s) return set_error(PLT_ENOMEM, "plt_scene_create: out of host memory");
+    s->ctx = ctx;
+    plt::HostScene& h = s->host;
+    h.positions.assign(d->positions, d->positions + 3 * size_t(d->n_vertices));
+    h.normals.resize(3 * size_t(d->n_vertices));
+    for (size_t i = 0; i < h.normals.size(); ++i) h.normals[i] = plt::h_to_f32(plt::f32_to_h(d->normals[i]));
+    h.indices.assign(d->indices, d->indices + 3 * size_t(d->n_triangles));
+    h.tri_mesh.assign(d->tri_mesh, d->tri_mesh + d->n_triangles);
+    h.mesh_first_tri.assign(d->n_meshes, 0xFFFFFFFFu);
+    for (uint32_t t = d->n_triangles; t-- > 0;) h.mesh_first_tri[d->tri_mesh[t]] = t;
+    h.mesh_material.assign(d->mesh_material, d->mesh_material + d->n_meshes);
+    h.materials.assign(d->materials, d->materials + d->n_materials);
+    if (d->n_profiles) h.profiles.assign(d->profiles, d->profiles + d->n_profiles);
+    h.camera = d->camera;
+    h.bvh = plt::build_bvh(h.positions, h.indices, 4);
+    if (!ctx) {
+        *out = s;
+        return PLT_OK;
+    }
+    const int rc = plt::upload_scene(s->host, s->dev);
+    if (rc < 0) {
+        plt::free_scene(s->dev);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return PLT_OK;
+}
+
 int plt_scene_destroy(plt_scene* scene) {
     if (!scene) return PLT_OK;
     plt::free_scene(scene->dev);

