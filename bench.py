@@ -2,8 +2,8 @@
 """Benchmark of the PLT path-tracing hot path on one MI355X. Prints one JSON line.
 
 Default workload (one "step"): the PLTPT pass renders 4 frames (4 spp) of a 1920x1080 image at
-max depth 8 of the procedural scene with 256 subdivision-5 icospheres (5,242,884 triangles) on a
-20 x 20 ground quad under one 3 x 3 quad area light (6500 K blackbody emission profile), spectral
+max depth 8 of the procedural scene with 256 subdivision-5 icospheres (5,242,884 triangles) in 8
+material kinds on a 20 x 20 ground quad under one 3 x 3 quad area light (6500 K blackbody), spectral
 MIS over 4 hero wavelengths, NEE + MIS, Russian roulette. Scene, BVH, materials and bounce records
 are resident in HBM before timing starts. `value` is path samples per second (pixels x spp / s).
 
@@ -266,9 +266,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"PLTPT {WIDTH}x{HEIGHT}, {SPP} spp (frames 0-{SPP - 1}) per step, max depth "
                                f"{MAX_BOUNCES}, procedural scene seed 1: 20x20 ground + 256 icospheres subdiv 5 "
-                               "(5,242,884 triangles), one 3x3 quad light (6500 K blackbody), materials uniform "
-                               "over dielectric / Drude conductor / coated opaque dielectric / grated conductor; "
-                               "HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE off",
+                               "(5,242,884 triangles), one 3x3 quad light (6500 K blackbody), 8 materials uniform "
+                               "over dielectric / Drude conductor / coated opaque dielectric / grated conductor / "
+                               "birefringent thin dielectric / 20-layer stack (FP64 steps) / coated conductor / "
+                               "opaque dielectric; HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE off",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
                    "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
                    "segments_per_frame": seg_per_exec},

@@ -121,7 +121,10 @@ typedef enum plt_material_type {
     PLT_MAT_DIFFUSE = 2,                  /* PLTDiffuseBSDF.slang:39-155 (Lambertian; the emitter material) */
     PLT_MAT_OPAQUE_DIELECTRIC = 3,        /* PLTOpaqueDielectricBSDF.slang:43-212 */
     PLT_MAT_COATED_OPAQUE_DIELECTRIC = 4, /* PLTCoatedOpaqueDielectricBSDF.slang:45-312 */
-    PLT_MAT_GRATED_CONDUCTOR = 5          /* PLTDiffractionGratedConductorBSDF.slang:45-290 */
+    PLT_MAT_GRATED_CONDUCTOR = 5,         /* PLTDiffractionGratedConductorBSDF.slang:45-290 */
+    PLT_MAT_COATED_CONDUCTOR = 6,         /* PLTCoatedConductorBSDF.slang:45-313 */
+    PLT_MAT_THIN_DIELECTRIC = 7,          /* PLTThinDielectricBSDF.slang:43-269 (+ Birefringence.slang) */
+    PLT_MAT_MULTILAYER_STACK = 8          /* PLTMultiLayeredStackBSDF.slang:42-201 */
 } plt_material_type;
 
 /* Synthetic batch: query i draws 14 numbers from UniformSampleGenerator(pixel=(i & 0xFFFF,
@@ -147,6 +150,12 @@ int plt_bsdf_eval_queries(plt_context* ctx, uint32_t n, const uint32_t* queries,
  *                  -, -, coat IoR), ior = ext IoR
  *   grated cond. : base_color, specular = (height scale, power mult), data1 = (pitch x, pitch y,
  *                  direction, type|lobes bits), data2 = (height, roughness), ior, n/k profiles
+ *   coated cond. : base_color = specular colour, specular = (-, roughness, thickness scale, -),
+ *                  data1 = (thickness um, -, -, coat IoR), ior = ext IoR, n/k profiles
+ *   thin diel.   : base_color = (birefringence, optic axis x, optic axis y, alpha),
+ *                  specular = (birefringence scale, -, -, thickness mm), ior = mean IoR
+ *   multilayer   : base_color = base albedo, data1 = (thickness 1 um, thickness 2 um, -, layer
+ *                  count as a raw uint16), data2 = (IoR 1, IoR 2), ior = ext IoR
  * ------------------------------------------------------------------------------------- */
 #define PLT_NO_PROFILE 0xFFFFFFFFu
 typedef struct plt_material {
@@ -161,6 +170,28 @@ typedef struct plt_material {
     uint32_t ior_n_profile;
     uint32_t ior_k_profile;
 } plt_material;
+
+/* Batched material interface: the reference's IMaterialInstance eval / pdf / lambdaPdf and
+ * sample (Rendering/Materials/IMaterialInstance.slang:122-184) for `n` queries of any material of
+ * the device array `materials`, as SoA float planes (`n` floats each; integer fields are stored as
+ * their bit patterns). Queries live in a local frame with N = +z. eval: wi = sd.V (towards the
+ * light), wo = toEye, the beam arrives along -wi with 4 HWSS wavelengths and Stokes vectors and
+ * the given coherence area (um^2); results are the 4 outgoing Stokes vectors, pdf(toLight = wi
+ * | sd.V = wo) at wavelength 0 and lambdaPdf. sample: query i draws from the sample generator
+ * (pixel (i, 0), sample number `frame`). Profiles are the context's. A material index out of
+ * range gives zeros (eval) or valid = 0 (sample). */
+enum {
+    PLT_MQ_WI_X = 0, PLT_MQ_WI_Y, PLT_MQ_WI_Z, PLT_MQ_WO_X, PLT_MQ_WO_Y, PLT_MQ_WO_Z,
+    PLT_MQ_LOBE, PLT_MQ_MAT, PLT_MQ_LAMBDA /* 4 planes */, PLT_MQ_S = PLT_MQ_LAMBDA + 4 /* 16 planes, [j*4 + c] */,
+    PLT_MQ_AREA = PLT_MQ_S + 16, PLT_MQ_PLANES
+};
+enum { PLT_MR_S = 0 /* 16 planes */, PLT_MR_PDF = 16, PLT_MR_LAMBDA_PDF /* 4 planes */, PLT_MR_PLANES = PLT_MR_LAMBDA_PDF + 4 };
+enum { PLT_MSQ_WI_X = 0, PLT_MSQ_WI_Y, PLT_MSQ_WI_Z, PLT_MSQ_MAT, PLT_MSQ_LAMBDA, PLT_MSQ_PLANES };
+enum { PLT_MSR_WO_X = 0, PLT_MSR_WO_Y, PLT_MSR_WO_Z, PLT_MSR_PDF, PLT_MSR_WEIGHT, PLT_MSR_LOBE, PLT_MSR_VALID, PLT_MSR_PLANES };
+int plt_material_eval_batch(plt_context* ctx, const plt_material* materials, uint32_t n_materials, uint32_t n,
+                            const float* queries, float* results, plt_stream stream);
+int plt_material_sample_batch(plt_context* ctx, const plt_material* materials, uint32_t n_materials, uint32_t n,
+                              uint32_t frame, const float* queries, float* results, plt_stream stream);
 
 /* Pinhole camera (Scene/Camera/Camera.slang:46-78, Camera.cpp:187-193). */
 typedef struct plt_camera {
@@ -183,7 +214,9 @@ typedef struct plt_scene plt_scene;
 /* Procedural workload scene: a ground quad (PLTDiffuse), `sphere_count` icospheres of
  * subdivision `subdivision` (20*4^s triangles each) at uniform random positions on the quad,
  * resting on it, radius U[0.2, 1.0], materials drawn uniformly from the first
- * `material_kinds` of {dielectric, conductor, coated opaque dielectric, grated conductor}
+ * `material_kinds` (1-8) of {dielectric, conductor, coated opaque dielectric, grated conductor,
+ * birefringent thin dielectric (thickness U[0.2, 2] um, birefringence U[0.01, 0.2]), multilayer
+ * stack (20 layers 1.38 / 2.3, quarter-wave at 550 nm), coated conductor, opaque dielectric}
  * (grating pitch U[0.5, 2] um), one quad area light (PLTDiffuse emitter, 6500 K blackbody
  * tabulated in 5 nm bins), and a pinhole camera of the given aspect. Deterministic in `seed`. */
 typedef struct plt_procedural_scene_desc {

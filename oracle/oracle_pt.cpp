@@ -448,6 +448,54 @@ void buildLights(OScene& s) {
 }
 }  // namespace
 
+namespace {
+// One material query through the material interface in a local frame with N = +z (tests):
+// `mat` is a plt_material block, wi = sd.V (towards the light), wo = toEye, 4 HWSS wavelengths,
+// 4 incident Stokes vectors (16 values, component-major), coherence area `area` (um^2) for the
+// beam (r = -wi, r_m = 1 mm). Writes the 4 outgoing Stokes vectors (16), the pdf of sampling
+// `wi` from `wo` (1) and lambdaPdf (4). dbl selects the FP64 instantiation.
+template <class T>
+static void material_eval_t(const void* mat, const void* prof, uint32_t nprof, const double* wi3, const double* wo3,
+                            uint32_t lobe, const double* lambda, const double* stokes, double area, double* out,
+                            double* pdf, double* lpdf) {
+    const Instance<T> in = setupMaterialInstance<T>(*static_cast<const MatData*>(mat), static_cast<const SpectralProfile*>(prof), nprof);
+    SD<T> sd;
+    sd.f = Frame<T>{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    sd.posW = {0, 0, 0};
+    sd.faceN = {0, 0, 1};
+    sd.V = {T(wi3[0]), T(wi3[1]), T(wi3[2])};
+    sd.frontFacing = true;
+    sd.materialID = 0;
+    const V3<T> wo = {T(wo3[0]), T(wo3[1]), T(wo3[2])};
+    PLTBeam<T, 4> beam;
+    for (int j = 0; j < 4; ++j) {
+        beam.L[j] = T(stokes[4 * j]); beam.L1[j] = T(stokes[4 * j + 1]); beam.L2[j] = T(stokes[4 * j + 2]); beam.L3[j] = T(stokes[4 * j + 3]);
+        beam.wavelengths[j] = T(lambda[j]);
+        beam.spectralPDFs[j] = 1;
+    }
+    beam.hwss_comps = 4;
+    const T d = T(M_PI) * sqr(T(1e-3) * T(lambda[0])) / T(area);
+    beam.Theta.D[0] = d; beam.Theta.D[1] = 0; beam.Theta.D[2] = d;
+    beam.Theta.r_m = T(1e-3);
+    beam.posW = {0, 0, 0};
+    beam.r = -sd.V;
+    beam.t = perp_stark(beam.r);
+    beam.distant = false;
+    T lp[4];
+    materialLambdaPdf<T, 4>(in, sd, wo, lobe, beam, lp);
+    materialEval<T, 4>(in, sd, wo, lobe, beam);
+    for (int j = 0; j < 4; ++j) {
+        out[4 * j] = double(beam.L[j]); out[4 * j + 1] = double(beam.L1[j]);
+        out[4 * j + 2] = double(beam.L2[j]); out[4 * j + 3] = double(beam.L3[j]);
+        lpdf[j] = double(lp[j]);
+    }
+    SD<T> sd2 = sd;
+    sd2.V = wo;
+    *pdf = double(materialPdf<T>(in, sd2, sd.V, T(lambda[0])));
+}
+
+}  // namespace
+
 extern "C" {
 
 // Camera: pos[3], target[3], up[3], fov_y, aspect, focal_distance, jitter[2] (14 floats).
@@ -548,6 +596,32 @@ int oracle_alias_table(const float* pos, uint32_t ntri, const uint32_t* idx, con
         pdf[i] = s.lightPdf[i];
     }
     return int(n);
+}
+
+int oracle_material_eval(const void* mat, const void* prof, uint32_t nprof, const double* wi, const double* wo, uint32_t lobe,
+                         const double* lambda, const double* stokes, double area, int dbl, double* out, double* pdf, double* lpdf) {
+    if (dbl) material_eval_t<double>(mat, prof, nprof, wi, wo, lobe, lambda, stokes, area, out, pdf, lpdf);
+    else material_eval_t<float>(mat, prof, nprof, wi, wo, lobe, lambda, stokes, area, out, pdf, lpdf);
+    return 0;
+}
+
+// One FP32 sample per query (as the path tracer draws them): query i has its own wi (3 floats,
+// local frame) and wavelength and draws from generator (i, 0, frame); writes wo (3n), pdf, weight,
+// lobe, valid.
+int oracle_material_sample(const void* mat, const void* prof, uint32_t nprof, const float* wi3, const float* lambda, uint32_t n,
+                           uint32_t frame, float* wo, float* pdf, float* weight, uint32_t* lobe, uint32_t* valid) {
+    const Instance<float> in = setupMaterialInstance<float>(*static_cast<const MatData*>(mat), static_cast<const SpectralProfile*>(prof), nprof);
+    SD<float> sd;
+    sd.f = Frame<float>{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (uint32_t i = 0; i < n; ++i) {
+        sd.V = {wi3[3 * i], wi3[3 * i + 1], wi3[3 * i + 2]};
+        UniformSampleGenerator sg(i, 0, frame);
+        BSDFSample r;
+        valid[i] = materialSample(in, sd, sg, r, lambda[i]) ? 1u : 0u;
+        wo[3 * i] = r.wo.x; wo[3 * i + 1] = r.wo.y; wo[3 * i + 2] = r.wo.z;
+        pdf[i] = r.pdf; weight[i] = r.weight; lobe[i] = r.lobe;
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -355,6 +355,41 @@ template <class T> inline M4<T> MuellerFresnelTDielectric(T cosThetaI, T eta) { 
 template <class T> inline M4<T> MuellerFresnelDielectric(T cosThetaI, T eta, bool reflection) {  // :140-144
     return reflection ? MuellerFresnelRDielectric(cosThetaI, eta) : MuellerFresnelTDielectric(cosThetaI, eta);
 }
+template <class T> inline M4<T> add4(const M4<T>& a, const M4<T>& b) {
+    M4<T> r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.m[i][j] = a.m[i][j] + b.m[i][j];
+    return r;
+}
+template <class T> inline bool tryInvert2(const M2<T>& M, M2<T>& inv) {  // Utils/Math/MathHelpers.slang:484-496
+    const T det = M.m[0][0] * M.m[1][1] - M.m[0][1] * M.m[1][0];
+    if (std::fabs(det) < T(4e-6)) return false;
+    const T id = T(1) / det;
+    inv.m[0][0] = M.m[1][1] * id;
+    inv.m[1][1] = M.m[0][0] * id;
+    inv.m[0][1] = -M.m[0][1] * id;
+    inv.m[1][0] = -M.m[1][0] * id;
+    return true;
+}
+// (1 - M)^-1 of a Fresnel Mueller matrix, block by block; zero when a block is singular (:89-110).
+template <class T> inline M4<T> invOneMinusMuellerFresnel(V2<T> fs, V2<T> fp) {
+    const T Rp = cnorm2(fp), Rs = cnorm2(fs);
+    const T m00 = (Rp + Rs) / T(2), m01 = (Rp - Rs) / T(2);
+    const V2<T> c = cmul(fs, cconj(fp));
+    const M2<T> minor1 = m2<T>(T(1) - m00, -m01, -m01, T(1) - m00);
+    const M2<T> minor2 = m2<T>(T(1) - c.x, -c.y, c.y, T(1) - c.x);
+    M2<T> im1, im2;
+    if (!tryInvert2(minor1, im1) || !tryInvert2(minor2, im2)) return zero4<T>();
+    M4<T> r = zero4<T>();
+    r.m[0][0] = im1.m[0][0]; r.m[0][1] = im1.m[0][1]; r.m[1][0] = im1.m[0][1]; r.m[1][1] = im1.m[0][0];
+    r.m[2][2] = im2.m[0][0]; r.m[2][3] = im2.m[0][1]; r.m[3][2] = im2.m[1][0]; r.m[3][3] = im2.m[0][0];
+    return r;
+}
+template <class T> inline M4<T> invOneMinusMuellerFresnelRDielectric(T cosThetaI, T eta) {  // :128-133
+    T rs, rp, ts, tp, c;
+    fresnel_dielectric(cosThetaI, eta, rs, rp, ts, tp, c);
+    return invOneMinusMuellerFresnel(V2<T>{rs, 0}, V2<T>{rp, 0});
+}
 
 // ---------------------------------------------------------------------------------------
 // Spectral utilities (Utils/Color/SpectrumUtils.slang:43-239, SpectrumSampler.slang:38-54,

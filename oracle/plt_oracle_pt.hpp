@@ -17,7 +17,8 @@ struct MatData {
     uint16_t base_color[4], specular[4], data1[4], data2[4], transmission[3], ior;
     uint32_t emission, ior_n, ior_k;
 };
-enum : uint32_t { MT_DIELECTRIC = 0, MT_CONDUCTOR = 1, MT_DIFFUSE = 2, MT_OPAQUE = 3, MT_COATED_OPAQUE = 4, MT_GRATED_CONDUCTOR = 5 };
+enum : uint32_t { MT_DIELECTRIC = 0, MT_CONDUCTOR = 1, MT_DIFFUSE = 2, MT_OPAQUE = 3, MT_COATED_OPAQUE = 4, MT_GRATED_CONDUCTOR = 5,
+                  MT_COATED_CONDUCTOR = 6, MT_THIN_DIELECTRIC = 7, MT_MULTILAYER = 8 };
 constexpr uint32_t kNoProfile = 0xFFFFFFFFu;
 inline float hf(uint16_t h) { return f16_bits_to_f32(h); }
 
@@ -201,6 +202,278 @@ template <class T> struct CoatedOpaqueBxDF {
     }
 };
 
+// PLTCoatedConductorBxDF (PLTCoatedConductorBSDF.slang:45-217): a thin dielectric coat over a
+// K-correlation conductor with tabulated n,k. sample / pdf / lambdaPdf / coherenceTransform are
+// those of the coated opaque dielectric (the `albedo` member is the specular colour here).
+template <class T> struct CoatedConductorBxDF : CoatedOpaqueBxDF<T> {
+    const SpectralProfile* etaN = nullptr;
+    const SpectralProfile* etaK = nullptr;
+    template <int N> void eval(V3<T> toLight, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) const {  // :72-128
+        using B = CoatedOpaqueBxDF<T>;
+        const bool isReflection = toLight.z * toEye.z > T(0);
+        const bool isScatter = (lobe & Lobe_ScatterReflection) != 0, isDirect = (lobe & Lobe_DeltaReflection) != 0;
+        const T coatior = this->coatEta();
+        const T eta1 = coatior / this->extEta;
+        T cTin;
+        (void)evalFresnelDielectric(eta1, toLight.z, cTin);
+        const V3<T> wiIn = -B::refractCoat(toLight, eta1, cTin);
+        V3<T> woIn = {-wiIn.x, -wiIn.y, wiIn.z};
+        if (!isDirect) { T cTout; (void)evalFresnelDielectric(eta1, toEye.z, cTout); woIn = -B::refractCoat(toEye, eta1, cTout); }
+        if (!isReflection || (!isScatter && !isDirect) || wiIn.z <= T(0) || woIn.z <= T(0)) { beam.setZero(); return; }
+        T k[N], a[N], coh[N], iorN[N], iorK[N];
+        for (int j = 0; j < N; ++j) {
+            k[j] = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            a[j] = this->gfs.alpha(wiIn, woIn, k[j]);
+            iorN[j] = evalSpectrum<T>(etaN, beam.wavelengths[j]);
+            iorK[j] = evalSpectrum<T>(etaK, beam.wavelengths[j]);
+        }
+        const V3<T> h = wiIn + woIn;
+        const V3<T> m = normalize(h);
+        const T LdotH = hmax(T(0), dot(wiIn, m));
+        const V3<T> Z = beam.toLocal(V3<T>{0, 0, 1});
+        beam.mutualCoherence(k, Z * this->tau, coh);
+        const M4<T> MiR = MuellerFresnelRDielectric(toLight.z, eta1);
+        const M4<T> MiT = MuellerFresnelTDielectric(toLight.z, eta1);
+        const M4<T> MoT = MuellerFresnelTDielectric(-woIn.z, eta1);
+        for (int j = 0; j < beam.hwss_comps; ++j) {
+            V2<T> rs, rp;
+            const V2<T> ior = {iorN[j] / coatior, iorK[j] / coatior};
+            fresnel_conductor(LdotH, ior, rs, rp);
+            const M4<T> M = mul4(mul4(MiT, MuellerFresnel(rs, rp)), MoT);
+            const V4<T> SP = beam.getSP(j);
+            const T m00 = Spectrum<T>::rgbToSpectrum(this->albedo, beam.wavelengths[j]);
+            V4<T> newSP = m00 * mul(M, SP);
+            if (isScatter) {
+                const T D = this->gfs.eval(k[j], h);
+                newSP = ((T(1) - a[j]) * std::fabs(toLight.z) * D) * newSP;
+            } else {
+                // phaseShift(eta, dist, k, r) = sin(dist k eta + arg r) (Interference.slang:36-39)
+                const T dist = T(2) * this->tau / wiIn.z;
+                const T pS = std::sin(dist * k[j] * coatior + std::atan2(rs.y, rs.x));
+                const T pP = std::sin(dist * k[j] * coatior + std::atan2(rp.y, rp.x));
+                newSP = interfere(a[j] * newSP, mul(MiR, SP), pS, pP, coh[j]);
+            }
+            beam.setSP(j, newSP);
+        }
+    }
+};
+
+// Birefringence (Rendering/Materials/PLT/Birefringence.slang:33-197): normal modes, fields, Poynting
+// vectors and Fresnel coefficients of an isotropic / uniaxial interface (y up, incidence in yz).
+template <class T> struct Biref {
+    static T qo(T eo, T Kt) { return std::sqrt(sqr(eo) - sqr(Kt)); }
+    static V2<T> qe(T eo, T ee, V3<T> A, T Kt) {
+        const T e = sqr(ee) - sqr(eo);
+        const T a = A.x, b = A.y, c = A.z;
+        const T G = sqr(b) * e + sqr(eo);
+        const T r = std::sqrt(hmax(T(0), sqr(Kt) * (sqr(a) * e - sqr(ee)) + sqr(ee) * G));
+        return {(eo * r + c * b * Kt * e) / G, (-eo * r + c * b * Kt * e) / G};
+    }
+    static V3<T> E_e(T eo, V3<T> A, T Kt, T q) {
+        return normalize(V3<T>{A.x * sqr(eo), Kt * A.z * q + A.y * (sqr(eo) - sqr(q)), Kt * A.y * q + A.z * (sqr(eo) - sqr(Kt))});
+    }
+    static V3<T> E_o(V3<T> A, T Kt, T q) { return normalize(V3<T>{-A.y * Kt - A.z * q, A.x * Kt, A.x * q}); }
+    static V3<T> Poynting(T q, V3<T> E, T Kt) {
+        return normalize(V3<T>{E.x * (Kt * E.z - q * E.y), Kt * E.z * E.y + q * (sqr(E.x) + sqr(E.z)),
+                               -q * E.z * E.y - Kt * (sqr(E.x) + sqr(E.y))});
+    }
+    static void fresnel_iso_aniso(T cos_theta, T sin_theta, T ei, T eo, T ee, V3<T> A, T& rss, T& rsp, T& tso, T& tse,
+                                  T& rps, T& rpp, T& tpo, T& tpe) {
+        const T Kt = ei * sin_theta;
+        const T qi = -qo(ei, Kt);
+        const T q2om = -qo(eo, Kt);
+        const T q2em = qe(eo, ee, A, Kt).y;
+        const V3<T> E2om = E_o(A, Kt, q2om);
+        const V3<T> E2em = E_e(eo, A, Kt, q2em);
+        const T A_1 = -E2em.x * E2om.y * Kt * cos_theta;
+        const T A_2 = -E2em.y * E2om.x * Kt * cos_theta;
+        const T C_1 = qi * cos_theta - Kt * sin_theta;
+        const T C_2 = E2em.x * E2om.z * (C_1 + q2om * cos_theta) - A_1;
+        const T C_3 = E2em.z * E2om.x * (C_1 + q2em * cos_theta) - A_2;
+        const T Ns = C_2 * (qi + q2em) - C_3 * (qi + q2om);
+        rps = (T(2) * E2em.x * E2om.x * cos_theta * C_1 * (q2om - q2em)) / Ns;
+        rpp = (-E2em.x * E2om.z * (C_1 - q2om * cos_theta) * (qi + q2em) + E2em.z * E2om.x * (C_1 - q2em * cos_theta) * (qi + q2om) -
+               A_1 * (q2em + qi) + A_2 * (q2om + qi)) / Ns;
+        tpo = -(T(2) * E2em.x * cos_theta * (q2em + qi) * C_1) / Ns;
+        tpe = (T(2) * E2om.x * cos_theta * (q2om + qi) * C_1) / Ns;
+        rss = (C_2 * (qi - q2em) + C_3 * (q2om - qi)) / Ns;
+        rsp = (T(2) * qi * (E2em.z * E2om.y * Kt - E2em.y * E2om.z * Kt + E2em.z * E2om.z * (q2om - q2em))) / Ns;
+        tso = -(T(2) * qi * (E2em.y * Kt * cos_theta + E2em.z * C_1 + E2em.z * q2em * cos_theta)) / Ns;
+        tse = (T(2) * qi * (E2om.y * Kt * cos_theta + E2om.z * C_1 + E2om.z * q2om * cos_theta)) / Ns;
+    }
+    static void fresnel_aniso_iso(T cos_phi, T sin_phi, T ei, T eo, T ee, V3<T> A, T& roo, T& roe, T& tos, T& top, T& reo,
+                                  T& ree, T& tes, T& tep) {
+        const T Kt = ei * sin_phi;
+        const T q2om = -qo(ei, Kt);
+        const T qop = qo(eo, Kt);
+        const V2<T> q2 = qe(eo, ee, A, Kt);
+        const T qep = q2.x, qem = q2.y;
+        const V3<T> Eop = E_o(A, Kt, qop), Eom = E_o(A, Kt, -qop);
+        const V3<T> Eep = E_e(eo, A, Kt, qep), Eem = E_e(eo, A, Kt, qem);
+        const T B_1 = -Eep.x * Kt * cos_phi * (q2om - qep);
+        const T B_2 = -Eom.x * Kt * cos_phi * (q2om + qop);
+        const T B_3 = -Eop.x * Kt * cos_phi * (qop - q2om);
+        const T D_1 = Kt * sin_phi - q2om * cos_phi;
+        const T D_2 = Eep.z * (D_1 + qep * cos_phi);
+        const T D_3 = Eem.z * (D_1 + qem * cos_phi);
+        const T D_4 = Eop.z * (D_1 + qop * cos_phi);
+        const T N2 = Eep.x * (qep - q2om) * D_4 + Eop.x * (q2om - qop) * D_2 + Eop.y * B_1 + Eep.y * B_3;
+        roo = -(Eep.x * Eom.z * (q2om - qep) * (qop * cos_phi - D_1) + Eom.x * (q2om + qop) * D_2 + Eom.y * B_1 - Eep.y * B_2) / N2;
+        roe = -(-Eom.x * Eop.z * (qop * (qop * cos_phi + Kt * sin_phi) + q2om * D_1) +
+                Eom.z * Eop.x * (qop * (qop * cos_phi - Kt * sin_phi) + q2om * D_1) + Eop.y * B_2 + Eom.y * B_3) / N2;
+        tos = (Eep.x * qep * roe - Eom.x * qop + Eop.x * qop * roo) / q2om;
+        top = -(Eom.z * qop - Eom.y * Kt - (Eep.y * Kt + Eep.z * qep) * roe - (Eop.y * Kt + Eop.z * qop) * roo) / D_1;
+        reo = (Eem.x * (qem - q2om) * (D_2 + Eep.y * Kt * cos_phi) + Eep.x * (q2om - qep) * (D_3 + Eem.y * Kt * cos_phi)) / N2;
+        ree = -(Eem.x * (qem - q2om) * D_4 - Eem.x * Eop.y * Kt * cos_phi * (q2om - qem) + Eem.y * B_3 + Eop.x * (q2om - qop) * D_3) / N2;
+        tes = (Eem.x * qem + Eep.x * qep * ree + Eop.x * qop * reo) / q2om;
+        tep = -(Eem.z + Eep.z * ree + Eop.z * reo) / cos_phi;
+    }
+    // The ordinary and extraordinary ray directions in the slab and the effective indices (:168-195).
+    static void vectors_in_slab(V3<T> I, T ei, T eo, T ee, V3<T> A, V3<T>& Io, V3<T>& Ie, T& e_eff) {
+        const T Kt = ei * I.z;
+        const T qom = -qo(eo, Kt);
+        const T qem = qe(eo, ee, A, Kt).y;
+        const V3<T> Eom = E_o(A, Kt, qom);
+        const V3<T> Eem = E_e(eo, A, Kt, qem);
+        Io = Poynting(qom, Eom, Kt);
+        Ie = Poynting(qem, Eem, Kt);
+        const V3<T> We = normalize(V3<T>{0, qem, -Kt});
+        e_eff = Kt / std::sqrt(hmax(T(0), T(1) - sqr(We.y)));
+    }
+};
+
+// PLTThinDielectricBxDF (PLTThinDielectricBSDF.slang:43-200): a slab in air with the incoherent
+// sum of internal reflections T(1-R)^-1 T, and optional uniaxial birefringence in transmission.
+template <class T> struct ThinDielectricBxDF {
+    V2<T> opticAxisXY{T(.15), T(.04)};
+    T etaAvg, tau, birefringence;
+    V3<T> opticAxis() const {
+        return {opticAxisXY.x, opticAxisXY.y, std::sqrt(hmax(T(0), T(1) - dot(opticAxisXY, opticAxisXY)))};
+    }
+    template <int N> void handle_birefringence(PLTBeam<T, N>& beam, V3<T> wi, const T* k) const {  // :60-118
+        const V3<T> OA = opticAxis();
+        const T phii = std::atan2(wi.y, wi.x);
+        const T phiA = std::atan2(OA.y, OA.x) - phii;
+        const V3<T> I = {0, std::fabs(wi.z), std::sqrt(T(1) - sqr(wi.z))};
+        const T s = std::sqrt(T(1) - sqr(OA.z));
+        V3<T> A = {std::cos(phiA) * s, 0, std::sin(phiA) * s};
+        A.y = sgn(wi.z) * OA.z;
+        const V3<T> A2 = {A.x, -A.y, A.z};
+        const V3<T> Z = beam.toLocal(V3<T>{std::cos(phii), std::sin(phii), 0});
+        const T ei = 1, eo = etaAvg - T(.5) * birefringence, ee = etaAvg + T(.5) * birefringence;
+        const T tauum = tau * T(1e+3);
+        V3<T> Io, Ie;
+        T e_eff;
+        Biref<T>::vectors_in_slab(I, ei, eo, ee, A, Io, Ie, e_eff);
+        T rss, rsp, tso, tse, rps, rpp, tpo, tpe, roo, roe, tos, top, reo, ree, tes, tep;
+        Biref<T>::fresnel_iso_aniso(I.y, I.z, ei, eo, ee, A, rss, rsp, tso, tse, rps, rpp, tpo, tpe);
+        Biref<T>::fresnel_aniso_iso(I.y, I.z, ei, eo, ee, A2, roo, roe, tos, top, reo, ree, tes, tep);
+        const T ss = tso * tos + tse * tes, sp = tso * top + tse * tep;
+        const T ps = tpo * tos + tpe * tes, pp = tpo * top + tpe * tep;
+        const T Ioz = std::fabs(tauum / Io.y) * Io.z, Iez = std::fabs(tauum / Ie.y) * Ie.z;
+        const T OPLo = std::fabs(tauum / Io.y) * eo, OPLe = std::fabs(tauum / Ie.y) * e_eff;
+        const T oez = Ioz - Iez;
+        T coh[N], nLx[N], nLy[N], newL[N];
+        beam.mutualCoherence(k, Z * oez, coh);
+        bool allPos = true, allFinite = true;
+        for (int j = 0; j < N; ++j) {
+            const T t = coh[j] * T(2) * std::sin(-k[j] * (ei * oez * I.z + OPLo - OPLe));
+            const T Lx = T(.5) * (beam.L[j] + beam.L1[j]), Ly = T(.5) * (beam.L[j] - beam.L1[j]);
+            const T sq = std::sqrt(Lx * Ly);
+            nLx[j] = std::fmax(T(0), sqr(ss) * Lx + sqr(ps) * Ly + ss * ps * t * sq);
+            nLy[j] = std::fmax(T(0), sqr(sp) * Lx + sqr(pp) * Ly + sp * pp * t * sq);
+            newL[j] = nLx[j] + nLy[j];
+            allFinite = allFinite && std::isfinite(newL[j]);
+            allPos = allPos && beam.L[j] > T(1e-6) && std::isfinite(newL[j]);
+        }
+        for (int j = 0; j < N; ++j) {
+            const T norm = allPos ? newL[j] / beam.L[j] : T(0);
+            beam.L[j] = allFinite ? newL[j] : T(0);
+            beam.L1[j] = allFinite ? nLx[j] - nLy[j] : T(0);
+            beam.L2[j] *= norm;
+            beam.L3[j] *= norm;
+        }
+    }
+    template <int N> void eval(V3<T> toLight, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) const {  // :120-149
+        const T eta = etaAvg;
+        const bool isReflection = toLight.z * toEye.z > T(0);
+        const bool isDirect = (lobe & Lobe_Delta) != 0;
+        const bool hasTransmission = (lobe & Lobe_Transmission) != 0, hasReflection = (lobe & Lobe_Reflection) != 0;
+        if ((isReflection && eta == T(1)) || toLight.z * toEye.z == T(0) || !isDirect || (!hasTransmission && !isReflection) ||
+            (!hasReflection && isReflection)) {
+            beam.setZero();
+            return;
+        }
+        T k[N];
+        for (int j = 0; j < N; ++j) k[j] = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+        const T c = std::fabs(toLight.z);
+        const M4<T> R = MuellerFresnelRDielectric(c, eta), iR = invOneMinusMuellerFresnelRDielectric(c, eta),
+                    Tm = MuellerFresnelTDielectric(c, eta);
+        const M4<T> M = isReflection ? (iR.m[0][0] > T(0) ? add4(R, mul4(mul4(Tm, R), mul4(iR, Tm))) : R)
+                                     : (iR.m[0][0] > T(0) ? mul4(Tm, mul4(iR, Tm)) : mul4(Tm, Tm));
+        if (birefringence > T(0) && !isReflection) handle_birefringence(beam, toLight, k);
+        for (int j = 0; j < beam.hwss_comps; ++j) beam.setSP(j, mul(M, beam.getSP(j)));
+    }
+    template <class S> bool sample(V3<T> wi, V3<T>& wo, T& pdf, T& weight, uint32_t& lobe, S& sg, T /*wavelength*/, uint32_t lobeMask) const {
+        const bool hasReflection = (lobeMask & Lobe_Reflection) != 0, hasTransmission = (lobeMask & Lobe_Transmission) != 0;
+        if (!hasReflection && !hasTransmission) return false;
+        const T Fd = evalFresnelDielectric(etaAvg, std::fabs(wi.z));
+        const bool isReflection = hasReflection ? (Fd > T(0) && sampleNext1D<T>(sg) <= Fd) : false;
+        wo = isReflection ? V3<T>{-wi.x, -wi.y, wi.z} : -wi;
+        weight = 1;
+        pdf = hasReflection && hasTransmission ? (isReflection ? Fd : T(1) - Fd) : T(1);
+        lobe = isReflection ? Lobe_DeltaReflection : Lobe_DeltaTransmission;
+        return true;
+    }
+    T pdf(V3<T>, V3<T>, T) const { return 0; }
+    M2<T> coherenceTransform(V3<T>, V3<T>, uint32_t) const { return ident2<T>(); }
+};
+
+// PLTMultiLayeredStackBxDF (PLTMultiLayeredStackBSDF.slang:42-139): alternating two-index stack
+// over a base of reflectance albedo; scalar Airy recursion of the amplitude, output Re(G)^2.
+template <class T> struct MultiLayerBxDF {
+    V3<T> albedo{1, 1, 1};
+    T extEta;
+    T ior[2], thickness[2];
+    uint32_t layers;
+    template <int N> void eval(V3<T> toLight, V3<T> toEye, uint32_t lobe, PLTBeam<T, N>& beam) const {  // :58-95
+        const bool isReflection = toLight.z * toEye.z > T(0);
+        const bool isDirect = (lobe & Lobe_Delta) != 0;
+        if (!isDirect || !isReflection) { beam.setZero(); return; }
+        const T eta1 = ior[0] / extEta, eta2 = ior[1] / ior[0];
+        T cT[2], r[2];
+        const T Rout = std::sqrt(evalFresnelDielectric(eta1, toLight.z, cT[0]));
+        r[0] = std::sqrt(evalFresnelDielectric(eta2, -cT[0], cT[1]));
+        r[1] = std::sqrt(evalFresnelDielectric(eta2, cT[1]));
+        cT[0] = std::fabs(cT[0]);
+        cT[1] = std::fabs(cT[1]);
+        for (int j = 0; j < beam.hwss_comps; ++j) {
+            const T k = K<T>::_2PI / (T(1e-3) * beam.wavelengths[j]);
+            V2<T> G = {std::sqrt(Spectrum<T>::rgbToSpectrum(albedo, beam.wavelengths[j])), 0};
+            for (uint32_t l = 0; l < layers; ++l) {
+                const uint32_t idx = l % 2;
+                const T d = T(2) * thickness[idx] / cT[idx] * k * ior[idx];
+                const V2<T> p = {std::cos(d), std::sin(d)};
+                const T R = l == layers - 1 ? Rout : r[idx];
+                G = cmul(G, p);
+                G = cdiv(V2<T>{R + G.x, G.y}, V2<T>{T(1) + R * G.x, R * G.y});
+            }
+            beam.setSP(j, sqr(G.x) * beam.getSP(j));
+        }
+    }
+    template <class S> bool sample(V3<T> wi, V3<T>& wo, T& pdf, T& weight, uint32_t& lobe, S&, T wavelength, uint32_t lobeMask) const {
+        const bool hasReflection = (lobeMask & Lobe_Reflection) != 0, hasDirect = (lobeMask & Lobe_Delta) != 0;
+        if (!hasReflection || !hasDirect || wi.z <= T(0)) return false;
+        pdf = 1;
+        wo = {-wi.x, -wi.y, wi.z};
+        lobe = Lobe_DeltaReflection;
+        weight = Spectrum<T>::rgbToSpectrum(albedo, wavelength);
+        return wi.z * wo.z > T(0);
+    }
+    T pdf(V3<T>, V3<T>, T) const { return 0; }
+    M2<T> coherenceTransform(V3<T>, V3<T>, uint32_t) const { return ident2<T>(); }
+};
+
 // BesselJ (Utils/Math/MathHelpers.slang:813-876)
 template <class T> inline T BesselJ(int nu, T x) {
     static const uint32_t table[13] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880, 3628800, 39916800, 479001600};
@@ -373,6 +646,9 @@ template <class T> struct Instance {
     OpaqueDielectricBxDF<T> opq;
     CoatedOpaqueBxDF<T> cot;
     GratedConductorBxDF<T> grt;
+    CoatedConductorBxDF<T> ccn;
+    ThinDielectricBxDF<T> thn;
+    MultiLayerBxDF<T> mls;
     uint32_t emission = kNoProfile;
 };
 template <class T> inline Instance<T> setupMaterialInstance(const MatData& md, const SpectralProfile* prof, uint32_t nprof) {
@@ -433,6 +709,27 @@ template <class T> inline Instance<T> setupMaterialInstance(const MatData& md, c
             in.grt.grating.gy = std::sin(dir);
             break;
         }
+        case MT_COATED_CONDUCTOR:  // PLTCoatedConductorMaterial.slang:40-60
+            in.ccn.albedo = base;
+            in.ccn.gfs = FractalSurface<T>::makeRef(T(sy));
+            in.ccn.etaN = pn; in.ccn.etaK = pk;
+            in.ccn.extEta = T(hf(md.ior));
+            in.ccn.tau = T(hf(md.data1[0]) * sz);
+            in.ccn.coatIOR = T(hf(md.data1[3]));
+            break;
+        case MT_THIN_DIELECTRIC:  // PLTThinDielectricMaterial.slang:40-56
+            in.thn.tau = T(sw);
+            in.thn.birefringence = T(hf(md.base_color[0]) * sx);
+            in.thn.etaAvg = T(hf(md.ior));
+            in.thn.opticAxisXY = V2<T>{T(hf(md.base_color[1])), T(hf(md.base_color[2]))};
+            break;
+        case MT_MULTILAYER:  // PLTMultiLayeredStackMaterial.slang:40-57
+            in.mls.albedo = base;
+            in.mls.extEta = T(hf(md.ior));
+            in.mls.ior[0] = T(hf(md.data2[0])); in.mls.ior[1] = T(hf(md.data2[1]));
+            in.mls.thickness[0] = T(hf(md.data1[0])); in.mls.thickness[1] = T(hf(md.data1[1]));
+            in.mls.layers = md.data1[3];
+            break;
     }
     return in;
 }
@@ -448,6 +745,9 @@ template <class T> inline uint32_t getLobes(const Instance<T>& in) {
         case MT_DIFFUSE: return Lobe_DiffuseReflection;
         case MT_OPAQUE: return in.opq.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection);
         case MT_COATED_OPAQUE: return in.cot.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection);
+        case MT_COATED_CONDUCTOR: return in.ccn.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection);
+        case MT_THIN_DIELECTRIC: return Lobe_DeltaReflection | Lobe_DeltaTransmission;
+        case MT_MULTILAYER: return Lobe_DeltaReflection;
         default:
             return (in.grt.base.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection)) | Lobe_DispersiveDeltaReflection;
     }
@@ -461,7 +761,9 @@ template <class T> inline uint32_t getPropertyFlags(const Instance<T>& in) {
         case MT_DIFFUSE: return isEmissive(in) ? Prop_Emissive : 0u;
         case MT_OPAQUE: return in.opq.gfs.q < cut ? Prop_MWR : 0u;
         case MT_COATED_OPAQUE: return in.cot.gfs.q < cut ? Prop_MWR : 0u;
-        default: return Prop_MWR;
+        case MT_COATED_CONDUCTOR: return in.ccn.gfs.q < cut ? Prop_MWR : 0u;
+        case MT_THIN_DIELECTRIC: return Prop_Transmissive | Prop_MWR | Prop_MWT;
+        default: return Prop_MWR;  // grated conductor, multilayer stack
     }
 }
 
@@ -490,6 +792,9 @@ template <class T, int N> inline void materialEval(const Instance<T>& in, const 
         case MT_DIFFUSE: bsdfEval<T, N>(in.dif, f, sd.V, toEye, lobe, beam); break;
         case MT_OPAQUE: bsdfEval<T, N>(in.opq, f, sd.V, toEye, lobe, beam); break;
         case MT_COATED_OPAQUE: bsdfEval<T, N>(in.cot, f, sd.V, toEye, lobe, beam); break;
+        case MT_COATED_CONDUCTOR: bsdfEval<T, N>(in.ccn, f, sd.V, toEye, lobe, beam); break;
+        case MT_THIN_DIELECTRIC: bsdfEval<T, N>(in.thn, f, sd.V, toEye, lobe, beam); break;
+        case MT_MULTILAYER: bsdfEval<T, N>(in.mls, f, sd.V, toEye, lobe, beam); break;
         default: bsdfEval<T, N>(in.grt, f, sd.V, toEye, lobe, beam); break;
     }
 }
@@ -502,6 +807,7 @@ template <class T, int N> inline void materialLambdaPdf(const Instance<T>& in, c
         case MT_DIELECTRIC: in.die.lambdaPdf(a, b, lobe, beam, ret); break;
         case MT_CONDUCTOR: in.con.lambdaPdf(a, b, lobe, beam, ret); break;
         case MT_COATED_OPAQUE: in.cot.lambdaPdf(a, b, lobe, beam, ret); break;
+        case MT_COATED_CONDUCTOR: in.ccn.lambdaPdf(a, b, lobe, beam, ret); break;
         case MT_GRATED_CONDUCTOR: in.grt.lambdaPdf(a, b, lobe, beam, ret); break;
         default: break;
     }
@@ -515,6 +821,9 @@ template <class T> inline T materialPdf(const Instance<T>& in, const SD<T>& sd, 
         case MT_DIFFUSE: return in.dif.pdf(a, b, wavelength);
         case MT_OPAQUE: return in.opq.pdf(a, b, wavelength);
         case MT_COATED_OPAQUE: return in.cot.pdf(a, b, wavelength);
+        case MT_COATED_CONDUCTOR: return in.ccn.pdf(a, b, wavelength);
+        case MT_THIN_DIELECTRIC: return in.thn.pdf(a, b, wavelength);
+        case MT_MULTILAYER: return in.mls.pdf(a, b, wavelength);
         default: return in.grt.pdf(a, b, wavelength);
     }
 }
@@ -529,6 +838,9 @@ template <class S> inline bool materialSample(const Instance<float>& in, const S
         case MT_DIFFUSE: valid = in.dif.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); r.wo = f.fromLocal(r.wo); return valid;
         case MT_OPAQUE: valid = in.opq.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         case MT_COATED_OPAQUE: valid = in.cot.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
+        case MT_COATED_CONDUCTOR: valid = in.ccn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
+        case MT_THIN_DIELECTRIC: valid = in.thn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
+        case MT_MULTILAYER: valid = in.mls.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         default: valid = in.grt.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
     }
     if (valid) r.wo = f.fromLocal(r.wo);

@@ -23,6 +23,12 @@ STATUS_NAMES = {0: "PLT_OK", 1: "PLT_WARN_UNKNOWN_KEY", -1: "PLT_EINVAL", -2: "P
                 -4: "PLT_ENOTIMPL", -5: "PLT_ESTATE"}
 
 
+MQ_WI, MQ_WO, MQ_LOBE, MQ_MAT, MQ_LAMBDA, MQ_S, MQ_AREA, MQ_PLANES = 0, 3, 6, 7, 8, 12, 28, 29
+MR_S, MR_PDF, MR_LAMBDA_PDF, MR_PLANES = 0, 16, 17, 21
+MSQ_WI, MSQ_MAT, MSQ_LAMBDA, MSQ_PLANES = 0, 3, 4, 5
+MSR_WO, MSR_PDF, MSR_WEIGHT, MSR_LOBE, MSR_VALID, MSR_PLANES = 0, 3, 4, 5, 6, 7
+
+
 class PLTError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
@@ -87,6 +93,10 @@ SIGNATURES = [
     ("plt_bsdf_generate_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("plt_bsdf_eval_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),
+    ("plt_material_eval_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]),
+    ("plt_material_sample_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                            C.c_void_p, C.c_void_p]),
     ("plt_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
     ("plt_scene_destroy", C.c_int, [C.c_void_p]),
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),

@@ -94,3 +94,38 @@ def sample_generator_stream(n_streams: int, width: int, sample_number: int, draw
     _lib.check(lib.plt_sample_generator_stream(n_streams, width, sample_number, draws, C.c_void_p(out.data_ptr()),
                                                _stream_ptr(stream)))
     return out
+
+
+class MaterialTable:
+    """A device array of plt_material blocks for the batched material interface
+    (plt_material_eval_batch / plt_material_sample_batch)."""
+
+    def __init__(self, ctx: Context, materials: bytes, device: str = "cuda"):
+        self.ctx = ctx
+        self.n = len(materials) // C.sizeof(_lib.Material)
+        self.data = torch.frombuffer(bytearray(materials), dtype=torch.uint8).to(device)
+
+    def eval(self, queries: torch.Tensor, stream=None) -> torch.Tensor:
+        """queries: float32 [PLT_MQ_PLANES, n] on the device; returns [PLT_MR_PLANES, n]."""
+        if queries.dtype != torch.float32 or queries.dim() != 2 or queries.shape[0] != _lib.MQ_PLANES or not queries.is_cuda:
+            raise ValueError(f"queries must be a float32 CUDA tensor of shape [{_lib.MQ_PLANES}, n]")
+        q = queries.contiguous()
+        n = q.shape[1]
+        out = torch.empty((_lib.MR_PLANES, n), dtype=torch.float32, device=q.device)
+        _lib.check(_lib.load().plt_material_eval_batch(self.ctx.handle, C.c_void_p(self.data.data_ptr()), self.n, n,
+                                                       C.c_void_p(q.data_ptr()), C.c_void_p(out.data_ptr()),
+                                                       _stream_ptr(stream)))
+        return out
+
+    def sample(self, queries: torch.Tensor, frame: int = 0, stream=None) -> torch.Tensor:
+        """queries: float32 [PLT_MSQ_PLANES, n] (wi xyz, material index bits, wavelength); returns
+        [PLT_MSR_PLANES, n] (wo xyz, pdf, weight, lobe bits, valid bits)."""
+        if queries.dtype != torch.float32 or queries.dim() != 2 or queries.shape[0] != _lib.MSQ_PLANES or not queries.is_cuda:
+            raise ValueError(f"queries must be a float32 CUDA tensor of shape [{_lib.MSQ_PLANES}, n]")
+        q = queries.contiguous()
+        n = q.shape[1]
+        out = torch.empty((_lib.MSR_PLANES, n), dtype=torch.float32, device=q.device)
+        _lib.check(_lib.load().plt_material_sample_batch(self.ctx.handle, C.c_void_p(self.data.data_ptr()), self.n, n,
+                                                         frame, C.c_void_p(q.data_ptr()), C.c_void_p(out.data_ptr()),
+                                                         _stream_ptr(stream)))
+        return out

@@ -8,11 +8,6 @@
 #include <vector>
 #include "plt_internal.hpp"
 
-struct plt_context {
-    int device = 0;
-    plt_spectral_profile* d_profiles = nullptr;
-    uint32_t n_profiles = 0;
-};
 
 namespace plt {
 namespace {

@@ -215,9 +215,26 @@ PLT_DEV float fresnel_dielectric_F(float eta, float cosI_, float& cosT_signed) {
     cosT_signed = cosT * -sgnf(cosI_);
     return 0.5f * (rs * rs + rp * rp);
 }
-// Conductor reflection with complex relative IoR (n, k) (Fresnel.slang:60-73).
-PLT_DEV Fres fresnel_conductor_mueller(float cosI, float n, float k) {
-    if ((n == 0.f && k == 0.f) || cosI < 0.f) return Fres{0.f, 0.f, 0.f, 0.f};
+// Mueller block products and sums: [[a,b],[b,a]] and [[c,d],[-d,c]] blocks compose like the
+// Mueller matrices of Mueller.slang:77-88 (X*Y, X+Y).
+PLT_DEV Fres fres_mul(const Fres& x, const Fres& y) {
+    return Fres{x.a * y.a + x.b * y.b, x.a * y.b + x.b * y.a, x.c * y.c - x.d * y.d, x.c * y.d + x.d * y.c};
+}
+PLT_DEV Fres fres_add(const Fres& x, const Fres& y) { return Fres{x.a + y.a, x.b + y.b, x.c + y.c, x.d + y.d}; }
+// (1 - M)^-1 block by block, zero if a block's determinant is below 4e-6 (Mueller.slang:89-110,
+// tryInvert MathHelpers.slang:484-496).
+PLT_DEV Fres fres_inv_one_minus(const Fres& m) {
+    const float a = 1.f - m.a, c = 1.f - m.c;
+    const float det1 = a * a - m.b * m.b, det2 = c * c + m.d * m.d;
+    if (fabsf(det1) < 4e-6f || fabsf(det2) < 4e-6f) return Fres{0.f, 0.f, 0.f, 0.f};
+    const float i1 = 1.f / det1, i2 = 1.f / det2;
+    return Fres{a * i1, m.b * i1, c * i2, m.d * i2};
+}
+// Conductor amplitude coefficients rs, rp (complex) for a complex relative IoR (n, k)
+// (fresnel_conductor, Fresnel.slang:60-73); false (and zeros) for n = k = 0 or cosI < 0.
+PLT_DEV bool fresnel_conductor_amp(float cosI, float n, float k, float& rs_r, float& rs_i, float& rp_r, float& rp_i) {
+    rs_r = rs_i = rp_r = rp_i = 0.f;
+    if ((n == 0.f && k == 0.f) || cosI < 0.f) return false;
     // 1/eta = conj(eta)/|eta|^2 ; t2 = 1 - sin^2 * (1/eta)^2 ; t = sqrt(t2)
     const float inv = 1.f / (n * n + k * k);
     const float ir = n * inv, ii = -k * inv;
@@ -241,16 +258,27 @@ PLT_DEV Fres fresnel_conductor_mueller(float cosI, float n, float k) {
     // rs = (c - eta t)/(c + eta t)
     const float rsn_r = cosI - etr, rsn_i = -eti, rsd_r = cosI + etr, rsd_i = eti;
     const float rsd = 1.f / (rsd_r * rsd_r + rsd_i * rsd_i);
-    const float rs_r = (rsn_r * rsd_r + rsn_i * rsd_i) * rsd, rs_i = (rsn_i * rsd_r - rsn_r * rsd_i) * rsd;
+    rs_r = (rsn_r * rsd_r + rsn_i * rsd_i) * rsd;
+    rs_i = (rsn_i * rsd_r - rsn_r * rsd_i) * rsd;
     // rp = (eta c - t)/(eta c + t)
     const float ecr = n * cosI, eci = k * cosI;
     const float rpn_r = ecr - tr, rpn_i = eci - ti, rpd_r = ecr + tr, rpd_i = eci + ti;
     const float rpd = 1.f / (rpd_r * rpd_r + rpd_i * rpd_i);
-    const float rp_r = (rpn_r * rpd_r + rpn_i * rpd_i) * rpd, rp_i = (rpn_i * rpd_r - rpn_r * rpd_i) * rpd;
+    rp_r = (rpn_r * rpd_r + rpn_i * rpd_i) * rpd;
+    rp_i = (rpn_i * rpd_r - rpn_r * rpd_i) * rpd;
+    return true;
+}
+PLT_DEV Fres fres_from_amp(float rs_r, float rs_i, float rp_r, float rp_i) {
     const float Rp = rp_r * rp_r + rp_i * rp_i, Rs = rs_r * rs_r + rs_i * rs_i;
     // rs * conj(rp)
     const float cr = rs_r * rp_r + rs_i * rp_i, ci = rs_i * rp_r - rs_r * rp_i;
     return Fres{(Rp + Rs) * 0.5f, (Rp - Rs) * 0.5f, cr, ci};
+}
+// Conductor reflection Mueller block (MuellerFresnelRConductor, Mueller.slang:111-116).
+PLT_DEV Fres fresnel_conductor_mueller(float cosI, float n, float k) {
+    float a, b, c, d;
+    if (!fresnel_conductor_amp(cosI, n, k, a, b, c, d)) return Fres{0.f, 0.f, 0.f, 0.f};
+    return fres_from_amp(a, b, c, d);
 }
 
 // ---------------------------------------------------------------------------------------

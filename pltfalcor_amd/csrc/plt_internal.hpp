@@ -4,6 +4,14 @@
 #include <string>
 #include "../../include/plt_abi.h"
 
+// A device context: the GPU ordinal and the spectral profiles (device copy) that materials and
+// BSDF batches reference by id.
+struct plt_context {
+    int device = 0;
+    plt_spectral_profile* d_profiles = nullptr;
+    uint32_t n_profiles = 0;
+};
+
 namespace plt {
 
 // Records a message as the calling thread's last error and returns `code`.

@@ -230,8 +230,13 @@ struct Mat {
     const plt_spectral_profile* pn;
     const plt_spectral_profile* pk;
     float extEta;
-    // coated
+    // coated (tau in um) / thin dielectric (tau in mm)
     float coatIor, tau;
+    // thin dielectric: birefringence and optic axis xy
+    float bir, oax, oay;
+    // multilayer stack
+    float mlIor[2], mlThick[2];
+    uint32_t layers;
     // grating
     Grating gr;
     // emission
@@ -285,6 +290,24 @@ PLT_DEV Mat setup_material(const plt_material& m, const plt_spectral_profile* pr
             r.gr.gy = sinf(dir);
             break;
         }
+        case PLT_MAT_COATED_CONDUCTOR:  // PLTCoatedConductorMaterial.slang:40-60
+            r.fs = Fractal::from_roughness(sy, 3.f);
+            r.tau = h16(m.data1[0]) * sz;
+            r.coatIor = h16(m.data1[3]);
+            break;
+        case PLT_MAT_THIN_DIELECTRIC:  // PLTThinDielectricMaterial.slang:40-56
+            r.tau = sw;
+            r.bir = r.col[0] * sx;
+            r.oax = r.col[1];
+            r.oay = r.col[2];
+            break;
+        case PLT_MAT_MULTILAYER_STACK:  // PLTMultiLayeredStackMaterial.slang:40-57
+            r.mlIor[0] = h16(m.data2[0]);
+            r.mlIor[1] = h16(m.data2[1]);
+            r.mlThick[0] = h16(m.data1[0]);
+            r.mlThick[1] = h16(m.data1[1]);
+            r.layers = m.data1[3];
+            break;
         default:
             break;
     }
@@ -299,6 +322,8 @@ PLT_DEV uint32_t mat_lobes(const Mat& m) {
                                      : (kLobeDeltaReflection | dt | kLobeScatterReflection | kLobeScatterTransmission);
         }
         case PLT_MAT_DIFFUSE: return kLobeDiffuseReflection;
+        case PLT_MAT_THIN_DIELECTRIC: return kLobeDeltaReflection | kLobeDeltaTransmission;
+        case PLT_MAT_MULTILAYER_STACK: return kLobeDeltaReflection;
         case PLT_MAT_GRATED_CONDUCTOR:
             return (m.fs.q == 0.f ? kLobeDeltaReflection : (kLobeDeltaReflection | kLobeScatterReflection)) |
                    kLobeDispersiveDeltaReflection;
@@ -311,6 +336,8 @@ PLT_DEV uint32_t mat_props(const Mat& m) {
         case PLT_MAT_DIELECTRIC: return kPropTransmissive | (m.gs.q < kQcutoffForMS ? kPropMWTransmissive : 0u);
         case PLT_MAT_DIFFUSE: return m.emission != PLT_NO_PROFILE ? kPropEmissive : 0u;
         case PLT_MAT_GRATED_CONDUCTOR: return kPropMWReflective;
+        case PLT_MAT_MULTILAYER_STACK: return kPropMWReflective;
+        case PLT_MAT_THIN_DIELECTRIC: return kPropTransmissive | kPropMWReflective | kPropMWTransmissive;
         default: return m.fs.q < kQcutoffForMS ? kPropMWReflective : 0u;
     }
 }
@@ -335,6 +362,275 @@ PLT_DEV void conductor_like_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam
     c.etaN = m.pn; c.etaK = m.pk; c.extEta = m.extEta; c.T = m.fs.T; c.gamma = m.fs.gamma; c.q = m.fs.q;
     c.spec[0] = m.col[0]; c.spec[1] = m.col[1]; c.spec[2] = m.col[2];
     conductor_eval<N>(c, wi, wo, lobe, beam);
+}
+
+// interfere (Interference.slang:45-57): two Stokes vectors combined per transverse component with
+// phasors ps, pp and mutual coherence coh; the result replaces a.
+PLT_DEV void interfere4(float a[4], const float b[4], float ps, float pp, float coh) {
+    const float S1 = (a[0] + a[1]) / 2.f, P1 = (a[0] - a[1]) / 2.f, S2 = (b[0] + b[1]) / 2.f, P2 = (b[0] - b[1]) / 2.f;
+    const float S = S1 + S2 + 2.f * sqrtf(fmaxf(0.f, S1 * S2)) * ps * coh;
+    const float P = P1 + P2 + 2.f * sqrtf(fmaxf(0.f, P1 * P2)) * pp * coh;
+    const float nrm = a[0] + b[0] > 1e-6f ? (S + P) / (a[0] + b[0]) : 0.f;
+    const float a2 = (a[2] + b[2]) * nrm, a3 = (a[3] + b[3]) * nrm;
+    a[0] = S + P;
+    a[1] = S - P;
+    a[2] = a2;
+    a[3] = a3;
+}
+
+// PLTCoatedConductorBxDF::eval (PLTCoatedConductorBSDF.slang:72-128): coat transmission in and
+// out around the conductor's Fresnel at the half vector; the direct lobe interferes with the
+// coat's own reflection (phase 2 tau / cos k eta + arg r per polarisation, mutual coherence
+// over the coat thickness).
+template <int N>
+PLT_DEV void coated_conductor_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    const bool isRefl = wi.z * wo.z > 0.f;
+    const bool isScatter = (lobe & kLobeScatterReflection) != 0, isDirect = (lobe & kLobeDeltaReflection) != 0;
+    const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
+    const float eta1 = coatior / m.extEta;
+    float cT;
+    (void)fresnel_dielectric_F(eta1, wi.z, cT);
+    const V3 wiIn = -refract_dir(wi, cT, eta1);
+    V3 woIn = v3(-wiIn.x, -wiIn.y, wiIn.z);
+    if (!isDirect) {
+        float cT2;
+        (void)fresnel_dielectric_F(eta1, wo.z, cT2);
+        woIn = -refract_dir(wo, cT2, eta1);
+    }
+    if (!isRefl || (!isScatter && !isDirect) || wiIn.z <= 0.f || woIn.z <= 0.f) { beam.set_zero(); return; }
+    float k[N], coh[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) k[j] = k2Pi / (1e-3f * beam.lambda[j]);
+    const V3 h = wiIn + woIn;
+    const V3 mh = normalize(h);
+    const float LdotH = fmaxf(0.f, dot(wiIn, mh));
+    const V3 Z = beam.to_local(v3(0.f, 0.f, 1.f));
+    beam.mutual_coherence(k, Z * m.tau, coh);
+    const Fres MiR = fresnel_dielectric_mueller(wi.z, eta1, true);
+    const Fres MiT = fresnel_dielectric_mueller(wi.z, eta1, false);
+    const Fres MoT = fresnel_dielectric_mueller(-woIn.z, eta1, false);
+    const float icoat = 1.f / coatior;
+    const float dist = 2.f * m.tau / wiIn.z;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j < beam.comps) {
+            const float n = (m.pn ? eval_spectrum(m.pn, beam.lambda[j]) : 0.f) * icoat;
+            const float kk = (m.pk ? eval_spectrum(m.pk, beam.lambda[j]) : 0.f) * icoat;
+            float rsr, rsi, rpr, rpi;
+            const bool ok = fresnel_conductor_amp(LdotH, n, kk, rsr, rsi, rpr, rpi);
+            const Fres R2 = ok ? fres_from_amp(rsr, rsi, rpr, rpi) : Fres{0.f, 0.f, 0.f, 0.f};
+            const Fres M = fres_mul(fres_mul(MiT, R2), MoT);
+            const float m00 = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]);
+            float a[4] = {beam.S[j][0], beam.S[j][1], beam.S[j][2], beam.S[j][3]};
+            float b[4] = {a[0], a[1], a[2], a[3]};
+            apply(M, a[0], a[1], a[2], a[3]);
+            if (isScatter) {
+                const float w = m00 * m.fs.one_minus_alpha(wiIn.z, woIn.z, k[j]) * fabsf(wi.z) * m.fs.pgf(k[j] * h.x, k[j] * h.y);
+                a[0] *= w; a[1] *= w; a[2] *= w; a[3] *= w;
+            } else {
+                const float w = m00 * m.fs.alpha(wiIn.z, woIn.z, k[j]);
+                a[0] *= w; a[1] *= w; a[2] *= w; a[3] *= w;
+                apply(MiR, b[0], b[1], b[2], b[3]);
+                // phaseShift(eta, dist, k, r) = sin(dist k eta + arg r) (Interference.slang:36-39)
+                const float pS = sinf(dist * k[j] * coatior + atan2f(rsi, rsr));
+                const float pP = sinf(dist * k[j] * coatior + atan2f(rpi, rpr));
+                interfere4(a, b, pS, pP, coh[j]);
+            }
+            beam.S[j][0] = a[0]; beam.S[j][1] = a[1]; beam.S[j][2] = a[2]; beam.S[j][3] = a[3];
+        }
+    }
+}
+
+// Uniaxial slab optics (Rendering/Materials/PLT/Birefringence.slang:33-197): y is the surface
+// normal, incidence in the yz plane, unit wavenumber.
+struct Biref {
+    PLT_DEV static float qo(float eo, float K) { return sqrtf(eo * eo - K * K); }
+    PLT_DEV static void qe(float eo, float ee, V3 A, float K, float& qp, float& qm) {
+        const float e = ee * ee - eo * eo;
+        const float G = A.y * A.y * e + eo * eo;
+        const float r = sqrtf(fmaxf(0.f, K * K * (A.x * A.x * e - ee * ee) + ee * ee * G));
+        const float c = A.z * A.y * K * e;
+        qp = (eo * r + c) / G;
+        qm = (-eo * r + c) / G;
+    }
+    PLT_DEV static V3 E_e(float eo, V3 A, float K, float q) {
+        return normalize(v3(A.x * eo * eo, K * A.z * q + A.y * (eo * eo - q * q), K * A.y * q + A.z * (eo * eo - K * K)));
+    }
+    PLT_DEV static V3 E_o(V3 A, float K, float q) { return normalize(v3(-A.y * K - A.z * q, A.x * K, A.x * q)); }
+    PLT_DEV static V3 poynting(float q, V3 E, float K) {
+        return normalize(v3(E.x * (K * E.z - q * E.y), K * E.z * E.y + q * (E.x * E.x + E.z * E.z),
+                            -q * E.z * E.y - K * (E.x * E.x + E.y * E.y)));
+    }
+};
+
+// PLTThinDielectricBxDF::handle_birefringence (PLTThinDielectricBSDF.slang:60-118): ordinary and
+// extraordinary transmission through the slab and their interference (mutual coherence over the
+// lateral o/e offset), applied to the (x, y) intensities of the beam.
+template <int N>
+PLT_DEV void thin_birefringence(const Mat& m, Beam<N>& beam, V3 wi, const float* k) {
+    const float oaz = sqrtf(fmaxf(0.f, 1.f - (m.oax * m.oax + m.oay * m.oay)));
+    const float phii = atan2f(wi.y, wi.x);
+    const float phiA = atan2f(m.oay, m.oax) - phii;
+    const V3 I = v3(0.f, fabsf(wi.z), sqrtf(1.f - wi.z * wi.z));
+    const float sA = sqrtf(1.f - oaz * oaz);
+    const V3 A = v3(cosf(phiA) * sA, sgnf(wi.z) * oaz, sinf(phiA) * sA);
+    const V3 A2 = v3(A.x, -A.y, A.z);
+    const V3 Z = beam.to_local(v3(cosf(phii), sinf(phii), 0.f));
+    const float ei = 1.f, eo = m.extEta - .5f * m.bir, ee = m.extEta + .5f * m.bir;
+    const float tauum = m.tau * 1e+3f;
+    const float ct = I.y, st = I.z;
+    // vectors_in_slab (:168-195)
+    float qepA, qemA;
+    const float Kin = ei * I.z;
+    Biref::qe(eo, ee, A, Kin, qepA, qemA);
+    const float qomA = -Biref::qo(eo, Kin);
+    const V3 Io = Biref::poynting(qomA, Biref::E_o(A, Kin, qomA), Kin);
+    const V3 Ie = Biref::poynting(qemA, Biref::E_e(eo, A, Kin, qemA), Kin);
+    const V3 We = normalize(v3(0.f, qemA, -Kin));
+    const float e_eff = Kin / sqrtf(fmaxf(0.f, 1.f - We.y * We.y));
+    // fresnel_iso_aniso (:98-125), transmission coefficients
+    float tso, tse, tpo, tpe;
+    {
+        const float K = ei * st;
+        const float qi = -Biref::qo(ei, K);
+        const float q2om = -Biref::qo(eo, K);
+        float qp, q2em;
+        Biref::qe(eo, ee, A, K, qp, q2em);
+        const V3 E2om = Biref::E_o(A, K, q2om);
+        const V3 E2em = Biref::E_e(eo, A, K, q2em);
+        const float A_1 = -E2em.x * E2om.y * K * ct;
+        const float A_2 = -E2em.y * E2om.x * K * ct;
+        const float C_1 = qi * ct - K * st;
+        const float C_2 = E2em.x * E2om.z * (C_1 + q2om * ct) - A_1;
+        const float C_3 = E2em.z * E2om.x * (C_1 + q2em * ct) - A_2;
+        const float Ns = C_2 * (qi + q2em) - C_3 * (qi + q2om);
+        tpo = -(2.f * E2em.x * ct * (q2em + qi) * C_1) / Ns;
+        tpe = (2.f * E2om.x * ct * (q2om + qi) * C_1) / Ns;
+        tso = -(2.f * qi * (E2em.y * K * ct + E2em.z * C_1 + E2em.z * q2em * ct)) / Ns;
+        tse = (2.f * qi * (E2om.y * K * ct + E2om.z * C_1 + E2om.z * q2om * ct)) / Ns;
+    }
+    // fresnel_aniso_iso (:126-166) with the mirrored axis, transmission coefficients
+    float tos, top, tes, tep;
+    {
+        const float K = ei * st;
+        const float q2om = -Biref::qo(ei, K);
+        const float qop = Biref::qo(eo, K);
+        float qep, qem;
+        Biref::qe(eo, ee, A2, K, qep, qem);
+        const V3 Eop = Biref::E_o(A2, K, qop), Eom = Biref::E_o(A2, K, -qop);
+        const V3 Eep = Biref::E_e(eo, A2, K, qep), Eem = Biref::E_e(eo, A2, K, qem);
+        const float B_1 = -Eep.x * K * ct * (q2om - qep);
+        const float B_2 = -Eom.x * K * ct * (q2om + qop);
+        const float B_3 = -Eop.x * K * ct * (qop - q2om);
+        const float D_1 = K * st - q2om * ct;
+        const float D_2 = Eep.z * (D_1 + qep * ct);
+        const float D_3 = Eem.z * (D_1 + qem * ct);
+        const float D_4 = Eop.z * (D_1 + qop * ct);
+        const float N2 = Eep.x * (qep - q2om) * D_4 + Eop.x * (q2om - qop) * D_2 + Eop.y * B_1 + Eep.y * B_3;
+        const float roo = -(Eep.x * Eom.z * (q2om - qep) * (qop * ct - D_1) + Eom.x * (q2om + qop) * D_2 + Eom.y * B_1 - Eep.y * B_2) / N2;
+        const float roe = -(-Eom.x * Eop.z * (qop * (qop * ct + K * st) + q2om * D_1) +
+                            Eom.z * Eop.x * (qop * (qop * ct - K * st) + q2om * D_1) + Eop.y * B_2 + Eom.y * B_3) / N2;
+        tos = (Eep.x * qep * roe - Eom.x * qop + Eop.x * qop * roo) / q2om;
+        top = -(Eom.z * qop - Eom.y * K - (Eep.y * K + Eep.z * qep) * roe - (Eop.y * K + Eop.z * qop) * roo) / D_1;
+        const float reo = (Eem.x * (qem - q2om) * (D_2 + Eep.y * K * ct) + Eep.x * (q2om - qep) * (D_3 + Eem.y * K * ct)) / N2;
+        const float ree = -(Eem.x * (qem - q2om) * D_4 - Eem.x * Eop.y * K * ct * (q2om - qem) + Eem.y * B_3 + Eop.x * (q2om - qop) * D_3) / N2;
+        tes = (Eem.x * qem + Eep.x * qep * ree + Eop.x * qop * reo) / q2om;
+        tep = -(Eem.z + Eep.z * ree + Eop.z * reo) / ct;
+    }
+    const float ss = tso * tos + tse * tes, sp = tso * top + tse * tep;
+    const float ps = tpo * tos + tpe * tes, pp = tpo * top + tpe * tep;
+    const float Ioz = fabsf(tauum / Io.y) * Io.z, Iez = fabsf(tauum / Ie.y) * Ie.z;
+    const float OPLo = fabsf(tauum / Io.y) * eo, OPLe = fabsf(tauum / Ie.y) * e_eff;
+    const float oez = Ioz - Iez;
+    float coh[N], nLx[N], nLy[N], newL[N];
+    beam.mutual_coherence(k, Z * oez, coh);
+    bool allPos = true, allFinite = true;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const float t = coh[j] * 2.f * sinf(-k[j] * (ei * oez * I.z + OPLo - OPLe));
+        const float Lx = .5f * (beam.S[j][0] + beam.S[j][1]), Ly = .5f * (beam.S[j][0] - beam.S[j][1]);
+        const float sq = sqrtf(Lx * Ly);
+        nLx[j] = fmaxf(0.f, ss * ss * Lx + ps * ps * Ly + ss * ps * t * sq);
+        nLy[j] = fmaxf(0.f, sp * sp * Lx + pp * pp * Ly + sp * pp * t * sq);
+        newL[j] = nLx[j] + nLy[j];
+        allFinite = allFinite && isfinite(newL[j]);
+        allPos = allPos && beam.S[j][0] > 1e-6f && isfinite(newL[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const float nrm = allPos ? newL[j] / beam.S[j][0] : 0.f;
+        beam.S[j][0] = allFinite ? newL[j] : 0.f;
+        beam.S[j][1] = allFinite ? nLx[j] - nLy[j] : 0.f;
+        beam.S[j][2] *= nrm;
+        beam.S[j][3] *= nrm;
+    }
+}
+
+// PLTThinDielectricBxDF::eval (PLTThinDielectricBSDF.slang:120-149): reflection R + T R (1-R)^-1 T,
+// transmission T (1-R)^-1 T (Mueller blocks), birefringence in transmission.
+template <int N>
+PLT_DEV void thin_dielectric_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    const float eta = m.extEta;
+    const bool isRefl = wi.z * wo.z > 0.f;
+    const bool isDirect = (lobe & kLobeDelta) != 0;
+    const bool hasT = (lobe & kLobeTransmission) != 0, hasR = (lobe & kLobeReflection) != 0;
+    if ((isRefl && eta == 1.f) || wi.z * wo.z == 0.f || !isDirect || (!hasT && !isRefl) || (!hasR && isRefl)) {
+        beam.set_zero();
+        return;
+    }
+    const float c = fabsf(wi.z);
+    const Fres R = fresnel_dielectric_mueller(c, eta, true);
+    const Fres T = fresnel_dielectric_mueller(c, eta, false);
+    const Fres iR = fres_inv_one_minus(R);
+    const Fres M = isRefl ? (iR.a > 0.f ? fres_add(R, fres_mul(fres_mul(T, R), fres_mul(iR, T))) : R)
+                          : (iR.a > 0.f ? fres_mul(T, fres_mul(iR, T)) : fres_mul(T, T));
+    if (m.bir > 0.f && !isRefl) {
+        float k[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) k[j] = k2Pi / (1e-3f * beam.lambda[j]);
+        thin_birefringence<N>(m, beam, wi, k);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        if (j < beam.comps) apply(M, beam.S[j][0], beam.S[j][1], beam.S[j][2], beam.S[j][3]);
+}
+
+// PLTMultiLayeredStackBxDF::eval (PLTMultiLayeredStackBSDF.slang:58-95): Airy recursion of the
+// amplitude G <- (R + G p) / (1 + R G p) over the layers from the base, output Re(G)^2. The
+// recursion runs in FP64 (20 complex steps per wavelength at the workload's stack).
+template <int N>
+PLT_DEV void multilayer_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    const bool isRefl = wi.z * wo.z > 0.f;
+    if ((lobe & kLobeDelta) == 0 || !isRefl) { beam.set_zero(); return; }
+    const float eta1 = m.mlIor[0] / m.extEta, eta2 = m.mlIor[1] / m.mlIor[0];
+    float cT0, cT1, cT2;
+    const float Rout = sqrtf(fresnel_dielectric_F(eta1, wi.z, cT0));
+    const float r0 = sqrtf(fresnel_dielectric_F(eta2, -cT0, cT1));
+    const float r1 = sqrtf(fresnel_dielectric_F(eta2, cT1, cT2));
+    const double c0 = fabs(double(cT0)), c1 = fabs(double(cT1));
+    const double ph0 = 2.0 * double(m.mlThick[0]) / c0 * double(m.mlIor[0]);
+    const double ph1 = 2.0 * double(m.mlThick[1]) / c1 * double(m.mlIor[1]);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j < beam.comps) {
+            const double k = 6.283185307179586 / (1e-3 * double(beam.lambda[j]));
+            double gr = sqrt(double(rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]))), gi = 0.0;
+            for (uint32_t l = 0; l < m.layers; ++l) {
+                const bool odd = (l & 1u) != 0;
+                double ps, pc;
+                sincos((odd ? ph1 : ph0) * k, &ps, &pc);
+                const double R = l + 1 == m.layers ? double(Rout) : double(odd ? r1 : r0);
+                const double tr = gr * pc - gi * ps, ti = gr * ps + gi * pc;
+                // (R + G) / (1 + R G)
+                const double nr = R + tr, ni = ti, dr = 1.0 + R * tr, di = R * ti;
+                const double id = 1.0 / (dr * dr + di * di);
+                gr = (nr * dr + ni * di) * id;
+                gi = (ni * dr - nr * di) * id;
+            }
+            const float w = float(gr * gr);
+            beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
+        }
+    }
 }
 
 template <int N>
@@ -437,6 +733,15 @@ PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& 
             }
             break;
         }
+        case PLT_MAT_COATED_CONDUCTOR:
+            coated_conductor_eval<N>(m, wi, wo, lobe, beam);
+            break;
+        case PLT_MAT_THIN_DIELECTRIC:
+            thin_dielectric_eval<N>(m, wi, wo, lobe, beam);
+            break;
+        case PLT_MAT_MULTILAYER_STACK:
+            multilayer_eval<N>(m, wi, wo, lobe, beam);
+            break;
         case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:68-107
             const bool isDirect = (lobe & kLobeDelta) != 0;
             float a[N], lam[N];
@@ -554,6 +859,7 @@ PLT_DEV void mat_lambda_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 woW, uint3
                 for (int j = 0; j < N; ++j) ret[j] = m.fs.pdf(a1, a0, k2Pi / (1e-3f * beam.lambda[j]));
             }
             break;
+        case PLT_MAT_COATED_CONDUCTOR:          // PLTCoatedConductorBSDF.slang:190-213 (same as below)
         case PLT_MAT_COATED_OPAQUE_DIELECTRIC:  // PLTCoatedOpaqueDielectricBSDF.slang:190-213
             if (isScatter) {
                 const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
@@ -610,6 +916,7 @@ PLT_DEV float mat_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 toLightW, float 
             return wo.z * k1_Pi;  // cosine_hemisphere_concentric_pdf(woLocal), woLocal = toLocal(sd.V)
         case PLT_MAT_OPAQUE_DIELECTRIC:
             return m.fs.one_minus_alpha(wi.z, wi.z, kMeanK) * m.fs.pdf(wi, wo, kMeanK);
+        case PLT_MAT_COATED_CONDUCTOR:
         case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {
             if (wi.z <= 0.f || wo.z <= 0.f) return 0.f;
             const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
@@ -696,7 +1003,29 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             valid = true;
             break;
         }
-        case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {  // PLTCoatedOpaqueDielectricBSDF.slang:131-180
+        case PLT_MAT_THIN_DIELECTRIC: {  // PLTThinDielectricBSDF.slang:164-183 (both lobes allowed)
+            float cTd;
+            const float Fd = fresnel_dielectric_F(m.extEta, fabsf(wi.z), cTd);
+            const float uR = sg.next1D_if(Fd > 0.f);
+            const bool isRefl = Fd > 0.f && uR <= Fd;
+            wo = isRefl ? v3(-wi.x, -wi.y, wi.z) : -wi;
+            weight = 1.f;
+            pdf = isRefl ? Fd : 1.f - Fd;
+            lobe = isRefl ? kLobeDeltaReflection : kLobeDeltaTransmission;
+            valid = true;
+            break;
+        }
+        case PLT_MAT_MULTILAYER_STACK: {  // PLTMultiLayeredStackBSDF.slang:102-115
+            if (wi.z <= 0.f) break;
+            pdf = 1.f;
+            wo = v3(-wi.x, -wi.y, wi.z);
+            lobe = kLobeDeltaReflection;
+            weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
+            valid = wi.z * wo.z > 0.f;
+            break;
+        }
+        case PLT_MAT_COATED_CONDUCTOR:
+        case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {  // PLTCoatedOpaqueDielectricBSDF.slang:131-180 (= coated conductor's)
             if (wi.z <= 0.f) break;
             const float coatior = m.extEta + (m.coatIor - m.extEta) * (fminf(fmaxf(m.tau, 0.f), .05f) / .05f);
             const float eta1 = coatior / m.extEta;

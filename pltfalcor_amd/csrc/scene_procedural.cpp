@@ -154,6 +154,59 @@ plt_material grated_conductor(float pitch, float direction, uint32_t pn, uint32_
     return m;
 }
 
+// PLTCoatedConductorMaterial (PLTCoatedConductorMaterial.cpp:45-110): specular colour 1, ext IoR 1,
+// coat thickness scale 1, then roughness, coat IoR and coat thickness (um) setters.
+plt_material coated_conductor(float roughness, float coat_ior, float thickness, uint32_t pn, uint32_t pk) {
+    plt_material m = blank(PLT_MAT_COATED_CONDUCTOR);
+    for (int i = 0; i < 4; ++i) m.base_color[i] = f32_to_h(1.f);
+    m.ior = f32_to_h(1.f);
+    m.specular[1] = f32_to_h(roughness);
+    m.specular[2] = f32_to_h(1.f);
+    m.data1[3] = f32_to_h(coat_ior);
+    m.data1[0] = f32_to_h(thickness);
+    m.ior_n_profile = pn;
+    m.ior_k_profile = pk;
+    return m;
+}
+// PLTThinDielectricMaterial (PLTThinDielectricMaterial.cpp:45-90): IoR, thickness (mm) in
+// specular.w, birefringence in baseColor.x with the scale specular.x set to 1, optic axis
+// (normalised, z >= 0) xy in baseColor.yz, baseColor.a = 1 (opacity 0.5, not masked).
+plt_material thin_dielectric(float ior, float thickness_mm, float birefringence, float ax, float ay, float az) {
+    plt_material m = blank(PLT_MAT_THIN_DIELECTRIC);
+    m.ior = f32_to_h(ior);
+    const float l = std::sqrt(ax * ax + ay * ay + az * az);
+    const float s = az < 0.f ? -1.f / l : 1.f / l;
+    m.base_color[0] = f32_to_h(birefringence);
+    m.base_color[1] = f32_to_h(ax * s);
+    m.base_color[2] = f32_to_h(ay * s);
+    m.base_color[3] = f32_to_h(1.f);
+    m.specular[0] = f32_to_h(1.f);
+    m.specular[3] = f32_to_h(thickness_mm);
+    return m;
+}
+// PLTMultiLayeredStackMaterial (PLTMultiLayeredStackMaterial.cpp:45-115): base colour 1, ext IoR 1,
+// layer IoRs in data2.xy, thicknesses (um) in data1.xy, layer count as raw bits in data1.w.
+plt_material multilayer(float ior1, float ior2, float t1, float t2, uint32_t layers) {
+    plt_material m = blank(PLT_MAT_MULTILAYER_STACK);
+    for (int i = 0; i < 4; ++i) m.base_color[i] = f32_to_h(1.f);
+    m.ior = f32_to_h(1.f);
+    m.data2[0] = f32_to_h(ior1);
+    m.data2[1] = f32_to_h(ior2);
+    m.data1[0] = f32_to_h(t1);
+    m.data1[1] = f32_to_h(t2);
+    m.data1[3] = uint16_t(layers);
+    return m;
+}
+// PLTOpaqueDielectricMaterial (PLTOpaqueDielectricMaterial.cpp): albedo and roughness.
+plt_material opaque_dielectric(float r, float g, float b, float roughness) {
+    plt_material m = blank(PLT_MAT_OPAQUE_DIELECTRIC);
+    m.base_color[0] = f32_to_h(r); m.base_color[1] = f32_to_h(g); m.base_color[2] = f32_to_h(b);
+    m.base_color[3] = f32_to_h(1.f);
+    m.ior = f32_to_h(1.f);
+    m.specular[1] = f32_to_h(roughness);
+    return m;
+}
+
 void add_quad(HostScene& s, const float c[4][3], float nrm_y, uint32_t material) {
     const uint32_t mesh = uint32_t(s.mesh_first_tri.size());
     const uint32_t v0 = uint32_t(s.positions.size() / 3);
@@ -169,7 +222,7 @@ void add_quad(HostScene& s, const float c[4][3], float nrm_y, uint32_t material)
 }  // namespace
 
 int build_procedural_scene(const plt_procedural_scene_desc& d, HostScene& s) {
-    if (d.subdivision > 7 || d.sphere_count > 65535 || d.material_kinds < 1 || d.material_kinds > 4 ||
+    if (d.subdivision > 7 || d.sphere_count > 65535 || d.material_kinds < 1 || d.material_kinds > 8 ||
         !(d.ground_size > 0.f) || !(d.aspect > 0.f))
         return PLT_EINVAL;
 
@@ -239,7 +292,14 @@ int build_procedural_scene(const plt_procedural_scene_desc& d, HostScene& s) {
             case 1: s.materials.push_back(conductor(0.3f * p0, 0, 1)); break;
             case 2: s.materials.push_back(coated_opaque(.2f + .7f * p0, .2f + .7f * p1, .2f + .7f * p2, .05f + .25f * p3, 1.5f,
                                                         1.f + 4.f * p4)); break;
-            default: s.materials.push_back(grated_conductor(0.5f + 1.5f * p0, 3.14159265f * p1, 0, 1)); break;
+            case 3: s.materials.push_back(grated_conductor(0.5f + 1.5f * p0, 3.14159265f * p1, 0, 1)); break;
+            // thickness U[0.2, 2] um (in mm), birefringence U[0.01, 0.2], IoR U[1.3, 1.8]
+            case 4: s.materials.push_back(thin_dielectric(1.3f + .5f * p3, 2e-4f + 1.8e-3f * p0, .01f + .19f * p1,
+                                                          .3f * (2.f * p2 - 1.f), .3f * (2.f * p4 - 1.f), 1.f)); break;
+            // 20 layers of 1.38 / 2.3, quarter-wave at 550 nm
+            case 5: s.materials.push_back(multilayer(1.38f, 2.3f, .55f / (4.f * 1.38f), .55f / (4.f * 2.3f), 20)); break;
+            case 6: s.materials.push_back(coated_conductor(.05f + .25f * p0, 1.5f, 1.f + 4.f * p1, 0, 1)); break;
+            default: s.materials.push_back(opaque_dielectric(.2f + .7f * p0, .2f + .7f * p1, .2f + .7f * p2, .05f + .3f * p3)); break;
         }
         const uint32_t mesh = uint32_t(s.mesh_first_tri.size());
         const uint32_t vbase = uint32_t(s.positions.size() / 3);

@@ -14,7 +14,7 @@ from ._lib import Camera, Material, ProceduralSceneDesc, SceneArrays  # noqa: F4
 
 # Workload scenes: sphere count / subdivision / material kinds
 PATH_1 = dict(seed=1, sphere_count=64, subdivision=5, material_kinds=4)   # 512x512, ~1.31M triangles
-PATH_2 = dict(seed=1, sphere_count=256, subdivision=5, material_kinds=4)  # 1920x1080, ~5.24M triangles
+PATH_2 = dict(seed=1, sphere_count=256, subdivision=5, material_kinds=8)  # 1920x1080, ~5.24M triangles
 
 
 class Scene:

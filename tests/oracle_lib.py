@@ -226,3 +226,45 @@ def alias_table(scene: dict):
     n = fn(ptr(pos, C.c_float), idx.shape[0], ptr(idx, C.c_uint32), ptr(tm, C.c_uint32), ptr(mm, C.c_uint32),
            scene["materials"], ptr(thr, C.c_float), ptr(red, C.c_uint32), ptr(perm, C.c_uint32), ptr(pdf, C.c_float), cap)
     return thr[:n], red[:n], perm[:n], pdf[:n]
+
+
+def material_eval(material: bytes, profiles: bytes, n_profiles: int, wi, wo, lobe: int, lambdas, stokes, area: float,
+                  dbl: bool = True):
+    """One material query in the local frame (N = +z) through the oracle's material interface.
+    Returns (stokes_out [4, 4], pdf, lambda_pdf [4])."""
+    l = lib()
+    fn = l.oracle_material_eval
+    fn.restype = C.c_int
+    D = C.POINTER(C.c_double)
+    fn.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, D, D, C.c_uint32, D, D, C.c_double, C.c_int, D, D, D]
+    a = lambda v, n: np.ascontiguousarray(np.asarray(v, np.float64).reshape(n))  # noqa: E731
+    wi_, wo_, lam, st = a(wi, 3), a(wo, 3), a(lambdas, 4), a(stokes, 16)
+    out = np.zeros(16)
+    pdf = np.zeros(1)
+    lp = np.zeros(4)
+    fn(material, profiles, n_profiles, wi_.ctypes.data_as(D), wo_.ctypes.data_as(D), lobe, lam.ctypes.data_as(D),
+       st.ctypes.data_as(D), area, int(dbl), out.ctypes.data_as(D), pdf.ctypes.data_as(D), lp.ctypes.data_as(D))
+    return out.reshape(4, 4), float(pdf[0]), lp
+
+
+def material_sample(material: bytes, profiles: bytes, n_profiles: int, wi, wavelength, n: int = None, frame: int = 0):
+    """FP32 samples, query i with direction wi[i] (or one shared wi) and generator (i, 0, frame):
+    (wo [n, 3], pdf [n], weight [n], lobe [n], valid [n])."""
+    l = lib()
+    fn = l.oracle_material_sample
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, _F, _F, C.c_uint32, C.c_uint32, _F, _F, _F, _U32, _U32]
+    w = np.asarray(wi, np.float32)
+    if w.ndim == 1:
+        w = np.tile(w, (n, 1))
+    w = np.ascontiguousarray(w.reshape(-1, 3))
+    n = w.shape[0]
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(wavelength, np.float32), (n,)))
+    wo = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    weight = np.zeros(n, np.float32)
+    lobe = np.zeros(n, np.uint32)
+    valid = np.zeros(n, np.uint32)
+    fn(material, profiles, n_profiles, ptr(w, C.c_float), ptr(lam, C.c_float), n, frame, ptr(wo, C.c_float),
+       ptr(pdf, C.c_float), ptr(weight, C.c_float), ptr(lobe, C.c_uint32), ptr(valid, C.c_uint32))
+    return wo, pdf, weight, lobe, valid

@@ -287,6 +287,9 @@ BASE_OPTS = {"maxBounces": 6, "doMNEE": 0, "HWSS": 4}
     dict(name="noNEE", scene=dict(sphere_count=12, subdivision=3, kinds=4, seed=7), opts={"doNEE": 0}),
     dict(name="noIS-noHWSSMIS", scene=dict(sphere_count=12, subdivision=3, kinds=4, seed=8),
          opts={"doImportanceSampleEmitters": 0, "HWSSDoMIS": 0}),
+    dict(name="8-materials-d8", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=3), opts={"maxBounces": 8}),
+    dict(name="8-materials-noHWSSMIS", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=9),
+         opts={"maxBounces": 8, "HWSSDoMIS": 0}),
 ], ids=lambda c: c["name"])
 def test_gpu_pltpt_matches_oracle(gpu, case):
     from pltfalcor_amd.scene import Scene
