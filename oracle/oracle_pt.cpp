@@ -48,7 +48,10 @@ struct Options {
     uint32_t width, height, frame, maxBounces;
     int doNEE, doMIS, doRR, hwssMIS, isEmitters, directLights, emissiveLights;
     float sourcingArea, maxOmega;
+    int doMNEE = 0, mneeMaxOccluders = 2, mneeMaxIterations = 60;
+    float mneeThreshold = 5e-5f;
 };
+constexpr int kMaxMSOccluders = 8;
 
 // ndir_to_oct_equal_area_snorm / oct_to_ndir_equal_area_snorm (MathHelpers.slang:199-240)
 V2<F> ndirToOct(V3<F> n) {
@@ -80,7 +83,7 @@ enum : uint32_t { RF_LastDelta = 0x1, RF_Dispersive = 0x2, RF_OnlyDelta = 0x4, R
 
 // getVertexData + prepareShadingData (Scene.slang:530-567, MaterialFactory.slang:79-130,
 // ShadingUtils.slang:126-153). No tangents -> perp_stark frame.
-SD<F> shadingData(const OScene& s, uint32_t tri, float b1, float b2, V3<F> viewDir) {
+SD<F> shadingData(const OScene& s, uint32_t tri, float b1, float b2, V3<F> viewDir, const V3<F>* posOverride = nullptr) {
     const V3<F> bary = {F(1) - b1 - b2, b1, b2};
     V3<F> p[3], n[3];
     for (int k = 0; k < 3; ++k) {
@@ -92,6 +95,7 @@ SD<F> shadingData(const OScene& s, uint32_t tri, float b1, float b2, V3<F> viewD
     sd.posW = p[0] * bary.x;
     sd.posW = sd.posW + p[1] * bary.y;
     sd.posW = sd.posW + p[2] * bary.z;
+    if (posOverride) sd.posW = *posOverride;
     V3<F> nw = n[0] * bary.x;
     nw = nw + n[1] * bary.y;
     nw = nw + n[2] * bary.z;
@@ -218,19 +222,169 @@ void samplePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, std
 // ---------------------------------------------------------------- solve pass
 struct SolveData { V3<F> Lrgb{0, 0, 0}; V3<F> pathOrigin, pathNormal; float wavelength; };
 
+// forwardUnwindStep (pltpt_solve.rt.slang:48-66); posW optionally overridden (manifold vertices).
+void unwindStep(const OScene& s, const Options& o, uint32_t tri, float b1, float b2, const V3<F>* posW, V3<F> wo, uint32_t lobe, Beam& beam) {
+    const SD<F> sd = shadingData(s, tri, b1, b2, -beam.r, posW);
+    beam.propagate(sd.posW);
+    const Instance<F> in = instanceAt(s, sd.materialID);
+    materialEval<F, HW>(in, sd, wo, lobe, beam);
+    if (o.hwssMIS) {
+        float pd[HW];
+        materialLambdaPdf<F, HW>(in, sd, wo, lobe, beam, pd);
+        for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+    }
+}
 void forwardUnwind(const OScene& s, const Options& o, const std::vector<Bounce>& rec, int bounceIdx, Beam& beam) {
     for (int p = bounceIdx - 1; p >= 0; --p) {
         const Bounce& b = rec[size_t(p)];
-        const V3<F> wo = b.wi();
-        const SD<F> sd = shadingData(s, b.tri, b.b1, b.b2, -beam.r);
-        beam.propagate(sd.posW);
+        unwindStep(s, o, b.tri, b.b1, b.b2, nullptr, b.wi(), b.lobe, beam);
+    }
+}
+
+// ---------------------------------------------------------------- manifold NEE (Manifolds.slangh)
+struct MSHit { uint32_t tri; float b1, b2; V3<F> posW; uint32_t lobe; };
+struct MSData {  // ManifoldSampleData (:57-81)
+    MSHit hits[kMaxMSOccluders];
+    V3<F> woLastHitToLight{0, 0, 0};
+    int hitCount = 0;
+    int iterations = 0;
+    float wavelength = 0, pdf = 1;
+    bool dispersive = false;
+};
+OHit shadowRayClosest(const OScene& s, V3<F> org, V3<F> dir, float tmin, float tmax) {
+    ORay r;
+    r.o[0] = org.x; r.o[1] = org.y; r.o[2] = org.z;
+    r.d[0] = dir.x; r.d[1] = dir.y; r.d[2] = dir.z;
+    r.tmin = tmin; r.tmax = tmax;
+    return s.bvh.closest(s.mesh, r, false);
+}
+bool visible(const OScene& s, V3<F> org, V3<F> dir, float tmin, float tmax) {
+    ORay r;
+    r.o[0] = org.x; r.o[1] = org.y; r.o[2] = org.z;
+    r.d[0] = dir.x; r.d[1] = dir.y; r.d[2] = dir.z;
+    r.tmin = tmin; r.tmax = tmax;
+    return s.bvh.closest(s.mesh, r, true).tri == 0xFFFFFFFFu;
+}
+DSG<F> dsgAt(const OScene& s, uint32_t tri, float b1, float b2) {
+    V3<F> P[3], Nv[3];
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = s.idx[3 * size_t(tri) + k];
+        P[k] = V3<F>{s.pos[3 * size_t(v)], s.pos[3 * size_t(v) + 1], s.pos[3 * size_t(v) + 2]};
+        Nv[k] = V3<F>{s.nrm[3 * size_t(v)], s.nrm[3 * size_t(v) + 1], s.nrm[3 * size_t(v) + 2]};
+    }
+    return makeDSG<F>(P, Nv, b1, b2);
+}
+template <class S>
+bool manifoldSampleVertex(uint32_t tri, float b1, float b2, const SD<F>& sd, const Instance<F>& in, V3<F> lightDir, S& sg, MSData& d) {  // :84-105
+    BSDFSample smp;
+    if (!materialSampleHinted(in, sd, sg, smp, d.wavelength, Lobe_Delta, lightDir)) return false;
+    d.pdf *= smp.pdf;
+    d.hits[d.hitCount] = MSHit{tri, b1, b2, sd.posW, smp.lobe};
+    d.dispersive |= (smp.lobe & Lobe_DispersiveDelta) != 0;
+    ++d.hitCount;
+    return true;
+}
+template <class S>
+bool traceMNEEOccluders(const OScene& s, const Options& o, V3<F> org, V3<F> dir, float tmax, int maxMSBounces, S& sg, MSData& d) {  // :107-150
+    const V3<F> wi = -dir;
+    for (;;) {
+        const OHit h = shadowRayClosest(s, org, dir, 0.f, tmax);
+        if (h.tri == 0xFFFFFFFFu) return true;
+        if (d.hitCount >= o.mneeMaxOccluders || d.hitCount >= maxMSBounces) return false;
+        const V3<F> posW = org + dir * h.t;
+        const SD<F> sd = shadingData(s, h.tri, h.b1, h.b2, wi, &posW);
         const Instance<F> in = instanceAt(s, sd.materialID);
-        materialEval<F, HW>(in, sd, wo, b.lobe, beam);
-        if (o.hwssMIS) {
-            float pd[HW];
-            materialLambdaPdf<F, HW>(in, sd, wo, b.lobe, beam, pd);
-            for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+        if (!(getPropertyFlags(in) & Prop_MWT)) return false;
+        if (!manifoldSampleVertex(h.tri, h.b1, h.b2, sd, in, -wi, sg, d) || (d.hits[d.hitCount - 1].lobe & Lobe_Transmission) == 0) return false;
+        org = rayOrigin(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);
+        tmax -= h.t;
+    }
+}
+V2<F> sphcoords(V3<F> w) {  // :314-319
+    const F theta = std::acos(clampT(w.z, F(-1), F(1)));
+    F phi = std::atan2(w.y, w.x);
+    phi += phi < F(0) ? K<F>::_2PI : F(0);
+    return {theta, phi};
+}
+// Newton step on the angle-difference constraint (:283-312). Matrices as rows (dTheta, dPhi).
+bool manifoldOneStep(const PDV<F>& w, const KPDV<F>& ws, const PDV<F>& p, float& lenC, V3<F>& stepDir) {
+    const V2<F> so = sphcoords(w.v), si = sphcoords(ws.v);
+    const F dt = so.x - si.x;
+    F dp = so.y - si.y;
+    dp = std::fabs(dp) > K<F>::PI ? dp - sgn(dp) * K<F>::_2PI : dp;
+    lenC = std::sqrt(dt * dt + dp * dp);
+    auto derivs = [](V3<F> v, V3<F> ddu, V3<F> ddv, F out[2][2]) {  // :321-333
+        const F dAcos = F(-1) / std::sqrt(hmax(F(0), F(1) - sqr(v.z)));
+        const F invNorm = F(1) / (v.x * v.x + v.y * v.y);
+        out[0][0] = dAcos * ddu.z; out[0][1] = dAcos * ddv.z;
+        out[1][0] = invNorm * (v.x * ddu.y - v.y * ddu.x);
+        out[1][1] = invNorm * (v.x * ddv.y - v.y * ddv.x);
+    };
+    F a[2][2], b[2][2];
+    derivs(w.v, w.ddu, w.ddv, a);
+    derivs(ws.v, ws.ddu, ws.ddv, b);  // the wavelength column of the 2x3 form is not used
+    const M2<F> J = m2<F>(a[0][0] - b[0][0], a[0][1] - b[0][1], a[1][0] - b[1][0], a[1][1] - b[1][1]);
+    M2<F> inv;
+    if (!tryInvert2(J, inv)) return false;
+    const F dx = inv.m[0][0] * dt + inv.m[0][1] * dp, dy = inv.m[1][0] * dt + inv.m[1][1] * dp;
+    stepDir = p.ddu * dx + p.ddv * dy;
+    return true;
+}
+template <class S>
+bool manifoldNEESample(const OScene& s, const Options& o, V3<F> visOrg, V3<F> visDir, float visTmax, V3<F> origin, V3<F> lightPos,
+                       V3<F> lightDir, int maxMSBounces, S& sg, MSData& d) {  // :165-281 (area lights: not distant)
+    if (!traceMNEEOccluders(s, o, visOrg, visDir, visTmax, maxMSBounces, sg, d)) return false;
+    if (d.hitCount == 0) return true;
+    float beta[kMaxMSOccluders];
+    for (int i = 0; i < kMaxMSOccluders; ++i) beta[i] = 1.f;
+    bool success = false;
+    while (!success) {
+        success = true;
+        for (int idx = 0; idx < d.hitCount; ++idx) {
+            MSHit& h = d.hits[idx];
+            const DSG<F> dsg = dsgAt(s, h.tri, h.b1, h.b2);
+            const V3<F> p0 = idx == 0 ? origin : d.hits[idx - 1].posW;
+            const PDV<F> pdv0 = dsg.pdw(p0);
+            const V3<F> p2 = idx + 1 == d.hitCount ? lightPos : d.hits[idx + 1].posW;
+            const PDV<F> pdv2 = dsg.pdw(p2);
+            const bool woIsIn = dot(pdv2.v, dsg.n.v) < F(0);
+            const PDV<F> wi = woIsIn ? pdv0 : pdv2, wo = woIsIn ? pdv2 : pdv0;
+            const SD<F> sd = shadingData(s, h.tri, h.b1, h.b2, wi.v);
+            const Instance<F> in = instanceAt(s, sd.materialID);
+            const KPDV<F> wos = materialPartialDerivatives(in, wi, dsg, h.lobe, d.wavelength);
+            if (wos.v.x == F(0) && wos.v.y == F(0) && wos.v.z == F(0)) return false;
+            float lenC;
+            V3<F> stepDir;
+            if (!manifoldOneStep(wo, wos, dsg.p, lenC, stepDir)) return false;
+            if (lenC < o.mneeThreshold) continue;
+            success = false;
+            for (;;) {  // line search: project the proposal onto the surface
+                if (++d.iterations > o.mneeMaxIterations) return false;
+                const V3<F> proposal = h.posW - stepDir * beta[idx];
+                V3<F> dirTo = proposal - p0;
+                const F dist = length(dirTo);
+                dirTo = dirTo / dist;
+                const OHit ch = shadowRayClosest(s, p0, dirTo, std::fmax(1e-4f, .99f * dist), 1.01f * dist);
+                if (ch.tri == 0xFFFFFFFFu || s.triMesh[ch.tri] != s.triMesh[h.tri]) { beta[idx] /= 4.f; continue; }
+                h.tri = ch.tri; h.b1 = ch.b1; h.b2 = ch.b2;
+                h.posW = p0 + dirTo * ch.t;
+                beta[idx] = std::fmin(1.f, 2.f * beta[idx]);
+                break;
+            }
         }
+    }
+    const V3<F> p = d.hits[d.hitCount - 1].posW;
+    V3<F> dir = lightPos - p;
+    const F dist = length(dir);
+    dir = dir / dist;
+    d.woLastHitToLight = dir;
+    return visible(s, p, dir, 1e-4f, dist);
+}
+void manifoldForwardUnwind(const OScene& s, const Options& o, const MSData& d, V3<F> origin, Beam& beam) {  // :80-93
+    for (int idx = d.hitCount - 1; idx >= 0; --idx) {
+        const MSHit& h = d.hits[idx];
+        const V3<F> p2 = idx == 0 ? origin : d.hits[idx - 1].posW;
+        unwindStep(s, o, h.tri, h.b1, h.b2, &h.posW, normalize(p2 - h.posW), h.lobe, beam);
     }
 }
 void measure(const OScene& s, const Options& o, SolveData& sv, int bounceIdx, float pathContribution, Beam& beam) {
@@ -264,7 +418,10 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
         const uint32_t props = getPropertyFlags(bsdf);
         if (doNEE) {
             const uint32_t lobes = getLobes(bsdf);
-            const bool sampleRefl = (lobes & Lobe_NonDeltaReflection) != 0, sampleTrans = (lobes & Lobe_NonDeltaTransmission) != 0;
+            // a delta reflector can take part in a manifold walk from here (pltpt_solve.rt.slang:254-258)
+            const bool allowMS = o.doMNEE && p > 0 && !(bounce.flags & RF_LastDelta) && (props & Prop_MWR);
+            const bool sampleRefl = (lobes & Lobe_NonDeltaReflection) != 0 || allowMS;
+            const bool sampleTrans = (lobes & Lobe_NonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
                 // generateLightSample: light type selection (only emissive lights present)
                 const float pType = useEmissive ? 1.f : 0.f;
@@ -326,25 +483,54 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                 for (int j = 0; j < HW; ++j) anyL |= Le[j] > 0.f;
                 if (valid && anyL) {
                     const int comps = (!o.hwssMIS && (bounce.flags & RF_Dispersive)) ? 1 : HW;
-                    ORay vr;
-                    vr.o[0] = lsOrigin.x; vr.o[1] = lsOrigin.y; vr.o[2] = lsOrigin.z;
-                    vr.d[0] = lsDir.x; vr.d[1] = lsDir.y; vr.d[2] = lsDir.z;
-                    vr.tmin = 0.f; vr.tmax = lsDist;
-                    const OHit vh = s.bvh.closest(s.mesh, vr, true);
-                    if (vh.tri == 0xFFFFFFFFu) {
-                        Beam beam = sourceBeamArea(tPos, -lsDir, o.sourcingArea, lsDist, Le, lam, o.maxOmega, comps);
-                        if (doMIS) {
+                    Beam beam = sourceBeamArea(tPos, -lsDir, o.sourcingArea, lsDist, Le, lam, o.maxOmega, comps);
+                    bool evalCurrent = true, ok = true;
+                    const int maxMSBounces = int(o.maxBounces) - p;
+                    if (!o.doMNEE || maxMSBounces <= 0) {
+                        ok = visible(s, lsOrigin, lsDir, 0.f, lsDist);
+                        if (ok && doMIS) {
                             const float bp = materialPdf(bsdf, sd, lsDir, beam.wavelengths[0]);
                             beam.scaleAll(lsPdf / (lsPdf + bp));
                         }
-                        // evalNEE
-                        const SD<F> fsd = shadingData(s, bounce.tri, bounce.b1, bounce.b2, -beam.r);
-                        beam.propagate(fsd.posW);
-                        materialEval<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam);
-                        if (o.hwssMIS) {
-                            float pd[HW];
-                            materialLambdaPdf<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam, pd);
-                            for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+                    } else {  // manifold sampling on NEE (pltpt_solve.rt.slang:296-366)
+                        const V3<F> lightDir = -beam.r;
+                        V3<F> dirL = lsDir;
+                        float misBsdfPdf = 0.f;
+                        MSData ms;
+                        ms.wavelength = beam.wavelengths[0];
+                        const bool isRefl2 = dot(sd.f.N, lightDir) * dot(sd.f.N, sd.V) > 0.f;
+                        const bool currentVertexMS = isRefl2 && allowMS;
+                        if (currentVertexMS)
+                            ok = manifoldSampleVertex(bounce.tri, bounce.b1, bounce.b2, sd, bsdf, lightDir, sg, ms) &&
+                                 (ms.hits[0].lobe & Lobe_Reflection) != 0;
+                        const V3<F> origin = !currentVertexMS ? sd.posW : sv.pathOrigin;
+                        ok = ok && manifoldNEESample(s, o, lsOrigin, lsDir, lsDist, origin, beam.posW, lightDir, maxMSBounces, sg, ms);
+                        ok = ok && !(ms.pdf < 1e-6f);
+                        if (ok && ms.hitCount > 0) {
+                            if (!o.hwssMIS && ms.dispersive) beam.hwss_comps = 1;
+                            dirL = ms.woLastHitToLight;
+                            beam.r = -ms.woLastHitToLight;
+                            beam.t = perp_stark(beam.r);
+                            beam.scaleAll(1.f / ms.pdf);
+                            manifoldForwardUnwind(s, o, ms, origin, beam);
+                            if (currentVertexMS) { evalCurrent = false; misBsdfPdf = bounce.lastNonDeltaPdf; }
+                        }
+                        if (ok && doMIS) {
+                            if (ms.hitCount == 0 || !currentVertexMS) misBsdfPdf = materialPdf(bsdf, sd, dirL, beam.wavelengths[0]);
+                            beam.scaleAll(lsPdf / (lsPdf + misBsdfPdf * ms.pdf));
+                        }
+                    }
+                    if (ok) {
+                        // evalNEE (pltpt_solve.rt.slang:219-243)
+                        if (evalCurrent) {
+                            const SD<F> fsd = shadingData(s, bounce.tri, bounce.b1, bounce.b2, -beam.r);
+                            beam.propagate(fsd.posW);
+                            materialEval<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam);
+                            if (o.hwssMIS) {
+                                float pd[HW];
+                                materialLambdaPdf<F, HW>(bsdf, fsd, bwi, Lobe_NonDelta, beam, pd);
+                                for (int j = 0; j < HW; ++j) beam.spectralPDFs[j] *= pd[j];
+                            }
                         }
                         forwardUnwind(s, o, rec, p, beam);
                         measure(s, o, sv, p, bounce.pathContribution, beam);
@@ -353,7 +539,9 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
             }
         }
         // evalEmissive
-        const bool didNEE = doNEE && p > 0 && !(bounce.flags & RF_LastDelta);
+        // wasNEEAvailable (pltpt_solve.rt.slang:205-210)
+        const bool didNEE = doNEE && p > 0 &&
+                            (!(bounce.flags & RF_LastDelta) || (o.doMNEE && int(bounce.flags >> 16) <= o.mneeMaxOccluders && p > 1));
         if (useEmissive && (props & Prop_Emissive) && (doMIS || !didNEE)) {
             const bool needMIS = doMIS && didNEE;
             float lightPdf = 0.f;
@@ -500,7 +688,8 @@ extern "C" {
 
 // Camera: pos[3], target[3], up[3], fov_y, aspect, focal_distance, jitter[2] (14 floats).
 // options: width, height, frame, maxBounces, doNEE, doMIS, doRR, HWSSDoMIS, doImportanceSampleEmitters,
-// useDirectLights, useEmissiveLights (11 ints); fopts: sourcingArea, maxOmega.
+// useDirectLights, useEmissiveLights, doMNEE, MNEEMaxOccluders, MNEEMaxIterations (14 ints); fopts:
+// sourcingArea, maxOmega, MNEESolverThreshold.
 // Renders pixels [first, first + count) of the row-major frame; color: 4 floats per rendered pixel;
 // records (optional): (maxBounces+1) x 9 planes of `count` words, as the product's bounce records.
 void* oracle_scene_create(const float* pos, const float* nrm, uint32_t ntri, const uint32_t* idx, const uint32_t* triMesh,
@@ -540,6 +729,10 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
     o.doNEE = iopt[4]; o.doMIS = iopt[5]; o.doRR = iopt[6]; o.hwssMIS = iopt[7]; o.isEmitters = iopt[8];
     o.directLights = iopt[9]; o.emissiveLights = iopt[10];
     o.sourcingArea = fopt[0]; o.maxOmega = fopt[1];
+    o.doMNEE = iopt[11];
+    o.mneeMaxOccluders = std::min(std::max(iopt[12], 0), kMaxMSOccluders);
+    o.mneeMaxIterations = iopt[13];
+    o.mneeThreshold = fopt[2];
     if (threads > 0) omp_set_num_threads(threads);
     const size_t n = count;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -577,6 +770,34 @@ int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint3
     const int rc = oracle_scene_render(h, cam, iopt, fopt, first, count, color, records, threads);
     oracle_scene_destroy(h);
     return rc;
+}
+
+// One manifold walk (manifoldNEESample) from `origin` towards the point `lightPos` on a light, with
+// the straight segment, stopped 1e-4 short of the light, as the visibility ray. Writes the converged vertices, their sampled lobes, the
+// Newton iteration count and the walk's pdf; returns the vertex count, or -1 when the walk fails.
+int oracle_mnee_probe(void* h, const float* origin, const float* lightPos, float wavelength, int maxMSBounces, int maxOccluders,
+                      int maxIterations, float threshold, uint32_t seed, float* hitPos, uint32_t* hitLobe, int* iterations,
+                      float* pdf) {
+    const OScene& s = *static_cast<const OScene*>(h);
+    Options o;
+    o.doMNEE = 1;
+    o.mneeMaxOccluders = std::min(std::max(maxOccluders, 0), kMaxMSOccluders);
+    o.mneeMaxIterations = maxIterations;
+    o.mneeThreshold = threshold;
+    const V3<F> org{origin[0], origin[1], origin[2]}, lp{lightPos[0], lightPos[1], lightPos[2]};
+    const V3<F> tl = lp - org;
+    const float dist = length(tl);
+    UniformSampleGenerator sg(seed, 0u, 0u);
+    MSData ms;
+    ms.wavelength = wavelength;
+    const bool ok = manifoldNEESample(s, o, org, tl / dist, dist * (1.f - 1e-4f), org, lp, tl / dist, maxMSBounces, sg, ms);
+    *iterations = ms.iterations;
+    *pdf = ms.pdf;
+    for (int i = 0; i < ms.hitCount; ++i) {
+        hitPos[3 * i] = ms.hits[i].posW.x; hitPos[3 * i + 1] = ms.hits[i].posW.y; hitPos[3 * i + 2] = ms.hits[i].posW.z;
+        hitLobe[i] = ms.hits[i].lobe;
+    }
+    return ok ? ms.hitCount : -1;
 }
 
 // The power sampler's alias table, decoded: per entry the binary16 threshold (as float), the

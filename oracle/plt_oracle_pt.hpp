@@ -525,6 +525,26 @@ template <class T> struct DiffractionGrating {
                    ? V3<T>{a * std::sqrt(hmax(T(0), qq)), b * std::sqrt(m), std::sqrt(hmax(T(0), T(1) - sqr(a) * qq - sqr(b) * m))}
                    : V3<T>{0, 0, 0};
     }
+    // closestLobeHint (DiffractionGrating.slang:92-107): the diffraction order whose direction is
+    // nearest the hint (manifold walks).
+    void closestLobeHint(V3<T> wi, V3<T> woHint, T alpha, T lambda, int& lx, int& ly) const {
+        lx = ly = 0;
+        if (alpha == T(1)) return;
+        const T lix = sqr(wi.x) + sqr(wi.z), liy = sqr(wi.y) + sqr(wi.z);
+        const T lox = sqr(woHint.x) + sqr(woHint.z), loy = sqr(woHint.y) + sqr(woHint.z);
+        const T six = lix > K<T>::FLT_EPS ? wi.x / std::sqrt(lix) : T(0), siy = liy > K<T>::FLT_EPS ? wi.y / std::sqrt(liy) : T(0);
+        const T sox = lox > K<T>::FLT_EPS ? -woHint.x / std::sqrt(lox) : T(0), soy = loy > K<T>::FLT_EPS ? -woHint.y / std::sqrt(loy) : T(0);
+        const int n = int(lobeCount);
+        auto toLobe = [&](T l) { return std::min(std::max(int(l + sgn(l) * T(.5)), -n), n); };
+        // (a 1-D grating has no y period: its y order is 0 whatever the hint)
+        const T l_x = (six - sox) / (ipx * lambda);
+        const T l_y = is1D() ? T(0) : (siy - soy) / (ipy * lambda);
+        lx = toLobe(l_x);
+        ly = toLobe(l_y);
+        if (alpha == T(0) && lx == 0) lx = int(sgn(l_x));
+        if (alpha == T(0) && ly == 0) ly = int(sgn(l_y));
+        if (is1D()) ly = 0;
+    }
     template <class S> void sampleLobe(V3<T> wi, T& pdfx, T& pdfy, int& lx, int& ly, T lambda, S& sg) const {
         T intensity[16];
         T total = 0;
@@ -592,13 +612,19 @@ template <class T> struct GratedConductorBxDF {
             }
         } else beam.setZero();
     }
-    template <class S> bool sample(V3<T> wi, V3<T>& wo, T& pdf, T& weight, uint32_t& bsdfLobe, S& sg, T wavelength, uint32_t lobeMask) const {
+    template <class S> bool sample(V3<T> wi, V3<T>& wo, T& pdf, T& weight, uint32_t& bsdfLobe, S& sg, T wavelength, uint32_t lobeMask,
+                                   V3<T> woHint = V3<T>{0, 0, 0}) const {  // :115-166
         const bool hasReflection = (lobeMask & Lobe_Reflection) != 0, hasDirect = (lobeMask & Lobe_Delta) != 0, hasScatter = (lobeMask & Lobe_NonDelta) != 0;
         if (!hasReflection || (!hasDirect && !hasScatter)) return false;
         const T lam = T(1e-3) * wavelength;
         const T a = grating.alpha(wi, K<T>::_2PI / lam);
+        const bool hasHint = hasDirect && (woHint.x != T(0) || woHint.y != T(0) || woHint.z != T(0));
+        int lx = 0, ly = 0;
+        if (hasHint && a < T(1)) grating.closestLobeHint(wi, woHint, a, lam, lx, ly);
         bool isDirect = hasDirect;
-        if (hasDirect && hasScatter) { isDirect = a == T(1) || sampleNext1D<T>(sg) < a; pdf = isDirect ? a : T(1) - a; }
+        pdf = 1;
+        if (hasHint) { isDirect = lx == 0 && ly == 0; pdf = 1; }
+        else if (hasDirect && hasScatter) { isDirect = a == T(1) || sampleNext1D<T>(sg) < a; pdf = isDirect ? a : T(1) - a; }
         if (isDirect) {
             T basePdf;
             base.sample(wi, wo, basePdf, weight, bsdfLobe, sg, wavelength, lobeMask);
@@ -606,9 +632,8 @@ template <class T> struct GratedConductorBxDF {
             return true;
         }
         weight = Spectrum<T>::rgbToSpectrum(base.specular, wavelength);
-        T px, py;
-        int lx, ly;
-        grating.sampleLobe(wi, px, py, lx, ly, lam, sg);
+        T px = 1, py = 1;
+        if (!hasHint) grating.sampleLobe(wi, px, py, lx, ly, lam, sg);
         wo = grating.diffract(wi, lx, ly, lam);
         bsdfLobe = encode(lx, ly, Lobe_DispersiveDeltaReflection);
         pdf *= px * py;
@@ -842,6 +867,177 @@ template <class S> inline bool materialSample(const Instance<float>& in, const S
         case MT_THIN_DIELECTRIC: valid = in.thn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         case MT_MULTILAYER: valid = in.mls.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         default: valid = in.grt.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
+    }
+    if (valid) r.wo = f.fromLocal(r.wo);
+    return valid;
+}
+
+// ------------------------------------------------------------------------------------------
+// Differential geometry for manifold walks (Rendering/Utils/DifferentialShadingGeometry.slang).
+// ------------------------------------------------------------------------------------------
+template <class T> struct PDV {
+    V3<T> v, ddu, ddv;
+    void flip() { v = -v; ddu = -ddu; ddv = -ddv; }
+};
+template <class T> struct KPDV { V3<T> v{0, 0, 0}, ddu{0, 0, 0}, ddv{0, 0, 0}, ddl{0, 0, 0}; };
+template <class T> struct DSG {
+    PDV<T> p, n, s, t;
+    V3<T> ngeo;
+    PDV<T> pdw(V3<T> dst) const {  // :50-58
+        PDV<T> w;
+        w.v = dst - p.v;
+        const T ilo = T(1) / length(w.v);
+        w.v = w.v * ilo;
+        w.ddu = (p.ddu - w.v * dot(w.v, p.ddu)) * -ilo;
+        w.ddv = (p.ddv - w.v * dot(w.v, p.ddv)) * -ilo;
+        return w;
+    }
+    V3<T> fromLocal(V3<T> v) const { return s.v * v.x + t.v * v.y + n.v * v.z; }
+};
+// differentialShadingGeometry (:120-190) for a triangle without texture coordinates: the basic
+// parameterisation uv0 = 0, uv1 = (1, 0), uv2 = (0, 1).
+template <class T> inline DSG<T> makeDSG(const V3<T> P[3], const V3<T> Nv[3], T b1, T b2) {
+    const T b0 = T(1) - b1 - b2;
+    DSG<T> g;
+    g.ngeo = normalize(cross(P[1] - P[0], P[2] - P[0]));
+    g.p.v = P[0] * b0 + P[1] * b1 + P[2] * b2;
+    g.n.v = Nv[0] * b0 + Nv[1] * b1 + Nv[2] * b2;
+    const T il = T(1) / length(g.n.v);
+    g.n.v = g.n.v * il;
+    g.n.ddu = (Nv[1] - Nv[0]) * il;
+    g.n.ddv = (Nv[2] - Nv[0]) * il;
+    g.n.ddu = g.n.ddu - g.n.v * dot(g.n.v, g.n.ddu);
+    g.n.ddv = g.n.ddv - g.n.v * dot(g.n.v, g.n.ddv);
+    // det of (duv0, duv1) = 1: the position derivatives are the edges
+    g.p.ddu = P[1] - P[0];
+    g.p.ddv = P[2] - P[0];
+    g.s.v = g.p.ddu - g.n.v * dot(g.n.v, g.p.ddu);
+    T ils = T(1) / length(g.s.v);
+    g.s.v = g.s.v * ils;
+    if (g.p.ddu.x == T(0) && g.p.ddu.y == T(0) && g.p.ddu.z == T(0)) { g.s.v = perp_stark(g.n.v); ils = 1; }
+    g.t.v = cross(g.n.v, g.s.v);
+    g.s.ddu = (-g.n.ddu * dot(g.n.v, g.p.ddu) - g.n.v * dot(g.n.ddu, g.p.ddu)) * ils;
+    g.s.ddv = (-g.n.ddv * dot(g.n.v, g.p.ddu) - g.n.v * dot(g.n.ddv, g.p.ddu)) * ils;
+    g.s.ddu = g.s.ddu - g.s.v * dot(g.s.ddu, g.s.v);
+    g.s.ddv = g.s.ddv - g.s.v * dot(g.s.ddv, g.s.v);
+    g.t.ddu = cross(g.n.ddu, g.s.v) + cross(g.n.v, g.s.ddu);
+    g.t.ddv = cross(g.n.ddv, g.s.v) + cross(g.n.v, g.s.ddv);
+    return g;
+}
+template <class T> inline KPDV<T> reflectDerivatives(const PDV<T>& wi, const PDV<T>& n) {  // :192-205
+    const T wn = dot(wi.v, n.v);
+    const T dwdun = dot(wi.ddu, n.v), dwdvn = dot(wi.ddv, n.v);
+    const T dnduw = dot(n.ddu, wi.v), dndvw = dot(n.ddv, wi.v);
+    KPDV<T> r;
+    r.v = n.v * (T(2) * wn) - wi.v;
+    r.ddu = (n.v * (dwdun + dnduw) + n.ddu * wn) * T(2) - wi.ddu;
+    r.ddv = (n.v * (dwdvn + dndvw) + n.ddv * wn) * T(2) - wi.ddv;
+    return r;
+}
+template <class T> inline KPDV<T> refractDerivatives(const PDV<T>& wi, PDV<T> n, T eta) {  // :206-222 (no dispersion)
+    const T wn = dot(wi.v, n.v);
+    eta = wn < T(0) ? eta : T(1) / eta;
+    if (wn < T(0)) n.flip();
+    const T eta2 = sqr(eta);
+    const T dwdun = dot(wi.ddu, n.v), dwdvn = dot(wi.ddv, n.v);
+    const T dnduw = dot(n.ddu, wi.v), dndvw = dot(n.ddv, wi.v);
+    const T root = std::sqrt(hmax(T(0), T(1) - eta2 * (T(1) - sqr(wn))));
+    if (root == T(0)) return KPDV<T>{};
+    const T a = eta - eta2 * wn / root;
+    KPDV<T> r;
+    r.v = (wi.v - n.v * wn) * -eta - n.v * root;
+    r.ddu = n.v * (a * (dnduw + dwdun)) + n.ddu * (eta * wn - root) - wi.ddu * eta;
+    r.ddv = n.v * (a * (dndvw + dwdvn)) + n.ddv * (eta * wn - root) - wi.ddv * eta;
+    return r;
+}
+// DiffractionGrating::partialDerivatives (DiffractionGrating.slang:188-265) in world space.
+template <class T> inline KPDV<T> gratingDerivatives(const DiffractionGrating<T>& g, const DSG<T>& dsg, const PDV<T>& wi, int lx, int ly, T wavelength) {
+    const PDV<T>& n = dsg.n;
+    PDV<T> X;
+    X.v = dsg.fromLocal(V3<T>{g.gx, g.gy, 0});
+    X.ddu = dsg.s.ddu * g.gx + dsg.t.ddu * g.gy;
+    X.ddv = dsg.s.ddv * g.gx + dsg.t.ddv * g.gy;
+    if (lx == 0 && ly == 0) return reflectDerivatives(wi, n);
+    const T lambda = T(1e-3) * wavelength;
+    const T lpx = T(lx) * g.ipx, lpy = T(ly) * g.ipy;
+    const V3<T> Y = cross(n.v, X.v);
+    const T x = dot(wi.v, X.v), y = dot(wi.v, Y), z = dot(wi.v, n.v);
+    const T px = std::sqrt(sqr(x) + sqr(z)), py = std::sqrt(sqr(y) + sqr(z));
+    const T ipx_ = px > K<T>::FLT_EPS ? T(1) / px : T(0), ipy_ = py > K<T>::FLT_EPS ? T(1) / py : T(0);
+    const T a = lambda * lpx - x * ipx_, b = lambda * lpy - y * ipy_;
+    const T oneMinusA2 = hmax(T(0), T(1) - sqr(a));
+    const T m = -oneMinusA2 / (sqr(a * b) - T(1));
+    const T q = T(1) - sqr(b) * m;
+    if (a > T(1) || a < T(-1) || b > T(1) || b < T(-1) || q < T(0) || !(m >= T(0))) return KPDV<T>{};
+    const T sqrtOneM = std::sqrt(q), sqrtM = std::sqrt(m);
+    const T c = T(.5) / (sqrtOneM * std::sqrt(oneMinusA2));
+    const T dxdu = dot(wi.v, X.ddu) + dot(wi.ddu, X.v), dxdv = dot(wi.v, X.ddv) + dot(wi.ddv, X.v);
+    const T dzdu = dot(wi.v, n.ddu) + dot(wi.ddu, n.v), dzdv = dot(wi.v, n.ddv) + dot(wi.ddv, n.v);
+    const T ipx3 = sqr(ipx_) * ipx_, ipy3 = sqr(ipy_) * ipy_;
+    const T dadu = z * (x * dzdu - z * dxdu) * ipx3, dbdu = z * (y * dzdu - z * dxdu) * ipy3;
+    const T dadv = z * (x * dzdv - z * dxdv) * ipx3, dbdv = z * (y * dzdv - z * dxdv) * ipy3;
+    const T dadl = lpx, dbdl = lpy;
+    const T den = sqr(sqr(a * b) - T(1));
+    const T dmdu = T(2) * a * (a * b * oneMinusA2 * dbdu - dadu * (T(1) - sqr(b))) / den;
+    const T dmdv = T(2) * a * (a * b * oneMinusA2 * dbdv - dadv * (T(1) - sqr(b))) / den;
+    const T dmdl = T(2) * a * (a * b * oneMinusA2 * dbdl - dadl * (T(1) - sqr(b))) / den;
+    KPDV<T> r;
+    r.v = X.v * (a * sqrtOneM) + Y * (b * sqrtM) + n.v * std::sqrt(hmax(T(0), T(1) - sqr(a) * q - sqr(b) * m));
+    auto d = [&](T dad, T dbd, T dmd) {
+        return X.v * ((q * dad - a * b * (b * dmd / T(2) + m * dbd)) / sqrtOneM) + Y * ((b * dmd / T(2) + m * dbd) / sqrtM) +
+               n.v * ((T(-2) * a * dad * q - b * (b * dmd + T(2) * m * dbd) + sqr(a) * b * (b * dmd + T(2) * m * dbd)) * c);
+    };
+    r.ddu = d(dadu, dbdu, dmdu);
+    r.ddv = d(dadv, dbdv, dmdv);
+    r.ddl = d(dadl, dbdl, dmdl);
+    return r;
+}
+
+// IMaterialInstance::partialDerivatives of each material (world space; lobe = the sampled lobe).
+template <class T> inline KPDV<T> materialPartialDerivatives(const Instance<T>& in, const PDV<T>& wi, const DSG<T>& dsg, uint32_t lobe, T wavelength) {
+    const bool isTransmission = (lobe & Lobe_Transmission) != 0;
+    switch (in.type) {
+        case MT_DIELECTRIC:  // PLTDielectricBSDF.slang:139-157
+            if (isTransmission) return refractDerivatives(wi, dsg.n, in.die.indexOfRefraction(K<T>::_2PI / (T(1e-3) * wavelength)));
+            return reflectDerivatives(wi, dsg.n);
+        case MT_DIFFUSE: return KPDV<T>{};  // PLTDiffuseBSDF.slang:113-117 (unsupported)
+        case MT_THIN_DIELECTRIC:  // PLTThinDielectricBSDF.slang:151-162
+            if (isTransmission) { PDV<T> w = wi; w.flip(); KPDV<T> r; r.v = w.v; r.ddu = w.ddu; r.ddv = w.ddv; return r; }
+            return reflectDerivatives(wi, dsg.n);
+        case MT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:104-112
+            int lx, ly;
+            GratedConductorBxDF<T>::decode(lobe, lx, ly);
+            if (lx == 0 && ly == 0) return reflectDerivatives(wi, dsg.n);
+            return gratingDerivatives(in.grt.grating, dsg, wi, lx, ly, wavelength);
+        }
+        default: return reflectDerivatives(wi, dsg.n);  // conductor, opaque, coated, multilayer stack
+    }
+}
+
+// IMaterialInstance::sample with a lobe mask and a direction hint (manifold walks): the dielectric
+// and thin dielectric restrict themselves to reflection or transmission by the side of the hint,
+// the grating picks the diffraction order nearest the hint; the others ignore it.
+template <class S> inline bool materialSampleHinted(const Instance<float>& in, const SD<float>& sd, S& sg, BSDFSample& r, float wavelength,
+                                                    uint32_t lobeMask, V3<float> woHint) {
+    const Frame<float> f = bsdfFrame(in, sd.f);
+    const V3<float> wi = f.toLocal(sd.V);
+    const float dh = dot(sd.V, woHint);
+    const uint32_t forced = (dh > 0.f ? uint32_t(Lobe_Reflection) : dh < 0.f ? uint32_t(Lobe_Transmission) : uint32_t(Lobe_All)) & lobeMask;
+    bool valid = false;
+    switch (in.type) {
+        case MT_DIELECTRIC:  // PLTDielectricBSDF.slang:309-320
+            valid = in.die.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, forced);
+            r.wo = f.fromLocal(r.wo);
+            return valid;
+        case MT_THIN_DIELECTRIC: valid = in.thn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, forced); break;
+        case MT_GRATED_CONDUCTOR: valid = in.grt.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask, f.toLocal(woHint)); break;
+        case MT_CONDUCTOR: valid = in.con.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
+        case MT_DIFFUSE: valid = in.dif.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); r.wo = f.fromLocal(r.wo); return valid;
+        case MT_OPAQUE: valid = in.opq.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
+        case MT_COATED_OPAQUE: valid = in.cot.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
+        case MT_COATED_CONDUCTOR: valid = in.ccn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
+        case MT_MULTILAYER: valid = in.mls.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
+        default: break;
     }
     if (valid) r.wo = f.fromLocal(r.wo);
     return valid;

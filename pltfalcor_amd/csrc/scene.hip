@@ -130,7 +130,7 @@ int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out)
         if (d->mesh_material[m] >= d->n_materials) return set_error(PLT_EINVAL, "plt_scene_create: material id out of range");
     for (uint32_t m = 0; m < d->n_materials; ++m) {
         const plt_material& mt = d->materials[m];
-        if (mt.type > PLT_MAT_GRATED_CONDUCTOR) return set_error(PLT_EINVAL, "plt_scene_create: unknown material type");
+        if (mt.type > PLT_MAT_MULTILAYER_STACK) return set_error(PLT_EINVAL, "plt_scene_create: unknown material type");
         for (uint32_t id : {mt.emission_profile, mt.ior_n_profile, mt.ior_k_profile})
             if (id != PLT_NO_PROFILE && id >= d->n_profiles) return set_error(PLT_EINVAL, "plt_scene_create: profile id out of range");
     }

@@ -169,15 +169,16 @@ class OracleScene:
 
     def render(self, width, height, frame=0, max_bounces=6, first=0, count=None, do_nee=1, do_mis=1, do_rr=1,
                hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0, max_omega=0.025, threads=0,
-               records=False, camera=None):
+               records=False, camera=None, do_mnee=0, mnee_max_occluders=2, mnee_max_iterations=60,
+               mnee_threshold=5e-5):
         """Renders pixels [first, first+count) (row-major). Returns color [count, 4] and optionally the
         bounce records [(max_bounces+1), 9, count] (uint32)."""
         if count is None:
             count = width * height - first
         cam = self.camera if camera is None else np.ascontiguousarray(camera, np.float32)
         iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct,
-                         emissive], np.int32)
-        fopt = np.array([sourcing_area, max_omega], np.float32)
+                         emissive, do_mnee, mnee_max_occluders, mnee_max_iterations], np.int32)
+        fopt = np.array([sourcing_area, max_omega, mnee_threshold], np.float32)
         color = np.zeros((count, 4), np.float32)
         rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
         self.lib.oracle_scene_render(self.handle, ptr(cam, C.c_float), iopt.ctypes.data_as(C.POINTER(C.c_int32)),
@@ -186,6 +187,26 @@ class OracleScene:
         if records:
             return color, rec.reshape(max_bounces + 1, REC_WORDS, count)
         return color
+
+    def mnee_probe(self, origin, light_pos, wavelength=550.0, max_ms_bounces=2, max_occluders=2, max_iterations=60,
+                   threshold=5e-5, seed=0):
+        """One manifold walk from `origin` to `light_pos`. Returns (vertex count or -1 on failure,
+        vertices [count, 3], lobes [count], iterations, pdf)."""
+        l = self.lib
+        l.oracle_mnee_probe.restype = C.c_int
+        l.oracle_mnee_probe.argtypes = [C.c_void_p, _F, _F, C.c_float, C.c_int, C.c_int, C.c_int, C.c_float, C.c_uint32, _F,
+                                        _U32, C.POINTER(C.c_int), _F]
+        o = np.ascontiguousarray(origin, np.float32)
+        lp = np.ascontiguousarray(light_pos, np.float32)
+        pos = np.zeros((8, 3), np.float32)
+        lobes = np.zeros(8, np.uint32)
+        it = C.c_int(0)
+        pdf = np.zeros(1, np.float32)
+        n = l.oracle_mnee_probe(self.handle, ptr(o, C.c_float), ptr(lp, C.c_float), wavelength, max_ms_bounces,
+                                max_occluders, max_iterations, threshold, seed, ptr(pos, C.c_float), ptr(lobes, C.c_uint32),
+                                C.byref(it), ptr(pdf, C.c_float))
+        k = max(n, 0)
+        return n, pos[:k].copy(), lobes[:k].copy(), it.value, float(pdf[0])
 
     def close(self):
         if getattr(self, "handle", None):

@@ -354,3 +354,122 @@ def test_gpu_pltpt_frames_accumulate_and_options(gpu):
     with pytest.raises(_lib.PLTError):
         p.execute(w, h, color, albedo, normal)
     p.close()
+
+
+
+def _quad_scene(quads, materials):
+    """Scene of horizontal squares: quads = [(y, half size, normal y (+1 / -1), material)], the
+    procedural scene's spectral profiles (profile 2 is the emission spectrum)."""
+    from pltfalcor_amd.scene import Scene
+    proc = Scene(None, seed=1, sphere_count=0, subdivision=1)
+    pa = proc.numpy()
+    pos, nrm, idx, tm, mm = [], [], [], [], []
+    for m, (y, hs, ny, mat) in enumerate(quads):
+        v0 = len(pos)
+        pos += [[-hs, y, -hs], [hs, y, -hs], [hs, y, hs], [-hs, y, hs]]
+        nrm += [[0, ny, 0]] * 4
+        # winding so that the face normal follows the vertex normal
+        idx += [[v0, v0 + 2, v0 + 1], [v0, v0 + 3, v0 + 2]] if ny > 0 else [[v0, v0 + 1, v0 + 2], [v0, v0 + 2, v0 + 3]]
+        tm += [m, m]
+        mm.append(mat)
+    from pltfalcor_amd import materials as M
+    return Scene.from_arrays(None, np.array(pos, np.float32), np.array(nrm, np.float32), np.array(idx, np.uint32),
+                             np.array(tm, np.uint32), np.array(mm, np.uint32), M.pack(materials), pa["profiles"],
+                             proc.arrays.camera).numpy()
+
+
+def _look_down(a, at=(0.3, 0.2), height=2.0):
+    cam = a["camera"].copy()
+    cam[0:3], cam[3:6], cam[6:9] = [at[0], height, at[1]], [at[0], 0.0, at[1]], [0.0, 0.0, 1.0]
+    cam[9], cam[10], cam[12:14] = np.radians(60.0), 1.0, 0.0
+    a["camera"] = cam
+    return a
+
+
+def _fresnel_unpolarised(c, eta):
+    ct = np.sqrt(1 - (1 - c * c) / eta ** 2)
+    rs = (c - eta * ct) / (c + eta * ct)
+    rp = (eta * c - ct) / (eta * c + ct)
+    return (rs * rs + rp * rp) / 2
+
+
+def test_oracle_mnee_through_thin_slab_equals_unoccluded_connection():
+    """Manifold NEE through a thin dielectric sheet lying between the ground and the light: the
+    sheet does not deviate light, so the walk converges on the straight segment at once, the hinted
+    lobe choice is forced to transmission (pdf 1), and the sheet's transmission is 1 - R per
+    polarisation (T (1 - R)^-1 T). Each ground sample therefore equals the same sample with the sheet
+    removed times 1 - R(theta) (unpolarised source), R between its normal-incidence value and its
+    value at the widest angle of the configuration; without manifold NEE the sheet occludes the
+    light. The reference's final visibility ray ends on the light's own surface
+    (Manifolds.slangh:266-276 with ls.beam.posW), so some connections are rejected by the light's
+    triangle; those samples are 0 and are not counted here."""
+    from pltfalcor_amd import materials as M
+    eta = float(np.float16(1.5))
+    mats = [M.diffuse(.5, .5, .5), M.diffuse(.8, .8, .8, emission=2), M.thin_dielectric(1.5, 1e-3)]
+    ground_light = [(0.0, 10.0, 1.0, 0), (7.0, 1.5, -1.0, 1)]
+    free = O.OracleScene(_look_down(_quad_scene(ground_light, mats)))
+    sheet = O.OracleScene(_look_down(_quad_scene(ground_light + [(4.0, 10.0, 1.0, 2)], mats)))
+    kw = dict(max_bounces=1, do_rr=0, do_mis=0, is_emitters=0, threads=8)
+    assert np.all(sheet.render(16, 16, frame=0, do_mnee=0, **kw)[:, :3] == 0)
+    # widest angle: ground corner seen by the camera (~1.15 off-axis at most) to the far light corner
+    cmax = 3.0 / np.hypot(3.0, np.hypot(1.5 + 1.5, 1.5 + 1.5))
+    lo, hi = 1 - _fresnel_unpolarised(cmax, eta), 1 - _fresnel_unpolarised(1.0, eta)
+    connected = 0
+    for f in range(6):
+        A = free.render(16, 16, frame=f, do_mnee=1, **kw)[:, :3]
+        B = sheet.render(16, 16, frame=f, do_mnee=1, **kw)[:, :3]
+        nz = (B != 0).any(1)
+        assert np.all(A[nz] > 0)
+        ratio = B[nz] / A[nz]
+        assert np.all(ratio > lo - 1e-4) and np.all(ratio < hi + 1e-4), (ratio.min(), ratio.max(), lo, hi)
+        assert np.all(np.ptp(ratio, axis=1) < 1e-4)  # the same factor in every colour channel
+        connected += nz.sum()
+    assert connected > 0.05 * 6 * 256
+
+
+def test_oracle_mnee_walk_satisfies_snell_through_glass_slab():
+    """The Newton walk (Manifolds.slangh:165-281) through both faces of a non-dispersive glass slab
+    (eta 1.5, faces at y = 3.5 and 4) from a ground point to a light point converges on the refracted
+    path: the vertices agree with the flat-slab solution found by bisection on the incidence angle,
+    the exit direction is parallel to the entry direction, and Snell's law holds at both faces. With
+    one occluder allowed the walk is refused."""
+    from pltfalcor_amd import materials as M
+    glass = M.dielectric(1.5, 0.0)
+    glass.specular[0] = M.h(0.0)  # no dispersion
+    mats = [M.diffuse(.5, .5, .5), M.diffuse(.8, .8, .8, emission=2), glass]
+    a = _quad_scene([(0.0, 10.0, 1.0, 0), (7.0, 1.5, -1.0, 1), (4.0, 10.0, 1.0, 2), (3.5, 10.0, -1.0, 2)], mats)
+    osc = O.OracleScene(a)
+    eta = 1.5
+    # end points just below the light's plane, so that the final visibility ray does not end on it
+    cases = [([1.0, 0.01, -0.5], [0.0, 6.99, 0.0]), ([1.0, 0.01, -0.5], [1.2, 6.99, 1.2]), ([2.5, 0.01, 1.0], [1.2, 6.99, 1.2]),
+             ([-1.5, 0.01, 0.7], [0.5, 6.99, -1.0]), ([0.2, 0.01, 0.3], [-1.0, 6.99, 1.4])]
+    converged = 0
+    for x, L in cases:
+        x, L = np.array(x, np.float64), np.array(L, np.float64)
+        n, v, lobes, it, pdf = osc.mnee_probe(x, L)
+        assert osc.mnee_probe(x, L, max_occluders=1)[0] == -1
+        if n < 0:
+            continue  # iteration cap
+        converged += 1
+        assert n == 2 and it <= 60 and pdf == 1.0
+        assert np.all(lobes == 0x40)  # delta transmission (not dispersive)
+        # flat-slab solution: horizontal distance = (h_below + h_above) tan(t) + thickness tan(t_inside)
+        hb, ha, th = 3.5 - x[1], L[1] - 4.0, 0.5
+        D = np.hypot(*(L - x)[[0, 2]])
+        lo_t, hi_t = 0.0, np.pi / 2 - 1e-9
+        for _ in range(200):
+            t = (lo_t + hi_t) / 2
+            s2 = np.sin(t) / eta
+            f = (hb + ha) * np.tan(t) + th * s2 / np.sqrt(1 - s2 * s2) - D
+            lo_t, hi_t = (t, hi_t) if f < 0 else (lo_t, t)
+        u = (L - x)[[0, 2]] / D
+        r1 = hb * np.tan(t)
+        r2 = r1 + th * np.tan(np.arcsin(np.sin(t) / eta))
+        want = np.array([[x[0] + u[0] * r1, 3.5, x[2] + u[1] * r1], [x[0] + u[0] * r2, 4.0, x[2] + u[1] * r2]])
+        assert np.abs(v - want).max() < 2e-3, (v, want)
+        d0, d1, d2 = v[0] - x, v[1] - v[0], L - v[1]
+        d0, d1, d2 = (d / np.linalg.norm(d) for d in (d0, d1, d2))
+        assert np.linalg.norm(d0 - d2) < 1e-3
+        sin0, sin1 = np.hypot(d0[0], d0[2]), np.hypot(d1[0], d1[2])
+        assert abs(sin0 - eta * sin1) < 1e-3
+    assert converged >= 4
