@@ -301,8 +301,8 @@ bool traceMNEEOccluders(const OScene& s, const Options& o, V3<F> org, V3<F> dir,
     }
 }
 V2<F> sphcoords(V3<F> w) {  // :314-319
-    const F theta = std::acos(clampT(w.z, F(-1), F(1)));
-    F phi = std::atan2(w.y, w.x);
+    const F theta = walkAcos(clampT(w.z, F(-1), F(1)));
+    F phi = walkAtan2(w.y, w.x);
     phi += phi < F(0) ? K<F>::_2PI : F(0);
     return {theta, phi};
 }
@@ -798,6 +798,14 @@ int oracle_mnee_probe(void* h, const float* origin, const float* lightPos, float
         hitLobe[i] = ms.hits[i].lobe;
     }
     return ok ? ms.hitCount : -1;
+}
+
+// The manifold walk's acos and atan2 (walkAcos, walkAtan2) over arrays.
+void oracle_walk_trig(const float* x, const float* y, uint32_t n, float* acosx, float* atan2yx) {
+    for (uint32_t i = 0; i < n; ++i) {
+        acosx[i] = walkAcos(x[i]);
+        atan2yx[i] = walkAtan2(y[i], x[i]);
+    }
 }
 
 // The power sampler's alias table, decoded: per entry the binary16 threshold (as float), the

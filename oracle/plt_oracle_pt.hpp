@@ -875,6 +875,43 @@ template <class S> inline bool materialSample(const Instance<float>& in, const S
 // ------------------------------------------------------------------------------------------
 // Differential geometry for manifold walks (Rendering/Utils/DifferentialShadingGeometry.slang).
 // ------------------------------------------------------------------------------------------
+// Spherical angles of the manifold walk: one fixed single-precision evaluation of asin/acos/atan/
+// atan2 (the Cephes library's single-precision polynomials and range reductions, S. L. Moshier),
+// written with +, -, *, / and sqrt only, so that a strict-IEEE host and the device round every step
+// of the walk identically.
+inline float walkAsin(float xx) {
+    const float a = std::fabs(xx);
+    float x, z;
+    bool flag;
+    if (a < 1e-4f) return xx;
+    if (a > .5f) { z = .5f * (1.f - a); x = std::sqrt(z); flag = true; }
+    else { x = a; z = x * x; flag = false; }
+    float r = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z + 1.6666752422e-1f) * z * x + x;
+    if (flag) { r = r + r; r = 1.5707963267948966192f - r; }
+    return xx < 0.f ? -r : r;
+}
+inline float walkAcos(float x) {
+    if (x < -.5f) return 3.14159265358979323846f - 2.f * walkAsin(std::sqrt(.5f * (1.f + x)));
+    if (x > .5f) return 2.f * walkAsin(std::sqrt(.5f * (1.f - x)));
+    return 1.5707963267948966192f - walkAsin(x);
+}
+inline float walkAtan(float xx) {
+    float x = std::fabs(xx), y;
+    if (x > 2.414213562373095f) { y = 1.5707963267948966192f; x = -(1.f / x); }
+    else if (x > .4142135623730950f) { y = .7853981633974483096f; x = (x - 1.f) / (x + 1.f); }
+    else y = 0.f;
+    const float z = x * x;
+    y += (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x;
+    return xx < 0.f ? -y : y;
+}
+inline float walkAtan2(float y, float x) {
+    const int code = (x < 0.f ? 2 : 0) | (y < 0.f ? 1 : 0);
+    if (x == 0.f) return (code & 1) ? -1.5707963267948966192f : (y == 0.f ? 0.f : 1.5707963267948966192f);
+    if (y == 0.f) return (code & 2) ? 3.14159265358979323846f : 0.f;
+    const float w = code == 2 ? 3.14159265358979323846f : code == 3 ? -3.14159265358979323846f : 0.f;
+    return w + walkAtan(y / x);
+}
+
 template <class T> struct PDV {
     V3<T> v, ddu, ddv;
     void flip() { v = -v; ddu = -ddu; ddv = -ddv; }

@@ -263,7 +263,11 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (!rd->color || !rd->albedo || !rd->normal || rd->width == 0 || rd->height == 0)
         return set_error(PLT_EINVAL, "plt_pass_execute: missing output buffers or empty frame");
     auto o = [&](const char* k) { return p->opt[k]; };
-    if (o("doMNEE") != 0) return set_error(PLT_ENOTIMPL, "doMNEE: manifold next-event estimation is not built yet; set doMNEE=0");
+    const double mneeOcc = o("MNEEMaxOccluders"), mneeIt = o("MNEEMaxIterations"), mneeThr = o("MNEESolverThreshold");
+    if (!(mneeOcc >= 0) || !(mneeIt >= 0) || mneeIt > 1e6 || !std::isfinite(mneeThr))
+        return set_error(PLT_EINVAL, "MNEEMaxOccluders and MNEEMaxIterations must be non-negative (iterations <= 1e6), "
+                                     "MNEESolverThreshold finite");
+    if (mneeOcc > plt::kMaxMsOccluders) return set_error(PLT_ENOTIMPL, "MNEEMaxOccluders above 8 is not built");
     if (o("NEEUsePerTileSG") != 0) return set_error(PLT_ENOTIMPL, "NEEUsePerTileSG=1 is not built");
     if (o("emissiveSampler") != 1) return set_error(PLT_ENOTIMPL, "only the power emissive sampler (1) is built");
     if (o("sampleGenerator") != 0) return set_error(PLT_ENOTIMPL, "only the uniform sample generator (0) is built");
@@ -323,7 +327,11 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (o("doImportanceSampleEmitters") != 0) opts |= plt::kOptISEmitters;
     if (o("useDirectLights") != 0) opts |= plt::kOptDirectLights;
     if (o("useEmissiveLights") != 0) opts |= plt::kOptEmissiveLights;
+    if (o("doMNEE") != 0) opts |= plt::kOptMNEE;
     fr.options = opts;
+    fr.mnee_max_occluders = uint32_t(mneeOcc);
+    fr.mnee_max_iterations = uint32_t(mneeIt);
+    fr.mnee_threshold = float(mneeThr);
     fr.path_base = 0;
     fr.n_paths = uint32_t(npaths);
     fr.sourcing_area = float(o("sourcingAreaFromEmissiveGeometry"));

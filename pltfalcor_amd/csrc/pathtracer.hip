@@ -6,7 +6,11 @@
 // SoA planes. Solve pass: replays the records; per vertex it samples an emissive triangle
 // (power alias table, spectral importance sampling), casts an any-hit shadow ray, sources a PLT
 // beam and carries it forward through the stored vertices (forwardUnwind: eval + spectral MIS
-// per vertex), measures it at the camera, and does the same for emissive hits.
+// per vertex), measures it at the camera, and does the same for emissive hits. With manifold NEE
+// on, a shadow ray blocked by specular transmitters becomes a Newton walk of those vertices onto
+// the specular connection to the light (Manifolds.slangh), and a delta reflector at the current
+// vertex takes part in that walk.
+#include "manifold.hpp"
 #include "pathtracer.hpp"
 #include "plt_internal.hpp"
 #include "plt_materials.hpp"
@@ -334,10 +338,13 @@ PLT_DEV void measure(const PtScene& sc, const PtFrame& fr, Solve& sv, int bounce
 }
 
 // One forward step through a stored vertex (forwardUnwindStep, pltpt_solve.rt.slang:48-66).
+// posW replaces the interpolated hit position (manifold walk vertices).
 template <int N>
-PLT_DEV void unwind_step(const PtScene& sc, const PtFrame& fr, uint32_t tri, float b1, float b2, V3 wo, uint32_t lobe, Beam<N>& beam) {
+PLT_DEV void unwind_step(const PtScene& sc, const PtFrame& fr, uint32_t tri, float b1, float b2, V3 wo, uint32_t lobe, Beam<N>& beam,
+                         const V3* posW = nullptr) {
     const V3 V = -beam.r;
-    const Surf s = surface_at(sc, tri, b1, b2, V);
+    Surf s = surface_at(sc, tri, b1, b2, V);
+    if (posW) s.pos = *posW;
     beam.propagate(s.pos);
     const Mat m = material_at(sc, s.material);
     mat_eval<N>(m, s.f, V, wo, lobe, beam);
@@ -356,6 +363,145 @@ PLT_DEV void forward_unwind(const PtScene& sc, const PtFrame& fr, size_t n, size
     }
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Manifold walks (Manifolds.slangh:57-281).
+// ---------------------------------------------------------------------------------------
+struct MsHit {
+    uint32_t tri;
+    float b1, b2;
+    V3 pos;
+    uint32_t lobe;
+};
+struct MsData {  // ManifoldSampleData
+    MsHit hits[kMaxMsOccluders];
+    V3 woLast;
+    int count, iterations;
+    float wavelength, pdf;
+    bool dispersive;
+};
+PLT_DEV V3 walk_point(V3 o, V3 d, float t) {
+#pragma clang fp contract(off)
+    return v3(o.x + d.x * t, o.y + d.y * t, o.z + d.z * t);
+}
+PLT_DEV DSG dsg_at(const PtScene& sc, uint32_t tri, float b1, float b2) {
+    V3 P[3], Nv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = sc.indices[3 * tri + k];
+        P[k] = v3(sc.positions[3 * v], sc.positions[3 * v + 1], sc.positions[3 * v + 2]);
+        Nv[k] = v3(sc.normals[3 * v], sc.normals[3 * v + 1], sc.normals[3 * v + 2]);
+    }
+    return make_dsg(P[0], P[1], P[2], Nv[0], Nv[1], Nv[2], b1, b2);
+}
+// manifoldSampleVertex (:84-105): a delta lobe towards the light, recorded as the next walk vertex.
+PLT_DEV bool ms_sample_vertex(uint32_t tri, float b1, float b2, V3 pos, const Surf& s, const Mat& m, V3 sdV, V3 lightDir, Rng& sg,
+                              MsData& d) {
+    BsdfSample smp;
+    if (!mat_sample_delta_hinted(m, s.f, sdV, sg, d.wavelength, lightDir, smp)) return false;
+    d.pdf *= smp.pdf;
+    d.hits[d.count] = MsHit{tri, b1, b2, pos, smp.lobe};
+    d.dispersive |= (smp.lobe & kLobeDispersiveDelta) != 0;
+    ++d.count;
+    return true;
+}
+// traceMNEEOccluders (:107-150): closest hits along the shadow ray; each must be a specular
+// transmitter that samples a transmission lobe. Ends after at most maxOcc hits.
+PLT_DEV bool ms_trace_occluders(const PtScene& sc, V3 org, V3 dir, float tmax, int maxOcc, int maxMSBounces, Rng& sg, MsData& d,
+                                int* stk) {
+    const V3 wi = -dir;
+    for (;;) {
+        const Ray r{org.x, org.y, org.z, dir.x, dir.y, dir.z, 0.f, tmax};
+        const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, r, stk, kPtBlock);
+        if (h.tri == kNoHit) return true;
+        if (d.count >= maxOcc || d.count >= maxMSBounces) return false;
+        const V3 posW = walk_point(org, dir, h.t);
+        Surf s = surface_at(sc, h.tri, h.b1, h.b2, wi);
+        s.pos = posW;
+        const Mat m = material_at(sc, s.material);
+        if (!(mat_props(m) & kPropMWTransmissive)) return false;
+        if (!ms_sample_vertex(h.tri, h.b1, h.b2, posW, s, m, wi, -wi, sg, d) || (d.hits[d.count - 1].lobe & kLobeTransmission) == 0)
+            return false;
+        org = ray_origin(posW, wdot(s.faceN, dir) >= 0.f ? s.faceN : -s.faceN);
+        tmax -= h.t;
+    }
+}
+// manifoldNEESample (:165-281) for an area light at lightPos: Newton steps per vertex, each
+// projected back onto its mesh by a ray from the previous vertex, with a step size that quarters on
+// a miss and doubles (up to 1) on a hit; then the visibility of the light from the last vertex.
+// Every pass over unconverged vertices costs at least one iteration, so the walk ends after at most
+// maxIterations ray casts.
+PLT_DEV bool ms_nee_sample(const PtScene& sc, const PtFrame& fr, V3 visOrg, V3 visDir, float visTmax, V3 origin, V3 lightPos,
+                           int maxMSBounces, Rng& sg, MsData& d, int* stk) {
+    if (!ms_trace_occluders(sc, visOrg, visDir, visTmax, int(fr.mnee_max_occluders), maxMSBounces, sg, d, stk)) return false;
+    if (d.count == 0) return true;
+    float beta[kMaxMsOccluders];
+#pragma unroll
+    for (int i = 0; i < int(kMaxMsOccluders); ++i) beta[i] = 1.f;
+    bool success = false;
+    while (!success) {
+        success = true;
+        for (int idx = 0; idx < d.count; ++idx) {
+            MsHit& h = d.hits[idx];
+            const DSG dsg = dsg_at(sc, h.tri, h.b1, h.b2);
+            const V3 p0 = idx == 0 ? origin : d.hits[idx - 1].pos;
+            const PDV pdv0 = dsg.pdw(p0);
+            const V3 p2 = idx + 1 == d.count ? lightPos : d.hits[idx + 1].pos;
+            const PDV pdv2 = dsg.pdw(p2);
+            // the scattered direction is taken on the into-material side (avoids total internal reflection)
+            const bool woIsIn = wdot(pdv2.v, dsg.n.v) < 0.f;
+            const PDV wi = woIsIn ? pdv0 : pdv2, wo = woIsIn ? pdv2 : pdv0;
+            const Mat m = material_at(sc, sc.mesh_material[sc.tri_mesh[h.tri]]);
+            const PDV wos = mat_partial_derivatives(m, wi, dsg, h.lobe, d.wavelength);
+            if (wos.v.x == 0.f && wos.v.y == 0.f && wos.v.z == 0.f) return false;
+            float lenC;
+            V3 stepDir;
+            if (!manifold_one_step(wo, wos, dsg.p, lenC, stepDir)) return false;
+            if (lenC < fr.mnee_threshold) continue;
+            success = false;
+            for (;;) {  // line search
+                if (++d.iterations > int(fr.mnee_max_iterations)) return false;
+                V3 proposal;
+                {
+#pragma clang fp contract(off)
+                    proposal = h.pos - wscale(stepDir, beta[idx]);
+                }
+                V3 dirTo = proposal - p0;
+                const float dist = wlength(dirTo);
+                dirTo = wdiv(dirTo, dist);
+                const Ray r{p0.x, p0.y, p0.z, dirTo.x, dirTo.y, dirTo.z, fmaxf(1e-4f, .99f * dist), 1.01f * dist};
+                const Hit ch = traverse<false, kTraversalStackDepth>(sc.tsc, r, stk, kPtBlock);
+                if (ch.tri == kNoHit || sc.tri_mesh[ch.tri] != sc.tri_mesh[h.tri]) {
+                    beta[idx] /= 4.f;
+                    continue;
+                }
+                h.tri = ch.tri;
+                h.b1 = ch.b1;
+                h.b2 = ch.b2;
+                h.pos = walk_point(p0, dirTo, ch.t);
+                beta[idx] = fminf(1.f, 2.f * beta[idx]);
+                break;
+            }
+        }
+    }
+    const V3 p = d.hits[d.count - 1].pos;
+    V3 dir = lightPos - p;
+    const float dist = wlength(dir);
+    dir = wdiv(dir, dist);
+    d.woLast = dir;
+    const Ray vr{p.x, p.y, p.z, dir.x, dir.y, dir.z, 1e-4f, dist};
+    return traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock).tri == kNoHit;
+}
+// manifoldForwardUnwind (pltpt_solve.rt.slang:80-93): the beam through the walk's vertices, last first.
+template <int N>
+PLT_DEV void ms_forward_unwind(const PtScene& sc, const PtFrame& fr, const MsData& d, V3 origin, Beam<N>& beam) {
+    for (int idx = d.count - 1; idx >= 0; --idx) {
+        const MsHit& h = d.hits[idx];
+        const V3 p2 = idx == 0 ? origin : d.hits[idx - 1].pos;
+        unwind_step<N>(sc, fr, h.tri, h.b1, h.b2, wnormalize(p2 - h.pos), h.lobe, beam, &h.pos);
+    }
+}
+
 template <int N>
 __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr) {
     __shared__ int stack[kTraversalStackDepth * kPtBlock];
@@ -368,6 +514,7 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
     const bool doNEE = (fr.options & kOptNEE) != 0, doMIS = doNEE && (fr.options & kOptMIS) != 0;
     const bool useEmissive = (fr.options & kOptEmissiveLights) != 0 && sc.n_lights > 0;
     const bool importance = (fr.options & kOptISEmitters) != 0;
+    const bool doMNEE = doNEE && (fr.options & kOptMNEE) != 0;
 
     Rng sg = Rng::make(px, py, fr.frame);
     const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
@@ -383,10 +530,12 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
         const Mat m = material_at(sc, s.material);
         const uint32_t props = mat_props(m);
 
-        // ---- NEE (pltpt_solve.rt.slang:245-366; manifold walks off) ----
+        // ---- NEE (pltpt_solve.rt.slang:245-366) ----
         if (doNEE) {
             const uint32_t lobes = mat_lobes(m);
-            const bool sampleRefl = (lobes & kLobeNonDeltaReflection) != 0;
+            // a delta reflector can take part in a manifold walk from here (pltpt_solve.rt.slang:254-258)
+            const bool allowMS = doMNEE && p > 0 && !(bounce.flags & kFlagLastDelta) && (props & kPropMWReflective);
+            const bool sampleRefl = (lobes & kLobeNonDeltaReflection) != 0 || allowMS;
             const bool sampleTrans = (lobes & kLobeNonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
                 // selectLightType: only emissive geometry is present -> one draw, p = 1
@@ -458,24 +607,66 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
                 for (int j = 0; j < N; ++j) anyLe |= Le[j] > 0.f;
                 if (valid && anyLe) {
                     const int comps = (!(fr.options & kOptHwssMis) && (bounce.flags & kFlagDispersive)) ? 1 : N;
-                    Ray vr{lsOrigin.x, lsOrigin.y, lsOrigin.z, lsDir.x, lsDir.y, lsDir.z, 0.f, lsDist};
-                    const Hit vh = traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock);
-                    if (vh.tri == kNoHit) {
-                        Beam<N> beam = source_beam_area<N>(tPos, -lsDir, fr.sourcing_area, lsDist, Le, lam, fr.max_omega, comps);
-                        if (doMIS) {
+                    Beam<N> beam = source_beam_area<N>(tPos, -lsDir, fr.sourcing_area, lsDist, Le, lam, fr.max_omega, comps);
+                    bool evalCurrent = true, ok = true;
+                    const int maxMSBounces = int(fr.max_bounces) - p;
+                    if (!doMNEE || maxMSBounces <= 0) {
+                        const Ray vr{lsOrigin.x, lsOrigin.y, lsOrigin.z, lsDir.x, lsDir.y, lsDir.z, 0.f, lsDist};
+                        ok = traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock).tri == kNoHit;
+                        if (ok && doMIS) {
                             const float bp = mat_pdf(m, s.f, bounce.wi, lsDir, lam[0]);
                             beam.scale(lsPdf / (lsPdf + bp));
                         }
+                    } else {  // manifold sampling on NEE (pltpt_solve.rt.slang:296-366)
+                        const V3 lightDir = -beam.r;
+                        V3 dirL = lsDir;
+                        float misBsdfPdf = 0.f;
+                        MsData ms;
+                        ms.count = 0;
+                        ms.iterations = 0;
+                        ms.wavelength = beam.lambda[0];
+                        ms.pdf = 1.f;
+                        ms.dispersive = false;
+                        ms.woLast = v3(0.f, 0.f, 0.f);
+                        // MS on the current vertex too (never on the first bounce)?
+                        const bool isRefl2 = dot(s.f.N, lightDir) * dot(s.f.N, bounce.wi) > 0.f;
+                        const bool currentVertexMS = isRefl2 && allowMS;
+                        if (currentVertexMS)
+                            ok = ms_sample_vertex(bounce.tri, bounce.b1, bounce.b2, s.pos, s, m, bounce.wi, lightDir, sg, ms) &&
+                                 (ms.hits[0].lobe & kLobeReflection) != 0;
+                        const V3 origin = !currentVertexMS ? s.pos : sv.origin;
+                        ok = ok && ms_nee_sample(sc, fr, lsOrigin, lsDir, lsDist, origin, beam.pos, maxMSBounces, sg, ms, stk);
+                        ok = ok && !(ms.pdf < 1e-6f);
+                        if (ok && ms.count > 0) {
+                            if (!(fr.options & kOptHwssMis) && ms.dispersive) beam.comps = 1;
+                            dirL = ms.woLast;
+                            beam.r = -ms.woLast;
+                            beam.t = perp_stark(beam.r);
+                            beam.scale(1.f / ms.pdf);
+                            ms_forward_unwind<N>(sc, fr, ms, origin, beam);
+                            if (currentVertexMS) {
+                                evalCurrent = false;
+                                misBsdfPdf = bounce.lastPdf;
+                            }
+                        }
+                        if (ok && doMIS) {
+                            if (ms.count == 0 || !currentVertexMS) misBsdfPdf = mat_pdf(m, s.f, bounce.wi, dirL, beam.lambda[0]);
+                            beam.scale(lsPdf / (lsPdf + misBsdfPdf * ms.pdf));
+                        }
+                    }
+                    if (ok) {
                         // evalNEE (pltpt_solve.rt.slang:219-243)
-                        const V3 V = -beam.r;
-                        const Surf fs = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, V);
-                        beam.propagate(fs.pos);
-                        mat_eval<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam);
-                        if (fr.options & kOptHwssMis) {
-                            float pp[N];
-                            mat_lambda_pdf<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam, pp);
+                        if (evalCurrent) {
+                            const V3 V = -beam.r;
+                            const Surf fs = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, V);
+                            beam.propagate(fs.pos);
+                            mat_eval<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam);
+                            if (fr.options & kOptHwssMis) {
+                                float pp[N];
+                                mat_lambda_pdf<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam, pp);
 #pragma unroll
-                            for (int j = 0; j < N; ++j) beam.spdf[j] *= pp[j];
+                                for (int j = 0; j < N; ++j) beam.spdf[j] *= pp[j];
+                            }
                         }
                         forward_unwind<N>(sc, fr, n, li, p, beam);
                         measure<N>(sc, fr, sv, p, bounce.contrib, beam);
@@ -485,7 +676,10 @@ __global__ void __launch_bounds__(kPtBlock) solve_kernel(PtScene sc, PtFrame fr)
         }
 
         // ---- emissive hit (evalEmissive, pltpt_solve.rt.slang:152-200) ----
-        const bool didNEE = doNEE && p > 0 && !(bounce.flags & kFlagLastDelta);
+        // wasNEEAvailable (pltpt_solve.rt.slang:205-210)
+        const bool didNEE = doNEE && p > 0 &&
+                            (!(bounce.flags & kFlagLastDelta) ||
+                             (doMNEE && (bounce.flags >> 16) <= fr.mnee_max_occluders && p > 1));
         if (useEmissive && (props & kPropEmissive) && (doMIS || !didNEE)) {
             const bool needMIS = doMIS && didNEE;
             float lightPdf = 0.f;

@@ -20,8 +20,11 @@ enum : uint32_t {
 
 enum : uint32_t {  // option bits
     kOptNEE = 0x1, kOptMIS = 0x2, kOptRR = 0x4, kOptHwssMis = 0x8, kOptISEmitters = 0x10, kOptDirectLights = 0x20,
-    kOptEmissiveLights = 0x40
+    kOptEmissiveLights = 0x40, kOptMNEE = 0x80
 };
+
+// Largest number of transmissive occluders a manifold walk may pass (MNEE_MAX_MS_OCCLUDERS).
+constexpr uint32_t kMaxMsOccluders = 8;
 
 struct PtScene {
     TraceScene tsc;
@@ -48,6 +51,9 @@ struct PtFrame {
     uint32_t width, height, frame, max_bounces, hwss, options;
     uint32_t path_base, n_paths;  // paths [path_base, path_base + n_paths) of this launch (row-major pixels)
     float sourcing_area, max_omega;
+    // manifold NEE: occluder limit, Newton iteration limit, constraint threshold (Manifolds.slangh:42-54)
+    uint32_t mnee_max_occluders, mnee_max_iterations;
+    float mnee_threshold;
     uint32_t* records;
     float4* color;
     float4* albedo;

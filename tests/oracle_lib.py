@@ -289,3 +289,15 @@ def material_sample(material: bytes, profiles: bytes, n_profiles: int, wi, wavel
     fn(material, profiles, n_profiles, ptr(w, C.c_float), ptr(lam, C.c_float), n, frame, ptr(wo, C.c_float),
        ptr(pdf, C.c_float), ptr(weight, C.c_float), ptr(lobe, C.c_uint32), ptr(valid, C.c_uint32))
     return wo, pdf, weight, lobe, valid
+
+
+def walk_trig(x, y):
+    """The manifold walk's single-precision acos(x) and atan2(y, x)."""
+    l = lib()
+    l.oracle_walk_trig.restype = None
+    l.oracle_walk_trig.argtypes = [_F, _F, C.c_uint32, _F, _F]
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    a, t = np.zeros_like(x), np.zeros_like(x)
+    l.oracle_walk_trig(ptr(x, C.c_float), ptr(y, C.c_float), x.size, ptr(a, C.c_float), ptr(t, C.c_float))
+    return a, t

@@ -31,7 +31,8 @@ REFERENCE_DEFAULTS = {
 }
 
 OPTION_KW = dict(doNEE="do_nee", doMIS="do_mis", doRussianRoulette="do_rr", HWSSDoMIS="hwss_mis",
-                 doImportanceSampleEmitters="is_emitters", useDirectLights="direct", useEmissiveLights="emissive")
+                 doImportanceSampleEmitters="is_emitters", useDirectLights="direct", useEmissiveLights="emissive",
+                 doMNEE="do_mnee", MNEEMaxOccluders="mnee_max_occluders", MNEEMaxIterations="mnee_max_iterations")
 
 
 def test_pass_options_defaults_and_unknown_keys():
@@ -237,6 +238,8 @@ def _render_pair(gpu_scene, host_arrays, w, h, frame, opts, oracle_threads=8):
     g_rec = p.bounce_records(w, h)
     g_col = color.cpu().numpy().reshape(-1, 4)
     kw = {OPTION_KW[k]: int(v) for k, v in opts.items() if k in OPTION_KW}
+    if "MNEESolverThreshold" in opts:
+        kw["mnee_threshold"] = float(opts["MNEESolverThreshold"])
     o_col, o_rec = O.render(host_arrays, w, h, frame=frame, max_bounces=int(opts["maxBounces"]), records=True,
                             threads=oracle_threads, **kw)
     p.close()
@@ -290,6 +293,13 @@ BASE_OPTS = {"maxBounces": 6, "doMNEE": 0, "HWSS": 4}
     dict(name="8-materials-d8", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=3), opts={"maxBounces": 8}),
     dict(name="8-materials-noHWSSMIS", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=9),
          opts={"maxBounces": 8, "HWSSDoMIS": 0}),
+    dict(name="mnee-8-materials-d8", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=3),
+         opts={"maxBounces": 8, "doMNEE": 1}),
+    dict(name="mnee-dielectric", scene=dict(sphere_count=12, subdivision=3, kinds=1, seed=5), opts={"doMNEE": 1}),
+    dict(name="mnee-noMIS-4occ", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=11),
+         opts={"doMNEE": 1, "doMIS": 0, "MNEEMaxOccluders": 4, "MNEEMaxIterations": 30}),
+    dict(name="mnee-noHWSSMIS", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=12),
+         opts={"doMNEE": 1, "HWSSDoMIS": 0, "MNEESolverThreshold": 1e-4}),
 ], ids=lambda c: c["name"])
 def test_gpu_pltpt_matches_oracle(gpu, case):
     from pltfalcor_amd.scene import Scene
@@ -350,10 +360,64 @@ def test_gpu_pltpt_frames_accumulate_and_options(gpu):
     p.set_scene(s)  # resets the frame counter (PLTPT.cpp:510)
     p.execute(w, h, color, albedo, normal)
     assert np.array_equal(color.cpu().numpy().reshape(-1, 4), frames[0])
-    p.set_option("doMNEE", 1)
+    p.set_option("MNEEMaxOccluders", 9)
     with pytest.raises(_lib.PLTError):
         p.execute(w, h, color, albedo, normal)
     p.close()
+
+
+def _mnee_scene_pair(with_sheet, glass):
+    """Ground, light and either a thin dielectric sheet (y = 4) or a glass slab (faces at y = 1 and
+    1.5, camera below it at y = 0.6), as GPU scene and host arrays. The occluder trace re-offsets its
+    origin at every crossing while only subtracting the hit distance from its extent
+    (Manifolds.slangh:146-147), so the ray's end drifts towards the light by the crossings' offsets;
+    faces at |y| < 2 keep that drift (2 x 9.2e-5) short of the light point's own offset (3.7e-4 at
+    y = 7), where the light's triangle would become an occluder."""
+    from pltfalcor_amd import materials as M
+    from pltfalcor_amd.scene import Scene
+    from pltfalcor_amd.bsdf import default_context
+    if glass:
+        g = M.dielectric(1.5, 0.0)
+        g.specular[0] = M.h(0.0)
+        mats = [M.diffuse(.5, .5, .5), M.diffuse(.8, .8, .8, emission=2), g]
+        quads = [(0.0, 10.0, 1.0, 0), (7.0, 1.5, -1.0, 1), (1.5, 10.0, 1.0, 2), (1.0, 10.0, -1.0, 2)]
+    else:
+        mats = [M.diffuse(.5, .5, .5), M.diffuse(.8, .8, .8, emission=2), M.thin_dielectric(1.5, 1e-3)]
+        quads = [(0.0, 10.0, 1.0, 0), (7.0, 1.5, -1.0, 1)] + ([(4.0, 10.0, 1.0, 2)] if with_sheet else [])
+    a = _look_down(_quad_scene(quads, mats), height=0.6 if glass else 2.0)
+    s = Scene.from_arrays(default_context(), a["positions"], a["normals"], a["indices"], a["tri_mesh"], a["mesh_material"],
+                          a["materials"], a["profiles"], _camera_struct(a["camera"]))
+    return s, a
+
+
+def _camera_struct(cam):
+    from pltfalcor_amd import _lib
+    c = _lib.Camera()
+    for i in range(3):
+        c.pos[i], c.target[i], c.up[i] = float(cam[i]), float(cam[3 + i]), float(cam[6 + i])
+    c.fov_y, c.aspect, c.focal_distance = float(cam[9]), float(cam[10]), float(cam[11])
+    c.jitter[0], c.jitter[1] = float(cam[12]), float(cam[13])
+    return c
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("glass", [False, True], ids=["thin-sheet", "glass-slab"])
+def test_gpu_mnee_flat_occluders_match_oracle(gpu, glass):
+    """Manifold NEE through a thin sheet (straight connection) and through both faces of a glass slab
+    (a two-vertex Newton walk): the GPU image equals the oracle's, including which connections the
+    light's own triangle rejects, and manifold NEE reaches the ground (it is black without it)."""
+    s, a = _mnee_scene_pair(True, glass)
+    opts = dict(BASE_OPTS, maxBounces=2, doMNEE=1, doRussianRoulette=0)
+    w, h = 48, 48
+    g_col, g_rec, o_col, o_rec, _ = _render_pair(s, a, w, h, frame=3, opts=opts)
+    assert np.array_equal(g_rec[0, 0], o_rec[0, 0])
+    assert (o_col[:, :3] > 0).any(axis=1).mean() > 0.05
+    lit_g, lit_o = (g_col[:, :3] > 0).any(axis=1), (o_col[:, :3] > 0).any(axis=1)
+    print(f"[{'glass' if glass else 'sheet'}] lit pixels gpu {lit_g.mean():.4f} oracle {lit_o.mean():.4f} "
+          f"agree {(lit_g == lit_o).mean():.5f}")
+    assert (lit_g == lit_o).mean() >= 0.995
+    close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3])), axis=1)
+    assert close.mean() >= 0.995
 
 
 
@@ -473,3 +537,24 @@ def test_oracle_mnee_walk_satisfies_snell_through_glass_slab():
         sin0, sin1 = np.hypot(d0[0], d0[2]), np.hypot(d1[0], d1[2])
         assert abs(sin0 - eta * sin1) < 1e-3
     assert converged >= 4
+
+
+def test_oracle_walk_trig_accuracy():
+    """The walk's acos / atan2 (fixed polynomial evaluations shared by host and device) against
+    float64: acos within 1.5 ulp on [-1, 1], atan2 within 4 ulp over all quadrants and the axes."""
+    n = 400001
+    x = np.linspace(-1, 1, n).astype(np.float32)
+    rng = np.random.default_rng(1)
+    y = rng.uniform(-1, 1, n).astype(np.float32)
+    y[::11] = 0.0
+    x2 = x.copy()
+    x2[::13] = 0.0
+    ac, _ = O.walk_trig(x, y)
+    _, at = O.walk_trig(x2, y)
+    ra = np.arccos(x.astype(np.float64))
+    rt = np.arctan2(y.astype(np.float64), x2.astype(np.float64))
+    ulp = lambda r: np.spacing(np.abs(r).astype(np.float32)).astype(np.float64)  # noqa: E731
+    assert np.max(np.abs(ac - ra) / ulp(ra)) <= 1.5
+    nz = rt != 0
+    assert np.max(np.abs(at[nz] - rt[nz]) / ulp(rt[nz])) <= 4.0
+    assert np.all(at[~nz] == 0)
