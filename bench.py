@@ -156,7 +156,9 @@ def path_cpu_baseline(scene_np, budget_s: float, opts):
     import oracle_lib as O  # the oracle is the CPU baseline (test infrastructure)
     threads = max(1, min(16, os.cpu_count() or 1))
     osc = O.OracleScene(scene_np)
-    kw = dict(max_bounces=MAX_BOUNCES, threads=threads, do_rr=opts["doRussianRoulette"])
+    kw = dict(max_bounces=MAX_BOUNCES, threads=threads, do_rr=opts["doRussianRoulette"], do_mnee=opts["doMNEE"],
+              mnee_max_occluders=opts["MNEEMaxOccluders"], mnee_max_iterations=opts["MNEEMaxIterations"],
+              mnee_threshold=opts["MNEESolverThreshold"])
     # chunks of 64 rows in frame order, frames 0..SPP-1 in turn, until the budget is spent
     osc.render(WIDTH, HEIGHT, frame=0, first=WIDTH * (HEIGHT // 2), count=WIDTH, **kw)  # warm-up
     chunk = 64 * WIDTH
@@ -210,7 +212,9 @@ def main():
     t_build = time.perf_counter()
     scene = Scene(ctx, aspect=WIDTH / HEIGHT, **PATH_2)
     t_build = time.perf_counter() - t_build
-    opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4, "doMNEE": 0}
+    # the reference's PLTPathTracer graph creates PLTPT with its defaults (manifold NEE on, 2 occluders,
+    # 60 Newton iterations, threshold 5e-5); only the depth and the HWSS count are set here
+    opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4}
     pltpt = PLTPT(ctx, opts)
     pltpt.set_scene(scene)
     dev = torch.device("cuda")
@@ -269,7 +273,8 @@ def main():
                                "(5,242,884 triangles), one 3x3 quad light (6500 K blackbody), 8 materials uniform "
                                "over dielectric / Drude conductor / coated opaque dielectric / grated conductor / "
                                "birefringent thin dielectric / 20-layer stack (FP64 steps) / coated conductor / "
-                               "opaque dielectric; HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE off",
+                               "opaque dielectric; HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE on (<= 2 "
+                               "specular occluders, <= 60 Newton iterations, threshold 5e-5)",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
                    "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
                    "segments_per_frame": seg_per_exec},
@@ -283,7 +288,7 @@ def main():
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = path_cpu_baseline(scene.numpy(), args.cpu_seconds, {"doRussianRoulette": 1})
+        out["cpu_baseline"] = path_cpu_baseline(scene.numpy(), args.cpu_seconds, pltpt.get_scripting_dictionary())
     pltpt.close()
     print(json.dumps(out), flush=True)
 

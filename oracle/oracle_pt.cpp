@@ -800,6 +800,25 @@ int oracle_mnee_probe(void* h, const float* origin, const float* lightPos, float
     return ok ? ms.hitCount : -1;
 }
 
+// The walk's specular direction at barycentrics (b1, b2) of a triangle (positions P, vertex normals
+// N, row-major 3x3) lit from `src`, with its derivatives along the surface parameters
+// (materialPartialDerivatives of dsg.pdw(src)), in FP64: out = v, ddu, ddv.
+void oracle_walk_derivatives(const void* mat, const void* prof, uint32_t nprof, const double* P, const double* N, double b1,
+                             double b2, const double* src, uint32_t lobe, double wavelength, double* out) {
+    using D = double;
+    const Instance<D> in = setupMaterialInstance<D>(*static_cast<const MatData*>(mat), static_cast<const SpectralProfile*>(prof), nprof);
+    V3<D> Pv[3], Nv[3];
+    for (int k = 0; k < 3; ++k) {
+        Pv[k] = V3<D>{P[3 * k], P[3 * k + 1], P[3 * k + 2]};
+        Nv[k] = V3<D>{N[3 * k], N[3 * k + 1], N[3 * k + 2]};
+    }
+    const DSG<D> dsg = makeDSG<D>(Pv, Nv, b1, b2);
+    const PDV<D> wi = dsg.pdw(V3<D>{src[0], src[1], src[2]});
+    const KPDV<D> r = materialPartialDerivatives<D>(in, wi, dsg, lobe, wavelength);
+    const V3<D> o[3] = {r.v, r.ddu, r.ddv};
+    for (int k = 0; k < 3; ++k) { out[3 * k] = o[k].x; out[3 * k + 1] = o[k].y; out[3 * k + 2] = o[k].z; }
+}
+
 // The manifold walk's acos and atan2 (walkAcos, walkAtan2) over arrays.
 void oracle_walk_trig(const float* x, const float* y, uint32_t n, float* acosx, float* atan2yx) {
     for (uint32_t i = 0; i < n; ++i) {

@@ -301,3 +301,19 @@ def walk_trig(x, y):
     a, t = np.zeros_like(x), np.zeros_like(x)
     l.oracle_walk_trig(ptr(x, C.c_float), ptr(y, C.c_float), x.size, ptr(a, C.c_float), ptr(t, C.c_float))
     return a, t
+
+
+def walk_derivatives(material: bytes, profiles: bytes, nprof: int, P, N, b1, b2, src, lobe, wavelength):
+    """FP64 specular direction and its (u, v) derivatives at a walk vertex: array [3 (v, ddu, ddv), 3]."""
+    l = lib()
+    _D = C.POINTER(C.c_double)
+    l.oracle_walk_derivatives.restype = None
+    l.oracle_walk_derivatives.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, _D, _D, C.c_double, C.c_double, _D,
+                                          C.c_uint32, C.c_double, _D]
+    P = np.ascontiguousarray(P, np.float64)
+    N = np.ascontiguousarray(N, np.float64)
+    src = np.ascontiguousarray(src, np.float64)
+    out = np.zeros(9)
+    l.oracle_walk_derivatives(material, profiles, nprof, ptr(P, C.c_double), ptr(N, C.c_double), b1, b2,
+                              ptr(src, C.c_double), lobe, wavelength, ptr(out, C.c_double))
+    return out.reshape(3, 3)

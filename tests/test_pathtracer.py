@@ -558,3 +558,34 @@ def test_oracle_walk_trig_accuracy():
     nz = rt != 0
     assert np.max(np.abs(at[nz] - rt[nz]) / ulp(rt[nz])) <= 4.0
     assert np.all(at[~nz] == 0)
+
+
+def test_oracle_walk_derivatives_match_finite_differences():
+    """The walk's specular direction derivatives along the surface parameters (differential shading
+    geometry of a triangle with non-parallel vertex normals, then the material's partial
+    derivatives) equal central finite differences of the direction itself, in FP64: mirror
+    reflection (conductor, coated conductor), refraction from the normal's side (dielectric) and
+    the thin sheet's straight transmission. (The grating's Jacobian is the reference's
+    approximation, DiffractionGrating.slang:229-232, and refraction from behind the normal is a
+    branch the walk never takes; neither is pinned here.)"""
+    from pltfalcor_amd import materials as M
+    from pltfalcor_amd.scene import Scene
+    pa = Scene(None, seed=1, sphere_count=0, subdivision=1).numpy()
+    prof, nprof = pa["profiles"], pa["n_profiles"]
+    P = np.array([[0, 0, 0], [1.0, 0.1, 0], [0.2, -0.05, 1.0]])
+    N = np.array([[0.1, 1, 0.05], [-0.2, 1, 0.1], [0.05, 1, -0.3]])
+    N /= np.linalg.norm(N, axis=1, keepdims=True)
+    cases = [(M.conductor(0.0, 0, 1), 0x04), (M.coated_conductor(0.0, 1.5, 2.0, 0, 1), 0x04),
+             (M.dielectric(1.5, 0.0), 0x40), (M.dielectric(1.8, 0.0), 0x04), (M.thin_dielectric(1.5, 1e-3), 0x40)]
+    h = 1e-6
+    for mat, lobe in cases:
+        mb = M.pack([mat])
+        for src in ([0.7, 2.0, 0.4], [-1.0, 0.8, 1.5], [2.0, 1.5, -0.5]):
+            for b1, b2 in ((0.3, 0.25), (0.1, 0.7), (0.6, 0.1)):
+                f = lambda u, v: O.walk_derivatives(mb, prof, nprof, P, N, u, v, src, lobe, 550.0)  # noqa: E731
+                r = f(b1, b2)
+                assert abs(np.linalg.norm(r[0]) - 1) < 1e-9
+                du = (f(b1 + h, b2)[0] - f(b1 - h, b2)[0]) / (2 * h)
+                dv = (f(b1, b2 + h)[0] - f(b1, b2 - h)[0]) / (2 * h)
+                assert np.abs(du - r[1]).max() < 1e-6 * max(1.0, np.abs(r[1]).max()), (mat.type, src, b1, b2)
+                assert np.abs(dv - r[2]).max() < 1e-6 * max(1.0, np.abs(r[2]).max()), (mat.type, src, b1, b2)
