@@ -264,6 +264,7 @@ typedef struct plt_scene_arrays {
     const plt_material* materials;
     const plt_spectral_profile* profiles;
     plt_camera camera;
+    uint32_t bvh_max_depth;          /* deepest BVH level (root = 1): the traversal stack bound */
 } plt_scene_arrays;
 int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
 

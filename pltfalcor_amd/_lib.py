@@ -67,7 +67,7 @@ class SceneArrays(C.Structure):
                 ("indices", C.POINTER(C.c_uint32)), ("tri_mesh", C.POINTER(C.c_uint32)),
                 ("mesh_first_tri", C.POINTER(C.c_uint32)), ("mesh_material", C.POINTER(C.c_uint32)),
                 ("materials", C.POINTER(Material)), ("profiles", C.POINTER(SpectralProfile)),
-                ("camera", Camera)]
+                ("camera", Camera), ("bvh_max_depth", C.c_uint32)]
 
 
 class SceneDesc(C.Structure):

@@ -183,6 +183,7 @@ int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out) {
     out->n_materials = uint32_t(h.materials.size());
     out->n_profiles = uint32_t(h.profiles.size());
     out->n_bvh_nodes = uint32_t(h.bvh.nodes.size());
+    out->bvh_max_depth = uint32_t(h.bvh.max_depth);
     out->positions = h.positions.data();
     out->normals = h.normals.data();
     out->indices = h.indices.data();
