@@ -333,6 +333,76 @@ int plt_pass_copy_bounce_records(plt_pass* pass, void* dst, size_t bytes);
 int plt_pass_set_profiling(plt_pass* pass, int enable);
 int plt_pass_get_kernel_times(plt_pass* pass, double* sample_ms, double* solve_ms, uint32_t* executes);
 
+/* ---------------------------------------------------------------------------------------
+ * Learned BSDF surrogate. No counterpart in the reference: a small MLP (22 inputs, three hidden
+ * layers of 128 ReLU units, 16 outputs = 38,032 parameters) that stands in for the exact Mueller
+ * matrix of the costly first-order scattering (SPM) lobes -- the coated opaque dielectric, the
+ * coated conductor and the grated conductor's base lobe -- trained on exact samples of the
+ * material layer above. (The multilayer stack has no scattering lobe, only its delta reflection,
+ * whose Re(G)^2 oscillates too fast in wavelength and angle for a network this size; it stays
+ * exact.) Forward, backward and Adam run on the matrix cores (BF16 operands, FP32 accumulation,
+ * FP32 master weights).
+ *
+ * Inputs X [n x 22] float32, one sample per row (produced by plt_surrogate_generate): wi (3),
+ * wo (3) in the local frame (N = +z), (lambda_nm - 550) / 170, (q_um - 0.325) / 0.275,
+ * (sqrt(T_um2) - 0.46) / 0.41 of the K-correlation surface, (coherence area_um2 - 50.5) / 49.5, a
+ * one-hot family (coated opaque dielectric, coated conductor, grated conductor), the lobe's derived
+ * inputs (ln(k l |wi_t + wo_t| + 1e-3) - 1) / 3, (ln(k q) - 1) / 1.2, (ln(k l) - 2) / 1.5,
+ * (ln(|wi_t + wo_t| + 1e-4) + 1.5) / 2.5 with k = 2 pi / lambda_um, and the material: coated --
+ * 2 albedo(lambda) - 1 (0 for the conductor), coat IoR and thickness mapped from [1.3, 1.8] and
+ * [0.2, 5] um to [-1, 1], 0, 0; grated -- pitch from [0.5, 2] um, the grating direction (cos,
+ * sin), height from [0.05, 0.5] um, 0.
+ * Outputs Y [n x 16]: the network's raw (standardised) targets; plt_surrogate_eval decodes them
+ * into Mueller matrices with the statistics given to plt_surrogate_set_stats.
+ * ------------------------------------------------------------------------------------- */
+typedef struct plt_surrogate plt_surrogate;
+#define PLT_SURROGATE_INPUTS 22
+#define PLT_SURROGATE_OUTPUTS 16
+
+/* max_batch: samples per internal batch (a multiple of 32); seed: weight initialisation. */
+int plt_surrogate_create(plt_context* ctx, uint32_t max_batch, uint32_t seed, plt_surrogate** out);
+void plt_surrogate_destroy(plt_surrogate* s);
+/* Parameter count (38,032) and host copies in the order W1 [128x22], b1, W2 [128x128], b2, W3, b3,
+ * W4 [16x128], b4 (row-major, out x in). Setting parameters resets the Adam state. */
+uint32_t plt_surrogate_param_count(void);
+int plt_surrogate_params(plt_surrogate* s, float* host, int set);
+/* Y = MLP(X) for n samples (device pointers; any n). */
+int plt_surrogate_forward(plt_surrogate* s, uint32_t n, const float* X, float* Y, plt_stream stream);
+/* Mean-squared-error loss over the 16 outputs against targets T [n x 16] and its gradient with
+ * respect to every parameter (host copy in the parameter order when grads_host is not NULL);
+ * n a multiple of 32, at most max_batch. */
+int plt_surrogate_gradients(plt_surrogate* s, uint32_t n, const float* X, const float* T, float* grads_host, float* loss,
+                            plt_stream stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the gradient of a batch. */
+int plt_surrogate_train_step(plt_surrogate* s, uint32_t n, const float* X, const float* T, float lr, float* loss,
+                             plt_stream stream);
+/* Target statistics (targets are y0 = ln M00, yk = asinh(Mk / M00) for k = 1..15, standardised with
+ * mean/std) and the training box of the inputs: a sample with any input outside [x_lo, x_hi]
+ * is out of distribution and is evaluated exactly. */
+int plt_surrogate_set_stats(plt_surrogate* s, const float* y_mean, const float* y_std, const float* x_lo, const float* x_hi);
+
+/* Exact samples for training and judging (the material layer of the path tracer, FP32 with FP64
+ * multilayer steps): per sample i a material (written to materials[i], binary16 fields as
+ * plt_material), its geometry (PLT_SG_* planes of n floats), the inputs X[i] and the exact
+ * Mueller matrix mueller[i] (row-major 4x4, hero wavelength: S_out = M S_in in the beam frames of
+ * plt_material_eval_batch). Sample i draws from the sample generator of pixel (i & 0xFFFF, i >> 16)
+ * at frame `seed`. The conductors use spectral profiles 0 (n) and 1 (k) of the context. */
+enum {
+    PLT_SG_WI_X = 0, PLT_SG_WI_Y, PLT_SG_WI_Z, PLT_SG_WO_X, PLT_SG_WO_Y, PLT_SG_WO_Z, PLT_SG_LAMBDA, PLT_SG_LOBE,
+    PLT_SG_AREA, PLT_SG_PLANES
+};
+int plt_surrogate_generate(plt_context* ctx, uint32_t n, uint32_t seed, plt_material* materials, float* geometry, float* X,
+                           float* mueller, plt_stream stream);
+/* Outgoing Stokes vectors stokes_out [n x 4] for incident stokes_in [n x 4]: mode 0 = the
+ * surrogate, with the exact evaluation for out-of-distribution samples (their count is added to
+ * *fallbacks when not NULL, synchronously); mode 1 = exact evaluation of every sample (the
+ * surrogate's judge). */
+int plt_surrogate_eval(plt_surrogate* s, uint32_t n, const plt_material* materials, const float* geometry, const float* X,
+                       const float* stokes_in, float* stokes_out, int mode, uint32_t* fallbacks, plt_stream stream);
+/* The matrix-core GEMM the MLP is built from: C [M x N] (FP32) = A [M x K] * Bt [N x K]^T with
+ * bf16 row-major operands, K a multiple of 32 (for tests of the kernel). */
+int plt_mlp_gemm_nt(uint32_t M, uint32_t N, uint32_t K, const void* A, const void* Bt, float* C, plt_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

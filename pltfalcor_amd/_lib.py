@@ -95,6 +95,22 @@ SIGNATURES = [
     ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),
     ("plt_material_eval_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
+    ("plt_surrogate_create", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("plt_surrogate_destroy", None, [C.c_void_p]),
+    ("plt_surrogate_param_count", C.c_uint32, []),
+    ("plt_surrogate_params", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_int]),
+    ("plt_surrogate_forward", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("plt_surrogate_gradients", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float), C.c_void_p]),
+    ("plt_surrogate_train_step", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_float,
+                                           C.POINTER(C.c_float), C.c_void_p]),
+    ("plt_surrogate_set_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float)]),
+    ("plt_surrogate_generate", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
+    ("plt_surrogate_eval", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.c_void_p]),
+    ("plt_mlp_gemm_nt", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("plt_material_sample_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                             C.c_void_p, C.c_void_p]),
     ("plt_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),

@@ -1,0 +1,221 @@
+"""The learned BSDF surrogate: the CPU restatement of the MLP pinned by finite differences and by
+torch's bfloat16 rounding; on the GPU, the matrix-core GEMM, forward pass and gradients against
+that restatement, the exact training samples against the FP64 oracle's material evaluation, the
+exact fallback, and what training gains."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import mlp_ref as R
+import oracle_lib as O
+
+
+def test_mlp_restatement_gradient_matches_finite_differences():
+    """Float64 back-propagation of the restated MLP equals central finite differences of its loss."""
+    rng = np.random.default_rng(3)
+    p = R.he_init(1)
+    X = rng.uniform(-1, 1, (12, 22))
+    T = rng.normal(0, 1, (12, 16))
+    _, g = R.loss_and_grad(p, X, T)
+    idx = rng.choice(p.size, 300, replace=False)
+    h = 1e-6
+    for k in idx:
+        q = p.copy()
+        q[k] += h
+        lp, _ = R.loss_and_grad(q, X, T)
+        q[k] -= 2 * h
+        lm, _ = R.loss_and_grad(q, X, T)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+def test_bf16_rounding_matches_torch():
+    import torch
+    x = np.random.default_rng(0).normal(0, 10, 100000).astype(np.float32)
+    x[:10] = [0.0, -0.0, 1.0, 1.00390625, 1.001953125, 3.0e-39, -7.5, 65504.0, 1e30, 2.0 ** -130]
+    ref = torch.tensor(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(R.bf16(x), ref)
+
+
+def test_target_transform_round_trip():
+    from pltfalcor_amd.surrogate import mueller_to_targets, targets_to_mueller
+    M = np.random.default_rng(1).normal(0, 1, (50, 16))
+    M[:, 0] = np.abs(M[:, 0]) + 0.1
+    assert np.allclose(targets_to_mueller(mueller_to_targets(M)), M, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------ GPU
+
+
+def _ctx_with_drude():
+    """The default context (the synthetic Drude conductor's n, k profiles at ids 0 and 1)."""
+    from pltfalcor_amd.bsdf import default_context
+    from pltfalcor_amd.spectra import drude_profiles
+    n_prof, k_prof = drude_profiles()
+    return default_context(), bytes(n_prof) + bytes(k_prof)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("M,N,K", [(16, 128, 32), (100, 70, 64), (64, 64, 128), (300, 16, 256), (128, 128, 4096)])
+def test_gpu_mfma_gemm_matches_numpy(gpu, M, N, K):
+    """C = A Bt^T on the matrix cores equals the float64 product of the same bf16 operands (their
+    products are exact in FP32; only the FP32 accumulation differs)."""
+    import torch
+    from pltfalcor_amd import _lib
+    g = torch.Generator().manual_seed(M * 7 + N * 13 + K)
+    A = torch.randn((M, K), generator=g).to(torch.bfloat16)
+    B = torch.randn((N, K), generator=g).to(torch.bfloat16)
+    Ad, Bd = A.to("cuda"), B.to("cuda")
+    Cd = torch.full((M, N), float("nan"), dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().plt_mlp_gemm_nt(M, N, K, C.c_void_p(Ad.data_ptr()), C.c_void_p(Bd.data_ptr()),
+                                           C.c_void_p(Cd.data_ptr()), None))
+    torch.cuda.synchronize()
+    ref = A.double().numpy() @ B.double().numpy().T
+    got = Cd.cpu().numpy()
+    assert np.all(np.abs(got - ref) <= 1e-5 * np.sqrt(K) * (np.abs(A.double().numpy()) @ np.abs(B.double().numpy()).T) + 1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_forward_and_gradients_match_restatement(gpu):
+    """Forward outputs, loss and every parameter's gradient of the matrix-core MLP equal the CPU
+    restatement with bf16 rounding at the same places (up to FP32 accumulation order, which can
+    flip an occasional bf16 rounding)."""
+    import torch
+    from pltfalcor_amd.surrogate import Surrogate
+    ctx, _ = _ctx_with_drude()
+    s = Surrogate(ctx, max_batch=512, seed=5)
+    p = R.he_init(7).astype(np.float32)
+    s.set_params(p)
+    assert np.array_equal(s.get_params(), p)
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, (448, 22)).astype(np.float32)
+    T = rng.normal(0, 1, (448, 16)).astype(np.float32)
+    Xd, Td = torch.tensor(X, device="cuda"), torch.tensor(T, device="cuda")
+    Y = s.forward(Xd).cpu().numpy()
+    Yr, _ = R.forward(p.astype(np.float64), X.astype(np.float64), R.bf16)
+    scale = np.abs(Yr).max()
+    assert np.abs(Y - Yr).max() <= 1e-2 * scale
+    assert np.abs(Y - Yr).mean() <= 1e-3 * scale
+    g, loss = s.gradients(Xd, Td)
+    lr, gr = R.loss_and_grad(p.astype(np.float64), X.astype(np.float64), T.astype(np.float64), R.bf16)
+    assert abs(loss - lr) <= 1e-3 * lr
+    o = 0
+    for name, shape in R.SHAPES:
+        k = int(np.prod(shape))
+        a, b = g[o:o + k], gr[o:o + k]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"{name}: relative gradient error {err:.2e}")
+        assert err <= 2e-2, name
+        o += k
+    # a train step moves the parameters along -gradient (Adam's first step is lr * sign(g))
+    s.train_step(Xd, Td, 1e-3)
+    d = s.get_params() - p
+    big = np.abs(gr) > 1e-3 * np.abs(gr).max()
+    assert np.mean(np.sign(d[big]) == -np.sign(gr[big])) > 0.99
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_samples_match_fp64_oracle(gpu):
+    """The generator's materials, inputs and exact Mueller matrices: each column equals the FP64
+    oracle's evaluation of the same binary16 material for that unit Stokes vector."""
+    import torch
+    from pltfalcor_amd.surrogate import ExactSamples
+    from pltfalcor_amd import materials as PM
+    ctx, prof = _ctx_with_drude()
+    n = 2048
+    S = ExactSamples(ctx, n, seed=3)
+    mats = S.materials.cpu().numpy()
+    geo = S.geometry.cpu().numpy()
+    X = S.X.cpu().numpy()
+    M = S.M.cpu().numpy()
+    assert np.isfinite(X).all() and np.isfinite(M).all()
+    fam = X[:, 10:13].argmax(1)
+    assert np.all(X[:, 10:13].sum(1) == 1)
+    types = mats[:, :4].copy().view(np.uint32)[:, 0]
+    want = np.array([PM.COATED_OPAQUE_DIELECTRIC, PM.COATED_CONDUCTOR, PM.GRATED_CONDUCTOR])[fam]
+    assert np.array_equal(types, want)
+    assert all((fam == f).mean() > 0.28 for f in range(3))
+    # the derived lobe input is the log of k l |wi_t + wo_t| of the sample's own geometry
+    lam_um = geo[6] * 1e-3
+    ht = np.hypot(geo[0] + geo[3], geo[1] + geo[4])
+    l_um = X[:, 8] * .41 + .46
+    assert np.allclose(X[:, 13], (np.log(2 * np.pi / lam_um * l_um * ht + 1e-3) - 1) / 3, atol=2e-5)
+    bad = 0
+    for i in range(0, n, 7):
+        wi, wo = geo[0:3, i].astype(np.float64), geo[3:6, i].astype(np.float64)
+        lobe = int(geo[7, i:i + 1].view(np.uint32)[0])
+        lam = np.full(4, geo[6, i], np.float64)
+        Mo = np.zeros((4, 4))
+        for c in range(4):
+            st = np.zeros(16)
+            st[c] = 1.0
+            out, _, _ = O.material_eval(bytes(mats[i]), prof, 2, wi, wo, lobe, lam, st, float(geo[8, i]), True)
+            Mo[:, c] = out[0]
+        tol = 2e-3 * max(abs(Mo[0, 0]), 1e-12) + 1e-9
+        bad += np.abs(M[i].reshape(4, 4) - Mo).max() > tol
+    assert bad <= 0.01 * len(range(0, n, 7)), bad
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_out_of_distribution_falls_back_to_exact(gpu):
+    """Inputs outside the training box are evaluated exactly (bit-identical to the exact mode) and
+    counted; inside the box the network is used."""
+    import torch
+    from pltfalcor_amd.surrogate import ExactSamples, Surrogate
+    ctx, _ = _ctx_with_drude()
+    S = ExactSamples(ctx, 1000, seed=9)
+    s = Surrogate(ctx, max_batch=256, seed=1)
+    rng = np.random.default_rng(2)
+    Sin = torch.tensor(np.c_[np.ones(1000), rng.uniform(-.5, .5, (1000, 3))].astype(np.float32), device="cuda")
+    ymu, ysd = np.zeros(16), np.ones(16)
+    X = S.X.cpu().numpy()
+    # box that excludes the samples with wi.z < 0.5
+    lo, hi = X.min(0) - 1, X.max(0) + 1
+    lo[2] = 0.5
+    s.set_stats(ymu, ysd, lo, hi)
+    out, fb = s.eval(S, Sin, mode=0)
+    ex, fb1 = s.eval(S, Sin, mode=1)
+    ood = X[:, 2] < 0.5
+    assert fb == ood.sum() and fb1 == 0
+    o, e = out.cpu().numpy(), ex.cpu().numpy()
+    assert np.array_equal(o[ood], e[ood])
+    assert not np.array_equal(o[~ood], e[~ood])
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_training_reduces_error(gpu):
+    """A short training run on exact samples (matrix-core forward/backward + Adam) lowers the
+    training loss and predicts held-out radiance (S0 for random partially polarised incident
+    light) much better than the best input-independent predictor (each target's mean)."""
+    import torch
+    from pltfalcor_amd.surrogate import ExactSamples, Surrogate, train, mueller_to_targets
+    ctx, _ = _ctx_with_drude()
+    tr = ExactSamples(ctx, 1 << 18, seed=21)
+    te = ExactSamples(ctx, 1 << 14, seed=22)
+    s = Surrogate(ctx, max_batch=4096, seed=3)
+    losses = train(s, tr, epochs=8, batch=4096, lr=2e-3)
+    print("losses", losses)
+    assert losses[-1] < 0.7 * losses[0]
+    rng = np.random.default_rng(4)
+    n = te.n
+    Sin = torch.tensor(np.c_[np.ones(n), rng.uniform(-.5, .5, (n, 3))].astype(np.float32), device="cuda")
+    out, fb = s.eval(te, Sin, mode=0)
+    ex, _ = s.eval(te, Sin, mode=1)
+    o, e = out.cpu().numpy(), ex.cpu().numpy()
+    ok = e[:, 0] > 1e-6
+    rel = np.abs(o[ok, 0] - e[ok, 0]) / e[ok, 0]
+    Mtr = tr.M.cpu().numpy()
+    good = Mtr[:, 0] > 1e-12
+    Mb = np.sinh(mueller_to_targets(Mtr[good]).mean(0))
+    Mb[0] = np.exp(mueller_to_targets(Mtr[good]).mean(0)[0])
+    Mb[1:] *= Mb[0]
+    base = np.einsum("j,ij->i", Mb[:4], Sin.cpu().numpy())
+    rel_base = np.abs(base[ok] - e[ok, 0]) / e[ok, 0]
+    print(f"held-out S0 relative error: surrogate median {np.median(rel):.4f} mean {rel.mean():.4f}; "
+          f"mean-target baseline median {np.median(rel_base):.4f}; fallbacks {fb}")
+    assert np.median(rel) < 0.5 * np.median(rel_base)
+    assert fb < 0.01 * n
+    s.close()
