@@ -12,6 +12,7 @@
 // Layers (parameter order of plt_surrogate_params): W1 [128 x 22], b1 [128], W2 [128 x 128], b2,
 // W3 [128 x 128], b3, W4 [16 x 128], b4 [16] = 38,032 FP32 values. Inputs are padded to 32
 // features with zeros (W1's padding columns stay zero).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -133,6 +134,134 @@ int gemm(const GemmArgs& g, int splits, hipStream_t st) {
     const dim3 grid((g.N + kTile - 1) / kTile, (g.M + kTile - 1) / kTile, splits);
     hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(kGemmThreads), 0, st, g);
     return check_launch("gemm_nt_kernel");
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused inference: all four layers in one persistent kernel. A workgroup of four waves loads
+// the bf16 weights once into registers -- wave w holds the B fragments of output columns
+// 32w..32w+31 of layers 1-3 (40 VGPRs... 72 with layer 1) and all of layer 4 (16 VGPRs) -- and
+// then walks 64-sample tiles: each layer reads its input rows from LDS (A fragments), multiplies
+// by its register-resident columns, and writes ReLU(acc + bias) back to LDS for the next layer.
+// ---------------------------------------------------------------------------------------
+constexpr int kFTile = 64, kFThreads = 256;
+constexpr int kXRow = kInPad + 8, kHRow = kHid + 8;  // padded LDS rows (bank spread, 16 B aligned)
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+// With Sout: instead of writing the raw outputs, each sample inside the training box
+// (stats[32..54) / [54..76)) has its Mueller matrix decoded (M00 = exp(y0), Mk = M00 sinh(yk)
+// after de-standardising with stats[0..16) / [16..32)) and applied to Sin; samples outside the box
+// are left for the exact pass.
+__global__ void __launch_bounds__(kFThreads) mlp_infer_kernel(int n, const float* __restrict__ X, const __bf16* __restrict__ W1,
+                                                              const __bf16* __restrict__ W2, const __bf16* __restrict__ W3,
+                                                              const __bf16* __restrict__ W4, const float* __restrict__ params,
+                                                              float* __restrict__ Y, const float* __restrict__ stats,
+                                                              const float* __restrict__ Sin, float* __restrict__ Sout) {
+    __shared__ __bf16 xs[kFTile * kXRow];
+    __shared__ __bf16 ha[kFTile * kHRow];
+    __shared__ __bf16 hb[kFTile * kHRow];
+    __shared__ int outside[kFTile];  // sample outside the training box (apply mode)
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    // register-resident weights: B fragment (col block cb, k block kb) = W[col][32 kb + fk .. +7]
+    bf16x8 w1[2], w2[2][4], w3[2][4], w4[4];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+        const int col = 32 * w + 16 * cb + fr;
+        w1[cb] = *reinterpret_cast<const bf16x8*>(W1 + size_t(col) * kInPad + fk);
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) {
+            w2[cb][kb] = *reinterpret_cast<const bf16x8*>(W2 + size_t(col) * kHid + 32 * kb + fk);
+            w3[cb][kb] = *reinterpret_cast<const bf16x8*>(W3 + size_t(col) * kHid + 32 * kb + fk);
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) w4[kb] = *reinterpret_cast<const bf16x8*>(W4 + size_t(fr) * kHid + 32 * kb + fk);
+    const float* b1 = params + kHid * kIn;
+    const float* b2 = b1 + kHid + kHid * kHid;
+    const float* b3 = b2 + kHid + kHid * kHid;
+    const float* b4 = b3 + kHid + kOut * kHid;
+    const int ntiles = (n + kFTile - 1) / kFTile;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int r0 = tile * kFTile;
+        // stage the inputs as bf16 (zero padded rows and features); flag samples outside the box
+        if (t < kFTile) outside[t] = 0;
+        __syncthreads();
+        for (int e = t; e < kFTile * kInPad; e += kFThreads) {
+            const int r = e / kInPad, f = e % kInPad;
+            const float v = (r0 + r < n && f < kIn) ? X[size_t(r0 + r) * kIn + f] : 0.f;
+            xs[r * kXRow + f] = static_cast<__bf16>(v);
+            if (Sout && f < kIn && !(v >= stats[2 * kOut + f] && v <= stats[2 * kOut + kIn + f])) outside[r] = 1;
+        }
+        __syncthreads();
+        // hidden layers: wave w computes columns 32w..32w+31 of all 64 rows
+        auto hidden = [&](const __bf16* in, int in_row, int kblocks, const bf16x8 (*wf)[4], const bf16x8* wf1, const float* bias,
+                          __bf16* out) {
+            f32x4 acc[4][2];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int kb = 0; kb < 4; ++kb) {
+                    if (kb >= kblocks) break;
+                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(in + (16 * rb + fr) * in_row + 32 * kb + fk);
+#pragma unroll
+                    for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mfma(a, wf ? wf[cb][kb] : wf1[cb], acc[rb][cb]);
+                }
+#pragma unroll
+            for (int cb = 0; cb < 2; ++cb) {
+                const int col = 32 * w + 16 * cb + fr;
+                const float bv = bias[col];
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        out[(16 * rb + 4 * (lane >> 4) + r) * kHRow + col] = static_cast<__bf16>(fmaxf(acc[rb][cb][r] + bv, 0.f));
+            }
+        };
+        hidden(xs, kXRow, 1, nullptr, w1, b1, ha);
+        __syncthreads();
+        hidden(ha, kHRow, 4, w2, nullptr, b2, hb);
+        __syncthreads();
+        hidden(hb, kHRow, 4, w3, nullptr, b3, ha);
+        __syncthreads();
+        // output layer: wave w computes rows 16w..16w+15, all 16 outputs
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < 4; ++kb) acc = mfma(*reinterpret_cast<const bf16x8*>(ha + (16 * w + fr) * kHRow + 32 * kb + fk), w4[kb], acc);
+        if (!Sout) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r0 + 16 * w + 4 * (lane >> 4) + r;
+                if (row < n) Y[size_t(row) * kOut + fr] = acc[r] + b4[fr];
+            }
+        } else {
+            // gather each sample's 16 outputs in one lane through LDS (the input tile is no longer needed)
+            float* ys = reinterpret_cast<float*>(xs);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ys[(16 * w + 4 * (lane >> 4) + r) * kOut + fr] = acc[r] + b4[fr];
+            __syncthreads();
+            // thread t: Stokes component r = t % 4 of sample t / 4
+            const int lr = t >> 2, r = t & 3, row = r0 + lr;
+            if (row < n && !outside[lr]) {
+                const float* yr = ys + lr * kOut;
+                const float m00 = expf(yr[0] * stats[kOut] + stats[0]);
+                const float* S = Sin + size_t(row) * 4;
+                float acc4 = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int k = 4 * r + c;
+                    const float v = k == 0 ? 1.f : sinhf(yr[k] * stats[kOut + k] + stats[k]);
+                    acc4 += v * S[c];
+                }
+                Sout[size_t(row) * 4 + r] = m00 * acc4;
+            }
+        }
+        __syncthreads();  // the next tile overwrites xs and ha
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -362,10 +491,15 @@ int pack(plt_surrogate* s, int n, const float* X, bool with_t, hipStream_t st) {
 
 }  // namespace
 
-int plt::surrogate_forward_batch(plt_surrogate* s, int n, const float* X, hipStream_t st) {
-    int rc;
-    if ((rc = pack(s, n, X, false, st)) < 0) return rc;
-    return forward(s, n, false, st);
+int plt::surrogate_infer(plt_surrogate* s, int n, const float* X, float* Y, const float* Sin, float* Sout, hipStream_t st) {
+    if (n <= 0) return PLT_OK;
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int tiles = (n + kFTile - 1) / kFTile;
+    const int grid = std::max(1, std::min(tiles, 2 * cus));
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(kFThreads), 0, st, n, X, s->W1, s->W2, s->W3, s->W4, s->params, Y, s->stats,
+                       Sin, Sout);
+    return check_launch("mlp_infer_kernel");
 }
 
 extern "C" {
@@ -448,15 +582,8 @@ int plt_surrogate_params(plt_surrogate* s, float* host, int set) {
 
 int plt_surrogate_forward(plt_surrogate* s, uint32_t n, const float* X, float* Y, plt_stream stream) {
     if (!s || (n && (!X || !Y))) return set_error(PLT_EINVAL, "plt_surrogate_forward: bad arguments");
-    const hipStream_t st = static_cast<hipStream_t>(stream);
-    for (uint32_t b = 0; b < n; b += s->max_batch) {
-        const int m = int(std::min(s->max_batch, n - b));
-        int rc;
-        if ((rc = surrogate_forward_batch(s, m, X + size_t(b) * kIn, st)) < 0) return rc;
-        if (hipMemcpyAsync(Y + size_t(b) * kOut, s->Y, size_t(m) * kOut * sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess)
-            return set_error(PLT_EHIP, "plt_surrogate_forward: copy failed");
-    }
-    return PLT_OK;
+    if (n > (1u << 30)) return set_error(PLT_EINVAL, "plt_surrogate_forward: n too large");
+    return surrogate_infer(s, int(n), X, Y, nullptr, nullptr, static_cast<hipStream_t>(stream));
 }
 
 int plt_surrogate_gradients(plt_surrogate* s, uint32_t n, const float* X, const float* T, float* grads_host, float* loss,

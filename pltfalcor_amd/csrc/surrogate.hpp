@@ -25,6 +25,8 @@ struct plt_surrogate {
 };
 
 namespace plt {
-// Forward pass of n <= max_batch samples X [n x 22] (device); the raw outputs are left in s->Y.
-int surrogate_forward_batch(plt_surrogate* s, int n, const float* X, hipStream_t st);
+// Fused inference of n samples X [n x 22] (device): the raw outputs into Y [n x 16], or, with
+// Sout, the decoded Mueller matrix applied to Sin [n x 4] for samples inside the training box
+// (the others are left untouched for the exact pass).
+int surrogate_infer(plt_surrogate* s, int n, const float* X, float* Y, const float* Sin, float* Sout, hipStream_t st);
 }  // namespace plt

@@ -157,47 +157,29 @@ __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t 
     }
 }
 
-// Surrogate (decoded outputs applied to the incident Stokes vector) or exact evaluation.
-__global__ void __launch_bounds__(kSBlock) eval_kernel(uint32_t n, size_t stride, const plt_material* __restrict__ mats,
-                                                     const float* __restrict__ geo,
-                                                     const float* __restrict__ X, const float* __restrict__ Y,
-                                                     const float* __restrict__ stats, const float* __restrict__ Sin,
-                                                     float* __restrict__ Sout, int mode, uint32_t* __restrict__ fallbacks,
-                                                     const plt_spectral_profile* prof, uint32_t nprof) {
+// Exact evaluation: of every sample (mode 1), or (mode 0) of the samples outside the training box
+// (stats[32..54) / [54..76)), counting them; the surrogate has written the others.
+__global__ void __launch_bounds__(kSBlock) exact_kernel(uint32_t n, const plt_material* __restrict__ mats, const float* __restrict__ geo,
+                                                      const float* __restrict__ X, const float* __restrict__ stats,
+                                                      const float* __restrict__ Sin, float* __restrict__ Sout, int mode,
+                                                      uint32_t* __restrict__ fallbacks, const plt_spectral_profile* prof, uint32_t nprof) {
     const uint32_t i = blockIdx.x * kSBlock + threadIdx.x;
     if (i >= n) return;
-    const float* S = Sin + size_t(i) * 4;
-    float* out = Sout + size_t(i) * 4;
-    bool exact = mode != 0;
-    if (!exact) {
-        const float* lo = stats + 2 * kOut;
-        const float* hi = lo + kIn;
+    if (mode == 0) {
+        bool inside = true;
         for (int k = 0; k < kIn; ++k) {
             const float v = X[size_t(i) * kIn + k];
-            exact |= !(v >= lo[k] && v <= hi[k]);
+            inside &= v >= stats[2 * kOut + k] && v <= stats[2 * kOut + kIn + k];
         }
-        if (exact) atomicAdd(fallbacks, 1u);
+        if (inside) return;
+        atomicAdd(fallbacks, 1u);
     }
-    if (exact) {
-        const size_t N = stride;  // geometry planes hold all samples of the call
-        const V3 wi = v3(geo[PLT_SG_WI_X * N + i], geo[PLT_SG_WI_Y * N + i], geo[PLT_SG_WI_Z * N + i]);
-        const V3 wo = v3(geo[PLT_SG_WO_X * N + i], geo[PLT_SG_WO_Y * N + i], geo[PLT_SG_WO_Z * N + i]);
-        const Mat m = setup_material(mats[i], prof, nprof);
-        exact_stokes(m, wi, wo, __float_as_uint(geo[PLT_SG_LOBE * N + i]), geo[PLT_SG_LAMBDA * N + i], geo[PLT_SG_AREA * N + i], S, out);
-        return;
-    }
-    // decode: y = t * std + mean; M00 = exp(y0), Mk = sinh(yk) * M00
-    float Mv[kOut];
-    const float* y = Y + size_t(i) * kOut;
-#pragma unroll
-    for (int k = 0; k < kOut; ++k) Mv[k] = y[k] * stats[kOut + k] + stats[k];
-    const float m00 = expf(Mv[0]);
-    Mv[0] = 1.f;
-#pragma unroll
-    for (int k = 1; k < kOut; ++k) Mv[k] = sinhf(Mv[k]);
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-        out[r] = m00 * (Mv[4 * r] * S[0] + Mv[4 * r + 1] * S[1] + Mv[4 * r + 2] * S[2] + Mv[4 * r + 3] * S[3]);
+    const size_t N = n;
+    const V3 wi = v3(geo[PLT_SG_WI_X * N + i], geo[PLT_SG_WI_Y * N + i], geo[PLT_SG_WI_Z * N + i]);
+    const V3 wo = v3(geo[PLT_SG_WO_X * N + i], geo[PLT_SG_WO_Y * N + i], geo[PLT_SG_WO_Z * N + i]);
+    const Mat m = setup_material(mats[i], prof, nprof);
+    exact_stokes(m, wi, wo, __float_as_uint(geo[PLT_SG_LOBE * N + i]), geo[PLT_SG_LAMBDA * N + i], geo[PLT_SG_AREA * N + i],
+                 Sin + size_t(i) * 4, Sout + size_t(i) * 4);
 }
 
 }  // namespace
@@ -237,15 +219,11 @@ int plt_surrogate_eval(plt_surrogate* s, uint32_t n, const plt_material* materia
     uint32_t* dcount = s->counter;
     int rc = PLT_OK;
     if (hipMemsetAsync(dcount, 0, sizeof(uint32_t), st) != hipSuccess) return set_error(PLT_EHIP, "plt_surrogate_eval: clear failed");
-    for (uint32_t b = 0; b < n; b += s->max_batch) {
-        const uint32_t m = std::min(s->max_batch, n - b);
-        if (mode == 0 && (rc = surrogate_forward_batch(s, int(m), X + size_t(b) * kIn, st)) < 0) return rc;
-        // geometry planes are n floats long: sample b + j of plane p is (geometry + b)[p * n + j]
-        hipLaunchKernelGGL(eval_kernel, dim3((m + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, st, m, size_t(n), materials + b, geometry + b,
-                           X + size_t(b) * kIn, s->Y, s->stats, stokes_in + size_t(b) * 4, stokes_out + size_t(b) * 4, mode, dcount,
-                           s->ctx->d_profiles, s->ctx->n_profiles);
-        if ((rc = check_launch("surrogate eval_kernel")) < 0) return rc;
-    }
+    if (n > (1u << 30)) return set_error(PLT_EINVAL, "plt_surrogate_eval: n too large");
+    if (mode == 0 && (rc = surrogate_infer(s, int(n), X, nullptr, stokes_in, stokes_out, st)) < 0) return rc;
+    hipLaunchKernelGGL(exact_kernel, dim3((n + kSBlock - 1) / kSBlock), dim3(kSBlock), 0, st, n, materials, geometry, X, s->stats, stokes_in,
+                       stokes_out, mode, dcount, s->ctx->d_profiles, s->ctx->n_profiles);
+    if ((rc = check_launch("surrogate exact_kernel")) < 0) return rc;
     if (fallbacks) {
         uint32_t c = 0;
         if (hipMemcpyAsync(&c, dcount, sizeof(c), hipMemcpyDeviceToHost, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)

@@ -147,29 +147,34 @@ def fit_stats(X, M):
 def train(sur: Surrogate, samples: ExactSamples, epochs: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0,
           log=None):
     """Standardises the targets of `samples` (valid M00 > 0 only), sets the statistics, and runs
-    `epochs` passes of Adam with a cosine-decayed learning rate. Returns the per-epoch mean loss."""
+    `epochs` passes of Adam with a cosine-decayed learning rate. Returns the per-epoch mean loss
+    (sampled every 16 batches)."""
     import torch
-    X, M = samples.X.cpu().numpy(), samples.M.cpu().numpy()
-    ok = np.isfinite(M).all(1) & (M[:, 0] > 1e-12)
-    mean, std, lo, hi = fit_stats(X[ok], M[ok])
-    sur.set_stats(mean, std, lo, hi)
-    T = ((mueller_to_targets(np.where(ok[:, None], M, 1.0)) - mean) / std).astype(np.float32)
-    idx = np.nonzero(ok)[0]
-    dev = samples.X.device
-    Xd = samples.X
-    Td = torch.tensor(T, device=dev)
-    rng = np.random.default_rng(seed)
-    n = (idx.size // batch) * batch
+    M, X = samples.M.double(), samples.X
+    ok = torch.isfinite(M).all(1) & (M[:, 0] > 1e-12)
+    m00 = torch.where(ok, M[:, 0], torch.ones_like(M[:, 0]))
+    y = torch.asinh(M / m00[:, None])
+    y[:, 0] = torch.log(m00)
+    yv = y[ok]
+    mean, std = yv.mean(0), yv.std(0, unbiased=False)
+    std = torch.where(std > 1e-6, std, torch.ones_like(std)) / torch.tensor(OUTPUT_SCALE, device=std.device)
+    Xv = X[ok]
+    sur.set_stats(mean.cpu().numpy(), std.cpu().numpy(), Xv.min(0).values.cpu().numpy(), Xv.max(0).values.cpu().numpy())
+    T = ((y - mean) / std).float()
+    del y, yv, Xv, M
+    idx = torch.nonzero(ok).flatten()
+    gen = torch.Generator(device=idx.device).manual_seed(seed)
+    n = (idx.numel() // batch) * batch
     steps_total = epochs * (n // batch)
     step = 0
     losses = []
     for ep in range(epochs):
-        perm = torch.tensor(rng.permutation(idx)[:n], device=dev)
+        perm = idx[torch.randperm(idx.numel(), generator=gen, device=idx.device)[:n]]
         ep_loss = []
         for b in range(0, n, batch):
             sel = perm[b:b + batch]
-            xb = Xd.index_select(0, sel).contiguous()
-            tb = Td.index_select(0, sel).contiguous()
+            xb = X.index_select(0, sel).contiguous()
+            tb = T.index_select(0, sel).contiguous()
             lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / max(1, steps_total)))
             want = (b // batch) % 16 == 0
             l = sur.train_step(xb, tb, max(lr_t, 1e-6), want_loss=want)
