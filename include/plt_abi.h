@@ -324,6 +324,15 @@ int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream str
  * Returns PLT_EINVAL if `bytes` is smaller than the record buffer. */
 int plt_pass_copy_bounce_records(plt_pass* pass, void* dst, size_t bytes);
 
+/* Schedule of the sample pass (no counterpart in the reference, whose sample pass is one DXR ray
+ * generation shader): PLT_SCHEDULE_MEGAKERNEL (default, the faster on MI355X) runs one kernel in
+ * which each lane follows its path; PLT_SCHEDULE_WAVEFRONT runs it as per-bounce kernels over
+ * queues of live paths -- a traversal-only kernel, a counting sort of the hits by material type,
+ * and a shading kernel over the material-sorted queue. Both produce identical bounce records and
+ * outputs. */
+enum { PLT_SCHEDULE_MEGAKERNEL = 0, PLT_SCHEDULE_WAVEFRONT = 1 };
+int plt_pass_set_schedule(plt_pass* pass, int schedule);
+
 /* Kernel timing, the counterpart of the reference's profiler scopes around its sample and solve
  * dispatches (FALCOR_PROFILE in Falcor passes; PLTPT.cpp:487-495 issues the two dispatches):
  * while enabled, every execute records HIP events on its stream around the sample and solve

@@ -95,6 +95,7 @@ SIGNATURES = [
     ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),
     ("plt_material_eval_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
+    ("plt_pass_set_schedule", C.c_int, [C.c_void_p, C.c_int]),
     ("plt_surrogate_create", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     ("plt_surrogate_destroy", None, [C.c_void_p]),
     ("plt_surrogate_param_count", C.c_uint32, []),

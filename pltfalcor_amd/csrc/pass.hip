@@ -150,6 +150,10 @@ struct plt_pass {
     bool profiling = false;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
+    // sample-pass schedule (PLT_SCHEDULE_*) and the wavefront schedule's buffers
+    int schedule = PLT_SCHEDULE_MEGAKERNEL;
+    void* d_wf = nullptr;
+    size_t wf_paths = 0;
 };
 
 namespace plt {
@@ -203,8 +207,16 @@ int plt_pass_destroy(plt_pass* p) {
     if (!p) return PLT_OK;
     plt::free_scene_state(p);
     if (p->d_records) (void)hipFree(p->d_records);
+    if (p->d_wf) (void)hipFree(p->d_wf);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     delete p;
+    return PLT_OK;
+}
+
+int plt_pass_set_schedule(plt_pass* p, int schedule) {
+    if (!p || (schedule != PLT_SCHEDULE_MEGAKERNEL && schedule != PLT_SCHEDULE_WAVEFRONT))
+        return set_error(PLT_EINVAL, "plt_pass_set_schedule: bad arguments");
+    p->schedule = schedule;
     return PLT_OK;
 }
 
@@ -352,7 +364,35 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         p->events_used += 3;
         (void)hipEventRecord(ev[0], st);
     }
-    int rc = plt::launch_sample_pass(sc, fr, st);
+    int rc;
+    if (p->schedule == PLT_SCHEDULE_WAVEFRONT) {
+        if (npaths > p->wf_paths) {  // per path: 5 x 16 B state + 4 B length + 4 x 4 B queues and keys
+            if (p->d_wf) (void)hipFree(p->d_wf);
+            p->d_wf = nullptr;
+            p->wf_paths = 0;
+            void* q = nullptr;
+            const size_t bytes = npaths * (5 * 16 + 4 + 4 * 4) + plt::kWfCounters * 4 + 256;
+            if (hipMalloc(&q, bytes) != hipSuccess) return set_error(PLT_ENOMEM, "plt_pass_execute: wavefront buffers allocation failed");
+            p->d_wf = q;
+            p->wf_paths = npaths;
+        }
+        char* b = static_cast<char*>(p->d_wf);
+        const size_t cap = p->wf_paths;
+        plt::WfBuffers w;
+        w.rng = reinterpret_cast<uint4*>(b);
+        w.org = reinterpret_cast<float4*>(b + 16 * cap);
+        w.dir = reinterpret_cast<float4*>(b + 32 * cap);
+        w.pdfs = reinterpret_cast<float4*>(b + 48 * cap);
+        w.hit = reinterpret_cast<float4*>(b + 64 * cap);
+        w.len = reinterpret_cast<uint32_t*>(b + 80 * cap);
+        w.queue = reinterpret_cast<uint32_t*>(b + 84 * cap);
+        w.sorted = reinterpret_cast<uint32_t*>(b + 92 * cap);
+        w.keys = reinterpret_cast<uint32_t*>(b + 96 * cap);
+        w.counters = reinterpret_cast<uint32_t*>(b + 100 * cap);
+        rc = plt::launch_sample_pass_wavefront(sc, fr, w, st);
+    } else {
+        rc = plt::launch_sample_pass(sc, fr, st);
+    }
     if (rc < 0) return rc;
     if (ev[1]) (void)hipEventRecord(ev[1], st);
     rc = plt::launch_solve_pass(sc, fr, st);

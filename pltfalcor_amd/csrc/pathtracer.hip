@@ -10,6 +10,7 @@
 // on, a shadow ray blocked by specular transmitters becomes a Newton walk of those vertices onto
 // the specular connection to the light (Manifolds.slangh), and a delta reflector at the current
 // vertex takes part in that walk.
+#include <algorithm>
 #include "manifold.hpp"
 #include "pathtracer.hpp"
 #include "plt_internal.hpp"
@@ -230,79 +231,237 @@ PLT_DEV V3 camera_dir(const PtScene& sc, const PtFrame& fr, uint32_t px, uint32_
 // ---------------------------------------------------------------------------------------
 // Sample pass (pltpt_sample.rt.slang:44-185).
 // ---------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(kPtBlock) sample_kernel(PtScene sc, PtFrame fr) {
+// Camera-path state between vertices (the reference's Path payload, Path.slang:146-196).
+struct PathState {
+    V3 org, dir;
+    float lambda, weight, pdf, thp, lastPdf;
+    uint32_t flags, len;
+};
+
+// Path start (pltpt_sample.rt.slang:146-185): the hero wavelength is the first draw; albedo and
+// normal outputs start at zero; the camera ray.
+PLT_DEV PathState start_path(const PtScene& sc, const PtFrame& fr, uint32_t px, uint32_t py, uint32_t pid, Rng& sg) {
+    PathState ps;
+    ps.lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
+    fr.albedo[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    fr.normal[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.dir = camera_dir(sc, fr, px, py);
+    ps.org = v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]);
+    ps.weight = ps.pdf = ps.thp = 1.f;
+    ps.lastPdf = 0.f;
+    ps.flags = kFlagOnlyDelta;
+    ps.len = 0;
+    return ps;
+}
+
+// One camera-path vertex given the closest hit `h` of the path's current ray
+// (scatterMiss / scatterTriangleMeshClosestHit, pltpt_sample.rt.slang:55-141): Russian roulette,
+// BSDF sampling in the hero wavelength, the bounce record, the next ray. Returns true when the path
+// continues (ps.org / ps.dir then hold the next ray).
+PLT_DEV bool sample_vertex(const PtScene& sc, const PtFrame& fr, const Hit& h, uint32_t li, uint32_t pid, PathState& ps, Rng& sg) {
+    const size_t n = fr.n_paths;
+    const int len = int(ps.len);
+    const V3 wi = -ps.dir;
+    if (h.tri == kNoHit) {  // scatterMiss
+        ps.flags |= kFlagMiss;
+        write_rec(fr.records, n, li, len, kNoHit, 0.f, 0.f, wi, 0u, ps.flags, 1.f / (ps.pdf * ps.thp), ps.lastPdf);
+        return false;
+    }
+    const Surf s = surface_at(sc, h.tri, h.b1, h.b2, wi);
+    const Mat m = material_at(sc, s.material);
+    const uint32_t props = mat_props(m);
+    // Russian roulette: RR(1 - weight, 2, u) (pltpt.slangh:170-176)
+    float rcp_thp = 1.f;
+    if (fr.options & kOptRR) {
+        const float u = sg.next1D();
+        const float pr = powf(1.f - ps.weight, 2.f);
+        rcp_thp = u < pr ? 0.f : 1.f - pr;
+    }
+    bool end = len == int(fr.max_bounces) || (rcp_thp == 0.f && len > 0);
+    BsdfSample bs{v3(0.f, 0.f, 0.f), 0.f, 0.f, 0u};
+    if (!end) {
+        const bool valid = mat_sample(m, s.f, wi, sg, ps.lambda, bs);
+        end = !valid || bs.pdf == 0.f;
+        end |= ps.pdf * bs.pdf * ps.thp * rcp_thp < 1e-6f;
+    }
+    const bool isDelta = (bs.lobe & kLobeDelta) != 0;
+    const bool isDisp = (bs.lobe & kLobeDispersiveDelta) != 0;
+    const bool isRefl = (bs.lobe & kLobeReflection) != 0;
+    const bool isMS = (isRefl && (props & kPropMWReflective)) || (!isRefl && (props & kPropMWTransmissive));
+    if (!isDelta && (ps.flags & kFlagOnlyDelta)) {
+        fr.albedo[pid] = make_float4(m.col[0], m.col[1], m.col[2], 1.f);
+        fr.normal[pid] = make_float4(s.f.N.x, s.f.N.y, s.f.N.z, 1.f);
+    }
+    if (end) {
+        ps.flags |= kFlagTerminated;
+        write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, 0u, ps.flags, 1.f / (ps.pdf * ps.thp), ps.lastPdf);
+        return false;
+    }
+    ps.weight *= bs.weight;
+    ps.thp *= rcp_thp;
+    ps.pdf *= bs.pdf;
+    if (isDisp) ps.flags |= kFlagDispersive;
+    write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, bs.lobe, ps.flags, 1.f / (ps.pdf * ps.thp), ps.lastPdf);
+    ++ps.len;
+    ps.lastPdf = !isDelta ? bs.pdf : ps.lastPdf;
+    const uint32_t msc = isMS ? (ps.flags >> 16) + 1u : 0u;
+    ps.flags = (ps.flags & 0xffffu) | (msc << 16);
+    ps.flags = isDelta ? (ps.flags | kFlagLastDelta) : (ps.flags & ~kFlagLastDelta);
+    if (!isDelta) ps.flags &= ~kFlagOnlyDelta;
+    // ShadingData.computeNewRayOrigin(viewside = isReflection) (ShadingData.slang:77-80)
+    ps.org = ray_origin(s.pos, (s.front == isRefl) ? s.faceN : -s.faceN);
+    ps.dir = bs.wo;
+    return true;
+}
+
+// Megakernel schedule: one lane follows its path to the end.
+__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) sample_kernel(PtScene sc, PtFrame fr) {
     __shared__ int stack[kTraversalStackDepth * kPtBlock];
     const uint32_t li = blockIdx.x * kPtBlock + threadIdx.x;
     if (li >= fr.n_paths) return;
     const uint32_t pid = fr.path_base + li;
     const uint32_t px = pid % fr.width, py = pid / fr.width;
-    const size_t n = fr.n_paths;
     int* stk = stack + threadIdx.x;
-
     Rng sg = Rng::make(px, py, fr.frame);
-    const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
-    fr.albedo[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-    fr.normal[pid] = make_float4(0.f, 0.f, 0.f, 0.f);
-
-    V3 dir = camera_dir(sc, fr, px, py);
-    V3 org = v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]);
-    float weight = 1.f, pdf = 1.f, thp = 1.f, lastNonDeltaPdf = 0.f;
-    uint32_t flags = kFlagOnlyDelta;
-    const bool rr = (fr.options & kOptRR) != 0;
-
-    for (int len = 0; len < int(fr.max_bounces) + 1;) {
-        Ray ray{org.x, org.y, org.z, dir.x, dir.y, dir.z, 0.f, kFltMax};
+    PathState ps = start_path(sc, fr, px, py, pid, sg);
+    while (ps.len < fr.max_bounces + 1) {
+        const Ray ray{ps.org.x, ps.org.y, ps.org.z, ps.dir.x, ps.dir.y, ps.dir.z, 0.f, kFltMax};
         const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, ray, stk, kPtBlock);
-        const V3 wi = -dir;
-        if (h.tri == kNoHit) {  // scatterMiss
-            flags |= kFlagMiss;
-            write_rec(fr.records, n, li, len, kNoHit, 0.f, 0.f, wi, 0u, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
-            break;
+        if (!sample_vertex(sc, fr, h, li, pid, ps, sg)) break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Wavefront schedule of the sample pass: per bounce, a traversal-only kernel over the queue of
+// live paths, a counting sort of the hits by material type (the sort key; misses last), and a
+// shading kernel over the material-sorted queue that runs the same per-vertex code and appends the
+// surviving paths to the next queue. Each path's random stream lives in its state, so every path
+// draws the same numbers in the same order as in the megakernel: the records are identical.
+// Every kernel is launched for the largest possible queue and reads the live count from device
+// memory, so the host never waits between bounces.
+// ---------------------------------------------------------------------------------------
+constexpr int kWfBlock = 256;
+constexpr uint32_t kMissKey = kWfKeys - 1;
+
+PLT_DEV void store_path(const WfBuffers& w, uint32_t li, const PathState& ps, const Rng& sg) {
+    w.rng[li] = make_uint4(sg.s0, sg.s1, sg.s2, sg.s3);
+    w.org[li] = make_float4(ps.org.x, ps.org.y, ps.org.z, ps.lambda);
+    w.dir[li] = make_float4(ps.dir.x, ps.dir.y, ps.dir.z, ps.weight);
+    w.pdfs[li] = make_float4(ps.pdf, ps.thp, ps.lastPdf, __uint_as_float(ps.flags));
+    w.len[li] = ps.len;
+}
+PLT_DEV void load_path(const WfBuffers& w, uint32_t li, PathState& ps, Rng& sg) {
+    const uint4 r = w.rng[li];
+    sg = Rng{r.x, r.y, r.z, r.w};
+    const float4 o = w.org[li], d = w.dir[li], p = w.pdfs[li];
+    ps.org = v3(o.x, o.y, o.z);
+    ps.lambda = o.w;
+    ps.dir = v3(d.x, d.y, d.z);
+    ps.weight = d.w;
+    ps.pdf = p.x;
+    ps.thp = p.y;
+    ps.lastPdf = p.z;
+    ps.flags = __float_as_uint(p.w);
+    ps.len = w.len[li];
+}
+
+__global__ void __launch_bounds__(kWfBlock) wf_start_kernel(PtScene sc, PtFrame fr, WfBuffers w) {
+    const uint32_t li = blockIdx.x * kWfBlock + threadIdx.x;
+    if (li == 0) w.counters[kWfCount0] = fr.n_paths;
+    if (li >= fr.n_paths) return;
+    const uint32_t pid = fr.path_base + li;
+    const uint32_t px = pid % fr.width, py = pid / fr.width;
+    Rng sg = Rng::make(px, py, fr.frame);
+    const PathState ps = start_path(sc, fr, px, py, pid, sg);
+    store_path(w, li, ps, sg);
+    w.queue[li] = li;
+}
+
+__global__ void __launch_bounds__(kPtBlock) wf_trace_kernel(PtScene sc, PtFrame fr, WfBuffers w, int cur) {
+    __shared__ int stack[kTraversalStackDepth * kPtBlock];
+    __shared__ uint32_t hist[kWfKeys];
+    if (threadIdx.x < kWfKeys) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t count = w.counters[kWfCount0 + cur];
+    const uint32_t* q = w.queue + size_t(cur) * fr.n_paths;
+    for (uint32_t i = blockIdx.x * kPtBlock + threadIdx.x; i < count; i += gridDim.x * kPtBlock) {
+        const uint32_t li = q[i];
+        const float4 o = w.org[li], d = w.dir[li];
+        const Ray ray{o.x, o.y, o.z, d.x, d.y, d.z, 0.f, kFltMax};
+        const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, ray, stack + threadIdx.x, kPtBlock);
+        w.hit[li] = make_float4(__uint_as_float(h.tri), h.b1, h.b2, h.t);
+        const uint32_t key = h.tri == kNoHit ? kMissKey : min(sc.materials[sc.mesh_material[sc.tri_mesh[h.tri]]].type, kMissKey - 1);
+        w.keys[i] = key;
+        atomicAdd(&hist[key], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kWfKeys && hist[threadIdx.x]) atomicAdd(&w.counters[kWfHist + threadIdx.x], hist[threadIdx.x]);
+}
+
+// Exclusive prefix sum of the key histogram into the scatter cursors; clears the histogram and the
+// next queue's count.
+__global__ void wf_scan_kernel(WfBuffers w, int cur) {
+    if (threadIdx.x != 0) return;
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < kWfKeys; ++k) {
+        w.counters[kWfCursor + k] = acc;
+        acc += w.counters[kWfHist + k];
+        w.counters[kWfHist + k] = 0u;
+    }
+    w.counters[kWfCount0 + (cur ^ 1)] = 0u;
+}
+
+__global__ void __launch_bounds__(kWfBlock) wf_scatter_kernel(PtFrame fr, WfBuffers w, int cur) {
+    __shared__ uint32_t local[kWfKeys], base[kWfKeys];
+    const uint32_t count = w.counters[kWfCount0 + cur];
+    // block-uniform loop: every thread of the block takes part in each round's barriers
+    for (uint32_t b0 = blockIdx.x * kWfBlock; b0 < count; b0 += gridDim.x * kWfBlock) {
+        if (threadIdx.x < kWfKeys) local[threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t i = b0 + threadIdx.x;
+        uint32_t key = 0, rank = 0;
+        const bool live = i < count;
+        if (live) {
+            key = w.keys[i];
+            rank = atomicAdd(&local[key], 1u);
         }
-        const Surf s = surface_at(sc, h.tri, h.b1, h.b2, wi);
-        const Mat m = material_at(sc, s.material);
-        const uint32_t props = mat_props(m);
-        // Russian roulette: RR(1 - weight, 2, u) (pltpt.slangh:170-176)
-        float rcp_thp = 1.f;
-        if (rr) {
-            const float u = sg.next1D();
-            const float pr = powf(1.f - weight, 2.f);
-            rcp_thp = u < pr ? 0.f : 1.f - pr;
+        __syncthreads();
+        if (threadIdx.x < kWfKeys) base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(&w.counters[kWfCursor + threadIdx.x], local[threadIdx.x]) : 0u;
+        __syncthreads();
+        if (live) w.sorted[base[key] + rank] = w.queue[size_t(cur) * fr.n_paths + i];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) wf_shade_kernel(PtScene sc, PtFrame fr,
+                                                                                                       WfBuffers w, int cur) {
+    const uint32_t count = w.counters[kWfCount0 + cur];
+    const int lane = int(threadIdx.x & 63u);
+    // wave-uniform loop (one wave per block), so the ballot below sees every lane
+    for (uint32_t b0 = blockIdx.x * kPtBlock; b0 < count; b0 += gridDim.x * kPtBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        bool cont = false;
+        uint32_t li = 0;
+        if (i < count) {
+            li = w.sorted[i];
+            PathState ps;
+            Rng sg;
+            load_path(w, li, ps, sg);
+            const float4 hv = w.hit[li];
+            const Hit h{__float_as_uint(hv.x), hv.y, hv.z, hv.w};
+            cont = sample_vertex(sc, fr, h, li, fr.path_base + li, ps, sg) && ps.len < fr.max_bounces + 1;
+            if (cont) store_path(w, li, ps, sg);
         }
-        bool end = len == int(fr.max_bounces) || (rcp_thp == 0.f && len > 0);
-        BsdfSample bs{v3(0.f, 0.f, 0.f), 0.f, 0.f, 0u};
-        if (!end) {
-            const bool valid = mat_sample(m, s.f, wi, sg, lambda, bs);
-            end = !valid || bs.pdf == 0.f;
-            end |= pdf * bs.pdf * thp * rcp_thp < 1e-6f;
+        // append the surviving paths to the next queue, one atomic per wave
+        const uint64_t mask = __ballot(cont);
+        const int leader = mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 0;
+        uint32_t wbase = 0;
+        if (mask && lane == leader) wbase = atomicAdd(&w.counters[kWfCount0 + (cur ^ 1)], uint32_t(__popcll(mask)));
+        wbase = __shfl(wbase, leader);
+        if (cont) {
+            const uint32_t pos = wbase + uint32_t(__popcll(mask & ((uint64_t(1) << lane) - 1u)));
+            w.queue[size_t(cur ^ 1) * fr.n_paths + pos] = li;
         }
-        const bool isDelta = (bs.lobe & kLobeDelta) != 0;
-        const bool isDisp = (bs.lobe & kLobeDispersiveDelta) != 0;
-        const bool isRefl = (bs.lobe & kLobeReflection) != 0;
-        const bool isMS = (isRefl && (props & kPropMWReflective)) || (!isRefl && (props & kPropMWTransmissive));
-        if (!isDelta && (flags & kFlagOnlyDelta)) {
-            fr.albedo[pid] = make_float4(m.col[0], m.col[1], m.col[2], 1.f);
-            fr.normal[pid] = make_float4(s.f.N.x, s.f.N.y, s.f.N.z, 1.f);
-        }
-        if (end) {
-            flags |= kFlagTerminated;
-            write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, 0u, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
-            break;
-        }
-        weight *= bs.weight;
-        thp *= rcp_thp;
-        pdf *= bs.pdf;
-        if (isDisp) flags |= kFlagDispersive;
-        write_rec(fr.records, n, li, len, h.tri, h.b1, h.b2, wi, bs.lobe, flags, 1.f / (pdf * thp), lastNonDeltaPdf);
-        ++len;
-        lastNonDeltaPdf = !isDelta ? bs.pdf : lastNonDeltaPdf;
-        const uint32_t msc = isMS ? (flags >> 16) + 1u : 0u;
-        flags = (flags & 0xffffu) | (msc << 16);
-        flags = isDelta ? (flags | kFlagLastDelta) : (flags & ~kFlagLastDelta);
-        if (!isDelta) flags &= ~kFlagOnlyDelta;
-        // ShadingData.computeNewRayOrigin(viewside = isReflection) (ShadingData.slang:77-80)
-        org = ray_origin(s.pos, (s.front == isRefl) ? s.faceN : -s.faceN);
-        dir = bs.wo;
     }
 }
 
@@ -736,6 +895,27 @@ int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     hipLaunchKernelGGL(sample_kernel, dim3((fr.n_paths + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, sc, fr);
     return check_launch("sample_kernel");
 }
+int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, hipStream_t st) {
+    if (fr.n_paths == 0) return PLT_OK;
+    if (hipMemsetAsync(w.counters, 0, kWfCounters * sizeof(uint32_t), st) != hipSuccess)
+        return set_error(PLT_EHIP, "wavefront sample pass: counter clear failed");
+    const uint32_t n = fr.n_paths;
+    hipLaunchKernelGGL(wf_start_kernel, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, sc, fr, w);
+    // persistent grids sized to the resident capacity (the kernels stride over the live queue)
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t g64 = std::min((n + kPtBlock - 1) / kPtBlock, uint32_t(16 * cus));
+    const uint32_t g256 = std::min((n + kWfBlock - 1) / kWfBlock, uint32_t(8 * cus));
+    for (uint32_t b = 0; b <= fr.max_bounces; ++b) {
+        const int cur = int(b & 1u);
+        hipLaunchKernelGGL(wf_trace_kernel, dim3(g64), dim3(kPtBlock), 0, st, sc, fr, w, cur);
+        hipLaunchKernelGGL(wf_scan_kernel, dim3(1), dim3(64), 0, st, w, cur);
+        hipLaunchKernelGGL(wf_scatter_kernel, dim3(g256), dim3(kWfBlock), 0, st, fr, w, cur);
+        hipLaunchKernelGGL(wf_shade_kernel, dim3(g64), dim3(kPtBlock), 0, st, sc, fr, w, cur);
+    }
+    return check_launch("wavefront sample pass");
+}
+
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
     const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);

@@ -60,7 +60,25 @@ struct PtFrame {
     float4* normal;
 };
 
+// Wavefront sample pass: per-path state (SoA, n_paths entries each), two path queues of
+// n_paths entries, the material-sorted queue and its keys, and counters.
+constexpr uint32_t kWfKeys = 16;  // material types 0..14, misses last
+enum : uint32_t { kWfCount0 = 0, kWfHist = 2, kWfCursor = kWfHist + kWfKeys, kWfCounters = kWfCursor + kWfKeys };
+struct WfBuffers {
+    uint4* rng;
+    float4* org;   // origin, hero wavelength
+    float4* dir;   // direction, path weight
+    float4* pdfs;  // pdf, thp, last non-delta pdf, flags (bits)
+    uint32_t* len;
+    float4* hit;   // triangle (bits), b1, b2, t
+    uint32_t* queue;   // 2 x n_paths
+    uint32_t* sorted;  // n_paths
+    uint32_t* keys;    // n_paths
+    uint32_t* counters;
+};
+
 int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 
 }  // namespace plt

@@ -96,6 +96,13 @@ class PLTPT:
         _lib.check(self.lib.plt_pass_copy_bounce_records(self.handle, out.ctypes.data_as(C.c_void_p), out.nbytes))
         return out
 
+    MEGAKERNEL, WAVEFRONT = 0, 1
+
+    def set_schedule(self, schedule: int):
+        """Sample-pass schedule: PLTPT.MEGAKERNEL (default; one kernel, a lane per path) or
+        PLTPT.WAVEFRONT (material-sorted per-bounce queues); both write identical records."""
+        _lib.check(self.lib.plt_pass_set_schedule(self.handle, int(schedule)))
+
     def set_profiling(self, enable: bool):
         _lib.check(self.lib.plt_pass_set_profiling(self.handle, int(bool(enable))))
 
