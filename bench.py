@@ -16,6 +16,10 @@ algorithmic bytes counted from the bounce records of the timed frames:
 Shadow rays, BVH nodes and triangles are not algorithmic bytes; rocprofv3's FETCH_SIZE /
 WRITE_SIZE give the actual traffic (profiles/).
 
+Each frame also runs the passes that follow PLTPT in the reference's render graph: the colour is
+accumulated in FP64 and albedo and normal in FP32 (AccumulatePass), and the accumulated colour is
+tone-mapped with auto exposure (ToneMapper); `secondary.graph_tail` gives their per-launch times.
+
 `secondary.bsdf_eval_batch` is the 2^20-query BSDF eval + pdf batch (BSDF evals/s), the second
 part of the metric. The CPU baseline is the oracle restatement (C++, OpenMP over the host cores)
 rendering a bounded sample of the same frame.
@@ -251,6 +255,32 @@ def segment_count(pltpt, frames):
     return counts
 
 
+def postfx_timing(acc_double, acc_single, tm, color, albedo, out, ldr, reps: int):
+    """Per-launch time of the passes after PLTPT at 1080p, from HIP events on the stream they run
+    on: FP64 accumulation (16 B in + 32 B sum read + 32 B sum written + 16 B out per pixel), FP32
+    accumulation (16 + 16 + 16 + 16 B) and the auto-exposure tone mapper (the luminance pass reads
+    each source texel about once, 16 B, and the tone-mapping pass reads and writes 16 B each)."""
+    import torch
+    pixels = color.shape[0] * color.shape[1]
+    res = {}
+    for name, fn, nbytes in (("accumulate_double", lambda: acc_double.execute(color, out), 96 * pixels),
+                             ("accumulate_single", lambda: acc_single.execute(albedo, out), 64 * pixels),
+                             ("tonemap_auto_exposure", lambda: tm.execute(out, ldr), 48 * pixels)):
+        for _ in range(5):
+            fn()
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB/s": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    return res
+
+
 def path_cpu_baseline(scene_np, budget_s: float, opts):
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the oracle is the CPU baseline (test infrastructure)
@@ -321,6 +351,7 @@ def main():
 
     from pltfalcor_amd.bsdf import default_context
     from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.postfx import AccumulatePass, AccumulatePrecision, ToneMapper
     from pltfalcor_amd.scene import PATH_2, Scene
 
     ctx = default_context(0)
@@ -336,11 +367,27 @@ def main():
     color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device=dev)
     albedo, normal = torch.empty_like(color), torch.empty_like(color)
     pltpt.execute_frame = lambda: pltpt.execute(WIDTH, HEIGHT, color, albedo, normal)  # noqa: E731
+    # the passes after PLTPT in the reference graph: colour accumulated in FP64, albedo and normal in
+    # FP32, tone mapping with auto exposure (the OptiX denoiser between them is not part of this build)
+    acc_c = AccumulatePass(ctx, {"enabled": True, "precisionMode": AccumulatePrecision.Double})
+    acc_a = AccumulatePass(ctx, {"enabled": True, "precisionMode": AccumulatePrecision.Single})
+    acc_n = AccumulatePass(ctx, {"enabled": True, "precisionMode": AccumulatePrecision.Single})
+    tm = ToneMapper(ctx, {"autoExposure": True, "exposureCompensation": 0.0})
+    out_c, out_a, out_n, ldr = (torch.empty_like(color) for _ in range(4))
+
+    def graph_tail():
+        acc_c.execute(color, out_c)
+        acc_a.execute(albedo, out_a)
+        acc_n.execute(normal, out_n)
+        tm.execute(out_c, ldr)
 
     def step():
         pltpt.set_frame(0)  # one step = frames 0..SPP-1 (SPP samples per pixel)
+        for p in (acc_c, acc_a, acc_n):
+            p.reset()
         for _ in range(SPP):
             pltpt.execute_frame()
+            graph_tail()
 
     for _ in range(args.warmup):
         step()
@@ -369,6 +416,7 @@ def main():
         traffic = traffic_from_profile("pltpt_sample_1080p_d8_pmc.md")
     achieved = kbytes / (kms * 1e-3) / 1e9
 
+    post = postfx_timing(acc_c, acc_a, tm, color, albedo, out_c, ldr, 50)
     bsdf = None if args.no_secondary else bsdf_batch(500, 20)[1]
     sur = None if args.no_secondary else surrogate_run(1 << 22, 6, 1 << 22, 5)
     out = {
@@ -401,14 +449,16 @@ def main():
         "cpu_baseline": None,
         "secondary": {"bsdf_eval_batch": {k: bsdf[k] for k in ("value", "unit", "roofline")},
                       "surrogate": {k: sur[k] for k in ("value", "unit", "exact_value", "surrogate_over_exact",
-                                                        "error_vs_exact", "training", "roofline")}}
-                     if bsdf else None,
+                                                        "error_vs_exact", "training", "roofline")},
+                      "graph_tail": post}
+                     if bsdf else {"graph_tail": post},
     }
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = path_cpu_baseline(scene.numpy(), args.cpu_seconds, pltpt.get_scripting_dictionary())
-    pltpt.close()
+    for p in (acc_c, acc_a, acc_n, tm, pltpt):
+        p.close()
     print(json.dumps(out), flush=True)
 
 

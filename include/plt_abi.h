@@ -412,6 +412,75 @@ int plt_surrogate_eval(plt_surrogate* s, uint32_t n, const plt_material* materia
  * bf16 row-major operands, K a multiple of 32 (for tests of the kernel). */
 int plt_mlp_gemm_nt(uint32_t M, uint32_t N, uint32_t K, const void* A, const void* Bt, float* C, plt_stream stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Temporal accumulation: the AccumulatePass that follows PLTPT in the reference's render graph
+ * (Source/RenderPasses/AccumulatePass/AccumulatePass.{h,cpp}, Accumulate.cs.slang;
+ * scripts/PLTPathTracer.py:19-24 uses Double for PLTPT.color and Single for albedo and normal).
+ * Replaces the plugin class's create / execute / reset / setEnabled / setMaxAccumulatedFrames
+ * (AccumulatePass.cpp:31-47,88-116,144-301,370-382; AccumulatePass.h:66-67). Options are its dictionary keys and
+ * defaults: enabled 1, autoReset 1, precisionMode 1 (Single), maxFrameCount 0 (unlimited),
+ * overflowMode 0 (Stop), outputSize 0 (Default) and outputFormat 0 (RGBA32Float) -- other
+ * output sizes or formats return PLT_ENOTIMPL. Setting precisionMode, maxFrameCount or
+ * overflowMode resets the frame count (as the reference's UI and setMaxAccumulatedFrames do);
+ * setting `enabled` behaves as setEnabled (a change resets).
+ * ------------------------------------------------------------------------------------- */
+typedef struct plt_accumulate plt_accumulate;
+enum { PLT_ACCUM_DOUBLE = 0, PLT_ACCUM_SINGLE = 1, PLT_ACCUM_SINGLE_COMPENSATED = 2 };
+enum { PLT_ACCUM_OVERFLOW_STOP = 0, PLT_ACCUM_OVERFLOW_RESET = 1, PLT_ACCUM_OVERFLOW_EMA = 2 };
+/* Input texel formats: RGBA32Float, RGBA32Uint, RGBA32Sint (the shader's _INPUT_FORMAT,
+ * Accumulate.cs.slang:48-59, AccumulatePass.cpp:245-260). */
+enum { PLT_ACCUM_INPUT_FLOAT = 0, PLT_ACCUM_INPUT_UINT = 1, PLT_ACCUM_INPUT_SINT = 2 };
+
+int plt_accumulate_create(plt_context* ctx, plt_accumulate** out);
+int plt_accumulate_destroy(plt_accumulate* pass);
+int plt_accumulate_set_option(plt_accumulate* pass, const char* key, double value);
+int plt_accumulate_get_option(const plt_accumulate* pass, const char* key, double* value);
+const char* plt_accumulate_option_name(uint32_t i);
+int plt_accumulate_set_enabled(plt_accumulate* pass, int enabled);
+int plt_accumulate_reset(plt_accumulate* pass);
+/* Frames accumulated so far (mFrameCount). */
+int plt_accumulate_get_frame_count(const plt_accumulate* pass, uint32_t* count);
+/* One execute on `stream`: src is width*height texels of src_format (16 B each, 16-byte aligned),
+ * dst width*height float4 (the running average). refresh_flags != 0 reports what the reference
+ * reads from the render data's refresh flags and scene/camera updates (a reset when autoReset is
+ * on; AccumulatePass.cpp:146-174). With maxFrameCount reached and overflowMode Stop, dst is not
+ * written. A change of width or height resets. */
+int plt_accumulate_execute(plt_accumulate* pass, const void* src, int src_format, uint32_t width, uint32_t height,
+                           float* dst, uint32_t refresh_flags, plt_stream stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Tone mapping: the ToneMapper at the end of the reference's render graph
+ * (Source/RenderPasses/ToneMapper/ToneMapper.{h,cpp}, Luminance.ps.slang, ToneMapping.ps.slang;
+ * scripts/PLTPathTracer.py:26-27 sets autoExposure 1, exposureCompensation 0). Options are the
+ * keys of ToneMapper::getScriptingDictionary and its property exposureValue
+ * (ToneMapper.cpp:57-76,99-127,181-200), applied through the reference's setters with their clamps and
+ * coupled updates (ToneMapper.cpp:406-515); outputSize other than 0 (Default) or outputFormat
+ * other than 0 (RGBA32Float) return PLT_ENOTIMPL.
+ * ------------------------------------------------------------------------------------- */
+typedef struct plt_tonemapper plt_tonemapper;
+/* ToneMapperOperator (ToneMapperParams.slang:33-41). */
+enum {
+    PLT_TONEMAP_LINEAR = 0, PLT_TONEMAP_REINHARD = 1, PLT_TONEMAP_REINHARD_MODIFIED = 2, PLT_TONEMAP_HEJI_HABLE_ALU = 3,
+    PLT_TONEMAP_HABLE_UC2 = 4, PLT_TONEMAP_ACES = 5
+};
+
+int plt_tonemapper_create(plt_context* ctx, plt_tonemapper** out);
+int plt_tonemapper_destroy(plt_tonemapper* pass);
+int plt_tonemapper_set_option(plt_tonemapper* pass, const char* key, double value);
+int plt_tonemapper_get_option(const plt_tonemapper* pass, const char* key, double* value);
+const char* plt_tonemapper_option_name(uint32_t i);
+/* setScene with scene metadata (ToneMapper.cpp:395-405): each value that is not NaN is applied
+ * through its setter when useSceneMetadata is on. */
+int plt_tonemapper_set_scene_metadata(plt_tonemapper* pass, float film_iso, float f_number, float shutter_speed);
+/* The 3x3 colour transform (row-major) the next execute applies: white balance x
+ * 2^exposureCompensation x manual exposure (ToneMapper.cpp:527-547). */
+int plt_tonemapper_get_color_transform(const plt_tonemapper* pass, float* out9);
+/* Tone-maps src (width*height float4, 16-byte aligned) into dst (same size) on `stream`. With
+ * autoExposure on, mean_log_luminance (a device float, may be NULL) receives the mean log2
+ * luminance the exposure used. */
+int plt_tonemapper_execute(plt_tonemapper* pass, const float* src, uint32_t width, uint32_t height, float* dst,
+                           float* mean_log_luminance, plt_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

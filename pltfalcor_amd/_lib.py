@@ -130,6 +130,25 @@ SIGNATURES = [
     ("plt_pass_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("plt_pass_get_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_uint32)]),
+    ("plt_accumulate_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("plt_accumulate_destroy", C.c_int, [C.c_void_p]),
+    ("plt_accumulate_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("plt_accumulate_get_option", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("plt_accumulate_option_name", C.c_char_p, [C.c_uint32]),
+    ("plt_accumulate_set_enabled", C.c_int, [C.c_void_p, C.c_int]),
+    ("plt_accumulate_reset", C.c_int, [C.c_void_p]),
+    ("plt_accumulate_get_frame_count", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    ("plt_accumulate_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_uint32, C.c_void_p]),
+    ("plt_tonemapper_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("plt_tonemapper_destroy", C.c_int, [C.c_void_p]),
+    ("plt_tonemapper_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("plt_tonemapper_get_option", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
+    ("plt_tonemapper_option_name", C.c_char_p, [C.c_uint32]),
+    ("plt_tonemapper_set_scene_metadata", C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    ("plt_tonemapper_get_color_transform", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("plt_tonemapper_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
 ]
 
 _lib = None

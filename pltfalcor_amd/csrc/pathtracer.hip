@@ -820,19 +820,8 @@ solve_kernel(PtScene sc, PtFrame fr) {
                         }
                     }
                     if (ok) {
-                        // evalNEE (pltpt_solve.rt.slang:219-243)
-                        if (evalCurrent) {
-                            const V3 V = -beam.r;
-                            const Surf fs = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, V);
-                            beam.propagate(fs.pos);
-                            mat_eval<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam);
-                            if (fr.options & kOptHwssMis) {
-                                float pp[N];
-                                mat_lambda_pdf<N>(m, fs.f, V, bounce.wi, kLobeNonDelta, beam, pp);
-#pragma unroll
-                                for (int j = 0; j < N; ++j) beam.spdf[j] *= pp[j];
-                            }
-                        }
+                        // evalNEE (pltpt_solve.rt.slang:219-243): the current vertex's non-delta lobes
+                        if (evalCurrent) unwind_step<N>(sc, fr, bounce.tri, bounce.b1, bounce.b2, bounce.wi, kLobeNonDelta, beam);
                         forward_unwind<N>(sc, fr, n, li, p, beam);
                         measure<N>(sc, fr, sv, p, bounce.contrib, beam);
                     }
