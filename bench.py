@@ -362,6 +362,7 @@ def main():
     # 60 Newton iterations, threshold 5e-5); only the depth and the HWSS count are set here
     opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4}
     pltpt = PLTPT(ctx, opts)
+    pltpt.set_sample_pattern(PLTPT.HALTON, 16)  # the graph's GBufferRT jitter (samplePattern Halton, 16)
     pltpt.set_scene(scene)
     dev = torch.device("cuda")
     color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device=dev)
@@ -438,7 +439,8 @@ def main():
                                "over dielectric / Drude conductor / coated opaque dielectric / grated conductor / "
                                "birefringent thin dielectric / 20-layer stack (FP64 steps) / coated conductor / "
                                "opaque dielectric; HWSS 4 + spectral MIS, NEE + MIS, RR; manifold NEE on (<= 2 "
-                               "specular occluders, <= 60 Newton iterations, threshold 5e-5)",
+                               "specular occluders, <= 60 Newton iterations, threshold 5e-5); Halton-16 camera "
+                               "jitter; FP64/FP32 accumulation and auto-exposure tone mapping per frame",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
                    "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
                    "segments_per_frame": seg_per_exec},

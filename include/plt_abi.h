@@ -69,6 +69,21 @@ typedef struct plt_spectral_profile {
  * compute_icdf != 0 normalises pdf and fills icdf (emission spectra); 0 keeps raw values (n, k). */
 int plt_spectral_profile_build(uint32_t npts, const float* wavelengths, const float* values, int compute_icdf,
                                plt_spectral_profile* out);
+/* Spectral tables from files (host only). A `.spd` file holds "wavelength value" lines; blank and
+ * '#' lines are skipped and the first line that does not parse ends the table
+ * (PiecewiseLinearSpectrum::fromFile, Utils/Color/Spectrum.cpp:89-112). plt_spectrum_load_spd
+ * writes up to `capacity` entries and the table's size to *count (capacity 0 with NULL arrays
+ * queries the size). */
+int plt_spectrum_load_spd(const char* path, uint32_t capacity, float* wavelengths, float* values, uint32_t* count);
+/* The profile of a constant spectrum (SampledSpectrum<float>(value): one bin over 380-720 nm). */
+int plt_spectral_profile_constant(float value, plt_spectral_profile* out);
+/* SceneBuilder::addSpectralProfileFromMaterial (SceneBuilder.cpp:972-983): the n and k profiles of
+ * <tables_dir>/ior/<name>.eta.spd and .k.spd (no inverse CDF); name "none" gives n = 1, k = 0. */
+int plt_spectral_profiles_from_material(const char* tables_dir, const char* name, plt_spectral_profile* n_out,
+                                        plt_spectral_profile* k_out);
+/* SceneBuilder::addSpectralProfileForEmitterType (SceneBuilder.cpp:984-1000): the emission profile
+ * (with inverse CDF) of <tables_dir>/emission/<name>.spd scaled by `scale`; PLT_EINVAL if absent. */
+int plt_spectral_profile_from_emitter(const char* tables_dir, const char* name, float scale, plt_spectral_profile* out);
 /* Uploads `count` host profiles to the device; profile ids are their indices. */
 int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profile* profiles, uint32_t count);
 
@@ -201,7 +216,10 @@ typedef struct plt_camera {
     float fov_y;          /* radians */
     float aspect;         /* width / height */
     float focal_distance;
-    float jitter[2];      /* +-0.5 px sample offset (data.jitterX/Y); 0 = pixel centre */
+    float jitter[2];      /* sample offset as a fraction of the image width / height (the reference's
+                             data.jitterX/Y: a pixel offset in [-0.5, 0.5) divided by the frame size);
+                             0 = pixel centre. Replaced per frame by a jitter pattern
+                             (plt_pass_set_sample_pattern). */
 } plt_camera;
 
 /* ---------------------------------------------------------------------------------------
@@ -300,6 +318,21 @@ const char* plt_pass_option_name(uint32_t i);
 int plt_pass_set_scene(plt_pass* pass, plt_scene* scene);
 /* Frame counter: seeds UniformSampleGenerator(pixel, frame) of the next execute. */
 int plt_pass_set_frame(plt_pass* pass, uint32_t frame);
+
+/* Camera jitter pattern of the G-buffer pass that runs before PLTPT in the reference graph
+ * (scripts/PLTPathTracer.py:16: GBufferRT samplePattern Halton, sampleCount 16;
+ * GBufferBase.h:39-45, GBufferBase.cpp:202-234, Camera.cpp:63-70): each execute draws the next
+ * offset in [-0.5, 0.5)^2 pixels and uses offset / (width, height) as the camera jitter.
+ * Center (the default) keeps the camera's own jitter. Setting a pattern, or a scene, restarts it. */
+enum { PLT_SAMPLE_PATTERN_CENTER = 0, PLT_SAMPLE_PATTERN_DIRECTX = 1, PLT_SAMPLE_PATTERN_HALTON = 2,
+       PLT_SAMPLE_PATTERN_STRATIFIED = 3 };
+int plt_pass_set_sample_pattern(plt_pass* pass, int pattern, uint32_t sample_count);
+/* The camera jitter (fractions of width and height) the last execute used. */
+int plt_pass_get_jitter(const plt_pass* pass, float* xy);
+/* The first n offsets (pixels, xy pairs) of a fresh generator of the pattern (host memory), as
+ * CPUSampleGenerator::next returns them (HaltonSamplePattern.cpp:56-70, DxSamplePattern.h,
+ * StratifiedSamplePattern.cpp:67-88). */
+int plt_sample_pattern_generate(int pattern, uint32_t sample_count, uint32_t n, float* xy);
 
 /* I/O of one execute (PLTPT.cpp:65-78): caller-owned device buffers of width*height float4
  * (RGBA32F): color (spectral radiance -> linear Rec.709), albedo and shading normal of the first

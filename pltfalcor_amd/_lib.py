@@ -130,6 +130,15 @@ SIGNATURES = [
     ("plt_pass_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("plt_pass_get_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_uint32)]),
+    ("plt_spectrum_load_spd", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                        C.POINTER(C.c_uint32)]),
+    ("plt_spectral_profile_constant", C.c_int, [C.c_float, C.POINTER(SpectralProfile)]),
+    ("plt_spectral_profiles_from_material", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(SpectralProfile),
+                                                      C.POINTER(SpectralProfile)]),
+    ("plt_spectral_profile_from_emitter", C.c_int, [C.c_char_p, C.c_char_p, C.c_float, C.POINTER(SpectralProfile)]),
+    ("plt_pass_set_sample_pattern", C.c_int, [C.c_void_p, C.c_int, C.c_uint32]),
+    ("plt_pass_get_jitter", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("plt_sample_pattern_generate", C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     ("plt_accumulate_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("plt_accumulate_destroy", C.c_int, [C.c_void_p]),
     ("plt_accumulate_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),

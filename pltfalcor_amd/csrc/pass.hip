@@ -15,6 +15,7 @@
 #include <vector>
 #include "pathtracer.hpp"
 #include "plt_internal.hpp"
+#include "sample_pattern.hpp"
 #include "scene_device.hpp"
 #include "scene_host.hpp"
 
@@ -154,6 +155,9 @@ struct plt_pass {
     int schedule = PLT_SCHEDULE_MEGAKERNEL;
     void* d_wf = nullptr;
     size_t wf_paths = 0;
+    // camera jitter pattern of the G-buffer pass (Center: the camera's own jitter)
+    plt::SamplePattern pattern;
+    float last_jitter[2] = {0.f, 0.f};
 };
 
 namespace plt {
@@ -248,6 +252,7 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     plt::free_scene_state(p);
     p->scene = scene;
     p->frame = 0;  // PLTPT.cpp:510
+    p->pattern.configure(p->pattern.type, p->pattern.count);  // a new generator (GBufferBase.cpp:185-189)
     if (!scene) return PLT_OK;
     if (!scene->ctx || !p->ctx) return set_error(PLT_ESTATE, "plt_pass_set_scene: host-only scene or pass");
     const plt::LightTables lt = plt::build_lights(scene->host);
@@ -259,6 +264,18 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     if ((rc = plt::upload_vec(lt.tri_light, &p->d_tri_light, p->owned)) < 0) return rc;
     p->n_lights = uint32_t(lt.pdf.size());
     plt::camera_frame(scene->host.camera, p->cam);
+    return PLT_OK;
+}
+
+int plt_pass_set_sample_pattern(plt_pass* p, int pattern, uint32_t sample_count) {
+    if (!p) return set_error(PLT_EINVAL, "plt_pass_set_sample_pattern: NULL pass");
+    return p->pattern.configure(pattern, sample_count);
+}
+
+int plt_pass_get_jitter(const plt_pass* p, float* xy) {
+    if (!p || !xy) return set_error(PLT_EINVAL, "plt_pass_get_jitter: bad arguments");
+    xy[0] = p->last_jitter[0];
+    xy[1] = p->last_jitter[1];
     return PLT_OK;
 }
 
@@ -319,6 +336,16 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     }
     sc.jitter[0] = s->host.camera.jitter[0];
     sc.jitter[1] = s->host.camera.jitter[1];
+    if (p->pattern.type != PLT_SAMPLE_PATTERN_CENTER) {
+        // Camera::beginFrame: jitter = next() * (1 / frame size) (Camera.cpp:63-70, GBufferBase.cpp:221-229)
+        float j[2];
+        p->pattern.next(j);
+        const float inv_w = 1.f / float(rd->width), inv_h = 1.f / float(rd->height);
+        sc.jitter[0] = j[0] * inv_w;
+        sc.jitter[1] = j[1] * inv_h;
+    }
+    p->last_jitter[0] = sc.jitter[0];
+    p->last_jitter[1] = sc.jitter[1];
     sc.lights = p->d_lights;
     sc.alias = p->d_alias;
     sc.light_pdf = p->d_light_pdf;

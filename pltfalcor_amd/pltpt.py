@@ -97,6 +97,18 @@ class PLTPT:
         return out
 
     MEGAKERNEL, WAVEFRONT = 0, 1
+    CENTER, DIRECTX, HALTON, STRATIFIED = 0, 1, 2, 3
+
+    def set_sample_pattern(self, pattern: int, sample_count: int = 16):
+        """Camera jitter of the G-buffer pass before PLTPT (GBufferRT's samplePattern / sampleCount;
+        the reference graph uses Halton, 16): each execute jitters the camera by the next offset."""
+        _lib.check(self.lib.plt_pass_set_sample_pattern(self.handle, int(pattern), int(sample_count)))
+
+    def jitter(self):
+        """(jitterX, jitterY) of the last execute, as fractions of the frame size."""
+        xy = (C.c_float * 2)()
+        _lib.check(self.lib.plt_pass_get_jitter(self.handle, xy))
+        return xy[0], xy[1]
 
     def set_schedule(self, schedule: int):
         """Sample-pass schedule: PLTPT.MEGAKERNEL (default; one kernel, a lane per path) or

@@ -626,3 +626,100 @@ def test_oracle_walk_derivatives_match_finite_differences():
                 dv = (f(b1, b2 + h)[0] - f(b1, b2 - h)[0]) / (2 * h)
                 assert np.abs(du - r[1]).max() < 1e-6 * max(1.0, np.abs(r[1]).max()), (mat.type, src, b1, b2)
                 assert np.abs(dv - r[2]).max() < 1e-6 * max(1.0, np.abs(r[2]).max()), (mat.type, src, b1, b2)
+
+
+# ------------------------------------------------------------------------------ camera jitter patterns
+
+
+def _radical_inverse(i, base):
+    """Van der Corput radical inverse as an exact fraction."""
+    from fractions import Fraction
+    r, f = Fraction(0), Fraction(1)
+    while i:
+        f /= base
+        r += f * (i % base)
+        i //= base
+    return r
+
+
+def _pattern(kind, count, n):
+    from pltfalcor_amd import _lib
+    xy = (C.c_float * (2 * n))()
+    _lib.check(_lib.load().plt_sample_pattern_generate(kind, count, n, xy))
+    return np.array(list(xy), np.float64).reshape(n, 2)
+
+
+def test_halton_jitter_pattern_is_the_shifted_radical_inverse():
+    """HaltonSamplePattern::next: (h2(i), h3(i)) shifted by fract(h + 0.5) - 0.5, cyclic with the
+    sample count; every value within float rounding of the exact radical inverse."""
+    from fractions import Fraction
+    got = _pattern(2, 16, 40)
+    for k in range(40):
+        i = k % 16
+        for d, base in enumerate((2, 3)):
+            h = _radical_inverse(i, base) + Fraction(1, 2)
+            want = float(h - (h.numerator // h.denominator) - Fraction(1, 2))
+            assert abs(got[k, d] - want) <= 2e-7, (k, d, got[k, d], want)
+    assert np.array_equal(got[0], [0, 0])
+    assert np.allclose(got[1], [-0.5, 1 / 3], atol=1e-6)  # fract(1/2 + 1/2) - 1/2, fract(1/3 + 1/2) - 1/2
+    # count 0 never repeats
+    assert not np.array_equal(_pattern(2, 0, 40)[16], got[0])
+
+
+def test_directx_and_stratified_jitter_patterns():
+    dx = _pattern(1, 5, 16)  # DxSamplePattern uses 8 whatever the count
+    table = np.array([[1, -3], [-1, 3], [5, 1], [-3, -5], [-5, 5], [-7, -1], [3, 7], [7, -7]]) / 16
+    assert np.array_equal(dx, np.vstack([table, table]))
+    for count, (bx, by) in ((16, (4, 4)), (12, (3, 4)), (7, (7, 1))):  # bx from floor(sqrt(count)) up
+        st = _pattern(3, count, 3 * count)
+        assert np.all((st >= -0.5) & (st < 0.5))
+        for r in range(3):  # each round visits every bin exactly once
+            b = st[r * count:(r + 1) * count] + 0.5
+            bins = np.floor(b[:, 0] * bx).astype(int) + bx * np.floor(b[:, 1] * by).astype(int)
+            assert sorted(bins) == list(range(count)), (count, r)
+    from pltfalcor_amd import _lib
+    with pytest.raises(_lib.PLTError):
+        _pattern(4, 16, 1)
+
+
+@pytest.mark.gpu
+def test_gpu_halton_jittered_frames_match_oracle(gpu):
+    """The reference graph's jitter (Halton, 16 samples): frame k uses offset k scaled by the frame
+    size; primary hits bit-identical to the oracle rendered with that jitter, colour to the path
+    tracer's parity bar; set_scene restarts the pattern."""
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import Scene
+    from pltfalcor_amd.bsdf import default_context
+    ctx = default_context()
+    s = Scene(ctx, seed=4, sphere_count=8, subdivision=3)
+    a = s.numpy()
+    w, h = 48, 40
+    p = PLTPT(ctx, dict(BASE_OPTS))
+    p.set_sample_pattern(PLTPT.HALTON, 16)
+    p.set_scene(s)
+    pat = _pattern(2, 16, 4).astype(np.float32)
+    color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    jit = []
+    for f in range(4):
+        p.execute(w, h, color, albedo, normal)
+        torch.cuda.synchronize()
+        j = p.jitter()
+        jit.append(j)
+        assert j == (float(pat[f, 0] * np.float32(1 / w)), float(pat[f, 1] * np.float32(1 / h))), (f, j)
+        cam = np.array(a["camera"], np.float32).copy()
+        cam[12], cam[13] = j
+        g_rec = p.bounce_records(w, h)
+        g_col = color.cpu().numpy().reshape(-1, 4)
+        o_col, o_rec = O.render(a, w, h, frame=f, max_bounces=6, records=True, camera=cam)
+        assert np.array_equal(g_rec[0, :3], o_rec[0, :3]), f
+        topo = _topology_equal(g_rec, o_rec)
+        assert topo.mean() >= 0.99
+        close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3])), axis=1)
+        assert close[topo].mean() >= 0.995
+    assert len(set(jit)) == 4
+    p.set_scene(s)
+    p.execute(w, h, color, albedo, normal)
+    assert p.jitter() == jit[0] == (0.0, 0.0)
+    p.close()
