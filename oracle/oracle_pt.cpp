@@ -42,6 +42,9 @@ struct OScene {
     std::vector<V3<F>> lightN;
     std::vector<uint32_t> triToLight;
     std::vector<uint32_t> aliasX, aliasY;
+    // EmissiveUniformSampler: lights with flux > 0 and their selection pdf 1 / count
+    std::vector<uint32_t> activeLights;
+    std::vector<float> lightPdfUniform;
 };
 
 struct Options {
@@ -50,6 +53,7 @@ struct Options {
     float sourcingArea, maxOmega;
     int doMNEE = 0, mneeMaxOccluders = 2, mneeMaxIterations = 60;
     float mneeThreshold = 5e-5f;
+    int uniformEmissive = 0;  // emissiveSampler Uniform (else Power)
 };
 constexpr int kMaxMSOccluders = 8;
 
@@ -407,7 +411,9 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
     sv.wavelength = Spectrum<float>::sampleOne(sampleNext1D<float>(sg));
     sv.pathOrigin = s.camPos;
     sv.pathNormal = normalize(s.camW);
-    const bool useEmissive = o.emissiveLights && !s.lightTri.empty();
+    // Scene::useEmissiveLights needs an active (flux > 0) emissive triangle
+    const bool useEmissive = o.emissiveLights && !s.activeLights.empty();
+    const std::vector<float>& selPdfs = o.uniformEmissive ? s.lightPdfUniform : s.lightPdf;
     const bool doNEE = o.doNEE != 0, doMIS = doNEE && o.doMIS;
     for (int p = 0;; ++p) {
         const Bounce& bounce = rec[size_t(p)];
@@ -432,15 +438,22 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                 float Le[HW], lam[HW];
                 for (int j = 0; j < HW; ++j) { Le[j] = 0; lam[j] = sv.wavelength; }
                 if (u < pType) {
-                    // EmissivePowerSampler.sampleLight
-                    const uint32_t nl = uint32_t(s.lightTri.size());
                     const float uL = sampleNext1D<float>(sg);
-                    uint32_t ti = std::min(uint32_t(uL * float(nl)), nl - 1);
-                    const uint32_t ax = s.aliasX[ti], ay = s.aliasY[ti];
-                    const float threshold = f16_bits_to_f32(uint16_t(ax >> 16u));
-                    const uint32_t above = ((ax & 0xFFFFu) << 8u) | ((ay >> 24u) & 0xFFu), below = ay & 0xFFFFFFu;
-                    ti = sampleNext1D<float>(sg) >= threshold ? above : below;
-                    const float selPdf = s.lightPdf[ti];
+                    uint32_t ti;
+                    if (o.uniformEmissive) {
+                        // EmissiveUniformSampler.sampleLight (EmissiveUniformSampler.slang:52-72)
+                        const uint32_t na = uint32_t(s.activeLights.size());
+                        ti = s.activeLights[std::min(uint32_t(uL * float(na)), na - 1)];
+                    } else {
+                        // EmissivePowerSampler.sampleLight
+                        const uint32_t nl = uint32_t(s.lightTri.size());
+                        ti = std::min(uint32_t(uL * float(nl)), nl - 1);
+                        const uint32_t ax = s.aliasX[ti], ay = s.aliasY[ti];
+                        const float threshold = f16_bits_to_f32(uint16_t(ax >> 16u));
+                        const uint32_t above = ((ax & 0xFFFFu) << 8u) | ((ay >> 24u) & 0xFFu), below = ay & 0xFFFFFFu;
+                        ti = sampleNext1D<float>(sg) >= threshold ? above : below;
+                    }
+                    const float selPdf = selPdfs[ti];
                     // sampleTriangle
                     const V2<F> u2 = sampleNext2D<float>(sg);
                     const float su = std::sqrt(u2.x);
@@ -553,7 +566,7 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                 if (d2 > kFLT_MIN) {
                     const V3<F> L = toL / std::sqrt(d2);
                     const float ct = dot(hn, -L);
-                    if (ct > 0.f) lightPdf = s.lightPdf[lt] * (d2 / std::fmax(kFLT_MIN, ct * s.lightArea[lt]));
+                    if (ct > 0.f) lightPdf = selPdfs[lt] * (d2 / std::fmax(kFLT_MIN, ct * s.lightArea[lt]));
                 }
             }
             const float mis = needMIS ? bounce.lastNonDeltaPdf / (bounce.lastNonDeltaPdf + lightPdf) : 1.f;
@@ -598,6 +611,11 @@ void buildLights(OScene& s) {
     }
     const uint32_t N = uint32_t(flux.size());
     if (!N) return;
+    // LightCollection::updateActiveTriangleList (LightCollection.cpp:497-523): flux > 0
+    for (uint32_t i = 0; i < N; ++i)
+        if (flux[i] > 0.f) s.activeLights.push_back(i);
+    const float uni = s.activeLights.empty() ? 0.f : 1.f / float(s.activeLights.size());
+    for (uint32_t i = 0; i < N; ++i) s.lightPdfUniform.push_back(flux[i] > 0.f ? uni : 0.f);
     // EmissivePowerSampler::generateAliasTable (EmissivePowerSampler.cpp:86-176)
     double sum = 0;
     for (float f : flux) sum += f;
@@ -733,6 +751,7 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
     o.mneeMaxOccluders = std::min(std::max(iopt[12], 0), kMaxMSOccluders);
     o.mneeMaxIterations = iopt[13];
     o.mneeThreshold = fopt[2];
+    o.uniformEmissive = iopt[14];
     if (threads > 0) omp_set_num_threads(threads);
     const size_t n = count;
 #pragma omp parallel for schedule(dynamic, 16)

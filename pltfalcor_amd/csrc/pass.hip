@@ -29,12 +29,13 @@ struct OptionSpec {
     double def;
 };
 // Keys and defaults of PLTPT (PLTPT.h:74-108). sampleGenerator 0 = uniform (xoshiro128**),
-// emissiveSampler 1 = power (EmissiveLightSamplerType::Power).
+// emissiveSampler 2 = power (EmissiveLightSamplerType: Uniform 0, LightBVH 1, Power 2;
+// EmissiveLightSamplerType.slangh:36-42).
 const OptionSpec kOptions[] = {
     {"maxBounces", 32}, {"tileSize", 512}, {"HWSS", 4}, {"HWSSDoMIS", 1}, {"sampleGenerator", 0},
     {"debugView", 0}, {"debugViewIntensity", 1}, {"sourcingAreaFromEmissiveGeometry", 1},
     {"sourcingMaxBeamOmega", .025}, {"useDirectLights", 1}, {"useEnvLights", 1}, {"useEmissiveLights", 1},
-    {"useAnalyticLights", 1}, {"alphaMasking", 1}, {"doNEE", 1}, {"doMIS", 1}, {"emissiveSampler", 1},
+    {"useAnalyticLights", 1}, {"alphaMasking", 1}, {"doNEE", 1}, {"doMIS", 1}, {"emissiveSampler", 2},
     {"doRussianRoulette", 1}, {"NEEUsePerTileSG", 0}, {"doImportanceSampleEmitters", 1}, {"doMNEE", 1},
     {"MNEEMaxOccluders", 2}, {"MNEEMaxIterations", 60}, {"MNEESolverThreshold", 5e-5},
 };
@@ -42,8 +43,10 @@ const OptionSpec kOptions[] = {
 struct LightTables {
     std::vector<float4> lights;
     std::vector<uint2> alias;
-    std::vector<float> pdf;
+    std::vector<float> pdf;          // power sampler selection pdf
     std::vector<uint32_t> tri_light;
+    std::vector<float> pdf_uniform;  // uniform sampler selection pdf: 1 / active count, 0 if inactive
+    std::vector<uint32_t> active;    // lights with non-zero flux (LightCollection::updateActiveTriangleList)
 };
 
 // Emissive triangles in mesh order (LightCollection), flux = pi * area (the single emissive
@@ -78,6 +81,11 @@ LightTables build_lights(const HostScene& s) {
     }
     const uint32_t N = uint32_t(flux.size());
     if (N == 0) return t;
+    // active triangles: flux > 0 (LightCollection.cpp:497-523)
+    for (uint32_t i = 0; i < N; ++i)
+        if (flux[i] > 0.f) t.active.push_back(i);
+    const float uni = t.active.empty() ? 0.f : 1.f / float(t.active.size());
+    for (uint32_t i = 0; i < N; ++i) t.pdf_uniform.push_back(flux[i] > 0.f ? uni : 0.f);
     double sum = 0.0;
     for (float f : flux) sum += f;
     const float invSum = 1.f / float(sum);
@@ -143,6 +151,9 @@ struct plt_pass {
     float* d_light_pdf = nullptr;
     uint32_t* d_tri_light = nullptr;
     uint32_t n_lights = 0;
+    float* d_light_pdf_uniform = nullptr;
+    uint32_t* d_active = nullptr;
+    uint32_t n_active = 0;
     uint32_t* d_records = nullptr;
     size_t records_words = 0;
     size_t last_words = 0;  // record words written by the last execute
@@ -171,6 +182,9 @@ void free_scene_state(plt_pass* p) {
     p->d_light_pdf = nullptr;
     p->d_tri_light = nullptr;
     p->n_lights = 0;
+    p->d_light_pdf_uniform = nullptr;
+    p->d_active = nullptr;
+    p->n_active = 0;
 }
 // Camera frame (Camera.cpp:187-193).
 void camera_frame(const plt_camera& c, float out[12]) {
@@ -262,7 +276,11 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     if ((rc = plt::upload_vec(lt.alias, &p->d_alias, p->owned)) < 0) return rc;
     if ((rc = plt::upload_vec(lt.pdf, &p->d_light_pdf, p->owned)) < 0) return rc;
     if ((rc = plt::upload_vec(lt.tri_light, &p->d_tri_light, p->owned)) < 0) return rc;
-    p->n_lights = uint32_t(lt.pdf.size());
+    if ((rc = plt::upload_vec(lt.pdf_uniform, &p->d_light_pdf_uniform, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(lt.active, &p->d_active, p->owned)) < 0) return rc;
+    p->n_active = uint32_t(lt.active.size());
+    // Scene::useEmissiveLights requires an active triangle; otherwise emissive lighting is off
+    p->n_lights = p->n_active > 0 ? uint32_t(lt.pdf.size()) : 0;
     plt::camera_frame(scene->host.camera, p->cam);
     return PLT_OK;
 }
@@ -298,7 +316,11 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
                                      "MNEESolverThreshold finite");
     if (mneeOcc > plt::kMaxMsOccluders) return set_error(PLT_ENOTIMPL, "MNEEMaxOccluders above 8 is not built");
     if (o("NEEUsePerTileSG") != 0) return set_error(PLT_ENOTIMPL, "NEEUsePerTileSG=1 is not built");
-    if (o("emissiveSampler") != 1) return set_error(PLT_ENOTIMPL, "only the power emissive sampler (1) is built");
+    // EmissiveLightSamplerType: Uniform 0, Power 2; the pass creates no other sampler (PLTPT.cpp:399-412)
+    const double emissiveSampler = o("emissiveSampler");
+    if (emissiveSampler != 0 && emissiveSampler != 2)
+        return set_error(PLT_ENOTIMPL, "emissiveSampler must be 0 (Uniform) or 2 (Power); the reference's pass "
+                                       "creates no other sampler");
     if (o("sampleGenerator") != 0) return set_error(PLT_ENOTIMPL, "only the uniform sample generator (0) is built");
     if (o("debugView") != 0) return set_error(PLT_ENOTIMPL, "debug views are not built");
     const uint32_t hwss = uint32_t(o("HWSS"));
@@ -348,7 +370,9 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     p->last_jitter[1] = sc.jitter[1];
     sc.lights = p->d_lights;
     sc.alias = p->d_alias;
-    sc.light_pdf = p->d_light_pdf;
+    sc.light_pdf = emissiveSampler == 0 ? p->d_light_pdf_uniform : p->d_light_pdf;
+    sc.active_lights = p->d_active;
+    sc.n_active = p->n_active;
     sc.tri_light = p->d_tri_light;
     sc.n_lights = p->n_lights;
 
@@ -367,6 +391,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (o("useDirectLights") != 0) opts |= plt::kOptDirectLights;
     if (o("useEmissiveLights") != 0) opts |= plt::kOptEmissiveLights;
     if (o("doMNEE") != 0) opts |= plt::kOptMNEE;
+    if (emissiveSampler == 0) opts |= plt::kOptUniformEmissive;
     fr.options = opts;
     fr.mnee_max_occluders = uint32_t(mneeOcc);
     fr.mnee_max_iterations = uint32_t(mneeIt);

@@ -715,14 +715,22 @@ solve_kernel(PtScene sc, PtFrame fr) {
                     // Uncontracted, division-normalised geometry: the cosine test and the shadow ray
                     // round exactly as a strict-IEEE evaluation of the same expressions.
 #pragma clang fp contract(off)
-                    // EmissivePowerSampler.sampleLight (EmissivePowerSampler.slang:62-83)
                     const float uL = sg.next1D();
-                    uint32_t ti = min(uint32_t(uL * float(sc.n_lights)), sc.n_lights - 1);
-                    const uint2 pk = sc.alias[ti];
-                    const float thr = float(__builtin_bit_cast(_Float16, uint16_t(pk.x >> 16)));
-                    const uint32_t above = ((pk.x & 0xFFFFu) << 8) | ((pk.y >> 24) & 0xFFu);
-                    const uint32_t below = pk.y & 0xFFFFFFu;
-                    ti = sg.next1D() >= thr ? above : below;
+                    uint32_t ti;
+                    if (fr.options & kOptUniformEmissive) {
+                        // EmissiveUniformSampler.sampleLight (EmissiveUniformSampler.slang:52-72); the
+                        // pass rejects an empty active list before launch
+                        ti = sc.active_lights[min(uint32_t(uL * float(sc.n_active)), sc.n_active - 1)];
+                    } else {
+                        // EmissivePowerSampler.sampleLight (EmissivePowerSampler.slang:62-83)
+                        ti = min(uint32_t(uL * float(sc.n_lights)), sc.n_lights - 1);
+                        const uint2 pk = sc.alias[ti];
+                        const float thr = float(__builtin_bit_cast(_Float16, uint16_t(pk.x >> 16)));
+                        const uint32_t above = ((pk.x & 0xFFFFu) << 8) | ((pk.y >> 24) & 0xFFu);
+                        const uint32_t below = pk.y & 0xFFFFFFu;
+                        ti = sg.next1D() >= thr ? above : below;
+                    }
+                    // the sampler's selection pdf (power: flux / total; uniform: 1 / active count)
                     const float selPdf = sc.light_pdf[ti];
                     // sampleTriangle (EmissiveLightSamplerHelpers.slang:76-125)
                     const float4 q0 = sc.lights[4 * ti], q1 = sc.lights[4 * ti + 1], q2 = sc.lights[4 * ti + 2], qn = sc.lights[4 * ti + 3];

@@ -20,7 +20,7 @@ enum : uint32_t {
 
 enum : uint32_t {  // option bits
     kOptNEE = 0x1, kOptMIS = 0x2, kOptRR = 0x4, kOptHwssMis = 0x8, kOptISEmitters = 0x10, kOptDirectLights = 0x20,
-    kOptEmissiveLights = 0x40, kOptMNEE = 0x80
+    kOptEmissiveLights = 0x40, kOptMNEE = 0x80, kOptUniformEmissive = 0x100
 };
 
 // Largest number of transmissive occluders a manifold walk may pass (MNEE_MAX_MS_OCCLUDERS).
@@ -45,6 +45,9 @@ struct PtScene {
     const float* light_pdf;
     const uint32_t* tri_light;
     uint32_t n_lights;
+    // lights with non-zero flux, for the uniform emissive sampler
+    const uint32_t* active_lights;
+    uint32_t n_active;
 };
 
 struct PtFrame {

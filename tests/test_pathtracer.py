@@ -25,14 +25,15 @@ REFERENCE_DEFAULTS = {
     "maxBounces": 32, "tileSize": 512, "HWSS": 4, "HWSSDoMIS": 1, "sampleGenerator": 0, "debugView": 0,
     "debugViewIntensity": 1.0, "sourcingAreaFromEmissiveGeometry": 1, "sourcingMaxBeamOmega": 0.025,
     "useDirectLights": 1, "useEnvLights": 1, "useEmissiveLights": 1, "useAnalyticLights": 1, "alphaMasking": 1,
-    "doNEE": 1, "doMIS": 1, "emissiveSampler": 1, "doRussianRoulette": 1, "NEEUsePerTileSG": 0,
+    "doNEE": 1, "doMIS": 1, "emissiveSampler": 2, "doRussianRoulette": 1, "NEEUsePerTileSG": 0,
     "doImportanceSampleEmitters": 1, "doMNEE": 1, "MNEEMaxOccluders": 2, "MNEEMaxIterations": 60,
     "MNEESolverThreshold": 5e-5,
 }
 
 OPTION_KW = dict(doNEE="do_nee", doMIS="do_mis", doRussianRoulette="do_rr", HWSSDoMIS="hwss_mis",
                  doImportanceSampleEmitters="is_emitters", useDirectLights="direct", useEmissiveLights="emissive",
-                 doMNEE="do_mnee", MNEEMaxOccluders="mnee_max_occluders", MNEEMaxIterations="mnee_max_iterations")
+                 doMNEE="do_mnee", MNEEMaxOccluders="mnee_max_occluders", MNEEMaxIterations="mnee_max_iterations",
+                 emissiveSampler="emissive_sampler")
 
 
 def test_pass_options_defaults_and_unknown_keys():
@@ -208,6 +209,59 @@ def test_oracle_direct_lighting_matches_lambert_closed_form():
         assert np.abs(ratio - 1).max() < 0.04, (do_mis, np.abs(ratio - 1).max())
 
 
+def _fan_light_scene():
+    """The procedural ground and light materials with the 3 x 3 light polygon split into a fan of four
+    triangles of unequal area around an off-centre point (uniform and power selection then differ)."""
+    from pltfalcor_amd.scene import Scene
+    proc = Scene(None, seed=2, sphere_count=0, subdivision=1, material_kinds=1)
+    base = proc.numpy()
+    corners = [[-1.5, 7, -1.5], [1.5, 7, -1.5], [1.5, 7, 1.5], [-1.5, 7, 1.5]]
+    pos = [[-10, 0, -10], [10, 0, -10], [10, 0, 10], [-10, 0, 10]] + corners + [[0.9, 7, -0.6]]
+    nrm = [[0, 1, 0]] * 4 + [[0, -1, 0]] * 5
+    idx = [[0, 2, 1], [0, 3, 2]] + [[8, 4 + i, 4 + (i + 1) % 4] for i in range(4)]
+    a = Scene.from_arrays(None, np.array(pos, np.float32), np.array(nrm, np.float32), np.array(idx, np.uint32),
+                          np.array([0, 0, 1, 1, 1, 1], np.uint32), np.array([0, 1], np.uint32), base["materials"],
+                          base["profiles"], proc.arrays.camera).numpy()
+    e1, e2 = a["positions"][a["indices"][2:, 1]] - a["positions"][a["indices"][2:, 0]], \
+        a["positions"][a["indices"][2:, 2]] - a["positions"][a["indices"][2:, 0]]
+    n = np.cross(e1, e2)
+    assert np.all(n[:, 1] < 0)  # the fan faces the ground
+    areas = 0.5 * np.linalg.norm(n, axis=1)
+    assert areas.max() > 1.5 * areas.min()
+    return a, corners
+
+
+def test_oracle_uniform_emissive_sampler_matches_lambert_closed_form():
+    """maxBounces 0, NEE only, no MIS, under the fan-split light: both the uniform sampler
+    (1/4 per triangle) and the power sampler (area/total) are unbiased for Lambert's polygon
+    irradiance, while their images differ sample by sample."""
+    a, corners = _fan_light_scene()
+    frames, w, h = 64, 16, 16
+    a = _look_down(a, at=(0.3, 0.2), height=4.0)
+    cam = a["camera"]
+    o, d = _camera_rays(cam, w, h)
+    pts = o + (-o[1] / d[:, 1])[:, None] * d
+    poly = [np.array(p, float) for p in corners]
+    E = np.array([_lambert_polygon_irradiance(p, np.array([0, 1.0, 0]), poly) for p in pts])
+    b = dict(a)
+    camL = cam.copy()
+    camL[0:3], camL[3:6], camL[9] = [0.0, 1.0, 0.0], [0.0, 7.0, 0.0], np.radians(10.0)
+    b["camera"] = camL
+    Le = sum(O.render(b, 8, 8, frame=f, max_bounces=0, do_rr=0, is_emitters=0)[:, :3].mean(0) for f in range(frames)) / frames
+    imgs = {}
+    for sampler in (0, 2):
+        acc = np.zeros((w * h, 4))
+        for f in range(frames):
+            acc += O.render(a, w, h, frame=f, max_bounces=0, do_rr=0, is_emitters=0, do_mis=0, threads=8,
+                            emissive_sampler=sampler)
+        acc /= frames
+        imgs[sampler] = acc
+        expected = 0.5 / np.pi * E[:, None] * Le[None, :]
+        rel = np.abs(acc[:, :3].mean(0) - expected.mean(0)) / expected.mean(0)
+        assert (rel < 0.02).all(), (sampler, acc[:, :3].mean(0), expected.mean(0))
+    assert not np.array_equal(imgs[0], imgs[2])
+
+
 def test_oracle_alias_table_reproduces_power_pdf():
     """The power sampler's alias table selects each emissive triangle with probability flux/sum."""
     a = _scene(sphere_count=0, subdivision=1, kinds=1)
@@ -300,6 +354,10 @@ BASE_OPTS = {"maxBounces": 6, "doMNEE": 0, "HWSS": 4}
          opts={"doMNEE": 1, "doMIS": 0, "MNEEMaxOccluders": 4, "MNEEMaxIterations": 30}),
     dict(name="mnee-noHWSSMIS", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=12),
          opts={"doMNEE": 1, "HWSSDoMIS": 0, "MNEESolverThreshold": 1e-4}),
+    dict(name="uniform-sampler", scene=dict(sphere_count=12, subdivision=3, kinds=4, seed=13),
+         opts={"emissiveSampler": 0}),
+    dict(name="uniform-sampler-mnee", scene=dict(sphere_count=24, subdivision=3, kinds=8, seed=14),
+         opts={"emissiveSampler": 0, "doMNEE": 1}),
 ], ids=lambda c: c["name"])
 def test_gpu_pltpt_matches_oracle(gpu, case):
     from pltfalcor_amd.scene import Scene
