@@ -408,7 +408,7 @@ def main():
     seg_per_exec = sum(segs) / SPP
     sample_avg, solve_avg = sample_ms / executes, solve_ms / executes
     if solve_avg >= sample_avg:
-        kname, kms = "solve_kernel<4>", solve_avg
+        kname, kms = "solve_kernel<4, false>", solve_avg
         kbytes = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
         traffic = traffic_from_profile("pltpt_solve_1080p_d8_pmc.md")
     else:

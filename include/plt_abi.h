@@ -286,6 +286,35 @@ typedef struct plt_scene_arrays {
 } plt_scene_arrays;
 int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
 
+/* Analytic light sources: the reference's LightData (Scene/Lights/LightData.slang:36-66), with
+ * the fields its light classes set (Scene/Lights/Light.cpp:115-400): position, direction, spot
+ * opening/penumbra angles, cosine of the subtended half-angle (PLT beam solid angle), surface
+ * area (PLT beam area for point lights, shape area for area lights), the spectral profile of the
+ * intensity, and for area and distant lights the local-to-world matrix and its inverse transpose
+ * (row-major 4x4). Area lights keep the position the reference leaves them (the origin unless
+ * set); their beams are sourced there, as the reference sources them at LightData::posW. */
+typedef enum plt_light_type {
+    PLT_LIGHT_POINT = 0, PLT_LIGHT_DIRECTIONAL = 1, PLT_LIGHT_DISTANT = 2, PLT_LIGHT_RECT = 3, PLT_LIGHT_DISC = 4,
+    PLT_LIGHT_SPHERE = 5
+} plt_light_type;
+typedef struct plt_light {
+    float pos[3];
+    uint32_t type;
+    float dir[3];
+    float opening_angle;
+    float cos_opening_angle;
+    float cos_subtended_angle;
+    float penumbra_angle;
+    uint32_t intensity_profile;
+    float surface_area;
+    float pad[3];
+    float trans[16];
+    float trans_it[16];
+} plt_light;
+/* Replaces the scene's analytic lights (Scene::addLight). A pass reads them when the scene is
+ * bound (plt_pass_set_scene). Returns PLT_EINVAL for a bad type or profile id. */
+int plt_scene_set_lights(plt_scene* scene, const plt_light* lights, uint32_t count);
+
 /* ---------------------------------------------------------------------------------------
  * Ray casting against a scene (device pointers). Rays: 8 planes of n floats (origin xyz,
  * direction xyz, tmin, tmax). Closest hit (mode 0, the reference's TraceRay with a

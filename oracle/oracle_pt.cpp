@@ -10,6 +10,7 @@
 // Camera, shading frames and emissive tables are built here from the scene arrays, independently
 // of the product.
 #include <omp.h>
+#include <limits>
 #include <vector>
 #include "plt_oracle_pt.hpp"
 
@@ -45,7 +46,19 @@ struct OScene {
     // EmissiveUniformSampler: lights with flux > 0 and their selection pdf 1 / count
     std::vector<uint32_t> activeLights;
     std::vector<float> lightPdfUniform;
+    // analytic lights (LightData, Scene/Lights/LightData.slang:47-66)
+    struct Light {
+        float pos[3];
+        uint32_t type;
+        float dir[3];
+        float openingAngle, cosOpeningAngle, cosSubtendedAngle, penumbraAngle;
+        uint32_t profile;
+        float surfaceArea, pad[3];
+        float M[16], MI[16];
+    };
+    std::vector<Light> analytic;
 };
+enum { LT_Point = 0, LT_Directional, LT_Distant, LT_Rect, LT_Disc, LT_Sphere };
 
 struct Options {
     uint32_t width, height, frame, maxBounces;
@@ -54,6 +67,7 @@ struct Options {
     int doMNEE = 0, mneeMaxOccluders = 2, mneeMaxIterations = 60;
     float mneeThreshold = 5e-5f;
     int uniformEmissive = 0;  // emissiveSampler Uniform (else Power)
+    int analyticLights = 1;   // useAnalyticLights
 };
 constexpr int kMaxMSOccluders = 8;
 
@@ -141,6 +155,97 @@ template <class S> void sampleSpectrum(const SpectralProfile* sp, float hero, S&
         for (int j = 0; j < HW; ++j) Le[j] = evalSpectrum<float>(sp, lam[j]);
     }
 }
+// pltSourceBeamDistant (PLT.slang:253-269)
+Beam sourceBeamDistant(V3<F> dir, float solidAngle, const float* Le, const float* lam, float maxOmega, int comps) {
+    solidAngle = std::fmin(solidAngle, maxOmega);
+    Beam b;
+    for (int j = 0; j < HW; ++j) { b.L[j] = Le[j]; b.L1[j] = b.L2[j] = b.L3[j] = 0; b.wavelengths[j] = lam[j]; b.spectralPDFs[j] = 1; }
+    b.hwss_comps = comps;
+    b.Theta.D[0] = solidAngle; b.Theta.D[1] = 0; b.Theta.D[2] = solidAngle;
+    b.Theta.r_m = 0;
+    b.posW = V3<F>{0, 0, 0};
+    b.r = dir;
+    b.t = perp_stark(dir);
+    b.distant = true;
+    return b;
+}
+// An analytic light sample: generateAnalyticLightSample (LightSampling.slangh:122-163) over sampleLight
+// (Rendering/Lights/LightHelpers.slang:70-290). Returns false where the reference's sampler does.
+template <class S>
+bool sampleAnalytic(const OScene& s, const Options& o, const SD<F>& sd, float hero, S& sg, V3<F>& dir, V3<F>& origin,
+                    float& dist, float& pdf, V3<F>& bPos, float& bSize, bool& distant, float* Li, float* lam) {
+    const uint32_t cnt = uint32_t(s.analytic.size());
+    const uint32_t idx = std::min(uint32_t(sampleNext1D<float>(sg) * float(cnt)), cnt - 1);
+    const OScene::Light& L = s.analytic[idx];
+    const SpectralProfile* sp = L.profile < s.nprof ? s.prof + L.profile : nullptr;
+    const float* M = L.M;
+    const float* MI = L.MI;
+    const float k2Pi = 6.28318530717958647692f;
+    float lpdf = 0.f;
+    distant = false;
+    if (L.type == LT_Rect || L.type == LT_Disc || L.type == LT_Sphere) {
+        const float u0 = sampleNext1D<float>(sg), u1 = sampleNext1D<float>(sg);
+        V3<F> pl, nl{0, 0, 1};
+        if (L.type == LT_Rect) {
+            pl = V3<F>{u0 * 2.f - 1.f, u1 * 2.f - 1.f, 0.f};
+        } else if (L.type == LT_Disc) {  // sample_disk (MathHelpers.slang:255-263)
+            const float r = std::sqrt(u0), phi = k2Pi * u1;
+            pl = V3<F>{r * std::cos(phi), r * std::sin(phi), 0.f};
+        } else {  // sample_sphere (MathHelpers.slang:282-288)
+            const float phi = k2Pi * u1, ct = 1.f - 2.f * u0;
+            const float st = std::sqrt(std::fmax(0.f, 1.f - ct * ct));
+            pl = V3<F>{st * std::cos(phi), st * std::sin(phi), ct};
+            nl = pl;
+        }
+        const V3<F> pw{M[0] * pl.x + M[1] * pl.y + M[2] * pl.z + M[3], M[4] * pl.x + M[5] * pl.y + M[6] * pl.z + M[7],
+                       M[8] * pl.x + M[9] * pl.y + M[10] * pl.z + M[11]};
+        const V3<F> nw = normalize(V3<F>{MI[0] * nl.x + MI[1] * nl.y + MI[2] * nl.z, MI[4] * nl.x + MI[5] * nl.y + MI[6] * nl.z,
+                                         MI[8] * nl.x + MI[9] * nl.y + MI[10] * nl.z});
+        // finalizeAreaLightSample (LightHelpers.slang:70-93)
+        const V3<F> toL = pw - sd.posW;
+        const float d2 = std::fmax(dot(toL, toL), 1e-9f);
+        dist = std::sqrt(d2);
+        dir = toL / dist;
+        const float ct = dot(nw, -dir);
+        if (ct <= 0.f) return false;
+        sampleSpectrum(sp, hero, sg, o.isEmitters != 0, Li, lam);
+        const float f = L.surfaceArea * ct / d2;
+        for (int j = 0; j < HW; ++j) Li[j] *= f;
+        lpdf = d2 / (ct * L.surfaceArea);
+    } else if (L.type == LT_Distant) {  // sampleDistantLight (:170-198), sample_cone (MathHelpers.slang:270-276)
+        const float u0 = sampleNext1D<float>(sg), u1 = sampleNext1D<float>(sg);
+        const float c = L.cosSubtendedAngle;
+        const float z = u0 * (1.f - c) + c;
+        const float r = std::sqrt(1.f - z * z), phi = k2Pi * u1;
+        const V3<F> d{r * std::cos(phi), r * std::sin(phi), z};
+        dir = normalize(V3<F>{M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z,
+                              M[8] * d.x + M[9] * d.y + M[10] * d.z});
+        dist = std::numeric_limits<float>::max();
+        sampleSpectrum(sp, hero, sg, o.isEmitters != 0, Li, lam);
+        lpdf = 1.f / (k2Pi * (1.f - c));
+        distant = true;
+    } else if (L.type == LT_Directional) {  // sampleDirectionalLight (:200-225)
+        dir = V3<F>{-L.dir[0], -L.dir[1], -L.dir[2]};
+        dist = std::numeric_limits<float>::max();
+        sampleSpectrum(sp, hero, sg, o.isEmitters != 0, Li, lam);
+        lpdf = 0.f;
+        distant = true;
+    } else {  // samplePointLight (:227-269); pdf 0, so the sample is rejected
+        const V3<F> toL = V3<F>{L.pos[0], L.pos[1], L.pos[2]} - sd.posW;
+        const float d2 = std::fmax(dot(toL, toL), 1e-9f);
+        dist = std::sqrt(d2);
+        dir = toL / dist;
+        sampleSpectrum(sp, hero, sg, o.isEmitters != 0, Li, lam);
+        lpdf = 0.f;
+    }
+    for (int j = 0; j < HW; ++j) Li[j] *= float(cnt);
+    pdf = lpdf / float(cnt);
+    origin = rayOrigin(sd.posW, dot(sd.faceN, dir) >= 0.f ? sd.faceN : -sd.faceN);  // computeNewRayOrigin
+    bPos = V3<F>{L.pos[0], L.pos[1], L.pos[2]};
+    bSize = distant ? 4.f * 3.14159265358979323846f * (1.f - L.cosSubtendedAngle * L.cosSubtendedAngle) : L.surfaceArea;
+    return true;
+}
+
 // pltSourceBeamArea (PLT.slang:275-290)
 Beam sourceBeamArea(V3<F> posW, V3<F> dir, float A, float dist, const float* Le, const float* lam, float maxOmega, int comps) {
     const float r2 = sqr(dist * 1e+3f);
@@ -336,7 +441,7 @@ bool manifoldOneStep(const PDV<F>& w, const KPDV<F>& ws, const PDV<F>& p, float&
 }
 template <class S>
 bool manifoldNEESample(const OScene& s, const Options& o, V3<F> visOrg, V3<F> visDir, float visTmax, V3<F> origin, V3<F> lightPos,
-                       V3<F> lightDir, int maxMSBounces, S& sg, MSData& d) {  // :165-281 (area lights: not distant)
+                       V3<F> lightDir, bool distantLight, int maxMSBounces, S& sg, MSData& d) {  // :165-281
     if (!traceMNEEOccluders(s, o, visOrg, visDir, visTmax, maxMSBounces, sg, d)) return false;
     if (d.hitCount == 0) return true;
     float beta[kMaxMSOccluders];
@@ -349,8 +454,10 @@ bool manifoldNEESample(const OScene& s, const Options& o, V3<F> visOrg, V3<F> vi
             const DSG<F> dsg = dsgAt(s, h.tri, h.b1, h.b2);
             const V3<F> p0 = idx == 0 ? origin : d.hits[idx - 1].posW;
             const PDV<F> pdv0 = dsg.pdw(p0);
-            const V3<F> p2 = idx + 1 == d.hitCount ? lightPos : d.hits[idx + 1].posW;
-            const PDV<F> pdv2 = dsg.pdw(p2);
+            const bool last = idx + 1 == d.hitCount;
+            const V3<F> p2 = last ? lightPos : d.hits[idx + 1].posW;
+            // a distant light fixes the last direction (:199-202)
+            const PDV<F> pdv2 = last && distantLight ? PDV<F>{lightDir, V3<F>{0, 0, 0}, V3<F>{0, 0, 0}} : dsg.pdw(p2);
             const bool woIsIn = dot(pdv2.v, dsg.n.v) < F(0);
             const PDV<F> wi = woIsIn ? pdv0 : pdv2, wo = woIsIn ? pdv2 : pdv0;
             const SD<F> sd = shadingData(s, h.tri, h.b1, h.b2, wi.v);
@@ -378,9 +485,16 @@ bool manifoldNEESample(const OScene& s, const Options& o, V3<F> visOrg, V3<F> vi
         }
     }
     const V3<F> p = d.hits[d.hitCount - 1].posW;
-    V3<F> dir = lightPos - p;
-    const F dist = length(dir);
-    dir = dir / dist;
+    V3<F> dir;
+    F dist;
+    if (!distantLight) {
+        dir = lightPos - p;
+        dist = length(dir);
+        dir = dir / dist;
+    } else {
+        dir = lightDir;
+        dist = std::numeric_limits<float>::max();
+    }
     d.woLastHitToLight = dir;
     return visible(s, p, dir, 1e-4f, dist);
 }
@@ -414,6 +528,13 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
     // Scene::useEmissiveLights needs an active (flux > 0) emissive triangle
     const bool useEmissive = o.emissiveLights && !s.activeLights.empty();
     const std::vector<float>& selPdfs = o.uniformEmissive ? s.lightPdfUniform : s.lightPdf;
+    const bool useAnalytic = o.analyticLights && !s.analytic.empty();
+    // getLightTypeSelectionProbabilities (LightSampling.slangh:37-55)
+    float pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
+    {
+        const float sum = pEmissive + pAnalytic;
+        if (sum != 0.f) { const float inv = 1.f / sum; pEmissive *= inv; pAnalytic *= inv; }
+    }
     const bool doNEE = o.doNEE != 0, doMIS = doNEE && o.doMIS;
     for (int p = 0;; ++p) {
         const Bounce& bounce = rec[size_t(p)];
@@ -429,15 +550,25 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
             const bool sampleRefl = (lobes & Lobe_NonDeltaReflection) != 0 || allowMS;
             const bool sampleTrans = (lobes & Lobe_NonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
-                // generateLightSample: light type selection (only emissive lights present)
-                const float pType = useEmissive ? 1.f : 0.f;
+                // selectLightType (LightSampling.slangh:57-85): environment (absent), emissive, analytic
                 float u = sampleNext1D<float>(sg);
+                int ltype = -1;
+                float typePdf = 0.f;
+                if (u < pEmissive) {
+                    ltype = 1; typePdf = pEmissive;
+                } else {
+                    u -= pEmissive;
+                    if (u < pAnalytic) { ltype = 2; typePdf = pAnalytic; }
+                }
                 bool valid = false;
                 float lsPdf = 0, lsDist = 0;
                 V3<F> lsDir{0, 0, 0}, lsOrigin{0, 0, 0}, tPos{0, 0, 0};
+                V3<F> bPos{0, 0, 0};
+                float bSize = o.sourcingArea;
+                bool bDistant = false;
                 float Le[HW], lam[HW];
                 for (int j = 0; j < HW; ++j) { Le[j] = 0; lam[j] = sv.wavelength; }
-                if (u < pType) {
+                if (ltype == 1) {
                     const float uL = sampleNext1D<float>(sg);
                     uint32_t ti;
                     if (o.uniformEmissive) {
@@ -486,22 +617,35 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                         const V3<F> tl = lightPos - lsOrigin;
                         lsDist = length(tl);
                         lsDir = normalize(tl);
+                        bPos = tPos;
                         valid = false;
                         for (int j = 0; j < HW; ++j) valid |= Le[j] > 0.f;
                     }
                 }
+                if (ltype == 2) {
+                    valid = sampleAnalytic(s, o, sd, sv.wavelength, sg, lsDir, lsOrigin, lsDist, lsPdf, bPos, bSize, bDistant, Le, lam);
+                    if (valid) {
+                        bool anyL = false;
+                        for (int j = 0; j < HW; ++j) anyL |= Le[j] > 0.f;
+                        valid = anyL;
+                    }
+                }
+                lsPdf *= typePdf;
                 const bool isRefl = dot(sd.faceN, lsDir) * dot(sd.faceN, sd.V) > 0.f;
                 valid = valid && lsPdf > 1e-6f && !(!sampleRefl && isRefl) && !(!sampleTrans && !isRefl);
                 bool anyL = false;
                 for (int j = 0; j < HW; ++j) anyL |= Le[j] > 0.f;
                 if (valid && anyL) {
                     const int comps = (!o.hwssMIS && (bounce.flags & RF_Dispersive)) ? 1 : HW;
-                    Beam beam = sourceBeamArea(tPos, -lsDir, o.sourcingArea, lsDist, Le, lam, o.maxOmega, comps);
+                    Beam beam = bDistant ? sourceBeamDistant(-lsDir, bSize, Le, lam, o.maxOmega, comps)
+                                         : sourceBeamArea(bPos, -lsDir, bSize, lsDist, Le, lam, o.maxOmega, comps);
+                    beam.scaleAll(1.f / typePdf);
+                    const bool misL = doMIS && ltype != 2;  // no MIS for analytic lights (pltpt_solve.rt.slang:275)
                     bool evalCurrent = true, ok = true;
                     const int maxMSBounces = int(o.maxBounces) - p;
                     if (!o.doMNEE || maxMSBounces <= 0) {
                         ok = visible(s, lsOrigin, lsDir, 0.f, lsDist);
-                        if (ok && doMIS) {
+                        if (ok && misL) {
                             const float bp = materialPdf(bsdf, sd, lsDir, beam.wavelengths[0]);
                             beam.scaleAll(lsPdf / (lsPdf + bp));
                         }
@@ -517,18 +661,21 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                             ok = manifoldSampleVertex(bounce.tri, bounce.b1, bounce.b2, sd, bsdf, lightDir, sg, ms) &&
                                  (ms.hits[0].lobe & Lobe_Reflection) != 0;
                         const V3<F> origin = !currentVertexMS ? sd.posW : sv.pathOrigin;
-                        ok = ok && manifoldNEESample(s, o, lsOrigin, lsDir, lsDist, origin, beam.posW, lightDir, maxMSBounces, sg, ms);
+                        ok = ok && manifoldNEESample(s, o, lsOrigin, lsDir, lsDist, origin, beam.posW, lightDir, beam.distant,
+                                                     maxMSBounces, sg, ms);
                         ok = ok && !(ms.pdf < 1e-6f);
                         if (ok && ms.hitCount > 0) {
                             if (!o.hwssMIS && ms.dispersive) beam.hwss_comps = 1;
-                            dirL = ms.woLastHitToLight;
-                            beam.r = -ms.woLastHitToLight;
-                            beam.t = perp_stark(beam.r);
+                            if (!beam.distant) {
+                                dirL = ms.woLastHitToLight;
+                                beam.r = -ms.woLastHitToLight;
+                                beam.t = perp_stark(beam.r);
+                            }
                             beam.scaleAll(1.f / ms.pdf);
                             manifoldForwardUnwind(s, o, ms, origin, beam);
                             if (currentVertexMS) { evalCurrent = false; misBsdfPdf = bounce.lastNonDeltaPdf; }
                         }
-                        if (ok && doMIS) {
+                        if (ok && misL) {
                             if (ms.hitCount == 0 || !currentVertexMS) misBsdfPdf = materialPdf(bsdf, sd, dirL, beam.wavelengths[0]);
                             beam.scaleAll(lsPdf / (lsPdf + misBsdfPdf * ms.pdf));
                         }
@@ -566,7 +713,7 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                 if (d2 > kFLT_MIN) {
                     const V3<F> L = toL / std::sqrt(d2);
                     const float ct = dot(hn, -L);
-                    if (ct > 0.f) lightPdf = selPdfs[lt] * (d2 / std::fmax(kFLT_MIN, ct * s.lightArea[lt]));
+                    if (ct > 0.f) lightPdf = pEmissive * (selPdfs[lt] * (d2 / std::fmax(kFLT_MIN, ct * s.lightArea[lt])));
                 }
             }
             const float mis = needMIS ? bounce.lastNonDeltaPdf / (bounce.lastNonDeltaPdf + lightPdf) : 1.f;
@@ -725,6 +872,16 @@ void* oracle_scene_create(const float* pos, const float* nrm, uint32_t ntri, con
 
 void oracle_scene_destroy(void* h) { delete static_cast<OScene*>(h); }
 
+// Analytic lights of a scene made by oracle_scene_create: `count` records of 48 floats in the
+// plt_light layout (LightData).
+int oracle_scene_set_lights(void* h, const float* lights, uint32_t count) {
+    OScene& s = *static_cast<OScene*>(h);
+    s.analytic.resize(count);
+    static_assert(sizeof(OScene::Light) == 48 * sizeof(float), "light record");
+    if (count) std::memcpy(s.analytic.data(), lights, count * sizeof(OScene::Light));
+    return 0;
+}
+
 // Renders with a scene made by oracle_scene_create (whose arrays the caller keeps alive).
 int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count,
                         float* color, uint32_t* records, int threads) {
@@ -752,6 +909,7 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
     o.mneeMaxIterations = iopt[13];
     o.mneeThreshold = fopt[2];
     o.uniformEmissive = iopt[14];
+    o.analyticLights = iopt[15];
     if (threads > 0) omp_set_num_threads(threads);
     const size_t n = count;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -809,7 +967,7 @@ int oracle_mnee_probe(void* h, const float* origin, const float* lightPos, float
     UniformSampleGenerator sg(seed, 0u, 0u);
     MSData ms;
     ms.wavelength = wavelength;
-    const bool ok = manifoldNEESample(s, o, org, tl / dist, dist * (1.f - 1e-4f), org, lp, tl / dist, maxMSBounces, sg, ms);
+    const bool ok = manifoldNEESample(s, o, org, tl / dist, dist * (1.f - 1e-4f), org, lp, tl / dist, false, maxMSBounces, sg, ms);
     *iterations = ms.iterations;
     *pdf = ms.pdf;
     for (int i = 0; i < ms.hitCount; ++i) {

@@ -70,6 +70,14 @@ class SceneArrays(C.Structure):
                 ("camera", Camera), ("bvh_max_depth", C.c_uint32)]
 
 
+class Light(C.Structure):
+    """plt_light: the reference's LightData (Scene/Lights/LightData.slang:47-66)."""
+    _fields_ = [("pos", C.c_float * 3), ("type", C.c_uint32), ("dir", C.c_float * 3), ("opening_angle", C.c_float),
+                ("cos_opening_angle", C.c_float), ("cos_subtended_angle", C.c_float), ("penumbra_angle", C.c_float),
+                ("intensity_profile", C.c_uint32), ("surface_area", C.c_float), ("pad", C.c_float * 3),
+                ("trans", C.c_float * 16), ("trans_it", C.c_float * 16)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32), ("n_meshes", C.c_uint32),
                 ("n_materials", C.c_uint32), ("n_profiles", C.c_uint32),
@@ -118,6 +126,7 @@ SIGNATURES = [
     ("plt_scene_destroy", C.c_int, [C.c_void_p]),
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
     ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    ("plt_scene_set_lights", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     ("plt_pass_create", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("plt_pass_destroy", C.c_int, [C.c_void_p]),
     ("plt_pass_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),

@@ -154,6 +154,8 @@ struct plt_pass {
     float* d_light_pdf_uniform = nullptr;
     uint32_t* d_active = nullptr;
     uint32_t n_active = 0;
+    plt_light* d_analytic = nullptr;
+    uint32_t n_analytic = 0;
     uint32_t* d_records = nullptr;
     size_t records_words = 0;
     size_t last_words = 0;  // record words written by the last execute
@@ -185,6 +187,8 @@ void free_scene_state(plt_pass* p) {
     p->d_light_pdf_uniform = nullptr;
     p->d_active = nullptr;
     p->n_active = 0;
+    p->d_analytic = nullptr;
+    p->n_analytic = 0;
 }
 // Camera frame (Camera.cpp:187-193).
 void camera_frame(const plt_camera& c, float out[12]) {
@@ -279,6 +283,8 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     if ((rc = plt::upload_vec(lt.pdf_uniform, &p->d_light_pdf_uniform, p->owned)) < 0) return rc;
     if ((rc = plt::upload_vec(lt.active, &p->d_active, p->owned)) < 0) return rc;
     p->n_active = uint32_t(lt.active.size());
+    if ((rc = plt::upload_vec(scene->host.lights, &p->d_analytic, p->owned)) < 0) return rc;
+    p->n_analytic = uint32_t(scene->host.lights.size());
     // Scene::useEmissiveLights requires an active triangle; otherwise emissive lighting is off
     p->n_lights = p->n_active > 0 ? uint32_t(lt.pdf.size()) : 0;
     plt::camera_frame(scene->host.camera, p->cam);
@@ -373,6 +379,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     sc.light_pdf = emissiveSampler == 0 ? p->d_light_pdf_uniform : p->d_light_pdf;
     sc.active_lights = p->d_active;
     sc.n_active = p->n_active;
+    sc.analytic = p->d_analytic;
+    sc.n_analytic = p->n_analytic;
     sc.tri_light = p->d_tri_light;
     sc.n_lights = p->n_lights;
 
@@ -392,6 +400,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (o("useEmissiveLights") != 0) opts |= plt::kOptEmissiveLights;
     if (o("doMNEE") != 0) opts |= plt::kOptMNEE;
     if (emissiveSampler == 0) opts |= plt::kOptUniformEmissive;
+    if (o("useAnalyticLights") != 0) opts |= plt::kOptAnalyticLights;
     fr.options = opts;
     fr.mnee_max_occluders = uint32_t(mneeOcc);
     fr.mnee_max_iterations = uint32_t(mneeIt);

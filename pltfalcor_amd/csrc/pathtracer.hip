@@ -176,6 +176,120 @@ PLT_DEV Beam<N> source_beam_area(V3 pos, V3 dir, float A, float dist, const floa
     return b;
 }
 
+// pltSourceBeamDistant (PLT.slang:253-269): a beam from a source at infinity subtending solidAngle.
+template <int N>
+PLT_DEV Beam<N> source_beam_distant(V3 dir, float solidAngle, const float* Le, const float* lam, float maxOmega, int comps) {
+    Beam<N> b;
+    solidAngle = fminf(solidAngle, maxOmega);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        b.S[j][0] = Le[j]; b.S[j][1] = b.S[j][2] = b.S[j][3] = 0.f;
+        b.lambda[j] = lam[j];
+        b.spdf[j] = 1.f;
+    }
+    b.comps = comps;
+    b.D00 = solidAngle; b.D01 = 0.f; b.D11 = solidAngle;
+    b.r_m = 0.f;
+    b.pos = v3(0.f, 0.f, 0.f);
+    b.r = dir;
+    b.t = perp_stark(dir);
+    b.distant = true;
+    return b;
+}
+
+// An analytic light sample (generateAnalyticLightSample, LightSampling.slangh:122-163, over
+// sampleLight, Rendering/Lights/LightHelpers.slang:70-290): uniform choice of a light, its sample,
+// radiance scaled by the light count and pdf divided by it. Point and directional lights sample
+// with pdf 0 (a Dirac), which the caller's acceptance test then rejects, as the reference does.
+// Returns false where the reference's sampler does (an area light seen from behind).
+struct AnalyticSample {
+    V3 dir, origin, beam_pos;
+    float dist, pdf, beam_size;  // beam_size: area (mm^2) or, for distant beams, solid angle
+    bool distant;
+};
+// Kept out of line: inlined, its registers would count against the whole solve kernel, which
+// scenes without analytic lights never pay for otherwise.
+template <int N>
+__device__ __attribute__((noinline)) bool sample_analytic(const PtScene& sc, const Surf& s, float lambda, bool importance,
+                                                           Rng& sg, AnalyticSample& a, float* Li, float* lam) {
+#pragma clang fp contract(off)
+    const uint32_t cnt = sc.n_analytic;
+    const uint32_t idx = min(uint32_t(sg.next1D() * float(cnt)), cnt - 1);
+    const plt_light& L = sc.analytic[idx];
+    const plt_spectral_profile* sp = L.intensity_profile < sc.n_profiles ? sc.profiles + L.intensity_profile : nullptr;
+    const float* M = L.trans;
+    const float* MI = L.trans_it;
+    constexpr float k2Pi = 6.28318530717958647692f;
+    float lpdf = 0.f;
+    a.distant = false;
+    if (L.type == PLT_LIGHT_RECT || L.type == PLT_LIGHT_DISC || L.type == PLT_LIGHT_SPHERE) {
+        const float u0 = sg.next1D(), u1 = sg.next1D();
+        V3 pl, nl = v3(0.f, 0.f, 1.f);
+        if (L.type == PLT_LIGHT_RECT) {
+            pl = v3(u0 * 2.f - 1.f, u1 * 2.f - 1.f, 0.f);
+        } else if (L.type == PLT_LIGHT_DISC) {  // sample_disk (MathHelpers.slang:255-263)
+            const float r = sqrtf(u0), phi = k2Pi * u1;
+            pl = v3(r * cosf(phi), r * sinf(phi), 0.f);
+        } else {  // sample_sphere (MathHelpers.slang:282-288)
+            const float phi = k2Pi * u1, ct = 1.f - 2.f * u0;
+            const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+            pl = v3(st * cosf(phi), st * sinf(phi), ct);
+            nl = pl;
+        }
+        const V3 pw = v3(M[0] * pl.x + M[1] * pl.y + M[2] * pl.z + M[3], M[4] * pl.x + M[5] * pl.y + M[6] * pl.z + M[7],
+                         M[8] * pl.x + M[9] * pl.y + M[10] * pl.z + M[11]);
+        const V3 nw = normalize(v3(MI[0] * nl.x + MI[1] * nl.y + MI[2] * nl.z, MI[4] * nl.x + MI[5] * nl.y + MI[6] * nl.z,
+                                   MI[8] * nl.x + MI[9] * nl.y + MI[10] * nl.z));
+        // finalizeAreaLightSample (LightHelpers.slang:70-93)
+        const V3 toL = pw - s.pos;
+        const float d2 = fmaxf(toL.x * toL.x + toL.y * toL.y + toL.z * toL.z, 1e-9f);
+        a.dist = sqrtf(d2);
+        a.dir = v3(toL.x / a.dist, toL.y / a.dist, toL.z / a.dist);
+        const float ct = nw.x * -a.dir.x + nw.y * -a.dir.y + nw.z * -a.dir.z;
+        if (ct <= 0.f) return false;
+        sample_spectrum<N>(sp, lambda, sg, importance, Li, lam);
+        const float f = L.surface_area * ct / d2;
+#pragma unroll
+        for (int j = 0; j < N; ++j) Li[j] *= f;
+        lpdf = d2 / (ct * L.surface_area);
+    } else if (L.type == PLT_LIGHT_DISTANT) {  // sampleDistantLight (:170-198), sample_cone (MathHelpers.slang:270-276)
+        const float u0 = sg.next1D(), u1 = sg.next1D();
+        const float c = L.cos_subtended_angle;
+        const float z = u0 * (1.f - c) + c;
+        const float r = sqrtf(1.f - z * z), phi = k2Pi * u1;
+        const V3 d = v3(r * cosf(phi), r * sinf(phi), z);
+        a.dir = normalize(v3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z,
+                             M[8] * d.x + M[9] * d.y + M[10] * d.z));
+        a.dist = kFltMax;
+        sample_spectrum<N>(sp, lambda, sg, importance, Li, lam);
+        lpdf = 1.f / (k2Pi * (1.f - c));
+        a.distant = true;
+    } else if (L.type == PLT_LIGHT_DIRECTIONAL) {  // sampleDirectionalLight (:200-225)
+        a.dir = v3(-L.dir[0], -L.dir[1], -L.dir[2]);
+        a.dist = kFltMax;
+        sample_spectrum<N>(sp, lambda, sg, importance, Li, lam);
+        lpdf = 0.f;
+        a.distant = true;
+    } else {  // samplePointLight (:227-269): a Dirac, rejected by the caller (pdf 0)
+        const V3 toL = v3(L.pos[0], L.pos[1], L.pos[2]) - s.pos;
+        const float d2 = fmaxf(toL.x * toL.x + toL.y * toL.y + toL.z * toL.z, 1e-9f);
+        a.dist = sqrtf(d2);
+        a.dir = v3(toL.x / a.dist, toL.y / a.dist, toL.z / a.dist);
+        sample_spectrum<N>(sp, lambda, sg, importance, Li, lam);
+        lpdf = 0.f;
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) Li[j] *= float(cnt);
+    a.pdf = lpdf / float(cnt);
+    // computeNewRayOrigin(sd, dir) (ShadowRays.slangh:50-52)
+    const float side = s.faceN.x * a.dir.x + s.faceN.y * a.dir.y + s.faceN.z * a.dir.z;
+    a.origin = ray_origin(s.pos, side >= 0.f ? s.faceN : -s.faceN);
+    a.beam_pos = v3(L.pos[0], L.pos[1], L.pos[2]);
+    a.beam_size = a.distant ? 4.f * 3.14159265358979323846f * (1.f - L.cos_subtended_angle * L.cos_subtended_angle)
+                            : L.surface_area;
+    return true;
+}
+
 struct Rec {
     uint32_t tri;
     float b1, b2;
@@ -591,7 +705,7 @@ PLT_DEV bool ms_trace_occluders(const PtScene& sc, V3 org, V3 dir, float tmax, i
 // Every pass over unconverged vertices costs at least one iteration, so the walk ends after at most
 // maxIterations ray casts.
 PLT_DEV bool ms_nee_sample(const PtScene& sc, const PtFrame& fr, V3 visOrg, V3 visDir, float visTmax, V3 origin, V3 lightPos,
-                           int maxMSBounces, Rng& sg, MsData& d, int* stk) {
+                           bool distantLight, V3 lightDir, int maxMSBounces, Rng& sg, MsData& d, int* stk) {
     if (!ms_trace_occluders(sc, visOrg, visDir, visTmax, int(fr.mnee_max_occluders), maxMSBounces, sg, d, stk)) return false;
     if (d.count == 0) return true;
     float beta[kMaxMsOccluders];
@@ -605,8 +719,10 @@ PLT_DEV bool ms_nee_sample(const PtScene& sc, const PtFrame& fr, V3 visOrg, V3 v
             const DSG dsg = dsg_at(sc, h.tri, h.b1, h.b2);
             const V3 p0 = idx == 0 ? origin : d.hits[idx - 1].pos;
             const PDV pdv0 = dsg.pdw(p0);
-            const V3 p2 = idx + 1 == d.count ? lightPos : d.hits[idx + 1].pos;
-            const PDV pdv2 = dsg.pdw(p2);
+            const bool last = idx + 1 == d.count;
+            const V3 p2 = last ? lightPos : d.hits[idx + 1].pos;
+            // a distant light fixes the last direction (Manifolds.slangh:199-202)
+            const PDV pdv2 = last && distantLight ? PDV{lightDir, v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f)} : dsg.pdw(p2);
             // the scattered direction is taken on the into-material side (avoids total internal reflection)
             const bool woIsIn = wdot(pdv2.v, dsg.n.v) < 0.f;
             const PDV wi = woIsIn ? pdv0 : pdv2, wo = woIsIn ? pdv2 : pdv0;
@@ -644,9 +760,16 @@ PLT_DEV bool ms_nee_sample(const PtScene& sc, const PtFrame& fr, V3 visOrg, V3 v
         }
     }
     const V3 p = d.hits[d.count - 1].pos;
-    V3 dir = lightPos - p;
-    const float dist = wlength(dir);
-    dir = wdiv(dir, dist);
+    V3 dir;
+    float dist;
+    if (!distantLight) {
+        dir = lightPos - p;
+        dist = wlength(dir);
+        dir = wdiv(dir, dist);
+    } else {
+        dir = lightDir;
+        dist = kFltMax;
+    }
     d.woLast = dir;
     const Ray vr{p.x, p.y, p.z, dir.x, dir.y, dir.z, 1e-4f, dist};
     return traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock).tri == kNoHit;
@@ -665,8 +788,9 @@ PLT_DEV void ms_forward_unwind(const PtScene& sc, const PtFrame& fr, const MsDat
 // over 256 registers at its peak, which allows one wave per SIMD; capping it at 128 registers
 // (spilling the rest to scratch) gives the traversals four waves to hide latency behind. At
 // 1080p depth 8 with manifold NEE: 18.0 ms per frame at one wave, 11.3 ms at two, 10.5 ms at
-// three, 10.1 ms at four.
-template <int N>
+// three, 10.1 ms at four. `Analytic` instantiates the analytic-light code only for scenes that
+// have analytic lights, so scenes lit by emissive geometry alone do not carry its registers.
+template <int N, bool Analytic>
 __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 solve_kernel(PtScene sc, PtFrame fr) {
     __shared__ int stack[kTraversalStackDepth * kPtBlock];
@@ -678,8 +802,20 @@ solve_kernel(PtScene sc, PtFrame fr) {
     int* stk = stack + threadIdx.x;
     const bool doNEE = (fr.options & kOptNEE) != 0, doMIS = doNEE && (fr.options & kOptMIS) != 0;
     const bool useEmissive = (fr.options & kOptEmissiveLights) != 0 && sc.n_lights > 0;
+    const bool useAnalytic = Analytic && (fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0;
     const bool importance = (fr.options & kOptISEmitters) != 0;
     const bool doMNEE = doNEE && (fr.options & kOptMNEE) != 0;
+    // getLightTypeSelectionProbabilities (LightSampling.slangh:37-55): equal weights over the
+    // present light types (no environment map here)
+    float pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
+    {
+        const float psum = pEmissive + pAnalytic;
+        if (psum != 0.f) {
+            const float inv = 1.f / psum;
+            pEmissive *= inv;
+            pAnalytic *= inv;
+        }
+    }
 
     Rng sg = Rng::make(px, py, fr.frame);
     const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
@@ -703,11 +839,27 @@ solve_kernel(PtScene sc, PtFrame fr) {
             const bool sampleRefl = (lobes & kLobeNonDeltaReflection) != 0 || allowMS;
             const bool sampleTrans = (lobes & kLobeNonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
-                // selectLightType: only emissive geometry is present -> one draw, p = 1
-                const float uType = sg.next1D();
-                bool valid = useEmissive && uType < 1.f;
+                // selectLightType (LightSampling.slangh:57-85)
+                float uType = sg.next1D();
+                int ltype = -1;  // SampledLightType: 1 emissive, 2 analytic
+                float typePdf = 0.f;
+                if (uType < pEmissive) {
+                    ltype = 1;
+                    typePdf = pEmissive;
+                } else {
+                    uType -= pEmissive;
+                    if (uType < pAnalytic) {
+                        ltype = 2;
+                        typePdf = pAnalytic;
+                    }
+                }
+                bool valid = ltype == 1;
                 float lsPdf = 0.f, lsDist = 0.f;
                 V3 lsDir = v3(0.f, 0.f, 0.f), lsOrigin = v3(0.f, 0.f, 0.f), lightPos = v3(0.f, 0.f, 0.f), tPos = v3(0.f, 0.f, 0.f);
+                // the sourced beam: position, area (or solid angle) and whether it is distant
+                V3 bPos = v3(0.f, 0.f, 0.f);
+                float bSize = fr.sourcing_area;
+                bool bDistant = false;
                 float Le[N], lam[N];
 #pragma unroll
                 for (int j = 0; j < N; ++j) { Le[j] = 0.f; lam[j] = lambda; }
@@ -766,12 +918,31 @@ solve_kernel(PtScene sc, PtFrame fr) {
                         const V3 tl = lightPos - lsOrigin;
                         lsDist = sqrtf(tl.x * tl.x + tl.y * tl.y + tl.z * tl.z);
                         lsDir = v3(tl.x / lsDist, tl.y / lsDist, tl.z / lsDist);
+                        bPos = tPos;
                         bool anyL = false;
 #pragma unroll
                         for (int j = 0; j < N; ++j) anyL |= Le[j] > 0.f;
                         valid = anyL;
                     }
                 }
+                if (Analytic && ltype == 2) {
+                    AnalyticSample as;
+                    valid = sample_analytic<N>(sc, s, lambda, importance, sg, as, Le, lam);
+                    if (valid) {
+                        lsDir = as.dir;
+                        lsOrigin = as.origin;
+                        lsDist = as.dist;
+                        lsPdf = as.pdf;
+                        bPos = as.beam_pos;
+                        bSize = as.beam_size;
+                        bDistant = as.distant;
+                        bool anyL = false;
+#pragma unroll
+                        for (int j = 0; j < N; ++j) anyL |= Le[j] > 0.f;
+                        valid = anyL;
+                    }
+                }
+                lsPdf *= typePdf;  // account for the light type selection
                 // generateLightSample's acceptance (LightSampling.slangh:196-205)
                 const bool isRefl = dot(s.faceN, lsDir) * dot(s.faceN, bounce.wi) > 0.f;
                 valid = valid && lsPdf > 1e-6f && !(!sampleRefl && isRefl) && !(!sampleTrans && !isRefl);
@@ -780,13 +951,17 @@ solve_kernel(PtScene sc, PtFrame fr) {
                 for (int j = 0; j < N; ++j) anyLe |= Le[j] > 0.f;
                 if (valid && anyLe) {
                     const int comps = (!(fr.options & kOptHwssMis) && (bounce.flags & kFlagDispersive)) ? 1 : N;
-                    Beam<N> beam = source_beam_area<N>(tPos, -lsDir, fr.sourcing_area, lsDist, Le, lam, fr.max_omega, comps);
+                    Beam<N> beam = bDistant ? source_beam_distant<N>(-lsDir, bSize, Le, lam, fr.max_omega, comps)
+                                            : source_beam_area<N>(bPos, -lsDir, bSize, lsDist, Le, lam, fr.max_omega, comps);
+                    beam.scale(1.f / typePdf);
+                    // analytic lights are not hit by BSDF samples: no MIS (pltpt_solve.rt.slang:275)
+                    const bool misL = doMIS && ltype != 2;
                     bool evalCurrent = true, ok = true;
                     const int maxMSBounces = int(fr.max_bounces) - p;
                     if (!doMNEE || maxMSBounces <= 0) {
                         const Ray vr{lsOrigin.x, lsOrigin.y, lsOrigin.z, lsDir.x, lsDir.y, lsDir.z, 0.f, lsDist};
                         ok = traverse<true, kTraversalStackDepth>(sc.tsc, vr, stk, kPtBlock).tri == kNoHit;
-                        if (ok && doMIS) {
+                        if (ok && misL) {
                             const float bp = mat_pdf(m, s.f, bounce.wi, lsDir, lam[0]);
                             beam.scale(lsPdf / (lsPdf + bp));
                         }
@@ -808,13 +983,16 @@ solve_kernel(PtScene sc, PtFrame fr) {
                             ok = ms_sample_vertex(bounce.tri, bounce.b1, bounce.b2, s.pos, s, m, bounce.wi, lightDir, sg, ms) &&
                                  (ms.hits[0].lobe & kLobeReflection) != 0;
                         const V3 origin = !currentVertexMS ? s.pos : sv.origin;
-                        ok = ok && ms_nee_sample(sc, fr, lsOrigin, lsDir, lsDist, origin, beam.pos, maxMSBounces, sg, ms, stk);
+                        ok = ok && ms_nee_sample(sc, fr, lsOrigin, lsDir, lsDist, origin, beam.pos, beam.distant, lightDir,
+                                                 maxMSBounces, sg, ms, stk);
                         ok = ok && !(ms.pdf < 1e-6f);
                         if (ok && ms.count > 0) {
                             if (!(fr.options & kOptHwssMis) && ms.dispersive) beam.comps = 1;
-                            dirL = ms.woLast;
-                            beam.r = -ms.woLast;
-                            beam.t = perp_stark(beam.r);
+                            if (!beam.distant) {
+                                dirL = ms.woLast;
+                                beam.r = -ms.woLast;
+                                beam.t = perp_stark(beam.r);
+                            }
                             beam.scale(1.f / ms.pdf);
                             ms_forward_unwind<N>(sc, fr, ms, origin, beam);
                             if (currentVertexMS) {
@@ -822,7 +1000,7 @@ solve_kernel(PtScene sc, PtFrame fr) {
                                 misBsdfPdf = bounce.lastPdf;
                             }
                         }
-                        if (ok && doMIS) {
+                        if (ok && misL) {
                             if (ms.count == 0 || !currentVertexMS) misBsdfPdf = mat_pdf(m, s.f, bounce.wi, dirL, beam.lambda[0]);
                             beam.scale(lsPdf / (lsPdf + misBsdfPdf * ms.pdf));
                         }
@@ -854,7 +1032,8 @@ solve_kernel(PtScene sc, PtFrame fr) {
                 if (d2 > kFltMin) {
                     const V3 L = toL * (1.f / sqrtf(d2));
                     const float ct = dot(hn, -L);
-                    if (ct > 0.f) lightPdf = sc.light_pdf[lt] * (d2 / fmaxf(kFltMin, ct * sc.lights[4 * lt].w));
+                    // getEmissiveSelectionProbability() * emissiveSampler.evalPdf (pltpt_solve.rt.slang:172)
+                    if (ct > 0.f) lightPdf = pEmissive * (sc.light_pdf[lt] * (d2 / fmaxf(kFltMin, ct * sc.lights[4 * lt].w)));
                 }
             }
             const float mis = needMIS ? bounce.lastPdf / (bounce.lastPdf + lightPdf) : 1.f;
@@ -916,8 +1095,11 @@ int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfB
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
     const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);
-    if (fr.hwss == 4) hipLaunchKernelGGL(solve_kernel<4>, g, b, 0, st, sc, fr);
-    else if (fr.hwss == 1) hipLaunchKernelGGL(solve_kernel<1>, g, b, 0, st, sc, fr);
+    const bool analytic = (fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0;
+    if (fr.hwss == 4 && analytic) hipLaunchKernelGGL((solve_kernel<4, true>), g, b, 0, st, sc, fr);
+    else if (fr.hwss == 4) hipLaunchKernelGGL((solve_kernel<4, false>), g, b, 0, st, sc, fr);
+    else if (fr.hwss == 1 && analytic) hipLaunchKernelGGL((solve_kernel<1, true>), g, b, 0, st, sc, fr);
+    else if (fr.hwss == 1) hipLaunchKernelGGL((solve_kernel<1, false>), g, b, 0, st, sc, fr);
     else return set_error(PLT_ENOTIMPL, "HWSS sample counts other than 1 and 4 are not built");
     return check_launch("solve_kernel");
 }

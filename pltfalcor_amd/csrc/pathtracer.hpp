@@ -20,7 +20,7 @@ enum : uint32_t {
 
 enum : uint32_t {  // option bits
     kOptNEE = 0x1, kOptMIS = 0x2, kOptRR = 0x4, kOptHwssMis = 0x8, kOptISEmitters = 0x10, kOptDirectLights = 0x20,
-    kOptEmissiveLights = 0x40, kOptMNEE = 0x80, kOptUniformEmissive = 0x100
+    kOptEmissiveLights = 0x40, kOptMNEE = 0x80, kOptUniformEmissive = 0x100, kOptAnalyticLights = 0x200
 };
 
 // Largest number of transmissive occluders a manifold walk may pass (MNEE_MAX_MS_OCCLUDERS).
@@ -48,6 +48,9 @@ struct PtScene {
     // lights with non-zero flux, for the uniform emissive sampler
     const uint32_t* active_lights;
     uint32_t n_active;
+    // analytic lights (point, directional, distant, rect, disc, sphere)
+    const plt_light* analytic;
+    uint32_t n_analytic;
 };
 
 struct PtFrame {

@@ -167,6 +167,17 @@ int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out)
     return PLT_OK;
 }
 
+int plt_scene_set_lights(plt_scene* scene, const plt_light* lights, uint32_t count) {
+    if (!scene || (count && !lights)) return set_error(PLT_EINVAL, "plt_scene_set_lights: bad arguments");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (lights[i].type > PLT_LIGHT_SPHERE) return set_error(PLT_EINVAL, "plt_scene_set_lights: unknown light type");
+        if (lights[i].intensity_profile >= scene->host.profiles.size())
+            return set_error(PLT_EINVAL, "plt_scene_set_lights: intensity profile id out of range");
+    }
+    scene->host.lights.assign(lights, lights + count);
+    return PLT_OK;
+}
+
 int plt_scene_destroy(plt_scene* scene) {
     if (!scene) return PLT_OK;
     plt::free_scene(scene->dev);

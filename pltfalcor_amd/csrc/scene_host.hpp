@@ -53,6 +53,7 @@ struct HostScene {
     std::vector<plt_spectral_profile> profiles;
     plt_camera camera{};
     HostBvh bvh;
+    std::vector<plt_light> lights;  // analytic lights
 };
 
 // Procedural workload scene (plt_procedural_scene_desc).

@@ -31,6 +31,7 @@ class Scene:
             _lib.check(self.lib.plt_scene_create_procedural(ctx.handle if ctx is not None else None, C.byref(d),
                                                             C.byref(_handle)))
         self.handle = _handle
+        self.lights = b""
         a = SceneArrays()
         _lib.check(self.lib.plt_scene_get_arrays(self.handle, C.byref(a)))
         self.arrays = a
@@ -59,6 +60,16 @@ class Scene:
         _lib.check(lib.plt_scene_create(ctx.handle if ctx is not None else None, C.byref(d), C.byref(h)))
         return cls(ctx, _handle=h)
 
+    def set_lights(self, lights):
+        """Analytic lights (pltfalcor_amd.lights objects or packed plt_light bytes); bind the scene to
+        a pass afterwards (the pass reads them then)."""
+        from .lights import pack
+        data = lights if isinstance(lights, (bytes, bytearray)) else pack(lights)
+        n = len(data) // C.sizeof(_lib.Light)
+        buf = C.create_string_buffer(bytes(data), len(data)) if n else None
+        _lib.check(self.lib.plt_scene_set_lights(self.handle, buf, n))
+        self.lights = bytes(data)
+
     def numpy(self):
         """Host copies of the scene arrays."""
         a = self.arrays
@@ -73,6 +84,7 @@ class Scene:
             materials=C.string_at(a.materials, a.n_materials * C.sizeof(Material)),
             profiles=C.string_at(a.profiles, a.n_profiles * C.sizeof(_lib.SpectralProfile)),
             n_profiles=a.n_profiles,
+            lights=self.lights,
             camera=np.array(list(a.camera.pos) + list(a.camera.target) + list(a.camera.up) +
                             [a.camera.fov_y, a.camera.aspect, a.camera.focal_distance] + list(a.camera.jitter),
                             dtype=np.float32),

@@ -101,6 +101,7 @@ class SceneBuilder:
         self.profiles: list[_lib.SpectralProfile] = []
         self.materials: list[_lib.Material] = []
         self.meshes: list[_Mesh] = []
+        self.lights = []
         self.camera = _lib.Camera((0, 0, 1), (0, 0, 0), (0, 1, 0), math.radians(45.0), 1.0, 1.0, (0, 0))
 
     # --- spectral profiles (SceneBuilder.cpp:911-1000)
@@ -143,6 +144,11 @@ class SceneBuilder:
         self.meshes.append(_Mesh(pos, nrm, idx, int(materialID)))
         return len(self.meshes) - 1
 
+    def addLight(self, light) -> int:
+        """An analytic light (pltfalcor_amd.lights): Scene::addLight."""
+        self.lights.append(light)
+        return len(self.lights) - 1
+
     def addCamera(self, position, target, up, fovY: float | None = None, focalLength: float | None = None,
                   frameHeight: float = 24.0, aspect: float = 1.0, focalDistance: float = 1.0):
         """Pinhole camera: vertical field of view in radians, or a focal length in mm
@@ -165,8 +171,11 @@ class SceneBuilder:
         mesh_material = np.array([m.material for m in self.meshes], np.uint32)
         mats = M.pack(self.materials)
         profs = b"".join(bytes(p) for p in self.profiles)
-        return Scene.from_arrays(ctx, np.concatenate(pos), np.concatenate(nrm), np.concatenate(idx),
-                                 np.concatenate(tri_mesh), mesh_material, mats, profs, self.camera)
+        scene = Scene.from_arrays(ctx, np.concatenate(pos), np.concatenate(nrm), np.concatenate(idx),
+                                  np.concatenate(tri_mesh), mesh_material, mats, profs, self.camera)
+        if self.lights:
+            scene.set_lights(self.lights)
+        return scene
 
 
 def load_spd(path: str):

@@ -166,11 +166,17 @@ class OracleScene:
         self.handle = l.oracle_scene_create(ptr(a["pos"], C.c_float), ptr(a["nrm"], C.c_float), a["idx"].shape[0],
                                             ptr(a["idx"], C.c_uint32), ptr(a["tm"], C.c_uint32), ptr(a["mm"], C.c_uint32),
                                             a["mats"], a["prof"], scene["n_profiles"])
+        lights = scene.get("lights", b"")
+        if lights:
+            l.oracle_scene_set_lights.restype = C.c_int
+            l.oracle_scene_set_lights.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
+            self._lights = bytes(lights)
+            l.oracle_scene_set_lights(self.handle, self._lights, len(self._lights) // 192)
 
     def render(self, width, height, frame=0, max_bounces=6, first=0, count=None, do_nee=1, do_mis=1, do_rr=1,
                hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0, max_omega=0.025, threads=0,
                records=False, camera=None, do_mnee=0, mnee_max_occluders=2, mnee_max_iterations=60,
-               mnee_threshold=5e-5, emissive_sampler=2):
+               mnee_threshold=5e-5, emissive_sampler=2, analytic=1):
         """Renders pixels [first, first+count) (row-major). Returns color [count, 4] and optionally the
         bounce records [(max_bounces+1), 9, count] (uint32)."""
         if count is None:
@@ -179,8 +185,8 @@ class OracleScene:
         if emissive_sampler not in (0, 2):
             raise ValueError("emissive_sampler: 0 (Uniform) or 2 (Power)")
         iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct,
-                         emissive, do_mnee, mnee_max_occluders, mnee_max_iterations, int(emissive_sampler == 0)],
-                        np.int32)
+                         emissive, do_mnee, mnee_max_occluders, mnee_max_iterations, int(emissive_sampler == 0),
+                         analytic], np.int32)
         fopt = np.array([sourcing_area, max_omega, mnee_threshold], np.float32)
         color = np.zeros((count, 4), np.float32)
         rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
