@@ -315,6 +315,16 @@ typedef struct plt_light {
  * bound (plt_pass_set_scene). Returns PLT_EINVAL for a bad type or profile id. */
 int plt_scene_set_lights(plt_scene* scene, const plt_light* lights, uint32_t count);
 
+/* Environment map light (Scene/Lights/EnvMap.{h,cpp}, EnvMap.slang, EnvMapData.slang) with the
+ * importance sampling of Rendering/Lights/EnvMapSampler.{cpp,slang}: a lat-long RGBA32F image of
+ * width x height texels (host memory, row 0 at the +y pole, u = 0.5 towards -z; bilinear lookup
+ * wrapping in u and clamped in v), the local-to-world rotation (row-major 3x3, NULL = identity;
+ * the reference builds it from its rotation angles), and radiance scale intensity x tint (tint
+ * NULL = white). The 512 x 512 hierarchical importance map is built here. width or height 0
+ * removes the map. A pass reads it when the scene is bound (plt_pass_set_scene). */
+int plt_scene_set_env_map(plt_scene* scene, uint32_t width, uint32_t height, const float* rgba, const float* transform,
+                          float intensity, const float* tint);
+
 /* ---------------------------------------------------------------------------------------
  * Ray casting against a scene (device pointers). Rays: 8 planes of n floats (origin xyz,
  * direction xyz, tmin, tmax). Closest hit (mode 0, the reference's TraceRay with a

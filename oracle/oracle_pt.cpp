@@ -57,7 +57,15 @@ struct OScene {
         float M[16], MI[16];
     };
     std::vector<Light> analytic;
+    // environment map (EnvMap.slang) and its importance map mip chain (EnvMapSampler)
+    uint32_t envW = 0, envH = 0;
+    std::vector<float> envRGBA;
+    float envM[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, envMI[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, envScale[3] = {1, 1, 1};
+    std::vector<float> envImp;
+    std::vector<uint32_t> envMip;
+    uint32_t envBaseMip = 0;
 };
+constexpr uint32_t kEnvDim = 512;
 enum { LT_Point = 0, LT_Directional, LT_Distant, LT_Rect, LT_Disc, LT_Sphere };
 
 struct Options {
@@ -68,6 +76,7 @@ struct Options {
     float mneeThreshold = 5e-5f;
     int uniformEmissive = 0;  // emissiveSampler Uniform (else Power)
     int analyticLights = 1;   // useAnalyticLights
+    int envLight = 1;         // useEnvLights
 };
 constexpr int kMaxMSOccluders = 8;
 
@@ -245,6 +254,113 @@ bool sampleAnalytic(const OScene& s, const Options& o, const SD<F>& sd, float he
     bSize = distant ? 4.f * 3.14159265358979323846f * (1.f - L.cosSubtendedAngle * L.cosSubtendedAngle) : L.surfaceArea;
     return true;
 }
+
+// ---- environment map ----
+V3<F> mul3(const float* M, V3<F> d) {
+    return V3<F>{M[0] * d.x + M[1] * d.y + M[2] * d.z, M[3] * d.x + M[4] * d.y + M[5] * d.z, M[6] * d.x + M[7] * d.y + M[8] * d.z};
+}
+// world_to_latlong_map (MathHelpers.slang:106-113); acos argument clamped to [-1, 1]
+void latlong(V3<F> d, float& u, float& v) {
+    const float l = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+    const V3<F> p{d.x / l, d.y / l, d.z / l};
+    u = std::atan2(p.x, -p.z) * 0.159154943091895335769f + 0.5f;
+    v = std::acos(std::fmin(std::fmax(p.y, -1.f), 1.f)) * 0.318309886183790671538f;
+}
+// bilinear lookup, u wrapping, v clamped (EnvMap.cpp:123-126)
+void envTexel(const OScene& s, float u, float v, float rgb[3]) {
+    const float ss = u * float(s.envW) - 0.5f, t = v * float(s.envH) - 0.5f;
+    const float fs = std::floor(ss), ft = std::floor(t);
+    const float a = ss - fs, b = t - ft;
+    const int W = int(s.envW), H = int(s.envH);
+    int x0 = int(fs) % W;
+    if (x0 < 0) x0 += W;
+    const int x1 = (x0 + 1) % W;
+    const int y0 = std::min(std::max(int(ft), 0), H - 1), y1 = std::min(std::max(int(ft) + 1, 0), H - 1);
+    for (int c = 0; c < 3; ++c) {
+        const float c00 = s.envRGBA[4 * (size_t(y0) * W + x0) + c], c10 = s.envRGBA[4 * (size_t(y0) * W + x1) + c];
+        const float c01 = s.envRGBA[4 * (size_t(y1) * W + x0) + c], c11 = s.envRGBA[4 * (size_t(y1) * W + x1) + c];
+        const float top = c00 + (c10 - c00) * a, bot = c01 + (c11 - c01) * a;
+        rgb[c] = top + (bot - top) * b;
+    }
+}
+// EnvMap::eval(dir): radiance from world direction dir
+void envEval(const OScene& s, V3<F> dir, float rgb[3]) {
+    float u, v;
+    latlong(mul3(s.envMI, dir), u, v);
+    envTexel(s, u, v, rgb);
+    for (int c = 0; c < 3; ++c) rgb[c] *= s.envScale[c];
+}
+// oct_to_ndir_equal_area_unorm / ndir_to_oct_equal_area_unorm (MathHelpers.slang:199-243)
+V3<F> octUnormToDir(float u, float v) { return octToNdir(V2<F>{u * 2.f - 1.f, v * 2.f - 1.f}); }
+// EnvMapSamplerSetup.cs.slang + Texture::generateMips: 512^2 texels of mean luminance over 8x8
+// samples of the octahedral map, then 2x2 box levels down to 1x1
+void buildEnvImportance(OScene& s) {
+    const uint32_t spp = 8;
+    std::vector<float> level(size_t(kEnvDim) * kEnvDim);
+    const float inv = 1.f / float(spp * spp), dimS = float(kEnvDim * spp);
+    for (uint32_t y = 0; y < kEnvDim; ++y)
+        for (uint32_t x = 0; x < kEnvDim; ++x) {
+            float L = 0.f;
+            for (uint32_t sy = 0; sy < spp; ++sy)
+                for (uint32_t sx = 0; sx < spp; ++sx) {
+                    const float px = (float(x * spp + sx) + 0.5f) / dimS, py = (float(y * spp + sy) + 0.5f) / dimS;
+                    float u, v, rgb[3];
+                    latlong(octUnormToDir(px, py), u, v);
+                    envTexel(s, u, v, rgb);
+                    L += rgb[0] * 0.2126f + rgb[1] * 0.7152f + rgb[2] * 0.0722f;
+                }
+            level[size_t(y) * kEnvDim + x] = L * inv;
+        }
+    s.envImp.clear();
+    s.envMip.clear();
+    uint32_t d = kEnvDim;
+    for (;;) {
+        s.envMip.push_back(uint32_t(s.envImp.size()));
+        s.envImp.insert(s.envImp.end(), level.begin(), level.end());
+        if (d == 1) break;
+        const uint32_t h = d / 2;
+        std::vector<float> next(size_t(h) * h);
+        for (uint32_t y = 0; y < h; ++y)
+            for (uint32_t x = 0; x < h; ++x)
+                next[size_t(y) * h + x] = ((level[size_t(2 * y) * d + 2 * x] + level[size_t(2 * y) * d + 2 * x + 1]) +
+                                           (level[size_t(2 * y + 1) * d + 2 * x] + level[size_t(2 * y + 1) * d + 2 * x + 1])) * 0.25f;
+        level.swap(next);
+        d = h;
+    }
+    s.envBaseMip = uint32_t(s.envMip.size() - 1);
+}
+// EnvMapSampler.evalPdf (:150-163)
+float envPdf(const OScene& s, V3<F> dir) {
+    const V2<F> o = ndirToOct(mul3(s.envMI, dir));
+    const float u = o.x * 0.5f + 0.5f, v = o.y * 0.5f + 0.5f;
+    const int dim = int(kEnvDim);
+    const int x = std::min(std::max(int(std::floor(u * float(dim))), 0), dim - 1);
+    const int y = std::min(std::max(int(std::floor(v * float(dim))), 0), dim - 1);
+    return s.envImp[size_t(y) * dim + x] / s.envImp[s.envMip[s.envBaseMip]] * (1.f / 12.5663706143591729539f);
+}
+// EnvMapSampler.sample (:56-148)
+void envSample(const OScene& s, float px, float py, V3<F>& dir, float& pdf) {
+    uint32_t x = 0, y = 0;
+    for (int mip = int(s.envBaseMip) - 1; mip >= 0; --mip) {
+        x *= 2; y *= 2;
+        const uint32_t dim = kEnvDim >> mip;
+        const float* L = s.envImp.data() + s.envMip[mip];
+        const float w0 = L[size_t(y) * dim + x], w1 = L[size_t(y) * dim + x + 1];
+        const float w2 = L[size_t(y + 1) * dim + x], w3 = L[size_t(y + 1) * dim + x + 1];
+        const float q0 = w0 + w2, q1 = w1 + w3;
+        const float dd = q0 / (q0 + q1);
+        uint32_t ox, oy;
+        if (px < dd) { ox = 0; px = px / dd; } else { ox = 1; px = (px - dd) / (1.f - dd); }
+        const float e = ox == 0 ? (w0 / q0) : (w1 / q1);
+        if (py < e) { oy = 0; py = py / e; } else { oy = 1; py = (py - e) / (1.f - e); }
+        x += ox; y += oy;
+    }
+    const float inv = 1.f / float(kEnvDim);
+    const V3<F> dl = octUnormToDir((float(x) + px) * inv, (float(y) + py) * inv);
+    pdf = s.envImp[size_t(y) * kEnvDim + x] / s.envImp[s.envMip[s.envBaseMip]] * 0.0795774715459476678844f;
+    dir = mul3(s.envM, dl);
+}
+void hwssLambdas(float hero, float* lam) { Spectrum<float>::hwssWavelengths(hero, HW, lam); }
 
 // pltSourceBeamArea (PLT.slang:275-290)
 Beam sourceBeamArea(V3<F> posW, V3<F> dir, float A, float dist, const float* Le, const float* lam, float maxOmega, int comps) {
@@ -529,16 +645,38 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
     const bool useEmissive = o.emissiveLights && !s.activeLights.empty();
     const std::vector<float>& selPdfs = o.uniformEmissive ? s.lightPdfUniform : s.lightPdf;
     const bool useAnalytic = o.analyticLights && !s.analytic.empty();
+    const bool useEnv = o.envLight && s.envW > 0;
     // getLightTypeSelectionProbabilities (LightSampling.slangh:37-55)
-    float pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
+    float pEnv = useEnv ? 1.f : 0.f, pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
     {
-        const float sum = pEmissive + pAnalytic;
-        if (sum != 0.f) { const float inv = 1.f / sum; pEmissive *= inv; pAnalytic *= inv; }
+        const float sum = pEnv + pEmissive + pAnalytic;
+        if (sum != 0.f) { const float inv = 1.f / sum; pEnv *= inv; pEmissive *= inv; pAnalytic *= inv; }
     }
     const bool doNEE = o.doNEE != 0, doMIS = doNEE && o.doMIS;
     for (int p = 0;; ++p) {
         const Bounce& bounce = rec[size_t(p)];
-        if (bounce.flags & RF_Miss) break;
+        if (bounce.flags & RF_Miss) {
+            // onMiss (pltpt_solve.rt.slang:371-396)
+            if (useEnv) {
+                const bool didNEE = doNEE && p > 0 &&
+                                    (!(bounce.flags & RF_LastDelta) || (o.doMNEE && int(bounce.flags >> 16) <= o.mneeMaxOccluders && p > 1));
+                if (doMIS || !didNEE) {
+                    const bool needMIS = doMIS && didNEE;
+                    const V3<F> wo = bounce.wi();
+                    const float lightPdf = pEnv * envPdf(s, -wo);
+                    const float mis = needMIS ? bounce.lastNonDeltaPdf / (bounce.lastNonDeltaPdf + lightPdf) : 1.f;
+                    float lam[HW], Le[HW], rgb[3];
+                    hwssLambdas(sv.wavelength, lam);
+                    envEval(s, -wo, rgb);
+                    for (int j = 0; j < HW; ++j) Le[j] = mis * Spectrum<float>::rgbToSpectrum(V3<F>{rgb[0], rgb[1], rgb[2]}, lam[j]);
+                    const int comps = (!o.hwssMIS && (bounce.flags & RF_Dispersive)) ? 1 : HW;
+                    Beam beam = sourceBeamDistant(wo, o.maxOmega, Le, lam, o.maxOmega, comps);
+                    forwardUnwind(s, o, rec, p, beam);
+                    measure(s, o, sv, p, bounce.pathContribution, beam);
+                }
+            }
+            break;
+        }
         const V3<F> bwi = bounce.wi();
         const SD<F> sd = shadingData(s, bounce.tri, bounce.b1, bounce.b2, bwi);
         const Instance<F> bsdf = instanceAt(s, sd.materialID);
@@ -554,11 +692,16 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                 float u = sampleNext1D<float>(sg);
                 int ltype = -1;
                 float typePdf = 0.f;
-                if (u < pEmissive) {
-                    ltype = 1; typePdf = pEmissive;
+                if (u < pEnv) {
+                    ltype = 0; typePdf = pEnv;
                 } else {
-                    u -= pEmissive;
-                    if (u < pAnalytic) { ltype = 2; typePdf = pAnalytic; }
+                    u -= pEnv;
+                    if (u < pEmissive) {
+                        ltype = 1; typePdf = pEmissive;
+                    } else {
+                        u -= pEmissive;
+                        if (u < pAnalytic) { ltype = 2; typePdf = pAnalytic; }
+                    }
                 }
                 bool valid = false;
                 float lsPdf = 0, lsDist = 0;
@@ -621,6 +764,26 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
                         valid = false;
                         for (int j = 0; j < HW; ++j) valid |= Le[j] > 0.f;
                     }
+                }
+                if (ltype == 0) {
+                    // generateEnvMapSample (LightSampling.slangh:87-104)
+                    const float u0 = sampleNext1D<float>(sg), u1 = sampleNext1D<float>(sg);
+                    float epdf, rgb[3];
+                    envSample(s, u0, u1, lsDir, epdf);
+                    hwssLambdas(sv.wavelength, lam);
+                    envEval(s, lsDir, rgb);
+                    bool anyL = false;
+                    for (int j = 0; j < HW; ++j) {
+                        const float le = Spectrum<float>::rgbToSpectrum(V3<F>{rgb[0], rgb[1], rgb[2]}, lam[j]);
+                        Le[j] = epdf > 0.f ? le / epdf : 0.f;
+                        anyL |= Le[j] > 0.f;
+                    }
+                    lsPdf = epdf;
+                    lsOrigin = rayOrigin(sd.posW, dot(sd.faceN, lsDir) >= 0.f ? sd.faceN : -sd.faceN);
+                    lsDist = std::numeric_limits<float>::max();
+                    bDistant = true;
+                    bSize = o.maxOmega;
+                    valid = anyL;
                 }
                 if (ltype == 2) {
                     valid = sampleAnalytic(s, o, sd, sv.wavelength, sg, lsDir, lsOrigin, lsDist, lsPdf, bPos, bSize, bDistant, Le, lam);
@@ -882,6 +1045,20 @@ int oracle_scene_set_lights(void* h, const float* lights, uint32_t count) {
     return 0;
 }
 
+// Environment map of a scene made by oracle_scene_create: RGBA32F lat-long image, local-to-world
+// rotation (row-major 3x3), radiance scale; the importance map is built here.
+int oracle_scene_set_env(void* h, uint32_t w, uint32_t hgt, const float* rgba, const float* M, const float* scale) {
+    OScene& s = *static_cast<OScene*>(h);
+    s.envW = w; s.envH = hgt;
+    s.envRGBA.assign(rgba, rgba + size_t(w) * hgt * 4);
+    for (int i = 0; i < 9; ++i) s.envM[i] = M[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) s.envMI[3 * i + j] = M[3 * j + i];
+    for (int c = 0; c < 3; ++c) s.envScale[c] = scale[c];
+    buildEnvImportance(s);
+    return 0;
+}
+
 // Renders with a scene made by oracle_scene_create (whose arrays the caller keeps alive).
 int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count,
                         float* color, uint32_t* records, int threads) {
@@ -910,6 +1087,7 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
     o.mneeThreshold = fopt[2];
     o.uniformEmissive = iopt[14];
     o.analyticLights = iopt[15];
+    o.envLight = iopt[16];
     if (threads > 0) omp_set_num_threads(threads);
     const size_t n = count;
 #pragma omp parallel for schedule(dynamic, 16)

@@ -127,6 +127,8 @@ SIGNATURES = [
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
     ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("plt_scene_set_lights", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    ("plt_scene_set_env_map", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_float,
+                                        C.c_void_p]),
     ("plt_pass_create", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("plt_pass_destroy", C.c_int, [C.c_void_p]),
     ("plt_pass_set_option", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),

@@ -156,6 +156,8 @@ struct plt_pass {
     uint32_t n_active = 0;
     plt_light* d_analytic = nullptr;
     uint32_t n_analytic = 0;
+    float* d_env_rgba = nullptr;
+    float* d_env_imp = nullptr;
     uint32_t* d_records = nullptr;
     size_t records_words = 0;
     size_t last_words = 0;  // record words written by the last execute
@@ -189,6 +191,8 @@ void free_scene_state(plt_pass* p) {
     p->n_active = 0;
     p->d_analytic = nullptr;
     p->n_analytic = 0;
+    p->d_env_rgba = nullptr;
+    p->d_env_imp = nullptr;
 }
 // Camera frame (Camera.cpp:187-193).
 void camera_frame(const plt_camera& c, float out[12]) {
@@ -285,6 +289,8 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     p->n_active = uint32_t(lt.active.size());
     if ((rc = plt::upload_vec(scene->host.lights, &p->d_analytic, p->owned)) < 0) return rc;
     p->n_analytic = uint32_t(scene->host.lights.size());
+    if ((rc = plt::upload_vec(scene->host.env.rgba, &p->d_env_rgba, p->owned)) < 0) return rc;
+    if ((rc = plt::upload_vec(scene->host.env.importance, &p->d_env_imp, p->owned)) < 0) return rc;
     // Scene::useEmissiveLights requires an active triangle; otherwise emissive lighting is off
     p->n_lights = p->n_active > 0 ? uint32_t(lt.pdf.size()) : 0;
     plt::camera_frame(scene->host.camera, p->cam);
@@ -381,6 +387,20 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     sc.n_active = p->n_active;
     sc.analytic = p->d_analytic;
     sc.n_analytic = p->n_analytic;
+    {
+        const plt::HostEnvMap& e = s->host.env;
+        sc.env_rgba = reinterpret_cast<const float4*>(p->d_env_rgba);
+        sc.env_w = p->d_env_rgba ? e.width : 0;
+        sc.env_h = e.height;
+        for (int i = 0; i < 9; ++i) {
+            sc.env_m[i] = e.transform[i];
+            sc.env_mi[i] = e.inv_transform[i];
+        }
+        for (int c = 0; c < 3; ++c) sc.env_scale[c] = e.scale[c];
+        sc.env_imp = p->d_env_imp;
+        sc.env_base_mip = e.base_mip;
+        for (size_t k = 0; k < 12; ++k) sc.env_mip[k] = k < e.mip_offset.size() ? e.mip_offset[k] : 0u;
+    }
     sc.tri_light = p->d_tri_light;
     sc.n_lights = p->n_lights;
 
@@ -401,6 +421,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (o("doMNEE") != 0) opts |= plt::kOptMNEE;
     if (emissiveSampler == 0) opts |= plt::kOptUniformEmissive;
     if (o("useAnalyticLights") != 0) opts |= plt::kOptAnalyticLights;
+    if (o("useEnvLights") != 0) opts |= plt::kOptEnvLight;
     fr.options = opts;
     fr.mnee_max_occluders = uint32_t(mneeOcc);
     fr.mnee_max_iterations = uint32_t(mneeIt);

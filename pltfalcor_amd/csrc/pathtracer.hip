@@ -197,6 +197,102 @@ PLT_DEV Beam<N> source_beam_distant(V3 dir, float solidAngle, const float* Le, c
     return b;
 }
 
+// ---- environment map (Scene/Lights/EnvMap.slang, Rendering/Lights/EnvMapSampler.slang) ----
+// Radiance arriving from world direction `dir`: world_to_latlong_map of the local direction, a
+// bilinear lookup wrapping in u and clamped in v, times intensity x tint.
+PLT_DEV void env_eval(const PtScene& sc, V3 dir, float rgb[3]) {
+    const float* M = sc.env_mi;
+    const V3 d = v3(M[0] * dir.x + M[1] * dir.y + M[2] * dir.z, M[3] * dir.x + M[4] * dir.y + M[5] * dir.z,
+                    M[6] * dir.x + M[7] * dir.y + M[8] * dir.z);
+    const float l = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    const V3 p = v3(d.x / l, d.y / l, d.z / l);
+    const float u = atan2f(p.x, -p.z) * 0.159154943091895335769f + 0.5f;
+    const float v = acosf(fminf(fmaxf(p.y, -1.f), 1.f)) * 0.318309886183790671538f;
+    const float s = u * float(sc.env_w) - 0.5f, t = v * float(sc.env_h) - 0.5f;
+    const float fs = floorf(s), ft = floorf(t);
+    const float a = s - fs, b = t - ft;
+    const int W = int(sc.env_w), H = int(sc.env_h);
+    int x0 = int(fs) % W;
+    if (x0 < 0) x0 += W;
+    const int x1 = (x0 + 1) % W;
+    const int y0 = min(max(int(ft), 0), H - 1), y1 = min(max(int(ft) + 1, 0), H - 1);
+    const float4 c00 = sc.env_rgba[size_t(y0) * W + x0], c10 = sc.env_rgba[size_t(y0) * W + x1];
+    const float4 c01 = sc.env_rgba[size_t(y1) * W + x0], c11 = sc.env_rgba[size_t(y1) * W + x1];
+    const float t0 = c00.x + (c10.x - c00.x) * a, b0 = c01.x + (c11.x - c01.x) * a;
+    const float t1 = c00.y + (c10.y - c00.y) * a, b1 = c01.y + (c11.y - c01.y) * a;
+    const float t2 = c00.z + (c10.z - c00.z) * a, b2 = c01.z + (c11.z - c01.z) * a;
+    rgb[0] = (t0 + (b0 - t0) * b) * sc.env_scale[0];
+    rgb[1] = (t1 + (b1 - t1) * b) * sc.env_scale[1];
+    rgb[2] = (t2 + (b2 - t2) * b) * sc.env_scale[2];
+}
+// EnvMapSampler.evalPdf (:150-163): the point-sampled importance of the direction's octahedral
+// texel over the map's mean, per 4 pi steradians.
+PLT_DEV float env_pdf(const PtScene& sc, V3 dir) {
+    const float* M = sc.env_mi;
+    const V3 d = v3(M[0] * dir.x + M[1] * dir.y + M[2] * dir.z, M[3] * dir.x + M[4] * dir.y + M[5] * dir.z,
+                    M[6] * dir.x + M[7] * dir.y + M[8] * dir.z);
+    float ou, ov;
+    oct_encode(d, ou, ov);
+    const float u = ou * 0.5f + 0.5f, v = ov * 0.5f + 0.5f;
+    const int dim = int(kEnvImportanceDim);
+    const int x = min(max(int(floorf(u * float(dim))), 0), dim - 1), y = min(max(int(floorf(v * float(dim))), 0), dim - 1);
+    const float avg = sc.env_imp[sc.env_mip[sc.env_base_mip]];
+    return sc.env_imp[size_t(y) * dim + x] / avg * (1.f / 12.5663706143591729539f);
+}
+// EnvMapSampler.sample (:56-148): descend the importance mip chain choosing a child of each 2x2
+// block in proportion to its weight, warping the sample; then the octahedral direction of the
+// sample's position and its pdf.
+PLT_DEV void env_sample(const PtScene& sc, float px, float py, V3& dir, float& pdf) {
+    uint32_t x = 0, y = 0;
+    for (int mip = int(sc.env_base_mip) - 1; mip >= 0; --mip) {
+        x *= 2;
+        y *= 2;
+        const uint32_t dim = kEnvImportanceDim >> mip;
+        const float* L = sc.env_imp + sc.env_mip[mip];
+        const float w0 = L[size_t(y) * dim + x], w1 = L[size_t(y) * dim + x + 1];
+        const float w2 = L[size_t(y + 1) * dim + x], w3 = L[size_t(y + 1) * dim + x + 1];
+        const float q0 = w0 + w2, q1 = w1 + w3;
+        const float dd = q0 / (q0 + q1);
+        uint32_t ox, oy;
+        if (px < dd) {
+            ox = 0;
+            px = px / dd;
+        } else {
+            ox = 1;
+            px = (px - dd) / (1.f - dd);
+        }
+        const float e = ox == 0 ? (w0 / q0) : (w1 / q1);
+        if (py < e) {
+            oy = 0;
+            py = py / e;
+        } else {
+            oy = 1;
+            py = (py - e) / (1.f - e);
+        }
+        x += ox;
+        y += oy;
+    }
+    const float inv = 1.f / float(kEnvImportanceDim);
+    const float u = (float(x) + px) * inv, v = (float(y) + py) * inv;
+    const V3 dl = oct_decode(u * 2.f - 1.f, v * 2.f - 1.f);
+    const float avg = sc.env_imp[sc.env_mip[sc.env_base_mip]];
+    pdf = sc.env_imp[size_t(y) * kEnvImportanceDim + x] / avg * 0.0795774715459476678844f;
+    const float* M = sc.env_m;
+    dir = v3(M[0] * dl.x + M[1] * dl.y + M[2] * dl.z, M[3] * dl.x + M[4] * dl.y + M[5] * dl.z,
+             M[6] * dl.x + M[7] * dl.y + M[8] * dl.z);
+}
+// SpectrumSampler.hwssWavelengths: N equally spaced wavelengths from the hero, wrapping.
+template <int N>
+PLT_DEV void hwss_lambdas(float hero, float* lam) {
+    const float span = kMaxWavelength - kMinWavelength;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        float h = hero + float(j) * (span / float(N));
+        h -= h > kMaxWavelength ? span : 0.f;
+        lam[j] = h;
+    }
+}
+
 // An analytic light sample (generateAnalyticLightSample, LightSampling.slangh:122-163, over
 // sampleLight, Rendering/Lights/LightHelpers.slang:70-290): uniform choice of a light, its sample,
 // radiance scaled by the light count and pdf divided by it. Point and directional lights sample
@@ -788,9 +884,9 @@ PLT_DEV void ms_forward_unwind(const PtScene& sc, const PtFrame& fr, const MsDat
 // over 256 registers at its peak, which allows one wave per SIMD; capping it at 128 registers
 // (spilling the rest to scratch) gives the traversals four waves to hide latency behind. At
 // 1080p depth 8 with manifold NEE: 18.0 ms per frame at one wave, 11.3 ms at two, 10.5 ms at
-// three, 10.1 ms at four. `Analytic` instantiates the analytic-light code only for scenes that
-// have analytic lights, so scenes lit by emissive geometry alone do not carry its registers.
-template <int N, bool Analytic>
+// three, 10.1 ms at four. `Lights` instantiates the analytic-light and environment-map code only
+// for scenes that have them, so scenes lit by emissive geometry alone do not carry its registers.
+template <int N, bool Lights>
 __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 solve_kernel(PtScene sc, PtFrame fr) {
     __shared__ int stack[kTraversalStackDepth * kPtBlock];
@@ -802,16 +898,18 @@ solve_kernel(PtScene sc, PtFrame fr) {
     int* stk = stack + threadIdx.x;
     const bool doNEE = (fr.options & kOptNEE) != 0, doMIS = doNEE && (fr.options & kOptMIS) != 0;
     const bool useEmissive = (fr.options & kOptEmissiveLights) != 0 && sc.n_lights > 0;
-    const bool useAnalytic = Analytic && (fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0;
+    const bool useAnalytic = Lights && (fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0;
+    const bool useEnv = Lights && (fr.options & kOptEnvLight) != 0 && sc.env_w > 0;
     const bool importance = (fr.options & kOptISEmitters) != 0;
     const bool doMNEE = doNEE && (fr.options & kOptMNEE) != 0;
     // getLightTypeSelectionProbabilities (LightSampling.slangh:37-55): equal weights over the
-    // present light types (no environment map here)
-    float pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
+    // present light types
+    float pEnv = useEnv ? 1.f : 0.f, pEmissive = useEmissive ? 1.f : 0.f, pAnalytic = useAnalytic ? 1.f : 0.f;
     {
-        const float psum = pEmissive + pAnalytic;
+        const float psum = pEnv + pEmissive + pAnalytic;
         if (psum != 0.f) {
             const float inv = 1.f / psum;
+            pEnv *= inv;
             pEmissive *= inv;
             pAnalytic *= inv;
         }
@@ -826,7 +924,31 @@ solve_kernel(PtScene sc, PtFrame fr) {
 
     for (int p = 0;; ++p) {
         const Rec bounce = read_rec(fr.records, n, li, p);
-        if (bounce.flags & kFlagMiss) break;  // onMiss: no environment light in these scenes
+        if (bounce.flags & kFlagMiss) {
+            // onMiss (pltpt_solve.rt.slang:371-396): the environment map, MIS-weighted against its
+            // own sampling when NEE could have found it
+            if (Lights && useEnv) {
+                const bool didNEE = doNEE && p > 0 &&
+                                    (!(bounce.flags & kFlagLastDelta) ||
+                                     (doMNEE && (bounce.flags >> 16) <= fr.mnee_max_occluders && p > 1));
+                if (doMIS || !didNEE) {
+                    const bool needMIS = doMIS && didNEE;
+                    const V3 wo = bounce.wi;
+                    const float lightPdf = pEnv * env_pdf(sc, -wo);
+                    const float mis = needMIS ? bounce.lastPdf / (bounce.lastPdf + lightPdf) : 1.f;
+                    float lam[N], Le[N], rgb[3];
+                    hwss_lambdas<N>(lambda, lam);
+                    env_eval(sc, -wo, rgb);
+#pragma unroll
+                    for (int j = 0; j < N; ++j) Le[j] = mis * rgb_to_spectrum(rgb[0], rgb[1], rgb[2], lam[j]);
+                    const int comps = (!(fr.options & kOptHwssMis) && (bounce.flags & kFlagDispersive)) ? 1 : N;
+                    Beam<N> beam = source_beam_distant<N>(wo, fr.max_omega, Le, lam, fr.max_omega, comps);
+                    forward_unwind<N>(sc, fr, n, li, p, beam);
+                    measure<N>(sc, fr, sv, p, bounce.contrib, beam);
+                }
+            }
+            break;
+        }
         const Surf s = surface_at(sc, bounce.tri, bounce.b1, bounce.b2, bounce.wi);
         const Mat m = material_at(sc, s.material);
         const uint32_t props = mat_props(m);
@@ -841,16 +963,22 @@ solve_kernel(PtScene sc, PtFrame fr) {
             if (sampleRefl || sampleTrans) {
                 // selectLightType (LightSampling.slangh:57-85)
                 float uType = sg.next1D();
-                int ltype = -1;  // SampledLightType: 1 emissive, 2 analytic
+                int ltype = -1;  // SampledLightType: 0 environment map, 1 emissive, 2 analytic
                 float typePdf = 0.f;
-                if (uType < pEmissive) {
-                    ltype = 1;
-                    typePdf = pEmissive;
+                if (uType < pEnv) {
+                    ltype = 0;
+                    typePdf = pEnv;
                 } else {
-                    uType -= pEmissive;
-                    if (uType < pAnalytic) {
-                        ltype = 2;
-                        typePdf = pAnalytic;
+                    uType -= pEnv;
+                    if (uType < pEmissive) {
+                        ltype = 1;
+                        typePdf = pEmissive;
+                    } else {
+                        uType -= pEmissive;
+                        if (uType < pAnalytic) {
+                            ltype = 2;
+                            typePdf = pAnalytic;
+                        }
                     }
                 }
                 bool valid = ltype == 1;
@@ -925,7 +1053,30 @@ solve_kernel(PtScene sc, PtFrame fr) {
                         valid = anyL;
                     }
                 }
-                if (Analytic && ltype == 2) {
+                if (Lights && ltype == 0) {
+                    // generateEnvMapSample (LightSampling.slangh:87-104): radiance over pdf, sourced
+                    // from infinity with the largest beam solid angle
+                    const float u0 = sg.next1D(), u1 = sg.next1D();
+                    float epdf, rgb[3];
+                    env_sample(sc, u0, u1, lsDir, epdf);
+                    hwss_lambdas<N>(lambda, lam);
+                    env_eval(sc, lsDir, rgb);
+                    bool anyL = false;
+#pragma unroll
+                    for (int j = 0; j < N; ++j) {
+                        const float le = rgb_to_spectrum(rgb[0], rgb[1], rgb[2], lam[j]);
+                        Le[j] = epdf > 0.f ? le / epdf : 0.f;
+                        anyL |= Le[j] > 0.f;
+                    }
+                    lsPdf = epdf;
+                    const float side = s.faceN.x * lsDir.x + s.faceN.y * lsDir.y + s.faceN.z * lsDir.z;
+                    lsOrigin = ray_origin(s.pos, side >= 0.f ? s.faceN : -s.faceN);
+                    lsDist = kFltMax;
+                    bDistant = true;
+                    bSize = fr.max_omega;
+                    valid = anyL;
+                }
+                if (Lights && ltype == 2) {
                     AnalyticSample as;
                     valid = sample_analytic<N>(sc, s, lambda, importance, sg, as, Le, lam);
                     if (valid) {
@@ -1095,7 +1246,8 @@ int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfB
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
     const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);
-    const bool analytic = (fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0;
+    const bool analytic = ((fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0) ||
+                          ((fr.options & kOptEnvLight) != 0 && sc.env_w > 0);
     if (fr.hwss == 4 && analytic) hipLaunchKernelGGL((solve_kernel<4, true>), g, b, 0, st, sc, fr);
     else if (fr.hwss == 4) hipLaunchKernelGGL((solve_kernel<4, false>), g, b, 0, st, sc, fr);
     else if (fr.hwss == 1 && analytic) hipLaunchKernelGGL((solve_kernel<1, true>), g, b, 0, st, sc, fr);

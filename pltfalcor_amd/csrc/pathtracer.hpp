@@ -20,7 +20,8 @@ enum : uint32_t {
 
 enum : uint32_t {  // option bits
     kOptNEE = 0x1, kOptMIS = 0x2, kOptRR = 0x4, kOptHwssMis = 0x8, kOptISEmitters = 0x10, kOptDirectLights = 0x20,
-    kOptEmissiveLights = 0x40, kOptMNEE = 0x80, kOptUniformEmissive = 0x100, kOptAnalyticLights = 0x200
+    kOptEmissiveLights = 0x40, kOptMNEE = 0x80, kOptUniformEmissive = 0x100, kOptAnalyticLights = 0x200,
+    kOptEnvLight = 0x400
 };
 
 // Largest number of transmissive occluders a manifold walk may pass (MNEE_MAX_MS_OCCLUDERS).
@@ -51,6 +52,14 @@ struct PtScene {
     // analytic lights (point, directional, distant, rect, disc, sphere)
     const plt_light* analytic;
     uint32_t n_analytic;
+    // environment map: RGBA32F lat-long image, rotation and inverse (row-major 3x3), radiance
+    // scale, importance map mip chain (level k at env_mip[k], (512 >> k)^2 texels)
+    const float4* env_rgba;
+    uint32_t env_w, env_h;
+    float env_m[9], env_mi[9], env_scale[3];
+    const float* env_imp;
+    uint32_t env_base_mip;
+    uint32_t env_mip[12];
 };
 
 struct PtFrame {

@@ -42,6 +42,21 @@ struct HostBvh {
 // Builds a binned-SAH BVH over triangles (positions as 3 floats per vertex, indices 3 per tri).
 HostBvh build_bvh(const std::vector<float>& pos, const std::vector<uint32_t>& idx, int max_leaf = 4);
 
+// Environment map light: RGBA32F lat-long image, local-to-world rotation (row-major 3x3) and its
+// inverse, radiance scale (intensity x tint), and the importance map's mip chain (level k of
+// (512 >> k)^2 texels at mip_offset[k]).
+constexpr uint32_t kEnvImportanceDim = 512;
+struct HostEnvMap {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;
+    float transform[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float inv_transform[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float scale[3] = {1, 1, 1};
+    std::vector<float> importance;
+    std::vector<uint32_t> mip_offset;
+    uint32_t base_mip = 0;
+};
+
 struct HostScene {
     std::vector<float> positions;       // 3 per vertex
     std::vector<float> normals;         // 3 per vertex (binary16-rounded)
@@ -54,6 +69,7 @@ struct HostScene {
     plt_camera camera{};
     HostBvh bvh;
     std::vector<plt_light> lights;  // analytic lights
+    HostEnvMap env;                 // environment map (width 0: none)
 };
 
 // Procedural workload scene (plt_procedural_scene_desc).

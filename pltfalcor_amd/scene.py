@@ -32,6 +32,7 @@ class Scene:
                                                             C.byref(_handle)))
         self.handle = _handle
         self.lights = b""
+        self.env = None
         a = SceneArrays()
         _lib.check(self.lib.plt_scene_get_arrays(self.handle, C.byref(a)))
         self.arrays = a
@@ -70,6 +71,31 @@ class Scene:
         _lib.check(self.lib.plt_scene_set_lights(self.handle, buf, n))
         self.lights = bytes(data)
 
+    def set_env_map(self, image, rotation=(0.0, 0.0, 0.0), intensity: float = 1.0, tint=(1.0, 1.0, 1.0)):
+        """Environment map light (EnvMap): `image` a lat-long radiance image [height, width, 3 or 4]
+        (row 0 at the +y pole, u = 0.5 towards -z), rotated by the reference's XYZ Euler angles in
+        degrees (EnvMap::setRotation: eulerAngleXYZ = Rx Ry Rz), scaled by intensity x tint. None
+        removes it."""
+        if image is None:
+            _lib.check(self.lib.plt_scene_set_env_map(self.handle, 0, 0, None, None, 1.0, None))
+            self.env = None
+            return
+        img = np.asarray(image, np.float32)
+        h, w = img.shape[:2]
+        rgba = np.ones((h, w, 4), np.float32)
+        rgba[..., :3] = img[..., :3]
+        ax, ay, az = (np.radians(float(a)) for a in rotation)
+        rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+        ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+        rz = np.array([[np.cos(az), -np.sin(az), 0], [np.sin(az), np.cos(az), 0], [0, 0, 1]])
+        m = (rx @ ry @ rz).astype(np.float32)
+        t = np.asarray(tint, np.float32)
+        _lib.check(self.lib.plt_scene_set_env_map(self.handle, w, h, C.c_void_p(rgba.ctypes.data),
+                                                  C.c_void_p(m.ctypes.data), float(intensity),
+                                                  C.c_void_p(t.ctypes.data)))
+        scale = (np.float32(intensity) * t).astype(np.float32)
+        self.env = dict(width=w, height=h, rgba=rgba, transform=m.reshape(9), scale=scale)
+
     def numpy(self):
         """Host copies of the scene arrays."""
         a = self.arrays
@@ -85,6 +111,7 @@ class Scene:
             profiles=C.string_at(a.profiles, a.n_profiles * C.sizeof(_lib.SpectralProfile)),
             n_profiles=a.n_profiles,
             lights=self.lights,
+            env=self.env,
             camera=np.array(list(a.camera.pos) + list(a.camera.target) + list(a.camera.up) +
                             [a.camera.fov_y, a.camera.aspect, a.camera.focal_distance] + list(a.camera.jitter),
                             dtype=np.float32),

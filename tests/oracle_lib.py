@@ -172,11 +172,18 @@ class OracleScene:
             l.oracle_scene_set_lights.argtypes = [C.c_void_p, C.c_char_p, C.c_uint32]
             self._lights = bytes(lights)
             l.oracle_scene_set_lights(self.handle, self._lights, len(self._lights) // 192)
+        env = scene.get("env")
+        if env is not None:
+            l.oracle_scene_set_env.restype = C.c_int
+            l.oracle_scene_set_env.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, _F, _F, _F]
+            self._env = [np.ascontiguousarray(env[k], np.float32) for k in ("rgba", "transform", "scale")]
+            l.oracle_scene_set_env(self.handle, env["width"], env["height"], ptr(self._env[0], C.c_float),
+                                   ptr(self._env[1], C.c_float), ptr(self._env[2], C.c_float))
 
     def render(self, width, height, frame=0, max_bounces=6, first=0, count=None, do_nee=1, do_mis=1, do_rr=1,
                hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0, max_omega=0.025, threads=0,
                records=False, camera=None, do_mnee=0, mnee_max_occluders=2, mnee_max_iterations=60,
-               mnee_threshold=5e-5, emissive_sampler=2, analytic=1):
+               mnee_threshold=5e-5, emissive_sampler=2, analytic=1, env=1):
         """Renders pixels [first, first+count) (row-major). Returns color [count, 4] and optionally the
         bounce records [(max_bounces+1), 9, count] (uint32)."""
         if count is None:
@@ -186,7 +193,7 @@ class OracleScene:
             raise ValueError("emissive_sampler: 0 (Uniform) or 2 (Power)")
         iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct,
                          emissive, do_mnee, mnee_max_occluders, mnee_max_iterations, int(emissive_sampler == 0),
-                         analytic], np.int32)
+                         analytic, env], np.int32)
         fopt = np.array([sourcing_area, max_omega, mnee_threshold], np.float32)
         color = np.zeros((count, 4), np.float32)
         rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
