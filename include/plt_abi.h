@@ -385,8 +385,9 @@ typedef struct plt_render_data {
 } plt_render_data;
 
 /* Renders one frame (sample pass + solve pass) on `stream`, then increments the frame counter.
- * Options not built yet (doMNEE=1, NEEUsePerTileSG=1, non-power emissive sampler, debug views,
- * HWSS other than 1 or 4) return PLT_ENOTIMPL. */
+ * Options whose code is not built (NEEUsePerTileSG=1, emissiveSampler other than 0 Uniform or
+ * 2 Power, sampleGenerator other than 0, debug views, MNEEMaxOccluders above 8, HWSS other than
+ * 1 or 4) return PLT_ENOTIMPL. */
 int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream stream);
 
 /* Copies the bounce records of the last execute to host memory `dst` (synchronous). Layout:
