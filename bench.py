@@ -20,6 +20,10 @@ Each frame also runs the passes that follow PLTPT in the reference's render grap
 accumulated in FP64 and albedo and normal in FP32 (AccumulatePass), and the accumulated colour is
 tone-mapped with auto exposure (ToneMapper); `secondary.graph_tail` gives their per-launch times.
 
+`secondary.lights` renders the same workload with an importance-sampled environment map and rect,
+disc, sphere and distant analytic lights added (every light type of the reference's light
+sampling at once).
+
 `secondary.bsdf_eval_batch` is the 2^20-query BSDF eval + pdf batch (BSDF evals/s), the second
 part of the metric. The CPU baseline is the oracle restatement (C++, OpenMP over the host cores)
 rendering a bounded sample of the same frame.
@@ -255,6 +259,64 @@ def segment_count(pltpt, frames):
     return counts
 
 
+def lights_run(ctx, opts, steps: int):
+    """The default workload with the other light types of the reference's light sampling added: a
+    256 x 128 sky environment map (importance-sampled) and four analytic lights (rect, disc,
+    sphere, distant) beside the emissive quad. Returns path samples/s and per-frame kernel times."""
+    import numpy as np
+    import torch
+    from pltfalcor_amd.lights import DiscLight, DistantLight, RectLight, SphereLight
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import PATH_2, Scene
+    scene = Scene(ctx, aspect=WIDTH / HEIGHT, **PATH_2)
+    v = np.linspace(0, 1, 128, dtype=np.float32)[:, None, None]
+    sky = np.broadcast_to(np.concatenate([0.2 + 0.8 * (1 - v), 0.4 + 0.6 * (1 - v), np.ones_like(v)], 2), (128, 256, 3))
+    sky = np.array(sky)
+    sky[10:20, 100:120] = (40.0, 35.0, 30.0)  # a sun-like patch
+    scene.set_env_map(sky, rotation=(0.0, 20.0, 0.0), intensity=0.3)
+    emission = 2  # the scene's blackbody emission profile
+    rect = RectLight()
+    m = np.eye(4, dtype=np.float32)
+    m[:3, :3] = [[1, 0, 0], [0, 0, -1], [0, 1, 0]]
+    m[:3, 3] = (-4.0, 6.0, 3.0)
+    rect.transform = m
+    disc = DiscLight()
+    m2 = m.copy()
+    m2[:3, 3] = (4.0, 6.0, -3.0)
+    disc.transform = m2
+    sph = SphereLight()
+    m3 = np.eye(4, dtype=np.float32)
+    m3[:3, 3] = (0.0, 5.0, 5.0)
+    sph.transform = m3
+    sph.scaling = (0.5, 0.5, 0.5)
+    sun = DistantLight()
+    sun.direction = (0.4, -1.0, 0.3)
+    for light in (rect, disc, sph, sun):
+        light.setIntensity(emission)
+    scene.set_lights([rect, disc, sph, sun])
+    p = PLTPT(ctx, opts)
+    p.set_sample_pattern(PLTPT.HALTON, 16)
+    p.set_scene(scene)
+    color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device="cuda")
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    for _ in range(2):
+        p.execute(WIDTH, HEIGHT, color, albedo, normal)
+    torch.cuda.synchronize()
+    p.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(steps * SPP):
+        p.execute(WIDTH, HEIGHT, color, albedo, normal)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sample_ms, solve_ms, executes = p.kernel_times()
+    p.close()
+    scene.close()
+    return {"value": WIDTH * HEIGHT * steps * SPP / wall, "unit": "path samples/s",
+            "workload": "the default workload plus a 256x128 importance-sampled sky and rect, disc, sphere and "
+                        "distant lights (light type chosen per NEE sample among environment, emissive, analytic)",
+            "sample_kernel_ms": sample_ms / executes, "solve_kernel_ms": solve_ms / executes}
+
+
 def postfx_timing(acc_double, acc_single, tm, color, albedo, out, ldr, reps: int):
     """Per-launch time of the passes after PLTPT at 1080p, from HIP events on the stream they run
     on: FP64 accumulation (16 B in + 32 B sum read + 32 B sum written + 16 B out per pixel), FP32
@@ -418,6 +480,7 @@ def main():
     achieved = kbytes / (kms * 1e-3) / 1e9
 
     post = postfx_timing(acc_c, acc_a, tm, color, albedo, out_c, ldr, 50)
+    lights = None if args.no_secondary else lights_run(ctx, opts, 2)
     bsdf = None if args.no_secondary else bsdf_batch(500, 20)[1]
     sur = None if args.no_secondary else surrogate_run(1 << 22, 6, 1 << 22, 5)
     out = {
@@ -452,7 +515,7 @@ def main():
         "secondary": {"bsdf_eval_batch": {k: bsdf[k] for k in ("value", "unit", "roofline")},
                       "surrogate": {k: sur[k] for k in ("value", "unit", "exact_value", "surrogate_over_exact",
                                                         "error_vs_exact", "training", "roofline")},
-                      "graph_tail": post}
+                      "graph_tail": post, "lights": lights}
                      if bsdf else {"graph_tail": post},
     }
     if args.copy_calibration > 0:
