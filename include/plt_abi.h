@@ -84,6 +84,11 @@ int plt_spectral_profiles_from_material(const char* tables_dir, const char* name
 /* SceneBuilder::addSpectralProfileForEmitterType (SceneBuilder.cpp:984-1000): the emission profile
  * (with inverse CDF) of <tables_dir>/emission/<name>.spd scaled by `scale`; PLT_EINVAL if absent. */
 int plt_spectral_profile_from_emitter(const char* tables_dir, const char* name, float scale, plt_spectral_profile* out);
+/* The profile of a SampledSpectrum given directly (n values spaced uniformly over
+ * [lambda_start, lambda_end], no resampling): genSpectralProfile on it, as SceneBuilder does for
+ * spectra made from RGB (SpectrumUtils::rgbToSpectrum, Utils/Color/SpectrumUtils.h:154-215). */
+int plt_spectral_profile_build_sampled(uint32_t n, float lambda_start, float lambda_end, const float* values,
+                                       int compute_icdf, plt_spectral_profile* out);
 /* Uploads `count` host profiles to the device; profile ids are their indices. */
 int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profile* profiles, uint32_t count);
 

@@ -144,6 +144,8 @@ SIGNATURES = [
     ("plt_spectrum_load_spd", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                         C.POINTER(C.c_uint32)]),
     ("plt_spectral_profile_constant", C.c_int, [C.c_float, C.POINTER(SpectralProfile)]),
+    ("plt_spectral_profile_build_sampled", C.c_int, [C.c_uint32, C.c_float, C.c_float, C.POINTER(C.c_float), C.c_int,
+                                                     C.POINTER(SpectralProfile)]),
     ("plt_spectral_profiles_from_material", C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(SpectralProfile),
                                                       C.POINTER(SpectralProfile)]),
     ("plt_spectral_profile_from_emitter", C.c_int, [C.c_char_p, C.c_char_p, C.c_float, C.POINTER(SpectralProfile)]),

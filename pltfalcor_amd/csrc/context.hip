@@ -45,6 +45,55 @@ int check_launch(const char* kernel) {
 
 using plt::set_error;
 
+namespace {
+
+// genSpectralProfile (SceneBuilder.cpp:911-960) of a SampledSpectrum: `ns` samples spaced
+// uniformly over [lo, hi], evaluated by linear interpolation between them.
+int gen_profile(const float* smp, size_t ns, float lo, float hi, int compute_icdf, plt_spectral_profile* out) {
+    auto sampled = [&](float l) {
+        if (l < lo || l > hi) return 0.f;
+        const float x = ((l - lo) / (hi - lo)) * (float(ns) - 1.0f);
+        const size_t i = size_t(std::floor(x));
+        if (i + 1 >= ns) return smp[ns - 1];
+        return plt::mix(smp[i], smp[i + 1], x - float(i));
+    };
+    plt_spectral_profile sp;
+    std::memset(&sp, 0, sizeof(sp));
+    sp.intensity = 1.f;
+    sp.min_wavelength = std::max(lo, 380.f);
+    sp.max_wavelength = std::min(hi, 720.f);
+    sp.bins = std::min<uint32_t>(PLT_SPECTRAL_PROFILE_MAX_BINS,
+                                 uint32_t(float(ns) * (sp.max_wavelength - sp.min_wavelength) / (hi - lo) + .5f));
+    if (sp.bins > 0) {
+        const float r = sp.bins > 1 ? 1.f / float(sp.bins - 1) : 1.f;
+        float sum = 0.f;
+        for (uint32_t i = 0; i < sp.bins; ++i) {
+            sp.pdf[i] = sampled(plt::mix(sp.min_wavelength, sp.max_wavelength, float(i) * r));
+            sum += sp.pdf[i];
+        }
+        if (sum >= 1.192092896e-07f && compute_icdf) {
+            for (uint32_t i = 0; i < sp.bins; ++i) sp.pdf[i] /= sum;
+            sp.intensity = sum;
+            float cdfa = 0.f, cdfb = sp.pdf[0];
+            uint32_t pdfi = 0;
+            for (uint32_t ic = 0; ic < sp.bins; ++ic) {
+                const float cdf = float(ic) * r;
+                while (cdf >= cdfb && pdfi + 1 < sp.bins) {
+                    cdfa = cdfb;
+                    ++pdfi;
+                    cdfb += sp.pdf[pdfi];
+                }
+                const float l = std::min(1.f, (cdf - cdfa) / (cdfb - cdfa));
+                sp.icdf[ic] = plt::mix(float(std::max<uint32_t>(1, pdfi)) - 1.f, float(pdfi), l) * r;
+            }
+        }
+    }
+    *out = sp;
+    return PLT_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* plt_last_error(void) { return plt::g_last_error.c_str(); }
@@ -85,47 +134,14 @@ int plt_spectral_profile_build(uint32_t npts, const float* wavelengths, const fl
     const size_t ns = std::max<size_t>(1, size_t((hi - lo) / bin + .5f));
     std::vector<float> smp(ns);
     for (size_t i = 0; i < ns; ++i) smp[i] = tab.at(lo + (hi - lo) / float(ns - 1) * float(i));
-    auto sampled = [&](float l) {
-        if (l < lo || l > hi) return 0.f;
-        const float x = ((l - lo) / (hi - lo)) * (float(ns) - 1.0f);
-        const size_t i = size_t(std::floor(x));
-        if (i + 1 >= ns) return smp[ns - 1];
-        return plt::mix(smp[i], smp[i + 1], x - float(i));
-    };
+    return gen_profile(smp.data(), ns, lo, hi, compute_icdf, out);
+}
 
-    plt_spectral_profile sp;
-    std::memset(&sp, 0, sizeof(sp));
-    sp.intensity = 1.f;
-    sp.min_wavelength = std::max(lo, 380.f);
-    sp.max_wavelength = std::min(hi, 720.f);
-    sp.bins = std::min<uint32_t>(PLT_SPECTRAL_PROFILE_MAX_BINS,
-                                 uint32_t(float(ns) * (sp.max_wavelength - sp.min_wavelength) / (hi - lo) + .5f));
-    if (sp.bins > 0) {
-        const float r = sp.bins > 1 ? 1.f / float(sp.bins - 1) : 1.f;
-        float sum = 0.f;
-        for (uint32_t i = 0; i < sp.bins; ++i) {
-            sp.pdf[i] = sampled(plt::mix(sp.min_wavelength, sp.max_wavelength, float(i) * r));
-            sum += sp.pdf[i];
-        }
-        if (sum >= 1.192092896e-07f && compute_icdf) {
-            for (uint32_t i = 0; i < sp.bins; ++i) sp.pdf[i] /= sum;
-            sp.intensity = sum;
-            float cdfa = 0.f, cdfb = sp.pdf[0];
-            uint32_t pdfi = 0;
-            for (uint32_t ic = 0; ic < sp.bins; ++ic) {
-                const float cdf = float(ic) * r;
-                while (cdf >= cdfb && pdfi + 1 < sp.bins) {
-                    cdfa = cdfb;
-                    ++pdfi;
-                    cdfb += sp.pdf[pdfi];
-                }
-                const float l = std::min(1.f, (cdf - cdfa) / (cdfb - cdfa));
-                sp.icdf[ic] = plt::mix(float(std::max<uint32_t>(1, pdfi)) - 1.f, float(pdfi), l) * r;
-            }
-        }
-    }
-    *out = sp;
-    return PLT_OK;
+int plt_spectral_profile_build_sampled(uint32_t n, float lambda_start, float lambda_end, const float* values,
+                                       int compute_icdf, plt_spectral_profile* out) {
+    if (n < 1 || !values || !out || !(lambda_end > lambda_start))
+        return set_error(PLT_EINVAL, "plt_spectral_profile_build_sampled: bad arguments");
+    return gen_profile(values, n, lambda_start, lambda_end, compute_icdf, out);
 }
 
 int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profile* profiles, uint32_t count) {
