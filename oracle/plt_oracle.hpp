@@ -622,8 +622,14 @@ template <class T, int N> struct PLTBeam {  // PLT.slang:83-245
         posW = p;
     }
     V3<T> toLocal(V3<T> v) const { return {dot(v, t), dot(v, cross(t, r)), dot(v, r)}; }
-    // mutualCoherence(hwss k, v) (PLT.slang:165-170)
+    // mutualCoherence(hwss k, v) (PLT.slang:165-170); a beam that has not propagated (r = 0:
+    // distant sources) takes the limit, 0 at any nonzero offset and 1 at none (as written the
+    // expression is 0 * inf or inf - inf there).
     void mutualCoherence(const T* k, V3<T> v, T* out) const {
+        if (Theta.r_m == T(0)) {
+            for (int j = 0; j < N; ++j) out[j] = (v.x == T(0) && v.y == T(0)) ? T(1) : T(0);
+            return;
+        }
         const M2<T> invT = invTheta();
         const V2<T> vv = {v.x, v.y};
         const T q = dot(vv, mul(invT, vv));

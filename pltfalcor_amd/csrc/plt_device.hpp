@@ -374,8 +374,16 @@ template <int N> struct Beam {
         pos = p;
     }
     // Spatial mutual coherence per HWSS component for a transverse offset v (um)
-    // (PLT.slang:165-170): exp(-(k/2pi)^2 v^T (D/r^2) v / 2).
+    // (PLT.slang:165-170): exp(-(k/2pi)^2 v^T (D/r^2) v / 2). A beam that has not propagated
+    // (r = 0: distant sources) takes the expression's limit, 0 at any nonzero offset and 1 at none;
+    // evaluated as written it is 0 * inf or inf - inf there, NaN or 0 depending on rounding.
     PLT_DEV void mutual_coherence(const float* k, V3 v, float* out) const {
+        if (r_m == 0.f) {
+            const float c = (v.x == 0.f && v.y == 0.f) ? 1.f : 0.f;
+#pragma unroll
+            for (int j = 0; j < N; ++j) out[j] = c;
+            return;
+        }
         const float ir = 1.f / r_mm();
         const float s = ir * ir;
         const float q = v.x * (s * D00 * v.x + s * D01 * v.y) + v.y * (s * D01 * v.x + s * D11 * v.y);

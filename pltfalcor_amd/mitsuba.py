@@ -2,14 +2,18 @@
 (Source/plugins/importers/MitsubaImporter/MitsubaImporter.cpp) on top of the scene builder.
 
 Built: `<default>` substitution, `<ref>` to named BSDFs, transforms (translate, rotate, scale,
-matrix, lookat); BSDFs diffuse, dielectric / roughdielectric, conductor / roughconductor,
-coated_conductor / coated_roughconductor and twosided, converted setter by setter as
-`buildBSDF` does (MitsubaImporter.cpp:287-460; the named IoR table of Tables.h; `roughness` or
-`convertGSSigma2(sigma2)`); shapes rectangle, cube, sphere, disk and obj, with a nested area
-emitter (`emission_profile` + `scale`, or `radiance` RGB); emitters constant (a 1 x 1 environment
-map), envmap from a `.pfm` or `.npy` image, point (sourcing area `A`) and directional (a distant
-light with `Omega` or `source_area` / `distance`); the perspective sensor (`fov` over the film
-width, 24 mm frame height) and the film size. Other types raise NotImplementedError naming them.
+matrix, lookat); BSDFs diffuse, dielectric / roughdielectric, thindielectric, conductor /
+roughconductor, coated_conductor / coated_roughconductor, grating (sinusoidal, rectangular and
+linear profiles) and twosided, converted setter by setter as `buildBSDF` does
+(MitsubaImporter.cpp:287-530; the named IoR table of Tables.h; `roughness` or
+`convertGSSigma2(sigma2)`); shapes rectangle, disk, cube and sphere built by the reference's mesh
+generators (TriangleMesh::createQuad / createDisk / createCube / createSphere with 128 x 32
+segments), obj and ply files, `flip_normals`, with a nested area emitter (`emission_profile` +
+`scale`, or `radiance` RGB); emitters constant (a 1 x 1 environment map), envmap from a `.pfm` or
+`.npy` image, point (sourcing area `A`) and directional (a distant light with `Omega` or
+`source_area` / `distance`); the perspective sensor (`fov` over the film width, 24 mm frame
+height) and the film size. Other types raise NotImplementedError naming them, as do radial
+gratings, which need texture coordinates.
 """
 from __future__ import annotations
 
@@ -21,6 +25,7 @@ from pathlib import Path
 import numpy as np
 
 from . import materials as M
+from . import meshes as G
 from .lights import DistantLight, PointLight
 from .scene_builder import SceneBuilder
 
@@ -223,6 +228,18 @@ class Parser:
     def roughness(self, p):
         return p["roughness"] if "roughness" in p else convert_gs_sigma2(p.get("sigma2", 100.0))
 
+    def conductor_profiles(self, p, message):
+        """n, k profile ids from a material name or eta / k values (RGB through rgbToSpectrum)."""
+        if "material" in p:
+            return self.builder.addSpectralProfileFromMaterial(p["material"])
+        if "eta" in p and "k" in p:
+            ids = []
+            for key in ("eta", "k"):
+                self.builder.profiles.append(_sampled_profile(rgb_to_spectrum(p[key]), False))
+                ids.append(len(self.builder.profiles) - 1)
+            return tuple(ids)
+        raise ValueError(message)
+
     def bsdf(self, e):
         t = self.attr(e, "type")
         p = self.props(e)
@@ -256,6 +273,35 @@ class Parser:
             m.specular[0] = h(float(F32(.52) * (ior_h - F32(1.0)) / F32(abbe)))  # setAbbeNumber
             m.specular[1] = h(self.roughness(p))
             return m
+        if t == "thindielectric":
+            m = M.thin_dielectric(1.5, 0.0003)  # the constructor: specular 0, optic axis +z, birefringence 0
+            m.specular[0] = h(0.0)
+            m.ior = h(self.ior(p, "int_ior", "bk7")[0])
+            m.specular[3] = h(p.get("thickness", .001))
+            axis = np.asarray(p.get("optic_axis", (0.0, 0.0, 1.0)), np.float32)
+            axis = axis / np.float32(np.sqrt(np.dot(axis, axis)))
+            if axis[2] < 0:
+                axis = -axis
+            m.base_color[1], m.base_color[2] = h(float(axis[0])), h(float(axis[1]))
+            m.base_color[0] = h(p.get("birefringence_scale", 1.0))
+            b = p.get("birefringence", 0.0)
+            m.specular[0] = h(b[0] if isinstance(b, tuple) else b)
+            return m
+        if t == "grating":
+            kinds = {"sinusoidal": 0, "rectangular": 1, "linear": 2}
+            kind = p.get("type", "sinusoidal")
+            if kind.startswith("radial_"):
+                raise NotImplementedError("radial gratings (they need texture coordinates)")
+            if kind not in kinds:
+                raise ValueError("Unsupported 'type' name.")
+            n_id, k_id = self.conductor_profiles(p, "'grating' BSDF must specify either 'eta' and 'k' IOR values or "
+                                                      "'material' name.")
+            m = M.grated_conductor(p.get("pitch", 5.0), float(F32(F32(p.get("grating_direction", 0.0)) * math.pi / 180.0)),
+                                   n_id, k_id, lobes=p.get("lobes", 3), grating_type=kinds[kind], height=1.0,
+                                   height_scale=p.get("q", .5), roughness=0.0)
+            m.ior = h(ext)
+            m.specular[1] = h(p.get("amplify", 1.0))
+            return m
         if t in ("conductor", "roughconductor"):
             m = M.blank(M.CONDUCTOR)
             for i in range(4):
@@ -264,18 +310,8 @@ class Parser:
             c = p.get("specular_reflectance", (1.0, 1.0, 1.0))
             for i in range(3):
                 m.base_color[i] = h(c[i])
-            if "material" in p:
-                m.ior_n_profile, m.ior_k_profile = self.builder.addSpectralProfileFromMaterial(p["material"])
-            elif "eta" in p and "k" in p:
-                b = self.builder
-                ids = []
-                for key in ("eta", "k"):
-                    pr = _sampled_profile(rgb_to_spectrum(p[key]), False)
-                    b.profiles.append(pr)
-                    ids.append(len(b.profiles) - 1)
-                m.ior_n_profile, m.ior_k_profile = ids
-            else:
-                raise ValueError("'conductor' BSDF must specify either 'eta' and 'k' or 'material'")
+            m.ior_n_profile, m.ior_k_profile = self.conductor_profiles(
+                p, "'conductor' BSDF must specify either 'eta' and 'k' or 'material'")
             m.ior = h(ext)
             m.specular[1] = h(self.roughness(p))
             return m
@@ -301,60 +337,36 @@ class Parser:
 
     # --- shapes (buildShape, MitsubaImporter.cpp:575-745)
     def shape_mesh(self, t, p):
+        """The reference's meshes for each shape type (TriangleMesh::createQuad / createDisk /
+        createCube / createSphere, OBJ and PLY files), placed by `to_world`."""
+        m = p.get("to_world", np.eye(4))
         if t == "rectangle":
-            pos = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float64)
-            nrm = np.tile([0.0, 0.0, 1.0], (4, 1))
-            idx = np.array([[0, 1, 2], [0, 2, 3]], np.uint32)
+            pos, nrm, idx, cw = G.create_quad((2.0, 2.0))
+            m = m @ G.Y_TO_Z
         elif t == "disk":
-            k = 64
-            ang = np.arange(k) * 2 * math.pi / k
-            pos = np.vstack([[0, 0, 0], np.c_[np.cos(ang), np.sin(ang), np.zeros(k)]])
-            nrm = np.tile([0.0, 0.0, 1.0], (k + 1, 1))
-            idx = np.array([[0, 1 + i, 1 + (i + 1) % k] for i in range(k)], np.uint32)
+            pos, nrm, idx, cw = G.create_disk(1.0)
+            m = m @ G.Y_TO_Z
         elif t == "cube":
-            pos, nrm, idx = [], [], []
-            for axis in range(3):
-                for sgn in (-1.0, 1.0):
-                    n = np.zeros(3)
-                    n[axis] = sgn
-                    u, v = np.zeros(3), np.zeros(3)
-                    u[(axis + 1) % 3], v[(axis + 2) % 3] = 1.0, 1.0
-                    if sgn < 0:
-                        u, v = v, u
-                    base = len(pos)
-                    for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
-                        pos.append(n + a * u + b * v)
-                        nrm.append(n)
-                    idx += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
-            pos, nrm, idx = np.array(pos), np.array(nrm), np.array(idx, np.uint32)
+            pos, nrm, idx, cw = G.create_cube((2.0, 2.0, 2.0))
         elif t == "sphere":
-            nl, nu = 32, 64
-            th = np.linspace(0, math.pi, nl + 1)
-            ph = np.linspace(0, 2 * math.pi, nu + 1)
-            T, P = np.meshgrid(th, ph, indexing="ij")
-            nrm = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
-            c = np.asarray(p.get("center", (0.0, 0.0, 0.0)))
-            pos = c + p.get("radius", 1.0) * nrm
-            idx = []
-            for i in range(nl):
-                for j in range(nu):
-                    a, b = i * (nu + 1) + j, i * (nu + 1) + j + 1
-                    cc, d = a + nu + 1, b + nu + 1
-                    if i > 0:
-                        idx.append([a, b, cc])
-                    if i < nl - 1:
-                        idx.append([b, d, cc])
-            idx = np.array(idx, np.uint32)
+            pos, nrm, idx, cw = G.create_sphere(p.get("radius", 1.0), 128, 32)
+            m = m @ G.translate(p.get("center", (0.0, 0.0, 0.0)))
         elif t == "obj":
             pos, nrm, idx = _load_obj(self.base / p["filename"])
+            cw = False
+        elif t == "ply":
+            pos, nrm, idx = G.load_ply(self.base / p["filename"])
+            if p.get("face_normals", False):
+                pos, nrm, idx = G.face_normal_mesh(pos, idx)
+            elif nrm is None:
+                nrm = G.smooth_normals(pos, idx)
+            cw = False
         else:
             raise NotImplementedError(f"shape type '{t}'")
-        m = p.get("to_world", np.eye(4))
-        pos = (np.c_[pos, np.ones(len(pos))] @ m.T)[:, :3]
-        nit = np.linalg.inv(m[:3, :3]).T
-        nrm = nrm @ nit.T
-        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
-        return pos.astype(F32), nrm.astype(F32), idx
+        if p.get("flip_normals", False) and t in ("rectangle", "disk", "cube", "sphere"):
+            pos, nrm, idx, cw = G.flip_normals(pos, nrm, idx, cw)
+        pos, nrm, cw = G.apply_transform(m, pos, nrm, cw)
+        return G.bake(pos, nrm, idx, cw)
 
     def emission_profile(self, p):
         """getEmissionSpectrum (MitsubaImporter.cpp:160-168)."""

@@ -48,6 +48,20 @@ SCENE = """
     <transform name="to_world"><rotate x="1" angle="90"/><translate x="0" y="5" z="0"/></transform>
     <emitter type="area"><string name="emission_profile" value="FL4"/><float name="scale" value="8"/></emitter>
   </shape>
+  <shape type="rectangle">
+    <transform name="to_world"><scale value="0.6"/><translate x="3" y="0.8" z="-1"/></transform>
+    <bsdf type="grating">
+      <string name="material" value="Au"/><float name="pitch" value="1.2"/><integer name="lobes" value="4"/>
+      <string name="type" value="rectangular"/><float name="grating_direction" value="30"/>
+    </bsdf>
+  </shape>
+  <shape type="disk">
+    <transform name="to_world"><scale value="0.7"/><translate x="-3" y="0.8" z="-1"/></transform>
+    <bsdf type="thindielectric">
+      <float name="thickness" value="0.0005"/><float name="birefringence" value="0.01"/>
+      <vector name="optic_axis" x="1" y="0" z="1"/>
+    </bsdf>
+  </shape>
   <emitter type="constant"><rgb name="radiance" value="0.05, 0.07, 0.1"/></emitter>
   <emitter type="directional">
     <vector name="direction" x="0.3" y="-1" z="0.2"/><rgb name="irradiance" value="0.5, 0.45, 0.4"/>
@@ -67,8 +81,9 @@ def test_parse_converts_bsdfs_like_the_reference():
     prs = Parser(SCENE, TABLES).run()
     b = prs.builder
     mats = b.materials
-    assert [m.type for m in mats] == [M.DIFFUSE, M.CONDUCTOR, M.DIELECTRIC, M.COATED_CONDUCTOR, M.DIFFUSE]
-    floor, cu, glass, coated, lamp = mats
+    assert [m.type for m in mats] == [M.DIFFUSE, M.CONDUCTOR, M.DIELECTRIC, M.COATED_CONDUCTOR, M.DIFFUSE,
+                                      M.GRATED_CONDUCTOR, M.THIN_DIELECTRIC]
+    floor, cu, glass, coated, lamp, grating, thin = mats
     assert [_f16(floor.base_color[i]) for i in range(3)] == [_f16(M.h(v)) for v in (0.6, 0.55, 0.5)]
     # conductor: material tables, $rough substituted, gamma 3, ext IoR of air
     assert _f16(cu.specular[1]) == _f16(M.h(0.2)) and _f16(cu.specular[0]) == 3.0
@@ -85,6 +100,16 @@ def test_parse_converts_bsdfs_like_the_reference():
     assert lamp.emission_profile != M.NO_PROFILE
     assert b.profiles[lamp.emission_profile].intensity > 0
     assert len(b.lights) == 1 and prs.env is not None and (prs.width, prs.height) == (96, 64)
+    # grating: the importer's setters over the constructor's defaults (height 1, power multiplier 1)
+    assert _f16(grating.data1[0]) == _f16(M.h(1.2)) and grating.data1[3] == (1 << 8) | 4
+    assert _f16(grating.data1[2]) == _f16(M.h(float(np.float32(np.float32(30.0) * math.pi / 180.0))))
+    assert _f16(grating.specular[0]) == 0.5 and _f16(grating.specular[1]) == 1.0
+    assert _f16(grating.data2[0]) == 1.0 and _f16(grating.data2[1]) == 0.0 and _f16(grating.data1[1]) == 0.0
+    assert _f16(grating.ior) == _f16(M.h(IOR_TABLE["air"][0])) and grating.ior_n_profile != M.NO_PROFILE
+    # thin dielectric: bk7, thickness, normalised optic axis (x, y), birefringence scale and value
+    assert _f16(thin.ior) == _f16(M.h(1.5046)) and _f16(thin.specular[3]) == _f16(M.h(0.0005))
+    assert _f16(thin.base_color[1]) == _f16(M.h(math.sqrt(0.5))) and _f16(thin.base_color[2]) == 0.0
+    assert _f16(thin.base_color[0]) == 1.0 and _f16(thin.specular[0]) == _f16(M.h(0.01))
 
 
 def test_parse_geometry_and_sensor():
@@ -94,7 +119,7 @@ def test_parse_geometry_and_sensor():
     # rectangle rotated -90 degrees about x and scaled by 6: the y = 0 plane, +y normal
     assert np.allclose(floor.positions[:, 1], 0, atol=1e-6) and np.allclose(np.abs(floor.positions[:, [0, 2]]), 6)
     assert np.allclose(floor.normals, [0, 1, 0], atol=1e-6)
-    lamp = prs.builder.meshes[-1]
+    lamp = prs.builder.meshes[4]
     assert np.allclose(lamp.positions[:, 1], 5) and np.allclose(lamp.normals, [0, -1, 0], atol=1e-6)
     cube = prs.builder.meshes[2]
     assert np.allclose(cube.positions.min(0), [-0.5, 0.0, -0.5]) and np.allclose(cube.positions.max(0), [0.5, 1.0, 0.5])
@@ -102,6 +127,48 @@ def test_parse_geometry_and_sensor():
     assert np.allclose(list(cam.pos), [0, 3, 8]) and abs(cam.aspect - 1.5) < 1e-6
     focal = (24 / 64 * 96) / (2 * math.tan(math.radians(45) / 2))  # fov over the film width
     assert abs(cam.fov_y - 2 * math.atan(12 / focal)) < 1e-5
+
+
+def _face_normals(mesh):
+    p, i = mesh.positions.astype(np.float64), mesh.indices
+    n = np.cross(p[i[:, 1]] - p[i[:, 0]], p[i[:, 2]] - p[i[:, 0]])
+    return n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+
+
+def test_reference_meshes_and_face_orientation():
+    """Shapes use the reference's generators; each triangle's face normal (from the winding) agrees
+    with its vertex normals, also after flip_normals and a mirroring transform."""
+    from pltfalcor_amd.mitsuba import Parser
+    prs = Parser(SCENE, TABLES).run()
+    sphere = prs.builder.meshes[1]
+    assert len(sphere.positions) == 129 * 33 and len(sphere.indices) == 128 * 32 * 2  # createSphere(r, 128, 32)
+    assert np.allclose(np.linalg.norm(sphere.positions - [-1.5, 0.6, 0], axis=1), 0.6, atol=1e-6)
+    disk = prs.builder.meshes[-1]
+    assert len(disk.positions) == 33 and np.allclose(disk.positions[:, 2], -1) and np.allclose(disk.normals, [0, 0, 1])
+    for k in (0, 1, 2, 4, 5, 6):
+        m = prs.builder.meshes[k]
+        fn = _face_normals(m)
+        good = np.linalg.norm(np.cross(m.positions[m.indices[:, 1]] - m.positions[m.indices[:, 0]],
+                                       m.positions[m.indices[:, 2]] - m.positions[m.indices[:, 0]]), axis=1) > 1e-9
+        vn = m.normals[m.indices[:, 0]]
+        assert (np.einsum("ij,ij->i", fn[good], vn[good]) > 0.5).all(), k
+    flip = Parser('<scene version="3.0.0"><shape type="rectangle"><boolean name="flip_normals" value="true"/>'
+                  '<transform name="to_world"><scale x="-1" y="1" z="1"/></transform></shape></scene>').run()
+    m = flip.builder.meshes[0]
+    assert np.allclose(m.normals, [0, 0, -1]) and np.allclose(_face_normals(m), [0, 0, -1])
+
+
+def test_ply_shape(tmp_path):
+    from pltfalcor_amd.mitsuba import Parser
+    pos = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+    (tmp_path / "q.ply").write_text(
+        "ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\nproperty float z\n"
+        "element face 1\nproperty list uchar int vertex_indices\nend_header\n"
+        + "".join(f"{a} {b} {c}\n" for a, b, c in pos) + "4 0 1 2 3\n")
+    (tmp_path / "s.xml").write_text('<scene version="3.0.0"><shape type="ply"><string name="filename" value="q.ply"/>'
+                                    '<boolean name="face_normals" value="true"/></shape></scene>')
+    m = Parser(str(tmp_path / "s.xml")).run().builder.meshes[0]
+    assert len(m.positions) == 6 and np.allclose(m.normals, [0, 0, 1]) and np.allclose(_face_normals(m), [0, 0, 1])
 
 
 def test_rgb_to_spectrum_matches_basis():
@@ -120,6 +187,26 @@ def test_unsupported_types_raise():
         Parser('<scene version="3.0.0"><shape type="hair"/></scene>').run()
     with pytest.raises(NotImplementedError):
         Parser('<scene version="3.0.0"><bsdf type="plastic" id="p"/></scene>').run()
+    with pytest.raises(NotImplementedError):
+        Parser('<scene version="3.0.0"><bsdf type="grating" id="g"><string name="material" value="Au"/>'
+               '<string name="type" value="radial_sinusoidal"/></bsdf></scene>', TABLES).run()
+
+
+def test_distant_light_through_birefringent_slab_is_never_dropped():
+    """Sky light seen through the birefringent thin dielectric: the o and e rays of a beam from a
+    distant source (which has not propagated, r = 0) are mutually incoherent, so the transmitted
+    intensity ss^2 Lx + ps^2 Ly is positive. Evaluated literally, the coherence is 0 * inf or
+    inf - inf and dropped about half of these paths by rounding."""
+    from pltfalcor_amd.mitsuba import load_scene
+    from test_pathtracer import _path_len
+    s, w, h = load_scene(SCENE, None, TABLES)
+    a = s.numpy()
+    col, rec = O.render(a, w, h, frame=0, max_bounces=6, records=True, do_mnee=1)
+    L = _path_len(rec)
+    thin_tris = np.where(a["mesh_material"][a["tri_mesh"]] == 6)[0]
+    through = (L == 2) & np.isin(rec[0, 0], thin_tris)  # camera -> slab -> sky
+    assert through.sum() > 50
+    assert (col[through, :3].sum(1) > 0).all()
 
 
 @pytest.mark.gpu
