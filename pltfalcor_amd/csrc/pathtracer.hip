@@ -884,7 +884,7 @@ PLT_DEV void ms_forward_unwind(const PtScene& sc, const PtFrame& fr, const MsDat
 // over 256 registers at its peak, which allows one wave per SIMD; capping it at 128 registers
 // (spilling the rest to scratch) gives the traversals four waves to hide latency behind. At
 // 1080p depth 8 with manifold NEE: 18.0 ms per frame at one wave, 11.3 ms at two, 10.5 ms at
-// three, 10.1 ms at four. `Lights` instantiates the analytic-light and environment-map code only
+// three, 10.1 ms at four, 13.0 ms at five (96 registers, three times the spills). `Lights` instantiates the analytic-light and environment-map code only
 // for scenes that have them, so scenes lit by emissive geometry alone do not carry its registers.
 template <int N, bool Lights>
 __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
