@@ -375,7 +375,7 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N>1 is not supported "
-                          "(paths and queries shard by pixel/query range across GPUs; see DESIGN.md)",
+                          "(paths and queries would shard by pixel/query range across GPUs)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -20,7 +20,12 @@
 //   Dielectric/Conductor   : FP32 vs FP64 instantiation, reciprocity-free closed forms
 //                            (smooth limits, unpolarised M00=(Rs+Rp)/2), Stokes physicality.
 // Nothing in the reference pins PLT-specific arithmetic with numbers (no tests exist there);
-// DESIGN.md lists which functions remain "parity unpinned" beyond the checks above.
+// Parity unpinned (restatement fidelity only, reviewed line by line against the Slang): the
+// coherence transforms and mutual-coherence terms inside the rough dielectric/conductor lobes beyond
+// their closed-form limits, the coated opaque dielectric's thin-film interference, the diffraction
+// grating lobe intensities (Bessel/sinc), spectral MIS (lambdaPdf) weights, the solve pass's
+// forward-unwind bookkeeping beyond depth 0, and the grating's walk Jacobian (the reference's own
+// approximation, DiffractionGrating.slang:229-232).
 #pragma once
 #include <cmath>
 #include <cstdint>
