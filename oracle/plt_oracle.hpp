@@ -22,7 +22,8 @@
 // Nothing in the reference pins PLT-specific arithmetic with numbers (no tests exist there);
 // Parity unpinned (restatement fidelity only, reviewed line by line against the Slang): the
 // coherence transforms and mutual-coherence terms inside the rough dielectric/conductor lobes beyond
-// their closed-form limits, the coated opaque dielectric's thin-film interference, the diffraction
+// their closed-form limits, the coated opaque dielectric's scattered lobe (its delta lobe is pinned
+// by the two-beam interference limits), the diffraction
 // grating lobe intensities (Bessel/sinc), spectral MIS (lambdaPdf) weights, the solve pass's
 // forward-unwind bookkeeping beyond depth 0, and the grating's walk Jacobian (the reference's own
 // approximation, DiffractionGrating.slang:229-232).

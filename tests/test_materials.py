@@ -187,6 +187,49 @@ def test_coated_conductor_limits():
             assert abs(out[j, 0] - exp) <= 1e-6 * exp, (th, lam, out[j, 0], exp)
 
 
+def test_coated_opaque_dielectric_interference_limits():
+    """The smooth coat's delta lobe is two-beam interference between the coat reflection (I2 = R)
+    and the light returned by the base through the coat (I1 = m00 T_in T_out). At normal incidence
+    the two beams are fully coherent: over the coat thickness the output swings between
+    I1 + I2 +- 2 sqrt(I1 I2), and a shift by a quarter wavelength of optical path (lambda / 4n) mirrors
+    it about I1 + I2. At grazing offsets with a tiny coherence area it is the incoherent sum, whatever
+    the thickness. R and T are textbook Fresnel, m00 the same coat with an index-matched (eta 1) coat."""
+    from pltfalcor_amd import materials as M
+    n_coat, lam = 1.5, 562.5  # lambda / 4n = 0.09375 um, exact in binary16
+    S = np.tile([1.0, 0.0, 0.0, 0.0], 4)
+    L = [lam] * 4
+    up = [0.0, 0.0, 1.0]
+
+    def s0(mat, wi, wo, area):
+        out, _, _ = O.material_eval(M.pack([mat]), b"", 0, wi, wo, 0x04, L, S, area)
+        return out[0, 0]
+
+    m00 = s0(M.coated_opaque(1, 1, 1, 0.0, 1.0, 1.0), up, up, 10.0)
+    R = ((n_coat - 1) / (n_coat + 1)) ** 2
+    I1, I2 = m00 * (1 - R) ** 2, R
+    taus = 1.0 + np.arange(64) * (lam * 1e-3 / (2 * n_coat)) / 64  # one optical period
+    v = np.array([s0(M.coated_opaque(1, 1, 1, 0.0, n_coat, t), up, up, 10.0) for t in taus])
+    amp = 2 * np.sqrt(I1 * I2)
+    assert abs(v.max() - (I1 + I2 + amp)) < 0.01 * amp and abs(v.min() - (I1 + I2 - amp)) < 0.01 * amp
+    assert abs(v.mean() - (I1 + I2)) < 0.02 * amp
+    for t in (1.0, 1.25, 1.5):
+        a = s0(M.coated_opaque(1, 1, 1, 0.0, n_coat, t), up, up, 10.0)
+        b = s0(M.coated_opaque(1, 1, 1, 0.0, n_coat, t + lam * 1e-3 / (4 * n_coat)), up, up, 10.0)
+        assert abs(a + b - 2 * (I1 + I2)) < 1e-3 * amp, (t, a, b)  # a wrong optical path misses by O(amp)
+    # incoherent: 40 degrees, coherence area 1e-6 um^2
+    th = np.radians(40.0)
+    wi = [np.sin(th), 0.0, np.cos(th)]
+    wo = [-wi[0], 0.0, wi[2]]
+    rs, rp, _, _, _ = fresnel_diel(np.cos(th), n_coat)
+    Rs, Rp = abs(rs) ** 2, abs(rp) ** 2
+    want = m00 * ((1 - Rs) ** 2 + (1 - Rp) ** 2) / 2 + (Rs + Rp) / 2
+    got = [s0(M.coated_opaque(1, 1, 1, 0.0, n_coat, t), wi, wo, 1e-6) for t in (1.0, 1.05, 1.3, 2.0)]
+    # the smooth coat keeps the fractal surface's specular factor alpha (1 to within 1e-4 here);
+    # a wrong structure (T for T^2, s and p mixed) misses by more than 1 %
+    np.testing.assert_allclose(got, want, rtol=2e-4)
+    assert np.ptp(got) < 1e-9  # no thickness dependence left
+
+
 def test_thin_dielectric_and_multilayer_sampling():
     from pltfalcor_amd import materials as M
     prof, _, _ = _profiles()
