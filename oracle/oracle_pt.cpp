@@ -1228,3 +1228,6 @@ int oracle_material_sample(const void* mat, const void* prof, uint32_t nprof, co
 }
 
 }  // extern "C"
+
+// Scalar entry point for the closed-form test of the grating's Bessel function (test infrastructure).
+extern "C" double oracle_bessel_j_f64(int nu, double x) { return BesselJ<double>(nu, x); }

@@ -199,3 +199,23 @@ def test_fp16_conversion_vs_numpy():
     assert np.array_equal(got, ref)
     back = np.array([O.lib().oracle_f16_to_f32(int(h)) for h in ref[:2000]], dtype=np.float32)
     assert np.array_equal(back, ref[:2000].view(np.float16).astype(np.float32))
+
+
+def test_grating_bessel_matches_scipy_in_both_regions():
+    """The grating's Bessel function J_nu (the reference's small-argument series blended into the
+    large-argument asymptotic form) against scipy.special.jv: within 2e-4 where the series is used,
+    and within 1.1 x the first neglected Hankel term (4 nu^2 - 1) / (8x) sqrt(2 / (pi x)) where the
+    asymptotic form is used. A wrong factorial index, sign or phase offset misses by the amplitude."""
+    import ctypes as C
+    import scipy.special as sp
+    f = O.lib().oracle_bessel_j_f64
+    f.restype = C.c_double
+    f.argtypes = [C.c_int, C.c_double]
+    for nu in range(1, 7):
+        for x in np.linspace(0.01, 0.5 * (nu + 1), 25):
+            assert abs(f(nu, x) - sp.jv(nu, x)) < 2e-4, (nu, x)
+    for nu, lo, hi in ((1, 6, 60), (2, 9, 60), (3, 12, 60), (7, 60, 100), (8, 60, 100)):
+        for x in np.linspace(lo, hi, 25):
+            bound = 1.1 * (4 * nu * nu - 1) / (8 * x) * np.sqrt(2 / (np.pi * x))
+            assert abs(f(nu, x) - sp.jv(nu, x)) < bound, (nu, x)
+    assert f(-3, 5.0) == -f(3, 5.0) and f(-2, 5.0) == f(2, 5.0)  # J_-n = (-1)^n J_n
