@@ -1231,3 +1231,12 @@ int oracle_material_sample(const void* mat, const void* prof, uint32_t nprof, co
 
 // Scalar entry point for the closed-form test of the grating's Bessel function (test infrastructure).
 extern "C" double oracle_bessel_j_f64(int nu, double x) { return BesselJ<double>(nu, x); }
+
+// Lobe intensity of a 1-D grating of the given profile type at normal incidence, for the
+// closed-form test of the lobe profiles (test infrastructure).
+extern "C" double oracle_grating_lobe_f64(int type, int order, double height_um, double lambda_um) {
+    DiffractionGrating<double> g{};
+    g.gx = 1; g.gy = 0; g.ipx = 1; g.ipy = 0; g.q = height_um;
+    g.lobeCount = 8; g.type = uint32_t(type); g.mult = 1;
+    return g.lobeIntensity(order, 0, V3<double>{0, 0, 1}, lambda_um);
+}

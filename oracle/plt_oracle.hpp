@@ -24,7 +24,8 @@
 // coherence transforms and mutual-coherence terms inside the rough dielectric/conductor lobes beyond
 // their closed-form limits, the coated opaque dielectric's scattered lobe (its delta lobe is pinned
 // by the two-beam interference limits), the diffraction
-// grating's rectangular and linear lobe profiles and the Bessel blend band (its Bessel function
+// grating's linear lobe profile and the Bessel blend band (the rectangular profile is pinned
+// against the Fourier coefficients of a binary phase grating) (its Bessel function
 // is pinned against scipy in the series and asymptotic regions), spectral MIS (lambdaPdf) weights, the solve pass's
 // forward-unwind bookkeeping beyond depth 0, and the grating's walk Jacobian (the reference's own
 // approximation, DiffractionGrating.slang:229-232).

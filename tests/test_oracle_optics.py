@@ -219,3 +219,21 @@ def test_grating_bessel_matches_scipy_in_both_regions():
             bound = 1.1 * (4 * nu * nu - 1) / (8 * x) * np.sqrt(2 / (np.pi * x))
             assert abs(f(nu, x) - sp.jv(nu, x)) < bound, (nu, x)
     assert f(-3, 5.0) == -f(3, 5.0) and f(-2, 5.0) == f(2, 5.0)  # J_-n = (-1)^n J_n
+
+
+def test_rectangular_grating_orders_match_fourier_coefficients():
+    """A 1-D rectangular (50 % duty binary phase) grating: each order's intensity equals |c_l|^2, the
+    Fourier coefficient of the phase profile exp(i a rect(x)) computed by FFT, with the phase a the
+    grating's 4 pi q / lambda at normal incidence. Even orders vanish."""
+    import ctypes as C
+    import math
+    f = O.lib().oracle_grating_lobe_f64
+    f.restype = C.c_double
+    f.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double]
+    lam, n = 0.55, 1 << 16
+    x = np.arange(n) / n
+    for a in (0.3, 1.0, 2.0, 3.0):
+        c = np.fft.fft(np.exp(1j * a * (x < 0.5))) / n
+        for order in (1, 2, 3, 4, 5, 7):
+            got = f(1, order, a * lam / (4 * math.pi), lam)
+            assert abs(got - abs(c[order]) ** 2) <= 1e-3 * abs(c[1]) ** 2, (a, order, got, abs(c[order]) ** 2)
