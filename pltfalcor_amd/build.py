@@ -1,7 +1,8 @@
 """Builds the HIP library libplt_amd.so in-tree for gfx950 (and the CPU oracle used by tests).
 
-Run as ``python -m pltfalcor_amd.build`` or through ``__graft_entry__.build()``. The library is
-compiled from every ``csrc/*.hip`` file by one hipcc invocation; nothing is JIT-compiled at
+Run as ``python -m pltfalcor_amd.build`` or through ``__graft_entry__.build()``. Every
+``csrc/*.hip`` / ``csrc/*.cpp`` file is compiled to its own object (in parallel, under
+``_build/``) and the objects are linked into one shared library; nothing is JIT-compiled at
 import time, so the built ``.so`` is what ships to the GPU host.
 """
 from __future__ import annotations
@@ -10,18 +11,19 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 LIB = PKG / "libplt_amd.so"
+OBJ_DIR = ROOT / "_build"
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-function",
     "-pthread",
@@ -37,12 +39,34 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm required to build libplt_amd.so)")
 
 
+def _jobs() -> int:
+    try:
+        return max(1, min(16, int(os.environ.get("MAX_JOBS", "")) if os.environ.get("MAX_JOBS") else (os.cpu_count() or 4)))
+    except ValueError:
+        return 4
+
+
 def build_library(verbose: bool = True) -> Path:
-    sources = sorted(str(p) for p in (PKG / "csrc").glob("*.hip")) + sorted(str(p) for p in (PKG / "csrc").glob("*.cpp"))
+    sources = sorted((PKG / "csrc").glob("*.hip")) + sorted((PKG / "csrc").glob("*.cpp"))
     if not sources:
         raise RuntimeError("no HIP sources under pltfalcor_amd/csrc")
+    OBJ_DIR.mkdir(exist_ok=True)
+    hipcc = _hipcc()
+    # the largest translation units first, so they start before the small ones
+    sources.sort(key=lambda p: -p.stat().st_size)
+
+    def compile_one(src: Path) -> Path:
+        obj = OBJ_DIR / (src.name + ".o")
+        cmd = [hipcc, *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-c", "-o", str(obj), str(src)]
+        if verbose:
+            print("[build]", " ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=_jobs()) as ex:
+        objs = list(ex.map(compile_one, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-o", str(tmp), *sources]
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", str(tmp), *map(str, objs)]
     if verbose:
         print("[build]", " ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
