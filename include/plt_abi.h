@@ -100,6 +100,12 @@ int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profi
  * ------------------------------------------------------------------------------------- */
 int plt_sample_generator_stream(uint32_t n_streams, uint32_t width, uint32_t sample_number, uint32_t draws,
                                 uint32_t* out, plt_stream stream);
+/* The same for either generator type: type 1 = UniformSampleGenerator (as above), type 0 =
+ * TinyUniformSampleGenerator (Utils/Sampling/TinyUniformSampleGenerator.slang:55-66: LCG
+ * (Pseudorandom/LCG.slang:45-61) seeded with blockCipherTEA(interleave_32bit(pixel),
+ * sampleNumber).x, Utils/Math/HashUtils.slang:59-71). */
+int plt_sample_generator_stream_typed(int type, uint32_t n_streams, uint32_t width, uint32_t sample_number,
+                                      uint32_t draws, uint32_t* out, plt_stream stream);
 
 /* ---------------------------------------------------------------------------------------
  * Batched BSDF queries (the IMaterialInstance eval + pdf pair that next-event estimation runs,
@@ -136,15 +142,18 @@ enum {
 enum { PLT_R_S0 = 0, PLT_R_S1, PLT_R_S2, PLT_R_S3, PLT_R_PDF, PLT_R_PLANES };
 
 typedef enum plt_material_type {
-    PLT_MAT_DIELECTRIC = 0,               /* PLTDielectricBSDF.slang:43-361 (Gaussian PSD) */
-    PLT_MAT_CONDUCTOR = 1,                /* PLTConductorBSDF.slang:44-242 (K-correlation PSD, tabulated n,k) */
-    PLT_MAT_DIFFUSE = 2,                  /* PLTDiffuseBSDF.slang:39-155 (Lambertian; the emitter material) */
-    PLT_MAT_OPAQUE_DIELECTRIC = 3,        /* PLTOpaqueDielectricBSDF.slang:43-212 */
-    PLT_MAT_COATED_OPAQUE_DIELECTRIC = 4, /* PLTCoatedOpaqueDielectricBSDF.slang:45-312 */
-    PLT_MAT_GRATED_CONDUCTOR = 5,         /* PLTDiffractionGratedConductorBSDF.slang:45-290 */
-    PLT_MAT_COATED_CONDUCTOR = 6,         /* PLTCoatedConductorBSDF.slang:45-313 */
-    PLT_MAT_THIN_DIELECTRIC = 7,          /* PLTThinDielectricBSDF.slang:43-269 (+ Birefringence.slang) */
-    PLT_MAT_MULTILAYER_STACK = 8          /* PLTMultiLayeredStackBSDF.slang:42-201 */
+    /* The reference's MaterialType values (Scene/Material/MaterialTypes.slang:36-53); Unknown (0)
+     * and Standard (1) are not PLT materials and are refused. */
+    PLT_MAT_DIFFUSE = 2,                   /* PLTDiffuseBSDF.slang:39-155 (Lambertian; the emitter material) */
+    PLT_MAT_CONDUCTOR = 3,                 /* PLTConductorBSDF.slang:44-242 (K-correlation PSD, tabulated n,k) */
+    PLT_MAT_DIELECTRIC = 4,                /* PLTDielectricBSDF.slang:43-361 (Gaussian PSD) */
+    PLT_MAT_OPAQUE_DIELECTRIC = 5,         /* PLTOpaqueDielectricBSDF.slang:43-212 */
+    PLT_MAT_THIN_DIELECTRIC = 6,           /* PLTThinDielectricBSDF.slang:43-269 (+ Birefringence.slang) */
+    PLT_MAT_COATED_CONDUCTOR = 7,          /* PLTCoatedConductorBSDF.slang:45-313 */
+    PLT_MAT_COATED_OPAQUE_DIELECTRIC = 8,  /* PLTCoatedOpaqueDielectricBSDF.slang:45-312 */
+    PLT_MAT_GRATED_CONDUCTOR = 9,          /* PLTDiffractionGratedConductorBSDF.slang:45-290 */
+    PLT_MAT_GRATED_OPAQUE_DIELECTRIC = 10, /* PLTDiffractionGratedOpaqueDielectricBSDF.slang:45-283 */
+    PLT_MAT_MULTILAYER_STACK = 11          /* PLTMultiLayeredStackBSDF.slang:42-201 */
 } plt_material_type;
 
 /* Synthetic batch: query i draws 14 numbers from UniformSampleGenerator(pixel=(i & 0xFFFF,
@@ -169,7 +178,11 @@ int plt_bsdf_eval_queries(plt_context* ctx, uint32_t n, const uint32_t* queries,
  *   coated opaque: base_color, specular = (-, roughness, thickness scale, -), data1 = (thickness,
  *                  -, -, coat IoR), ior = ext IoR
  *   grated cond. : base_color, specular = (height scale, power mult), data1 = (pitch x, pitch y,
- *                  direction, type|lobes bits), data2 = (height, roughness), ior, n/k profiles
+ *                  direction, type|lobes bits), data2 = (height, roughness), ior, n/k profiles.
+ *                  Radial gratings (type bit 0x10, DiffractionGrating.slangh:37) orient the grating
+ *                  by the texture coordinates, which these meshes do not carry: PLT_ENOTIMPL.
+ *   grated opaque: as the grated conductor over an opaque dielectric base: base_color = albedo,
+ *                  no ior or n/k profiles (PLTDiffractionGratedOpaqueDielectricMaterial.slang:40-70)
  *   coated cond. : base_color = specular colour, specular = (-, roughness, thickness scale, -),
  *                  data1 = (thickness um, -, -, coat IoR), ior = ext IoR, n/k profiles
  *   thin diel.   : base_color = (birefringence, optic axis x, optic axis y, alpha),
@@ -199,7 +212,8 @@ typedef struct plt_material {
  * the given coherence area (um^2); results are the 4 outgoing Stokes vectors, pdf(toLight = wi
  * | sd.V = wo) at wavelength 0 and lambdaPdf. sample: query i draws from the sample generator
  * (pixel (i, 0), sample number `frame`). Profiles are the context's. A material index out of
- * range gives zeros (eval) or valid = 0 (sample). */
+ * range, a type outside the PLT MaterialType values 2-11 or a radial grating gives zeros (eval) or
+ * valid = 0 (sample). */
 enum {
     PLT_MQ_WI_X = 0, PLT_MQ_WI_Y, PLT_MQ_WI_Z, PLT_MQ_WO_X, PLT_MQ_WO_Y, PLT_MQ_WO_Z,
     PLT_MQ_LOBE, PLT_MQ_MAT, PLT_MQ_LAMBDA /* 4 planes */, PLT_MQ_S = PLT_MQ_LAMBDA + 4 /* 16 planes, [j*4 + c] */,
@@ -225,6 +239,11 @@ typedef struct plt_camera {
                              data.jitterX/Y: a pixel offset in [-0.5, 0.5) divided by the frame size);
                              0 = pixel centre. Replaced per frame by a jitter pattern
                              (plt_pass_set_sample_pattern). */
+    float polarizer;      /* camera polarizer strength in [0, 1] (CameraData.slang:67-68, Camera.cpp:315-321):
+                             0 = none; p mixes (1 - p) I + p MuellerPolarizer(polarizer_theta) into every
+                             measured beam (pltpt_solve.rt.slang:111-126) */
+    float polarizer_theta;/* the reference's polarizer angle (radians); MuellerPolarizer(theta) takes
+                             cos/sin of theta itself, so the transmission axis lies at theta / 2 */
 } plt_camera;
 
 /* ---------------------------------------------------------------------------------------
@@ -380,7 +399,8 @@ int plt_sample_pattern_generate(int pattern, uint32_t sample_count, uint32_t n, 
 
 /* I/O of one execute (PLTPT.cpp:65-78): caller-owned device buffers of width*height float4
  * (RGBA32F): color (spectral radiance -> linear Rec.709), albedo and shading normal of the first
- * non-delta vertex (world space). vbuffer is accepted and ignored, as in the reference. */
+ * non-delta vertex, the normal in view space (the rotation of the camera's right-handed look-at
+ * view matrix, pltpt_sample.rt.slang:111). vbuffer is accepted and ignored, as in the reference. */
 typedef struct plt_render_data {
     uint32_t width, height;
     float* color;
@@ -390,9 +410,13 @@ typedef struct plt_render_data {
 } plt_render_data;
 
 /* Renders one frame (sample pass + solve pass) on `stream`, then increments the frame counter.
- * Options whose code is not built (NEEUsePerTileSG=1, emissiveSampler other than 0 Uniform or
- * 2 Power, sampleGenerator other than 0, debug views, MNEEMaxOccluders above 8, HWSS other than
- * 1 or 4) return PLT_ENOTIMPL. */
+ * sampleGenerator selects the reference's generators (SampleGeneratorType.slangh:32-36): 1 (the
+ * default) = UniformSampleGenerator (xoshiro128**), 0 = TinyUniformSampleGenerator (32-bit LCG
+ * seeded by blockCipherTEA); NEEUsePerTileSG = 1 draws the NEE light-type choice from a second
+ * generator seeded per tileSize x tileSize tile (pltpt_solve.rt.slang:410-412,
+ * LightSampling.slangh:181-185). Options whose code is not built (emissiveSampler other than 0
+ * Uniform or 2 Power, debug views, MNEEMaxOccluders above 8, HWSS other than 1 or 4) return
+ * PLT_ENOTIMPL. */
 int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream stream);
 
 /* Copies the bounce records of the last execute to host memory `dst` (synchronous). Layout:
@@ -401,6 +425,10 @@ int plt_pass_execute(plt_pass* pass, const plt_render_data* data, plt_stream str
  * path contribution 1/(pdf*thp), last non-delta pdf (the reference's BounceData, Path.slang:113-142).
  * Returns PLT_EINVAL if `bytes` is smaller than the record buffer. */
 int plt_pass_copy_bounce_records(plt_pass* pass, void* dst, size_t bytes);
+
+/* Test hook (no counterpart in the reference): the n-th device upload of the next scene bindings
+ * (plt_pass_set_scene) fails as an allocation failure would; n < 0 turns it off. */
+int plt_debug_fail_upload(int n);
 
 /* Schedule of the sample pass (no counterpart in the reference, whose sample pass is one DXR ray
  * generation shader): PLT_SCHEDULE_MEGAKERNEL (default, the faster on MI355X) runs one kernel in

@@ -14,7 +14,7 @@ enum Plane {
     Q_WI_X = 0, Q_WI_Y, Q_WI_Z, Q_WO_X, Q_WO_Y, Q_WO_Z, Q_LAMBDA, Q_MAT, Q_PARAM,
     Q_ROUGH_Q, Q_ROUGH_L, Q_COH_AREA, Q_S0, Q_S1, Q_S2, Q_S3, Q_PLANES
 };
-enum MatType { MAT_DIELECTRIC = 0, MAT_CONDUCTOR = 1 };
+enum MatType { MAT_CONDUCTOR = 3, MAT_DIELECTRIC = 4 };  // MaterialTypes.slang:36-53
 
 inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
@@ -150,6 +150,29 @@ void oracle_xoshiro128ss_seq(const uint32_t* state, uint32_t n, uint32_t* out) {
 void oracle_usg_seq(uint32_t px, uint32_t py, uint32_t sampleNumber, uint32_t n, uint32_t* out) {
     UniformSampleGenerator sg(px, py, sampleNumber);
     for (uint32_t i = 0; i < n; ++i) out[i] = sg.next();
+}
+// Raw uint32 streams of either generator for a block of pixels: stream s = pixel (x0 + s % w,
+// y0 + s / w), `draws` values each, out[s * draws + d]. type 0 = TinyUniformSampleGenerator,
+// 1 = UniformSampleGenerator.
+void oracle_sg_streams(int type, uint32_t x0, uint32_t y0, uint32_t w, uint32_t nStreams, uint32_t sampleNumber,
+                       uint32_t draws, uint32_t* out) {
+    for (uint32_t s = 0; s < nStreams; ++s) {
+        const uint32_t px = x0 + s % w, py = y0 + s / w;
+        uint32_t* o = out + size_t(s) * draws;
+        if (type == 0) {
+            TinyUniformSampleGenerator sg(px, py, sampleNumber);
+            for (uint32_t d = 0; d < draws; ++d) o[d] = sg.next();
+        } else {
+            UniformSampleGenerator sg(px, py, sampleNumber);
+            for (uint32_t d = 0; d < draws; ++d) o[d] = sg.next();
+        }
+    }
+}
+// TEA encryption of (v0, v1) with a 128-bit key and `iterations` cycles (blockCipherTEA's loop).
+void oracle_tea(uint32_t v0, uint32_t v1, const uint32_t* key, uint32_t iterations, uint32_t* out) {
+    teaEncrypt(v0, v1, key, iterations);
+    out[0] = v0;
+    out[1] = v1;
 }
 uint16_t oracle_f32_to_f16(float f) { return f32_to_f16_bits(f); }
 float oracle_f16_to_f32(uint16_t h) { return f16_bits_to_f32(h); }

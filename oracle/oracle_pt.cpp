@@ -34,9 +34,11 @@ struct OScene {
     uint32_t nprof, ntri;
     OMesh mesh;
     OBvh bvh;
-    // camera (Camera.cpp:187-193)
+    // camera (Camera.cpp:187-193), view matrix rotation rows, polarizer (CameraData.slang:67-68)
     V3<F> camPos, camU, camV, camW;
     float jitterX, jitterY;
+    V3<F> viewRow[3];
+    float polarizer = 0.f, polarizerTheta = 0.f;
     // emissive triangles
     std::vector<uint32_t> lightTri;
     std::vector<float> lightArea, lightPdf;
@@ -77,6 +79,9 @@ struct Options {
     int uniformEmissive = 0;  // emissiveSampler Uniform (else Power)
     int analyticLights = 1;   // useAnalyticLights
     int envLight = 1;         // useEnvLights
+    int sampleGenerator = 1;  // SampleGeneratorType.slangh:32-36: 0 tiny, 1 uniform
+    int perTileSG = 0;        // NEEUsePerTileSG
+    uint32_t tileSize = 512;  // tileSize (the tiles kTile of the per-tile generator)
 };
 constexpr int kMaxMSOccluders = 8;
 
@@ -379,8 +384,13 @@ Beam sourceBeamArea(V3<F> posW, V3<F> dir, float A, float dist, const float* Le,
 }
 
 // ---------------------------------------------------------------- sample pass
-void samplePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, std::vector<Bounce>& rec) {
-    UniformSampleGenerator sg(px, py, o.frame);
+// aov (optional): albedo and view-space normal of the first non-delta vertex (RGBA each), zero when
+// there is none (pltpt_sample.rt.slang:108-112, :162-163).
+template <class S>
+void samplePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, std::vector<Bounce>& rec, float* aov) {
+    S sg(px, py, o.frame);
+    if (aov)
+        for (int c = 0; c < 8; ++c) aov[c] = 0.f;
     const float lambda = Spectrum<float>::sampleOne(sampleNext1D<float>(sg));
     // Camera.computeRayPinhole (Camera.slang:46-78)
     float sx = (float(px) + 0.5f) / float(o.width), sy = (float(py) + 0.5f) / float(o.height);
@@ -422,6 +432,12 @@ void samplePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, std
         }
         const bool isDelta = (bs.lobe & Lobe_Delta) != 0, isDisp = (bs.lobe & Lobe_DispersiveDelta) != 0, isRefl = (bs.lobe & Lobe_Reflection) != 0;
         const bool isMS = (isRefl && (props & Prop_MWR)) || (!isRefl && (props & Prop_MWT));
+        if (aov && !isDelta && (flags & RF_OnlyDelta)) {
+            const V3<F> alb = materialAlbedo(in);
+            const V3<F> N = sd.f.N;
+            aov[0] = alb.x; aov[1] = alb.y; aov[2] = alb.z; aov[3] = 1.f;
+            aov[4] = dot(s.viewRow[0], N); aov[5] = dot(s.viewRow[1], N); aov[6] = dot(s.viewRow[2], N); aov[7] = 1.f;
+        }
         b.tri = h.tri; b.b1 = h.b1; b.b2 = h.b2; b.encWi = ndirToOct(wi); b.lastNonDeltaPdf = lastNonDeltaPdf;
         if (end) {
             flags |= RF_Terminated;
@@ -525,34 +541,42 @@ bool traceMNEEOccluders(const OScene& s, const Options& o, V3<F> org, V3<F> dir,
         tmax -= h.t;
     }
 }
-V2<F> sphcoords(V3<F> w) {  // :314-319
-    const F theta = walkAcos(clampT(w.z, F(-1), F(1)));
-    F phi = walkAtan2(w.y, w.x);
-    phi += phi < F(0) ? K<F>::_2PI : F(0);
+// sphcoords (:314-319) in FP64 of the FP32 direction.
+V2<double> sphcoords(V3<F> w) {
+    const double theta = walkAcos(std::fmin(std::fmax(double(w.z), -1.0), 1.0));
+    double phi = walkAtan2(double(w.y), double(w.x));
+    phi += phi < 0.0 ? 6.28318530717958647692 : 0.0;
     return {theta, phi};
 }
-// Newton step on the angle-difference constraint (:283-312). Matrices as rows (dTheta, dPhi).
+// Newton step on the angle-difference constraint (:283-312), in FP64 (the north-star precision of
+// the step; the reference's is FP32). Matrices as rows (dTheta, dPhi); the step is rounded back
+// to FP32 where the walk moves its vertex.
 bool manifoldOneStep(const PDV<F>& w, const KPDV<F>& ws, const PDV<F>& p, float& lenC, V3<F>& stepDir) {
-    const V2<F> so = sphcoords(w.v), si = sphcoords(ws.v);
-    const F dt = so.x - si.x;
-    F dp = so.y - si.y;
-    dp = std::fabs(dp) > K<F>::PI ? dp - sgn(dp) * K<F>::_2PI : dp;
-    lenC = std::sqrt(dt * dt + dp * dp);
-    auto derivs = [](V3<F> v, V3<F> ddu, V3<F> ddv, F out[2][2]) {  // :321-333
-        const F dAcos = F(-1) / std::sqrt(hmax(F(0), F(1) - sqr(v.z)));
-        const F invNorm = F(1) / (v.x * v.x + v.y * v.y);
-        out[0][0] = dAcos * ddu.z; out[0][1] = dAcos * ddv.z;
-        out[1][0] = invNorm * (v.x * ddu.y - v.y * ddu.x);
-        out[1][1] = invNorm * (v.x * ddv.y - v.y * ddv.x);
+    const V2<double> so = sphcoords(w.v), si = sphcoords(ws.v);
+    const double dt = so.x - si.x;
+    double dp = so.y - si.y;
+    dp = std::fabs(dp) > 3.14159265358979323846 ? dp - (dp > 0.0 ? 6.28318530717958647692 : -6.28318530717958647692) : dp;
+    lenC = float(std::sqrt(dt * dt + dp * dp));
+    auto derivs = [](V3<F> v, V3<F> ddu, V3<F> ddv, double out[2][2]) {  // :321-333
+        const double vx = v.x, vy = v.y, vz = v.z;
+        const double dAcos = -1.0 / std::sqrt(std::fmax(0.0, 1.0 - vz * vz));
+        const double invNorm = 1.0 / (vx * vx + vy * vy);
+        out[0][0] = dAcos * double(ddu.z); out[0][1] = dAcos * double(ddv.z);
+        out[1][0] = invNorm * (vx * double(ddu.y) - vy * double(ddu.x));
+        out[1][1] = invNorm * (vx * double(ddv.y) - vy * double(ddv.x));
     };
-    F a[2][2], b[2][2];
+    double a[2][2], b[2][2];
     derivs(w.v, w.ddu, w.ddv, a);
     derivs(ws.v, ws.ddu, ws.ddv, b);  // the wavelength column of the 2x3 form is not used
-    const M2<F> J = m2<F>(a[0][0] - b[0][0], a[0][1] - b[0][1], a[1][0] - b[1][0], a[1][1] - b[1][1]);
-    M2<F> inv;
-    if (!tryInvert2(J, inv)) return false;
-    const F dx = inv.m[0][0] * dt + inv.m[0][1] * dp, dy = inv.m[1][0] * dt + inv.m[1][1] * dp;
-    stepDir = p.ddu * dx + p.ddv * dy;
+    const double j00 = a[0][0] - b[0][0], j01 = a[0][1] - b[0][1], j10 = a[1][0] - b[1][0], j11 = a[1][1] - b[1][1];
+    // tryInvert (MathHelpers.slang:484-496)
+    const double det = j00 * j11 - j01 * j10;
+    if (std::fabs(det) < 4e-6) return false;
+    const double id = 1.0 / det;
+    const double i00 = j11 * id, i11 = j00 * id, i01 = -j01 * id, i10 = -j10 * id;
+    const double dx = i00 * dt + i01 * dp, dy = i10 * dt + i11 * dp;
+    stepDir = V3<F>{float(double(p.ddu.x) * dx + double(p.ddv.x) * dy), float(double(p.ddu.y) * dx + double(p.ddv.y) * dy),
+                    float(double(p.ddu.z) * dx + double(p.ddv.z) * dy)};
     return true;
 }
 template <class S>
@@ -621,9 +645,21 @@ void manifoldForwardUnwind(const OScene& s, const Options& o, const MSData& d, V
         unwindStep(s, o, h.tri, h.b1, h.b2, &h.posW, normalize(p2 - h.posW), h.lobe, beam);
     }
 }
+// The camera polarizer of measure (pltpt_solve.rt.slang:111-126): the beam rotated to the camera
+// frame (t = U, r = -W), then (1 - p) I + p MuellerPolarizer(theta) on each HWSS component.
+void applyCameraPolarizer(const OScene& s, Beam& beam) {
+    if (!(s.polarizer > .0f)) return;
+    beam.rotateFrame(normalize(s.camU), -normalize(s.camW));
+    const M4<F> Pm = MuellerPolarizer(s.polarizerTheta);
+    M4<F> P;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k) P.m[i][k] = (1.f - s.polarizer) * (i == k ? 1.f : 0.f) + s.polarizer * Pm.m[i][k];
+    for (int j = 0; j < beam.hwss_comps; ++j) beam.setSP(j, mul(P, beam.getSP(j)));
+}
 void measure(const OScene& s, const Options& o, SolveData& sv, int bounceIdx, float pathContribution, Beam& beam) {
     if (!o.directLights && bounceIdx <= 1) return;
     beam.propagate(s.camPos);
+    applyCameraPolarizer(s, beam);
     if (o.hwssMIS) {
         float sum = 0;
         for (int j = 0; j < HW; ++j) sum += beam.spectralPDFs[j];
@@ -636,7 +672,9 @@ void measure(const OScene& s, const Options& o, SolveData& sv, int bounceIdx, fl
 
 template <class S>
 void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, const std::vector<Bounce>& rec, float* out) {
-    UniformSampleGenerator sg(px, py, o.frame);
+    S sg(px, py, o.frame);
+    // the per-tile generator for NEE light-type selection (pltpt_solve.rt.slang:412)
+    S tilesg(px / o.tileSize, py / o.tileSize, o.frame << 2);
     SolveData sv;
     sv.wavelength = Spectrum<float>::sampleOne(sampleNext1D<float>(sg));
     sv.pathOrigin = s.camPos;
@@ -688,8 +726,8 @@ void solvePass(const OScene& s, const Options& o, uint32_t px, uint32_t py, cons
             const bool sampleRefl = (lobes & Lobe_NonDeltaReflection) != 0 || allowMS;
             const bool sampleTrans = (lobes & Lobe_NonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
-                // selectLightType (LightSampling.slangh:57-85): environment (absent), emissive, analytic
-                float u = sampleNext1D<float>(sg);
+                // selectLightType (LightSampling.slangh:57-85), on the per-tile generator if asked (:181-185)
+                float u = o.perTileSG ? sampleNext1D<float>(tilesg) : sampleNext1D<float>(sg);
                 int ltype = -1;
                 float typePdf = 0.f;
                 if (u < pEnv) {
@@ -1061,7 +1099,7 @@ int oracle_scene_set_env(void* h, uint32_t w, uint32_t hgt, const float* rgba, c
 
 // Renders with a scene made by oracle_scene_create (whose arrays the caller keeps alive).
 int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count,
-                        float* color, uint32_t* records, int threads) {
+                        float* color, uint32_t* records, float* aov, int threads) {
     OScene& s = *static_cast<OScene*>(h);
     // Camera.cpp:187-193 in float
     {
@@ -1075,6 +1113,13 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
         V = V * (fd * tn);
         s.camPos = P; s.camU = U; s.camV = V; s.camW = W;
         s.jitterX = cam[12]; s.jitterY = cam[13];
+        s.polarizer = cam[14]; s.polarizerTheta = cam[15];
+        // glm::lookAt (right-handed) rows: s = normalize(f x up), u = s x f, -f (Camera.cpp:135)
+        const V3<F> f = normalize(Tg - P);
+        const V3<F> sr = normalize(cross(f, Up));
+        s.viewRow[0] = sr;
+        s.viewRow[1] = cross(sr, f);
+        s.viewRow[2] = -f;
     }
     Options o;
     o.width = uint32_t(iopt[0]); o.height = uint32_t(iopt[1]); o.frame = uint32_t(iopt[2]); o.maxBounces = uint32_t(iopt[3]);
@@ -1088,6 +1133,9 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
     o.uniformEmissive = iopt[14];
     o.analyticLights = iopt[15];
     o.envLight = iopt[16];
+    o.sampleGenerator = iopt[17];
+    o.perTileSG = iopt[18];
+    o.tileSize = uint32_t(std::max(iopt[19], 1));
     if (threads > 0) omp_set_num_threads(threads);
     const size_t n = count;
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1095,8 +1143,14 @@ int oracle_scene_render(void* h, const float* cam, const int32_t* iopt, const fl
         const uint32_t pid = first + uint32_t(i);
         const uint32_t px = pid % o.width, py = pid / o.width;
         std::vector<Bounce> rec(o.maxBounces + 2);
-        samplePass(s, o, px, py, rec);
-        solvePass<UniformSampleGenerator>(s, o, px, py, rec, color + 4 * size_t(i));
+        float* a = aov ? aov + 8 * size_t(i) : nullptr;
+        if (o.sampleGenerator == 0) {
+            samplePass<TinyUniformSampleGenerator>(s, o, px, py, rec, a);
+            solvePass<TinyUniformSampleGenerator>(s, o, px, py, rec, color + 4 * size_t(i));
+        } else {
+            samplePass<UniformSampleGenerator>(s, o, px, py, rec, a);
+            solvePass<UniformSampleGenerator>(s, o, px, py, rec, color + 4 * size_t(i));
+        }
         if (records) {
             for (uint32_t p = 0; p <= o.maxBounces; ++p) {
                 const Bounce& b = rec[p];
@@ -1122,7 +1176,7 @@ int oracle_render(const float* pos, const float* nrm, uint32_t ntri, const uint3
                   const int32_t* iopt, const float* fopt, uint32_t first, uint32_t count, float* color, uint32_t* records,
                   int threads) {
     void* h = oracle_scene_create(pos, nrm, ntri, idx, triMesh, meshMat, mats, prof, nprof);
-    const int rc = oracle_scene_render(h, cam, iopt, fopt, first, count, color, records, threads);
+    const int rc = oracle_scene_render(h, cam, iopt, fopt, first, count, color, records, nullptr, threads);
     oracle_scene_destroy(h);
     return rc;
 }
@@ -1174,8 +1228,28 @@ void oracle_walk_derivatives(const void* mat, const void* prof, uint32_t nprof, 
     for (int k = 0; k < 3; ++k) { out[3 * k] = o[k].x; out[3 * k + 1] = o[k].y; out[3 * k + 2] = o[k].z; }
 }
 
-// The manifold walk's acos and atan2 (walkAcos, walkAtan2) over arrays.
-void oracle_walk_trig(const float* x, const float* y, uint32_t n, float* acosx, float* atan2yx) {
+// The camera polarizer alone: a beam with direction r, polarisation frame tangent t and Stokes
+// vector S (one wavelength) through measure's polarizer step for a camera with frame vectors U, W.
+void oracle_camera_polarizer(const float* camU, const float* camW, float p, float theta, const float* r, const float* t,
+                             const float* S, float* out) {
+    OScene s{};
+    s.camU = V3<F>{camU[0], camU[1], camU[2]};
+    s.camW = V3<F>{camW[0], camW[1], camW[2]};
+    s.polarizer = p;
+    s.polarizerTheta = theta;
+    Beam b;
+    b.setZero();
+    b.hwss_comps = 1;
+    b.r = V3<F>{r[0], r[1], r[2]};
+    b.t = V3<F>{t[0], t[1], t[2]};
+    b.setSP(0, V4<F>{S[0], S[1], S[2], S[3]});
+    applyCameraPolarizer(s, b);
+    const V4<F> o = b.getSP(0);
+    out[0] = o.x; out[1] = o.y; out[2] = o.z; out[3] = o.w;
+}
+
+// The manifold walk's FP64 acos and atan2 (walkAcos, walkAtan2) over arrays.
+void oracle_walk_trig(const double* x, const double* y, uint32_t n, double* acosx, double* atan2yx) {
     for (uint32_t i = 0; i < n; ++i) {
         acosx[i] = walkAcos(x[i]);
         atan2yx[i] = walkAtan2(y[i], x[i]);

@@ -179,6 +179,40 @@ struct UniformSampleGenerator {  // UniformSampleGenerator.slang:51-69
     }
     uint32_t next() { return rng.next(); }
 };
+// blockCipherTEA (Utils/Math/HashUtils.slang:59-71): the TEA block cipher (Wheeler & Needham 1994)
+// with Falcor's fixed key, 16 cycles by default.
+inline void blockCipherTEA(uint32_t& v0, uint32_t& v1, uint32_t iterations = 16) {
+    uint32_t sum = 0;
+    const uint32_t delta = 0x9e3779b9u;
+    const uint32_t k[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+    for (uint32_t i = 0; i < iterations; i++) {
+        sum += delta;
+        v0 += ((v1 << 4) + k[0]) ^ (v1 + sum) ^ ((v1 >> 5) + k[1]);
+        v1 += ((v0 << 4) + k[2]) ^ (v0 + sum) ^ ((v0 >> 5) + k[3]);
+    }
+}
+// The same cipher with a caller's key (for the published TEA test vectors).
+inline void teaEncrypt(uint32_t& v0, uint32_t& v1, const uint32_t k[4], uint32_t iterations) {
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < iterations; i++) {
+        sum += 0x9e3779b9u;
+        v0 += ((v1 << 4) + k[0]) ^ (v1 + sum) ^ ((v1 >> 5) + k[1]);
+        v1 += ((v0 << 4) + k[2]) ^ (v0 + sum) ^ ((v0 >> 5) + k[3]);
+    }
+}
+struct LCG {  // Utils/Sampling/Pseudorandom/LCG.slang:45-61 (Numerical Recipes constants)
+    uint32_t state;
+    uint32_t next() { state = 1664525u * state + 1013904223u; return state; }
+};
+struct TinyUniformSampleGenerator {  // TinyUniformSampleGenerator.slang:55-66
+    LCG rng;
+    TinyUniformSampleGenerator(uint32_t px, uint32_t py, uint32_t sampleNumber) {
+        uint32_t v0 = interleave_32bit(px, py), v1 = sampleNumber;
+        blockCipherTEA(v0, v1);
+        rng.state = v0;
+    }
+    uint32_t next() { return rng.next(); }
+};
 // sampleNext1D: (bits>>8) * 2^-24, exact in float (SampleGeneratorInterface.slang:46-52)
 template <class T, class S> inline T sampleNext1D(S& sg) { return T(float(sg.next() >> 8) * 0x1p-24f); }
 template <class T, class S> inline V2<T> sampleNext2D(S& sg) { V2<T> r; r.x = sampleNext1D<T>(sg); r.y = sampleNext1D<T>(sg); return r; }

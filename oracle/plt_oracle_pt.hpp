@@ -17,8 +17,9 @@ struct MatData {
     uint16_t base_color[4], specular[4], data1[4], data2[4], transmission[3], ior;
     uint32_t emission, ior_n, ior_k;
 };
-enum : uint32_t { MT_DIELECTRIC = 0, MT_CONDUCTOR = 1, MT_DIFFUSE = 2, MT_OPAQUE = 3, MT_COATED_OPAQUE = 4, MT_GRATED_CONDUCTOR = 5,
-                  MT_COATED_CONDUCTOR = 6, MT_THIN_DIELECTRIC = 7, MT_MULTILAYER = 8 };
+// MaterialType (Scene/Material/MaterialTypes.slang:36-53)
+enum : uint32_t { MT_DIFFUSE = 2, MT_CONDUCTOR = 3, MT_DIELECTRIC = 4, MT_OPAQUE = 5, MT_THIN_DIELECTRIC = 6, MT_COATED_CONDUCTOR = 7,
+                  MT_COATED_OPAQUE = 8, MT_GRATED_CONDUCTOR = 9, MT_GRATED_OPAQUE = 10, MT_MULTILAYER = 11 };
 constexpr uint32_t kNoProfile = 0xFFFFFFFFu;
 inline float hf(uint16_t h) { return f16_bits_to_f32(h); }
 
@@ -568,9 +569,22 @@ template <class T> struct DiffractionGrating {
     }
 };
 
-// PLTDiffractionGratedConductorBxDF (PLTDiffractionGratedConductorBSDF.slang:45-213)
-template <class T> struct GratedConductorBxDF {
-    ConductorBxDF<T> base;
+// The base lobe's share in the diffraction orders: the conductor's colour and Fresnel Mueller
+// matrix (PLTDiffractionGratedConductorBSDF.slang:84-98), the opaque dielectric's albedo and no
+// Mueller matrix (PLTDiffractionGratedOpaqueDielectricBSDF.slang:84-93).
+template <class T> inline T gratingBaseColor(const ConductorBxDF<T>& b, T wavelength) { return Spectrum<T>::rgbToSpectrum(b.specular, wavelength); }
+template <class T> inline T gratingBaseColor(const OpaqueDielectricBxDF<T>& b, T wavelength) { return Spectrum<T>::rgbToSpectrum(b.albedo, wavelength); }
+template <class T> inline V4<T> gratingBaseMueller(const ConductorBxDF<T>& b, V3<T> toLight, T wavelength, V4<T> sp) {
+    const V2<T> ior = {evalSpectrum<T>(b.etaN, wavelength) / b.extEta, evalSpectrum<T>(b.etaK, wavelength) / b.extEta};
+    return mul(MuellerFresnelRConductor(toLight.z, ior), sp);
+}
+template <class T> inline V4<T> gratingBaseMueller(const OpaqueDielectricBxDF<T>&, V3<T>, T, V4<T> sp) { return sp; }
+
+// PLTDiffractionGratedConductorBxDF (PLTDiffractionGratedConductorBSDF.slang:45-213) and
+// PLTDiffractionGratedOpaqueDielectricBxDF (PLTDiffractionGratedOpaqueDielectricBSDF.slang:45-209):
+// the same grating over a conductor or an opaque dielectric base.
+template <class T, class Base> struct GratedBxDF {
+    Base base;
     DiffractionGrating<T> grating;
     static void decode(uint32_t bsdfLobe, int& lx, int& ly) {
         const int x = int(bsdfLobe) >> 8;
@@ -602,13 +616,11 @@ template <class T> struct GratedConductorBxDF {
             beam.scaleEach(a);
         } else if ((lx != 0 || ly != 0) && anyBelow && isDirect) {
             for (int j = 0; j < beam.hwss_comps; ++j) {
-                const T m00 = Spectrum<T>::rgbToSpectrum(base.specular, beam.wavelengths[j]);
-                const V2<T> ior = {evalSpectrum<T>(base.etaN, beam.wavelengths[j]) / base.extEta, evalSpectrum<T>(base.etaK, beam.wavelengths[j]) / base.extEta};
-                const M4<T> M0 = MuellerFresnelRConductor(toLight.z, ior);
+                const T m00 = gratingBaseColor(base, beam.wavelengths[j]);
                 const V3<T> lobeWi = beam.toLocal(grating.diffract(toEye, lx, ly, lam[j]));
                 const T coh = j == 0 ? T(1) : beam.mutualCoherenceAngular(lobeWi, toLightBeam);
                 const T inten = grating.lobeIntensity(lx, ly, toLight, lam[j]);
-                beam.setSP(j, (m00 * (T(1) - a[j]) * coh * inten) * mul(M0, beam.getSP(j)));
+                beam.setSP(j, (m00 * (T(1) - a[j]) * coh * inten) * gratingBaseMueller(base, toLight, beam.wavelengths[j], beam.getSP(j)));
             }
         } else beam.setZero();
     }
@@ -631,7 +643,7 @@ template <class T> struct GratedConductorBxDF {
             pdf *= basePdf;
             return true;
         }
-        weight = Spectrum<T>::rgbToSpectrum(base.specular, wavelength);
+        weight = gratingBaseColor(base, wavelength);  // base.weight: the base's colour spectrum
         T px = 1, py = 1;
         if (!hasHint) grating.sampleLobe(wi, px, py, lx, ly, lam, sg);
         wo = grating.diffract(wi, lx, ly, lam);
@@ -670,7 +682,8 @@ template <class T> struct Instance {
     DiffuseBxDF<T> dif;
     OpaqueDielectricBxDF<T> opq;
     CoatedOpaqueBxDF<T> cot;
-    GratedConductorBxDF<T> grt;
+    GratedBxDF<T, ConductorBxDF<T>> grt;
+    GratedBxDF<T, OpaqueDielectricBxDF<T>> gro;
     CoatedConductorBxDF<T> ccn;
     ThinDielectricBxDF<T> thn;
     MultiLayerBxDF<T> mls;
@@ -734,6 +747,22 @@ template <class T> inline Instance<T> setupMaterialInstance(const MatData& md, c
             in.grt.grating.gy = std::sin(dir);
             break;
         }
+        case MT_GRATED_OPAQUE: {  // PLTDiffractionGratedOpaqueDielectricMaterial.slang:40-70
+            in.gro.base.albedo = base;
+            in.gro.base.gfs = FractalSurface<T>::makeRef(T(hf(md.data2[1])));
+            const float px = hf(md.data1[0]), py = hf(md.data1[1]);
+            in.gro.grating.ipx = T(px > 0.f ? 1.f / px : 0.f);
+            in.gro.grating.ipy = T(py > 0.f ? 1.f / py : 0.f);
+            in.gro.grating.q = T(hf(md.data2[0]) * sx);
+            in.gro.grating.mult = T(sy);
+            const uint32_t tl = md.data1[3];
+            in.gro.grating.lobeCount = std::min(std::max(tl & 0xFFu, 1u), 8u);
+            in.gro.grating.type = tl >> 8;
+            const T dir = T(hf(md.data1[2]));
+            in.gro.grating.gx = std::cos(dir);
+            in.gro.grating.gy = std::sin(dir);
+            break;
+        }
         case MT_COATED_CONDUCTOR:  // PLTCoatedConductorMaterial.slang:40-60
             in.ccn.albedo = base;
             in.ccn.gfs = FractalSurface<T>::makeRef(T(sy));
@@ -773,6 +802,8 @@ template <class T> inline uint32_t getLobes(const Instance<T>& in) {
         case MT_COATED_CONDUCTOR: return in.ccn.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection);
         case MT_THIN_DIELECTRIC: return Lobe_DeltaReflection | Lobe_DeltaTransmission;
         case MT_MULTILAYER: return Lobe_DeltaReflection;
+        case MT_GRATED_OPAQUE:
+            return (in.gro.base.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection)) | Lobe_DispersiveDeltaReflection;
         default:
             return (in.grt.base.gfs.isDeltaOnly() ? Lobe_DeltaReflection : (Lobe_DeltaReflection | Lobe_ScatterReflection)) | Lobe_DispersiveDeltaReflection;
     }
@@ -788,7 +819,29 @@ template <class T> inline uint32_t getPropertyFlags(const Instance<T>& in) {
         case MT_COATED_OPAQUE: return in.cot.gfs.q < cut ? Prop_MWR : 0u;
         case MT_COATED_CONDUCTOR: return in.ccn.gfs.q < cut ? Prop_MWR : 0u;
         case MT_THIN_DIELECTRIC: return Prop_Transmissive | Prop_MWR | Prop_MWT;
-        default: return Prop_MWR;  // grated conductor, multilayer stack
+        default: return Prop_MWR;  // grated conductor and opaque dielectric, multilayer stack
+    }
+}
+
+// BSDFProperties.albedo of each material's getProperties (the sample pass's albedo output):
+// PLTDielectricBSDF.slang:342-346, PLTConductorBSDF.slang:226-231, PLTDiffractionGratedConductorBSDF.slang:273-280,
+// PLTThinDielectricBSDF.slang:259, the base colour elsewhere.
+template <class T> inline V3<T> materialAlbedo(const Instance<T>& in) {
+    const T meanK = K<T>::_2PI / (T(550) * T(1e-3));
+    switch (in.type) {
+        case MT_DIELECTRIC: return in.die.specular * evalFresnelDielectric(in.die.indexOfRefraction(meanK), T(1));
+        case MT_CONDUCTOR:
+            return in.con.specular * evalFresnelConductor(V2<T>{evalSpectrum<T>(in.con.etaN, T(550)), evalSpectrum<T>(in.con.etaK, T(550))}, T(1));
+        case MT_GRATED_CONDUCTOR:
+            return in.grt.base.specular *
+                   evalFresnelConductor(V2<T>{evalSpectrum<T>(in.grt.base.etaN, T(550)), evalSpectrum<T>(in.grt.base.etaK, T(550))}, T(1));
+        case MT_DIFFUSE: return in.dif.albedo;
+        case MT_OPAQUE: return in.opq.albedo;
+        case MT_COATED_OPAQUE: return in.cot.albedo;
+        case MT_COATED_CONDUCTOR: return in.ccn.albedo;
+        case MT_GRATED_OPAQUE: return in.gro.base.albedo;
+        case MT_MULTILAYER: return in.mls.albedo;
+        default: return V3<T>{1, 1, 1};  // thin dielectric
     }
 }
 
@@ -801,9 +854,10 @@ template <class T> struct SD {
 };
 // The grating evaluates in a frame aligned with its direction (DiffractionGrating.slang:72-79).
 template <class T> inline Frame<T> bsdfFrame(const Instance<T>& in, const Frame<T>& sd) {
-    if (in.type != MT_GRATED_CONDUCTOR) return sd;
+    if (in.type != MT_GRATED_CONDUCTOR && in.type != MT_GRATED_OPAQUE) return sd;
+    const DiffractionGrating<T>& gr = in.type == MT_GRATED_CONDUCTOR ? in.grt.grating : in.gro.grating;
     Frame<T> g;
-    g.T_ = sd.fromLocal(V3<T>{in.grt.grating.gx, in.grt.grating.gy, 0});
+    g.T_ = sd.fromLocal(V3<T>{gr.gx, gr.gy, 0});
     g.N = sd.N;
     g.B = cross(g.N, g.T_);
     return g;
@@ -820,6 +874,7 @@ template <class T, int N> inline void materialEval(const Instance<T>& in, const 
         case MT_COATED_CONDUCTOR: bsdfEval<T, N>(in.ccn, f, sd.V, toEye, lobe, beam); break;
         case MT_THIN_DIELECTRIC: bsdfEval<T, N>(in.thn, f, sd.V, toEye, lobe, beam); break;
         case MT_MULTILAYER: bsdfEval<T, N>(in.mls, f, sd.V, toEye, lobe, beam); break;
+        case MT_GRATED_OPAQUE: bsdfEval<T, N>(in.gro, f, sd.V, toEye, lobe, beam); break;
         default: bsdfEval<T, N>(in.grt, f, sd.V, toEye, lobe, beam); break;
     }
 }
@@ -834,6 +889,7 @@ template <class T, int N> inline void materialLambdaPdf(const Instance<T>& in, c
         case MT_COATED_OPAQUE: in.cot.lambdaPdf(a, b, lobe, beam, ret); break;
         case MT_COATED_CONDUCTOR: in.ccn.lambdaPdf(a, b, lobe, beam, ret); break;
         case MT_GRATED_CONDUCTOR: in.grt.lambdaPdf(a, b, lobe, beam, ret); break;
+        case MT_GRATED_OPAQUE: in.gro.lambdaPdf(a, b, lobe, beam, ret); break;
         default: break;
     }
 }
@@ -849,6 +905,7 @@ template <class T> inline T materialPdf(const Instance<T>& in, const SD<T>& sd, 
         case MT_COATED_CONDUCTOR: return in.ccn.pdf(a, b, wavelength);
         case MT_THIN_DIELECTRIC: return in.thn.pdf(a, b, wavelength);
         case MT_MULTILAYER: return in.mls.pdf(a, b, wavelength);
+        case MT_GRATED_OPAQUE: return in.gro.pdf(a, b, wavelength);
         default: return in.grt.pdf(a, b, wavelength);
     }
 }
@@ -866,6 +923,7 @@ template <class S> inline bool materialSample(const Instance<float>& in, const S
         case MT_COATED_CONDUCTOR: valid = in.ccn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         case MT_THIN_DIELECTRIC: valid = in.thn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         case MT_MULTILAYER: valid = in.mls.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
+        case MT_GRATED_OPAQUE: valid = in.gro.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
         default: valid = in.grt.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, Lobe_All); break;
     }
     if (valid) r.wo = f.fromLocal(r.wo);
@@ -875,42 +933,35 @@ template <class S> inline bool materialSample(const Instance<float>& in, const S
 // ------------------------------------------------------------------------------------------
 // Differential geometry for manifold walks (Rendering/Utils/DifferentialShadingGeometry.slang).
 // ------------------------------------------------------------------------------------------
-// Spherical angles of the manifold walk: one fixed single-precision evaluation of asin/acos/atan/
-// atan2 (the Cephes library's single-precision polynomials and range reductions, S. L. Moshier),
-// written with +, -, *, / and sqrt only, so that a strict-IEEE host and the device round every step
-// of the walk identically.
-inline float walkAsin(float xx) {
-    const float a = std::fabs(xx);
-    float x, z;
-    bool flag;
-    if (a < 1e-4f) return xx;
-    if (a > .5f) { z = .5f * (1.f - a); x = std::sqrt(z); flag = true; }
-    else { x = a; z = x * x; flag = false; }
-    float r = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z + 1.6666752422e-1f) * z * x + x;
-    if (flag) { r = r + r; r = 1.5707963267948966192f - r; }
-    return xx < 0.f ? -r : r;
+// Spherical angles of the manifold walk's Newton step, in double precision: the Cephes library's
+// rational approximation of atan and its range reduction (S. L. Moshier, atan.c), atan2 by quadrant
+// and acos(x) = 2 atan(sqrt((1 - x) / (1 + x))), written with +, -, *, / and sqrt only, so that a
+// strict-IEEE host and the device compute the same bits.
+inline double walkAtan(double xx) {
+    double x = std::fabs(xx), y;
+    int flag;
+    if (x > 2.41421356237309504880) { y = 1.57079632679489661923; flag = 1; x = -(1.0 / x); }
+    else if (x <= 0.66) { y = 0.0; flag = 0; }
+    else { y = 0.785398163397448309616; flag = 2; x = (x - 1.0) / (x + 1.0); }
+    const double z = x * x;
+    const double P = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z - 7.500855792314704667340e1) * z -
+                      1.228866684490136173410e2) * z - 6.485021904942025371773e1;
+    const double Q = ((((z + 2.485846490142306297962e1) * z + 1.650270098316988542046e2) * z + 4.328810604912902668951e2) * z +
+                      4.853903996359136964868e2) * z + 1.945506571482613964425e2;
+    double r = x * (z * P / Q) + x;
+    if (flag == 2) r += 0.5 * 6.123233995736765886130e-17;
+    else if (flag == 1) r += 6.123233995736765886130e-17;
+    y += r;
+    return xx < 0.0 ? -y : y;
 }
-inline float walkAcos(float x) {
-    if (x < -.5f) return 3.14159265358979323846f - 2.f * walkAsin(std::sqrt(.5f * (1.f + x)));
-    if (x > .5f) return 2.f * walkAsin(std::sqrt(.5f * (1.f - x)));
-    return 1.5707963267948966192f - walkAsin(x);
-}
-inline float walkAtan(float xx) {
-    float x = std::fabs(xx), y;
-    if (x > 2.414213562373095f) { y = 1.5707963267948966192f; x = -(1.f / x); }
-    else if (x > .4142135623730950f) { y = .7853981633974483096f; x = (x - 1.f) / (x + 1.f); }
-    else y = 0.f;
-    const float z = x * x;
-    y += (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x;
-    return xx < 0.f ? -y : y;
-}
-inline float walkAtan2(float y, float x) {
-    const int code = (x < 0.f ? 2 : 0) | (y < 0.f ? 1 : 0);
-    if (x == 0.f) return (code & 1) ? -1.5707963267948966192f : (y == 0.f ? 0.f : 1.5707963267948966192f);
-    if (y == 0.f) return (code & 2) ? 3.14159265358979323846f : 0.f;
-    const float w = code == 2 ? 3.14159265358979323846f : code == 3 ? -3.14159265358979323846f : 0.f;
+inline double walkAtan2(double y, double x) {
+    const int code = (x < 0.0 ? 2 : 0) | (y < 0.0 ? 1 : 0);
+    if (x == 0.0) return (code & 1) ? -1.57079632679489661923 : (y == 0.0 ? 0.0 : 1.57079632679489661923);
+    if (y == 0.0) return (code & 2) ? 3.14159265358979323846 : 0.0;
+    const double w = code == 2 ? 3.14159265358979323846 : code == 3 ? -3.14159265358979323846 : 0.0;
     return w + walkAtan(y / x);
 }
+inline double walkAcos(double x) { return 2.0 * walkAtan(std::sqrt((1.0 - x) / (1.0 + x))); }
 
 template <class T> struct PDV {
     V3<T> v, ddu, ddv;
@@ -1041,11 +1092,12 @@ template <class T> inline KPDV<T> materialPartialDerivatives(const Instance<T>& 
         case MT_THIN_DIELECTRIC:  // PLTThinDielectricBSDF.slang:151-162
             if (isTransmission) { PDV<T> w = wi; w.flip(); KPDV<T> r; r.v = w.v; r.ddu = w.ddu; r.ddv = w.ddv; return r; }
             return reflectDerivatives(wi, dsg.n);
-        case MT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:104-112
+        case MT_GRATED_CONDUCTOR:     // PLTDiffractionGratedConductorBSDF.slang:104-112
+        case MT_GRATED_OPAQUE: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:101-107
             int lx, ly;
-            GratedConductorBxDF<T>::decode(lobe, lx, ly);
+            GratedBxDF<T, ConductorBxDF<T>>::decode(lobe, lx, ly);
             if (lx == 0 && ly == 0) return reflectDerivatives(wi, dsg.n);
-            return gratingDerivatives(in.grt.grating, dsg, wi, lx, ly, wavelength);
+            return gratingDerivatives(in.type == MT_GRATED_CONDUCTOR ? in.grt.grating : in.gro.grating, dsg, wi, lx, ly, wavelength);
         }
         default: return reflectDerivatives(wi, dsg.n);  // conductor, opaque, coated, multilayer stack
     }
@@ -1068,6 +1120,7 @@ template <class S> inline bool materialSampleHinted(const Instance<float>& in, c
             return valid;
         case MT_THIN_DIELECTRIC: valid = in.thn.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, forced); break;
         case MT_GRATED_CONDUCTOR: valid = in.grt.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask, f.toLocal(woHint)); break;
+        case MT_GRATED_OPAQUE: valid = in.gro.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask, f.toLocal(woHint)); break;
         case MT_CONDUCTOR: valid = in.con.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;
         case MT_DIFFUSE: valid = in.dif.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); r.wo = f.fromLocal(r.wo); return valid;
         case MT_OPAQUE: valid = in.opq.sample(wi, r.wo, r.pdf, r.weight, r.lobe, sg, wavelength, lobeMask); break;

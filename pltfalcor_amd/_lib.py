@@ -15,10 +15,11 @@ LIB_PATH = _PKG / "libplt_amd.so"
 Q_WI_X, Q_WI_Y, Q_WI_Z, Q_WO_X, Q_WO_Y, Q_WO_Z, Q_LAMBDA, Q_MAT, Q_PARAM, Q_ROUGH_Q, Q_ROUGH_L, Q_COH_AREA, \
     Q_S0, Q_S1, Q_S2, Q_S3, Q_PLANES = range(17)
 R_S0, R_S1, R_S2, R_S3, R_PDF, R_PLANES = range(6)
-MAT_DIELECTRIC, MAT_CONDUCTOR = 0, 1
+MAT_CONDUCTOR, MAT_DIELECTRIC = 3, 4  # MaterialType values (MaterialTypes.slang:36-53)
 
 PLT_OK = 0
 PLT_WARN_UNKNOWN_KEY = 1
+PLT_EINVAL, PLT_ENOMEM, PLT_EHIP, PLT_ENOTIMPL, PLT_ESTATE = -1, -2, -3, -4, -5
 STATUS_NAMES = {0: "PLT_OK", 1: "PLT_WARN_UNKNOWN_KEY", -1: "PLT_EINVAL", -2: "PLT_ENOMEM", -3: "PLT_EHIP",
                 -4: "PLT_ENOTIMPL", -5: "PLT_ESTATE"}
 
@@ -52,7 +53,8 @@ class Material(C.Structure):
 
 class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov_y", C.c_float),
-                ("aspect", C.c_float), ("focal_distance", C.c_float), ("jitter", C.c_float * 2)]
+                ("aspect", C.c_float), ("focal_distance", C.c_float), ("jitter", C.c_float * 2),
+                ("polarizer", C.c_float), ("polarizer_theta", C.c_float)]
 
 
 class ProceduralSceneDesc(C.Structure):
@@ -98,6 +100,9 @@ SIGNATURES = [
     ("plt_context_set_spectral_profiles", C.c_int, [C.c_void_p, C.POINTER(SpectralProfile), C.c_uint32]),
     ("plt_sample_generator_stream", C.c_int,
      [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("plt_sample_generator_stream_typed", C.c_int,
+     [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("plt_debug_fail_upload", C.c_int, [C.c_int]),
     ("plt_bsdf_generate_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     ("plt_bsdf_eval_queries", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("plt_scene_create_procedural", C.c_int, [C.c_void_p, C.POINTER(ProceduralSceneDesc), C.POINTER(C.c_void_p)]),

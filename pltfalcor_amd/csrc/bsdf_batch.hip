@@ -149,10 +149,10 @@ __global__ void __launch_bounds__(kBlock) eval_queries_kernel(uint32_t n, const 
 }
 
 __global__ void __launch_bounds__(kBlock) sample_generator_kernel(uint32_t n, uint32_t width, uint32_t sn, uint32_t draws,
-                                                                    uint32_t* __restrict__ out) {
+                                                                    bool tiny, uint32_t* __restrict__ out) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    Rng g = Rng::make(s % width, s / width, sn);
+    Rng g = Rng::make(s % width, s / width, sn, tiny);
     for (uint32_t d = 0; d < draws; ++d) out[size_t(d) * n + s] = g.next();
 }
 
@@ -169,9 +169,9 @@ int launch_eval_queries(uint32_t n, const uint32_t* q, float* res, const plt_spe
     hipLaunchKernelGGL(eval_queries_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, q, res, prof, nprof);
     return check_launch("eval_queries_kernel");
 }
-int launch_sample_generator(uint32_t n, uint32_t width, uint32_t sn, uint32_t draws, uint32_t* out, hipStream_t st) {
+int launch_sample_generator(uint32_t n, uint32_t width, uint32_t sn, uint32_t draws, bool tiny, uint32_t* out, hipStream_t st) {
     if (n == 0 || draws == 0) return PLT_OK;
-    hipLaunchKernelGGL(sample_generator_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, width, sn, draws, out);
+    hipLaunchKernelGGL(sample_generator_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, width, sn, draws, tiny, out);
     return check_launch("sample_generator_kernel");
 }
 

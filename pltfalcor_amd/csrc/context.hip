@@ -168,11 +168,17 @@ int plt_context_set_spectral_profiles(plt_context* ctx, const plt_spectral_profi
     return PLT_OK;
 }
 
-int plt_sample_generator_stream(uint32_t n_streams, uint32_t width, uint32_t sample_number, uint32_t draws, uint32_t* out,
-                                plt_stream stream) {
+int plt_sample_generator_stream_typed(int type, uint32_t n_streams, uint32_t width, uint32_t sample_number, uint32_t draws,
+                                      uint32_t* out, plt_stream stream) {
+    if (type != 0 && type != 1) return set_error(PLT_EINVAL, "plt_sample_generator_stream: type must be 0 (tiny) or 1 (uniform)");
     if (width == 0 || (n_streams && draws && !out)) return set_error(PLT_EINVAL, "plt_sample_generator_stream: bad arguments");
     if (uint64_t(n_streams) * draws > (uint64_t(1) << 34)) return set_error(PLT_EINVAL, "plt_sample_generator_stream: too large");
-    return plt::launch_sample_generator(n_streams, width, sample_number, draws, out, static_cast<hipStream_t>(stream));
+    return plt::launch_sample_generator(n_streams, width, sample_number, draws, type == 0, out, static_cast<hipStream_t>(stream));
+}
+
+int plt_sample_generator_stream(uint32_t n_streams, uint32_t width, uint32_t sample_number, uint32_t draws, uint32_t* out,
+                                plt_stream stream) {
+    return plt_sample_generator_stream_typed(1, n_streams, width, sample_number, draws, out, stream);
 }
 
 int plt_bsdf_generate_queries(plt_context* ctx, uint32_t n, uint32_t seed, uint32_t* queries, plt_stream stream) {

@@ -5,9 +5,9 @@
 // angle-difference constraint.
 //
 // Every expression is evaluated uncontracted, with correctly rounded division and square root, in
-// the operation order of a plain strict-IEEE evaluation, and the spherical angles use the fixed
-// polynomial asin/atan below: the walk's vertices then round identically on the host and the
-// device. That matters because the walk's last visibility ray ends on the light's own surface,
+// the operation order of a plain strict-IEEE evaluation, and the Newton step's spherical angles use
+// the fixed rational atan below (FP64): the walk's vertices then round identically on the host and
+// the device. That matters because the walk's last visibility ray ends on the light's own surface,
 // where an ulp decides whether the light's triangle occludes it.
 #pragma once
 #include "plt_materials.hpp"
@@ -15,45 +15,40 @@
 namespace plt {
 
 // ---------------------------------------------------------------------------------------
-// asin / acos / atan / atan2 in single precision: the Cephes library's polynomials and range
-// reductions (S. L. Moshier); acos within 1.3 ulp, atan2 within 3.1 ulp of the exact value.
+// atan / atan2 / acos in double precision for the Newton step: the Cephes library's rational
+// approximation of atan with its range reduction (S. L. Moshier, atan.c; within 1 ulp), atan2 by
+// quadrant, acos(x) = 2 atan(sqrt((1 - x) / (1 + x))). Written with +, -, *, / and sqrt only
+// (correctly rounded in double on both sides), so host and device compute the same bits.
 // ---------------------------------------------------------------------------------------
-PLT_DEV float walk_asin(float xx) {
+PLT_DEV double walk_atan(double xx) {
 #pragma clang fp contract(off)
-    const float a = fabsf(xx);
-    if (a < 1e-4f) return xx;
-    float x, z;
-    bool flag;
-    if (a > .5f) { z = .5f * (1.f - a); x = sqrtf(z); flag = true; }
-    else { x = a; z = x * x; flag = false; }
-    float r = ((((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
-               1.6666752422e-1f) * z * x + x;
-    if (flag) { r = r + r; r = 1.5707963267948966192f - r; }
-    return xx < 0.f ? -r : r;
+    double x = fabs(xx), y;
+    int flag;
+    if (x > 2.41421356237309504880) { y = 1.57079632679489661923; flag = 1; x = -(1.0 / x); }
+    else if (x <= 0.66) { y = 0.0; flag = 0; }
+    else { y = 0.785398163397448309616; flag = 2; x = (x - 1.0) / (x + 1.0); }
+    const double z = x * x;
+    const double P = (((-8.750608600031904122785e-1 * z - 1.615753718733365076637e1) * z - 7.500855792314704667340e1) * z -
+                      1.228866684490136173410e2) * z - 6.485021904942025371773e1;
+    const double Q = ((((z + 2.485846490142306297962e1) * z + 1.650270098316988542046e2) * z + 4.328810604912902668951e2) * z +
+                      4.853903996359136964868e2) * z + 1.945506571482613964425e2;
+    double r = x * (z * P / Q) + x;
+    if (flag == 2) r += 0.5 * 6.123233995736765886130e-17;
+    else if (flag == 1) r += 6.123233995736765886130e-17;
+    y += r;
+    return xx < 0.0 ? -y : y;
 }
-PLT_DEV float walk_acos(float x) {
+PLT_DEV double walk_atan2(double y, double x) {
 #pragma clang fp contract(off)
-    if (x < -.5f) return 3.14159265358979323846f - 2.f * walk_asin(sqrtf(.5f * (1.f + x)));
-    if (x > .5f) return 2.f * walk_asin(sqrtf(.5f * (1.f - x)));
-    return 1.5707963267948966192f - walk_asin(x);
-}
-PLT_DEV float walk_atan(float xx) {
-#pragma clang fp contract(off)
-    float x = fabsf(xx), y;
-    if (x > 2.414213562373095f) { y = 1.5707963267948966192f; x = -(1.f / x); }
-    else if (x > .4142135623730950f) { y = .7853981633974483096f; x = (x - 1.f) / (x + 1.f); }
-    else y = 0.f;
-    const float z = x * x;
-    y += (((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z * x + x;
-    return xx < 0.f ? -y : y;
-}
-PLT_DEV float walk_atan2(float y, float x) {
-#pragma clang fp contract(off)
-    const int code = (x < 0.f ? 2 : 0) | (y < 0.f ? 1 : 0);
-    if (x == 0.f) return (code & 1) ? -1.5707963267948966192f : (y == 0.f ? 0.f : 1.5707963267948966192f);
-    if (y == 0.f) return (code & 2) ? 3.14159265358979323846f : 0.f;
-    const float w = code == 2 ? 3.14159265358979323846f : code == 3 ? -3.14159265358979323846f : 0.f;
+    const int code = (x < 0.0 ? 2 : 0) | (y < 0.0 ? 1 : 0);
+    if (x == 0.0) return (code & 1) ? -1.57079632679489661923 : (y == 0.0 ? 0.0 : 1.57079632679489661923);
+    if (y == 0.0) return (code & 2) ? 3.14159265358979323846 : 0.0;
+    const double w = code == 2 ? 3.14159265358979323846 : code == 3 ? -3.14159265358979323846 : 0.0;
     return w + walk_atan(y / x);
+}
+PLT_DEV double walk_acos(double x) {
+#pragma clang fp contract(off)
+    return 2.0 * walk_atan(sqrt((1.0 - x) / (1.0 + x)));
 }
 
 // Strict-order vector helpers of the walk (division by the length, as a plain evaluation does).
@@ -234,7 +229,8 @@ PLT_DEV PDV mat_partial_derivatives(const Mat& m, const PDV& wi, const DSG& dsg,
             return PDV{v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f), v3(0.f, 0.f, 0.f)};
         case PLT_MAT_THIN_DIELECTRIC:  // PLTThinDielectricBSDF.slang:151-162
             return isTransmission ? pdv_flip(wi) : reflect_derivatives(wi, dsg.n);
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:104-112
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorBSDF.slang:104-112
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:101-107
             int lx, ly;
             decode_order(lobe, lx, ly);
             if (lx == 0 && ly == 0) return reflect_derivatives(wi, dsg.n);
@@ -350,7 +346,8 @@ PLT_DEV bool mat_sample_delta_hinted(const Mat& m, const Frame& sd, V3 sdV, Rng&
             valid = wi.z * wo.z > 0.f;
             break;
         }
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:115-166 with a hint
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorBSDF.slang:115-166 with a hint
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:110-146
             const float lam = 1e-3f * lambda;
             const float a = m.gr.alpha(wi.z, k2Pi / lam);
             const V3 hint = f.to_local(hintW);
@@ -379,42 +376,46 @@ PLT_DEV bool mat_sample_delta_hinted(const Mat& m, const Frame& sd, V3 sdV, Rng&
 
 // manifoldOneStep (Manifolds.slangh:283-333): the angle-difference constraint between the current
 // direction w and the specular direction ws in spherical coordinates, and the Newton step of the
-// vertex position along the surface.
-PLT_DEV void walk_sph(V3 w, float& theta, float& phi) {
+// vertex position along the surface. The step runs in FP64 (the directions and their derivatives
+// widened from FP32): the constraint, its 2 x 2 Jacobian, the inverse and the step; the result is
+// rounded back to FP32 where the walk moves the vertex.
+PLT_DEV void walk_sph(V3 w, double& theta, double& phi) {
 #pragma clang fp contract(off)
-    theta = walk_acos(fminf(fmaxf(w.z, -1.f), 1.f));
-    phi = walk_atan2(w.y, w.x);
-    phi += phi < 0.f ? k2Pi : 0.f;
+    theta = walk_acos(fmin(fmax(double(w.z), -1.0), 1.0));
+    phi = walk_atan2(double(w.y), double(w.x));
+    phi += phi < 0.0 ? 6.28318530717958647692 : 0.0;
 }
-PLT_DEV void walk_sph_derivs(V3 v, V3 ddu, V3 ddv, float out[4]) {
+PLT_DEV void walk_sph_derivs(V3 v, V3 ddu, V3 ddv, double out[4]) {
 #pragma clang fp contract(off)
-    const float dAcos = -1.f / sqrtf(fmaxf(0.f, 1.f - v.z * v.z));
-    const float invNorm = 1.f / (v.x * v.x + v.y * v.y);
-    out[0] = dAcos * ddu.z;
-    out[1] = dAcos * ddv.z;
-    out[2] = invNorm * (v.x * ddu.y - v.y * ddu.x);
-    out[3] = invNorm * (v.x * ddv.y - v.y * ddv.x);
+    const double vx = v.x, vy = v.y, vz = v.z;
+    const double dAcos = -1.0 / sqrt(fmax(0.0, 1.0 - vz * vz));
+    const double invNorm = 1.0 / (vx * vx + vy * vy);
+    out[0] = dAcos * double(ddu.z);
+    out[1] = dAcos * double(ddv.z);
+    out[2] = invNorm * (vx * double(ddu.y) - vy * double(ddu.x));
+    out[3] = invNorm * (vx * double(ddv.y) - vy * double(ddv.x));
 }
 PLT_DEV bool manifold_one_step(const PDV& w, const PDV& ws, const PDV& p, float& lenC, V3& stepDir) {
 #pragma clang fp contract(off)
-    float to, po, ti, pi;
+    double to, po, ti, pi;
     walk_sph(w.v, to, po);
     walk_sph(ws.v, ti, pi);
-    const float dt = to - ti;
-    float dp = po - pi;
-    dp = fabsf(dp) > kPi ? dp - sgnf(dp) * k2Pi : dp;
-    lenC = sqrtf(dt * dt + dp * dp);
-    float a[4], b[4];
+    const double dt = to - ti;
+    double dp = po - pi;
+    dp = fabs(dp) > 3.14159265358979323846 ? dp - (dp > 0.0 ? 6.28318530717958647692 : -6.28318530717958647692) : dp;
+    lenC = float(sqrt(dt * dt + dp * dp));
+    double a[4], b[4];
     walk_sph_derivs(w.v, w.ddu, w.ddv, a);
     walk_sph_derivs(ws.v, ws.ddu, ws.ddv, b);
-    const float j00 = a[0] - b[0], j01 = a[1] - b[1], j10 = a[2] - b[2], j11 = a[3] - b[3];
+    const double j00 = a[0] - b[0], j01 = a[1] - b[1], j10 = a[2] - b[2], j11 = a[3] - b[3];
     // tryInvert (MathHelpers.slang:484-496)
-    const float det = j00 * j11 - j01 * j10;
-    if (fabsf(det) < 4e-6f) return false;
-    const float id = 1.f / det;
-    const float i00 = j11 * id, i11 = j00 * id, i01 = -j01 * id, i10 = -j10 * id;
-    const float dx = i00 * dt + i01 * dp, dy = i10 * dt + i11 * dp;
-    stepDir = wmad2(p.ddu, dx, p.ddv, dy);
+    const double det = j00 * j11 - j01 * j10;
+    if (fabs(det) < 4e-6) return false;
+    const double id = 1.0 / det;
+    const double i00 = j11 * id, i11 = j00 * id, i01 = -j01 * id, i10 = -j10 * id;
+    const double dx = i00 * dt + i01 * dp, dy = i10 * dt + i11 * dp;
+    stepDir = v3(float(double(p.ddu.x) * dx + double(p.ddv.x) * dy), float(double(p.ddu.y) * dx + double(p.ddv.y) * dy),
+                 float(double(p.ddu.z) * dx + double(p.ddv.z) * dy));
     return true;
 }
 

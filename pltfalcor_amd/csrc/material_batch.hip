@@ -20,7 +20,7 @@ material_eval_kernel(uint32_t n, const float* __restrict__ q, float* __restrict_
     const uint32_t mi = __float_as_uint(Q(PLT_MQ_MAT));
     const uint32_t lobe = __float_as_uint(Q(PLT_MQ_LOBE));
     float* out = r + i;
-    if (mi >= nmats) {  // unknown material: zeros
+    if (mi >= nmats || !material_supported(mats[mi])) {  // unknown material or radial grating: zeros
         for (int p = 0; p < PLT_MR_PLANES; ++p) out[size_t(p) * n] = 0.f;
         return;
     }
@@ -71,7 +71,7 @@ material_sample_kernel(uint32_t n, uint32_t frame, const float* __restrict__ q, 
     float* out = r + i;
     BsdfSample bs{v3(0.f, 0.f, 0.f), 0.f, 0.f, 0u};
     bool valid = false;
-    if (mi < nmats) {
+    if (mi < nmats && material_supported(mats[mi])) {
         const Mat m = setup_material(mats[mi], prof, nprof);
         Frame sd;
         sd.T = v3(1.f, 0.f, 0.f);

@@ -28,11 +28,11 @@ struct OptionSpec {
     const char* key;
     double def;
 };
-// Keys and defaults of PLTPT (PLTPT.h:74-108). sampleGenerator 0 = uniform (xoshiro128**),
-// emissiveSampler 2 = power (EmissiveLightSamplerType: Uniform 0, LightBVH 1, Power 2;
-// EmissiveLightSamplerType.slangh:36-42).
+// Keys and defaults of PLTPT (PLTPT.h:74-108). sampleGenerator 1 = SAMPLE_GENERATOR_DEFAULT =
+// uniform (xoshiro128**), 0 = tiny (SampleGeneratorType.slangh:32-36); emissiveSampler 2 = power
+// (EmissiveLightSamplerType: Uniform 0, LightBVH 1, Power 2; EmissiveLightSamplerType.slangh:36-42).
 const OptionSpec kOptions[] = {
-    {"maxBounces", 32}, {"tileSize", 512}, {"HWSS", 4}, {"HWSSDoMIS", 1}, {"sampleGenerator", 0},
+    {"maxBounces", 32}, {"tileSize", 512}, {"HWSS", 4}, {"HWSSDoMIS", 1}, {"sampleGenerator", 1},
     {"debugView", 0}, {"debugViewIntensity", 1}, {"sourcingAreaFromEmissiveGeometry", 1},
     {"sourcingMaxBeamOmega", .025}, {"useDirectLights", 1}, {"useEnvLights", 1}, {"useEmissiveLights", 1},
     {"useAnalyticLights", 1}, {"alphaMasking", 1}, {"doNEE", 1}, {"doMIS", 1}, {"emissiveSampler", 2},
@@ -123,9 +123,15 @@ LightTables build_lights(const HostScene& s) {
     return t;
 }
 
+// Host-side fault injection for tests of the scene-binding error path: when >= 0, the upload that
+// finds it at 0 fails as an allocation failure would (it counts down once per upload).
+int g_fail_upload_countdown = -1;
+
 template <typename T>
 int upload_vec(const std::vector<T>& v, T** d, std::vector<void*>& owned) {
     *d = nullptr;
+    if (g_fail_upload_countdown >= 0 && g_fail_upload_countdown-- == 0)
+        return set_error(PLT_ENOMEM, "pass: scene upload failed (injected)");
     if (v.empty()) return PLT_OK;
     void* p = nullptr;
     if (hipMalloc(&p, v.size() * sizeof(T)) != hipSuccess) return set_error(PLT_ENOMEM, "pass: hipMalloc failed");
@@ -160,8 +166,11 @@ struct plt_pass {
     float* d_env_imp = nullptr;
     uint32_t* d_records = nullptr;
     size_t records_words = 0;
+    float* d_tile_draws = nullptr;  // NEEUsePerTileSG draws (tiles x (maxBounces + 1))
+    size_t tile_draws_cap = 0;
     size_t last_words = 0;  // record words written by the last execute
     float cam[12] = {};
+    float view_rot[9] = {};
     // kernel timing: three events per execute (before sample, between, after solve)
     bool profiling = false;
     std::vector<hipEvent_t> events;
@@ -194,6 +203,23 @@ void free_scene_state(plt_pass* p) {
     p->d_env_rgba = nullptr;
     p->d_env_imp = nullptr;
 }
+// The rotation of the camera's view matrix, rmcv::lookAt = glm::lookAt, right-handed
+// (Camera.cpp:135, Utils/Math/Matrix/Matrix.h): rows s = normalize(f x up), u = s x f, -f with
+// f = normalize(target - pos).
+void view_rotation(const plt_camera& c, float out[9]) {
+    float f[3] = {c.target[0] - c.pos[0], c.target[1] - c.pos[1], c.target[2] - c.pos[2]};
+    const float fl = 1.f / std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (float& x : f) x *= fl;
+    float sv[3] = {f[1] * c.up[2] - f[2] * c.up[1], f[2] * c.up[0] - f[0] * c.up[2], f[0] * c.up[1] - f[1] * c.up[0]};
+    const float sl = 1.f / std::sqrt(sv[0] * sv[0] + sv[1] * sv[1] + sv[2] * sv[2]);
+    for (float& x : sv) x *= sl;
+    const float u[3] = {sv[1] * f[2] - sv[2] * f[1], sv[2] * f[0] - sv[0] * f[2], sv[0] * f[1] - sv[1] * f[0]};
+    for (int a = 0; a < 3; ++a) {
+        out[a] = sv[a];
+        out[3 + a] = u[a];
+        out[6 + a] = -f[a];
+    }
+}
 // Camera frame (Camera.cpp:187-193).
 void camera_frame(const plt_camera& c, float out[12]) {
     auto norm = [](float v[3]) { const float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); v[0] /= l; v[1] /= l; v[2] /= l; };
@@ -213,10 +239,16 @@ void camera_frame(const plt_camera& c, float out[12]) {
         out[9 + a] = W[a];
     }
 }
+int bind_scene(plt_pass* p, plt_scene* scene);
 }  // namespace
 }  // namespace plt
 
 extern "C" {
+
+int plt_debug_fail_upload(int n) {
+    plt::g_fail_upload_countdown = n < 0 ? -1 : n;
+    return PLT_OK;
+}
 
 int plt_pass_create(plt_context* ctx, const char* type, plt_pass** out) {
     if (!out) return set_error(PLT_EINVAL, "plt_pass_create: bad arguments");
@@ -233,6 +265,7 @@ int plt_pass_destroy(plt_pass* p) {
     if (!p) return PLT_OK;
     plt::free_scene_state(p);
     if (p->d_records) (void)hipFree(p->d_records);
+    if (p->d_tile_draws) (void)hipFree(p->d_tile_draws);
     if (p->d_wf) (void)hipFree(p->d_wf);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     delete p;
@@ -272,11 +305,27 @@ const char* plt_pass_option_name(uint32_t i) {
 int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     if (!p) return set_error(PLT_EINVAL, "plt_pass_set_scene: bad arguments");
     plt::free_scene_state(p);
-    p->scene = scene;
+    p->scene = nullptr;
     p->frame = 0;  // PLTPT.cpp:510
     p->pattern.configure(p->pattern.type, p->pattern.count);  // a new generator (GBufferBase.cpp:185-189)
     if (!scene) return PLT_OK;
     if (!scene->ctx || !p->ctx) return set_error(PLT_ESTATE, "plt_pass_set_scene: host-only scene or pass");
+    // the scene is bound only once every buffer is on the device: a failed upload leaves the pass
+    // unbound (execute then returns PLT_ESTATE) with nothing allocated
+    const int rc = plt::bind_scene(p, scene);
+    if (rc < 0) {
+        plt::free_scene_state(p);
+        return rc;
+    }
+    p->scene = scene;
+    return PLT_OK;
+}
+
+}  // extern "C"
+
+namespace plt {
+namespace {
+int bind_scene(plt_pass* p, plt_scene* scene) {
     const plt::LightTables lt = plt::build_lights(scene->host);
     int rc;
     if ((rc = plt::upload_vec(scene->host.materials, &p->d_materials, p->owned)) < 0) return rc;
@@ -294,8 +343,13 @@ int plt_pass_set_scene(plt_pass* p, plt_scene* scene) {
     // Scene::useEmissiveLights requires an active triangle; otherwise emissive lighting is off
     p->n_lights = p->n_active > 0 ? uint32_t(lt.pdf.size()) : 0;
     plt::camera_frame(scene->host.camera, p->cam);
+    plt::view_rotation(scene->host.camera, p->view_rot);
     return PLT_OK;
 }
+}  // namespace
+}  // namespace plt
+
+extern "C" {
 
 int plt_pass_set_sample_pattern(plt_pass* p, int pattern, uint32_t sample_count) {
     if (!p) return set_error(PLT_EINVAL, "plt_pass_set_sample_pattern: NULL pass");
@@ -327,13 +381,16 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         return set_error(PLT_EINVAL, "MNEEMaxOccluders and MNEEMaxIterations must be non-negative (iterations <= 1e6), "
                                      "MNEESolverThreshold finite");
     if (mneeOcc > plt::kMaxMsOccluders) return set_error(PLT_ENOTIMPL, "MNEEMaxOccluders above 8 is not built");
-    if (o("NEEUsePerTileSG") != 0) return set_error(PLT_ENOTIMPL, "NEEUsePerTileSG=1 is not built");
     // EmissiveLightSamplerType: Uniform 0, Power 2; the pass creates no other sampler (PLTPT.cpp:399-412)
     const double emissiveSampler = o("emissiveSampler");
     if (emissiveSampler != 0 && emissiveSampler != 2)
         return set_error(PLT_ENOTIMPL, "emissiveSampler must be 0 (Uniform) or 2 (Power); the reference's pass "
                                        "creates no other sampler");
-    if (o("sampleGenerator") != 0) return set_error(PLT_ENOTIMPL, "only the uniform sample generator (0) is built");
+    // SampleGeneratorType.slangh:32-36: 0 = tiny uniform (LCG), 1 = uniform (xoshiro128**)
+    const double sgType = o("sampleGenerator");
+    if (sgType != 0 && sgType != 1) return set_error(PLT_EINVAL, "sampleGenerator must be 0 (tiny uniform) or 1 (uniform)");
+    const double tileSize = o("tileSize");
+    if (!(tileSize >= 1) || tileSize > 65536) return set_error(PLT_EINVAL, "tileSize must be in [1, 65536]");
     if (o("debugView") != 0) return set_error(PLT_ENOTIMPL, "debug views are not built");
     const uint32_t hwss = uint32_t(o("HWSS"));
     if (hwss != 1 && hwss != 4) return set_error(PLT_ENOTIMPL, "HWSS must be 1 or 4");
@@ -370,6 +427,9 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     }
     sc.jitter[0] = s->host.camera.jitter[0];
     sc.jitter[1] = s->host.camera.jitter[1];
+    for (int a = 0; a < 9; ++a) sc.view_rot[a] = p->view_rot[a];
+    sc.polarizer = s->host.camera.polarizer;
+    sc.polarizer_theta = s->host.camera.polarizer_theta;
     if (p->pattern.type != PLT_SAMPLE_PATTERN_CENTER) {
         // Camera::beginFrame: jitter = next() * (1 / frame size) (Camera.cpp:63-70, GBufferBase.cpp:221-229)
         float j[2];
@@ -428,6 +488,25 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     fr.mnee_threshold = float(mneeThr);
     fr.path_base = 0;
     fr.n_paths = uint32_t(npaths);
+    fr.sample_generator = uint32_t(sgType);
+    fr.tile_draws = nullptr;
+    fr.tile_size = uint32_t(tileSize);
+    fr.tiles_x = (rd->width + fr.tile_size - 1) / fr.tile_size;
+    const uint32_t tiles = fr.tiles_x * ((rd->height + fr.tile_size - 1) / fr.tile_size);
+    if (o("NEEUsePerTileSG") != 0 && (opts & plt::kOptNEE)) {
+        const size_t need = size_t(tiles) * (maxb + 1);
+        if (need > p->tile_draws_cap) {
+            if (p->d_tile_draws) (void)hipFree(p->d_tile_draws);
+            p->d_tile_draws = nullptr;
+            p->tile_draws_cap = 0;
+            void* q = nullptr;
+            if (hipMalloc(&q, need * sizeof(float)) != hipSuccess)
+                return set_error(PLT_ENOMEM, "plt_pass_execute: per-tile generator buffer allocation failed");
+            p->d_tile_draws = static_cast<float*>(q);
+            p->tile_draws_cap = need;
+        }
+        fr.tile_draws = p->d_tile_draws;
+    }
     fr.sourcing_area = float(o("sourcingAreaFromEmissiveGeometry"));
     fr.max_omega = float(o("sourcingMaxBeamOmega"));
     fr.records = p->d_records;
@@ -447,6 +526,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         (void)hipEventRecord(ev[0], st);
     }
     int rc;
+    if (fr.tile_draws && (rc = plt::launch_tile_draws(fr, tiles, p->d_tile_draws, st)) < 0) return rc;
     if (p->schedule == PLT_SCHEDULE_WAVEFRONT) {
         if (npaths > p->wf_paths) {  // per path: 5 x 16 B state + 4 B length + 4 x 4 B queues and keys
             if (p->d_wf) (void)hipFree(p->d_wf);

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4
     const uint32_t pid = fr.path_base + li;
     const uint32_t px = pid % fr.width, py = pid / fr.width;
     int* stk = stack + threadIdx.x;
-    Rng sg = Rng::make(px, py, fr.frame);
+    Rng sg = Rng::make(px, py, fr.frame, fr.sample_generator == 0);
     PathState ps = start_path(sc, fr, px, py, pid, sg);
     while (ps.len < fr.max_bounces + 1) {
         const Ray ray{ps.org.x, ps.org.y, ps.org.z, ps.dir.x, ps.dir.y, ps.dir.z, 0.f, kFltMax};
@@ -41,9 +41,9 @@ PLT_DEV void store_path(const WfBuffers& w, uint32_t li, const PathState& ps, co
     w.pdfs[li] = make_float4(ps.pdf, ps.thp, ps.lastPdf, __uint_as_float(ps.flags));
     w.len[li] = ps.len;
 }
-PLT_DEV void load_path(const WfBuffers& w, uint32_t li, PathState& ps, Rng& sg) {
+PLT_DEV void load_path(const WfBuffers& w, uint32_t li, bool tiny, PathState& ps, Rng& sg) {
     const uint4 r = w.rng[li];
-    sg = Rng{r.x, r.y, r.z, r.w};
+    sg = Rng::from_state(r, tiny);
     const float4 o = w.org[li], d = w.dir[li], p = w.pdfs[li];
     ps.org = v3(o.x, o.y, o.z);
     ps.lambda = o.w;
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_start_kernel(PtScene sc, PtFrame 
     if (li >= fr.n_paths) return;
     const uint32_t pid = fr.path_base + li;
     const uint32_t px = pid % fr.width, py = pid / fr.width;
-    Rng sg = Rng::make(px, py, fr.frame);
+    Rng sg = Rng::make(px, py, fr.frame, fr.sample_generator == 0);
     const PathState ps = start_path(sc, fr, px, py, pid, sg);
     store_path(w, li, ps, sg);
     w.queue[li] = li;
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4
             li = w.sorted[i];
             PathState ps;
             Rng sg;
-            load_path(w, li, ps, sg);
+            load_path(w, li, fr.sample_generator == 0, ps, sg);
             const float4 hv = w.hit[li];
             const Hit h{__float_as_uint(hv.x), hv.y, hv.z, hv.w};
             cont = sample_vertex(sc, fr, h, li, fr.path_base + li, ps, sg) && ps.len < fr.max_bounces + 1;
@@ -182,6 +182,23 @@ int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfB
         hipLaunchKernelGGL(wf_shade_kernel, dim3(g64), dim3(kPtBlock), 0, st, sc, fr, w, cur);
     }
     return check_launch("wavefront sample pass");
+}
+
+// The per-tile generator of NEEUsePerTileSG (pltpt_solve.rt.slang:412): SampleGenerator(kTile,
+// gFrameCount << 2) of each tile, its first max_bounces + 1 draws (at most one light-type choice per
+// path vertex), so the solve kernel reads the k-th draw instead of carrying a second generator.
+__global__ void tile_draws_kernel(uint32_t tiles_x, uint32_t tiles, uint32_t frame, uint32_t per_tile, bool tiny,
+                                  float* draws) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tiles) return;
+    Rng g = Rng::make(t % tiles_x, t / tiles_x, frame << 2, tiny);
+    for (uint32_t k = 0; k < per_tile; ++k) draws[size_t(t) * per_tile + k] = g.next1D();
+}
+
+int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st) {
+    hipLaunchKernelGGL(tile_draws_kernel, dim3((tiles + 63) / 64), dim3(64), 0, st, fr.tiles_x, tiles, fr.frame, fr.max_bounces + 1,
+                       fr.sample_generator == 0, draws);
+    return check_launch("tile_draws_kernel");
 }
 
 int launch_solve_4(const PtScene& sc, const PtFrame& fr, hipStream_t st);

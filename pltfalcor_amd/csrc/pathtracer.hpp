@@ -37,8 +37,11 @@ struct PtScene {
     const plt_material* materials;
     const plt_spectral_profile* profiles;
     uint32_t n_profiles;
-    // camera (Camera.cpp:187-193)
+    // camera (Camera.cpp:187-193); rows of the view rotation (the right-handed look-at view matrix's
+    // upper 3x3); polarizer strength and angle (CameraData.slang:67-68)
     float cam_pos[3], cam_u[3], cam_v[3], cam_w[3], jitter[2];
+    float view_rot[9];
+    float polarizer, polarizer_theta;
     // emissive triangles: 4 x float4 per light (p0+area, p1+material, p2, normal), alias table,
     // per-light selection pdf, triangle -> light index
     const float4* lights;
@@ -69,6 +72,12 @@ struct PtFrame {
     // manifold NEE: occluder limit, Newton iteration limit, constraint threshold (Manifolds.slangh:42-54)
     uint32_t mnee_max_occluders, mnee_max_iterations;
     float mnee_threshold;
+    // sample generator (SampleGeneratorType.slangh:32-36): 1 = uniform (xoshiro128**), 0 = tiny (LCG)
+    uint32_t sample_generator;
+    // NEEUsePerTileSG: the NEE light-type draws of the per-tile generator, tile_draws[tile * (max_bounces
+    // + 1) + k] = the k-th draw of tile `tile` (row-major tiles of tile_size^2 pixels); NULL = off
+    const float* tile_draws;
+    uint32_t tile_size, tiles_x;
     uint32_t* records;
     float4* color;
     float4* albedo;
@@ -95,5 +104,6 @@ struct WfBuffers {
 int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 
 }  // namespace plt

@@ -502,8 +502,13 @@ PLT_DEV bool sample_vertex(const PtScene& sc, const PtFrame& fr, const Hit& h, u
     const bool isRefl = (bs.lobe & kLobeReflection) != 0;
     const bool isMS = (isRefl && (props & kPropMWReflective)) || (!isRefl && (props & kPropMWTransmissive));
     if (!isDelta && (ps.flags & kFlagOnlyDelta)) {
-        fr.albedo[pid] = make_float4(m.col[0], m.col[1], m.col[2], 1.f);
-        fr.normal[pid] = make_float4(s.f.N.x, s.f.N.y, s.f.N.z, 1.f);
+        float alb[3];
+        mat_albedo(m, alb);
+        fr.albedo[pid] = make_float4(alb[0], alb[1], alb[2], 1.f);
+        // mul(float3x3(viewMat), sd.N): the view-space normal (pltpt_sample.rt.slang:111)
+        const float* R = sc.view_rot;
+        fr.normal[pid] = make_float4(R[0] * s.f.N.x + R[1] * s.f.N.y + R[2] * s.f.N.z, R[3] * s.f.N.x + R[4] * s.f.N.y + R[5] * s.f.N.z,
+                                     R[6] * s.f.N.x + R[7] * s.f.N.y + R[8] * s.f.N.z, 1.f);
     }
     if (end) {
         ps.flags |= kFlagTerminated;
@@ -539,6 +544,26 @@ template <int N>
 PLT_DEV void measure(const PtScene& sc, const PtFrame& fr, Solve& sv, int bounce, float contrib, Beam<N>& beam) {
     if ((fr.options & kOptDirectLights) == 0 && bounce <= 1) return;
     beam.propagate(v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]));
+    // camera polarizer (pltpt_solve.rt.slang:111-126): the beam in the camera frame (t = U, r = -W),
+    // then (1 - p) I + p MuellerPolarizer(theta) (Mueller.slang:60-74) on each HWSS component
+    if (sc.polarizer > 0.f) {
+        beam.rotate_frame(normalize(v3(sc.cam_u[0], sc.cam_u[1], sc.cam_u[2])), -normalize(v3(sc.cam_w[0], sc.cam_w[1], sc.cam_w[2])));
+        const float p = sc.polarizer, q = 1.f - sc.polarizer;
+        const float c = cosf(sc.polarizer_theta), sn = sinf(sc.polarizer_theta);
+        // M = q I + p (P / 2), P = [[1, c, s, 0], [c, c^2, cs, 0], [s, cs, s^2, 0], [0, 0, 0, 0]]
+        const float m01 = p * (.5f * c), m02 = p * (.5f * sn), m12 = p * (.5f * (c * sn));
+        const float m00 = q + p * .5f, m11 = q + p * (.5f * (c * c)), m22 = q + p * (.5f * (sn * sn));
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            if (j < beam.comps) {
+                const float s0 = beam.S[j][0], s1 = beam.S[j][1], s2 = beam.S[j][2];
+                beam.S[j][0] = m00 * s0 + m01 * s1 + m02 * s2;
+                beam.S[j][1] = m01 * s0 + m11 * s1 + m12 * s2;
+                beam.S[j][2] = m02 * s0 + m12 * s1 + m22 * s2;
+                beam.S[j][3] = q * beam.S[j][3];
+            }
+        }
+    }
     if (fr.options & kOptHwssMis) {
         float sum = 0.f;
 #pragma unroll
@@ -767,8 +792,11 @@ solve_kernel(PtScene sc, PtFrame fr) {
         }
     }
 
-    Rng sg = Rng::make(px, py, fr.frame);
+    Rng sg = Rng::make(px, py, fr.frame, fr.sample_generator == 0);
     const float lambda = kMinWavelength + sg.next1D() * (kMaxWavelength - kMinWavelength);
+    // per-tile NEE light-type draws (SampleGenerator(kTile, gFrameCount << 2), pltpt_solve.rt.slang:412)
+    const size_t tileBase = fr.tile_draws ? size_t((py / fr.tile_size) * fr.tiles_x + px / fr.tile_size) * (fr.max_bounces + 1) : 0;
+    uint32_t neeDraws = 0;
     Solve sv;
     sv.r = sv.g = sv.b = 0.f;
     sv.origin = v3(sc.cam_pos[0], sc.cam_pos[1], sc.cam_pos[2]);
@@ -813,8 +841,9 @@ solve_kernel(PtScene sc, PtFrame fr) {
             const bool sampleRefl = (lobes & kLobeNonDeltaReflection) != 0 || allowMS;
             const bool sampleTrans = (lobes & kLobeNonDeltaTransmission) != 0;
             if (sampleRefl || sampleTrans) {
-                // selectLightType (LightSampling.slangh:57-85)
-                float uType = sg.next1D();
+                // selectLightType (LightSampling.slangh:57-85), from the per-tile generator when
+                // NEEUsePerTileSG is on (LightSampling.slangh:181-185)
+                float uType = fr.tile_draws ? fr.tile_draws[tileBase + neeDraws++] : sg.next1D();
                 int ltype = -1;  // SampledLightType: 0 environment map, 1 emissive, 2 analytic
                 float typePdf = 0.f;
                 if (uType < pEnv) {

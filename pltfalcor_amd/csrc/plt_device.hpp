@@ -62,14 +62,39 @@ PLT_DEV uint64_t splitmix_next(uint64_t& s) {
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+// blockCipherTEA (Utils/Math/HashUtils.slang:59-71): TEA encryption of (v0, v1) with Falcor's
+// fixed 128-bit key and 16 cycles; the Tiny generator takes .x as its seed.
+PLT_DEV uint32_t tea_seed(uint32_t v0, uint32_t v1) {
+    uint32_t sum = 0u;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        sum += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+// Either of the reference's sample generators (SampleGeneratorType.slangh:32-36), chosen per
+// launch: `lcg` false = UniformSampleGenerator (xoshiro128** in s0..s3), true =
+// TinyUniformSampleGenerator (TinyUniformSampleGenerator.slang:55-66: the Numerical Recipes LCG,
+// Pseudorandom/LCG.slang:45-61, with its 32-bit state in s0).
 struct Rng {
     uint32_t s0, s1, s2, s3;
-    PLT_DEV static Rng make(uint32_t px, uint32_t py, uint32_t sampleNumber) {
-        uint64_t st = (uint64_t(sampleNumber) << 32) | uint64_t(spread16(px) | (spread16(py) << 1));
+    bool lcg;
+    PLT_DEV static Rng make(uint32_t px, uint32_t py, uint32_t sampleNumber, bool tiny = false) {
+        const uint32_t pix = spread16(px) | (spread16(py) << 1);  // interleave_32bit(pixel)
+        if (tiny) return Rng{tea_seed(pix, sampleNumber), 0u, 0u, 0u, true};
+        uint64_t st = (uint64_t(sampleNumber) << 32) | uint64_t(pix);
         const uint64_t a = splitmix_next(st), b = splitmix_next(st);
-        return Rng{uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32)};
+        return Rng{uint32_t(a), uint32_t(a >> 32), uint32_t(b), uint32_t(b >> 32), false};
     }
+    PLT_DEV static Rng from_state(uint4 st, bool tiny) { return Rng{st.x, st.y, st.z, st.w, tiny}; }
     PLT_DEV uint32_t next() {
+        if (lcg) {
+            s0 = 1664525u * s0 + 1013904223u;
+            return s0;
+        }
         const uint32_t m = s0 * 5u;
         const uint32_t res = ((m << 7) | (m >> 25)) * 9u;
         const uint32_t t = s1 << 9;

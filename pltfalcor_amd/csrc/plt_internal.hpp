@@ -22,6 +22,6 @@ int check_launch(const char* kernel);
 int launch_generate_queries(uint32_t n, uint32_t seed, uint32_t* q, hipStream_t st);
 int launch_eval_queries(uint32_t n, const uint32_t* q, float* res, const plt_spectral_profile* prof, uint32_t nprof,
                         hipStream_t st);
-int launch_sample_generator(uint32_t n, uint32_t width, uint32_t sn, uint32_t draws, uint32_t* out, hipStream_t st);
+int launch_sample_generator(uint32_t n, uint32_t width, uint32_t sn, uint32_t draws, bool tiny, uint32_t* out, hipStream_t st);
 
 }  // namespace plt

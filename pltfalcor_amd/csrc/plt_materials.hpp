@@ -245,6 +245,15 @@ struct Mat {
 
 PLT_DEV float h16(uint16_t h) { return float(__builtin_bit_cast(_Float16, h)); }
 
+// A parameter block this layer evaluates: a PLT MaterialType (2-11) and, for the gratings, not a
+// radial grating (type bit 0x10, DiffractionGrating.slangh:37), whose direction comes from texture
+// coordinates that the meshes here do not carry.
+__host__ __device__ inline bool material_supported(const plt_material& m) {
+    if (m.type < PLT_MAT_DIFFUSE || m.type > PLT_MAT_MULTILAYER_STACK) return false;
+    const bool grated = m.type == PLT_MAT_GRATED_CONDUCTOR || m.type == PLT_MAT_GRATED_OPAQUE_DIELECTRIC;
+    return !(grated && ((m.data1[3] >> 8) & 0x10u));
+}
+
 // setupMaterialInstance of each *Material.slang.
 PLT_DEV Mat setup_material(const plt_material& m, const plt_spectral_profile* prof, uint32_t nprof) {
     Mat r;
@@ -275,7 +284,8 @@ PLT_DEV Mat setup_material(const plt_material& m, const plt_spectral_profile* pr
             r.tau = h16(m.data1[0]) * sz;
             r.coatIor = h16(m.data1[3]);
             break;
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorMaterial.slang:40-70
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorMaterial.slang:40-70
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricMaterial.slang:40-70
             r.fs = Fractal::from_roughness(h16(m.data2[1]), 3.f);
             const float ptx = h16(m.data1[0]), pty = h16(m.data1[1]);
             r.gr.ipx = ptx > 0.f ? 1.f / ptx : 0.f;
@@ -325,6 +335,7 @@ PLT_DEV uint32_t mat_lobes(const Mat& m) {
         case PLT_MAT_THIN_DIELECTRIC: return kLobeDeltaReflection | kLobeDeltaTransmission;
         case PLT_MAT_MULTILAYER_STACK: return kLobeDeltaReflection;
         case PLT_MAT_GRATED_CONDUCTOR:
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC:
             return (m.fs.q == 0.f ? kLobeDeltaReflection : (kLobeDeltaReflection | kLobeScatterReflection)) |
                    kLobeDispersiveDeltaReflection;
         default:
@@ -335,17 +346,52 @@ PLT_DEV uint32_t mat_props(const Mat& m) {
     switch (m.type) {
         case PLT_MAT_DIELECTRIC: return kPropTransmissive | (m.gs.q < kQcutoffForMS ? kPropMWTransmissive : 0u);
         case PLT_MAT_DIFFUSE: return m.emission != PLT_NO_PROFILE ? kPropEmissive : 0u;
-        case PLT_MAT_GRATED_CONDUCTOR: return kPropMWReflective;
+        case PLT_MAT_GRATED_CONDUCTOR:
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: return kPropMWReflective;
         case PLT_MAT_MULTILAYER_STACK: return kPropMWReflective;
         case PLT_MAT_THIN_DIELECTRIC: return kPropTransmissive | kPropMWReflective | kPropMWTransmissive;
         default: return m.fs.q < kQcutoffForMS ? kPropMWReflective : 0u;
     }
 }
 
-// Shading frame used for eval / sample / pdf: the grated conductor evaluates in a frame whose
+// BSDFProperties.albedo of getProperties (the albedo output of the sample pass): the specular
+// colour times the normal-incidence Fresnel reflectance at 550 nm for the dielectric
+// (PLTDielectricBSDF.slang:342-346) and the conductors (PLTConductorBSDF.slang:226-231,
+// PLTDiffractionGratedConductorBSDF.slang:273-280, n and k not divided by the exterior IoR), 1 for
+// the thin dielectric (PLTThinDielectricBSDF.slang:259), the base colour for the others.
+PLT_DEV void mat_albedo(const Mat& m, float out[3]) {
+    float f = 1.f;
+    switch (m.type) {
+        case PLT_MAT_DIELECTRIC: {
+            const float uk = kMeanK * k1_2Pi;  // indexOfRefraction(2 pi / 0.55 um): Cauchy in 1 / lambda^2
+            float cT;
+            f = fresnel_dielectric_F((m.etaInt + m.dispB * (uk * uk)) / m.etaExt, 1.f, cT);
+            break;
+        }
+        case PLT_MAT_CONDUCTOR:
+        case PLT_MAT_GRATED_CONDUCTOR: {
+            const float n = m.pn ? eval_spectrum(m.pn, 550.f) : 0.f, k = m.pk ? eval_spectrum(m.pk, 550.f) : 0.f;
+            float rsr, rsi, rpr, rpi;
+            f = fresnel_conductor_amp(1.f, n, k, rsr, rsi, rpr, rpi) ? .5f * ((rsr * rsr + rsi * rsi) + (rpr * rpr + rpi * rpi)) : 0.f;
+            break;
+        }
+        case PLT_MAT_THIN_DIELECTRIC:
+            out[0] = out[1] = out[2] = 1.f;
+            return;
+        default:
+            break;
+    }
+    out[0] = m.col[0] * f;
+    out[1] = m.col[1] * f;
+    out[2] = m.col[2] * f;
+}
+
+PLT_DEV bool is_grated(const Mat& m) { return m.type == PLT_MAT_GRATED_CONDUCTOR || m.type == PLT_MAT_GRATED_OPAQUE_DIELECTRIC; }
+
+// Shading frame used for eval / sample / pdf: the grated materials evaluate in a frame whose
 // tangent follows the grating direction (DiffractionGrating.slang:72-79).
 PLT_DEV Frame mat_frame(const Mat& m, const Frame& sd) {
-    if (m.type != PLT_MAT_GRATED_CONDUCTOR) return sd;
+    if (!is_grated(m)) return sd;
     Frame g;
     g.T = sd.from_local(v3(m.gr.gx, m.gr.gy, 0.f));
     g.N = sd.N;
@@ -633,6 +679,26 @@ PLT_DEV void multilayer_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>&
     }
 }
 
+// PLTOpaqueDielectricBxDF::eval (PLTOpaqueDielectricBSDF.slang:57-82): a depolarising rough
+// dielectric evaluated at the mean wavenumber, scaled by the albedo spectrum.
+template <int N>
+PLT_DEV void opaque_dielectric_eval(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
+    const bool isDirect = (lobe & kLobeDelta) != 0, isScatter = (lobe & kLobeNonDelta) != 0;
+    if ((!isScatter && !isDirect) || (lobe & kLobeReflection) == 0) { beam.set_zero(); return; }
+    float M = m.fs.alpha(wi.z, wo.z, kMeanK);
+    if (isScatter) {
+        const V3 h = wi + wo;
+        M = m.fs.one_minus_alpha(wi.z, wo.z, kMeanK) * fabsf(wi.z) * m.fs.pgf(kMeanK * h.x, kMeanK * h.y);
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j < beam.comps) {
+            const float w = M * rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]);
+            beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
+        }
+    }
+}
+
 template <int N>
 PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& beam) {
     switch (m.type) {
@@ -656,23 +722,9 @@ PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& 
             }
             break;
         }
-        case PLT_MAT_OPAQUE_DIELECTRIC: {  // PLTOpaqueDielectricBSDF.slang:57-82
-            const bool isDirect = (lobe & kLobeDelta) != 0, isScatter = (lobe & kLobeNonDelta) != 0;
-            if ((!isScatter && !isDirect) || (lobe & kLobeReflection) == 0) { beam.set_zero(); break; }
-            float M = m.fs.alpha(wi.z, wo.z, kMeanK);
-            if (isScatter) {
-                const V3 h = wi + wo;
-                M = m.fs.one_minus_alpha(wi.z, wo.z, kMeanK) * fabsf(wi.z) * m.fs.pgf(kMeanK * h.x, kMeanK * h.y);
-            }
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-                if (j < beam.comps) {
-                    const float w = M * rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]);
-                    beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
-                }
-            }
+        case PLT_MAT_OPAQUE_DIELECTRIC:
+            opaque_dielectric_eval<N>(m, wi, wo, lobe, beam);
             break;
-        }
         case PLT_MAT_COATED_OPAQUE_DIELECTRIC: {  // PLTCoatedOpaqueDielectricBSDF.slang:69-124
             const bool isRefl = wi.z * wo.z > 0.f;
             const bool isScatter = (lobe & kLobeScatterReflection) != 0, isDirect = (lobe & kLobeDeltaReflection) != 0;
@@ -742,7 +794,9 @@ PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& 
         case PLT_MAT_MULTILAYER_STACK:
             multilayer_eval<N>(m, wi, wo, lobe, beam);
             break;
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:68-107
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorBSDF.slang:68-107
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:68-101
+            const bool opaque = m.type == PLT_MAT_GRATED_OPAQUE_DIELECTRIC;
             const bool isDirect = (lobe & kLobeDelta) != 0;
             float a[N], lam[N];
             bool anyPos = false, anyBelow1 = false;
@@ -756,7 +810,8 @@ PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& 
             int lx, ly;
             decode_order(lobe, lx, ly);
             if (lx == 0 && ly == 0 && anyPos) {
-                conductor_like_eval<N>(m, wi, wo, lobe, beam);
+                if (opaque) opaque_dielectric_eval<N>(m, wi, wo, lobe, beam);
+                else conductor_like_eval<N>(m, wi, wo, lobe, beam);
                 beam.scale_each(a);
             } else if ((lx != 0 || ly != 0) && anyBelow1 && isDirect) {
                 const V3 wiBeam = beam.to_local(wi);
@@ -764,14 +819,16 @@ PLT_DEV void mat_eval_local(const Mat& m, V3 wi, V3 wo, uint32_t lobe, Beam<N>& 
 #pragma unroll
                 for (int j = 0; j < N; ++j) {
                     if (j < beam.comps) {
-                        const float n = (m.pn ? eval_spectrum(m.pn, beam.lambda[j]) : 0.f) * iext;
-                        const float kk = (m.pk ? eval_spectrum(m.pk, beam.lambda[j]) : 0.f) * iext;
-                        const Fres F = fresnel_conductor_mueller(wi.z, n, kk);
                         float coh = 1.f;
                         if (j != 0) coh = beam.mutual_coherence_angular(beam.to_local(m.gr.diffract(wo, lx, ly, lam[j])), wiBeam);
                         const float inten = m.gr.lobe_intensity(lx, ly, wi.z, lam[j]);
                         const float w = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], beam.lambda[j]) * (1.f - a[j]) * coh * inten;
-                        apply(F, beam.S[j][0], beam.S[j][1], beam.S[j][2], beam.S[j][3]);
+                        if (!opaque) {  // the conductor's Fresnel Mueller matrix; the opaque dielectric has none
+                            const float n = (m.pn ? eval_spectrum(m.pn, beam.lambda[j]) : 0.f) * iext;
+                            const float kk = (m.pk ? eval_spectrum(m.pk, beam.lambda[j]) : 0.f) * iext;
+                            const Fres F = fresnel_conductor_mueller(wi.z, n, kk);
+                            apply(F, beam.S[j][0], beam.S[j][1], beam.S[j][2], beam.S[j][3]);
+                        }
                         beam.S[j][0] *= w; beam.S[j][1] *= w; beam.S[j][2] *= w; beam.S[j][3] *= w;
                     }
                 }
@@ -791,7 +848,8 @@ PLT_DEV void mat_coherence_transform(const Mat& m, V3 wo, uint32_t lobe, Beam<N>
     bool scatter;
     switch (m.type) {
         case PLT_MAT_DIFFUSE: scatter = true; break;
-        case PLT_MAT_GRATED_CONDUCTOR: {
+        case PLT_MAT_GRATED_CONDUCTOR:
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {
             int lx, ly;
             decode_order(lobe, lx, ly);
             scatter = (lx == 0 && ly == 0) && (lobe & kLobeDelta) == 0;
@@ -873,7 +931,8 @@ PLT_DEV void mat_lambda_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 woW, uint3
                 for (int j = 0; j < N; ++j) ret[j] = m.fs.pdf(wiIn, woIn, k2Pi / (1e-3f * beam.lambda[j]));
             }
             break;
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:167-190
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorBSDF.slang:167-190
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:164-188
             int lx, ly;
             decode_order(lobe, lx, ly);
             bool anyBelow1 = false;
@@ -929,6 +988,8 @@ PLT_DEV float mat_pdf(const Mat& m, const Frame& sd, V3 sdV, V3 toLightW, float 
         }
         case PLT_MAT_GRATED_CONDUCTOR:
             return m.fs.one_minus_alpha(wi.z, wi.z, k) * m.fs.pdf(wi, wo, k) * m.gr.alpha(wi.z, k);
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC:  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:148-156
+            return m.fs.one_minus_alpha(wi.z, wi.z, kMeanK) * m.fs.pdf(wi, wo, kMeanK) * m.gr.alpha(wi.z, k);
         default:
             return 0.f;
     }
@@ -1063,21 +1124,24 @@ PLT_DEV bool mat_sample(const Mat& m, const Frame& sd, V3 sdV, Rng& sg, float la
             valid = wi.z * wo.z > 0.f;
             break;
         }
-        case PLT_MAT_GRATED_CONDUCTOR: {  // PLTDiffractionGratedConductorBSDF.slang:115-153 (no hint)
+        case PLT_MAT_GRATED_CONDUCTOR:           // PLTDiffractionGratedConductorBSDF.slang:115-153 (no hint)
+        case PLT_MAT_GRATED_OPAQUE_DIELECTRIC: {  // PLTDiffractionGratedOpaqueDielectricBSDF.slang:110-146
             const float lam = 1e-3f * lambda;
             const float a = m.gr.alpha(wi.z, k2Pi / lam);
             const float uD = sg.next1D_if(a != 1.f);
             const bool isDirect = a == 1.f || uD < a;
             pdf = isDirect ? a : 1.f - a;
+            // the base lobe's wavenumber: the conductor's own, the opaque dielectric's mean
+            const float kb = m.type == PLT_MAT_GRATED_CONDUCTOR ? k : kMeanK;
             if (isDirect) {
-                const float ab = m.fs.alpha(wi.z, wi.z, k);
+                const float ab = m.fs.alpha(wi.z, wi.z, kb);
                 weight = rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda);
                 float bp = 1.f;
                 const float uB = sg.next1D_if(ab != 1.f);
                 const bool bd = ab == 1.f || uB < ab;
                 bp = bd ? ab : 1.f - ab;
                 if (bd) { wo = v3(-wi.x, -wi.y, wi.z); lobe = kLobeDeltaReflection; }
-                else { float sp; wo = m.fs.sample(wi, sg, k, sp); bp *= sp; lobe = kLobeScatterReflection; }
+                else { float sp; wo = m.fs.sample(wi, sg, kb, sp); bp *= sp; lobe = kLobeScatterReflection; }
                 pdf *= bp;
                 valid = true;
                 break;

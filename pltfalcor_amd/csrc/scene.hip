@@ -130,7 +130,11 @@ int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out)
         if (d->mesh_material[m] >= d->n_materials) return set_error(PLT_EINVAL, "plt_scene_create: material id out of range");
     for (uint32_t m = 0; m < d->n_materials; ++m) {
         const plt_material& mt = d->materials[m];
-        if (mt.type > PLT_MAT_MULTILAYER_STACK) return set_error(PLT_EINVAL, "plt_scene_create: unknown material type");
+        if (mt.type < PLT_MAT_DIFFUSE || mt.type > PLT_MAT_MULTILAYER_STACK)
+            return set_error(PLT_EINVAL, "plt_scene_create: unknown material type (the reference's MaterialType values 2-11)");
+        // radial gratings take their direction from texture coordinates (DiffractionGrating.slang:52-57)
+        if ((mt.type == PLT_MAT_GRATED_CONDUCTOR || mt.type == PLT_MAT_GRATED_OPAQUE_DIELECTRIC) && ((mt.data1[3] >> 8) & 0x10u))
+            return set_error(PLT_ENOTIMPL, "plt_scene_create: radial diffraction gratings need texture coordinates, which are not built");
         for (uint32_t id : {mt.emission_profile, mt.ior_n_profile, mt.ior_k_profile})
             if (id != PLT_NO_PROFILE && id >= d->n_profiles) return set_error(PLT_EINVAL, "plt_scene_create: profile id out of range");
     }

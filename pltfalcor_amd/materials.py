@@ -12,8 +12,9 @@ import numpy as np
 from ._lib import Material
 
 NO_PROFILE = 0xFFFFFFFF
-DIELECTRIC, CONDUCTOR, DIFFUSE, OPAQUE_DIELECTRIC, COATED_OPAQUE_DIELECTRIC, GRATED_CONDUCTOR, COATED_CONDUCTOR, \
-    THIN_DIELECTRIC, MULTILAYER_STACK = range(9)
+# MaterialType values (Scene/Material/MaterialTypes.slang:36-53)
+DIFFUSE, CONDUCTOR, DIELECTRIC, OPAQUE_DIELECTRIC, THIN_DIELECTRIC, COATED_CONDUCTOR, COATED_OPAQUE_DIELECTRIC, \
+    GRATED_CONDUCTOR, GRATED_OPAQUE_DIELECTRIC, MULTILAYER_STACK = range(2, 12)
 
 
 def h(x: float) -> int:
@@ -110,6 +111,20 @@ def grated_conductor(pitch_um: float, direction: float, n_profile: int, k_profil
     return m
 
 
+def grated_opaque_dielectric(pitch_um: float, direction: float, albedo=(1.0, 1.0, 1.0), lobes: int = 3,
+                             grating_type: int = 0, height: float = 1.0, height_scale: float = .5,
+                             roughness: float = 0.0) -> Material:
+    """PLTDiffractionGratedOpaqueDielectricMaterial.cpp:44-147 (1-D grating along `direction` over an
+    opaque dielectric base of colour `albedo`; no exterior IoR or n, k profiles)."""
+    m = blank(GRATED_OPAQUE_DIELECTRIC)
+    m.base_color[0], m.base_color[1], m.base_color[2] = (h(c) for c in albedo)
+    m.data1[0], m.data1[1], m.data1[2] = h(pitch_um), h(0.0), h(direction)
+    m.data2[0], m.data2[1] = h(height), h(roughness)
+    m.specular[0], m.specular[1] = h(height_scale), h(1.0)
+    m.data1[3] = (grating_type << 8) | lobes
+    return m
+
+
 def thin_dielectric(ior: float, thickness_mm: float, birefringence: float = 0.0, axis=(0.0, 0.0, 1.0)) -> Material:
     """PLTThinDielectricMaterial.cpp:45-90; the birefringence scale (specular.x) is set to 1."""
     m = blank(THIN_DIELECTRIC)
@@ -145,4 +160,5 @@ def pack(materials) -> bytes:
 
 
 __all__ = ["blank", "dielectric", "conductor", "diffuse", "opaque_dielectric", "coated_opaque", "coated_conductor",
+           "grated_opaque_dielectric",
            "grated_conductor", "thin_dielectric", "multilayer", "pack", "NO_PROFILE"]

@@ -113,7 +113,8 @@ class Scene:
             lights=self.lights,
             env=self.env,
             camera=np.array(list(a.camera.pos) + list(a.camera.target) + list(a.camera.up) +
-                            [a.camera.fov_y, a.camera.aspect, a.camera.focal_distance] + list(a.camera.jitter),
+                            [a.camera.fov_y, a.camera.aspect, a.camera.focal_distance] + list(a.camera.jitter) +
+                            [a.camera.polarizer, a.camera.polarizer_theta],
                             dtype=np.float32),
         )
 

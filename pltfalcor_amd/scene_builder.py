@@ -150,12 +150,15 @@ class SceneBuilder:
         return len(self.lights) - 1
 
     def addCamera(self, position, target, up, fovY: float | None = None, focalLength: float | None = None,
-                  frameHeight: float = 24.0, aspect: float = 1.0, focalDistance: float = 1.0):
+                  frameHeight: float = 24.0, aspect: float = 1.0, focalDistance: float = 1.0,
+                  polarizationIntensity: float = 0.0, polarizationAngle: float = 0.0):
         """Pinhole camera: vertical field of view in radians, or a focal length in mm
-        (Camera::setFocalLength, with the 24 mm frame height)."""
+        (Camera::setFocalLength, with the 24 mm frame height). The polarizer is the camera's
+        polarizationIntensity / polarizationAngle properties (Camera.cpp:315-323, 452-453)."""
         fov = fovY if fovY is not None else focal_length_to_fov_y(50.0 if focalLength is None else focalLength,
                                                                  frameHeight)
-        self.camera = _lib.Camera(tuple(position), tuple(target), tuple(up), fov, aspect, focalDistance, (0, 0))
+        self.camera = _lib.Camera(tuple(position), tuple(target), tuple(up), fov, aspect, focalDistance, (0, 0),
+                                  float(polarizationIntensity), float(polarizationAngle))
 
     def getScene(self, ctx) -> Scene:
         """Uploads the collected scene (host-only when ctx is None)."""

@@ -20,6 +20,8 @@ _SIGS = {
     "oracle_splitmix64_seq": (None, [C.c_uint64, C.c_uint32, _U64]),
     "oracle_xoshiro128ss_seq": (None, [_U32, C.c_uint32, _U32]),
     "oracle_usg_seq": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _U32]),
+    "oracle_sg_streams": (None, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _U32]),
+    "oracle_tea": (None, [C.c_uint32, C.c_uint32, _U32, C.c_uint32, _U32]),
     "oracle_f32_to_f16": (C.c_uint16, [C.c_float]),
     "oracle_f16_to_f32": (C.c_float, [C.c_uint16]),
     "oracle_spectral_profile_size": (C.c_uint32, []),
@@ -67,6 +69,21 @@ def usg_seq(px: int, py: int, sn: int, n: int) -> np.ndarray:
     out = np.zeros(n, dtype=np.uint32)
     lib().oracle_usg_seq(px, py, sn, n, ptr(out, C.c_uint32))
     return out
+
+
+def sg_streams(gen_type: int, x0: int, y0: int, w: int, n_streams: int, sample_number: int, draws: int) -> np.ndarray:
+    """Raw draws [n_streams, draws] of generator type 0 (tiny) or 1 (uniform) for pixels
+    (x0 + s % w, y0 + s // w)."""
+    out = np.zeros((n_streams, draws), np.uint32)
+    lib().oracle_sg_streams(gen_type, x0, y0, w, n_streams, sample_number, draws, ptr(out, C.c_uint32))
+    return out
+
+
+def tea(v0: int, v1: int, key, iterations: int):
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(2, np.uint32)
+    lib().oracle_tea(v0, v1, ptr(k, C.c_uint32), iterations, ptr(out, C.c_uint32))
+    return int(out[0]), int(out[1])
 
 
 def build_profile(wl, val, compute_icdf: bool) -> bytes:
@@ -152,7 +169,7 @@ class OracleScene:
         l.oracle_scene_destroy.argtypes = [C.c_void_p]
         l.oracle_scene_render.restype = C.c_int
         l.oracle_scene_render.argtypes = [C.c_void_p, _F, C.POINTER(C.c_int32), _F, C.c_uint32, C.c_uint32, _F, _U32,
-                                          C.c_int]
+                                          _F, C.c_int]
         self.lib = l
         self.arrays = dict(
             pos=np.ascontiguousarray(scene["positions"], np.float32),
@@ -183,26 +200,35 @@ class OracleScene:
     def render(self, width, height, frame=0, max_bounces=6, first=0, count=None, do_nee=1, do_mis=1, do_rr=1,
                hwss_mis=1, is_emitters=1, direct=1, emissive=1, sourcing_area=1.0, max_omega=0.025, threads=0,
                records=False, camera=None, do_mnee=0, mnee_max_occluders=2, mnee_max_iterations=60,
-               mnee_threshold=5e-5, emissive_sampler=2, analytic=1, env=1):
-        """Renders pixels [first, first+count) (row-major). Returns color [count, 4] and optionally the
-        bounce records [(max_bounces+1), 9, count] (uint32)."""
+               mnee_threshold=5e-5, emissive_sampler=2, analytic=1, env=1, sample_generator=1, per_tile_sg=0,
+               tile_size=512, aov=False):
+        """Renders pixels [first, first+count) (row-major). Returns color [count, 4], then optionally the
+        bounce records [(max_bounces+1), 9, count] (uint32) and the albedo and view-space normal outputs
+        [count, 8] (aov)."""
         if count is None:
             count = width * height - first
         cam = self.camera if camera is None else np.ascontiguousarray(camera, np.float32)
+        if cam.size < 16:  # polarizer off
+            cam = np.concatenate([cam, np.zeros(16 - cam.size, np.float32)])
         if emissive_sampler not in (0, 2):
             raise ValueError("emissive_sampler: 0 (Uniform) or 2 (Power)")
         iopt = np.array([width, height, frame, max_bounces, do_nee, do_mis, do_rr, hwss_mis, is_emitters, direct,
                          emissive, do_mnee, mnee_max_occluders, mnee_max_iterations, int(emissive_sampler == 0),
-                         analytic, env], np.int32)
+                         analytic, env, sample_generator, per_tile_sg, tile_size], np.int32)
         fopt = np.array([sourcing_area, max_omega, mnee_threshold], np.float32)
         color = np.zeros((count, 4), np.float32)
         rec = np.zeros(((max_bounces + 1) * REC_WORDS, count), np.uint32) if records else None
+        aovs = np.zeros((count, 8), np.float32) if aov else None
         self.lib.oracle_scene_render(self.handle, ptr(cam, C.c_float), iopt.ctypes.data_as(C.POINTER(C.c_int32)),
                                      ptr(fopt, C.c_float), first, count, ptr(color, C.c_float),
-                                     ptr(rec, C.c_uint32) if records else None, threads)
+                                     ptr(rec, C.c_uint32) if records else None, ptr(aovs, C.c_float) if aov else None,
+                                     threads)
+        out = [color]
         if records:
-            return color, rec.reshape(max_bounces + 1, REC_WORDS, count)
-        return color
+            out.append(rec.reshape(max_bounces + 1, REC_WORDS, count))
+        if aov:
+            out.append(aovs)
+        return out[0] if len(out) == 1 else tuple(out)
 
     def mnee_probe(self, origin, light_pos, wavelength=550.0, max_ms_bounces=2, max_occluders=2, max_iterations=60,
                    threshold=5e-5, seed=0):
@@ -308,14 +334,15 @@ def material_sample(material: bytes, profiles: bytes, n_profiles: int, wi, wavel
 
 
 def walk_trig(x, y):
-    """The manifold walk's single-precision acos(x) and atan2(y, x)."""
+    """The manifold walk's FP64 acos(x) and atan2(y, x) (its Newton step's spherical angles)."""
     l = lib()
+    _D = C.POINTER(C.c_double)
     l.oracle_walk_trig.restype = None
-    l.oracle_walk_trig.argtypes = [_F, _F, C.c_uint32, _F, _F]
-    x = np.ascontiguousarray(x, np.float32)
-    y = np.ascontiguousarray(y, np.float32)
+    l.oracle_walk_trig.argtypes = [_D, _D, C.c_uint32, _D, _D]
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
     a, t = np.zeros_like(x), np.zeros_like(x)
-    l.oracle_walk_trig(ptr(x, C.c_float), ptr(y, C.c_float), x.size, ptr(a, C.c_float), ptr(t, C.c_float))
+    l.oracle_walk_trig(ptr(x, C.c_double), ptr(y, C.c_double), x.size, ptr(a, C.c_double), ptr(t, C.c_double))
     return a, t
 
 

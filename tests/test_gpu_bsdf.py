@@ -52,6 +52,23 @@ def test_sample_generator_bit_exact(gpu):
             assert np.array_equal(out[:, s], ref), (sn, s)
 
 
+def test_tiny_sample_generator_bit_exact(gpu):
+    """TinyUniformSampleGenerator streams (sampleGenerator 0) on the GPU equal the oracle's."""
+    import ctypes as C
+    from pltfalcor_amd import _lib
+    lib = _lib.load()
+    w, h, draws = 64, 16, 24
+    for sn in (0, 1, 77, 1 << 30):
+        out = torch.empty((draws, w * h), dtype=torch.int32, device="cuda")
+        _lib.check(lib.plt_sample_generator_stream_typed(0, w * h, w, sn, draws, C.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        g = out.cpu().numpy().view(np.uint32)
+        ref = O.sg_streams(0, 0, 0, w, w * h, sn, draws)
+        assert np.array_equal(g.T, ref), sn
+    with pytest.raises(_lib.PLTError):
+        _lib.check(lib.plt_sample_generator_stream_typed(2, 4, 4, 0, 1, C.c_void_p(out.data_ptr()), None))
+
+
 def test_query_generator_matches_oracle(gpu):
     from pltfalcor_amd.bsdf import BSDFBatch
     ctx = _ctx()

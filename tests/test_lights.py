@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from parity import color_close
 from test_pathtracer import _camera_rays, _lambert_polygon_irradiance, _look_down, _topology_equal
 
 EMISSION_PROFILE = 2  # the procedural scene's blackbody emission profile
@@ -241,7 +242,7 @@ def test_gpu_analytic_lights_match_oracle(gpu, case):
     assert np.array_equal(g_rec[0, :3], o_rec[0, :3])
     topo = _topology_equal(g_rec, o_rec)
     assert topo.mean() >= 0.99, topo.mean()
-    close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3])), axis=1)
+    close = color_close(g_col, o_col)
     assert close[topo].mean() >= 0.995, close[topo].mean()
     gm, om = g_col[:, :3].mean(), o_col[:, :3].mean()
     assert np.isfinite(g_col).all() and om > 0

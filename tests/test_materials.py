@@ -260,7 +260,8 @@ def _all_materials():
     return [
         M.dielectric(1.5, 0.05), M.dielectric(1.7, 0.0), M.conductor(0.1, 0, 1, gamma=2.5), M.conductor(0.0, 0, 1),
         M.diffuse(.7, .4, .2), M.opaque_dielectric(.3, .6, .8, 0.2), M.coated_opaque(.8, .3, .2, 0.15, 1.5, 2.0),
-        M.grated_conductor(1.2, 0.7, 0, 1), M.coated_conductor(0.12, 1.5, 1.5, 0, 1),
+        M.grated_conductor(1.2, 0.7, 0, 1), M.grated_opaque_dielectric(0.9, -0.4, (.7, .5, .3), roughness=0.1),
+        M.grated_opaque_dielectric(1.6, 0.2, lobes=5, grating_type=1), M.coated_conductor(0.12, 1.5, 1.5, 0, 1),
         M.coated_conductor(0.0, 1.4, 3.0, 0, 1), M.thin_dielectric(1.5, 1e-3, 0.1, (0.3, -0.2, 1.0)),
         M.thin_dielectric(1.4, 5e-4, 0.0), M.multilayer(1.38, 2.3, .55 / (4 * 1.38), .55 / (4 * 2.3), 20),
         M.multilayer(1.5, 2.0, 0.2, 0.3, 7, albedo=(.5, .6, .7)),
@@ -298,7 +299,7 @@ def _queries(mats, prof, n_per, seed):
                 if rng.uniform() < 0.25:
                     v[2] = -v[2]
                 wo = (v / np.linalg.norm(v)).astype(np.float32)
-                lobe = 0x33 if mat.type != 2 else 0x01
+                lobe = 0x33 if mat.type != M.DIFFUSE else 0x01
             rows.append((mi, wi, np.asarray(wo, np.float32), lobe, lam, S, area))
     return rows
 
@@ -373,7 +374,7 @@ def test_gpu_material_sample_batch_matches_oracle(gpu):
             u = rng.uniform(size=2)
             r, ph = np.sqrt(u[0]), 2 * np.pi * u[1]
             wi = np.array([r * np.cos(ph), r * np.sin(ph), np.sqrt(max(0.0, 1 - u[0]))], np.float32)
-            if mats[mi].type in (0, 7) and k % 3 == 0:
+            if mats[mi].type in (M.DIELECTRIC, M.THIN_DIELECTRIC) and k % 3 == 0:
                 wi[2] = -wi[2]  # from inside for transmissive materials
             i = mi * n_per + k
             q[0:3, i] = wi

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from parity import color_close
 
 TABLES = str(Path(__file__).resolve().parent / "golden" / "tables")
 
@@ -231,7 +232,7 @@ def test_gpu_imported_scene_matches_oracle(gpu):
     assert np.array_equal(g_rec[0, :3], o_rec[0, :3])
     topo = _topology_equal(g_rec, o_rec)
     assert topo.mean() >= 0.99, topo.mean()
-    close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3])), axis=1)
+    close = color_close(g_col, o_col)
     assert close[topo].mean() >= 0.995, close[topo].mean()
     gm, om = g_col[:, :3].mean(), o_col[:, :3].mean()
     assert np.isfinite(g_col).all() and om > 0 and abs(gm - om) <= 0.02 * abs(om), (gm, om)

@@ -237,3 +237,45 @@ def test_rectangular_grating_orders_match_fourier_coefficients():
         for order in (1, 2, 3, 4, 5, 7):
             got = f(1, order, a * lam / (4 * math.pi), lam)
             assert abs(got - abs(c[order]) ** 2) <= 1e-3 * abs(c[1]) ** 2, (a, order, got, abs(c[order]) ** 2)
+
+
+def _polarizer(U, W, p, theta, r, t, S):
+    import ctypes as C
+    l = O.lib()
+    fn = l.oracle_camera_polarizer
+    fn.restype = None
+    fn.argtypes = [C.POINTER(C.c_float)] * 2 + [C.c_float, C.c_float] + [C.POINTER(C.c_float)] * 4
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (U, W, r, t, S)]
+    out = np.zeros(4, np.float32)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    fn(f(arrs[0]), f(arrs[1]), p, theta, f(arrs[2]), f(arrs[3]), f(arrs[4]), f(out))
+    return out.astype(np.float64)
+
+
+def test_camera_polarizer_malus_law():
+    """measure's camera polarizer (pltpt_solve.rt.slang:111-126): the beam is rotated into the camera
+    frame by MuellerT (Mueller.slang:34-57, whose rotation rows are (sin 2a, -cos 2a) and
+    (cos 2a, sin 2a)), then (1 - p) I + p MuellerPolarizer(theta) (Mueller.slang:60-74, cos and sin
+    of theta itself) is applied. For fully linearly polarised light at angle phi in a beam frame
+    rotated by alpha from the camera's, these compose to Malus's law
+        S0' = (1 - p) + p cos^2(alpha + phi - theta / 2 + pi / 4),
+    unpolarised light passes 1 - p / 2 and circular light keeps (1 - p) of S3. Doubling theta in
+    the polarizer, a standard rotation block or a dropped frame rotation each fail it."""
+    rng = np.random.default_rng(7)
+    U, W = np.array([1.0, 0, 0]), np.array([0, 0, -1.0])
+    r = np.array([0, 0, 1.0])
+    for _ in range(200):
+        alpha, phi, theta = rng.uniform(-np.pi, np.pi, 3)
+        p = rng.choice([1.0, rng.uniform(0, 1)])
+        t = np.array([np.cos(alpha), np.sin(alpha), 0.0])
+        S = np.array([1.0, np.cos(2 * phi), np.sin(2 * phi), 0.0])
+        out = _polarizer(U, W, p, theta, r, t, S)
+        want = (1 - p) + p * np.cos(alpha + phi - theta / 2 + np.pi / 4) ** 2
+        assert abs(out[0] - want) < 2e-6, (alpha, phi, theta, p, out, want)
+        un = _polarizer(U, W, p, theta, r, t, np.array([1.0, 0, 0, 0]))
+        assert abs(un[0] - (1 - p / 2)) < 2e-6
+        circ = _polarizer(U, W, p, theta, r, t, np.array([1.0, 0, 0, 1.0]))
+        assert abs(circ[3] - (1 - p)) < 2e-6 and abs(circ[0] - (1 - p / 2)) < 2e-6
+    # polarizer off: the Stokes vector passes unchanged (no frame rotation either)
+    S = np.array([1.0, 0.3, -0.4, 0.2])
+    assert np.array_equal(_polarizer(U, W, 0.0, 1.0, r, np.array([0.6, 0.8, 0]), S), S.astype(np.float32))

@@ -136,3 +136,71 @@ def test_sample_statistics():
     c = np.corrcoef(u.T)
     off = c[~np.eye(dims, dtype=bool)]
     assert np.max(np.abs(off)) < 0.05  # 4096 samples: sampling noise ~1/sqrt(4096)
+
+
+# ---------------------------------------------------------------- TinyUniformSampleGenerator
+
+def test_tea_published_vector():
+    """TEA (Wheeler & Needham 1994) with the all-zero key and plaintext and the full 32 cycles gives
+    the published ciphertext 41ea3a0a 94baa940; a slip in either half-round (shift, key word, sum)
+    changes it. blockCipherTEA (HashUtils.slang:59-71) is this cipher with Falcor's key and 16 cycles."""
+    assert O.tea(0, 0, [0, 0, 0, 0], 32) == (0x41EA3A0A, 0x94BAA940)
+
+
+def lcg_ref(state, n):
+    """The Numerical Recipes LCG of PseudorandomTests.cpp:85-97."""
+    out = []
+    for _ in range(n):
+        state = (1664525 * state + 1013904223) & M32
+        out.append(state)
+    return out
+
+
+def test_tiny_generator_is_tea_seeded_lcg():
+    """TinyUniformSampleGenerator(pixel, sampleNumber) (TinyUniformSampleGenerator.slang:55-66):
+    seed = blockCipherTEA(interleave_32bit(pixel), sampleNumber).x with Falcor's key
+    (a341316c c8013ea4 ad90777d 7e95761e) and 16 cycles, then the LCG of the reference's own CPU
+    test (PseudorandomTests.cpp:85-97)."""
+    key = [0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E]
+    for (px, py, sn) in [(0, 0, 0), (17, 3, 0), (511, 511, 5), (0xFFFF, 0x7FFF, 123456), (3, 9, 1 << 31)]:
+        seed, _ = O.tea(ref_interleave(px, py, 16), sn, key, 16)
+        got = O.sg_streams(0, px, py, 1, 1, sn, 48)[0]
+        assert [int(v) for v in got] == lcg_ref(seed, 48)
+
+
+def _reference_generator_statistics(gen_type, corr_threshold):
+    """SampleGeneratorTests.cpp:73-141: a 64 x 64 x 16 dispatch, 32 dimensions per instance drawn as
+    1D, 1D, 1D, 2D, 3D; every sample in [0, 1), mean within 0.01 of 0.5, and |Pearson correlation|
+    at strides of 1-8 dimensions, the 15 nearest pixels and 1-4 instances below the threshold."""
+    dims, nx, ny, nz = 32, 64, 64, 16
+    per = []
+    for z in range(nz):
+        per.append(O.sg_streams(gen_type, 0, 0, nx, nx * ny, z, dims))
+    raw = np.stack(per)  # [z, pixel, dim], pixel = y * 64 + x
+    u = (raw >> 8).astype(np.float64) * 2.0 ** -24
+    flat = u.reshape(-1).astype(np.float32)
+    assert np.all((flat >= 0) & (flat < 1))
+    assert abs(flat.astype(np.float64).mean() - 0.5) <= 0.01
+
+    def corr(stride):
+        x = flat[:-stride].astype(np.float64)
+        y = flat[stride:].astype(np.float64)
+        n = x.size
+        sx, sy = x.sum(), y.sum()
+        return (n * (x * y).sum() - sx * sy) / (np.sqrt(n * (x * x).sum() - sx * sx) * np.sqrt(n * (y * y).sum() - sy * sy))
+    for i in range(1, 9):
+        assert abs(corr(i)) <= corr_threshold, ("dimension", i)
+    for y in range(4):
+        for x in range(4):
+            if x or y:
+                assert abs(corr(x * dims + y * nx * dims)) <= corr_threshold, ("pixel", x, y)
+    for i in range(1, 5):
+        assert abs(corr(i * nx * ny * dims)) <= corr_threshold, ("instance", i)
+
+
+def test_tiny_generator_statistics_as_reference_test():
+    _reference_generator_statistics(0, 0.0025)  # SampleGenerator_TinyUniform
+
+
+def test_uniform_generator_statistics_as_reference_test():
+    _reference_generator_statistics(1, 0.002)  # SampleGenerator_Uniform

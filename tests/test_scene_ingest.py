@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from parity import color_close
 
 TABLES = Path(__file__).resolve().parent / "golden" / "tables"
 REF_TABLES = Path("/root/reference/Source/Tables")
@@ -210,7 +211,7 @@ def test_gpu_built_scene_with_tabulated_metals_matches_oracle(gpu):
     assert hit.mean() > 0.5
     topo = _topology_equal(g_rec, o_rec)
     assert topo.mean() >= 0.99, topo.mean()
-    close = np.all(np.abs(g_col[:, :3] - o_col[:, :3]) <= 2e-3 * np.maximum(1.0, np.abs(o_col[:, :3])), axis=1)
+    close = color_close(g_col, o_col)
     assert close[topo].mean() >= 0.995, close[topo].mean()
     assert np.isfinite(g_col).all() and g_col[:, :3].max() > 0
     gm, om = g_col[:, :3].mean(), o_col[:, :3].mean()
