@@ -271,6 +271,7 @@ def _all_materials():
 def _queries(mats, prof, n_per, seed):
     """Per material: wi cosine-distributed, (wo, lobe) from oracle samples (the path tracer's unwind
     evaluations) and random (wo, NonDelta) pairs (its NEE evaluations)."""
+    from pltfalcor_amd import materials as M
     rng = np.random.default_rng(seed)
     rows = []
     for mi, mat in enumerate(mats):

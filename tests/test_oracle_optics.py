@@ -279,3 +279,76 @@ def test_camera_polarizer_malus_law():
     # polarizer off: the Stokes vector passes unchanged (no frame rotation either)
     S = np.array([1.0, 0.3, -0.4, 0.2])
     assert np.array_equal(_polarizer(U, W, 0.0, 1.0, r, np.array([0.6, 0.8, 0]), S), S.astype(np.float32))
+
+
+def test_wavelength_to_xyz_fit_against_cie1931_table():
+    """SpectrumUtilsTests.cpp:40-82: the analytic fit wavelengthToXYZ (SpectrumUtils.slang:43-49, the
+    multi-lobe Gaussian fit of Wyman et al. 2013) against the CIE 1931 1 nm table the reference ships
+    (Utils/Color/Spectra.inl, read by tests/golden/make_cie1931.py), linearly interpolated and zero
+    outside 360-830 nm (SampledSpectrum::eval), on 20,000 stratified wavelengths over 300-900 nm:
+    non-negative, with the maximum squared errors within the reference test's bounds (2.0e-4,
+    6.6e-5, 5.2e-4 for x, y, z). A wrong lobe centre, width or weight exceeds them."""
+    from pathlib import Path
+    tab = np.loadtxt(Path(__file__).parent / "golden" / "cie1931_1nm.csv", delimiter=",", skiprows=1)
+    lam_t, xyz_t = tab[:, 0], tab[:, 1:]
+    assert lam_t[0] == 360 and lam_t[-1] == 830 and np.all(np.diff(lam_t) == 1)
+
+    def table_eval(lam):
+        if lam < 360 or lam > 830:
+            return np.zeros(3)
+        x = (lam - 360.0) / (830.0 - 360.0) * 470.0
+        i = int(np.floor(x))
+        if i + 1 >= 471:
+            return xyz_t[-1]
+        w = x - i
+        return xyz_t[i] * (1 - w) + xyz_t[i + 1] * w
+
+    rng = np.random.default_rng(0)
+    n = 20000
+    lams = 300.0 + (np.arange(n) + rng.uniform(size=n)) / n * 600.0
+    D = C.POINTER(C.c_double)
+    L = O.lib()
+    L.oracle_wavelength_to_xyz_f64.argtypes = [C.c_double, D]
+    max_sq = np.zeros(3)
+    xyz = np.zeros(3)
+    for lam in lams:
+        L.oracle_wavelength_to_xyz_f64(float(lam), O.ptr(xyz, C.c_double))
+        assert np.all(xyz >= 0)
+        max_sq = np.maximum(max_sq, (table_eval(lam) - xyz) ** 2)
+    print("max squared error", max_sq)
+    assert max_sq[0] <= 2.0e-4 and max_sq[1] <= 6.6e-5 and max_sq[2] <= 5.2e-4
+
+
+def _offset_ray_test_impl(p, n):
+    """offset_ray of GeometryHelpersTests.cpp:59-77 (Ray Tracing Gems ch. 6, Waechter and Binder),
+    vectorised in float32/int32: int offsets int(768 n) (truncated), added to the bit patterns of
+    p with the sign of p, and p + (3 / 65536) n where |p| < 1/16."""
+    p = np.asarray(p, np.float32)
+    n = np.asarray(n, np.float32)
+    of_i = (np.float32(3 * 256.0) * n).astype(np.int32)
+    pi = p.view(np.int32) + np.where(p < 0, -of_i, of_i)
+    p_i = pi.astype(np.int32).view(np.float32)
+    near = np.abs(p) < np.float32(1.0 / 16.0)
+    return np.where(near, p + np.float32(3.0 / 65536.0) * n, p_i).astype(np.float32)
+
+
+def test_compute_ray_origin_matches_reference_test_offset_ray():
+    """GeometryHelpersTests.cpp:217-251 (ComputeRayOrigin): computeRayOrigin (GeometryHelpers.slang:
+    47-61) equals the test's CPU offset_ray bit for bit on 2^16 cases, positions uniform in
+    [-1, 1]^3 scaled by 10^(60 i / 2^16 - 30) (1e-30..1e30) with normalised random normals, plus the
+    special cases the scale sweep skips: zeros, signed zeros, the 1/16 switch and axis normals."""
+    rng = np.random.default_rng(5489)
+    n_tests = 1 << 16
+    i = np.arange(n_tests)
+    scale = (10.0 ** (i / n_tests * 60.0 - 30.0)).astype(np.float32)
+    pos = (rng.uniform(-1, 1, (n_tests, 3)).astype(np.float32) * scale[:, None]).astype(np.float32)
+    nrm = rng.uniform(-1, 1, (n_tests, 3)).astype(np.float32)
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    special_p = np.array([[0, 0, 0], [-0.0, 0.0, -0.0], [1 / 16, -1 / 16, 0.0625001], [0.0624999, -0.0624999, 5.0],
+                          [-3.0, 2.0, 1e-38], [1e30, -1e30, 1e-30]], np.float32)
+    special_n = np.array([[0, 0, 1], [0, 1, 0], [1, 0, 0], [-1, 0, 0], [0.6, -0.8, 0], [0, 0, -1]], np.float32)
+    pos = np.concatenate([pos, special_p])
+    nrm = np.concatenate([nrm, special_n])
+    want = _offset_ray_test_impl(pos, nrm)
+    got = np.stack([O.compute_ray_origin(pos[k], nrm[k]) for k in range(len(pos))])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

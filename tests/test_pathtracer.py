@@ -585,15 +585,16 @@ def test_gpu_mnee_flat_occluders_match_oracle(gpu, glass):
 
 
 def _quad_scene(quads, materials):
-    """Scene of horizontal squares: quads = [(y, half size, normal y (+1 / -1), material)], the
-    procedural scene's spectral profiles (profile 2 is the emission spectrum)."""
+    """Scene of horizontal squares: quads = [(y, half size, normal y (+1 / -1), material[, centre x,
+    centre z])], the procedural scene's spectral profiles (profile 2 is the emission spectrum)."""
     from pltfalcor_amd.scene import Scene
     proc = Scene(None, seed=1, sphere_count=0, subdivision=1)
     pa = proc.numpy()
     pos, nrm, idx, tm, mm = [], [], [], [], []
-    for m, (y, hs, ny, mat) in enumerate(quads):
+    for m, q in enumerate(quads):
+        (y, hs, ny, mat), (cx, cz) = q[:4], (q[4:6] if len(q) > 4 else (0.0, 0.0))
         v0 = len(pos)
-        pos += [[-hs, y, -hs], [hs, y, -hs], [hs, y, hs], [-hs, y, hs]]
+        pos += [[cx - hs, y, cz - hs], [cx + hs, y, cz - hs], [cx + hs, y, cz + hs], [cx - hs, y, cz + hs]]
         nrm += [[0, ny, 0]] * 4
         # winding so that the face normal follows the vertex normal
         idx += [[v0, v0 + 2, v0 + 1], [v0, v0 + 3, v0 + 2]] if ny > 0 else [[v0, v0 + 1, v0 + 2], [v0, v0 + 2, v0 + 3]]
@@ -909,3 +910,116 @@ def test_gpu_set_scene_upload_failure_leaves_pass_unbound(gpu):
     torch.cuda.synchronize()
     assert torch.isfinite(color).all()
     p.close()
+
+
+def _band_rows(h, bands, rows):
+    """Row ranges of `bands` bands of `rows` full rows spread over the frame (sky, horizon, objects, ground)."""
+    starts = np.linspace(0, h - rows, bands).astype(int)
+    return [(int(r0), rows) for r0 in starts]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", ["C1", "C2"])
+def test_gpu_workload_configs_match_oracle(gpu, config):
+    """The pass on the workload scenes at their full shapes, against the oracle's render of 64 full
+    rows (four bands of 16 rows over the frame) of the same frame:
+      C1: 512 x 512, 64 icospheres of subdivision 5 (1,310,724 triangles), 4 material kinds, depth 6,
+          manifold NEE on, frame 0, no jitter;
+      C2: 1920 x 1080, 256 icospheres (5,242,884 triangles), 8 material kinds, depth 8, manifold NEE
+          on, frame 0 with the reference graph's Halton jitter (the bench's configuration).
+    On those rows: the parity bar of the small cases (primary hits exact, topology, records, colour
+    within 1e-3 relative + the image-scale floor, albedo and normal). Over the whole frame: finite,
+    non-negative colour with alpha 1, and primary hit ids on the sampled rows bit-identical."""
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import PATH_1, PATH_2, Scene
+    from pltfalcor_amd.bsdf import default_context
+    ctx = default_context()
+    if config == "C1":
+        w, h, cfg, opts = 512, 512, PATH_1, dict(REFERENCE_DEFAULTS, maxBounces=6)
+    else:
+        w, h, cfg, opts = 1920, 1080, PATH_2, dict(REFERENCE_DEFAULTS, maxBounces=8)
+    s = Scene(ctx, aspect=w / h, **cfg)
+    a = s.numpy()
+    assert a["indices"].shape[0] == cfg["sphere_count"] * 20 * 4 ** cfg["subdivision"] + 4  # ground and light quads
+    p = PLTPT(ctx, opts)
+    if config == "C2":
+        p.set_sample_pattern(PLTPT.HALTON, 16)
+    p.set_scene(s)
+    p.set_frame(0)
+    color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    p.execute(w, h, color, albedo, normal)
+    torch.cuda.synchronize()
+    g_rec_all = p.bounce_records(w, h)
+    g_col_all = color.cpu().numpy().reshape(-1, 4)
+    g_aov_all = np.concatenate([albedo.cpu().numpy().reshape(-1, 4), normal.cpu().numpy().reshape(-1, 4)], axis=1)
+    assert np.isfinite(g_col_all).all() and (g_col_all[:, :3] >= 0).all() and (g_col_all[:, 3] == 1).all()
+    cam = np.array(a["camera"], np.float32).copy()
+    cam = np.concatenate([cam, np.zeros(max(0, 16 - cam.size), np.float32)])
+    cam[12], cam[13] = p.jitter()
+    osc = O.OracleScene(a)
+    kw = {OPTION_KW[k]: int(v) for k, v in opts.items() if k in OPTION_KW}
+    kw["mnee_threshold"] = float(opts["MNEESolverThreshold"])
+    cols, recs, aovs, gcols, grecs, gaovs = [], [], [], [], [], []
+    for r0, rows in _band_rows(h, 4, 16):
+        first, count = r0 * w, rows * w
+        o_col, o_rec, o_aov = osc.render(w, h, frame=0, max_bounces=int(opts["maxBounces"]), records=True, aov=True,
+                                         first=first, count=count, camera=cam, threads=0, **kw)
+        cols.append(o_col); recs.append(o_rec); aovs.append(o_aov)
+        sl = slice(first, first + count)
+        gcols.append(g_col_all[sl]); grecs.append(g_rec_all[:, :, sl]); gaovs.append(g_aov_all[sl])
+    osc.close()
+    cat = lambda xs, ax=0: np.concatenate(xs, axis=ax)  # noqa: E731
+    _check_parity(config, cat(gcols), cat(grecs, 2), cat(cols), cat(recs, 2), cat(gaovs), cat(aovs))
+    p.close()
+
+
+def test_oracle_two_bounce_mirror_matches_lambert_closed_form():
+    """Forward unwind beyond depth 0: a camera at (7, 1, 0) looks up at a downward-facing smooth
+    opaque-dielectric mirror (albedo 0.8, roughness 0: a depolarising delta reflection of
+    reflectance 0.8) at y = 3 and sees, in it, the Lambertian ground (albedo 0.5) lit by the 3 x 3
+    quad light at y = 7 (the mirror casts no shadow on the seen patch). maxBounces 1, NEE without
+    MIS: every pixel's light arrives by NEE at the ground vertex (bounce 1) and is carried through the
+    mirror vertex (bounce 0) to the camera, so its expectation is 0.8 x 0.5 / pi x E(x') x Le_rgb,
+    E Lambert's polygon irradiance at the mirrored ground point x'. A dropped unwind step, a wrong
+    vertex order or a wrong contribution bookkeeping at bounce 1 changes it."""
+    from pltfalcor_amd import materials as M
+    mats = [M.diffuse(.5, .5, .5), M.diffuse(.8, .8, .8, emission=2), M.opaque_dielectric(.8, .8, .8, 0.0)]
+    a = _quad_scene([(0.0, 12.0, 1.0, 0), (7.0, 1.5, -1.0, 1), (3.0, 1.5, -1.0, 2, 7.0, 0.0)], mats)
+    w = h = 16
+    cam = a["camera"].copy()
+    cam[0:3], cam[3:6], cam[6:9] = [7.0, 1.0, 0.0], [7.0, 3.0, 0.0], [0.0, 0.0, 1.0]
+    cam[9], cam[10], cam[12:14] = np.radians(30.0), 1.0, 0.0
+    a["camera"] = cam
+    o, d = _camera_rays(cam, w, h)
+    t = (3.0 - o[1]) / d[:, 1]
+    hit = o + t[:, None] * d
+    assert np.all(np.abs(hit[:, 0] - 7.0) < 1.5) and np.all(np.abs(hit[:, 2]) < 1.5)  # every pixel sees the mirror
+    dr = d * np.array([1.0, -1.0, 1.0])
+    g = hit + (3.0 / -dr[:, 1])[:, None] * dr  # the mirrored ray reaches the ground y = 0
+    poly = [np.array(p, float) for p in ([-1.5, 7, -1.5], [1.5, 7, -1.5], [1.5, 7, 1.5], [-1.5, 7, 1.5])]
+    E = np.array([_lambert_polygon_irradiance(p, np.array([0, 1.0, 0]), poly) for p in g])
+    # the light's own colour, seen from below
+    b = dict(a)
+    camL = cam.copy()
+    camL[0:3], camL[3:6], camL[9] = [0.0, 1.0, 0.0], [0.0, 7.0, 0.0], np.radians(10.0)
+    b["camera"] = camL
+    frames = 96
+    Le = np.zeros(3)
+    for f in range(frames):
+        Le += O.render(b, 8, 8, frame=f, max_bounces=0, do_rr=0, is_emitters=0)[:, :3].mean(0)
+    Le /= frames
+    acc = np.zeros((w * h, 4))
+    for f in range(frames):
+        acc += O.render(a, w, h, frame=f, max_bounces=1, do_rr=0, is_emitters=0, do_mis=0, threads=8)
+    acc /= frames
+    expected = 0.8 * 0.5 / np.pi * E[:, None] * Le[None, :]
+    got = acc[:, :3]
+    rel = np.abs(got.mean(0) - expected.mean(0)) / expected.mean(0)
+    print("two-bounce mirror: got", got.mean(0), "expected", expected.mean(0))
+    assert (rel < 0.015).all(), (got.mean(0), expected.mean(0))
+    blk = lambda v: v.reshape(4, 4, 4, 4).sum((1, 3))  # noqa: E731
+    for c in range(3):
+        rb = np.abs(blk(got[:, c].reshape(16, 16)) - blk(expected[:, c].reshape(16, 16))) / blk(expected[:, c].reshape(16, 16))
+        assert rb.max() < 0.04, (c, rb.max())
