@@ -405,3 +405,58 @@ def test_gpu_material_sample_batch_matches_oracle(gpu):
             mism += 0 if ok else 1
     print(f"sample mismatches {mism} / {q.shape[1]}")
     assert mism <= 0.001 * q.shape[1] + 1
+
+
+def test_lambda_pdf_limits():
+    """lambdaPdf, the spectral-MIS pdfs of the HWSS components (IMaterialInstance.slang:122-184), where
+    the mathematics fixes them: lobes whose sampling does not depend on the wavelength give 1 for
+    every component (diffuse, opaque dielectric, thin dielectric, multilayer, and the delta
+    reflections of the conductor and the coated materials); the dielectric's delta transmission
+    gives 1 for every component when it does not disperse (B = 0, identical refracted directions),
+    and with dispersion 1 for the hero, tending to 1 as a component's wavelength tends to the hero's
+    and falling as the refracted directions separate; a scattering lobe evaluated at four equal
+    wavelengths gives four equal pdfs. A component evaluated at the wrong wavelength or a mutual
+    coherence of the wrong directions fails one of these."""
+    from pltfalcor_amd import materials as M
+    prof, _, _ = _profiles()
+    wi = np.array([0.3, -0.2, 0.93])
+    wi /= np.linalg.norm(wi)
+    wo = np.array([-0.25, 0.1, 0.96])
+    wo /= np.linalg.norm(wo)
+    S = np.tile([1.0, 0.2, -0.1, 0.05], 4)
+    lam = np.array([450.0, 535.0, 620.0, 705.0])
+    ones = [
+        (M.diffuse(.7, .4, .2), 0x01), (M.opaque_dielectric(.3, .6, .8, 0.2), 0x02),
+        (M.opaque_dielectric(.3, .6, .8, 0.2), 0x04), (M.thin_dielectric(1.5, 1e-3, 0.1, (0.3, -0.2, 1.0)), 0x04),
+        (M.multilayer(1.38, 2.3, 0.1, 0.06, 20), 0x04), (M.conductor(0.1, 0, 1), 0x04),
+        (M.coated_conductor(0.12, 1.5, 1.5, 0, 1), 0x04), (M.coated_opaque(.8, .3, .2, 0.15, 1.5, 2.0), 0x04),
+    ]
+    for mat, lobe in ones:
+        _, _, lp = O.material_eval(bytes(mat), prof, 2, wi, wo, lobe, lam, S, 30.0)
+        assert np.array_equal(lp, np.ones(4)), (mat.type, lobe, lp)
+    # dielectric delta transmission: wo on the far side, the refracted direction of wi
+    eta = 1.5
+    ct = np.sqrt(1 - (1 - wi[2] ** 2) / eta ** 2)
+    wt = np.array([-wi[0] / eta, -wi[1] / eta, -ct])
+    nodisp = M.dielectric(eta, 0.0)
+    nodisp.specular[0] = M.h(0.0)  # Cauchy B = 0
+    _, _, lp = O.material_eval(bytes(nodisp), prof, 2, wi, wt, 0x40, lam, S, 30.0)
+    assert np.allclose(lp, 1.0, rtol=0, atol=1e-12), lp
+    disp = M.dielectric(eta, 0.0)
+    # components j >= 1 compare wo refracted back at their own IoR with wi: the closer the Cauchy
+    # IoR B / lambda^2 + (ior - 2.897 B) of their wavelength to the 1.5 that made wo, the nearer to 1
+    B = float(np.array(disp.specular[0], np.uint16).view(np.float16))
+    lam2 = np.array([450.0, 590.0, 640.0, 400.0])
+    eta_j = (1.5 - 2.897 * B) + B / (lam2 * 1e-3) ** 2
+    _, _, lp = O.material_eval(bytes(disp), prof, 2, wi, wt, 0x80, lam2, S, 30.0)
+    assert lp[0] == 1.0 and np.all(lp[1:] > 0) and np.all(lp[1:] <= 1.0), lp
+    order = np.argsort(np.abs(eta_j[1:] - 1.5))
+    assert np.all(np.diff(lp[1:][order]) < 0), (lp, eta_j)
+    _, _, lp_same = O.material_eval(bytes(disp), prof, 2, wi, wt, 0x80, np.full(4, 590.0), S, 30.0)
+    assert lp_same[1] == lp_same[2] == lp_same[3] and lp_same[1] > 1 - 1e-6, lp_same
+    # scattering lobes at four equal wavelengths: four equal pdfs
+    for mat in (M.conductor(0.1, 0, 1, gamma=2.5), M.dielectric(1.5, 0.05), M.coated_conductor(0.12, 1.5, 1.5, 0, 1)):
+        _, _, lp = O.material_eval(bytes(mat), prof, 2, wi, wo, 0x02, np.full(4, 532.0), S, 30.0)
+        assert lp[0] > 0 and np.all(lp == lp[0]), (mat.type, lp)
+        _, _, lp2 = O.material_eval(bytes(mat), prof, 2, wi, wo, 0x02, lam, S, 30.0)
+        assert len(set(lp2.tolist())) > 1, (mat.type, lp2)  # the scattering pdf depends on the wavenumber
