@@ -439,6 +439,30 @@ int plt_debug_fail_upload(int n);
 enum { PLT_SCHEDULE_MEGAKERNEL = 0, PLT_SCHEDULE_WAVEFRONT = 1 };
 int plt_pass_set_schedule(plt_pass* pass, int schedule);
 
+/* Launch configuration of the pass (no counterpart in the reference, whose two DXR dispatches per
+ * tile are fixed): the sample-pass schedule, the megakernel's block size (64, 128 or 256 threads),
+ * the wavefront schedule's persistent grid (64-thread blocks per CU, 1..64), and graph replay: with
+ * `graph` set, each execute captures the sample pass (with its per-bounce wavefront launches) and
+ * the solve pass into HIP graphs on a stream of the pass's own, updates the instantiated graphs with
+ * the new frame's arguments and launches them on the caller's stream. Every configuration renders
+ * identical outputs. */
+typedef struct plt_launch_config {
+    uint32_t schedule;         /* PLT_SCHEDULE_* */
+    uint32_t sample_block;     /* 64, 128 or 256 */
+    uint32_t wf_blocks_per_cu; /* 1..64 */
+    uint32_t graph;            /* 0 or 1 */
+} plt_launch_config;
+int plt_pass_set_launch_config(plt_pass* pass, const plt_launch_config* config);
+int plt_pass_get_launch_config(const plt_pass* pass, plt_launch_config* config);
+int plt_pass_set_graph(plt_pass* pass, int enable);
+/* Autotuner: renders `frames` frames (from the current frame counter, outputs in `data`) with the
+ * current configuration and with each candidate -- the megakernel at 64, 128 and 256 threads per
+ * block, the wavefront schedule at 4, 8, 16 and 32 blocks per CU, then the fastest of these replayed
+ * from graphs -- keeps the fastest, and restores the frame counter and the jitter pattern. Writes
+ * the chosen configuration and the mean frame time of the starting and the chosen one (ms). */
+int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t frames, plt_stream stream,
+                      plt_launch_config* best, double* default_ms, double* best_ms);
+
 /* Kernel timing, the counterpart of the reference's profiler scopes around its sample and solve
  * dispatches (FALCOR_PROFILE in Falcor passes; PLTPT.cpp:487-495 issues the two dispatches):
  * while enabled, every execute records HIP events on its stream around the sample and solve

@@ -51,6 +51,12 @@ class Material(C.Structure):
                 ("ior_k_profile", C.c_uint32)]
 
 
+class LaunchConfig(C.Structure):
+    """plt_launch_config: sample-pass schedule, megakernel block size, wavefront blocks per CU, graph replay."""
+    _fields_ = [("schedule", C.c_uint32), ("sample_block", C.c_uint32), ("wf_blocks_per_cu", C.c_uint32),
+                ("graph", C.c_uint32)]
+
+
 class Camera(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov_y", C.c_float),
                 ("aspect", C.c_float), ("focal_distance", C.c_float), ("jitter", C.c_float * 2),
@@ -109,6 +115,11 @@ SIGNATURES = [
     ("plt_material_eval_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                           C.c_void_p]),
     ("plt_pass_set_schedule", C.c_int, [C.c_void_p, C.c_int]),
+    ("plt_pass_set_graph", C.c_int, [C.c_void_p, C.c_int]),
+    ("plt_pass_set_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("plt_pass_get_launch_config", C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    ("plt_pass_autotune", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(LaunchConfig),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("plt_surrogate_create", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     ("plt_surrogate_destroy", None, [C.c_void_p]),
     ("plt_surrogate_param_count", C.c_uint32, []),

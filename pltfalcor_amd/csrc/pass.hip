@@ -175,8 +175,13 @@ struct plt_pass {
     bool profiling = false;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
-    // sample-pass schedule (PLT_SCHEDULE_*) and the wavefront schedule's buffers
+    // sample-pass schedule (PLT_SCHEDULE_*), its launch parameters, and the wavefront schedule's buffers
     int schedule = PLT_SCHEDULE_MEGAKERNEL;
+    plt::LaunchConfig launch;
+    // graph replay: captured on capture_stream, instantiated once and updated per frame
+    bool graph = false;
+    hipStream_t capture_stream = nullptr;
+    hipGraphExec_t graph_sample = nullptr, graph_solve = nullptr;
     void* d_wf = nullptr;
     size_t wf_paths = 0;
     // camera jitter pattern of the G-buffer pass (Center: the camera's own jitter)
@@ -240,6 +245,48 @@ void camera_frame(const plt_camera& c, float out[12]) {
     }
 }
 int bind_scene(plt_pass* p, plt_scene* scene);
+
+void drop_graphs(plt_pass* p) {
+    if (p->graph_sample) (void)hipGraphExecDestroy(p->graph_sample);
+    if (p->graph_solve) (void)hipGraphExecDestroy(p->graph_solve);
+    p->graph_sample = p->graph_solve = nullptr;
+}
+
+// Captures the launches `seq` enqueues into a graph on the pass's capture stream, updates the
+// instantiated graph `exec` with it (kernel arguments change every frame: frame number, jitter),
+// instantiating anew when the update is refused (a different launch sequence), and launches it
+// on `st`.
+template <class Seq>
+int run_captured(plt_pass* p, hipGraphExec_t& exec, Seq&& seq, hipStream_t st) {
+    if (!p->capture_stream && hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_execute: capture stream creation failed");
+    if (hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_execute: hipStreamBeginCapture failed");
+    const int rc = seq(p->capture_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(p->capture_stream, &g);
+    if (rc < 0 || ec != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc < 0 ? rc : set_error(PLT_EHIP, "plt_pass_execute: hipStreamEndCapture failed");
+    }
+    if (exec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(exec, g, &err_node, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+        }
+    }
+    if (!exec && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        exec = nullptr;
+        return set_error(PLT_EHIP, "plt_pass_execute: hipGraphInstantiate failed");
+    }
+    (void)hipGraphDestroy(g);
+    if (hipGraphLaunch(exec, st) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_execute: hipGraphLaunch failed");
+    return PLT_OK;
+}
 }  // namespace
 }  // namespace plt
 
@@ -268,7 +315,41 @@ int plt_pass_destroy(plt_pass* p) {
     if (p->d_tile_draws) (void)hipFree(p->d_tile_draws);
     if (p->d_wf) (void)hipFree(p->d_wf);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
+    plt::drop_graphs(p);
+    if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
     delete p;
+    return PLT_OK;
+}
+
+int plt_pass_set_graph(plt_pass* p, int enable) {
+    if (!p) return set_error(PLT_EINVAL, "plt_pass_set_graph: NULL pass");
+    p->graph = enable != 0;
+    if (!p->graph) plt::drop_graphs(p);
+    return PLT_OK;
+}
+
+int plt_pass_set_launch_config(plt_pass* p, const plt_launch_config* c) {
+    if (!p || !c) return set_error(PLT_EINVAL, "plt_pass_set_launch_config: bad arguments");
+    if (c->schedule != PLT_SCHEDULE_MEGAKERNEL && c->schedule != PLT_SCHEDULE_WAVEFRONT)
+        return set_error(PLT_EINVAL, "plt_pass_set_launch_config: unknown schedule");
+    if (c->sample_block != 64 && c->sample_block != 128 && c->sample_block != 256)
+        return set_error(PLT_EINVAL, "plt_pass_set_launch_config: sample_block must be 64, 128 or 256");
+    if (c->wf_blocks_per_cu < 1 || c->wf_blocks_per_cu > 64)
+        return set_error(PLT_EINVAL, "plt_pass_set_launch_config: wf_blocks_per_cu must be in [1, 64]");
+    p->schedule = int(c->schedule);
+    p->launch.sample_block = c->sample_block;
+    p->launch.wf_blocks_per_cu = c->wf_blocks_per_cu;
+    p->graph = c->graph != 0;
+    if (!p->graph) plt::drop_graphs(p);
+    return PLT_OK;
+}
+
+int plt_pass_get_launch_config(const plt_pass* p, plt_launch_config* c) {
+    if (!p || !c) return set_error(PLT_EINVAL, "plt_pass_get_launch_config: bad arguments");
+    c->schedule = uint32_t(p->schedule);
+    c->sample_block = p->launch.sample_block;
+    c->wf_blocks_per_cu = p->launch.wf_blocks_per_cu;
+    c->graph = p->graph ? 1u : 0u;
     return PLT_OK;
 }
 
@@ -514,19 +595,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     fr.albedo = reinterpret_cast<float4*>(rd->albedo);
     fr.normal = reinterpret_cast<float4*>(rd->normal);
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    if (p->profiling) {
-        while (p->events.size() < p->events_used + 3) {
-            hipEvent_t e;
-            if (hipEventCreate(&e) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_execute: hipEventCreate failed");
-            p->events.push_back(e);
-        }
-        for (int i = 0; i < 3; ++i) ev[i] = p->events[p->events_used + i];
-        p->events_used += 3;
-        (void)hipEventRecord(ev[0], st);
-    }
-    int rc;
-    if (fr.tile_draws && (rc = plt::launch_tile_draws(fr, tiles, p->d_tile_draws, st)) < 0) return rc;
+    // device buffers first: nothing is allocated once launches are enqueued (or captured)
+    plt::WfBuffers w{};
     if (p->schedule == PLT_SCHEDULE_WAVEFRONT) {
         if (npaths > p->wf_paths) {  // per path: 5 x 16 B state + 4 B length + 4 x 4 B queues and keys
             if (p->d_wf) (void)hipFree(p->d_wf);
@@ -540,7 +610,6 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         }
         char* b = static_cast<char*>(p->d_wf);
         const size_t cap = p->wf_paths;
-        plt::WfBuffers w;
         w.rng = reinterpret_cast<uint4*>(b);
         w.org = reinterpret_cast<float4*>(b + 16 * cap);
         w.dir = reinterpret_cast<float4*>(b + 32 * cap);
@@ -551,14 +620,42 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         w.sorted = reinterpret_cast<uint32_t*>(b + 92 * cap);
         w.keys = reinterpret_cast<uint32_t*>(b + 96 * cap);
         w.counters = reinterpret_cast<uint32_t*>(b + 100 * cap);
-        rc = plt::launch_sample_pass_wavefront(sc, fr, w, st);
-    } else {
-        rc = plt::launch_sample_pass(sc, fr, st);
     }
-    if (rc < 0) return rc;
-    if (ev[1]) (void)hipEventRecord(ev[1], st);
-    rc = plt::launch_solve_pass(sc, fr, st);
-    if (rc < 0) return rc;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (p->profiling) {
+        while (p->events.size() < p->events_used + 3) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_execute: hipEventCreate failed");
+            p->events.push_back(e);
+        }
+        for (int i = 0; i < 3; ++i) ev[i] = p->events[p->events_used + i];
+        p->events_used += 3;
+    }
+    const int schedule = p->schedule;
+    const plt::LaunchConfig lc = p->launch;
+    float* tile_draws = p->d_tile_draws;
+    // the sample pass (with the per-tile generator's draws) and the solve pass, each a sequence of
+    // launches on one stream
+    auto sample_seq = [&](hipStream_t s) -> int {
+        int rc;
+        if (fr.tile_draws && (rc = plt::launch_tile_draws(fr, tiles, tile_draws, s)) < 0) return rc;
+        return schedule == PLT_SCHEDULE_WAVEFRONT ? plt::launch_sample_pass_wavefront(sc, fr, w, lc, s)
+                                                   : plt::launch_sample_pass(sc, fr, lc, s);
+    };
+    auto solve_seq = [&](hipStream_t s) -> int { return plt::launch_solve_pass(sc, fr, s); };
+    int rc;
+    if (ev[0]) (void)hipEventRecord(ev[0], st);
+    if (p->graph) {
+        // each pass replayed from a graph captured on the pass's own stream (a null caller stream
+        // cannot be captured); the graph launches on the caller's stream, so ordering is the caller's
+        if ((rc = plt::run_captured(p, p->graph_sample, sample_seq, st)) < 0) return rc;
+        if (ev[1]) (void)hipEventRecord(ev[1], st);
+        if ((rc = plt::run_captured(p, p->graph_solve, solve_seq, st)) < 0) return rc;
+    } else {
+        if ((rc = sample_seq(st)) < 0) return rc;
+        if (ev[1]) (void)hipEventRecord(ev[1], st);
+        if ((rc = solve_seq(st)) < 0) return rc;
+    }
     if (ev[2]) (void)hipEventRecord(ev[2], st);
     ++p->frame;  // PLTPT.cpp:498
     p->last_words = words;
@@ -579,6 +676,71 @@ int plt_pass_set_profiling(plt_pass* p, int enable) {
     p->profiling = enable != 0;
     p->events_used = 0;
     return PLT_OK;
+}
+
+int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, plt_stream stream, plt_launch_config* best,
+                      double* default_ms, double* best_ms) {
+    if (!p || !rd || frames == 0 || frames > 64) return set_error(PLT_EINVAL, "plt_pass_autotune: bad arguments (frames 1..64)");
+    if (!p->ctx || !p->scene) return set_error(PLT_ESTATE, "plt_pass_autotune: no device context or scene");
+    // the state the candidates' executes advance, restored afterwards
+    const uint32_t frame0 = p->frame;
+    const plt::SamplePattern pattern0 = p->pattern;
+    const bool prof0 = p->profiling;
+    p->profiling = false;
+    plt_launch_config start{};
+    (void)plt_pass_get_launch_config(p, &start);
+    std::vector<plt_launch_config> cands;
+    for (uint32_t b : {64u, 128u, 256u}) cands.push_back({PLT_SCHEDULE_MEGAKERNEL, b, start.wf_blocks_per_cu, 0u});
+    for (uint32_t g : {4u, 8u, 16u, 32u}) cands.push_back({PLT_SCHEDULE_WAVEFRONT, start.sample_block, g, 0u});
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    auto time_config = [&](const plt_launch_config& c, double& ms) -> int {
+        int rc = plt_pass_set_launch_config(p, &c);
+        if (rc < 0) return rc;
+        p->frame = frame0;
+        p->pattern = pattern0;
+        if ((rc = plt_pass_execute(p, rd, stream)) < 0) return rc;  // warm-up (and capture)
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_autotune: hipEventCreate failed");
+        p->frame = frame0;
+        p->pattern = pattern0;
+        p->profiling = false;
+        (void)hipEventRecord(e0, st);
+        for (uint32_t f = 0; f < frames && rc >= 0; ++f) rc = plt_pass_execute(p, rd, stream);
+        (void)hipEventRecord(e1, st);
+        float t = 0.f;
+        if (rc >= 0 && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            rc = set_error(PLT_EHIP, "plt_pass_autotune: event timing failed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ms = double(t) / frames;
+        return rc;
+    };
+    double dms = 0.0, bms = 1e300;
+    plt_launch_config bc = start;
+    int rc = time_config(start, dms);
+    for (const auto& c : cands) {
+        if (rc < 0) break;
+        double ms = 0.0;
+        rc = time_config(c, ms);
+        if (rc >= 0 && ms < bms) { bms = ms; bc = c; }
+    }
+    if (rc >= 0) {  // graph replay of the best launch sequence
+        plt_launch_config g = bc;
+        g.graph = 1u;
+        double ms = 0.0;
+        rc = time_config(g, ms);
+        if (rc >= 0 && ms < bms) { bms = ms; bc = g; }
+    }
+    p->frame = frame0;
+    p->pattern = pattern0;
+    p->profiling = prof0;
+    p->events_used = 0;
+    const int rc2 = plt_pass_set_launch_config(p, rc >= 0 ? &bc : &start);
+    if (rc < 0) return rc;
+    if (best) *best = bc;
+    if (default_ms) *default_ms = dms;
+    if (best_ms) *best_ms = bms;
+    return rc2;
 }
 
 int plt_pass_get_kernel_times(plt_pass* p, double* sample_ms, double* solve_ms, uint32_t* executes) {

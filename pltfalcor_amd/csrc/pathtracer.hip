@@ -5,10 +5,13 @@ namespace plt {
 namespace {
 
 
-// Megakernel schedule: one lane follows its path to the end.
-__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) sample_kernel(PtScene sc, PtFrame fr) {
-    __shared__ int stack[kTraversalStackDepth * kPtBlock];
-    const uint32_t li = blockIdx.x * kPtBlock + threadIdx.x;
+// Megakernel schedule: one lane follows its path to the end. B threads per block (64, 128 or 256:
+// a launch parameter the pass's autotuner picks), each with a traversal stack of
+// kTraversalStackDepth entries interleaved in LDS.
+template <int B>
+__global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 4))) sample_kernel(PtScene sc, PtFrame fr) {
+    __shared__ int stack[kTraversalStackDepth * B];
+    const uint32_t li = blockIdx.x * B + threadIdx.x;
     if (li >= fr.n_paths) return;
     const uint32_t pid = fr.path_base + li;
     const uint32_t px = pid % fr.width, py = pid / fr.width;
@@ -17,9 +20,15 @@ __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4
     PathState ps = start_path(sc, fr, px, py, pid, sg);
     while (ps.len < fr.max_bounces + 1) {
         const Ray ray{ps.org.x, ps.org.y, ps.org.z, ps.dir.x, ps.dir.y, ps.dir.z, 0.f, kFltMax};
-        const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, ray, stk, kPtBlock);
+        const Hit h = traverse<false, kTraversalStackDepth>(sc.tsc, ray, stk, B);
         if (!sample_vertex(sc, fr, h, li, pid, ps, sg)) break;
     }
+}
+
+// Clears the wavefront counters (a kernel, not a memset node, so the sequence can be replayed from
+// a captured graph).
+__global__ void wf_clear_kernel(uint32_t* counters) {
+    if (threadIdx.x < kWfCounters) counters[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -158,22 +167,30 @@ __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4
 
 }  // namespace
 
-int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
+int launch_sample_pass(const PtScene& sc, const PtFrame& fr, const LaunchConfig& lc, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
-    hipLaunchKernelGGL(sample_kernel, dim3((fr.n_paths + kPtBlock - 1) / kPtBlock), dim3(kPtBlock), 0, st, sc, fr);
+    const uint32_t n = fr.n_paths;
+    switch (lc.sample_block) {
+        case 64: hipLaunchKernelGGL(sample_kernel<64>, dim3((n + 63) / 64), dim3(64), 0, st, sc, fr); break;
+        case 128: hipLaunchKernelGGL(sample_kernel<128>, dim3((n + 127) / 128), dim3(128), 0, st, sc, fr); break;
+        case 256: hipLaunchKernelGGL(sample_kernel<256>, dim3((n + 255) / 256), dim3(256), 0, st, sc, fr); break;
+        default: return set_error(PLT_EINVAL, "sample pass: block size must be 64, 128 or 256");
+    }
     return check_launch("sample_kernel");
 }
-int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, hipStream_t st) {
+int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, const LaunchConfig& lc, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
-    if (hipMemsetAsync(w.counters, 0, kWfCounters * sizeof(uint32_t), st) != hipSuccess)
-        return set_error(PLT_EHIP, "wavefront sample pass: counter clear failed");
+    if (lc.wf_blocks_per_cu < 1 || lc.wf_blocks_per_cu > 64) return set_error(PLT_EINVAL, "wavefront sample pass: blocks per CU in [1, 64]");
+    hipLaunchKernelGGL(wf_clear_kernel, dim3(1), dim3(64), 0, st, w.counters);
     const uint32_t n = fr.n_paths;
     hipLaunchKernelGGL(wf_start_kernel, dim3((n + kWfBlock - 1) / kWfBlock), dim3(kWfBlock), 0, st, sc, fr, w);
-    // persistent grids sized to the resident capacity (the kernels stride over the live queue)
+    // persistent grids sized to the resident capacity (the kernels stride over the live queue):
+    // wf_blocks_per_cu 64-thread blocks per CU for traversal and shading, a quarter as many
+    // 256-thread blocks for the scatter
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint32_t g64 = std::min((n + kPtBlock - 1) / kPtBlock, uint32_t(16 * cus));
-    const uint32_t g256 = std::min((n + kWfBlock - 1) / kWfBlock, uint32_t(8 * cus));
+    const uint32_t g64 = std::min((n + kPtBlock - 1) / kPtBlock, uint32_t(lc.wf_blocks_per_cu * cus));
+    const uint32_t g256 = std::min((n + kWfBlock - 1) / kWfBlock, uint32_t(std::max(1u, lc.wf_blocks_per_cu / 2) * cus));
     for (uint32_t b = 0; b <= fr.max_bounces; ++b) {
         const int cur = int(b & 1u);
         hipLaunchKernelGGL(wf_trace_kernel, dim3(g64), dim3(kPtBlock), 0, st, sc, fr, w, cur);

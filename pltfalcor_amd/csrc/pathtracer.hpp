@@ -101,8 +101,15 @@ struct WfBuffers {
     uint32_t* counters;
 };
 
-int launch_sample_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
-int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, hipStream_t st);
+// Launch parameters of the sample pass (the autotuner's search space): the megakernel's block size
+// and the wavefront schedule's persistent grid, in 64-thread blocks per CU.
+struct LaunchConfig {
+    uint32_t sample_block = 64;
+    uint32_t wf_blocks_per_cu = 16;
+};
+
+int launch_sample_pass(const PtScene& sc, const PtFrame& fr, const LaunchConfig& lc, hipStream_t st);
+int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, const LaunchConfig& lc, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 

@@ -115,6 +115,39 @@ class PLTPT:
         PLTPT.WAVEFRONT (material-sorted per-bounce queues); both write identical records."""
         _lib.check(self.lib.plt_pass_set_schedule(self.handle, int(schedule)))
 
+    def set_launch_config(self, schedule: int | None = None, sample_block: int | None = None,
+                          wf_blocks_per_cu: int | None = None, graph: bool | None = None):
+        """Launch parameters (plt_launch_config); None keeps the current value."""
+        c = self.launch_config()
+        for k, v in (("schedule", schedule), ("sample_block", sample_block), ("wf_blocks_per_cu", wf_blocks_per_cu),
+                     ("graph", graph)):
+            if v is not None:
+                c[k] = int(v)
+        lc = _lib.LaunchConfig(c["schedule"], c["sample_block"], c["wf_blocks_per_cu"], c["graph"])
+        _lib.check(self.lib.plt_pass_set_launch_config(self.handle, C.byref(lc)))
+
+    def launch_config(self) -> dict:
+        lc = _lib.LaunchConfig()
+        _lib.check(self.lib.plt_pass_get_launch_config(self.handle, C.byref(lc)))
+        return {k: getattr(lc, k) for k, _ in lc._fields_}
+
+    def set_graph(self, enable: bool):
+        """Replay the sample and solve passes from captured HIP graphs (per-frame arguments updated)."""
+        _lib.check(self.lib.plt_pass_set_graph(self.handle, int(bool(enable))))
+
+    def autotune(self, width: int, height: int, color: torch.Tensor, albedo: torch.Tensor, normal: torch.Tensor,
+                 frames: int = 2, stream=None):
+        """Times the candidate launch configurations on `frames` frames, keeps the fastest (frame
+        counter and jitter pattern restored). Returns (chosen config, default ms/frame, best ms/frame)."""
+        rd = RenderData(width, height, color.data_ptr(), albedo.data_ptr(), normal.data_ptr(), None)
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        best = _lib.LaunchConfig()
+        d, b = C.c_double(), C.c_double()
+        _lib.check(self.lib.plt_pass_autotune(self.handle, C.byref(rd), int(frames), C.c_void_p(stream.cuda_stream),
+                                              C.byref(best), C.byref(d), C.byref(b)))
+        return {k: getattr(best, k) for k, _ in best._fields_}, d.value, b.value
+
     def set_profiling(self, enable: bool):
         _lib.check(self.lib.plt_pass_set_profiling(self.handle, int(bool(enable))))
 

@@ -1023,3 +1023,92 @@ def test_oracle_two_bounce_mirror_matches_lambert_closed_form():
     for c in range(3):
         rb = np.abs(blk(got[:, c].reshape(16, 16)) - blk(expected[:, c].reshape(16, 16))) / blk(expected[:, c].reshape(16, 16))
         assert rb.max() < 0.04, (c, rb.max())
+
+
+@pytest.mark.gpu
+def test_gpu_launch_configs_and_graph_replay_are_identical(gpu):
+    """Every launch configuration of the pass -- the megakernel at 64, 128 and 256 threads per block,
+    the wavefront schedule at 4 and 32 blocks per CU, each eager and replayed from captured HIP graphs
+    -- writes the default configuration's bounce records, colour, albedo and normal bit for bit. The
+    graphs are replayed for several frames with eager work and synchronisations between the replays,
+    and every replay is checked against an eager render of the same frame."""
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import Scene
+    from pltfalcor_amd.bsdf import default_context
+    ctx = default_context()
+    s = Scene(ctx, seed=3, sphere_count=24, subdivision=3, material_kinds=8)
+    w, h = 96, 64
+    opts = dict(BASE_OPTS, maxBounces=6, doMNEE=1)
+
+    def render(cfg, frame):
+        p = PLTPT(ctx, opts)
+        p.set_sample_pattern(PLTPT.HALTON, 16)
+        p.set_launch_config(**cfg)
+        p.set_scene(s)
+        out = []
+        color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+        albedo, normal = torch.empty_like(color), torch.empty_like(color)
+        for f in range(frame + 1):
+            color.fill_(float("nan"))
+            p.execute(w, h, color, albedo, normal)
+            torch.cuda.synchronize()
+            junk = torch.randn(1 << 16, device="cuda").sum().item()  # eager work between replays
+            assert np.isfinite(junk)
+            out.append((p.bounce_records(w, h), color.cpu().numpy().copy(), albedo.cpu().numpy().copy(),
+                        normal.cpu().numpy().copy()))
+        p.close()
+        return out
+
+    frames = 3
+    ref = render(dict(schedule=PLTPT.MEGAKERNEL, sample_block=64, graph=False), frames - 1)
+    L = [_path_len(r[0]) for r in ref]
+    for cfg in (dict(schedule=PLTPT.MEGAKERNEL, sample_block=128), dict(schedule=PLTPT.MEGAKERNEL, sample_block=256),
+                dict(schedule=PLTPT.MEGAKERNEL, sample_block=64, graph=True),
+                dict(schedule=PLTPT.MEGAKERNEL, sample_block=256, graph=True),
+                dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=4), dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=32),
+                dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=16, graph=True)):
+        got = render(dict(dict(sample_block=64, wf_blocks_per_cu=16, graph=False), **cfg), frames - 1)
+        for f in range(frames):
+            valid = np.arange(ref[f][0].shape[0])[:, None] < L[f][None, :]
+            assert np.array_equal(ref[f][0].transpose(0, 2, 1)[valid], got[f][0].transpose(0, 2, 1)[valid]), (cfg, f)
+            for k in (1, 2, 3):
+                assert np.array_equal(ref[f][k].view(np.uint32), got[f][k].view(np.uint32)), (cfg, f, k)
+
+
+@pytest.mark.gpu
+def test_gpu_autotune_picks_a_config_and_restores_state(gpu):
+    """The autotuner times the candidates, returns a valid configuration no slower than the starting
+    one, sets it, and leaves the frame counter and the jitter pattern where they were: the next frame
+    renders as it would have without the autotuning."""
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import Scene
+    from pltfalcor_amd.bsdf import default_context
+    ctx = default_context()
+    s = Scene(ctx, seed=5, sphere_count=16, subdivision=3, material_kinds=8)
+    w, h = 128, 96
+    opts = dict(BASE_OPTS, maxBounces=6)
+    color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    ref = PLTPT(ctx, opts)
+    ref.set_sample_pattern(PLTPT.HALTON, 16)
+    ref.set_scene(s)
+    ref.set_frame(2)
+    ref.execute(w, h, color, albedo, normal)
+    torch.cuda.synchronize()
+    want = color.cpu().numpy().copy()
+    p = PLTPT(ctx, opts)
+    p.set_sample_pattern(PLTPT.HALTON, 16)
+    p.set_scene(s)
+    p.set_frame(2)
+    best, d_ms, b_ms = p.autotune(w, h, color, albedo, normal, frames=2)
+    print("autotune:", best, f"default {d_ms:.3f} ms, best {b_ms:.3f} ms")
+    assert best["schedule"] in (0, 1) and best["sample_block"] in (64, 128, 256) and 1 <= best["wf_blocks_per_cu"] <= 64
+    assert b_ms <= d_ms * 1.0001 and p.launch_config() == best
+    color.zero_()
+    p.execute(w, h, color, albedo, normal)
+    torch.cuda.synchronize()
+    assert np.array_equal(color.cpu().numpy().view(np.uint32), want.view(np.uint32))
+    p.close()
+    ref.close()
