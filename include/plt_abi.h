@@ -610,6 +610,16 @@ int plt_tonemapper_get_color_transform(const plt_tonemapper* pass, float* out9);
  * luminance the exposure used. */
 int plt_tonemapper_execute(plt_tonemapper* pass, const float* src, uint32_t width, uint32_t height, float* dst,
                            float* mean_log_luminance, plt_stream stream);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): plt_tonemapper_execute
+ * with the same kernels and results, whose auto-exposure reduction works in a fixed 512 KiB
+ * workspace compiled into the library (one per GPU, held until the process ends) instead of a
+ * buffer it allocates. Calls no allocating routine. Images whose reduction needs more than 65,536
+ * tiles (bit_floor(width) / 32 x bit_floor(height) / 32, i.e. either bit_floor above 8192) return
+ * PLT_EINVAL. Every call on one GPU shares that workspace: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+int plt_tonemapper_execute_noalloc(plt_tonemapper* pass, const float* src, uint32_t width, uint32_t height, float* dst,
+                                   float* mean_log_luminance, plt_stream stream);
 
 #ifdef __cplusplus
 }

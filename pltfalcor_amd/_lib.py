@@ -187,6 +187,8 @@ SIGNATURES = [
     ("plt_tonemapper_get_color_transform", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("plt_tonemapper_execute", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                          C.c_void_p]),
+    ("plt_tonemapper_execute_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                                 C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None

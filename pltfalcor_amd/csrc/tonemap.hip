@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include "plt_internal.hpp"
 
@@ -436,33 +437,68 @@ int plt_tonemapper_get_color_transform(const plt_tonemapper* t, float* out9) {
     return PLT_OK;
 }
 
-int plt_tonemapper_execute(plt_tonemapper* t, const float* src, uint32_t width, uint32_t height, float* dst,
-                           float* mean_log_luminance, plt_stream stream) {
-    if (!t) return set_error(PLT_EINVAL, "plt_tonemapper_execute: NULL pass");
-    if (!t->ctx) return set_error(PLT_ESTATE, "plt_tonemapper_execute: pass created without a context");
-    if (!src || !dst || width == 0 || height == 0 || width > 65536 || height > 65536)
-        return set_error(PLT_EINVAL, "plt_tonemapper_execute: bad arguments (sizes 1..65536)");
-    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u)
-        return set_error(PLT_EINVAL, "plt_tonemapper_execute: src and dst must be 16-byte aligned");
-    const uint64_t n64 = uint64_t(width) * height;
-    if (n64 > (1ull << 31)) return set_error(PLT_EINVAL, "plt_tonemapper_execute: image larger than 2^31 pixels");
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (hipSetDevice(t->ctx->device) != hipSuccess) return set_error(PLT_EHIP, "plt_tonemapper_execute: hipSetDevice failed");
+}  // extern "C"
 
+namespace {
+
+// The fixed luminance-reduction workspace of plt_tonemapper_execute_noalloc: two halves of
+// kNoAllocTiles floats (the per-tile means ping-pong between them), 512 KiB on each device, held
+// until the process ends. Host code never passes this name to a kernel: it is a host-side stand-in,
+// and the device address of each GPU's copy comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr size_t kNoAllocTiles = 65536;  // 256 x 256 tiles: bit_floor(width), bit_floor(height) <= 8192
+__device__ __align__(256) float scratch[2 * kNoAllocTiles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's copy of `scratch`, looked up once per device with that device
+// current (each GPU holds its own copy at its own address).
+int noalloc_scratch(int device, float** out) {
+    if (device < 0 || device >= kMaxDevices) return set_error(PLT_EINVAL, "plt_tonemapper_execute_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            return set_error(PLT_EHIP, "plt_tonemapper_execute_noalloc: hipGetSymbolAddress of the scratch array failed");
+        g_scratch_ptr[device] = static_cast<float*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return PLT_OK;
+}
+
+// Size of the auto-exposure reduction: bit_floor of each side, and the level-0 tile count.
+void lum_extent(uint32_t width, uint32_t height, uint32_t& w, uint32_t& h, size_t& tiles0) {
+    auto bit_floor = [](uint32_t v) { uint32_t b = 1; while (b <= v / 2) b <<= 1; return b; };
+    w = bit_floor(width);
+    h = bit_floor(height);
+    tiles0 = size_t((w + kTile - 1) / kTile) * ((h + kTile - 1) / kTile);
+}
+
+int check_execute_args(const plt_tonemapper* t, const float* src, uint32_t width, uint32_t height, const float* dst,
+                       const char* fn) {
+    if (!t) return set_error(PLT_EINVAL, std::string(fn) + ": NULL pass");
+    if (!t->ctx) return set_error(PLT_ESTATE, std::string(fn) + ": pass created without a context");
+    if (!src || !dst || width == 0 || height == 0 || width > 65536 || height > 65536)
+        return set_error(PLT_EINVAL, std::string(fn) + ": bad arguments (sizes 1..65536)");
+    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u)
+        return set_error(PLT_EINVAL, std::string(fn) + ": src and dst must be 16-byte aligned");
+    if (uint64_t(width) * height > (1ull << 31)) return set_error(PLT_EINVAL, std::string(fn) + ": image larger than 2^31 pixels");
+    return PLT_OK;
+}
+
+// The launches of one execute: the luminance reduction into `lum` (two halves of `half_cap` floats,
+// at least the level-0 tile count each) when auto exposure is on, then the tone-mapping pass.
+// Nothing is allocated here.
+int execute_launches(const plt_tonemapper* t, const float* src, uint32_t width, uint32_t height, float* dst,
+                     float* mean_log_luminance, float* lum, size_t half_cap, hipStream_t st) {
+    const uint64_t n64 = uint64_t(width) * height;
     const float* mean = nullptr;
     if (t->auto_exposure) {
-        auto bit_floor = [](uint32_t v) { uint32_t b = 1; while (b <= v / 2) b <<= 1; return b; };
-        uint32_t w = bit_floor(width), h = bit_floor(height);
+        uint32_t w, h;
+        size_t need;
+        lum_extent(width, height, w, h, need);
         auto tiles = [](uint32_t v) { return (v + kTile - 1) / kTile; };
-        const size_t need = size_t(tiles(w)) * tiles(h);
-        if (need > t->lum_cap) {
-            if (t->d_lum) (void)hipFree(t->d_lum);
-            t->d_lum = nullptr;
-            t->lum_cap = 0;
-            if (hipMalloc(&t->d_lum, 2 * need * sizeof(float)) != hipSuccess)
-                return set_error(PLT_ENOMEM, "plt_tonemapper_execute: out of device memory for the luminance reduction");
-            t->lum_cap = need;
-        }
         LumArgs a{};
         a.src = reinterpret_cast<const float4*>(src);
         a.sw = width;
@@ -472,15 +508,15 @@ int plt_tonemapper_execute(plt_tonemapper* t, const float* src, uint32_t width, 
         a.sx = float(width) / float(w);
         a.sy = float(height) / float(h);
         int half = 0;
-        a.out = t->d_lum;
+        a.out = lum;
         luminance_reduce_kernel<true><<<dim3(tiles(w), tiles(h)), 256, 0, st>>>(a);
         if (int rc = plt::check_launch("luminance_reduce_kernel")) return rc;
         w = tiles(w);
         h = tiles(h);
         while (w > 1 || h > 1) {
-            a.img = t->d_lum + size_t(half) * t->lum_cap;
+            a.img = lum + size_t(half) * half_cap;
             half ^= 1;
-            a.out = t->d_lum + size_t(half) * t->lum_cap;
+            a.out = lum + size_t(half) * half_cap;
             a.w = w;
             a.h = h;
             luminance_reduce_kernel<false><<<dim3(tiles(w), tiles(h)), 256, 0, st>>>(a);
@@ -488,7 +524,7 @@ int plt_tonemapper_execute(plt_tonemapper* t, const float* src, uint32_t width, 
             w = tiles(w);
             h = tiles(h);
         }
-        mean = t->d_lum + size_t(half) * t->lum_cap;
+        mean = lum + size_t(half) * half_cap;
         if (mean_log_luminance &&
             hipMemcpyAsync(mean_log_luminance, mean, sizeof(float), hipMemcpyDeviceToDevice, st) != hipSuccess)
             return set_error(PLT_EHIP, "plt_tonemapper_execute: copy of the mean log luminance failed");
@@ -512,6 +548,49 @@ int plt_tonemapper_execute(plt_tonemapper* t, const float* src, uint32_t width, 
         default: launch_tonemap_op<PLT_TONEMAP_ACES>(ae, cl, a, blocks, st); break;
     }
     return plt::check_launch("tonemap_kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+int plt_tonemapper_execute(plt_tonemapper* t, const float* src, uint32_t width, uint32_t height, float* dst,
+                           float* mean_log_luminance, plt_stream stream) {
+    if (int rc = check_execute_args(t, src, width, height, dst, "plt_tonemapper_execute")) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (hipSetDevice(t->ctx->device) != hipSuccess) return set_error(PLT_EHIP, "plt_tonemapper_execute: hipSetDevice failed");
+    if (t->auto_exposure) {
+        uint32_t w, h;
+        size_t need;
+        lum_extent(width, height, w, h, need);
+        if (need > t->lum_cap) {
+            if (t->d_lum) (void)hipFree(t->d_lum);
+            t->d_lum = nullptr;
+            t->lum_cap = 0;
+            if (hipMalloc(&t->d_lum, 2 * need * sizeof(float)) != hipSuccess)
+                return set_error(PLT_ENOMEM, "plt_tonemapper_execute: out of device memory for the luminance reduction");
+            t->lum_cap = need;
+        }
+    }
+    return execute_launches(t, src, width, height, dst, mean_log_luminance, t->d_lum, t->lum_cap, st);
+}
+
+int plt_tonemapper_execute_noalloc(plt_tonemapper* t, const float* src, uint32_t width, uint32_t height, float* dst,
+                                   float* mean_log_luminance, plt_stream stream) {
+    if (int rc = check_execute_args(t, src, width, height, dst, "plt_tonemapper_execute_noalloc")) return rc;
+    uint32_t w, h;
+    size_t need;
+    lum_extent(width, height, w, h, need);
+    if (t->auto_exposure && need > kNoAllocTiles)
+        return set_error(PLT_EINVAL, "plt_tonemapper_execute_noalloc: the auto-exposure reduction of this image needs " +
+                                         std::to_string(need) + " tiles; the fixed workspace holds " + std::to_string(kNoAllocTiles) +
+                                         " (bit_floor(width) and bit_floor(height) up to 8192)");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (hipSetDevice(t->ctx->device) != hipSuccess) return set_error(PLT_EHIP, "plt_tonemapper_execute_noalloc: hipSetDevice failed");
+    float* lum = nullptr;
+    if (int rc = noalloc_scratch(t->ctx->device, &lum)) return rc;
+    // every texel the launches read is written earlier in the same sequence, so nothing needs clearing
+    return execute_launches(t, src, width, height, dst, mean_log_luminance, lum, kNoAllocTiles, st);
 }
 
 }  // extern "C"

@@ -191,7 +191,10 @@ class ToneMapper(_Pass):
         return np.array(list(m), np.float32).reshape(3, 3)
 
     def execute(self, src: torch.Tensor, dst: torch.Tensor | None = None, mean_log_luminance: torch.Tensor | None = None,
-                stream=None) -> torch.Tensor:
+                stream=None, noalloc: bool = False) -> torch.Tensor:
+        """Tone-maps src into dst. noalloc=True calls the entry point that allocates no device memory
+        (its auto-exposure reduction uses the library's fixed per-GPU workspace; images up to a
+        bit_floor of 8192 per side)."""
         _check_image(src, "src")
         h, w = src.shape[0], src.shape[1]
         if dst is None:
@@ -204,8 +207,9 @@ class ToneMapper(_Pass):
             if mean_log_luminance.dtype != torch.float32 or not mean_log_luminance.is_cuda:
                 raise ValueError("mean_log_luminance must be a float32 CUDA tensor")
             ml = C.c_void_p(mean_log_luminance.data_ptr())
-        _lib.check(self.lib.plt_tonemapper_execute(self.handle, C.c_void_p(src.data_ptr()), w, h,
-                                                   C.c_void_p(dst.data_ptr()), ml, _stream_handle(stream)))
+        fn = self.lib.plt_tonemapper_execute_noalloc if noalloc else self.lib.plt_tonemapper_execute
+        _lib.check(fn(self.handle, C.c_void_p(src.data_ptr()), w, h, C.c_void_p(dst.data_ptr()), ml,
+                      _stream_handle(stream)))
         return dst
 
 
