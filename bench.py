@@ -54,6 +54,11 @@ N_QUERIES = 1 << 20
 WIDTH, HEIGHT, SPP, MAX_BOUNCES = 1920, 1080, 4, 8
 SAMPLE_BYTES_PER_SEGMENT, SAMPLE_BYTES_PER_PIXEL = 84, 32
 SOLVE_BYTES_PER_SEGMENT, SOLVE_BYTES_PER_PIXEL = 36, 16
+# split solve pass: a light event is 20 words as written for an NEE sample (meta, 4 radiances, 4
+# wavelengths, position, direction, size, distance, type and light pdfs, walk pdf), a stored walk
+# vertex 7 words (+ 7 words of walk origin / direction / base per event that has vertices, folded
+# in here), and a region record 16 B per 64 paths
+EVENT_BYTES, WALK_VERTEX_BYTES, REGION_BYTES = 80, 28, 16
 
 
 def parse_args():
@@ -65,6 +70,9 @@ def parse_args():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the BSDF-batch secondary measurement")
+    ap.add_argument("--solve", choices=("megakernel", "split", "autotune"), default="autotune",
+                    help="solve-pass schedule; autotune (default) times the candidate launch configurations on "
+                         "the workload before the warm-up and keeps the fastest")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed region, run 5 plain device copies of MIB MiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -244,10 +252,11 @@ def surrogate_cpu_baseline(params, X, budget_s: float):
 # ------------------------------------------------------------------------------------ path tracing
 
 
-def segment_count(pltpt, frames):
-    """Traced path segments (bounce records written) per frame, read back from the pass."""
+def segment_count(pltpt, frames, split: bool):
+    """Traced path segments (bounce records written) per frame, read back from the pass, and the
+    split solve pass's event counts per frame."""
     import numpy as np
-    counts = []
+    counts, stats = [], []
     for f in frames:
         pltpt.set_frame(f)
         pltpt.execute_frame()
@@ -256,10 +265,12 @@ def segment_count(pltpt, frames):
         end = (flags & 0x30) != 0
         counts.append(int((np.argmax(end, axis=0) + 1).sum()))
         del rec, flags, end
-    return counts
+        if split:
+            stats.append(pltpt.solve_stats())
+    return counts, stats
 
 
-def lights_run(ctx, opts, steps: int):
+def lights_run(ctx, opts, steps: int, launch=None):
     """The default workload with the other light types of the reference's light sampling added: a
     256 x 128 sky environment map (importance-sampled) and four analytic lights (rect, disc,
     sphere, distant) beside the emissive quad. Returns path samples/s and per-frame kernel times."""
@@ -296,6 +307,8 @@ def lights_run(ctx, opts, steps: int):
     scene.set_lights([rect, disc, sph, sun])
     p = PLTPT(ctx, opts)
     p.set_sample_pattern(PLTPT.HALTON, 16)
+    if launch:
+        p.set_launch_config(**launch)
     p.set_scene(scene)
     color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device="cuda")
     albedo, normal = torch.empty_like(color), torch.empty_like(color)
@@ -313,8 +326,9 @@ def lights_run(ctx, opts, steps: int):
     scene.close()
     return {"value": WIDTH * HEIGHT * steps * SPP / wall, "unit": "path samples/s",
             "workload": "the default workload plus a 256x128 importance-sampled sky and rect, disc, sphere and "
-                        "distant lights (light type chosen per NEE sample among environment, emissive, analytic)",
-            "sample_kernel_ms": sample_ms / executes, "solve_kernel_ms": solve_ms / executes}
+                        "distant lights (light type chosen per NEE sample among environment, emissive, analytic), "
+                        "same launch configuration",
+            "sample_kernel_ms": sample_ms / executes, "solve_pass_ms": solve_ms / executes}
 
 
 def postfx_timing(acc_double, acc_single, tm, color, albedo, out, ldr, reps: int):
@@ -452,6 +466,20 @@ def main():
             pltpt.execute_frame()
             graph_tail()
 
+    # launch configuration: the autotuner (before the warm-up, outside the timed region) times the
+    # sample-pass schedules and block sizes, the solve megakernel against the split solve pass at
+    # several event-area sizes, and graph replay, on two frames of this workload
+    tune = None
+    if args.solve == "split":
+        pltpt.set_launch_config(solve_schedule=PLTPT.SOLVE_SPLIT)
+    elif args.solve == "autotune":
+        t0 = time.perf_counter()
+        best, d_ms, b_ms = pltpt.autotune(WIDTH, HEIGHT, color, albedo, normal, frames=2)
+        tune = {"chosen": best, "default_ms_per_frame": d_ms, "best_ms_per_frame": b_ms,
+                "seconds": round(time.perf_counter() - t0, 3)}
+    launch = pltpt.launch_config()
+    split = launch["solve_schedule"] == PLTPT.SOLVE_SPLIT
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -462,25 +490,44 @@ def main():
         step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    sample_ms, solve_ms, executes = pltpt.kernel_times()
+    sample_ms, light_ms, unwind_ms, mega_ms, executes = pltpt.kernel_times_split()
     pltpt.set_profiling(False)
 
-    segs = segment_count(pltpt, range(SPP))  # outside the timed region
+    segs, sstats = segment_count(pltpt, range(SPP), split)  # outside the timed region
     pixels = WIDTH * HEIGHT
+    regions = (pixels + 63) // 64
     seg_per_exec = sum(segs) / SPP
-    sample_avg, solve_avg = sample_ms / executes, solve_ms / executes
-    if solve_avg >= sample_avg:
-        kname, kms = "solve_kernel<4, false>", solve_avg
-        kbytes = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
-        traffic = traffic_from_profile("pltpt_solve_1080p_d8_pmc.md")
+    if split and not launch["graph"]:
+        avg = {k: v / executes for k, v in (("sample", sample_ms), ("light", light_ms), ("unwind", unwind_ms),
+                                            ("megakernel", mega_ms))}
+    else:  # one solve launch sequence, timed as a whole
+        avg = {"sample": sample_ms / executes, "megakernel": (light_ms + unwind_ms + mega_ms) / executes}
+    per_frame = {k: sum(st[k] for st in sstats) / SPP for k in ("events", "walk_vertices", "event_vertices",
+                                                                 "overflowed_paths")} if split else None
+    # algorithmic bytes per launch of each kernel
+    kb = {"sample": SAMPLE_BYTES_PER_SEGMENT * seg_per_exec + SAMPLE_BYTES_PER_PIXEL * pixels}
+    if split:
+        ev_b = EVENT_BYTES * per_frame["events"] + WALK_VERTEX_BYTES * per_frame["walk_vertices"] + REGION_BYTES * regions
+        kb["light"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + ev_b
+        kb["unwind"] = ev_b + SOLVE_BYTES_PER_SEGMENT * per_frame["event_vertices"] + SOLVE_BYTES_PER_PIXEL * pixels
+        kb["megakernel"] = REGION_BYTES * regions  # the fallback: its region check, plus overflowed paths' work
     else:
-        kname, kms = "sample_kernel", sample_avg
-        kbytes = SAMPLE_BYTES_PER_SEGMENT * seg_per_exec + SAMPLE_BYTES_PER_PIXEL * pixels
-        traffic = traffic_from_profile("pltpt_sample_1080p_d8_pmc.md")
+        kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
+    knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>", "unwind": "solve_unwind_kernel<4>",
+              "megakernel": "solve_kernel<4, false>" + (" (fallback of the split pass)" if split and not launch["graph"] else "")}
+    if split and launch["graph"]:
+        knames["megakernel"] = "split solve pass replayed from a graph (light + unwind + fallback)"
+        kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
+    profiles = {"sample": "pltpt_sample_1080p_d8_pmc.md", "light": "pltpt_solve_light_1080p_d8_pmc.md",
+                "unwind": "pltpt_solve_unwind_1080p_d8_pmc.md",
+                "megakernel": None if split else "pltpt_solve_1080p_d8_pmc.md"}
+    kdom = max(avg, key=lambda k: avg[k])
+    kname, kms, kbytes = knames[kdom], avg[kdom], kb[kdom]
+    traffic = traffic_from_profile(profiles[kdom]) if profiles[kdom] else None
     achieved = kbytes / (kms * 1e-3) / 1e9
 
     post = postfx_timing(acc_c, acc_a, tm, color, albedo, out_c, ldr, 50)
-    lights = None if args.no_secondary else lights_run(ctx, opts, 2)
+    lights = None if args.no_secondary else lights_run(ctx, opts, 2, launch)
     bsdf = None if args.no_secondary else bsdf_batch(500, 20)[1]
     sur = None if args.no_secondary else surrogate_run(1 << 22, 6, 1 << 22, 5)
     out = {
@@ -510,7 +557,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "kernel_ms": kms,
                      "algorithmic_bytes_per_launch": kbytes,
-                     "sample_kernel_ms": sample_avg, "solve_kernel_ms": solve_avg},
+                     "kernels": {knames[k]: {"ms": avg[k], "algorithmic_bytes": kb[k],
+                                             "GB/s": kb[k] / (avg[k] * 1e-3) / 1e9 if avg[k] > 0 else None}
+                                 for k in avg},
+                     "sample_kernel_ms": avg["sample"], "solve_pass_ms": sum(v for k, v in avg.items() if k != "sample")},
+        "launch": {"config": launch, "autotune": tune, "split_solve_per_frame": per_frame},
         "cpu_baseline": None,
         "secondary": {"bsdf_eval_batch": {k: bsdf[k] for k in ("value", "unit", "roofline")},
                       "surrogate": {k: sur[k] for k in ("value", "unit", "exact_value", "surrogate_over_exact",

@@ -439,27 +439,43 @@ int plt_debug_fail_upload(int n);
 enum { PLT_SCHEDULE_MEGAKERNEL = 0, PLT_SCHEDULE_WAVEFRONT = 1 };
 int plt_pass_set_schedule(plt_pass* pass, int schedule);
 
+/* Schedule of the solve pass (no counterpart in the reference, whose solve pass is one DXR ray
+ * generation shader): PLT_SOLVE_MEGAKERNEL runs one kernel in which each lane replays its path,
+ * samples lights and carries every sourced beam to the camera; PLT_SOLVE_SPLIT runs a
+ * light-sampling kernel (light samples, shadow rays, manifold walks; it appends one light event per
+ * beam) and an unwind kernel (each event's beam through the stored vertices, then per-path sums in
+ * the megakernel's order), then the megakernel for the paths whose events did not fit the event
+ * area (solve_events_per_path events and solve_hits_per_path manifold-walk vertices per path, pooled
+ * over regions of 64 paths). Both write identical colours. */
+enum { PLT_SOLVE_MEGAKERNEL = 0, PLT_SOLVE_SPLIT = 1 };
+
 /* Launch configuration of the pass (no counterpart in the reference, whose two DXR dispatches per
  * tile are fixed): the sample-pass schedule, the megakernel's block size (64, 128 or 256 threads),
- * the wavefront schedule's persistent grid (64-thread blocks per CU, 1..64), and graph replay: with
- * `graph` set, each execute captures the sample pass (with its per-bounce wavefront launches) and
- * the solve pass into HIP graphs on a stream of the pass's own, updates the instantiated graphs with
- * the new frame's arguments and launches them on the caller's stream. Every configuration renders
- * identical outputs. */
+ * the wavefront schedule's persistent grid (64-thread blocks per CU, 1..64), graph replay, and the
+ * solve-pass schedule with its event-area sizes (1..64 events and 0..64 walk vertices per path).
+ * With `graph` set, each execute captures the sample pass (with its per-bounce wavefront launches)
+ * and the solve pass into HIP graphs on a stream of the pass's own, updates the instantiated graphs
+ * with the new frame's arguments and launches them on the caller's stream. Every configuration
+ * renders identical outputs. */
 typedef struct plt_launch_config {
-    uint32_t schedule;         /* PLT_SCHEDULE_* */
-    uint32_t sample_block;     /* 64, 128 or 256 */
-    uint32_t wf_blocks_per_cu; /* 1..64 */
-    uint32_t graph;            /* 0 or 1 */
+    uint32_t schedule;              /* PLT_SCHEDULE_* */
+    uint32_t sample_block;          /* 64, 128 or 256 */
+    uint32_t wf_blocks_per_cu;      /* 1..64 */
+    uint32_t graph;                 /* 0 or 1 */
+    uint32_t solve_schedule;        /* PLT_SOLVE_* */
+    uint32_t solve_events_per_path; /* 1..64 */
+    uint32_t solve_hits_per_path;   /* 0..64 */
 } plt_launch_config;
 int plt_pass_set_launch_config(plt_pass* pass, const plt_launch_config* config);
 int plt_pass_get_launch_config(const plt_pass* pass, plt_launch_config* config);
 int plt_pass_set_graph(plt_pass* pass, int enable);
 /* Autotuner: renders `frames` frames (from the current frame counter, outputs in `data`) with the
- * current configuration and with each candidate -- the megakernel at 64, 128 and 256 threads per
- * block, the wavefront schedule at 4, 8, 16 and 32 blocks per CU, then the fastest of these replayed
- * from graphs -- keeps the fastest, and restores the frame counter and the jitter pattern. Writes
- * the chosen configuration and the mean frame time of the starting and the chosen one (ms). */
+ * current configuration and with each candidate -- the sample megakernel at 64, 128 and 256 threads
+ * per block, the wavefront sample schedule at 4, 8, 16 and 32 blocks per CU; then, with the fastest
+ * sample pass, the solve megakernel and the split solve pass at 2, 4 and 8 events per path; then
+ * the fastest of these replayed from graphs -- keeps the fastest, and restores the frame counter
+ * and the jitter pattern. Writes the chosen configuration and the mean frame time of the starting
+ * and the chosen one (ms). */
 int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t frames, plt_stream stream,
                       plt_launch_config* best, double* default_ms, double* best_ms);
 
@@ -471,6 +487,18 @@ int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t fram
  * the sums. Enabling or disabling restarts them too. */
 int plt_pass_set_profiling(plt_pass* pass, int enable);
 int plt_pass_get_kernel_times(plt_pass* pass, double* sample_ms, double* solve_ms, uint32_t* executes);
+/* The same sums with the solve pass resolved into its kernels: the split pass's light-sampling and
+ * unwind kernels and the solve megakernel (the whole solve pass of the megakernel schedule, the
+ * split pass's fallback launch, and the whole split pass when it is replayed from a graph, whose
+ * launches are not timed one by one). */
+int plt_pass_get_kernel_times_split(plt_pass* pass, double* sample_ms, double* light_ms, double* unwind_ms,
+                                    double* megakernel_ms, uint32_t* executes);
+/* Counts of the last execute's split solve pass, read back from the device (waits for the device):
+ * light events, manifold-walk vertices stored with them, the sum over events of (vertex index + 1)
+ * -- the stored vertices the unwind kernel replays for them -- and the paths left to the megakernel
+ * because their region's event area was full. PLT_ESTATE if the last execute used the megakernel. */
+int plt_pass_get_solve_stats(plt_pass* pass, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,
+                             uint64_t* overflowed_paths);
 
 /* ---------------------------------------------------------------------------------------
  * Learned BSDF surrogate. No counterpart in the reference: a small MLP (22 inputs, three hidden

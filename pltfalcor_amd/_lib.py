@@ -52,9 +52,11 @@ class Material(C.Structure):
 
 
 class LaunchConfig(C.Structure):
-    """plt_launch_config: sample-pass schedule, megakernel block size, wavefront blocks per CU, graph replay."""
+    """plt_launch_config: sample-pass schedule, megakernel block size, wavefront blocks per CU, graph
+    replay, solve-pass schedule and its event-area sizes."""
     _fields_ = [("schedule", C.c_uint32), ("sample_block", C.c_uint32), ("wf_blocks_per_cu", C.c_uint32),
-                ("graph", C.c_uint32)]
+                ("graph", C.c_uint32), ("solve_schedule", C.c_uint32), ("solve_events_per_path", C.c_uint32),
+                ("solve_hits_per_path", C.c_uint32)]
 
 
 class Camera(C.Structure):
@@ -157,6 +159,10 @@ SIGNATURES = [
     ("plt_pass_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
     ("plt_pass_get_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_uint32)]),
+    ("plt_pass_get_kernel_times_split", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+    ("plt_pass_get_solve_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("plt_spectrum_load_spd", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                         C.POINTER(C.c_uint32)]),
     ("plt_spectral_profile_constant", C.c_int, [C.c_float, C.POINTER(SpectralProfile)]),

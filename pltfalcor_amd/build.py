@@ -29,6 +29,11 @@ HIPCC_FLAGS = [
     "-pthread",
     # Packed FP32 (v_pk_*_f32) stays off: hipcc would otherwise form it from scalar float code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Multiply-adds fuse only within one source expression (the default, "fast", also fuses across
+    # statements wherever the backend finds the pattern, which depends on the code around it): the
+    # same device function then rounds the same way in every kernel it is inlined into, so the
+    # solve megakernel and the split solve pass produce identical colours.
+    "-ffp-contract=on",
 ]
 
 

@@ -171,7 +171,9 @@ struct plt_pass {
     size_t last_words = 0;  // record words written by the last execute
     float cam[12] = {};
     float view_rot[9] = {};
-    // kernel timing: three events per execute (before sample, between, after solve)
+    // kernel timing: kEventsPerExecute events per execute: before the sample pass, after it, after the
+    // split pass's light-sampling and unwind kernels (both at the solve pass's start otherwise), after
+    // the solve pass
     bool profiling = false;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
@@ -184,6 +186,12 @@ struct plt_pass {
     hipGraphExec_t graph_sample = nullptr, graph_solve = nullptr;
     void* d_wf = nullptr;
     size_t wf_paths = 0;
+    // split solve pass: event planes, walk-vertex planes and region table in one allocation, and the
+    // layout the last execute used
+    void* d_solve = nullptr;
+    size_t solve_bytes = 0;
+    plt::SolveBuffers last_solve{};
+    size_t last_regions = 0;
     // camera jitter pattern of the G-buffer pass (Center: the camera's own jitter)
     plt::SamplePattern pattern;
     float last_jitter[2] = {0.f, 0.f};
@@ -314,6 +322,7 @@ int plt_pass_destroy(plt_pass* p) {
     if (p->d_records) (void)hipFree(p->d_records);
     if (p->d_tile_draws) (void)hipFree(p->d_tile_draws);
     if (p->d_wf) (void)hipFree(p->d_wf);
+    if (p->d_solve) (void)hipFree(p->d_solve);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     plt::drop_graphs(p);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
@@ -336,9 +345,17 @@ int plt_pass_set_launch_config(plt_pass* p, const plt_launch_config* c) {
         return set_error(PLT_EINVAL, "plt_pass_set_launch_config: sample_block must be 64, 128 or 256");
     if (c->wf_blocks_per_cu < 1 || c->wf_blocks_per_cu > 64)
         return set_error(PLT_EINVAL, "plt_pass_set_launch_config: wf_blocks_per_cu must be in [1, 64]");
+    if (c->solve_schedule != PLT_SOLVE_MEGAKERNEL && c->solve_schedule != PLT_SOLVE_SPLIT)
+        return set_error(PLT_EINVAL, "plt_pass_set_launch_config: unknown solve schedule");
+    if (c->solve_events_per_path < 1 || c->solve_events_per_path > 64 || c->solve_hits_per_path > 64)
+        return set_error(PLT_EINVAL, "plt_pass_set_launch_config: solve_events_per_path must be in [1, 64], "
+                                     "solve_hits_per_path in [0, 64]");
     p->schedule = int(c->schedule);
     p->launch.sample_block = c->sample_block;
     p->launch.wf_blocks_per_cu = c->wf_blocks_per_cu;
+    p->launch.solve_schedule = c->solve_schedule;
+    p->launch.solve_events_per_path = c->solve_events_per_path;
+    p->launch.solve_hits_per_path = c->solve_hits_per_path;
     p->graph = c->graph != 0;
     if (!p->graph) plt::drop_graphs(p);
     return PLT_OK;
@@ -350,6 +367,9 @@ int plt_pass_get_launch_config(const plt_pass* p, plt_launch_config* c) {
     c->sample_block = p->launch.sample_block;
     c->wf_blocks_per_cu = p->launch.wf_blocks_per_cu;
     c->graph = p->graph ? 1u : 0u;
+    c->solve_schedule = p->launch.solve_schedule;
+    c->solve_events_per_path = p->launch.solve_events_per_path;
+    c->solve_hits_per_path = p->launch.solve_hits_per_path;
     return PLT_OK;
 }
 
@@ -621,15 +641,43 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         w.keys = reinterpret_cast<uint32_t*>(b + 96 * cap);
         w.counters = reinterpret_cast<uint32_t*>(b + 100 * cap);
     }
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    plt::SolveBuffers sbuf{};
+    if (p->launch.solve_schedule == PLT_SOLVE_SPLIT) {
+        const size_t regions = (npaths + plt::kSolveRegion - 1) / plt::kSolveRegion;
+        const size_t evCap = regions * plt::kSolveRegion * p->launch.solve_events_per_path;
+        const size_t hitCap = regions * plt::kSolveRegion * p->launch.solve_hits_per_path;
+        const size_t evBytes = evCap * plt::kEvWords * 4, hitBytes = hitCap * plt::kHitWords * 4;
+        const size_t bytes = evBytes + hitBytes + regions * plt::kRegWords * 4 + 256;
+        if (bytes > p->solve_bytes) {
+            if (p->d_solve) (void)hipFree(p->d_solve);
+            p->d_solve = nullptr;
+            p->solve_bytes = 0;
+            void* q = nullptr;
+            if (hipMalloc(&q, bytes) != hipSuccess) return set_error(PLT_ENOMEM, "plt_pass_execute: solve event area allocation failed");
+            p->d_solve = q;
+            p->solve_bytes = bytes;
+        }
+        char* b = static_cast<char*>(p->d_solve);
+        sbuf.ev = reinterpret_cast<uint32_t*>(b);
+        sbuf.hits = reinterpret_cast<uint32_t*>(b + evBytes);
+        sbuf.region = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes);
+        sbuf.ev_per_path = p->launch.solve_events_per_path;
+        sbuf.hits_per_path = p->launch.solve_hits_per_path;
+        sbuf.ev_cap = evCap;
+        sbuf.hit_cap = hitCap;
+    }
+    constexpr size_t kEv = plt::kEventsPerExecute;
+    hipEvent_t ev[kEv] = {};
+    p->last_solve = sbuf;
+    p->last_regions = sbuf.ev ? (npaths + plt::kSolveRegion - 1) / plt::kSolveRegion : 0;
     if (p->profiling) {
-        while (p->events.size() < p->events_used + 3) {
+        while (p->events.size() < p->events_used + kEv) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_execute: hipEventCreate failed");
             p->events.push_back(e);
         }
-        for (int i = 0; i < 3; ++i) ev[i] = p->events[p->events_used + i];
-        p->events_used += 3;
+        for (size_t i = 0; i < kEv; ++i) ev[i] = p->events[p->events_used + i];
+        p->events_used += kEv;
     }
     const int schedule = p->schedule;
     const plt::LaunchConfig lc = p->launch;
@@ -642,21 +690,34 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         return schedule == PLT_SCHEDULE_WAVEFRONT ? plt::launch_sample_pass_wavefront(sc, fr, w, lc, s)
                                                    : plt::launch_sample_pass(sc, fr, lc, s);
     };
-    auto solve_seq = [&](hipStream_t s) -> int { return plt::launch_solve_pass(sc, fr, s); };
+    const bool split = lc.solve_schedule == PLT_SOLVE_SPLIT;
+    // the split pass's own timing events are recorded between its launches when it runs eagerly
+    const bool split_marks = split && !p->graph && ev[2];
+    auto solve_seq = [&](hipStream_t s) -> int {
+        return split ? plt::launch_solve_pass_split(sc, fr, sbuf, s, split_marks ? ev[2] : nullptr, split_marks ? ev[3] : nullptr)
+                     : plt::launch_solve_pass(sc, fr, s);
+    };
     int rc;
     if (ev[0]) (void)hipEventRecord(ev[0], st);
+    auto mark_solve_start = [&]() {
+        if (ev[1]) (void)hipEventRecord(ev[1], st);
+        if (ev[2] && !split_marks) {
+            (void)hipEventRecord(ev[2], st);
+            (void)hipEventRecord(ev[3], st);
+        }
+    };
     if (p->graph) {
         // each pass replayed from a graph captured on the pass's own stream (a null caller stream
         // cannot be captured); the graph launches on the caller's stream, so ordering is the caller's
         if ((rc = plt::run_captured(p, p->graph_sample, sample_seq, st)) < 0) return rc;
-        if (ev[1]) (void)hipEventRecord(ev[1], st);
+        mark_solve_start();
         if ((rc = plt::run_captured(p, p->graph_solve, solve_seq, st)) < 0) return rc;
     } else {
         if ((rc = sample_seq(st)) < 0) return rc;
-        if (ev[1]) (void)hipEventRecord(ev[1], st);
+        mark_solve_start();
         if ((rc = solve_seq(st)) < 0) return rc;
     }
-    if (ev[2]) (void)hipEventRecord(ev[2], st);
+    if (ev[4]) (void)hipEventRecord(ev[4], st);
     ++p->frame;  // PLTPT.cpp:498
     p->last_words = words;
     return PLT_OK;
@@ -690,8 +751,12 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
     plt_launch_config start{};
     (void)plt_pass_get_launch_config(p, &start);
     std::vector<plt_launch_config> cands;
-    for (uint32_t b : {64u, 128u, 256u}) cands.push_back({PLT_SCHEDULE_MEGAKERNEL, b, start.wf_blocks_per_cu, 0u});
-    for (uint32_t g : {4u, 8u, 16u, 32u}) cands.push_back({PLT_SCHEDULE_WAVEFRONT, start.sample_block, g, 0u});
+    for (uint32_t b : {64u, 128u, 256u})
+        cands.push_back({PLT_SCHEDULE_MEGAKERNEL, b, start.wf_blocks_per_cu, 0u, start.solve_schedule, start.solve_events_per_path,
+                         start.solve_hits_per_path});
+    for (uint32_t g : {4u, 8u, 16u, 32u})
+        cands.push_back({PLT_SCHEDULE_WAVEFRONT, start.sample_block, g, 0u, start.solve_schedule, start.solve_events_per_path,
+                         start.solve_hits_per_path});
     const hipStream_t st = static_cast<hipStream_t>(stream);
     auto time_config = [&](const plt_launch_config& c, double& ms) -> int {
         int rc = plt_pass_set_launch_config(p, &c);
@@ -724,6 +789,25 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
         rc = time_config(c, ms);
         if (rc >= 0 && ms < bms) { bms = ms; bc = c; }
     }
+    // the solve pass's schedule and event-area size, with the fastest sample pass
+    if (rc >= 0) {
+        std::vector<plt_launch_config> solve;
+        plt_launch_config c = bc;
+        c.solve_schedule = PLT_SOLVE_MEGAKERNEL;
+        solve.push_back(c);
+        for (uint32_t e : {2u, 4u, 8u}) {
+            c.solve_schedule = PLT_SOLVE_SPLIT;
+            c.solve_events_per_path = e;
+            c.solve_hits_per_path = e / 2;
+            solve.push_back(c);
+        }
+        for (const auto& sc : solve) {
+            if (rc < 0) break;
+            double ms = 0.0;
+            rc = time_config(sc, ms);
+            if (rc >= 0 && ms < bms) { bms = ms; bc = sc; }
+        }
+    }
     if (rc >= 0) {  // graph replay of the best launch sequence
         plt_launch_config g = bc;
         g.graph = 1u;
@@ -743,23 +827,65 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
     return rc2;
 }
 
-int plt_pass_get_kernel_times(plt_pass* p, double* sample_ms, double* solve_ms, uint32_t* executes) {
-    if (!p || !sample_ms || !solve_ms || !executes) return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
-    double a = 0.0, b = 0.0;
-    const size_t n = p->events_used / 3;
-    for (size_t i = 0; i < n; ++i) {
-        hipEvent_t* e = &p->events[3 * i];
-        if (hipEventSynchronize(e[2]) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventSynchronize failed");
-        float t0 = 0.f, t1 = 0.f;
-        if (hipEventElapsedTime(&t0, e[0], e[1]) != hipSuccess || hipEventElapsedTime(&t1, e[1], e[2]) != hipSuccess)
-            return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventElapsedTime failed");
-        a += t0;
-        b += t1;
+int plt_pass_get_solve_stats(plt_pass* p, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,
+                              uint64_t* overflowed_paths) {
+    if (!p || !events || !walk_vertices || !event_vertices || !overflowed_paths)
+        return set_error(PLT_EINVAL, "plt_pass_get_solve_stats: bad arguments");
+    const plt::SolveBuffers& sb = p->last_solve;
+    if (!sb.ev || p->last_regions == 0) return set_error(PLT_ESTATE, "plt_pass_get_solve_stats: no split solve pass rendered yet");
+    std::vector<uint32_t> reg(p->last_regions * plt::kRegWords), meta(sb.ev_cap);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(reg.data(), sb.region, reg.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(meta.data(), sb.ev + size_t(plt::kEvMeta) * sb.ev_cap, meta.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_get_solve_stats: read-back failed");
+    uint64_t ne = 0, nh = 0, nv = 0, no = 0;
+    for (size_t r = 0; r < p->last_regions; ++r) {
+        const uint32_t* rg = &reg[r * plt::kRegWords];
+        ne += rg[plt::kRegEvents];
+        nh += rg[plt::kRegHits];
+        no += uint64_t(__builtin_popcount(rg[plt::kRegOverflowLo])) + uint64_t(__builtin_popcount(rg[plt::kRegOverflowHi]));
+        const size_t base = r * plt::kSolveRegion * sb.ev_per_path;
+        for (uint32_t k = 0; k < rg[plt::kRegEvents]; ++k) nv += ((meta[base + k] >> 6) & 127u) + 1u;
     }
-    *sample_ms = a;
-    *solve_ms = b;
+    *events = ne;
+    *walk_vertices = nh;
+    *event_vertices = nv;
+    *overflowed_paths = no;
+    return PLT_OK;
+}
+
+int plt_pass_get_kernel_times_split(plt_pass* p, double* sample_ms, double* light_ms, double* unwind_ms, double* megakernel_ms,
+                                    uint32_t* executes) {
+    if (!p || !sample_ms || !light_ms || !unwind_ms || !megakernel_ms || !executes)
+        return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
+    constexpr size_t kEv = plt::kEventsPerExecute;
+    double sums[kEv - 1] = {};
+    const size_t n = p->events_used / kEv;
+    for (size_t i = 0; i < n; ++i) {
+        hipEvent_t* e = &p->events[kEv * i];
+        if (hipEventSynchronize(e[kEv - 1]) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventSynchronize failed");
+        for (size_t k = 0; k + 1 < kEv; ++k) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, e[k], e[k + 1]) != hipSuccess)
+                return set_error(PLT_EHIP, "plt_pass_get_kernel_times: hipEventElapsedTime failed");
+            sums[k] += t;
+        }
+    }
+    *sample_ms = sums[0];
+    *light_ms = sums[1];
+    *unwind_ms = sums[2];
+    *megakernel_ms = sums[3];
     *executes = uint32_t(n);
     p->events_used = 0;
+    return PLT_OK;
+}
+
+int plt_pass_get_kernel_times(plt_pass* p, double* sample_ms, double* solve_ms, uint32_t* executes) {
+    if (!p || !sample_ms || !solve_ms || !executes) return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
+    double light = 0.0, unwind = 0.0, mega = 0.0;
+    const int rc = plt_pass_get_kernel_times_split(p, sample_ms, &light, &unwind, &mega, executes);
+    if (rc < 0) return rc;
+    *solve_ms = light + unwind + mega;
     return PLT_OK;
 }
 

@@ -221,6 +221,10 @@ int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream
 int launch_solve_4(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 int launch_solve_4_lights(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 int launch_solve_1(const PtScene& sc, const PtFrame& fr, hipStream_t st, bool lights);
+int launch_solve_sample_4(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
+int launch_solve_sample_4_lights(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
+int launch_solve_sample_1(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, bool lights);
+int launch_solve_unwind(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
@@ -229,6 +233,29 @@ int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.hwss == 4) return analytic ? launch_solve_4_lights(sc, fr, st) : launch_solve_4(sc, fr, st);
     if (fr.hwss == 1) return launch_solve_1(sc, fr, st, analytic);
     return set_error(PLT_ENOTIMPL, "HWSS sample counts other than 1 and 4 are not built");
+}
+
+// Split solve pass: light sampling, unwind, then the megakernel over the paths whose region's event
+// area overflowed (its other blocks return at once). Three launches on one stream, no host waits.
+int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, hipEvent_t after_light,
+                            hipEvent_t after_unwind) {
+    if (fr.n_paths == 0) {
+        if (after_light) (void)hipEventRecord(after_light, st);
+        if (after_unwind) (void)hipEventRecord(after_unwind, st);
+        return PLT_OK;
+    }
+    if (fr.hwss != 4 && fr.hwss != 1) return set_error(PLT_ENOTIMPL, "HWSS sample counts other than 1 and 4 are not built");
+    const bool analytic = ((fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0) ||
+                          ((fr.options & kOptEnvLight) != 0 && sc.env_w > 0);
+    int rc = fr.hwss == 4 ? (analytic ? launch_solve_sample_4_lights(sc, fr, sb, st) : launch_solve_sample_4(sc, fr, sb, st))
+                          : launch_solve_sample_1(sc, fr, sb, st, analytic);
+    if (rc < 0) return rc;
+    if (after_light) (void)hipEventRecord(after_light, st);
+    if ((rc = launch_solve_unwind(sc, fr, sb, st)) < 0) return rc;
+    if (after_unwind) (void)hipEventRecord(after_unwind, st);
+    PtFrame fb = fr;
+    fb.solve_overflow = sb.region;
+    return launch_solve_pass(sc, fb, st);
 }
 
 }  // namespace plt

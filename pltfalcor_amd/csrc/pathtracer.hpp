@@ -82,6 +82,53 @@ struct PtFrame {
     float4* color;
     float4* albedo;
     float4* normal;
+    // split solve pass: the region table of the light-sampling kernel; when set, the megakernel
+    // solves only the paths whose region overflowed its event buffer (NULL = every path)
+    const uint32_t* solve_overflow;
+};
+
+// Split solve pass. The light-sampling kernel (one 64-lane wave per region of 64 consecutive paths)
+// replays each path's vertices with the path's random stream -- light samples, shadow rays and
+// manifold walks -- and appends one light event per sourced beam that reaches the vertex: NEE,
+// emissive hit or environment miss. The unwind kernel (one wave per region) rebuilds each beam,
+// carries it through the stored vertices to the camera and sums every path's events in the order
+// the megakernel adds them. Events are SoA planes of kEvWords words; a region's events occupy
+// [region * 64 * ev_per_path, +64 * ev_per_path) in the order they were appended, which is each
+// path's own vertex order. Walk vertices of manifold NEE events live in a second area, kHitWords
+// planes per vertex, 64 * hits_per_path per region.
+enum EvField {
+    kEvMeta = 0,  // lane (6 bits) | vertex (7 bits) << 6 | kind (2 bits) << 13 | kEvBit* | walk vertices << 21
+    kEvLe,        // 4 planes: sourced spectral radiance per HWSS component
+    kEvLam = kEvLe + 4,  // 4 planes: wavelengths
+    kEvPos = kEvLam + 4, // 3 planes: beam position (area sources)
+    kEvDir = kEvPos + 3, // 3 planes: beam direction as sourced
+    kEvSize = kEvDir + 3,  // beam area (mm^2) or solid angle
+    kEvDist,      // source distance (area sources)
+    kEvTypePdf,   // NEE: light-type selection probability
+    kEvLsPdf,     // NEE: light-sample pdf (with the type selection)
+    kEvMsPdf,     // manifold NEE: walk pdf
+    kEvWo,        // 3 planes: manifold NEE: last walk direction towards the light
+    kEvOrg = kEvWo + 3,  // 3 planes: manifold NEE: walk origin
+    kEvHitBase = kEvOrg + 3,  // manifold NEE: first walk vertex in the region's vertex area
+    kEvWords
+};
+enum : uint32_t { kEvKindNee = 0, kEvKindEmissive = 1, kEvKindMiss = 2 };
+enum : uint32_t {
+    kEvBitComps1 = 1u << 15, kEvBitDistant = 1u << 16, kEvBitMnee = 1u << 17, kEvBitMisL = 1u << 18,
+    kEvBitCurrentMs = 1u << 19, kEvBitDispersive = 1u << 20
+};
+enum HitField { kHitTri = 0, kHitB1, kHitB2, kHitPos, kHitLobe = kHitPos + 3, kHitWords };
+constexpr uint32_t kSolveRegion = 64;  // paths per region: one wave of the split solve kernels
+constexpr size_t kEventsPerExecute = 5;  // kernel-timing events per execute of the pass
+// per region: event count, walk-vertex count, overflow mask (2 words: bit = lane whose path is left
+// to the megakernel)
+enum { kRegEvents = 0, kRegHits, kRegOverflowLo, kRegOverflowHi, kRegWords };
+struct SolveBuffers {
+    uint32_t* ev;
+    uint32_t* hits;
+    uint32_t* region;
+    uint32_t ev_per_path, hits_per_path;
+    size_t ev_cap, hit_cap;  // plane lengths: regions * 64 * per-path capacity
 };
 
 // Wavefront sample pass: per-path state (SoA, n_paths entries each), two path queues of
@@ -106,11 +153,18 @@ struct WfBuffers {
 struct LaunchConfig {
     uint32_t sample_block = 64;
     uint32_t wf_blocks_per_cu = 16;
+    uint32_t solve_schedule = 0;  // PLT_SOLVE_*
+    uint32_t solve_events_per_path = 4;
+    uint32_t solve_hits_per_path = 2;
 };
 
 int launch_sample_pass(const PtScene& sc, const PtFrame& fr, const LaunchConfig& lc, hipStream_t st);
 int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, const LaunchConfig& lc, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+// after_light / after_unwind (may be NULL): events recorded on `st` after the light-sampling and the
+// unwind kernels (kernel timing outside graph capture)
+int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st,
+                            hipEvent_t after_light = nullptr, hipEvent_t after_unwind = nullptr);
 int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 
 }  // namespace plt

@@ -769,6 +769,11 @@ solve_kernel(PtScene sc, PtFrame fr) {
     __shared__ int stack[kTraversalStackDepth * kPtBlock];
     const uint32_t li = blockIdx.x * kPtBlock + threadIdx.x;
     if (li >= fr.n_paths) return;
+    if (fr.solve_overflow) {  // the split pass's fallback: only paths whose events did not fit
+        const uint32_t* rg = fr.solve_overflow + size_t(li / 64u) * kRegWords;
+        const uint32_t lane = li & 63u;
+        if (((rg[lane < 32u ? kRegOverflowLo : kRegOverflowHi] >> (lane & 31u)) & 1u) == 0u) return;
+    }
     const uint32_t pid = fr.path_base + li;
     const uint32_t px = pid % fr.width, py = pid / fr.width;
     const size_t n = fr.n_paths;

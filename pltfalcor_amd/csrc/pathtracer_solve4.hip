@@ -1,6 +1,7 @@
 // Solve pass of the PLT path tracer with four HWSS wavelengths per beam, without the analytic-light
-// and environment-map code (one translation unit per large kernel, so they compile in parallel).
-#include "pathtracer_kernels.hpp"
+// and environment-map code: the megakernel and the split pass's light-sampling kernel (one
+// translation unit per large kernel group, so they compile in parallel).
+#include "pathtracer_split.hpp"
 
 namespace plt {
 
@@ -8,6 +9,12 @@ int launch_solve_4(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);
     hipLaunchKernelGGL((solve_kernel<4, false>), g, b, 0, st, sc, fr);
     return check_launch("solve_kernel<4>");
+}
+
+int launch_solve_sample_4(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st) {
+    const dim3 g((fr.n_paths + kPtBlock - 1) / kPtBlock), b(kPtBlock);
+    hipLaunchKernelGGL((solve_sample_kernel<4, false>), g, b, 0, st, sc, fr, sb);
+    return check_launch("solve_sample_kernel<4>");
 }
 
 }  // namespace plt

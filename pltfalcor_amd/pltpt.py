@@ -98,6 +98,7 @@ class PLTPT:
 
     MEGAKERNEL, WAVEFRONT = 0, 1
     CENTER, DIRECTX, HALTON, STRATIFIED = 0, 1, 2, 3
+    SOLVE_MEGAKERNEL, SOLVE_SPLIT = 0, 1
 
     def set_sample_pattern(self, pattern: int, sample_count: int = 16):
         """Camera jitter of the G-buffer pass before PLTPT (GBufferRT's samplePattern / sampleCount;
@@ -116,14 +117,17 @@ class PLTPT:
         _lib.check(self.lib.plt_pass_set_schedule(self.handle, int(schedule)))
 
     def set_launch_config(self, schedule: int | None = None, sample_block: int | None = None,
-                          wf_blocks_per_cu: int | None = None, graph: bool | None = None):
+                          wf_blocks_per_cu: int | None = None, graph: bool | None = None,
+                          solve_schedule: int | None = None, solve_events_per_path: int | None = None,
+                          solve_hits_per_path: int | None = None):
         """Launch parameters (plt_launch_config); None keeps the current value."""
         c = self.launch_config()
         for k, v in (("schedule", schedule), ("sample_block", sample_block), ("wf_blocks_per_cu", wf_blocks_per_cu),
-                     ("graph", graph)):
+                     ("graph", graph), ("solve_schedule", solve_schedule),
+                     ("solve_events_per_path", solve_events_per_path), ("solve_hits_per_path", solve_hits_per_path)):
             if v is not None:
                 c[k] = int(v)
-        lc = _lib.LaunchConfig(c["schedule"], c["sample_block"], c["wf_blocks_per_cu"], c["graph"])
+        lc = _lib.LaunchConfig(*(c[k] for k, _ in _lib.LaunchConfig._fields_))
         _lib.check(self.lib.plt_pass_set_launch_config(self.handle, C.byref(lc)))
 
     def launch_config(self) -> dict:
@@ -156,6 +160,21 @@ class PLTPT:
         a, b, n = C.c_double(), C.c_double(), C.c_uint32()
         _lib.check(self.lib.plt_pass_get_kernel_times(self.handle, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+    def kernel_times_split(self):
+        """(sample_ms, light_ms, unwind_ms, megakernel_ms, executes): the solve pass resolved into the
+        split pass's light-sampling and unwind kernels and the solve megakernel (see the C ABI)."""
+        v = [C.c_double() for _ in range(4)]
+        n = C.c_uint32()
+        _lib.check(self.lib.plt_pass_get_kernel_times_split(self.handle, *(C.byref(x) for x in v), C.byref(n)))
+        return (*(x.value for x in v), n.value)
+
+    def solve_stats(self) -> dict:
+        """Counts of the last execute's split solve pass (waits for the device): light events, stored
+        walk vertices, replayed vertices (sum of vertex index + 1 over events), overflowed paths."""
+        v = [C.c_uint64() for _ in range(4)]
+        _lib.check(self.lib.plt_pass_get_solve_stats(self.handle, *(C.byref(x) for x in v)))
+        return dict(zip(("events", "walk_vertices", "event_vertices", "overflowed_paths"), (x.value for x in v)))
 
     def close(self):
         if getattr(self, "handle", None):

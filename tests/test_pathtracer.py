@@ -1044,8 +1044,9 @@ def test_oracle_two_bounce_mirror_matches_lambert_closed_form():
 @pytest.mark.gpu
 def test_gpu_launch_configs_and_graph_replay_are_identical(gpu):
     """Every launch configuration of the pass -- the megakernel at 64, 128 and 256 threads per block,
-    the wavefront schedule at 4 and 32 blocks per CU, each eager and replayed from captured HIP graphs
-    -- writes the default configuration's bounce records, colour, albedo and normal bit for bit. The
+    the wavefront schedule at 4 and 32 blocks per CU, the split solve pass with ample and with
+    overflowing event areas, each eager and replayed from captured HIP graphs -- writes the default
+    configuration's bounce records, colour, albedo and normal bit for bit. The
     graphs are replayed for several frames with eager work and synchronisations between the replays,
     and every replay is checked against an eager render of the same frame."""
     import torch
@@ -1083,7 +1084,14 @@ def test_gpu_launch_configs_and_graph_replay_are_identical(gpu):
                 dict(schedule=PLTPT.MEGAKERNEL, sample_block=64, graph=True),
                 dict(schedule=PLTPT.MEGAKERNEL, sample_block=256, graph=True),
                 dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=4), dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=32),
-                dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=16, graph=True)):
+                dict(schedule=PLTPT.WAVEFRONT, wf_blocks_per_cu=16, graph=True),
+                # split solve pass: ample event area, then areas so small that most regions overflow
+                # into the megakernel fallback (1 event and no walk vertex per path), eager and replayed
+                dict(solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=8, solve_hits_per_path=4),
+                dict(solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=1, solve_hits_per_path=0),
+                dict(solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=2, solve_hits_per_path=1, graph=True),
+                dict(schedule=PLTPT.WAVEFRONT, solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=4,
+                     solve_hits_per_path=2, graph=True)):
         got = render(dict(dict(sample_block=64, wf_blocks_per_cu=16, graph=False), **cfg), frames - 1)
         for f in range(frames):
             valid = np.arange(ref[f][0].shape[0])[:, None] < L[f][None, :]
@@ -1128,3 +1136,79 @@ def test_gpu_autotune_picks_a_config_and_restores_state(gpu):
     assert np.array_equal(color.cpu().numpy().view(np.uint32), want.view(np.uint32))
     p.close()
     ref.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [
+    dict(name="C2", workload=True),
+    dict(name="env+analytic-mnee", lights=True, opts={"maxBounces": 8, "HWSS": 4, "doMNEE": 1}),
+    dict(name="env+analytic-hwss1", lights=True, opts={"maxBounces": 6, "HWSS": 1, "doMNEE": 1}),
+    dict(name="hwss1-noMIS", lights=False, opts={"maxBounces": 6, "HWSS": 1, "doMNEE": 1, "doMIS": 0}),
+    dict(name="no-hwss-mis", lights=False, opts={"maxBounces": 6, "HWSS": 4, "doMNEE": 1, "HWSSDoMIS": 0}),
+    dict(name="tiny-pertile", lights=True, opts={"maxBounces": 5, "HWSS": 4, "doMNEE": 1, "sampleGenerator": 0,
+                                                  "NEEUsePerTileSG": 1, "tileSize": 16}),
+], ids=lambda c: c["name"])
+def test_gpu_split_solve_equals_megakernel(gpu, case):
+    """The split solve pass (light-sampling kernel, unwind kernel, megakernel fallback for overflowed
+    regions) writes the megakernel's colour bit for bit: on the C2 workload at 1920 x 1080 (manifold
+    NEE, Halton jitter), and on scenes that take the analytic-light / environment-map kernels, one
+    HWSS wavelength, MIS off, the tiny generator with per-tile NEE draws -- with the default event area
+    and with one so small that the fallback solves most paths."""
+    import torch
+    from pltfalcor_amd.bsdf import default_context
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.scene import PATH_2, Scene
+    from test_env_map import _patchy_sky
+    from test_lights import _all_lights
+    ctx = default_context()
+    if case.get("workload"):
+        w, h = 1920, 1080
+        s = Scene(ctx, aspect=w / h, **PATH_2)
+        opts = dict(REFERENCE_DEFAULTS, maxBounces=8)
+        sizes = [(4, 2)]
+    else:
+        w, h = 96, 64
+        proc = Scene(None, seed=41, sphere_count=16, subdivision=3, material_kinds=8)
+        pa = proc.numpy()
+        s = Scene.from_arrays(ctx, pa["positions"], pa["normals"], pa["indices"], pa["tri_mesh"], pa["mesh_material"],
+                              pa["materials"], pa["profiles"], proc.arrays.camera)
+        if case["lights"]:
+            s.set_env_map(_patchy_sky(), rotation=(0.0, 45.0, 0.0), intensity=0.5)
+            s.set_lights(_all_lights())
+        opts = case["opts"]
+        sizes = [(4, 2), (1, 0)]
+
+    def render(cfg, frames=2):
+        p = PLTPT(ctx, opts)
+        p.set_sample_pattern(PLTPT.HALTON, 16)
+        p.set_launch_config(**cfg)
+        p.set_scene(s)
+        out = []
+        color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+        albedo, normal = torch.empty_like(color), torch.empty_like(color)
+        stats = None
+        for _ in range(frames):
+            color.fill_(float("nan"))
+            p.execute(w, h, color, albedo, normal)
+            torch.cuda.synchronize()
+            out.append(color.cpu().numpy().copy())
+        if cfg["solve_schedule"] == PLTPT.SOLVE_SPLIT:
+            stats = p.solve_stats()
+        p.close()
+        return out, stats
+
+    ref, _ = render(dict(solve_schedule=PLTPT.SOLVE_MEGAKERNEL))
+    assert all(np.isfinite(r).all() for r in ref) and ref[0][..., :3].mean() > 0
+    for ev, hits in sizes:
+        got, st = render(dict(solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=ev, solve_hits_per_path=hits))
+        print(case["name"], (ev, hits), st)
+        assert st["events"] > 0 and st["event_vertices"] >= st["events"]
+        if ev == 1:
+            assert st["overflowed_paths"] > 0  # the fallback took part
+        for f, (a, b) in enumerate(zip(ref, got)):
+            diff = a.view(np.uint32) != b.view(np.uint32)
+            if diff.any():
+                rel = np.abs(a - b) / np.maximum(np.abs(a), 1e-30)
+                print(f"{case['name']} {(ev, hits)} frame {f}: {int(diff.any(axis=2).sum())} pixels differ, max rel "
+                      f"{rel.max():.3e}, quantiles 50/99 % of differing {np.quantile(rel[diff], [.5, .99])}")
+            assert not diff.any(), (case["name"], ev, hits, f, int(diff.any(axis=2).sum()), np.argwhere(diff.any(axis=2))[:5])
