@@ -66,7 +66,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("path", "bsdf", "surrogate"), default="path")
+    ap.add_argument("--workload", choices=("path", "bsdf", "surrogate", "c4"), default="path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the BSDF-batch secondary measurement")
@@ -161,24 +161,31 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 MLP_FLOP_PER_QUERY = 2 * (22 * 128 + 128 * 128 + 128 * 128 + 128 * 16)  # 75,264
 
 
-def surrogate_run(n_train: int, epochs: int, n_test: int, reps: int, cpu_seconds: float = 0.0):
-    import ctypes as C
-    import numpy as np
+def train_surrogate(ctx, n_train: int, epochs: int):
+    """The surrogate trained on n_train exact samples (C3): (surrogate, training record)."""
     import torch
-    from pltfalcor_amd.bsdf import default_context
     from pltfalcor_amd.surrogate import ExactSamples, Surrogate, train
-    ctx = default_context(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     tr = ExactSamples(ctx, n_train, seed=21)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    te = ExactSamples(ctx, n_test, seed=22)
     s = Surrogate(ctx, max_batch=16384, seed=3)
     t0 = time.perf_counter()
     losses = train(s, tr, epochs=epochs, batch=16384, lr=2e-3)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
+    del tr
+    return s, {"samples": n_train, "epochs": epochs, "seconds": train_s, "samples_per_s": epochs * n_train / train_s,
+               "final_loss": losses[-1], "exact_sample_generation_s": gen_s}
+
+
+def surrogate_run(ctx, s, training, n_test: int, reps: int, cpu_seconds: float = 0.0):
+    import ctypes as C
+    import numpy as np
+    import torch
+    from pltfalcor_amd.surrogate import ExactSamples
+    te = ExactSamples(ctx, n_test, seed=22)
     n = te.n
     rng = np.random.default_rng(4)
     dop = rng.uniform(0, 1, n)
@@ -223,17 +230,93 @@ def surrogate_run(n_train: int, epochs: int, n_test: int, reps: int, cpu_seconds
                                                                                 "coated conductor",
                                                                                 "grated conductor base"))},
                            "fallbacks": fb},
-        "training": {"samples": n_train, "epochs": epochs, "seconds": train_s,
-                     "samples_per_s": epochs * n_train / train_s, "final_loss": losses[-1],
-                     "exact_sample_generation_s": gen_s},
+        "training": training,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / MFMA_BF16_PEAK_TFLOPS, "traffic": None, "kernel": "mlp_infer_kernel",
                      "kernel_ms": kern_ms, "flop_per_query": MLP_FLOP_PER_QUERY, "queries_per_launch": n},
     }
     if cpu_seconds > 0:
         res["cpu_baseline"] = surrogate_cpu_baseline(s.get_params(), te.X[:1 << 15].cpu().numpy(), cpu_seconds)
-    s.close()
     return res
+
+
+C4_SPP = 16
+
+
+def c4_run(ctx, sur, training):
+    """The render with the learned surrogate in the loop (C4): the 1080p scene with the grating plate
+    under a 5 mm glass slab, 16 spp (frames 0-15), depth 8, manifold NEE, rendered exactly and with
+    the surrogate standing for the coated and grated scattering lobes (split solve pass both times),
+    each accumulated in FP64. The Stokes vector per pixel comes from the camera polarizer at four
+    angles (I = (S0 + cos t S1 + sin t S2) / 2 for t = 0, pi/2, pi, 3 pi/2), rendered the same way;
+    the RMSE of each Stokes component is taken over pixels and RGB channels at matched seeds and given
+    relative to the exact image's mean S0."""
+    import numpy as np
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    from pltfalcor_amd.postfx import AccumulatePass, AccumulatePrecision
+    from pltfalcor_amd.scene import C4_GRATING, C4_SLAB, c4_scene
+    scene = c4_scene(ctx, WIDTH / HEIGHT)
+    opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4}
+    color = torch.empty((HEIGHT, WIDTH, 4), dtype=torch.float32, device="cuda")
+    albedo, normal, out = torch.empty_like(color), torch.empty_like(color), torch.empty_like(color)
+    acc = AccumulatePass(ctx, {"enabled": True, "precisionMode": AccumulatePrecision.Double})
+
+    def render(use_sur: bool, theta=None):
+        scene.set_camera(polarizer=0.0 if theta is None else 1.0, polarizer_theta=0.0 if theta is None else theta)
+        p = PLTPT(ctx, opts)
+        p.set_sample_pattern(PLTPT.HALTON, 16)
+        p.set_launch_config(solve_schedule=PLTPT.SOLVE_SPLIT)
+        p.set_scene(scene)
+        if use_sur:
+            p.set_surrogate(sur)
+        p.execute(WIDTH, HEIGHT, color, albedo, normal)  # warm-up (allocations)
+        p.set_frame(0)
+        acc.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(C4_SPP):
+            p.execute(WIDTH, HEIGHT, color, albedo, normal)
+            acc.execute(color, out)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = p.surrogate_stats() if use_sur else None
+        p.close()
+        return out[..., :3].cpu().numpy().astype(np.float64), dt, st
+
+    ex, t_ex, _ = render(False)
+    su, t_su, st = render(True)
+    pol = {}
+    for k, th in enumerate((0.0, 0.5 * np.pi, np.pi, 1.5 * np.pi)):
+        pol[(k, False)] = render(False, th)[0]
+        pol[(k, True)] = render(True, th)[0]
+    scene.set_camera(polarizer=0.0, polarizer_theta=0.0)
+    stokes = {}
+    for tag in (False, True):
+        i0, i90, i180, i270 = (pol[(k, tag)] for k in range(4))
+        stokes[tag] = (i0 + i180, i0 - i180, i90 - i270)
+    s0_mean = float(stokes[False][0].mean())
+    rmse = {f"S{k}": float(np.sqrt(np.mean((stokes[True][k] - stokes[False][k]) ** 2)) / s0_mean) for k in range(3)}
+    lum = lambda im: im @ np.array([0.2126, 0.7152, 0.0722])  # noqa: E731
+    le, ls = lum(ex), lum(su)
+    floor = 1e-3 * le.mean()
+    rel = np.abs(ls - le) / (le + floor)
+    acc.close()
+    scene.close()
+    eligible = st["rows"] // 4
+    return {"workload": f"C4: the 1080p depth-8 scene with a {C4_GRATING['size']} m grated-conductor plate (pitch "
+                        f"{C4_GRATING['pitch_um']} um, roughness {C4_GRATING['roughness']}) under a "
+                        f"{C4_SLAB['thickness'] * 1e3:.0f} mm glass slab (IoR {C4_SLAB['ior']}), {C4_SPP} spp, "
+                        "manifold NEE, split solve pass; surrogate for the coated and grated scattering lobes with "
+                        "exact fallback outside its training box",
+            "exact_path_samples_per_s": WIDTH * HEIGHT * C4_SPP / t_ex,
+            "surrogate_path_samples_per_s": WIDTH * HEIGHT * C4_SPP / t_su,
+            "surrogate_over_exact": t_ex / t_su,
+            "stokes_rmse_rel_to_mean_s0": rmse,
+            "luminance_rel_err": {"median": float(np.median(rel)), "mean": float(rel.mean()),
+                                  "image_mean": float(abs(ls.mean() - le.mean()) / le.mean())},
+            "network_share_of_eligible_steps": 1.0 - st["exact_steps"] / max(1, eligible),
+            "eligible_steps_per_frame": eligible, "training": training}
 
 
 def surrogate_cpu_baseline(params, X, budget_s: float):
@@ -410,8 +493,25 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
+    if args.workload in ("surrogate", "c4"):
+        from pltfalcor_amd.bsdf import default_context
+        sctx = default_context(0)
+        trained, training = train_surrogate(sctx, 1 << 24, 6)
+    if args.workload == "c4":
+        r = c4_run(sctx, trained, training)
+        trained.close()
+        out = {"metric": METRIC, "value": r["surrogate_path_samples_per_s"], "unit": "path samples/s", "n_gpus": 1,
+               "steps": 1, "warmup": 1, "ms_per_step": WIDTH * HEIGHT * C4_SPP / r["surrogate_path_samples_per_s"] * 1e3,
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 transport, bf16 surrogate (FP32 accumulation)", "data": "synthetic",
+               "config": {"workload": r["workload"], "width": WIDTH, "height": HEIGHT, "spp": C4_SPP,
+                          "max_bounces": MAX_BOUNCES},
+               "roofline": None, "cpu_baseline": None, "c4": r}
+        print(json.dumps(out), flush=True)
+        return
     if args.workload == "surrogate":
-        r = surrogate_run(1 << 24, 6, 1 << 22, max(3, args.steps), 0.0 if args.no_cpu_baseline else args.cpu_seconds)
+        r = surrogate_run(sctx, trained, training, 1 << 22, max(3, args.steps), 0.0 if args.no_cpu_baseline else args.cpu_seconds)
+        trained.close()
         out = {"metric": METRIC, "value": r["value"], "unit": r["unit"], "n_gpus": 1, "steps": max(3, args.steps),
                "warmup": 1, "ms_per_step": (1 << 22) / r["value"] * 1e3, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "bf16 (FP32 accumulation)", "data": "synthetic",
@@ -529,7 +629,14 @@ def main():
     post = postfx_timing(acc_c, acc_a, tm, color, albedo, out_c, ldr, 50)
     lights = None if args.no_secondary else lights_run(ctx, opts, 2, launch)
     bsdf = None if args.no_secondary else bsdf_batch(500, 20)[1]
-    sur = None if args.no_secondary else surrogate_run(1 << 22, 6, 1 << 22, 5)
+    sur, c4 = None, None
+    if not args.no_secondary:
+        from pltfalcor_amd.bsdf import default_context as _dc
+        sctx = _dc(0)  # the surrogate's context: the Drude profiles its exact samples use
+        trained, training = train_surrogate(sctx, 1 << 24, 6)
+        sur = surrogate_run(sctx, trained, training, 1 << 22, 5)
+        c4 = c4_run(ctx, trained, training)
+        trained.close()
     out = {
         "metric": METRIC,
         "value": pixels * SPP * args.steps / wall,
@@ -566,7 +673,7 @@ def main():
         "secondary": {"bsdf_eval_batch": {k: bsdf[k] for k in ("value", "unit", "roofline")},
                       "surrogate": {k: sur[k] for k in ("value", "unit", "exact_value", "surrogate_over_exact",
                                                         "error_vs_exact", "training", "roofline")},
-                      "graph_tail": post, "lights": lights}
+                      "graph_tail": post, "lights": lights, "c4": c4}
                      if bsdf else {"graph_tail": post},
     }
     if args.copy_calibration > 0:

@@ -338,6 +338,10 @@ typedef struct plt_light {
 /* Replaces the scene's analytic lights (Scene::addLight). A pass reads them when the scene is
  * bound (plt_pass_set_scene). Returns PLT_EINVAL for a bad type or profile id. */
 int plt_scene_set_lights(plt_scene* scene, const plt_light* lights, uint32_t count);
+/* Replaces the scene's camera (the reference's camera setters between frames, Camera.cpp:150-330,
+ * among them setPolarizer / setPolarizerTheta): a pass bound to the scene renders its next execute
+ * with it. */
+int plt_scene_set_camera(plt_scene* scene, const plt_camera* camera);
 
 /* Environment map light (Scene/Lights/EnvMap.{h,cpp}, EnvMap.slang, EnvMapData.slang) with the
  * importance sampling of Rendering/Lights/EnvMapSampler.{cpp,slang}: a lat-long RGBA32F image of
@@ -566,6 +570,26 @@ int plt_surrogate_generate(plt_context* ctx, uint32_t n, uint32_t seed, plt_mate
  * surrogate's judge). */
 int plt_surrogate_eval(plt_surrogate* s, uint32_t n, const plt_material* materials, const float* geometry, const float* X,
                        const float* stokes_in, float* stokes_out, int mode, uint32_t* fallbacks, plt_stream stream);
+
+/* The surrogate in the path tracer (C4: the render with the learned lobes). With a trained
+ * surrogate set (its training statistics set), every execute of the pass runs the split solve pass
+ * with the surrogate: the unwind of every light event first appends the network's inputs for each
+ * unwind step whose lobe the surrogate stands for -- the scattering lobes of the coated opaque
+ * dielectric and the coated conductor and the grated conductor's base lobe, one row per HWSS
+ * wavelength, with the coherence area the beam presents -- then one batched inference runs on the
+ * matrix cores, then the unwind runs again with the network's Mueller matrices at those steps.
+ * Steps whose inputs fall outside the training box are evaluated exactly, as are paths the split
+ * pass leaves to its megakernel fallback. NULL restores the exact render. The surrogate must
+ * outlive its use by the pass. Mode PLT_SURROGATE_TEACHER runs the same passes with the exact
+ * lobe's matrix, taken in the network's frames and output encoding, in place of the network's
+ * outputs (no inference): its images differ from the exact render only by rounding, which checks
+ * the query / apply plumbing apart from the network's error. */
+enum { PLT_SURROGATE_NETWORK = 0, PLT_SURROGATE_TEACHER = 1 };
+int plt_pass_set_surrogate(plt_pass* pass, plt_surrogate* s, int mode);
+/* Counts of the last execute with the surrogate (waits for the device): query rows appended (one
+ * per HWSS wavelength of every eligible step; rows beyond the query buffer, four per path, are not
+ * evaluated), and eligible steps evaluated exactly (outside the training box or beyond the buffer). */
+int plt_pass_get_surrogate_stats(plt_pass* pass, uint64_t* rows, uint64_t* exact_steps);
 /* The matrix-core GEMM the MLP is built from: C [M x N] (FP32) = A [M x K] * Bt [N x K]^T with
  * bf16 row-major operands, K a multiple of 32 (for tests of the kernel). */
 int plt_mlp_gemm_nt(uint32_t M, uint32_t N, uint32_t K, const void* A, const void* Bt, float* C, plt_stream stream);

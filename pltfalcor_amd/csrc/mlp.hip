@@ -152,11 +152,15 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __bu
 // (stats[32..54) / [54..76)) has its Mueller matrix decoded (M00 = exp(y0), Mk = M00 sinh(yk)
 // after de-standardising with stats[0..16) / [16..32)) and applied to Sin; samples outside the box
 // are left for the exact pass.
+// With n_dev, the row count is min(*n_dev, n), read from device memory (a count a previous kernel
+// on the stream produced), so the launch needs no host synchronisation.
 __global__ void __launch_bounds__(kFThreads) mlp_infer_kernel(int n, const float* __restrict__ X, const __bf16* __restrict__ W1,
                                                               const __bf16* __restrict__ W2, const __bf16* __restrict__ W3,
                                                               const __bf16* __restrict__ W4, const float* __restrict__ params,
                                                               float* __restrict__ Y, const float* __restrict__ stats,
-                                                              const float* __restrict__ Sin, float* __restrict__ Sout) {
+                                                              const float* __restrict__ Sin, float* __restrict__ Sout,
+                                                              const uint32_t* __restrict__ n_dev) {
+    if (n_dev) n = int(min(uint32_t(n), *n_dev));
     __shared__ __bf16 xs[kFTile * kXRow];
     __shared__ __bf16 ha[kFTile * kHRow];
     __shared__ __bf16 hb[kFTile * kHRow];
@@ -498,7 +502,19 @@ int plt::surrogate_infer(plt_surrogate* s, int n, const float* X, float* Y, cons
     const int tiles = (n + kFTile - 1) / kFTile;
     const int grid = std::max(1, std::min(tiles, 2 * cus));
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(kFThreads), 0, st, n, X, s->W1, s->W2, s->W3, s->W4, s->params, Y, s->stats,
-                       Sin, Sout);
+                       Sin, Sout, nullptr);
+    return check_launch("mlp_infer_kernel");
+}
+
+int plt::surrogate_infer_counted(plt_surrogate* s, const uint32_t* n_dev, uint32_t cap, const float* X, float* Y, hipStream_t st) {
+    if (cap == 0) return PLT_OK;
+    if (cap > (1u << 30)) return set_error(PLT_EINVAL, "surrogate inference: too many rows");
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int tiles = int((cap + kFTile - 1) / kFTile);
+    const int grid = std::max(1, std::min(tiles, 2 * cus));
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(kFThreads), 0, st, int(cap), X, s->W1, s->W2, s->W3, s->W4, s->params, Y,
+                       s->stats, nullptr, nullptr, n_dev);
     return check_launch("mlp_infer_kernel");
 }
 

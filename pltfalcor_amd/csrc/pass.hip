@@ -18,6 +18,7 @@
 #include "sample_pattern.hpp"
 #include "scene_device.hpp"
 #include "scene_host.hpp"
+#include "surrogate.hpp"
 
 using plt::set_error;
 
@@ -192,6 +193,11 @@ struct plt_pass {
     size_t solve_bytes = 0;
     plt::SolveBuffers last_solve{};
     size_t last_regions = 0;
+    // learned BSDF surrogate in the split solve pass (not owned), and its query buffers
+    plt_surrogate* surrogate = nullptr;
+    bool surrogate_teacher = false;
+    void* d_sur = nullptr;
+    size_t sur_bytes = 0;
     // camera jitter pattern of the G-buffer pass (Center: the camera's own jitter)
     plt::SamplePattern pattern;
     float last_jitter[2] = {0.f, 0.f};
@@ -323,6 +329,7 @@ int plt_pass_destroy(plt_pass* p) {
     if (p->d_tile_draws) (void)hipFree(p->d_tile_draws);
     if (p->d_wf) (void)hipFree(p->d_wf);
     if (p->d_solve) (void)hipFree(p->d_solve);
+    if (p->d_sur) (void)hipFree(p->d_sur);
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     plt::drop_graphs(p);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
@@ -510,6 +517,9 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         p->records_words = words;
     }
     const plt_scene* s = p->scene;
+    // the camera as the scene holds it now (plt_scene_set_camera may have changed it since binding)
+    plt::camera_frame(s->host.camera, p->cam);
+    plt::view_rotation(s->host.camera, p->view_rot);
     plt::PtScene sc{};
     sc.tsc = plt::TraceScene{s->dev.nodes, s->dev.tris};
     sc.positions = s->dev.positions;
@@ -642,7 +652,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         w.counters = reinterpret_cast<uint32_t*>(b + 100 * cap);
     }
     plt::SolveBuffers sbuf{};
-    if (p->launch.solve_schedule == PLT_SOLVE_SPLIT) {
+    const bool useSplit = p->launch.solve_schedule == PLT_SOLVE_SPLIT || p->surrogate;
+    if (useSplit) {
         const size_t regions = (npaths + plt::kSolveRegion - 1) / plt::kSolveRegion;
         const size_t evCap = regions * plt::kSolveRegion * p->launch.solve_events_per_path;
         const size_t hitCap = regions * plt::kSolveRegion * p->launch.solve_hits_per_path;
@@ -665,6 +676,33 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         sbuf.hits_per_path = p->launch.solve_hits_per_path;
         sbuf.ev_cap = evCap;
         sbuf.hit_cap = hitCap;
+    }
+    if (p->surrogate) {
+        // query rows: four per path (the 1080p depth-8 workload appends about 1.4 per path); steps
+        // beyond are evaluated exactly and counted
+        const uint32_t stepsMax = maxb + 2u + uint32_t(mneeOcc);
+        const size_t qcap = std::min(size_t(4) * npaths, size_t(1) << 28);
+        const size_t xBytes = qcap * PLT_SURROGATE_INPUTS * 4, yBytes = qcap * PLT_SURROGATE_OUTPUTS * 4;
+        const size_t qBytes = size_t(stepsMax) * sbuf.ev_cap * 4;
+        const size_t bytes = 256 + xBytes + yBytes + qBytes;
+        if (bytes > p->sur_bytes) {
+            if (p->d_sur) (void)hipFree(p->d_sur);
+            p->d_sur = nullptr;
+            p->sur_bytes = 0;
+            void* q = nullptr;
+            if (hipMalloc(&q, bytes) != hipSuccess) return set_error(PLT_ENOMEM, "plt_pass_execute: surrogate query buffers allocation failed");
+            p->d_sur = q;
+            p->sur_bytes = bytes;
+        }
+        char* b = static_cast<char*>(p->d_sur);
+        sbuf.sur.counter = reinterpret_cast<uint32_t*>(b);
+        sbuf.sur.X = reinterpret_cast<float*>(b + 256);
+        sbuf.sur.Y = reinterpret_cast<const float*>(b + 256 + xBytes);
+        sbuf.sur.qidx = reinterpret_cast<uint32_t*>(b + 256 + xBytes + yBytes);
+        sbuf.sur.qcap = uint32_t(qcap);
+        sbuf.sur.steps_max = stepsMax;
+        sbuf.sur.stats = p->surrogate->stats;
+        sbuf.sur.teacher = p->surrogate_teacher ? 1u : 0u;
     }
     constexpr size_t kEv = plt::kEventsPerExecute;
     hipEvent_t ev[kEv] = {};
@@ -690,11 +728,13 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         return schedule == PLT_SCHEDULE_WAVEFRONT ? plt::launch_sample_pass_wavefront(sc, fr, w, lc, s)
                                                    : plt::launch_sample_pass(sc, fr, lc, s);
     };
-    const bool split = lc.solve_schedule == PLT_SOLVE_SPLIT;
+    const bool split = useSplit;
+    plt_surrogate* surrogate = p->surrogate;
     // the split pass's own timing events are recorded between its launches when it runs eagerly
     const bool split_marks = split && !p->graph && ev[2];
     auto solve_seq = [&](hipStream_t s) -> int {
-        return split ? plt::launch_solve_pass_split(sc, fr, sbuf, s, split_marks ? ev[2] : nullptr, split_marks ? ev[3] : nullptr)
+        return split ? plt::launch_solve_pass_split(sc, fr, sbuf, s, split_marks ? ev[2] : nullptr, split_marks ? ev[3] : nullptr,
+                                                    surrogate)
                      : plt::launch_solve_pass(sc, fr, s);
     };
     int rc;
@@ -825,6 +865,27 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
     if (default_ms) *default_ms = dms;
     if (best_ms) *best_ms = bms;
     return rc2;
+}
+
+int plt_pass_set_surrogate(plt_pass* p, plt_surrogate* s, int mode) {
+    if (!p || (mode != PLT_SURROGATE_NETWORK && mode != PLT_SURROGATE_TEACHER)) return set_error(PLT_EINVAL, "plt_pass_set_surrogate: bad arguments");
+    if (s && !s->has_stats)
+        return set_error(PLT_ESTATE, "plt_pass_set_surrogate: the surrogate has no training statistics (plt_surrogate_set_stats)");
+    p->surrogate = s;
+    p->surrogate_teacher = mode == PLT_SURROGATE_TEACHER;
+    plt::drop_graphs(p);
+    return PLT_OK;
+}
+
+int plt_pass_get_surrogate_stats(plt_pass* p, uint64_t* rows, uint64_t* exact_steps) {
+    if (!p || !rows || !exact_steps) return set_error(PLT_EINVAL, "plt_pass_get_surrogate_stats: bad arguments");
+    if (!p->last_solve.sur.counter) return set_error(PLT_ESTATE, "plt_pass_get_surrogate_stats: no frame rendered with the surrogate");
+    uint32_t c[2] = {0u, 0u};
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(c, p->last_solve.sur.counter, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_get_surrogate_stats: read-back failed");
+    *rows = c[0];
+    *exact_steps = c[1];
+    return PLT_OK;
 }
 
 int plt_pass_get_solve_stats(plt_pass* p, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,

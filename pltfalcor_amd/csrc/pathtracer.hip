@@ -225,6 +225,7 @@ int launch_solve_sample_4(const PtScene& sc, const PtFrame& fr, const SolveBuffe
 int launch_solve_sample_4_lights(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_solve_sample_1(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, bool lights);
 int launch_solve_unwind(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
+int launch_solve_unwind_surrogate(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, plt_surrogate* s, hipStream_t st);
 
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
     if (fr.n_paths == 0) return PLT_OK;
@@ -238,7 +239,7 @@ int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
 // Split solve pass: light sampling, unwind, then the megakernel over the paths whose region's event
 // area overflowed (its other blocks return at once). Three launches on one stream, no host waits.
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, hipEvent_t after_light,
-                            hipEvent_t after_unwind) {
+                            hipEvent_t after_unwind, plt_surrogate* surrogate) {
     if (fr.n_paths == 0) {
         if (after_light) (void)hipEventRecord(after_light, st);
         if (after_unwind) (void)hipEventRecord(after_unwind, st);
@@ -251,7 +252,7 @@ int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuf
                           : launch_solve_sample_1(sc, fr, sb, st, analytic);
     if (rc < 0) return rc;
     if (after_light) (void)hipEventRecord(after_light, st);
-    if ((rc = launch_solve_unwind(sc, fr, sb, st)) < 0) return rc;
+    if ((rc = surrogate ? launch_solve_unwind_surrogate(sc, fr, sb, surrogate, st) : launch_solve_unwind(sc, fr, sb, st)) < 0) return rc;
     if (after_unwind) (void)hipEventRecord(after_unwind, st);
     PtFrame fb = fr;
     fb.solve_overflow = sb.region;

@@ -123,12 +123,28 @@ constexpr size_t kEventsPerExecute = 5;  // kernel-timing events per execute of 
 // per region: event count, walk-vertex count, overflow mask (2 words: bit = lane whose path is left
 // to the megakernel)
 enum { kRegEvents = 0, kRegHits, kRegOverflowLo, kRegOverflowHi, kRegWords };
+// The learned BSDF surrogate in the split solve pass: a query pass (the unwind kernel appending the
+// network's inputs for every unwind step whose lobe the surrogate stands for), one batched
+// inference, and an apply pass (the unwind kernel using the network's Mueller matrices at those
+// steps). Steps whose inputs leave the training box, or that find no room, are evaluated exactly.
+struct SurPath {
+    const float* stats;  // outputs' mean [16] and std [16], inputs' training box lo [22] and hi [22]
+    float* X;            // query rows [qcap x 22]
+    const float* Y;      // network outputs of the rows [qcap x 16]
+    uint32_t* counter;   // [0] rows appended this frame, [1] eligible steps evaluated exactly
+    uint32_t qcap;
+    uint32_t* qidx;      // [k * ev_cap + e]: first row of event e's k-th unwind step, ~0 = exact
+    uint32_t steps_max;  // unwind steps per event at most: walk vertices + current vertex + forward steps
+    uint32_t teacher;    // 1: the query pass writes the exact lobe's matrix as the outputs (no network)
+};
+enum : int { kUnwindExact = 0, kUnwindQuery = 1, kUnwindApply = 2 };
 struct SolveBuffers {
     uint32_t* ev;
     uint32_t* hits;
     uint32_t* region;
     uint32_t ev_per_path, hits_per_path;
     size_t ev_cap, hit_cap;  // plane lengths: regions * 64 * per-path capacity
+    SurPath sur;             // used by the query and apply unwind modes only
 };
 
 // Wavefront sample pass: per-path state (SoA, n_paths entries each), two path queues of
@@ -164,7 +180,8 @@ int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
 // after_light / after_unwind (may be NULL): events recorded on `st` after the light-sampling and the
 // unwind kernels (kernel timing outside graph capture)
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st,
-                            hipEvent_t after_light = nullptr, hipEvent_t after_unwind = nullptr);
+                            hipEvent_t after_light = nullptr, hipEvent_t after_unwind = nullptr,
+                            plt_surrogate* surrogate = nullptr);
 int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 
 }  // namespace plt

@@ -20,6 +20,7 @@
 // mask; the unwind kernel skips it and the megakernel, launched after it with the mask, solves it.
 #pragma once
 #include "pathtracer_kernels.hpp"
+#include "surrogate_features.hpp"
 
 namespace plt {
 namespace {
@@ -398,9 +399,179 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
     }
 }
 
-// One event's transport: the megakernel's sequence from the sourced beam to measure. Writes the
-// RGB contribution the megakernel adds to the path's colour (zeros where it adds nothing).
+// ---------------------------------------------------------------------------------------
+// The learned surrogate in the unwind (query and apply modes).
+// ---------------------------------------------------------------------------------------
+// `rows` consecutive rows of the frame's query area for each active lane, one atomic per wave.
+PLT_DEV uint32_t sur_append_rows(uint32_t* counter, uint32_t rows) {
+    const uint64_t act = __ballot(1);
+    const int lane = int(__lane_id());
+    const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
+    const uint32_t rank = uint32_t(__popcll(act & ((uint64_t(1) << lane) - 1u)));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, rows * uint32_t(__popcll(act)));
+    base = __shfl(base, leader);
+    return base + rank * rows;
+}
+// Adds the number of active lanes with `pred` to *counter, one atomic per wave.
+PLT_DEV void sur_count(uint32_t* counter, bool pred) {
+    const uint64_t m = __ballot(pred);
+    if (pred && int(__lane_id()) == __ffsll(static_cast<unsigned long long>(m)) - 1) atomicAdd(counter, uint32_t(__popcll(m)));
+}
+
+// Query mode at an eligible step: the network's inputs for each HWSS component, as one row each.
+// The coherence area is the one the coated opaque dielectric's interference term sees: its
+// coherence exp(-(k/2pi)^2 tau^2 Z^T (D/r^2) Z / 2) (Z the normal in the exitant beam frame) equals
+// that of an isotropic beam of area pi (lambda/1000)^2 |Z_t|^2 / (Z^T (D/r^2) Z) at 1 mm, the beams
+// the network was trained on; the other two families' lobes do not depend on coherence and take
+// the middle of the training range.
 template <int N>
+PLT_DEV void sur_query(const Mat& m, int fam, const Frame& sd, V3 V, V3 woW, uint32_t lobe, const Beam<N>& beam, const SolveBuffers& sb,
+                       size_t e, uint32_t k) {
+    const SurPath& sp = sb.sur;
+    const V3 wiS = sd.to_local(V), woS = sd.to_local(woW);
+    float areaScale = -1.f;  // area = areaScale * (lambda/1000)^2; < 0: the neutral area
+    if (fam == kSurCoatedOpaque) {
+        Beam<1> g;  // the beam's frame and coherence, taken to the exitant frame as the exact evaluation does
+        g.S[0][0] = g.S[0][1] = g.S[0][2] = g.S[0][3] = 0.f;
+        g.lambda[0] = beam.lambda[0];
+        g.spdf[0] = 1.f;
+        g.comps = 1;
+        g.D00 = beam.D00; g.D01 = beam.D01; g.D11 = beam.D11;
+        g.r_m = beam.r_m;
+        g.pos = beam.pos;
+        g.r = sd.to_local(beam.r);
+        g.t = sd.to_local(beam.t);
+        g.distant = beam.distant;
+        g.construct_local_frame(woS);
+        const V3 Z = g.to_local(v3(0.f, 0.f, 1.f));
+        const float zz = Z.x * Z.x + Z.y * Z.y;
+        if (g.r_m == 0.f) {
+            areaScale = 0.f;  // not propagated: incoherent across any offset (outside the training range)
+        } else if (zz > 1e-12f) {
+            const float ir = 1.f / g.r_mm(), sc2 = ir * ir;
+            const float q = Z.x * (sc2 * g.D00 * Z.x + sc2 * g.D01 * Z.y) + Z.y * (sc2 * g.D01 * Z.x + sc2 * g.D11 * Z.y);
+            areaScale = q > 0.f ? kPi * zz / q : 1e30f;
+        }
+    }
+    const uint32_t row0 = sur_append_rows(sp.counter, uint32_t(N));
+    bool inside = row0 + uint32_t(N) <= sp.qcap;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (!inside || j >= beam.comps) continue;
+        const float l0 = 1e-3f * beam.lambda[j];
+        const float area = areaScale < 0.f ? 50.5f : areaScale * (l0 * l0);
+        float x[kSurIn];
+        sur_features(m, fam, wiS, woS, beam.lambda[j], area, x);
+        float* xr = sp.X + size_t(row0 + j) * kSurIn;
+#pragma unroll
+        for (int f = 0; f < kSurIn; ++f) {
+            inside &= x[f] >= sp.stats[2 * kSurOut + f] && x[f] <= sp.stats[2 * kSurOut + kSurIn + f];
+            xr[f] = x[f];
+        }
+        if (sp.teacher) {
+            // the exact matrix of the same evaluation in the network's frames, in its output encoding
+            // (standardised ln M00 and asinh(Mk / M00))
+            float M[16];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float ec[4] = {c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f, c == 3 ? 1.f : 0.f};
+                float col[4];
+                exact_stokes(m, wiS, woS, lobe, beam.lambda[j], area, ec, col);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) M[4 * r + c] = col[r];
+            }
+            float* yr = const_cast<float*>(sp.Y) + size_t(row0 + j) * kSurOut;
+            const bool pos = M[0] > 0.f;
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const float t = q == 0 ? (pos ? logf(M[0]) : -1e30f) : (pos ? asinhf(M[q] / M[0]) : 0.f);
+                yr[q] = (t - sp.stats[q]) / sp.stats[kSurOut + q];
+            }
+        }
+    }
+    sp.qidx[size_t(k) * sb.ev_cap + e] = inside ? row0 : ~0u;
+    sur_count(sp.counter + 1, !inside);
+}
+
+// Apply mode at an eligible step: the network's Mueller matrices (M00 = exp(y0), Mk = M00 sinh(yk)
+// after de-standardising) on the Stokes vectors taken to the network's incident frame; the beam's
+// frame and coherence change as in the exact evaluation. Returns false for a step to evaluate exactly.
+template <int N>
+PLT_DEV bool sur_apply(const Mat& m, const Frame& sd, V3 V, V3 woW, uint32_t lobe, Beam<N>& beam, const SolveBuffers& sb, size_t e,
+                       uint32_t k) {
+    const SurPath& sp = sb.sur;
+    const uint32_t row0 = sp.qidx[size_t(k) * sb.ev_cap + e];
+    if (row0 == ~0u) return false;
+    const V3 wiS = sd.to_local(V);
+    beam.rotate_frame(sd.from_local(perp_stark(-wiS)), beam.r);
+    float Sc[N][4];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) Sc[j][c] = beam.S[j][c];
+    const Frame f = mat_frame(m, sd);
+    beam.r = f.to_local(beam.r);
+    beam.t = f.to_local(beam.t);
+    const V3 wo = f.to_local(woW);
+    beam.construct_local_frame(wo);
+    const float* st = sp.stats;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        if (j >= beam.comps) continue;
+        const float* y = sp.Y + size_t(row0 + j) * kSurOut;
+        const float m00 = expf(y[0] * st[kSurOut] + st[0]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float acc = 0.f;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int q = 4 * r + c;
+                const float v = q == 0 ? 1.f : sinhf(y[q] * st[kSurOut + q] + st[q]);
+                acc += v * Sc[j][c];
+            }
+            beam.S[j][r] = m00 * acc;
+        }
+    }
+    mat_coherence_transform<N>(m, wo, lobe, beam);
+    beam.r = f.from_local(beam.r);
+    beam.t = f.from_local(beam.t);
+    return true;
+}
+
+// unwind_step in the given mode; `k` is the step's index within its event (the query table's row).
+template <int N, int Mode>
+PLT_DEV void unwind_step_m(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, size_t e, uint32_t k, uint32_t tri, float b1,
+                           float b2, V3 wo, uint32_t lobe, Beam<N>& beam, const V3* posW = nullptr) {
+    if (Mode == kUnwindExact) {
+        unwind_step<N>(sc, fr, tri, b1, b2, wo, lobe, beam, posW);
+        return;
+    }
+    const V3 V = -beam.r;
+    Surf s = surface_at(sc, tri, b1, b2, V);
+    if (posW) s.pos = *posW;
+    beam.propagate(s.pos);
+    const Mat m = material_at(sc, s.material);
+    const int fam = sur_family(m, lobe);
+    bool done = false;
+    if (fam >= 0 && k < sb.sur.steps_max) {
+        if (Mode == kUnwindQuery) sur_query<N>(m, fam, s.f, V, wo, lobe, beam, sb, e, k);
+        else done = sur_apply<N>(m, s.f, V, wo, lobe, beam, sb, e, k);
+    }
+    if (!done) mat_eval<N>(m, s.f, V, wo, lobe, beam);
+    if (fr.options & kOptHwssMis) {
+        float p[N];
+        mat_lambda_pdf<N>(m, s.f, V, wo, lobe, beam, p);
+#pragma unroll
+        for (int j = 0; j < N; ++j) beam.spdf[j] *= p[j];
+    }
+}
+
+// One event's transport: the megakernel's sequence from the sourced beam to measure. Writes the
+// RGB contribution the megakernel adds to the path's colour (zeros where it adds nothing). Its
+// unwind steps are numbered in order (walk vertices, current vertex, stored vertices) for the
+// surrogate's query table.
+template <int N, int Mode>
 PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, size_t e, uint32_t meta, uint32_t region,
                           float* out) {
     const uint32_t lane = meta & 63u, kind = (meta >> 13) & 3u;
@@ -420,6 +591,7 @@ PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffe
     Beam<N> beam = (meta & kEvBitDistant) ? source_beam_distant<N>(dir, size, Le, lam, fr.max_omega, comps)
                                           : source_beam_area<N>(ev_get3(sb, e, kEvPos), dir, size, ev_getf(sb, e, kEvDist), Le, lam,
                                                                 fr.max_omega, comps);
+    uint32_t k = 0;  // unwind step index
     if (kind == kEvKindNee) {
         const float typePdf = ev_getf(sb, e, kEvTypePdf), lsPdf = ev_getf(sb, e, kEvLsPdf);
         beam.scale(1.f / typePdf);
@@ -456,8 +628,8 @@ PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffe
                     const V3 p2 = idx == 0 ? ev_get3(sb, e, kEvOrg)
                                            : v3(__uint_as_float(h[-1 + kHitPos * sb.hit_cap]), __uint_as_float(h[-1 + (kHitPos + 1) * sb.hit_cap]),
                                                 __uint_as_float(h[-1 + (kHitPos + 2) * sb.hit_cap]));
-                    unwind_step<N>(sc, fr, h[kHitTri * sb.hit_cap], __uint_as_float(h[kHitB1 * sb.hit_cap]),
-                                   __uint_as_float(h[kHitB2 * sb.hit_cap]), wnormalize(p2 - pos), h[kHitLobe * sb.hit_cap], beam, &pos);
+                    unwind_step_m<N, Mode>(sc, fr, sb, e, k++, h[kHitTri * sb.hit_cap], __uint_as_float(h[kHitB1 * sb.hit_cap]),
+                                           __uint_as_float(h[kHitB2 * sb.hit_cap]), wnormalize(p2 - pos), h[kHitLobe * sb.hit_cap], beam, &pos);
                     pos = p2;
                 }
                 if (currentVertexMS) {
@@ -475,9 +647,15 @@ PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffe
             }
         }
         // evalNEE (pltpt_solve.rt.slang:219-243): the current vertex's non-delta lobes
-        if (evalCurrent) unwind_step<N>(sc, fr, bounce.tri, bounce.b1, bounce.b2, bounce.wi, kLobeNonDelta, beam);
+        if (evalCurrent) unwind_step_m<N, Mode>(sc, fr, sb, e, k, bounce.tri, bounce.b1, bounce.b2, bounce.wi, kLobeNonDelta, beam);
+        ++k;
     }
-    forward_unwind<N>(sc, fr, n, li, p, beam);
+    // forwardUnwind (pltpt_solve.rt.slang:68-79): the stored vertices, last first
+    for (int q = p - 1; q >= 0; --q) {
+        const Rec r = read_rec(fr.records, n, li, q);
+        unwind_step_m<N, Mode>(sc, fr, sb, e, k++, r.tri, r.b1, r.b2, r.wi, r.lobe, beam);
+    }
+    if (Mode == kUnwindQuery) return;  // the query pass needs no colour
     Solve sv;
     sv.r = sv.g = sv.b = 0.f;
     measure<N>(sc, fr, sv, p, bounce.contrib, beam);
@@ -486,8 +664,13 @@ PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffe
     out[2] = sv.b;
 }
 
-template <int N>
-__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+// Two waves per SIMD: the beam, the material instance and the evaluation temporaries fit the 256
+// registers without spilling, and this kernel (no traversal, few dependent fetches) loses less to
+// the lower occupancy than it gains (1080p depth 8: 1.30 ms per frame at two waves, 2.14 ms at four
+// with 612 B of scratch per lane, 2.59 ms at six). Mode: exact unwind, or the surrogate's query or
+// apply pass (the query pass writes no colour).
+template <int N, int Mode>
+__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
 solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
     __shared__ float part[3][kPtBlock];
     __shared__ uint32_t owner[kPtBlock];
@@ -508,8 +691,9 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
             const uint32_t meta = ev_get(sb, e, kEvMeta);
             who = meta & 63u;
             if ((ovf >> who) & 1u) who = kPtBlock;  // left to the megakernel
-            else unwind_event<N>(sc, fr, sb, e, meta, region, c);
+            else unwind_event<N, Mode>(sc, fr, sb, e, meta, region, c);
         }
+        if (Mode == kUnwindQuery) continue;
         owner[lane] = who;
         part[0][lane] = c[0];
         part[1][lane] = c[1];
@@ -525,8 +709,14 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
         }
         __syncthreads();
     }
+    if (Mode == kUnwindQuery) return;
     const uint32_t li = region * kPtBlock + lane;
     if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sr, sgr, sbl, 1.f);
+}
+
+// Clears the surrogate's query counters (a kernel, so a captured graph replays it).
+__global__ void sur_clear_kernel(uint32_t* counter) {
+    if (threadIdx.x < 2) counter[threadIdx.x] = 0u;
 }
 
 }  // namespace

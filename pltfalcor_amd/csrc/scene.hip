@@ -1,6 +1,7 @@
 // Scene objects of the C ABI: procedural construction, upload of the BVH / triangle / shading
 // arrays to HBM, host views for callers, and the batched ray-cast entry point.
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -179,6 +180,18 @@ int plt_scene_set_lights(plt_scene* scene, const plt_light* lights, uint32_t cou
             return set_error(PLT_EINVAL, "plt_scene_set_lights: intensity profile id out of range");
     }
     scene->host.lights.assign(lights, lights + count);
+    return PLT_OK;
+}
+
+int plt_scene_set_camera(plt_scene* scene, const plt_camera* camera) {
+    if (!scene || !camera) return set_error(PLT_EINVAL, "plt_scene_set_camera: bad arguments");
+    const plt_camera& c = *camera;
+    const float f[3] = {c.target[0] - c.pos[0], c.target[1] - c.pos[1], c.target[2] - c.pos[2]};
+    if (!(f[0] * f[0] + f[1] * f[1] + f[2] * f[2] > 0.f) || !(c.fov_y > 0.f && c.fov_y < 3.1415926f) || !(c.aspect > 0.f) ||
+        !(c.focal_distance > 0.f) || !(c.polarizer >= 0.f && c.polarizer <= 1.f) || !std::isfinite(c.polarizer_theta))
+        return set_error(PLT_EINVAL, "plt_scene_set_camera: degenerate camera (target at the position, fov outside (0, pi), "
+                                     "aspect or focal distance not positive, polarizer outside [0, 1])");
+    scene->host.camera = c;
     return PLT_OK;
 }
 

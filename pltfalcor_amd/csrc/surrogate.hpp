@@ -29,4 +29,6 @@ namespace plt {
 // Sout, the decoded Mueller matrix applied to Sin [n x 4] for samples inside the training box
 // (the others are left untouched for the exact pass).
 int surrogate_infer(plt_surrogate* s, int n, const float* X, float* Y, const float* Sin, float* Sout, hipStream_t st);
+// Raw outputs Y of the first min(*n_dev, cap) rows of X, the count read on the device.
+int surrogate_infer_counted(plt_surrogate* s, const uint32_t* n_dev, uint32_t cap, const float* X, float* Y, hipStream_t st);
 }  // namespace plt

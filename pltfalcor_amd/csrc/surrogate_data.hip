@@ -10,49 +10,17 @@
 #include "plt_internal.hpp"
 #include "plt_materials.hpp"
 #include "surrogate.hpp"
+#include "surrogate_features.hpp"
 
 namespace plt {
 namespace {
 
 constexpr int kSBlock = 128;
 constexpr int kIn = PLT_SURROGATE_INPUTS, kOut = PLT_SURROGATE_OUTPUTS;
-enum { kFamCoatedOpaque = 0, kFamCoatedConductor = 1, kFamGrated = 2, kFamilies = 3 };
+constexpr int kFamCoatedOpaque = kSurCoatedOpaque, kFamCoatedConductor = kSurCoatedConductor, kFamGrated = kSurGrated,
+              kFamilies = kSurFamilies;
 
 __device__ __forceinline__ uint16_t h16_bits(float x) { return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x)); }
-
-// One-wavelength beam arriving along -wi with coherence area `area` (um^2), as the batched
-// material interface sources it (material_batch.hip).
-__device__ __forceinline__ Beam<1> unit_beam(V3 wi, float lambda, float area, const float* S) {
-    Beam<1> b;
-    b.S[0][0] = S[0]; b.S[0][1] = S[1]; b.S[0][2] = S[2]; b.S[0][3] = S[3];
-    b.lambda[0] = lambda;
-    b.spdf[0] = 1.f;
-    b.comps = 1;
-    const float l0 = 1e-3f * lambda;
-    const float d = kPi * (l0 * l0) / area;
-    b.D00 = d; b.D01 = 0.f; b.D11 = d;
-    b.r_m = 1e-3f;
-    b.pos = v3(0.f, 0.f, 0.f);
-    b.r = -wi;
-    b.t = perp_stark(b.r);
-    b.distant = false;
-    return b;
-}
-__device__ __forceinline__ Frame identity_frame() {
-    Frame f;
-    f.T = v3(1.f, 0.f, 0.f);
-    f.B = v3(0.f, 1.f, 0.f);
-    f.N = v3(0.f, 0.f, 1.f);
-    return f;
-}
-// Exact outgoing Stokes vector for incident S.
-__device__ __forceinline__ void exact_stokes(const Mat& m, V3 wi, V3 wo, uint32_t lobe, float lambda, float area, const float* S,
-                                             float* out) {
-    Beam<1> b = unit_beam(wi, lambda, area, S);
-    mat_eval<1>(m, identity_frame(), wi, wo, lobe, b);
-#pragma unroll
-    for (int c = 0; c < 4; ++c) out[c] = b.S[0][c];
-}
 
 __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t seed, plt_material* __restrict__ mats,
                                                          float* __restrict__ geo, float* __restrict__ X, float* __restrict__ M,
@@ -74,7 +42,6 @@ __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t 
     for (int k = 0; k < 4; ++k) { pm.specular[k] = h16_bits(0.f); pm.data1[k] = 0; pm.data2[k] = 0; }
     pm.ior = h16_bits(1.f);
     pm.emission_profile = pm.ior_n_profile = pm.ior_k_profile = PLT_NO_PROFILE;
-    float p[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // material inputs, about [-1, 1]
     const uint32_t lobe = kLobeScatterReflection;
     switch (fam) {
         case kFamCoatedOpaque:  // albedo U[0.1,0.9]^3, roughness U[0.05,0.6], coat IoR U[1.3,1.8], tau U[0.2,5] um
@@ -109,26 +76,6 @@ __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t 
     const V3 wo = cosine_hemisphere(u_wo0, u_wo1);
     const float lambda = 380.f + 340.f * u_lam;
     const float area = 1.f + 99.f * u_area;
-    const float k = k2Pi / (1e-3f * lambda);
-    const float q = m.fs.q, l = sqrtf(m.fs.T);
-    // derived inputs of the K-correlation lobe: log of its PSD argument k l |wi_t + wo_t|, of k q,
-    // of k l and of |wi_t + wo_t|
-    const float ht = sqrtf((wi.x + wo.x) * (wi.x + wo.x) + (wi.y + wo.y) * (wi.y + wo.y));
-    p[0] = (logf(k * l * ht + 1e-3f) - 1.f) / 3.f;
-    p[1] = (logf(k * q) - 1.f) / 1.2f;
-    p[2] = (logf(k * l) - 2.f) / 1.5f;
-    p[3] = (logf(ht + 1e-4f) + 1.5f) / 2.5f;
-    // the decoded (binary16-rounded) material parameters
-    if (fam != kFamGrated) {
-        p[4] = fam == kFamCoatedOpaque ? 2.f * rgb_to_spectrum(m.col[0], m.col[1], m.col[2], lambda) - 1.f : 0.f;
-        p[5] = 2.f * (m.coatIor - 1.3f) / .5f - 1.f;
-        p[6] = 2.f * (m.tau - .2f) / 4.8f - 1.f;
-    } else {
-        p[4] = 2.f * (1.f / m.gr.ipx - .5f) / 1.5f - 1.f;
-        p[5] = m.gr.gx;
-        p[6] = m.gr.gy;
-        p[7] = 2.f * (m.gr.q - .05f) / .45f - 1.f;
-    }
     mats[i] = pm;
     const size_t N = n;
     geo[PLT_SG_WI_X * N + i] = wi.x; geo[PLT_SG_WI_Y * N + i] = wi.y; geo[PLT_SG_WI_Z * N + i] = wi.z;
@@ -136,16 +83,7 @@ __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t 
     geo[PLT_SG_LAMBDA * N + i] = lambda;
     geo[PLT_SG_LOBE * N + i] = __uint_as_float(lobe);
     geo[PLT_SG_AREA * N + i] = area;
-    float* x = X + size_t(i) * kIn;
-    x[0] = wi.x; x[1] = wi.y; x[2] = wi.z; x[3] = wo.x; x[4] = wo.y; x[5] = wo.z;
-    x[6] = (lambda - 550.f) / 170.f;
-    x[7] = (q - .325f) / .275f;
-    x[8] = (l - .46f) / .41f;
-    x[9] = (area - 50.5f) / 49.5f;
-#pragma unroll
-    for (int f = 0; f < kFamilies; ++f) x[10 + f] = f == fam ? 1.f : 0.f;
-#pragma unroll
-    for (int f = 0; f < 9; ++f) x[13 + f] = p[f];
+    sur_features(m, fam, wi, wo, lambda, area, X + size_t(i) * kIn);
     float* Mi = M + size_t(i) * kOut;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {

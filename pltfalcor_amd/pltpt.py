@@ -169,6 +169,21 @@ class PLTPT:
         _lib.check(self.lib.plt_pass_get_kernel_times_split(self.handle, *(C.byref(x) for x in v), C.byref(n)))
         return (*(x.value for x in v), n.value)
 
+    SURROGATE_NETWORK, SURROGATE_TEACHER = 0, 1
+
+    def set_surrogate(self, surrogate, mode: int = 0):
+        """Renders with the learned BSDF surrogate (a trained pltfalcor_amd.surrogate.Surrogate) standing
+        for the coated and grated scattering lobes, or exactly again with None. mode SURROGATE_TEACHER
+        puts the exact lobe's matrix in the network's place (a check of the in-render plumbing)."""
+        self._surrogate = surrogate  # kept alive while the pass may use it
+        _lib.check(self.lib.plt_pass_set_surrogate(self.handle, None if surrogate is None else surrogate.handle, int(mode)))
+
+    def surrogate_stats(self) -> dict:
+        """Rows the network evaluated and eligible steps evaluated exactly, in the last execute."""
+        r, x = C.c_uint64(), C.c_uint64()
+        _lib.check(self.lib.plt_pass_get_surrogate_stats(self.handle, C.byref(r), C.byref(x)))
+        return {"rows": r.value, "exact_steps": x.value}
+
     def solve_stats(self) -> dict:
         """Counts of the last execute's split solve pass (waits for the device): light events, stored
         walk vertices, replayed vertices (sum of vertex index + 1 over events), overflowed paths."""

@@ -17,6 +17,51 @@ PATH_1 = dict(seed=1, sphere_count=64, subdivision=5, material_kinds=4)   # 512x
 PATH_2 = dict(seed=1, sphere_count=256, subdivision=5, material_kinds=8)  # 1920x1080, ~5.24M triangles
 
 
+# The render with the learned surrogate (C4): the 1080p workload scene plus a rough diffraction
+# grating on the ground under the light and, between them, a 5 mm glass slab (IoR 1.5, smooth), so
+# that light reaches the grating through the slab's two faces by manifold NEE.
+C4_GRATING = dict(size=3.0, height=0.01, pitch_um=1.0, direction=0.5, roughness=0.3)
+C4_SLAB = dict(size=3.6, thickness=0.005, center_y=2.6, ior=1.5)
+
+
+def c4_scene(ctx, aspect: float):
+    """The C2 procedural scene (PATH_2) with the grating plate and the glass slab of C4 added."""
+    from . import materials as M
+    from .meshes import create_cube, create_quad
+    base = Scene(None, aspect=aspect, **PATH_2)
+    a = base.numpy()
+    cam = base.arrays.camera
+    base.close()
+    pos, nrm, idx = [a["positions"]], [a["normals"]], [a["indices"].astype(np.int64)]
+    tri_mesh, mesh_mat = [a["tri_mesh"]], [a["mesh_material"]]
+    mats = bytearray(a["materials"])
+    n_mat = len(mats) // C.sizeof(Material)
+    n_mesh = int(a["mesh_material"].size)
+    nv = int(a["positions"].shape[0])
+
+    def add(p, n, i, material):
+        nonlocal nv, n_mesh, n_mat, mats
+        pos.append(np.asarray(p, np.float32))
+        nrm.append(np.asarray(n, np.float32))
+        idx.append(np.asarray(i, np.int64) + nv)
+        tri_mesh.append(np.full(len(i), n_mesh, np.uint32))
+        mesh_mat.append(np.array([n_mat], np.uint32))
+        mats += M.pack([material])
+        nv += len(p)
+        n_mesh += 1
+        n_mat += 1
+
+    g = C4_GRATING
+    qp, qn, qi, _ = create_quad((g["size"], g["size"]))
+    add(qp + np.array([0.0, g["height"], 0.0]), qn, qi,
+        M.grated_conductor(g["pitch_um"], g["direction"], 0, 1, roughness=g["roughness"]))
+    sl = C4_SLAB
+    cp, cn, ci, _ = create_cube((sl["size"], sl["thickness"], sl["size"]))
+    add(cp + np.array([0.0, sl["center_y"], 0.0]), cn, ci, M.dielectric(sl["ior"], 0.0))
+    return Scene.from_arrays(ctx, np.concatenate(pos), np.concatenate(nrm), np.concatenate(idx).astype(np.uint32),
+                             np.concatenate(tri_mesh), np.concatenate(mesh_mat), bytes(mats), a["profiles"], cam)
+
+
 class Scene:
     """A scene on one GPU (or host-only when ctx is None): the procedural workload scene, or
     `Scene.from_arrays` for caller geometry, materials and spectral profiles."""
@@ -95,6 +140,21 @@ class Scene:
                                                   C.c_void_p(t.ctypes.data)))
         scale = (np.float32(intensity) * t).astype(np.float32)
         self.env = dict(width=w, height=h, rgba=rgba, transform=m.reshape(9), scale=scale)
+
+    def set_camera(self, **fields):
+        """Changes camera fields (pos, target, up, fov_y, aspect, focal_distance, jitter, polarizer,
+        polarizer_theta) for the next execute of a bound pass."""
+        c = self.arrays.camera
+        for k, v in fields.items():
+            if not hasattr(c, k):
+                raise KeyError(k)
+            cur = getattr(c, k)
+            if isinstance(cur, C.Array):
+                for i, x in enumerate(v):
+                    cur[i] = float(x)
+            else:
+                setattr(c, k, float(v))
+        _lib.check(self.lib.plt_scene_set_camera(self.handle, C.byref(c)))
 
     def numpy(self):
         """Host copies of the scene arrays."""

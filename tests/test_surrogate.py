@@ -219,3 +219,109 @@ def test_gpu_surrogate_training_reduces_error(gpu):
     assert np.median(rel) < 0.5 * np.median(rel_base)
     assert fb < 0.01 * n
     s.close()
+
+
+def _render_frames(ctx, scene, w, h, opts, surrogate=None, mode=0, frames=2, solve_split=True):
+    """Colour of `frames` frames of the pass (split solve pass), with the surrogate in `mode` or
+    exact; and the pass's surrogate counts of the last frame."""
+    import torch
+    from pltfalcor_amd.pltpt import PLTPT
+    p = PLTPT(ctx, opts)
+    p.set_sample_pattern(PLTPT.HALTON, 16)
+    if solve_split:
+        p.set_launch_config(solve_schedule=PLTPT.SOLVE_SPLIT)
+    p.set_scene(scene)
+    if surrogate is not None:
+        p.set_surrogate(surrogate, mode)
+    color = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+    albedo, normal = torch.empty_like(color), torch.empty_like(color)
+    out = []
+    for _ in range(frames):
+        color.fill_(float("nan"))
+        p.execute(w, h, color, albedo, normal)
+        torch.cuda.synchronize()
+        out.append(color.cpu().numpy().copy())
+    stats = p.surrogate_stats() if surrogate is not None else None
+    p.close()
+    return np.stack(out), stats
+
+
+def _coated_scene(ctx):
+    from pltfalcor_amd.scene import Scene
+    # eight material kinds: the coated opaque dielectric, grated conductor and coated conductor among them
+    return Scene(ctx, seed=7, sphere_count=24, subdivision=3, material_kinds=8)
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_in_render_outside_box_equals_exact(gpu):
+    """With a training box that no input lies in, every step the surrogate would stand for falls
+    back to the exact lobe: the render (query pass, inference, apply pass) is bit-identical to the
+    exact split pass, every such step appends one row per HWSS wavelength and is counted as exact."""
+    from pltfalcor_amd.surrogate import Surrogate
+    ctx, _ = _ctx_with_drude()
+    s = Surrogate(ctx, max_batch=4096, seed=3)
+    s.set_stats(np.zeros(16), np.ones(16), np.full(22, 1e30), np.full(22, -1e30))
+    scene = _coated_scene(ctx)
+    w, h, opts = 96, 64, {"maxBounces": 6, "HWSS": 4}
+    ref, _ = _render_frames(ctx, scene, w, h, opts)
+    got, st = _render_frames(ctx, scene, w, h, opts, surrogate=s)
+    print("surrogate counts", st)
+    assert st["exact_steps"] > 0 and st["rows"] == 4 * st["exact_steps"]
+    assert np.array_equal(ref.view(np.uint32), got.view(np.uint32))
+    s.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hwss", [4, 1])
+def test_gpu_surrogate_teacher_render_matches_exact(gpu, hwss):
+    """The in-render plumbing apart from the network: with the exact lobe's matrix taken in the
+    network's frames (incident tangent perp_stark(-wi), exitant frame of the exact evaluation), its
+    coherence area mapped from the beam's coherence and encoded as the network's outputs, in place
+    of the network's outputs, the render equals the exact one up to rounding -- a wrong frame, area,
+    family or lobe, or a slip in the decoding, shows as percent-level differences on the coated and
+    grated spheres."""
+    from pltfalcor_amd.surrogate import Surrogate
+    ctx, _ = _ctx_with_drude()
+    s = Surrogate(ctx, max_batch=4096, seed=3)
+    s.set_stats(np.full(16, 0.25), np.full(16, 1.5), np.full(22, -1e30), np.full(22, 1e30))
+    scene = _coated_scene(ctx)
+    w, h, opts = 96, 64, {"maxBounces": 6, "HWSS": hwss}
+    ref, _ = _render_frames(ctx, scene, w, h, opts, frames=3)
+    got, st = _render_frames(ctx, scene, w, h, opts, surrogate=s, mode=1, frames=3)
+    print("teacher counts", st)
+    assert st["rows"] > 0 and st["exact_steps"] == 0
+    a, b = ref[..., :3].astype(np.float64), got[..., :3].astype(np.float64)
+    assert np.isfinite(b).all()
+    floor = 1e-4 * np.abs(a).mean()
+    rel = np.abs(a - b) / (np.abs(a) + floor)
+    print(f"teacher vs exact: rel quantiles 50/99/99.9/max {np.quantile(rel, [.5, .99, .999])} {rel.max():.3e}; "
+          f"mean {a.mean():.6f} {b.mean():.6f}")
+    assert np.quantile(rel, 0.999) < 1e-3 and abs(a.mean() - b.mean()) <= 1e-4 * abs(a.mean())
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_network_in_render_against_exact(gpu):
+    """A/B of the trained network against the exact lobes in the render: a surrogate trained on
+    exact samples of the three families renders the same frames; the network answers the steps
+    inside its training box (most of the eligible ones), the image stays finite, and its error
+    against the exact render is of the size of the network's held-out error on those lobes
+    (reported), not larger."""
+    from pltfalcor_amd.surrogate import ExactSamples, Surrogate, train
+    ctx, _ = _ctx_with_drude()
+    tr = ExactSamples(ctx, 1 << 19, seed=21)
+    s = Surrogate(ctx, max_batch=8192, seed=3)
+    train(s, tr, epochs=4, batch=8192, lr=2e-3)
+    scene = _coated_scene(ctx)
+    w, h, opts = 96, 64, {"maxBounces": 6, "HWSS": 4}
+    ref, _ = _render_frames(ctx, scene, w, h, opts, frames=4)
+    got, st = _render_frames(ctx, scene, w, h, opts, surrogate=s, frames=4)
+    n_steps = st["rows"] // 4
+    print("network counts", st, f"network share of eligible steps {1 - st['exact_steps'] / max(1, n_steps):.3f}")
+    assert np.isfinite(got).all() and (got[..., 3] == 1).all()
+    assert st["exact_steps"] < 0.5 * n_steps
+    a, b = ref[..., :3].mean(0), got[..., :3].mean(0)
+    err = np.abs(a - b).mean() / np.abs(a).mean()
+    print(f"network render: mean |difference| / mean {err:.4f}; image means exact {a.mean():.5f} network {b.mean():.5f}")
+    assert err < 0.5
+    s.close()
