@@ -281,6 +281,16 @@ def c4_run(ctx, sur, training):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         st = p.surrogate_stats() if use_sur else None
+        if use_sur and theta is None:
+            # which inputs of the last frame's queries lie outside the training box, per input
+            X = p.surrogate_queries()
+            lo, hi = sur.stats[2], sur.stats[3]
+            written = np.any(X != 0.0, axis=1)
+            Xw = X[written]
+            out_dim = (Xw < lo) | (Xw > hi)
+            st["query_rows_written"] = int(Xw.shape[0])
+            st["rows_outside_box"] = int(out_dim.any(axis=1).sum())
+            st["rows_outside_box_per_input"] = {int(k): int(v) for k, v in enumerate(out_dim.sum(0)) if v}
         p.close()
         return out[..., :3].cpu().numpy().astype(np.float64), dt, st
 
@@ -297,6 +307,13 @@ def c4_run(ctx, sur, training):
         stokes[tag] = (i0 + i180, i0 - i180, i90 - i270)
     s0_mean = float(stokes[False][0].mean())
     rmse = {f"S{k}": float(np.sqrt(np.mean((stokes[True][k] - stokes[False][k]) ** 2)) / s0_mean) for k in range(3)}
+    # the same without the 0.1 % of pixels where the exact S0 is largest (fireflies of 16-spp images,
+    # which dominate an RMSE), and the mean absolute error
+    keep = stokes[False][0] <= np.quantile(stokes[False][0], 0.999)
+    rmse_trim = {f"S{k}": float(np.sqrt(np.mean(((stokes[True][k] - stokes[False][k]) ** 2)[keep])) / s0_mean)
+                 for k in range(3)}
+    mae = {f"S{k}": float(np.mean(np.abs(stokes[True][k] - stokes[False][k]))) / s0_mean for k in range(3)}
+    rmse_unpolarized = float(np.sqrt(np.mean((su - ex) ** 2)) / ex.mean())
     lum = lambda im: im @ np.array([0.2126, 0.7152, 0.0722])  # noqa: E731
     le, ls = lum(ex), lum(su)
     floor = 1e-3 * le.mean()
@@ -312,10 +329,13 @@ def c4_run(ctx, sur, training):
             "exact_path_samples_per_s": WIDTH * HEIGHT * C4_SPP / t_ex,
             "surrogate_path_samples_per_s": WIDTH * HEIGHT * C4_SPP / t_su,
             "surrogate_over_exact": t_ex / t_su,
-            "stokes_rmse_rel_to_mean_s0": rmse,
+            "stokes_rmse_rel_to_mean_s0": rmse, "stokes_rmse_rel_without_top_0.1pct_s0": rmse_trim,
+            "stokes_mae_rel_to_mean_s0": mae, "unpolarized_rmse_rel_to_mean": rmse_unpolarized,
             "luminance_rel_err": {"median": float(np.median(rel)), "mean": float(rel.mean()),
                                   "image_mean": float(abs(ls.mean() - le.mean()) / le.mean())},
             "network_share_of_eligible_steps": 1.0 - st["exact_steps"] / max(1, eligible),
+            "queries_outside_training_box": {k: st[k] for k in ("query_rows_written", "rows_outside_box",
+                                                                "rows_outside_box_per_input")},
             "eligible_steps_per_frame": eligible, "training": training}
 
 

@@ -590,6 +590,10 @@ int plt_pass_set_surrogate(plt_pass* pass, plt_surrogate* s, int mode);
  * per HWSS wavelength of every eligible step; rows beyond the query buffer, four per path, are not
  * evaluated), and eligible steps evaluated exactly (outside the training box or beyond the buffer). */
 int plt_pass_get_surrogate_stats(plt_pass* pass, uint64_t* rows, uint64_t* exact_steps);
+/* Copies up to max_rows of the last execute's query rows (22 inputs each, in append order; rows of
+ * HWSS components a beam does not carry are not written) to host memory, for analysing which
+ * inputs leave the training box; *rows receives the number copied. */
+int plt_pass_copy_surrogate_queries(plt_pass* pass, float* dst, uint64_t max_rows, uint64_t* rows);
 /* The matrix-core GEMM the MLP is built from: C [M x N] (FP32) = A [M x K] * Bt [N x K]^T with
  * bf16 row-major operands, K a multiple of 32 (for tests of the kernel). */
 int plt_mlp_gemm_nt(uint32_t M, uint32_t N, uint32_t K, const void* A, const void* Bt, float* C, plt_stream stream);

@@ -165,6 +165,7 @@ SIGNATURES = [
                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("plt_pass_set_surrogate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("plt_scene_set_camera", C.c_int, [C.c_void_p, C.POINTER(Camera)]),
+    ("plt_pass_copy_surrogate_queries", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     ("plt_pass_get_surrogate_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("plt_spectrum_load_spd", C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                         C.POINTER(C.c_uint32)]),

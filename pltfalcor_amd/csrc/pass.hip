@@ -888,6 +888,20 @@ int plt_pass_get_surrogate_stats(plt_pass* p, uint64_t* rows, uint64_t* exact_st
     return PLT_OK;
 }
 
+int plt_pass_copy_surrogate_queries(plt_pass* p, float* dst, uint64_t max_rows, uint64_t* rows) {
+    if (!p || !rows || (max_rows && !dst)) return set_error(PLT_EINVAL, "plt_pass_copy_surrogate_queries: bad arguments");
+    const plt::SurPath& sp = p->last_solve.sur;
+    if (!sp.counter) return set_error(PLT_ESTATE, "plt_pass_copy_surrogate_queries: no frame rendered with the surrogate");
+    uint32_t c = 0;
+    if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&c, sp.counter, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_copy_surrogate_queries: read-back failed");
+    const uint64_t n = std::min<uint64_t>(std::min<uint64_t>(c, sp.qcap), max_rows);
+    if (n && hipMemcpy(dst, sp.X, n * PLT_SURROGATE_INPUTS * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_pass_copy_surrogate_queries: read-back failed");
+    *rows = n;
+    return PLT_OK;
+}
+
 int plt_pass_get_solve_stats(plt_pass* p, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,
                               uint64_t* overflowed_paths) {
     if (!p || !events || !walk_vertices || !event_vertices || !overflowed_paths)

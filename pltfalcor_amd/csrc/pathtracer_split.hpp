@@ -423,8 +423,8 @@ PLT_DEV void sur_count(uint32_t* counter, bool pred) {
 // The coherence area is the one the coated opaque dielectric's interference term sees: its
 // coherence exp(-(k/2pi)^2 tau^2 Z^T (D/r^2) Z / 2) (Z the normal in the exitant beam frame) equals
 // that of an isotropic beam of area pi (lambda/1000)^2 |Z_t|^2 / (Z^T (D/r^2) Z) at 1 mm, the beams
-// the network was trained on; the other two families' lobes do not depend on coherence and take
-// the middle of the training range.
+// the network was trained on (clamped near the top of its range, where the factor is 1 to 1e-6); the
+// other two families' lobes do not depend on coherence and take the middle of the range.
 template <int N>
 PLT_DEV void sur_query(const Mat& m, int fam, const Frame& sd, V3 V, V3 woW, uint32_t lobe, const Beam<N>& beam, const SolveBuffers& sb,
                        size_t e, uint32_t k) {
@@ -460,7 +460,7 @@ PLT_DEV void sur_query(const Mat& m, int fam, const Frame& sd, V3 V, V3 woW, uin
     for (int j = 0; j < N; ++j) {
         if (!inside || j >= beam.comps) continue;
         const float l0 = 1e-3f * beam.lambda[j];
-        const float area = areaScale < 0.f ? 50.5f : areaScale * (l0 * l0);
+        const float area = areaScale < 0.f ? kSurAreaNeutral : fminf(areaScale * (l0 * l0), kSurAreaClamp);
         float x[kSurIn];
         sur_features(m, fam, wiS, woS, beam.lambda[j], area, x);
         float* xr = sp.X + size_t(row0 + j) * kSurIn;

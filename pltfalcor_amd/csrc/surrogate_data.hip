@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(kSBlock) generate_kernel(uint32_t n, uint32_t 
     const V3 wi = cosine_hemisphere(u_wi0, u_wi1);
     const V3 wo = cosine_hemisphere(u_wo0, u_wo1);
     const float lambda = 380.f + 340.f * u_lam;
-    const float area = 1.f + 99.f * u_area;
+    const float area = expf(2.f * kSurLogAreaMid * u_area);  // log-uniform over [1, 1e8] um^2
     mats[i] = pm;
     const size_t N = n;
     geo[PLT_SG_WI_X * N + i] = wi.x; geo[PLT_SG_WI_Y * N + i] = wi.y; geo[PLT_SG_WI_Z * N + i] = wi.z;

@@ -16,6 +16,12 @@ namespace plt {
 
 enum SurFamily : int { kSurCoatedOpaque = 0, kSurCoatedConductor = 1, kSurGrated = 2, kSurFamilies = 3 };
 constexpr int kSurIn = PLT_SURROGATE_INPUTS, kSurOut = PLT_SURROGATE_OUTPUTS;
+// Coherence areas (um^2) the network is trained on, log-uniform over [1, 1e8]: from beams coherent
+// over a micrometre to beams coherent over centimetres. The render clamps larger areas to
+// kSurAreaClamp, inside the trained range (whose sampled maximum falls just short of 1e8), where the
+// coherence factor of a coat of at most 5 um is already 1 to within 1e-6. The input is ln(area)
+// scaled to [-1, 1]; kSurAreaNeutral is the value for lobes that do not depend on it.
+constexpr float kSurLogAreaMid = 9.21034037f, kSurAreaClamp = 5e7f, kSurAreaNeutral = 1e4f;
 
 // The family whose evaluation of `lobe` the surrogate stands for, or -1.
 PLT_DEV int sur_family(const Mat& m, uint32_t lobe) {
@@ -34,7 +40,7 @@ PLT_DEV int sur_family(const Mat& m, uint32_t lobe) {
 
 // The 22 inputs for directions wi (towards the light) and wo (towards the eye) in the shading
 // frame, one wavelength (nm) and the beam's coherence area (um^2): wi, wo, wavelength, roughness q
-// and correlation length sqrt(T) of the K-correlation surface, coherence area, the family one-hot,
+// and correlation length sqrt(T) of the K-correlation surface, log coherence area, the family one-hot,
 // four features of the K-correlation lobe (log of its PSD argument k l |wi_t + wo_t|, of k q, of
 // k l and of |wi_t + wo_t|) and the family's decoded (binary16-rounded) parameters, each scaled to
 // about [-1, 1] over the training ranges.
@@ -61,7 +67,7 @@ PLT_DEV void sur_features(const Mat& m, int fam, V3 wi, V3 wo, float lambda, flo
     x[6] = (lambda - 550.f) / 170.f;
     x[7] = (q - .325f) / .275f;
     x[8] = (l - .46f) / .41f;
-    x[9] = (area - 50.5f) / 49.5f;
+    x[9] = (logf(area) - kSurLogAreaMid) / kSurLogAreaMid;
 #pragma unroll
     for (int f = 0; f < kSurFamilies; ++f) x[10 + f] = f == fam ? 1.f : 0.f;
 #pragma unroll

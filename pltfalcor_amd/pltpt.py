@@ -184,6 +184,14 @@ class PLTPT:
         _lib.check(self.lib.plt_pass_get_surrogate_stats(self.handle, C.byref(r), C.byref(x)))
         return {"rows": r.value, "exact_steps": x.value}
 
+    def surrogate_queries(self, max_rows: int = 1 << 22):
+        """The last execute's surrogate query rows [rows, 22] (numpy float32)."""
+        import numpy as np
+        out = np.zeros((max_rows, 22), np.float32)
+        n = C.c_uint64()
+        _lib.check(self.lib.plt_pass_copy_surrogate_queries(self.handle, C.c_void_p(out.ctypes.data), max_rows, C.byref(n)))
+        return out[:n.value]
+
     def solve_stats(self) -> dict:
         """Counts of the last execute's split solve pass (waits for the device): light events, stored
         walk vertices, replayed vertices (sum of vertex index + 1 over events), overflowed paths."""
