@@ -59,6 +59,10 @@ SOLVE_BYTES_PER_SEGMENT, SOLVE_BYTES_PER_PIXEL = 36, 16
 # vertex 7 words (+ 7 words of walk origin / direction / base per event that has vertices, folded
 # in here), and a region record 16 B per 64 paths
 EVENT_BYTES, WALK_VERTEX_BYTES, REGION_BYTES = 80, 28, 16
+# a deferred manifold walk: its queue entry, the 12 event words it starts from (meta, vertex base,
+# origin, direction, position, wavelength) and the 3 words of its last direction; its vertices are
+# read once and written back once (2 x WALK_VERTEX_BYTES each)
+WALK_BYTES = 4 + 48 + 12
 
 
 def parse_args():
@@ -610,7 +614,7 @@ def main():
         step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    sample_ms, light_ms, unwind_ms, mega_ms, executes = pltpt.kernel_times_split()
+    sample_ms, light_ms, walk_ms, unwind_ms, mega_ms, executes = pltpt.kernel_times_split()
     pltpt.set_profiling(False)
 
     segs, sstats = segment_count(pltpt, range(SPP), split)  # outside the timed region
@@ -618,28 +622,31 @@ def main():
     regions = (pixels + 63) // 64
     seg_per_exec = sum(segs) / SPP
     if split and not launch["graph"]:
-        avg = {k: v / executes for k, v in (("sample", sample_ms), ("light", light_ms), ("unwind", unwind_ms),
-                                            ("megakernel", mega_ms))}
+        avg = {k: v / executes for k, v in (("sample", sample_ms), ("light", light_ms), ("walk", walk_ms),
+                                            ("unwind", unwind_ms), ("megakernel", mega_ms))}
     else:  # one solve launch sequence, timed as a whole
-        avg = {"sample": sample_ms / executes, "megakernel": (light_ms + unwind_ms + mega_ms) / executes}
+        avg = {"sample": sample_ms / executes, "megakernel": (light_ms + walk_ms + unwind_ms + mega_ms) / executes}
     per_frame = {k: sum(st[k] for st in sstats) / SPP for k in ("events", "walk_vertices", "event_vertices",
-                                                                 "overflowed_paths")} if split else None
+                                                                 "overflowed_paths", "walks", "failed_walks",
+                                                                 "walk_rays")} if split else None
     # algorithmic bytes per launch of each kernel
     kb = {"sample": SAMPLE_BYTES_PER_SEGMENT * seg_per_exec + SAMPLE_BYTES_PER_PIXEL * pixels}
     if split:
         ev_b = EVENT_BYTES * per_frame["events"] + WALK_VERTEX_BYTES * per_frame["walk_vertices"] + REGION_BYTES * regions
         kb["light"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + ev_b
+        kb["walk"] = WALK_BYTES * per_frame["walks"] + 2 * WALK_VERTEX_BYTES * per_frame["walk_vertices"]
         kb["unwind"] = ev_b + SOLVE_BYTES_PER_SEGMENT * per_frame["event_vertices"] + SOLVE_BYTES_PER_PIXEL * pixels
         kb["megakernel"] = REGION_BYTES * regions  # the fallback: its region check, plus overflowed paths' work
     else:
         kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
-    knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>", "unwind": "solve_unwind_kernel<4>",
+    knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>", "walk": "solve_walk_kernel",
+              "unwind": "solve_unwind_kernel<4>",
               "megakernel": "solve_kernel<4, false>" + (" (fallback of the split pass)" if split and not launch["graph"] else "")}
     if split and launch["graph"]:
-        knames["megakernel"] = "split solve pass replayed from a graph (light + unwind + fallback)"
+        knames["megakernel"] = "split solve pass replayed from a graph (light + walk + unwind + fallback)"
         kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
     profiles = {"sample": "pltpt_sample_1080p_d8_pmc.md", "light": "pltpt_solve_light_1080p_d8_pmc.md",
-                "unwind": "pltpt_solve_unwind_1080p_d8_pmc.md",
+                "walk": None, "unwind": "pltpt_solve_unwind_1080p_d8_pmc.md",
                 "megakernel": None if split else "pltpt_solve_1080p_d8_pmc.md"}
     kdom = max(avg, key=lambda k: avg[k])
     kname, kms, kbytes = knames[kdom], avg[kdom], kb[kdom]
@@ -687,6 +694,7 @@ def main():
                      "kernels": {knames[k]: {"ms": avg[k], "algorithmic_bytes": kb[k],
                                              "GB/s": kb[k] / (avg[k] * 1e-3) / 1e9 if avg[k] > 0 else None}
                                  for k in avg},
+                     "walk_rays_per_s": per_frame["walk_rays"] / (avg["walk"] * 1e-3) if split and avg.get("walk") else None,
                      "sample_kernel_ms": avg["sample"], "solve_pass_ms": sum(v for k, v in avg.items() if k != "sample")},
         "launch": {"config": launch, "autotune": tune, "split_solve_per_frame": per_frame},
         "cpu_baseline": None,

@@ -446,11 +446,12 @@ int plt_pass_set_schedule(plt_pass* pass, int schedule);
 /* Schedule of the solve pass (no counterpart in the reference, whose solve pass is one DXR ray
  * generation shader): PLT_SOLVE_MEGAKERNEL runs one kernel in which each lane replays its path,
  * samples lights and carries every sourced beam to the camera; PLT_SOLVE_SPLIT runs a
- * light-sampling kernel (light samples, shadow rays, manifold walks; it appends one light event per
- * beam) and an unwind kernel (each event's beam through the stored vertices, then per-path sums in
- * the megakernel's order), then the megakernel for the paths whose events did not fit the event
- * area (solve_events_per_path events and solve_hits_per_path manifold-walk vertices per path, pooled
- * over regions of 64 paths). Both write identical colours. */
+ * light-sampling kernel (light samples, shadow rays, the random part of manifold walks; it appends
+ * one light event per beam), a manifold-walk kernel (the walks' Newton iterations and visibility,
+ * from a queue), and an unwind kernel (each event's beam through the stored vertices, then per-path
+ * sums in the megakernel's order), then the megakernel for the paths whose events did not fit the
+ * event area (solve_events_per_path events and solve_hits_per_path manifold-walk vertices per path,
+ * pooled over regions of 64 paths; defaults 8 and 4). Both write identical colours. */
 enum { PLT_SOLVE_MEGAKERNEL = 0, PLT_SOLVE_SPLIT = 1 };
 
 /* Launch configuration of the pass (no counterpart in the reference, whose two DXR dispatches per
@@ -491,18 +492,20 @@ int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t fram
  * the sums. Enabling or disabling restarts them too. */
 int plt_pass_set_profiling(plt_pass* pass, int enable);
 int plt_pass_get_kernel_times(plt_pass* pass, double* sample_ms, double* solve_ms, uint32_t* executes);
-/* The same sums with the solve pass resolved into its kernels: the split pass's light-sampling and
- * unwind kernels and the solve megakernel (the whole solve pass of the megakernel schedule, the
- * split pass's fallback launch, and the whole split pass when it is replayed from a graph, whose
- * launches are not timed one by one). */
-int plt_pass_get_kernel_times_split(plt_pass* pass, double* sample_ms, double* light_ms, double* unwind_ms,
+/* The same sums with the solve pass resolved into its kernels: the split pass's light-sampling,
+ * manifold-walk and unwind kernels and the solve megakernel (the whole solve pass of the megakernel
+ * schedule, the split pass's fallback launch, and the whole split pass when it is replayed from a
+ * graph, whose launches are not timed one by one). */
+int plt_pass_get_kernel_times_split(plt_pass* pass, double* sample_ms, double* light_ms, double* walk_ms, double* unwind_ms,
                                     double* megakernel_ms, uint32_t* executes);
 /* Counts of the last execute's split solve pass, read back from the device (waits for the device):
  * light events, manifold-walk vertices stored with them, the sum over events of (vertex index + 1)
- * -- the stored vertices the unwind kernel replays for them -- and the paths left to the megakernel
- * because their region's event area was full. PLT_ESTATE if the last execute used the megakernel. */
+ * -- the stored vertices the unwind kernel replays for them -- the paths left to the megakernel
+ * because their region's event area was full, the manifold walks the walk kernel ran, those of
+ * them that failed (their events add nothing) and the rays the walk kernel cast (line-search steps
+ * and visibility). PLT_ESTATE if the last execute used the megakernel. */
 int plt_pass_get_solve_stats(plt_pass* pass, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,
-                             uint64_t* overflowed_paths);
+                             uint64_t* overflowed_paths, uint64_t* walks, uint64_t* failed_walks, uint64_t* walk_rays);
 
 /* ---------------------------------------------------------------------------------------
  * Learned BSDF surrogate. No counterpart in the reference: a small MLP (22 inputs, three hidden

@@ -160,8 +160,10 @@ SIGNATURES = [
     ("plt_pass_get_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_uint32)]),
     ("plt_pass_get_kernel_times_split", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+                                                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                                  C.POINTER(C.c_uint32)]),
     ("plt_pass_get_solve_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("plt_pass_set_surrogate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
     ("plt_scene_set_camera", C.c_int, [C.c_void_p, C.POINTER(Camera)]),

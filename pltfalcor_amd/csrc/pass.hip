@@ -173,8 +173,8 @@ struct plt_pass {
     float cam[12] = {};
     float view_rot[9] = {};
     // kernel timing: kEventsPerExecute events per execute: before the sample pass, after it, after the
-    // split pass's light-sampling and unwind kernels (both at the solve pass's start otherwise), after
-    // the solve pass
+    // split pass's light-sampling, walk and unwind kernels (all three at the solve pass's start
+    // otherwise), after the solve pass
     bool profiling = false;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
@@ -657,8 +657,10 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         const size_t regions = (npaths + plt::kSolveRegion - 1) / plt::kSolveRegion;
         const size_t evCap = regions * plt::kSolveRegion * p->launch.solve_events_per_path;
         const size_t hitCap = regions * plt::kSolveRegion * p->launch.solve_hits_per_path;
+        if (evCap > size_t(UINT32_MAX)) return set_error(PLT_EINVAL, "plt_pass_execute: frame too large for the split solve pass's event area");
         const size_t evBytes = evCap * plt::kEvWords * 4, hitBytes = hitCap * plt::kHitWords * 4;
-        const size_t bytes = evBytes + hitBytes + regions * plt::kRegWords * 4 + 256;
+        const size_t regBytes = regions * plt::kRegWords * 4, walkBytes = evCap * 4;
+        const size_t bytes = evBytes + hitBytes + regBytes + walkBytes + 256;
         if (bytes > p->solve_bytes) {
             if (p->d_solve) (void)hipFree(p->d_solve);
             p->d_solve = nullptr;
@@ -672,6 +674,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         sbuf.ev = reinterpret_cast<uint32_t*>(b);
         sbuf.hits = reinterpret_cast<uint32_t*>(b + evBytes);
         sbuf.region = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes);
+        sbuf.walk.items = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes + regBytes);
+        sbuf.walk.counters = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes + regBytes + walkBytes);
         sbuf.ev_per_path = p->launch.solve_events_per_path;
         sbuf.hits_per_path = p->launch.solve_hits_per_path;
         sbuf.ev_cap = evCap;
@@ -734,7 +738,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     const bool split_marks = split && !p->graph && ev[2];
     auto solve_seq = [&](hipStream_t s) -> int {
         return split ? plt::launch_solve_pass_split(sc, fr, sbuf, s, split_marks ? ev[2] : nullptr, split_marks ? ev[3] : nullptr,
-                                                    surrogate)
+                                                    split_marks ? ev[4] : nullptr, surrogate)
                      : plt::launch_solve_pass(sc, fr, s);
     };
     int rc;
@@ -744,6 +748,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         if (ev[2] && !split_marks) {
             (void)hipEventRecord(ev[2], st);
             (void)hipEventRecord(ev[3], st);
+            (void)hipEventRecord(ev[4], st);
         }
     };
     if (p->graph) {
@@ -757,7 +762,7 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         mark_solve_start();
         if ((rc = solve_seq(st)) < 0) return rc;
     }
-    if (ev[4]) (void)hipEventRecord(ev[4], st);
+    if (ev[5]) (void)hipEventRecord(ev[5], st);
     ++p->frame;  // PLTPT.cpp:498
     p->last_words = words;
     return PLT_OK;
@@ -835,7 +840,7 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
         plt_launch_config c = bc;
         c.solve_schedule = PLT_SOLVE_MEGAKERNEL;
         solve.push_back(c);
-        for (uint32_t e : {2u, 4u, 8u}) {
+        for (uint32_t e : {4u, 8u, 16u}) {
             c.solve_schedule = PLT_SOLVE_SPLIT;
             c.solve_events_per_path = e;
             c.solve_hits_per_path = e / 2;
@@ -903,35 +908,43 @@ int plt_pass_copy_surrogate_queries(plt_pass* p, float* dst, uint64_t max_rows, 
 }
 
 int plt_pass_get_solve_stats(plt_pass* p, uint64_t* events, uint64_t* walk_vertices, uint64_t* event_vertices,
-                              uint64_t* overflowed_paths) {
-    if (!p || !events || !walk_vertices || !event_vertices || !overflowed_paths)
+                              uint64_t* overflowed_paths, uint64_t* walks, uint64_t* failed_walks, uint64_t* walk_rays) {
+    if (!p || !events || !walk_vertices || !event_vertices || !overflowed_paths || !walks || !failed_walks || !walk_rays)
         return set_error(PLT_EINVAL, "plt_pass_get_solve_stats: bad arguments");
     const plt::SolveBuffers& sb = p->last_solve;
     if (!sb.ev || p->last_regions == 0) return set_error(PLT_ESTATE, "plt_pass_get_solve_stats: no split solve pass rendered yet");
     std::vector<uint32_t> reg(p->last_regions * plt::kRegWords), meta(sb.ev_cap);
+    uint32_t wc[3] = {0u, 0u, 0u};
     if (hipDeviceSynchronize() != hipSuccess ||
         hipMemcpy(reg.data(), sb.region, reg.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(meta.data(), sb.ev + size_t(plt::kEvMeta) * sb.ev_cap, meta.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(meta.data(), sb.ev + size_t(plt::kEvMeta) * sb.ev_cap, meta.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(wc, sb.walk.counters, sizeof(wc), hipMemcpyDeviceToHost) != hipSuccess)
         return set_error(PLT_EHIP, "plt_pass_get_solve_stats: read-back failed");
-    uint64_t ne = 0, nh = 0, nv = 0, no = 0;
+    uint64_t ne = 0, nh = 0, nv = 0, no = 0, nf = 0;
     for (size_t r = 0; r < p->last_regions; ++r) {
         const uint32_t* rg = &reg[r * plt::kRegWords];
         ne += rg[plt::kRegEvents];
         nh += rg[plt::kRegHits];
         no += uint64_t(__builtin_popcount(rg[plt::kRegOverflowLo])) + uint64_t(__builtin_popcount(rg[plt::kRegOverflowHi]));
         const size_t base = r * plt::kSolveRegion * sb.ev_per_path;
-        for (uint32_t k = 0; k < rg[plt::kRegEvents]; ++k) nv += ((meta[base + k] >> 6) & 127u) + 1u;
+        for (uint32_t k = 0; k < rg[plt::kRegEvents]; ++k) {
+            nv += ((meta[base + k] >> 6) & 127u) + 1u;
+            nf += ((meta[base + k] >> 13) & 3u) == plt::kEvKindVoid;
+        }
     }
     *events = ne;
     *walk_vertices = nh;
     *event_vertices = nv;
     *overflowed_paths = no;
+    *walks = wc[0];
+    *failed_walks = nf;
+    *walk_rays = wc[2];
     return PLT_OK;
 }
 
-int plt_pass_get_kernel_times_split(plt_pass* p, double* sample_ms, double* light_ms, double* unwind_ms, double* megakernel_ms,
-                                    uint32_t* executes) {
-    if (!p || !sample_ms || !light_ms || !unwind_ms || !megakernel_ms || !executes)
+int plt_pass_get_kernel_times_split(plt_pass* p, double* sample_ms, double* light_ms, double* walk_ms, double* unwind_ms,
+                                    double* megakernel_ms, uint32_t* executes) {
+    if (!p || !sample_ms || !light_ms || !walk_ms || !unwind_ms || !megakernel_ms || !executes)
         return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
     constexpr size_t kEv = plt::kEventsPerExecute;
     double sums[kEv - 1] = {};
@@ -948,8 +961,9 @@ int plt_pass_get_kernel_times_split(plt_pass* p, double* sample_ms, double* ligh
     }
     *sample_ms = sums[0];
     *light_ms = sums[1];
-    *unwind_ms = sums[2];
-    *megakernel_ms = sums[3];
+    *walk_ms = sums[2];
+    *unwind_ms = sums[3];
+    *megakernel_ms = sums[4];
     *executes = uint32_t(n);
     p->events_used = 0;
     return PLT_OK;
@@ -957,10 +971,10 @@ int plt_pass_get_kernel_times_split(plt_pass* p, double* sample_ms, double* ligh
 
 int plt_pass_get_kernel_times(plt_pass* p, double* sample_ms, double* solve_ms, uint32_t* executes) {
     if (!p || !sample_ms || !solve_ms || !executes) return set_error(PLT_EINVAL, "plt_pass_get_kernel_times: bad arguments");
-    double light = 0.0, unwind = 0.0, mega = 0.0;
-    const int rc = plt_pass_get_kernel_times_split(p, sample_ms, &light, &unwind, &mega, executes);
+    double light = 0.0, walk = 0.0, unwind = 0.0, mega = 0.0;
+    const int rc = plt_pass_get_kernel_times_split(p, sample_ms, &light, &walk, &unwind, &mega, executes);
     if (rc < 0) return rc;
-    *solve_ms = light + unwind + mega;
+    *solve_ms = light + walk + unwind + mega;
     return PLT_OK;
 }
 

@@ -225,6 +225,8 @@ int launch_solve_sample_4(const PtScene& sc, const PtFrame& fr, const SolveBuffe
 int launch_solve_sample_4_lights(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_solve_sample_1(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, bool lights);
 int launch_solve_unwind(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
+int launch_walk_clear(const SolveBuffers& sb, hipStream_t st);
+int launch_solve_walks(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_solve_unwind_surrogate(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, plt_surrogate* s, hipStream_t st);
 
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
@@ -239,19 +241,26 @@ int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
 // Split solve pass: light sampling, unwind, then the megakernel over the paths whose region's event
 // area overflowed (its other blocks return at once). Three launches on one stream, no host waits.
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, hipEvent_t after_light,
-                            hipEvent_t after_unwind, plt_surrogate* surrogate) {
+                            hipEvent_t after_walk, hipEvent_t after_unwind, plt_surrogate* surrogate) {
     if (fr.n_paths == 0) {
         if (after_light) (void)hipEventRecord(after_light, st);
+        if (after_walk) (void)hipEventRecord(after_walk, st);
         if (after_unwind) (void)hipEventRecord(after_unwind, st);
         return PLT_OK;
     }
     if (fr.hwss != 4 && fr.hwss != 1) return set_error(PLT_ENOTIMPL, "HWSS sample counts other than 1 and 4 are not built");
     const bool analytic = ((fr.options & kOptAnalyticLights) != 0 && sc.n_analytic > 0) ||
                           ((fr.options & kOptEnvLight) != 0 && sc.env_w > 0);
-    int rc = fr.hwss == 4 ? (analytic ? launch_solve_sample_4_lights(sc, fr, sb, st) : launch_solve_sample_4(sc, fr, sb, st))
-                          : launch_solve_sample_1(sc, fr, sb, st, analytic);
+    int rc = launch_walk_clear(sb, st);
+    if (rc < 0) return rc;
+    rc = fr.hwss == 4 ? (analytic ? launch_solve_sample_4_lights(sc, fr, sb, st) : launch_solve_sample_4(sc, fr, sb, st))
+                      : launch_solve_sample_1(sc, fr, sb, st, analytic);
     if (rc < 0) return rc;
     if (after_light) (void)hipEventRecord(after_light, st);
+    // manifold walks: only frames with manifold NEE queue any, but the kernel is cheap to launch
+    // empty and a captured graph then holds the same launches whatever the options
+    if ((rc = launch_solve_walks(sc, fr, sb, st)) < 0) return rc;
+    if (after_walk) (void)hipEventRecord(after_walk, st);
     if ((rc = surrogate ? launch_solve_unwind_surrogate(sc, fr, sb, surrogate, st) : launch_solve_unwind(sc, fr, sb, st)) < 0) return rc;
     if (after_unwind) (void)hipEventRecord(after_unwind, st);
     PtFrame fb = fr;

@@ -112,14 +112,15 @@ enum EvField {
     kEvHitBase = kEvOrg + 3,  // manifold NEE: first walk vertex in the region's vertex area
     kEvWords
 };
-enum : uint32_t { kEvKindNee = 0, kEvKindEmissive = 1, kEvKindMiss = 2 };
+// kEvKindVoid: a manifold NEE event whose deferred walk failed; it adds nothing to its path.
+enum : uint32_t { kEvKindNee = 0, kEvKindEmissive = 1, kEvKindMiss = 2, kEvKindVoid = 3 };
 enum : uint32_t {
     kEvBitComps1 = 1u << 15, kEvBitDistant = 1u << 16, kEvBitMnee = 1u << 17, kEvBitMisL = 1u << 18,
     kEvBitCurrentMs = 1u << 19, kEvBitDispersive = 1u << 20
 };
 enum HitField { kHitTri = 0, kHitB1, kHitB2, kHitPos, kHitLobe = kHitPos + 3, kHitWords };
 constexpr uint32_t kSolveRegion = 64;  // paths per region: one wave of the split solve kernels
-constexpr size_t kEventsPerExecute = 5;  // kernel-timing events per execute of the pass
+constexpr size_t kEventsPerExecute = 6;  // kernel-timing events per execute of the pass
 // per region: event count, walk-vertex count, overflow mask (2 words: bit = lane whose path is left
 // to the megakernel)
 enum { kRegEvents = 0, kRegHits, kRegOverflowLo, kRegOverflowHi, kRegWords };
@@ -138,12 +139,22 @@ struct SurPath {
     uint32_t teacher;    // 1: the query pass writes the exact lobe's matrix as the outputs (no network)
 };
 enum : int { kUnwindExact = 0, kUnwindQuery = 1, kUnwindApply = 2 };
+// Deferred manifold walks: the light-sampling kernel stores a walk's starting vertices (everything
+// that draws random numbers: the current vertex's and the occluders' delta-lobe samples) with its
+// event and appends the event's index here; the walk kernel, a persistent grid whose lanes each take
+// the next walk as soon as theirs ends, runs the Newton iterations and the light's visibility, writes
+// the converged vertices and the last direction into the event, or turns it into kEvKindVoid.
+struct WalkQueue {
+    uint32_t* items;     // event indices, ev_cap entries
+    uint32_t* counters;  // [0] walks appended this frame, [1] walks taken by the walk kernel, [2] rays cast
+};
 struct SolveBuffers {
     uint32_t* ev;
     uint32_t* hits;
     uint32_t* region;
     uint32_t ev_per_path, hits_per_path;
     size_t ev_cap, hit_cap;  // plane lengths: regions * 64 * per-path capacity
+    WalkQueue walk;
     SurPath sur;             // used by the query and apply unwind modes only
 };
 
@@ -170,17 +181,17 @@ struct LaunchConfig {
     uint32_t sample_block = 64;
     uint32_t wf_blocks_per_cu = 16;
     uint32_t solve_schedule = 0;  // PLT_SOLVE_*
-    uint32_t solve_events_per_path = 4;
-    uint32_t solve_hits_per_path = 2;
+    uint32_t solve_events_per_path = 8;
+    uint32_t solve_hits_per_path = 4;
 };
 
 int launch_sample_pass(const PtScene& sc, const PtFrame& fr, const LaunchConfig& lc, hipStream_t st);
 int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, const LaunchConfig& lc, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
-// after_light / after_unwind (may be NULL): events recorded on `st` after the light-sampling and the
-// unwind kernels (kernel timing outside graph capture)
+// after_light / after_walk / after_unwind (may be NULL): events recorded on `st` after the
+// light-sampling, the walk and the unwind kernels (kernel timing outside graph capture)
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st,
-                            hipEvent_t after_light = nullptr, hipEvent_t after_unwind = nullptr,
+                            hipEvent_t after_light = nullptr, hipEvent_t after_walk = nullptr, hipEvent_t after_unwind = nullptr,
                             plt_surrogate* surrogate = nullptr);
 int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 

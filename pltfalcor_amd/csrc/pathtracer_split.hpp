@@ -6,8 +6,11 @@
 // but instead of carrying each sourced beam through the stored vertices it appends a light event:
 // what the beam is (spectral radiance, wavelengths, position, direction, size), the scale factors
 // the megakernel applies to it in order (type selection, light pdf, walk pdf) and, for a manifold
-// walk, its vertices. It holds the traversal stack and the walk state but no beam and no BSDF
-// evaluation.
+// walk, its vertices. Of a manifold walk it runs only the part that draws random numbers (the
+// delta-lobe samples at the current vertex and at the occluders of the shadow ray) and queues the
+// event for the walk kernel (pathtracer_walk.hip), which runs the Newton iterations and the light's
+// visibility and completes the event or voids it. It holds the traversal stack but no beam and no
+// BSDF evaluation.
 //
 // solve_unwind_kernel rebuilds each event's beam and runs the rest of the megakernel's sequence
 // for it -- MIS weight, walk unwind (manifoldForwardUnwind, :80-93), evalNEE (:219-243),
@@ -50,6 +53,18 @@ PLT_DEV void ev_put_source(const SolveBuffers& sb, size_t e, const float* Le, co
     ev_put3(sb, e, kEvDir, dir);
     ev_putf(sb, e, kEvSize, size);
     ev_putf(sb, e, kEvDist, dist);
+}
+
+// `rows` consecutive entries of a counted area for each active lane, one atomic per wave.
+PLT_DEV uint32_t wave_append(uint32_t* counter, uint32_t rows) {
+    const uint64_t act = __ballot(1);
+    const int lane = int(__lane_id());
+    const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
+    const uint32_t rank = uint32_t(__popcll(act & ((uint64_t(1) << lane) - 1u)));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, rows * uint32_t(__popcll(act)));
+    base = __shfl(base, leader);
+    return base + rank * rows;
 }
 
 template <int N, bool Lights>
@@ -297,8 +312,10 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
                                 ok = ms_sample_vertex(bounce.tri, bounce.b1, bounce.b2, s.pos, s, m, bounce.wi, lightDir, sg, ms) &&
                                      (ms.hits[0].lobe & kLobeReflection) != 0;
                             origin = !currentVertexMS ? s.pos : prevPos;
-                            ok = ok && ms_nee_sample(sc, fr, lsOrigin, lsDir, lsDist, origin, bDistant ? v3(0.f, 0.f, 0.f) : bPos,
-                                                     bDistant, lightDir, maxMSBounces, sg, ms, stk);
+                            // the walk's random part (manifoldNEESample up to its Newton iterations,
+                            // Manifolds.slangh:165-182); the iterations and the light's visibility draw
+                            // no random numbers and are left to the walk kernel
+                            ok = ok && ms_trace_occluders(sc, lsOrigin, lsDir, lsDist, int(fr.mnee_max_occluders), maxMSBounces, sg, ms, stk);
                             ok = ok && !(ms.pdf < 1e-6f);
                         }
                         if (ok) {
@@ -324,7 +341,6 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
                             if (mnee) {
                                 ev_putf(sb, e, kEvMsPdf, ms.pdf);
                                 if (ms.count > 0) {
-                                    ev_put3(sb, e, kEvWo, ms.woLast);
                                     ev_put3(sb, e, kEvOrg, origin);
                                     ev_put(sb, e, kEvHitBase, hk);
                                     const size_t hb = size_t(region) * kPtBlock * sb.hits_per_path + hk;
@@ -338,6 +354,7 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
                                         hp[(kHitPos + 2) * sb.hit_cap] = __float_as_uint(ms.hits[k].pos.z);
                                         hp[kHitLobe * sb.hit_cap] = ms.hits[k].lobe;
                                     }
+                                    sb.walk.items[wave_append(sb.walk.counters, 1u)] = uint32_t(e);
                                 }
                             }
                         }
@@ -402,17 +419,6 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
 // ---------------------------------------------------------------------------------------
 // The learned surrogate in the unwind (query and apply modes).
 // ---------------------------------------------------------------------------------------
-// `rows` consecutive rows of the frame's query area for each active lane, one atomic per wave.
-PLT_DEV uint32_t sur_append_rows(uint32_t* counter, uint32_t rows) {
-    const uint64_t act = __ballot(1);
-    const int lane = int(__lane_id());
-    const int leader = __ffsll(static_cast<unsigned long long>(act)) - 1;
-    const uint32_t rank = uint32_t(__popcll(act & ((uint64_t(1) << lane) - 1u)));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, rows * uint32_t(__popcll(act)));
-    base = __shfl(base, leader);
-    return base + rank * rows;
-}
 // Adds the number of active lanes with `pred` to *counter, one atomic per wave.
 PLT_DEV void sur_count(uint32_t* counter, bool pred) {
     const uint64_t m = __ballot(pred);
@@ -454,7 +460,7 @@ PLT_DEV void sur_query(const Mat& m, int fam, const Frame& sd, V3 V, V3 woW, uin
             areaScale = q > 0.f ? kPi * zz / q : 1e30f;
         }
     }
-    const uint32_t row0 = sur_append_rows(sp.counter, uint32_t(N));
+    const uint32_t row0 = wave_append(sp.counter, uint32_t(N));
     bool inside = row0 + uint32_t(N) <= sp.qcap;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -690,7 +696,7 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
             const size_t e = evBase + k;
             const uint32_t meta = ev_get(sb, e, kEvMeta);
             who = meta & 63u;
-            if ((ovf >> who) & 1u) who = kPtBlock;  // left to the megakernel
+            if (((ovf >> who) & 1u) || ((meta >> 13) & 3u) == kEvKindVoid) who = kPtBlock;  // left to the megakernel, or void
             else unwind_event<N, Mode>(sc, fr, sb, e, meta, region, c);
         }
         if (Mode == kUnwindQuery) continue;

@@ -102,10 +102,12 @@ __device__ __forceinline__ float box_entry(float lox, float hix, float loy, floa
     return tn <= tf * 1.0000005f ? tn : __builtin_inff();
 }
 
-// Traverses the BVH. kAnyHit: stop at the first hit in [tmin, tmax] (visibility rays).
-// `stack` points at this lane's first LDS slot; consecutive slots are `stride` ints apart.
-template <bool kAnyHit, int kDepth>
-__device__ __forceinline__ Hit traverse(const TraceScene& sc, const Ray& r, int* stack, int stride) {
+// Traverses the BVH. anyHit: stop at the first hit in [tmin, tmax] (visibility rays); a per-lane
+// value lets one call serve a wave whose lanes cast both kinds (whether a ray hits anything in
+// [tmin, tmax] does not depend on it). `stack` points at this lane's first LDS slot; consecutive
+// slots are `stride` ints apart.
+template <int kDepth>
+__device__ __forceinline__ Hit traverse_rays(const TraceScene& sc, const Ray& r, int* stack, int stride, bool anyHit) {
     Hit h{kNoHit, 0.f, 0.f, r.tmax};
     const RayPrep p = prep_ray(r);
     const float dx = fabsf(r.dx) > 1e-30f ? r.dx : copysignf(1e-30f, r.dx);
@@ -146,7 +148,7 @@ __device__ __forceinline__ Hit traverse(const TraceScene& sc, const Ray& r, int*
                     if (t < best || id < h.tri) {
                         best = t;
                         h.tri = id; h.b1 = b1; h.b2 = b2; h.t = t;
-                        if (kAnyHit) return h;
+                        if (anyHit) return h;
                     }
                 }
             }
@@ -155,6 +157,10 @@ __device__ __forceinline__ Hit traverse(const TraceScene& sc, const Ray& r, int*
         node = stack[(--sp) * stride];
     }
     return h;
+}
+template <bool kAnyHit, int kDepth>
+__device__ __forceinline__ Hit traverse(const TraceScene& sc, const Ray& r, int* stack, int stride) {
+    return traverse_rays<kDepth>(sc, r, stack, stride, kAnyHit);
 }
 
 }  // namespace plt

@@ -162,9 +162,10 @@ class PLTPT:
         return a.value, b.value, n.value
 
     def kernel_times_split(self):
-        """(sample_ms, light_ms, unwind_ms, megakernel_ms, executes): the solve pass resolved into the
-        split pass's light-sampling and unwind kernels and the solve megakernel (see the C ABI)."""
-        v = [C.c_double() for _ in range(4)]
+        """(sample_ms, light_ms, walk_ms, unwind_ms, megakernel_ms, executes): the solve pass resolved
+        into the split pass's light-sampling, manifold-walk and unwind kernels and the solve megakernel
+        (see the C ABI)."""
+        v = [C.c_double() for _ in range(5)]
         n = C.c_uint32()
         _lib.check(self.lib.plt_pass_get_kernel_times_split(self.handle, *(C.byref(x) for x in v), C.byref(n)))
         return (*(x.value for x in v), n.value)
@@ -194,10 +195,12 @@ class PLTPT:
 
     def solve_stats(self) -> dict:
         """Counts of the last execute's split solve pass (waits for the device): light events, stored
-        walk vertices, replayed vertices (sum of vertex index + 1 over events), overflowed paths."""
-        v = [C.c_uint64() for _ in range(4)]
+        walk vertices, replayed vertices (sum of vertex index + 1 over events), overflowed paths,
+        manifold walks run by the walk kernel, those that failed, and the rays the walk kernel cast."""
+        v = [C.c_uint64() for _ in range(7)]
         _lib.check(self.lib.plt_pass_get_solve_stats(self.handle, *(C.byref(x) for x in v)))
-        return dict(zip(("events", "walk_vertices", "event_vertices", "overflowed_paths"), (x.value for x in v)))
+        keys = ("events", "walk_vertices", "event_vertices", "overflowed_paths", "walks", "failed_walks", "walk_rays")
+        return dict(zip(keys, (x.value for x in v)))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -1147,13 +1147,18 @@ def test_gpu_autotune_picks_a_config_and_restores_state(gpu):
     dict(name="no-hwss-mis", lights=False, opts={"maxBounces": 6, "HWSS": 4, "doMNEE": 1, "HWSSDoMIS": 0}),
     dict(name="tiny-pertile", lights=True, opts={"maxBounces": 5, "HWSS": 4, "doMNEE": 1, "sampleGenerator": 0,
                                                   "NEEUsePerTileSG": 1, "tileSize": 16}),
+    # walks that run out of iterations (their events turn void) and walks through three occluders
+    dict(name="mnee-3occ-it4", lights=False, opts={"maxBounces": 7, "HWSS": 4, "doMNEE": 1, "MNEEMaxOccluders": 3,
+                                                   "MNEEMaxIterations": 4}),
+    dict(name="mnee-off", lights=True, opts={"maxBounces": 6, "HWSS": 4, "doMNEE": 0}),
 ], ids=lambda c: c["name"])
 def test_gpu_split_solve_equals_megakernel(gpu, case):
-    """The split solve pass (light-sampling kernel, unwind kernel, megakernel fallback for overflowed
-    regions) writes the megakernel's colour bit for bit: on the C2 workload at 1920 x 1080 (manifold
-    NEE, Halton jitter), and on scenes that take the analytic-light / environment-map kernels, one
-    HWSS wavelength, MIS off, the tiny generator with per-tile NEE draws -- with the default event area
-    and with one so small that the fallback solves most paths."""
+    """The split solve pass (light-sampling kernel, deferred manifold walks, unwind kernel,
+    megakernel fallback for overflowed regions) writes the megakernel's colour bit for bit: on the C2
+    workload at 1920 x 1080 (manifold NEE, Halton jitter), and on scenes that take the analytic-light /
+    environment-map kernels, one HWSS wavelength, MIS off, the tiny generator with per-tile NEE draws,
+    walks cut short by a small iteration limit, manifold NEE off -- with the default event area and
+    with one so small that the fallback solves most paths."""
     import torch
     from pltfalcor_amd.bsdf import default_context
     from pltfalcor_amd.pltpt import PLTPT
@@ -1165,7 +1170,7 @@ def test_gpu_split_solve_equals_megakernel(gpu, case):
         w, h = 1920, 1080
         s = Scene(ctx, aspect=w / h, **PATH_2)
         opts = dict(REFERENCE_DEFAULTS, maxBounces=8)
-        sizes = [(4, 2)]
+        sizes = [(8, 4)]
     else:
         w, h = 96, 64
         proc = Scene(None, seed=41, sphere_count=16, subdivision=3, material_kinds=8)
@@ -1176,7 +1181,7 @@ def test_gpu_split_solve_equals_megakernel(gpu, case):
             s.set_env_map(_patchy_sky(), rotation=(0.0, 45.0, 0.0), intensity=0.5)
             s.set_lights(_all_lights())
         opts = case["opts"]
-        sizes = [(4, 2), (1, 0)]
+        sizes = [(8, 4), (1, 0)]
 
     def render(cfg, frames=2):
         p = PLTPT(ctx, opts)
@@ -1203,6 +1208,13 @@ def test_gpu_split_solve_equals_megakernel(gpu, case):
         got, st = render(dict(solve_schedule=PLTPT.SOLVE_SPLIT, solve_events_per_path=ev, solve_hits_per_path=hits))
         print(case["name"], (ev, hits), st)
         assert st["events"] > 0 and st["event_vertices"] >= st["events"]
+        assert st["failed_walks"] <= st["walks"] <= st["events"]
+        if opts.get("doMNEE", 1) and hits > 0:
+            assert st["walks"] > 0 and st["walk_vertices"] >= st["walks"]
+        else:
+            assert st["walks"] == 0 and st["walk_vertices"] == 0
+        if case["name"] == "mnee-3occ-it4" and hits > 0:
+            assert st["failed_walks"] > 0
         if ev == 1:
             assert st["overflowed_paths"] > 0  # the fallback took part
         for f, (a, b) in enumerate(zip(ref, got)):
