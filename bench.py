@@ -59,6 +59,8 @@ SOLVE_BYTES_PER_SEGMENT, SOLVE_BYTES_PER_PIXEL = 36, 16
 # vertex 7 words (+ 7 words of walk origin / direction / base per event that has vertices, folded
 # in here), and a region record 16 B per 64 paths
 EVENT_BYTES, WALK_VERTEX_BYTES, REGION_BYTES = 80, 28, 16
+# the staged unwind's sum kernel reads each event's meta word and stored RGB contribution
+SUM_BYTES_PER_EVENT = 16
 # a deferred manifold walk: its queue entry, the 12 event words it starts from (meta, vertex base,
 # origin, direction, position, wavelength) and the 3 words of its last direction; its vertices are
 # read once and written back once (2 x WALK_VERTEX_BYTES each)
@@ -634,13 +636,17 @@ def main():
     if split:
         ev_b = EVENT_BYTES * per_frame["events"] + WALK_VERTEX_BYTES * per_frame["walk_vertices"] + REGION_BYTES * regions
         kb["light"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + ev_b
-        kb["walk"] = WALK_BYTES * per_frame["walks"] + 2 * WALK_VERTEX_BYTES * per_frame["walk_vertices"]
-        kb["unwind"] = ev_b + SOLVE_BYTES_PER_SEGMENT * per_frame["event_vertices"] + SOLVE_BYTES_PER_PIXEL * pixels
+        # walk kernel + the unwind of every event (the non-walk events' beside it, then the walks')
+        kb["walk"] = (WALK_BYTES * per_frame["walks"] + 2 * WALK_VERTEX_BYTES * per_frame["walk_vertices"] + ev_b
+                      + SOLVE_BYTES_PER_SEGMENT * per_frame["event_vertices"] + 12 * per_frame["events"])
+        kb["unwind"] = SUM_BYTES_PER_EVENT * per_frame["events"] + REGION_BYTES * regions + SOLVE_BYTES_PER_PIXEL * pixels
         kb["megakernel"] = REGION_BYTES * regions  # the fallback: its region check, plus overflowed paths' work
     else:
         kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
-    knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>", "walk": "solve_walk_kernel",
-              "unwind": "solve_unwind_kernel<4>",
+    knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>",
+              "walk": "solve_walk_kernel + solve_walk_unwind_kernel<4> (the other events' solve_unwind_kernel<4> "
+                      "runs beside them on a second stream)",
+              "unwind": "solve_sum_kernel (per-path sums of the unwound contributions, colour)",
               "megakernel": "solve_kernel<4, false>" + (" (fallback of the split pass)" if split and not launch["graph"] else "")}
     if split and launch["graph"]:
         knames["megakernel"] = "split solve pass replayed from a graph (light + walk + unwind + fallback)"

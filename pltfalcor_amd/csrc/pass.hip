@@ -184,6 +184,8 @@ struct plt_pass {
     // graph replay: captured on capture_stream, instantiated once and updated per frame
     bool graph = false;
     hipStream_t capture_stream = nullptr;
+    // split solve pass: the second stream and its fork/join events (created at the first split execute)
+    plt::SplitFork fork;
     hipGraphExec_t graph_sample = nullptr, graph_solve = nullptr;
     void* d_wf = nullptr;
     size_t wf_paths = 0;
@@ -333,6 +335,9 @@ int plt_pass_destroy(plt_pass* p) {
     for (hipEvent_t e : p->events) (void)hipEventDestroy(e);
     plt::drop_graphs(p);
     if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
+    if (p->fork.aux) (void)hipStreamDestroy(p->fork.aux);
+    if (p->fork.fork) (void)hipEventDestroy(p->fork.fork);
+    if (p->fork.join) (void)hipEventDestroy(p->fork.join);
     delete p;
     return PLT_OK;
 }
@@ -659,8 +664,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         const size_t hitCap = regions * plt::kSolveRegion * p->launch.solve_hits_per_path;
         if (evCap > size_t(UINT32_MAX)) return set_error(PLT_EINVAL, "plt_pass_execute: frame too large for the split solve pass's event area");
         const size_t evBytes = evCap * plt::kEvWords * 4, hitBytes = hitCap * plt::kHitWords * 4;
-        const size_t regBytes = regions * plt::kRegWords * 4, walkBytes = evCap * 4;
-        const size_t bytes = evBytes + hitBytes + regBytes + walkBytes + 256;
+        const size_t regBytes = regions * plt::kRegWords * 4, walkBytes = evCap * 4, contribBytes = evCap * 3 * 4;
+        const size_t bytes = evBytes + hitBytes + regBytes + walkBytes + contribBytes + 256;
         if (bytes > p->solve_bytes) {
             if (p->d_solve) (void)hipFree(p->d_solve);
             p->d_solve = nullptr;
@@ -675,7 +680,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
         sbuf.hits = reinterpret_cast<uint32_t*>(b + evBytes);
         sbuf.region = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes);
         sbuf.walk.items = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes + regBytes);
-        sbuf.walk.counters = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes + regBytes + walkBytes);
+        sbuf.contrib = reinterpret_cast<float*>(b + evBytes + hitBytes + regBytes + walkBytes);
+        sbuf.walk.counters = reinterpret_cast<uint32_t*>(b + evBytes + hitBytes + regBytes + walkBytes + contribBytes);
         sbuf.ev_per_path = p->launch.solve_events_per_path;
         sbuf.hits_per_path = p->launch.solve_hits_per_path;
         sbuf.ev_cap = evCap;
@@ -734,11 +740,18 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     };
     const bool split = useSplit;
     plt_surrogate* surrogate = p->surrogate;
+    if (split && !surrogate && !p->fork.aux) {
+        if (hipStreamCreateWithFlags(&p->fork.aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->fork.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->fork.join, hipEventDisableTiming) != hipSuccess)
+            return set_error(PLT_EHIP, "plt_pass_execute: second stream of the split solve pass could not be created");
+    }
+    const plt::SplitFork fork = p->fork;
     // the split pass's own timing events are recorded between its launches when it runs eagerly
     const bool split_marks = split && !p->graph && ev[2];
     auto solve_seq = [&](hipStream_t s) -> int {
         return split ? plt::launch_solve_pass_split(sc, fr, sbuf, s, split_marks ? ev[2] : nullptr, split_marks ? ev[3] : nullptr,
-                                                    split_marks ? ev[4] : nullptr, surrogate)
+                                                    split_marks ? ev[4] : nullptr, surrogate, fork)
                      : plt::launch_solve_pass(sc, fr, s);
     };
     int rc;

@@ -226,6 +226,8 @@ int launch_solve_sample_4_lights(const PtScene& sc, const PtFrame& fr, const Sol
 int launch_solve_sample_1(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, bool lights);
 int launch_solve_unwind(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_walk_clear(const SolveBuffers& sb, hipStream_t st);
+int launch_solve_walk_unwind(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
+int launch_solve_sum(const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_solve_walks(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st);
 int launch_solve_unwind_surrogate(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, plt_surrogate* s, hipStream_t st);
 
@@ -241,7 +243,7 @@ int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st) {
 // Split solve pass: light sampling, unwind, then the megakernel over the paths whose region's event
 // area overflowed (its other blocks return at once). Three launches on one stream, no host waits.
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st, hipEvent_t after_light,
-                            hipEvent_t after_walk, hipEvent_t after_unwind, plt_surrogate* surrogate) {
+                            hipEvent_t after_walk, hipEvent_t after_unwind, plt_surrogate* surrogate, const SplitFork& fork) {
     if (fr.n_paths == 0) {
         if (after_light) (void)hipEventRecord(after_light, st);
         if (after_walk) (void)hipEventRecord(after_walk, st);
@@ -259,9 +261,27 @@ int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuf
     if (after_light) (void)hipEventRecord(after_light, st);
     // manifold walks: only frames with manifold NEE queue any, but the kernel is cheap to launch
     // empty and a captured graph then holds the same launches whatever the options
-    if ((rc = launch_solve_walks(sc, fr, sb, st)) < 0) return rc;
-    if (after_walk) (void)hipEventRecord(after_walk, st);
-    if ((rc = surrogate ? launch_solve_unwind_surrogate(sc, fr, sb, surrogate, st) : launch_solve_unwind(sc, fr, sb, st)) < 0) return rc;
+    if (surrogate || !fork.aux) {
+        if ((rc = launch_solve_walks(sc, fr, sb, st)) < 0) return rc;
+        if (after_walk) (void)hipEventRecord(after_walk, st);
+        if ((rc = surrogate ? launch_solve_unwind_surrogate(sc, fr, sb, surrogate, st) : launch_solve_unwind(sc, fr, sb, st)) < 0)
+            return rc;
+    } else {
+        // The walk kernel's last walks keep a few waves busy for milliseconds: the events that are
+        // not walks are unwound beside it on the second stream (their contributions stored), the
+        // walks that succeeded after it, and then every path's contributions are summed in order.
+        SolveBuffers staged = sb;
+        staged.stage = 1u;
+        if (hipEventRecord(fork.fork, st) != hipSuccess || hipStreamWaitEvent(fork.aux, fork.fork, 0) != hipSuccess)
+            return set_error(PLT_EHIP, "split solve pass: fork to the second stream failed");
+        if ((rc = launch_solve_unwind(sc, fr, staged, fork.aux)) < 0) return rc;
+        if (hipEventRecord(fork.join, fork.aux) != hipSuccess) return set_error(PLT_EHIP, "split solve pass: join event failed");
+        if ((rc = launch_solve_walks(sc, fr, sb, st)) < 0) return rc;
+        if ((rc = launch_solve_walk_unwind(sc, fr, sb, st)) < 0) return rc;
+        if (hipStreamWaitEvent(st, fork.join, 0) != hipSuccess) return set_error(PLT_EHIP, "split solve pass: join failed");
+        if (after_walk) (void)hipEventRecord(after_walk, st);
+        if ((rc = launch_solve_sum(fr, sb, st)) < 0) return rc;
+    }
     if (after_unwind) (void)hipEventRecord(after_unwind, st);
     PtFrame fb = fr;
     fb.solve_overflow = sb.region;

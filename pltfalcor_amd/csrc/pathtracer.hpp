@@ -155,6 +155,12 @@ struct SolveBuffers {
     uint32_t ev_per_path, hits_per_path;
     size_t ev_cap, hit_cap;  // plane lengths: regions * 64 * per-path capacity
     WalkQueue walk;
+    // Staged unwind: the unwind kernel with `stage` set writes the RGB contribution of every event
+    // that is not a manifold walk into `contrib` (3 planes of ev_cap) and sums nothing; the
+    // walk-unwind kernel adds the walks' contributions, and the sum kernel adds each path's
+    // contributions in slot order (the megakernel's order) and writes the colour.
+    float* contrib;
+    uint32_t stage;
     SurPath sur;             // used by the query and apply unwind modes only
 };
 
@@ -188,11 +194,19 @@ struct LaunchConfig {
 int launch_sample_pass(const PtScene& sc, const PtFrame& fr, const LaunchConfig& lc, hipStream_t st);
 int launch_sample_pass_wavefront(const PtScene& sc, const PtFrame& fr, const WfBuffers& w, const LaunchConfig& lc, hipStream_t st);
 int launch_solve_pass(const PtScene& sc, const PtFrame& fr, hipStream_t st);
+// A second stream for the split pass: with it, the unwind of the events that are not manifold walks
+// runs beside the walk kernel (fork and join through the two events); without it (aux = NULL) the
+// launches are sequential on `st`.
+struct SplitFork {
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 // after_light / after_walk / after_unwind (may be NULL): events recorded on `st` after the
-// light-sampling, the walk and the unwind kernels (kernel timing outside graph capture)
+// light-sampling kernel, after the walk kernel (with the other events' unwind beside it when
+// forked) and after the unwind (kernel timing outside graph capture)
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st,
                             hipEvent_t after_light = nullptr, hipEvent_t after_walk = nullptr, hipEvent_t after_unwind = nullptr,
-                            plt_surrogate* surrogate = nullptr);
+                            plt_surrogate* surrogate = nullptr, const SplitFork& fork = SplitFork{});
 int launch_tile_draws(const PtFrame& fr, uint32_t tiles, float* draws, hipStream_t st);
 
 }  // namespace plt

@@ -670,11 +670,38 @@ PLT_DEV void unwind_event(const PtScene& sc, const PtFrame& fr, const SolveBuffe
     out[2] = sv.b;
 }
 
+// A manifold NEE event whose walk the walk kernel ran (its contribution is the walk-unwind
+// kernel's in a staged unwind).
+PLT_DEV bool is_walk_event(uint32_t meta) {
+    return ((meta >> 13) & 3u) == kEvKindNee && (meta & kEvBitMnee) && ((meta >> 21) & 15u) != 0u;
+}
+PLT_DEV void contrib_put(const SolveBuffers& sb, size_t e, const float* c) {
+    sb.contrib[e] = c[0];
+    sb.contrib[sb.ev_cap + e] = c[1];
+    sb.contrib[2 * sb.ev_cap + e] = c[2];
+}
+
+// Each lane's path's contributions in slot order (the megakernel's order): one block-uniform round
+// per 64 slots; `owner` and `part` hold the round's lanes' paths and contributions.
+struct PathSums {
+    float r = 0.f, g = 0.f, b = 0.f;
+    PLT_DEV void add_round(const uint32_t* owner, const float (*part)[kPtBlock], uint32_t m, uint32_t lane) {
+        for (uint32_t j = 0; j < m; ++j) {
+            if (owner[j] == lane) {
+                r += part[0][j];
+                g += part[1][j];
+                b += part[2][j];
+            }
+        }
+    }
+};
+
 // Two waves per SIMD: the beam, the material instance and the evaluation temporaries fit the 256
 // registers without spilling, and this kernel (no traversal, few dependent fetches) loses less to
 // the lower occupancy than it gains (1080p depth 8: 1.30 ms per frame at two waves, 2.14 ms at four
 // with 612 B of scratch per lane, 2.59 ms at six). Mode: exact unwind, or the surrogate's query or
-// apply pass (the query pass writes no colour).
+// apply pass (the query pass writes no colour). With sb.stage set (exact mode) it writes the
+// contributions of the events that are not manifold walks and sums nothing.
 template <int N, int Mode>
 __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
 solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
@@ -685,7 +712,8 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
     const uint32_t count = rg[kRegEvents];
     const uint64_t ovf = uint64_t(rg[kRegOverflowLo]) | (uint64_t(rg[kRegOverflowHi]) << 32);
     const size_t evBase = size_t(region) * kPtBlock * sb.ev_per_path;
-    float sr = 0.f, sgr = 0.f, sbl = 0.f;
+    const bool stage = Mode == kUnwindExact && sb.stage != 0u;
+    PathSums sum;
     // block-uniform loop (one wave per block): 64 events per round, then each lane adds its own
     // path's contributions in slot order, which is the order the megakernel adds them in
     for (uint32_t base = 0; base < count; base += kPtBlock) {
@@ -696,28 +724,83 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
             const size_t e = evBase + k;
             const uint32_t meta = ev_get(sb, e, kEvMeta);
             who = meta & 63u;
-            if (((ovf >> who) & 1u) || ((meta >> 13) & 3u) == kEvKindVoid) who = kPtBlock;  // left to the megakernel, or void
-            else unwind_event<N, Mode>(sc, fr, sb, e, meta, region, c);
+            if (((ovf >> who) & 1u) || ((meta >> 13) & 3u) == kEvKindVoid || (stage && is_walk_event(meta))) {
+                who = kPtBlock;  // left to the megakernel, void, or the walk-unwind kernel's
+            } else {
+                unwind_event<N, Mode>(sc, fr, sb, e, meta, region, c);
+                if (stage) contrib_put(sb, e, c);
+            }
         }
-        if (Mode == kUnwindQuery) continue;
+        if (Mode == kUnwindQuery || stage) continue;
         owner[lane] = who;
         part[0][lane] = c[0];
         part[1][lane] = c[1];
         part[2][lane] = c[2];
         __syncthreads();
-        const uint32_t m = min(count - base, uint32_t(kPtBlock));
-        for (uint32_t j = 0; j < m; ++j) {
-            if (owner[j] == lane) {
-                sr += part[0][j];
-                sgr += part[1][j];
-                sbl += part[2][j];
-            }
-        }
+        sum.add_round(owner, part, min(count - base, uint32_t(kPtBlock)), lane);
         __syncthreads();
     }
-    if (Mode == kUnwindQuery) return;
+    if (Mode == kUnwindQuery || stage) return;
     const uint32_t li = region * kPtBlock + lane;
-    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sr, sgr, sbl, 1.f);
+    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sum.r, sum.g, sum.b, 1.f);
+}
+
+// Staged unwind, after the walk kernel: the contributions of the walks that succeeded, one lane per
+// queued walk (the void ones add nothing).
+template <int N>
+__global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(2, 2)))
+solve_walk_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
+    const uint32_t total = sb.walk.counters[0];
+    const size_t regionEvents = size_t(kPtBlock) * sb.ev_per_path;
+    for (uint32_t k = blockIdx.x * kPtBlock + threadIdx.x; k < total; k += gridDim.x * kPtBlock) {
+        const size_t e = sb.walk.items[k];
+        const uint32_t meta = ev_get(sb, e, kEvMeta);
+        if (((meta >> 13) & 3u) == kEvKindVoid) continue;
+        const uint32_t region = uint32_t(e / regionEvents), who = meta & 63u;
+        const uint32_t* rg = sb.region + size_t(region) * kRegWords;
+        if ((rg[who < 32u ? kRegOverflowLo : kRegOverflowHi] >> (who & 31u)) & 1u) continue;  // the megakernel's
+        float c[3] = {0.f, 0.f, 0.f};
+        unwind_event<N, kUnwindExact>(sc, fr, sb, e, meta, region, c);
+        contrib_put(sb, e, c);
+    }
+}
+
+// Staged unwind, last: each path's contributions summed in slot order, the colour written.
+__global__ void __launch_bounds__(kPtBlock) solve_sum_kernel(PtFrame fr, SolveBuffers sb) {
+    __shared__ float part[3][kPtBlock];
+    __shared__ uint32_t owner[kPtBlock];
+    const uint32_t lane = threadIdx.x, region = blockIdx.x;
+    const uint32_t* rg = sb.region + size_t(region) * kRegWords;
+    const uint32_t count = rg[kRegEvents];
+    const uint64_t ovf = uint64_t(rg[kRegOverflowLo]) | (uint64_t(rg[kRegOverflowHi]) << 32);
+    const size_t evBase = size_t(region) * kPtBlock * sb.ev_per_path;
+    PathSums sum;
+    for (uint32_t base = 0; base < count; base += kPtBlock) {
+        const uint32_t k = base + lane;
+        uint32_t who = kPtBlock;
+        float c[3] = {0.f, 0.f, 0.f};
+        if (k < count) {
+            const size_t e = evBase + k;
+            const uint32_t meta = ev_get(sb, e, kEvMeta);
+            who = meta & 63u;
+            if (((ovf >> who) & 1u) || ((meta >> 13) & 3u) == kEvKindVoid) {
+                who = kPtBlock;
+            } else {
+                c[0] = sb.contrib[e];
+                c[1] = sb.contrib[sb.ev_cap + e];
+                c[2] = sb.contrib[2 * sb.ev_cap + e];
+            }
+        }
+        owner[lane] = who;
+        part[0][lane] = c[0];
+        part[1][lane] = c[1];
+        part[2][lane] = c[2];
+        __syncthreads();
+        sum.add_round(owner, part, min(count - base, uint32_t(kPtBlock)), lane);
+        __syncthreads();
+    }
+    const uint32_t li = region * kPtBlock + lane;
+    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sum.r, sum.g, sum.b, 1.f);
 }
 
 // Clears the surrogate's query counters (a kernel, so a captured graph replays it).
