@@ -306,7 +306,7 @@ typedef struct plt_scene_arrays {
     const plt_material* materials;
     const plt_spectral_profile* profiles;
     plt_camera camera;
-    uint32_t bvh_max_depth;          /* deepest BVH level (root = 1): the traversal stack bound */
+    uint32_t bvh_max_depth;          /* levels of the 4-wide BVH (root = 1) */
 } plt_scene_arrays;
 int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
 

@@ -3,13 +3,17 @@
 // independent of the tree (watertight intersection with a deterministic tie-break), so the tree
 // only affects speed.
 //
-// Build: per-triangle boxes and centroids; recursive 32-bin SAH over all three axes; subtrees of
-// more than 32k triangles are built on their own threads (disjoint index ranges, partitioned in
-// place). The result is flattened depth-first so that a parent and its first child are adjacent.
+// Build: per-triangle boxes and centroids; recursive 32-bin SAH over all three axes, with an
+// object-median split wherever the SAH split would leave a child that cannot be finished within
+// kMaxBvh2Depth levels; subtrees of more than 32k triangles are built on their own threads
+// (disjoint index ranges, partitioned in place). The binary tree is then collapsed into 4-wide
+// nodes (each node adopts the children of its largest-area inner child until it has four) and
+// flattened depth-first so that a parent and its first child are adjacent.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <future>
+#include <limits>
 #include <memory>
 #include <thread>
 #include "scene_host.hpp"
@@ -50,7 +54,18 @@ struct Builder {
         }
     }
 
-    std::unique_ptr<BNode> build(uint32_t start, uint32_t count) {
+    // Inner levels a subtree of n triangles needs at least: median splits halve it down to leaves.
+    int levels_needed(uint32_t n) const {
+        int l = 0;
+        while (n > uint32_t(max_leaf)) {
+            n = (n + 1) / 2;
+            ++l;
+        }
+        return l;
+    }
+
+    // depth: this node's level (root = 1); invariant depth - 1 + levels_needed(count) <= kMaxBvh2Depth
+    std::unique_ptr<BNode> build(uint32_t start, uint32_t count, int depth) {
         auto n = std::make_unique<BNode>();
         Aabb cb;
         for (uint32_t i = start; i < start + count; ++i) {
@@ -116,14 +131,24 @@ struct Builder {
             mid = uint32_t(it - order.begin());
             if (mid == start || mid == start + count) mid = start + count / 2;
         }
+        if (depth + std::max(levels_needed(mid - start), levels_needed(start + count - mid)) > kMaxBvh2Depth) {
+            // the depth bound: split at the centroid median of the widest axis (children of
+            // ceil(count / 2) and floor(count / 2) triangles need one level less than this node)
+            int axis = 0;
+            for (int a = 1; a < 3; ++a)
+                if (cb.hi[a] - cb.lo[a] > cb.hi[axis] - cb.lo[axis]) axis = a;
+            mid = start + (count + 1) / 2;
+            std::nth_element(order.begin() + start, order.begin() + mid, order.begin() + start + count,
+                             [&](uint32_t x, uint32_t y) { return cent[3 * size_t(x) + axis] < cent[3 * size_t(y) + axis]; });
+        }
         const uint32_t lcnt = mid - start, rcnt = count - lcnt;
         if (count > kParallelThreshold && threads_left.fetch_sub(1) > 0) {
-            auto fut = std::async(std::launch::async, [this, start, lcnt] { return build(start, lcnt); });
-            n->r = build(mid, rcnt);
+            auto fut = std::async(std::launch::async, [this, start, lcnt, depth] { return build(start, lcnt, depth + 1); });
+            n->r = build(mid, rcnt, depth + 1);
             n->l = fut.get();
         } else {
-            n->l = build(start, lcnt);
-            n->r = build(mid, rcnt);
+            n->l = build(start, lcnt, depth + 1);
+            n->r = build(mid, rcnt, depth + 1);
         }
         return n;
     }
@@ -131,28 +156,47 @@ struct Builder {
 
 int32_t leaf_ref(uint32_t start, uint32_t count) { return ~int32_t((start << 3) | (count - 1)); }
 
-void set_child_box(BvhNode& n, int c, const Aabb& b) {
-    float* bx = c == 0 ? n.b0x : n.b1x;
-    float* by = c == 0 ? n.b0y : n.b1y;
-    bx[0] = b.lo[0]; bx[1] = b.hi[0];
-    by[0] = b.lo[1]; by[1] = b.hi[1];
-    n.bz[2 * c] = b.lo[2]; n.bz[2 * c + 1] = b.hi[2];
+BvhNode empty_node() {
+    BvhNode n{};
+    const float inf = std::numeric_limits<float>::infinity();
+    for (int c = 0; c < 4; ++c) {
+        n.lox[c] = n.hix[c] = n.loy[c] = n.hiy[c] = n.loz[c] = n.hiz[c] = inf;
+        n.child[c] = 0;
+    }
+    return n;
 }
 
-// Depth-first flattening; returns the index of the node created for `b` (an inner node).
+void set_child_box(BvhNode& n, int c, const Aabb& b) {
+    n.lox[c] = b.lo[0]; n.hix[c] = b.hi[0];
+    n.loy[c] = b.lo[1]; n.hiy[c] = b.hi[1];
+    n.loz[c] = b.lo[2]; n.hiz[c] = b.hi[2];
+}
+
+// Depth-first flattening into 4-wide nodes; returns the index of the node created for `b` (an
+// inner node of the binary tree).
 int32_t flatten(const BNode* b, std::vector<BvhNode>& out, int depth, int& max_depth) {
     max_depth = std::max(max_depth, depth);
+    const BNode* ch[4] = {b->l.get(), b->r.get(), nullptr, nullptr};
+    int n = 2;
+    while (n < 4) {  // adopt the children of the largest inner child
+        int best = -1;
+        float best_area = -1.f;
+        for (int i = 0; i < n; ++i)
+            if (ch[i]->count == 0 && ch[i]->box.area() > best_area) {
+                best_area = ch[i]->box.area();
+                best = i;
+            }
+        if (best < 0) break;
+        const BNode* x = ch[best];
+        ch[best] = x->l.get();
+        ch[n++] = x->r.get();
+    }
     const int32_t me = int32_t(out.size());
-    out.emplace_back();
-    const BNode* ch[2] = {b->l.get(), b->r.get()};
-    for (int c = 0; c < 2; ++c) {
+    out.push_back(empty_node());
+    for (int c = 0; c < n; ++c) {
         set_child_box(out[me], c, ch[c]->box);
-        if (ch[c]->count > 0) {
-            out[me].child[c] = leaf_ref(ch[c]->start, ch[c]->count);
-        } else {
-            const int32_t ci = flatten(ch[c], out, depth + 1, max_depth);
-            out[me].child[c] = ci;
-        }
+        const int32_t ref = ch[c]->count > 0 ? leaf_ref(ch[c]->start, ch[c]->count) : flatten(ch[c], out, depth + 1, max_depth);
+        out[me].child[c] = ref;
     }
     return me;
 }
@@ -163,19 +207,15 @@ HostBvh build_bvh(const std::vector<float>& pos, const std::vector<uint32_t>& id
     const uint32_t nt = uint32_t(idx.size() / 3);
     if (nt == 0) return out;
     Builder b(pos, idx, std::min(max_leaf, 8));
-    std::unique_ptr<BNode> root = b.build(0, nt);
-    if (root->count > 0) {
-        // Single leaf: a root whose two children are the same leaf (a duplicate test is harmless;
-        // an inverted "empty" box would pass a min/max slab test and must not be used).
-        BvhNode n{};
+    std::unique_ptr<BNode> root = b.build(0, nt, 1);
+    if (root->count > 0) {  // a single leaf: a root with one used slot
+        BvhNode n = empty_node();
         set_child_box(n, 0, root->box);
-        set_child_box(n, 1, root->box);
         n.child[0] = leaf_ref(root->start, root->count);
-        n.child[1] = leaf_ref(root->start, root->count);
         out.nodes.push_back(n);
         out.max_depth = 1;
     } else {
-        out.nodes.reserve(2 * size_t(nt) / 2 + 1);
+        out.nodes.reserve(size_t(nt) / 2 + 1);
         flatten(root.get(), out.nodes, 1, out.max_depth);
     }
     out.tri_order = std::move(b.order);

@@ -55,11 +55,12 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(TraceScene sc, uint3
 }  // namespace
 
 int upload_scene(const HostScene& hs, DeviceSceneBuffers& d) {
-    if (hs.bvh.max_depth > kTraversalStackDepth)
+    // a BVH4 of L levels pushes at most 3 (L - 1) entries; the builder's depth bound keeps L <= 32
+    if (3 * (hs.bvh.max_depth - 1) > kTraversalStackDepth + kTraversalSpill)
         return set_error(PLT_EINVAL, "BVH depth " + std::to_string(hs.bvh.max_depth) + " exceeds the traversal stack (" +
-                                         std::to_string(kTraversalStackDepth) + ")");
-    // Nodes as float4 x 4; triangles in leaf order as (v0, id), (v1, 0), (v2, 0).
-    std::vector<float4> nodes(hs.bvh.nodes.size() * 4);
+                                         std::to_string(kTraversalStackDepth + kTraversalSpill) + " entries)");
+    // Nodes as float4 x 8; triangles in leaf order as (v0, id), (v1, 0), (v2, 0).
+    std::vector<float4> nodes(hs.bvh.nodes.size() * 8);
     std::memcpy(nodes.data(), hs.bvh.nodes.data(), hs.bvh.nodes.size() * sizeof(BvhNode));
     const size_t nt = hs.bvh.tri_order.size();
     std::vector<float4> tris(3 * nt);

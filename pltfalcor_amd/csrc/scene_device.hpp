@@ -8,10 +8,8 @@
 
 namespace plt {
 
-// Deepest BVH the traversal kernels accept: the LDS stack holds one sibling per level, so a
-// tree of depth d needs d - 1 slots (scene creation rejects deeper trees). 32 slots x 4 B x 64
-// lanes = 8 KB per one-wave block, which leaves LDS room for 20 blocks per CU (the binned-SAH
-// trees of 1.3 M and 5.2 M triangles are 24 and 28 levels deep).
+// Traversal stack entries kept in LDS per lane; deeper entries go to private memory (traverse.hpp).
+// 32 slots x 4 B x 64 lanes = 8 KB per one-wave block, which leaves LDS room for 20 blocks per CU.
 constexpr int kTraversalStackDepth = 32;
 
 struct DeviceSceneBuffers {

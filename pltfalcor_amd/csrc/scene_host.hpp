@@ -22,21 +22,26 @@ struct Aabb {
     }
 };
 
-// BVH2 node as stored in HBM: both children's boxes live in the parent so one 64 B fetch tests
-// two boxes. Child reference >= 0: inner node index; < 0: leaf, ~ref = (first << 3) | (count - 1).
+// BVH4 node as stored in HBM (128 B, two cache lines fetched together): the four children's boxes
+// as SoA rows (lo x, hi x, lo y, hi y, lo z, hi z; one float per child) and the child references.
+// Child reference >= 0: inner node index; < 0: leaf, ~ref = (first << 3) | (count - 1). An unused
+// slot has the box [+inf, +inf] on every axis, which no ray enters.
 struct BvhNode {
-    float b0x[2], b0y[2];   // child 0 lo/hi x, lo/hi y
-    float b1x[2], b1y[2];   // child 1 lo/hi x, lo/hi y
-    float bz[4];            // child 0 lo/hi z, child 1 lo/hi z
-    int32_t child[2];
-    int32_t pad[2];
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    int32_t child[4];
+    int32_t pad[4];
 };
-static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 B");
+static_assert(sizeof(BvhNode) == 128, "BvhNode must be 128 B");
+
+// Inner levels of the binary SAH tree at most (root = 1): the builder splits at the object median
+// wherever a SAH split would need more, so every tree meets it. Collapsing to four children never
+// deepens the tree, so the BVH4 has at most as many levels.
+constexpr int kMaxBvh2Depth = 32;
 
 struct HostBvh {
     std::vector<BvhNode> nodes;
     std::vector<uint32_t> tri_order;  // triangle ids in leaf order
-    int max_depth = 0;
+    int max_depth = 0;                // BVH4 levels (root = 1)
 };
 
 // Builds a binned-SAH BVH over triangles (positions as 3 floats per vertex, indices 3 per tri).

@@ -1,7 +1,9 @@
-// Software ray traversal for CDNA4: BVH2 with both child boxes in the parent (one 64 B node
-// fetch = four 16 B loads tests two boxes), triangles re-ordered into leaf order as 48 B records
-// (v0 + triangle id, v1, v2), and a per-lane traversal stack in LDS laid out [depth][lane] so a
-// wave's pushes/pops hit 64 distinct banks.
+// Software ray traversal for CDNA4: BVH4 with the four child boxes in the parent as SoA rows (one
+// 128 B node = eight independent 16 B loads tests four boxes, so a level costs one memory latency),
+// triangles re-ordered into leaf order as 48 B records (v0 + triangle id, v1, v2) fetched one
+// triangle ahead of the test, and a per-lane traversal stack: its first kTraversalStackDepth
+// entries in LDS laid out [depth][lane] (a wave's pushes/pops hit 64 distinct banks), the rest in
+// private memory (scratch), touched only by rays deep in very deep trees.
 //
 // Hit results are independent of the tree: the watertight test (Woop, Benthin, Wald 2013,
 // "Watertight Ray/Triangle Intersection", JCGT 2(1)) runs with FP contraction off and a double
@@ -23,7 +25,7 @@ struct Hit {
 };
 
 struct TraceScene {
-    const float4* __restrict__ nodes;  // 4 per node
+    const float4* __restrict__ nodes;  // 8 per node (BvhNode)
     const float4* __restrict__ tris;   // 3 per triangle in leaf order
 };
 
@@ -102,10 +104,26 @@ __device__ __forceinline__ float box_entry(float lox, float hix, float loy, floa
     return tn <= tf * 1.0000005f ? tn : __builtin_inff();
 }
 
+// Entries of the traversal stack beyond the LDS part, in private memory. A BVH4 of L levels needs
+// at most 3 (L - 1) entries (three siblings pushed per level).
+constexpr int kTraversalSpill = 64;
+
+// Sorts four keys ascending (five compare-exchanges).
+__device__ __forceinline__ void sort4(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d) {
+    uint32_t t;
+    t = min(a, b); b = max(a, b); a = t;
+    t = min(c, d); d = max(c, d); c = t;
+    t = min(a, c); c = max(a, c); a = t;
+    t = min(b, d); d = max(b, d); b = t;
+    t = min(b, c); c = max(b, c); b = t;
+}
+
 // Traverses the BVH. anyHit: stop at the first hit in [tmin, tmax] (visibility rays); a per-lane
 // value lets one call serve a wave whose lanes cast both kinds (whether a ray hits anything in
 // [tmin, tmax] does not depend on it). `stack` points at this lane's first LDS slot; consecutive
-// slots are `stride` ints apart.
+// slots are `stride` ints apart. Children are visited nearest entry first (ties and order only
+// change the speed: the closest hit is the smallest (t, triangle id) over every triangle tested,
+// and boxes are conservative).
 template <int kDepth>
 __device__ __forceinline__ Hit traverse_rays(const TraceScene& sc, const Ray& r, int* stack, int stride, bool anyHit) {
     Hit h{kNoHit, 0.f, 0.f, r.tmax};
@@ -115,46 +133,71 @@ __device__ __forceinline__ Hit traverse_rays(const TraceScene& sc, const Ray& r,
     const float dz = fabsf(r.dz) > 1e-30f ? r.dz : copysignf(1e-30f, r.dz);
     const float ix = 1.f / dx, iy = 1.f / dy, iz = 1.f / dz;
     const float oix = r.ox * ix, oiy = r.oy * iy, oiz = r.oz * iz;
+    int spill[kTraversalSpill];
     int sp = 0;
     int node = 0;
     float best = r.tmax;
+    auto push = [&](int x) {
+        if (sp < kDepth) stack[sp * stride] = x;
+        else if (sp < kDepth + kTraversalSpill) spill[sp - kDepth] = x;
+        ++sp;
+    };
     while (true) {
         if (node >= 0) {
-            const float4 n0 = sc.nodes[4 * node + 0];
-            const float4 n1 = sc.nodes[4 * node + 1];
-            const float4 n2 = sc.nodes[4 * node + 2];
-            const float4 n3 = sc.nodes[4 * node + 3];
-            const float e0 = box_entry(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
-            const float e1 = box_entry(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
-            const int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
-            const bool h0 = e0 != __builtin_inff(), h1 = e1 != __builtin_inff();
-            if (h0 && h1) {
-                const bool first0 = e0 <= e1;
-                node = first0 ? c0 : c1;
-                if (sp < kDepth) stack[(sp++) * stride] = first0 ? c1 : c0;
+            const float4* q = sc.nodes + 8 * node;
+            const float4 lox = q[0], hix = q[1], loy = q[2], hiy = q[3], loz = q[4], hiz = q[5], cf = q[6];
+            const float e0 = box_entry(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            const float e1 = box_entry(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            const float e2 = box_entry(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            const float e3 = box_entry(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, ix, iy, iz, oix, oiy, oiz, r.tmin, best);
+            // sort keys: the entry distance's bits (non-negative floats order as integers) with the
+            // slot in the low two bits; a missed box sorts last
+            constexpr float kInf = __builtin_inff();
+            uint32_t k0 = e0 != kInf ? (__float_as_uint(e0) & ~3u) : 0xFFFFFFFFu;
+            uint32_t k1 = e1 != kInf ? ((__float_as_uint(e1) & ~3u) | 1u) : 0xFFFFFFFFu;
+            uint32_t k2 = e2 != kInf ? ((__float_as_uint(e2) & ~3u) | 2u) : 0xFFFFFFFFu;
+            uint32_t k3 = e3 != kInf ? ((__float_as_uint(e3) & ~3u) | 3u) : 0xFFFFFFFFu;
+            sort4(k0, k1, k2, k3);
+            if (k0 != 0xFFFFFFFFu) {
+                const int c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z), c3 = __float_as_int(cf.w);
+                auto child = [&](uint32_t k) {
+                    const uint32_t slot = k & 3u;
+                    return slot == 0u ? c0 : (slot == 1u ? c1 : (slot == 2u ? c2 : c3));
+                };
+                // the farther hit children on the stack, farthest first; the nearest next
+                if (k3 != 0xFFFFFFFFu) push(child(k3));
+                if (k2 != 0xFFFFFFFFu) push(child(k2));
+                if (k1 != 0xFFFFFFFFu) push(child(k1));
+                node = child(k0);
                 continue;
             }
-            if (h0 || h1) { node = h0 ? c0 : c1; continue; }
         } else {
             const uint32_t leaf = uint32_t(~node);
             const uint32_t start = leaf >> 3, cnt = (leaf & 7u) + 1u;
+            const float4* t = sc.tris + 3 * size_t(start);
+            float4 v0 = t[0], v1 = t[1], v2 = t[2];
             for (uint32_t i = 0; i < cnt; ++i) {
-                const float4 v0 = sc.tris[3 * (start + i) + 0];
-                const float4 v1 = sc.tris[3 * (start + i) + 1];
-                const float4 v2 = sc.tris[3 * (start + i) + 2];
-                float t, b1, b2;
-                if (intersect_tri(p, v0, v1, v2, r.tmin, best, t, b1, b2)) {
+                float4 w0 = v0, w1 = v1, w2 = v2;
+                if (i + 1 < cnt) {  // the next triangle's fetch overlaps this test
+                    w0 = t[3 * (i + 1) + 0];
+                    w1 = t[3 * (i + 1) + 1];
+                    w2 = t[3 * (i + 1) + 2];
+                }
+                float tt, b1, b2;
+                if (intersect_tri(p, v0, v1, v2, r.tmin, best, tt, b1, b2)) {
                     const uint32_t id = __float_as_uint(v0.w);
-                    if (t < best || id < h.tri) {
-                        best = t;
-                        h.tri = id; h.b1 = b1; h.b2 = b2; h.t = t;
+                    if (tt < best || id < h.tri) {
+                        best = tt;
+                        h.tri = id; h.b1 = b1; h.b2 = b2; h.t = tt;
                         if (anyHit) return h;
                     }
                 }
+                v0 = w0; v1 = w1; v2 = w2;
             }
         }
         if (sp == 0) break;
-        node = stack[(--sp) * stride];
+        --sp;
+        node = sp < kDepth ? stack[sp * stride] : spill[sp - kDepth];
     }
     return h;
 }
