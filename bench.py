@@ -636,23 +636,22 @@ def main():
     if split:
         ev_b = EVENT_BYTES * per_frame["events"] + WALK_VERTEX_BYTES * per_frame["walk_vertices"] + REGION_BYTES * regions
         kb["light"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + ev_b
-        # walk kernel + the unwind of every event (the non-walk events' beside it, then the walks')
-        kb["walk"] = (WALK_BYTES * per_frame["walks"] + 2 * WALK_VERTEX_BYTES * per_frame["walk_vertices"] + ev_b
-                      + SOLVE_BYTES_PER_SEGMENT * per_frame["event_vertices"] + 12 * per_frame["events"])
+        kb["walk"] = WALK_BYTES * per_frame["walks"] + 2 * WALK_VERTEX_BYTES * per_frame["walk_vertices"]
+        # the rest of the unwind (the part beside the walk kernel is in neither figure's time)
         kb["unwind"] = SUM_BYTES_PER_EVENT * per_frame["events"] + REGION_BYTES * regions + SOLVE_BYTES_PER_PIXEL * pixels
         kb["megakernel"] = REGION_BYTES * regions  # the fallback: its region check, plus overflowed paths' work
     else:
         kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
     knames = {"sample": "sample_kernel", "light": "solve_sample_kernel<4, false>",
-              "walk": "solve_walk_kernel + solve_walk_unwind_kernel<4> (the other events' solve_unwind_kernel<4> "
-                      "runs beside them on a second stream)",
-              "unwind": "solve_sum_kernel (per-path sums of the unwound contributions, colour)",
+              "walk": "solve_walk_kernel (the non-walk events' solve_unwind_kernel<4> runs beside it on a second "
+                      "stream)",
+              "unwind": "solve_walk_unwind_kernel<4>, the join with the second stream, solve_sum_kernel",
               "megakernel": "solve_kernel<4, false>" + (" (fallback of the split pass)" if split and not launch["graph"] else "")}
     if split and launch["graph"]:
         knames["megakernel"] = "split solve pass replayed from a graph (light + walk + unwind + fallback)"
         kb["megakernel"] = SOLVE_BYTES_PER_SEGMENT * seg_per_exec + SOLVE_BYTES_PER_PIXEL * pixels
     profiles = {"sample": "pltpt_sample_1080p_d8_pmc.md", "light": "pltpt_solve_light_1080p_d8_pmc.md",
-                "walk": None, "unwind": "pltpt_solve_unwind_1080p_d8_pmc.md",
+                "walk": "pltpt_solve_walk_1080p_d8_pmc.md", "unwind": None,
                 "megakernel": None if split else "pltpt_solve_1080p_d8_pmc.md"}
     kdom = max(avg, key=lambda k: avg[k])
     kname, kms, kbytes = knames[kdom], avg[kdom], kb[kdom]

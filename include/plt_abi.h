@@ -492,10 +492,12 @@ int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t fram
  * the sums. Enabling or disabling restarts them too. */
 int plt_pass_set_profiling(plt_pass* pass, int enable);
 int plt_pass_get_kernel_times(plt_pass* pass, double* sample_ms, double* solve_ms, uint32_t* executes);
-/* The same sums with the solve pass resolved into its kernels: the split pass's light-sampling,
- * manifold-walk and unwind kernels and the solve megakernel (the whole solve pass of the megakernel
- * schedule, the split pass's fallback launch, and the whole split pass when it is replayed from a
- * graph, whose launches are not timed one by one). */
+/* The same sums with the solve pass resolved into its kernels: the split pass's light-sampling
+ * kernel, its manifold-walk kernel, the rest of its unwind (the unwind of the non-walk events runs
+ * on a second stream beside the walk kernel; this is the walks' unwind, the join and the per-path
+ * sums), and the solve megakernel (the whole solve pass of the megakernel schedule, the split
+ * pass's fallback launch, and the whole split pass when it is replayed from a graph, whose launches
+ * are not timed one by one). */
 int plt_pass_get_kernel_times_split(plt_pass* pass, double* sample_ms, double* light_ms, double* walk_ms, double* unwind_ms,
                                     double* megakernel_ms, uint32_t* executes);
 /* Counts of the last execute's split solve pass, read back from the device (waits for the device):

@@ -277,9 +277,9 @@ int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuf
         if ((rc = launch_solve_unwind(sc, fr, staged, fork.aux)) < 0) return rc;
         if (hipEventRecord(fork.join, fork.aux) != hipSuccess) return set_error(PLT_EHIP, "split solve pass: join event failed");
         if ((rc = launch_solve_walks(sc, fr, sb, st)) < 0) return rc;
+        if (after_walk) (void)hipEventRecord(after_walk, st);
         if ((rc = launch_solve_walk_unwind(sc, fr, sb, st)) < 0) return rc;
         if (hipStreamWaitEvent(st, fork.join, 0) != hipSuccess) return set_error(PLT_EHIP, "split solve pass: join failed");
-        if (after_walk) (void)hipEventRecord(after_walk, st);
         if ((rc = launch_solve_sum(fr, sb, st)) < 0) return rc;
     }
     if (after_unwind) (void)hipEventRecord(after_unwind, st);

@@ -202,8 +202,9 @@ struct SplitFork {
     hipEvent_t fork = nullptr, join = nullptr;
 };
 // after_light / after_walk / after_unwind (may be NULL): events recorded on `st` after the
-// light-sampling kernel, after the walk kernel (with the other events' unwind beside it when
-// forked) and after the unwind (kernel timing outside graph capture)
+// light-sampling kernel, after the walk kernel, and after the unwind (when forked: the walks'
+// unwind, the join with the other events' unwind on the second stream, and the per-path sums);
+// kernel timing outside graph capture
 int launch_solve_pass_split(const PtScene& sc, const PtFrame& fr, const SolveBuffers& sb, hipStream_t st,
                             hipEvent_t after_light = nullptr, hipEvent_t after_walk = nullptr, hipEvent_t after_unwind = nullptr,
                             plt_surrogate* surrogate = nullptr, const SplitFork& fork = SplitFork{});

@@ -143,3 +143,60 @@ def test_scene_from_arrays_matches_procedural():
     with pytest.raises(_lib.PLTError):
         Scene.from_arrays(None, a["positions"], a["normals"], a["indices"], a["tri_mesh"], badm, a["materials"],
                           a["profiles"], s.arrays.camera)
+
+
+def _staircase(n=300):
+    """Triangles whose sizes and positions grow geometrically (x1.1 each): a binned SAH peels off
+    one triangle per level, so an unbounded binary tree would be about n levels deep."""
+    from pltfalcor_amd.scene import Scene
+    base = Scene(None, seed=3, sphere_count=1, subdivision=0)
+    a = base.numpy()
+    s = (1.1 ** np.arange(n)).astype(np.float32)
+    pos = np.zeros((3 * n, 3), np.float32)
+    pos[0::3, 0] = s
+    pos[1::3] = np.stack([s, 0.5 * s, np.zeros_like(s)], 1)
+    pos[2::3] = np.stack([s, np.zeros_like(s), 0.5 * s], 1)
+    nrm = np.tile(np.array([1, 0, 0], np.float32), (3 * n, 1))
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    return a, base, pos, nrm, idx
+
+
+def test_bvh_depth_is_bounded_on_a_geometric_staircase():
+    """The builder's depth bound: the staircase builds (no depth refusal) into a 4-wide BVH of at
+    most 32 levels (the bound the traversal stack is sized for)."""
+    from pltfalcor_amd.scene import Scene
+    a, base, pos, nrm, idx = _staircase()
+    t = Scene.from_arrays(None, pos, nrm, idx, np.zeros(len(idx), np.uint32), np.zeros(1, np.uint32), a["materials"],
+                          a["profiles"], base.arrays.camera)
+    d = t.arrays.bvh_max_depth
+    assert 8 <= d <= 32, d
+
+
+@pytest.mark.gpu
+def test_gpu_trace_bit_exact_on_a_deep_tree(gpu):
+    """The staircase's deep tree on the GPU: rays along the stair (deep stacks, past the LDS part of
+    the traversal stack) and random rays hit bit-exactly what brute force hits."""
+    import torch
+    from pltfalcor_amd.bsdf import Context
+    from pltfalcor_amd.scene import Scene
+    a, base, pos, nrm, idx = _staircase()
+    ctx = Context(0)
+    t = Scene.from_arrays(ctx, pos, nrm, idx, np.zeros(len(idx), np.uint32), np.zeros(1, np.uint32), a["materials"],
+                          a["profiles"], base.arrays.camera)
+    rng = np.random.default_rng(11)
+    n = 4096
+    s = pos[0::3, 0]
+    tgt = np.stack([rng.choice(s, n) * rng.uniform(1.0, 1.001, n), rng.uniform(0, 0.2, n) * s[-1],
+                    rng.uniform(0, 0.2, n) * s[-1]], 1).astype(np.float32)
+    o = np.stack([np.full(n, -10.0), tgt[:, 1] * 0.5, tgt[:, 2] * 0.5], 1).astype(np.float32)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.ascontiguousarray(np.concatenate([o.T, d.T, np.zeros((1, n), np.float32), np.full((1, n), 3.4e38, np.float32)]),
+                                dtype=np.float32)
+    for rr in (rays, random_rays(4096, 12, lo=-1e3, hi=1e3)):
+        g = t.trace(torch.from_numpy(rr).cuda()).cpu().numpy().view(np.uint32)
+        ref = O.trace(pos, idx, rr, brute=True)
+        assert np.array_equal(g, ref), f"{np.sum(g[0] != ref[0])} hit ids differ"
+        ga = t.trace(torch.from_numpy(rr).cuda(), any_hit=True).cpu().numpy()
+        assert np.array_equal(ga.astype(np.uint32), O.trace(pos, idx, rr, any_hit=True, brute=True))
+    assert (g[0] != 0xFFFFFFFF).mean() > 0.05
