@@ -79,6 +79,9 @@ def parse_args():
     ap.add_argument("--solve", choices=("megakernel", "split", "autotune"), default="autotune",
                     help="solve-pass schedule; autotune (default) times the candidate launch configurations on "
                          "the workload before the warm-up and keeps the fastest")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="override a PLTPT option of the path workload (repeatable), e.g. --option doMNEE=0 to see "
+                         "what the manifold walks cost in the per-kernel times; the line then names the overrides")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the timed region, run 5 plain device copies of MIB MiB (the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters to bytes)")
@@ -563,6 +566,11 @@ def main():
     # the reference's PLTPathTracer graph creates PLTPT with its defaults (manifold NEE on, 2 occluders,
     # 60 Newton iterations, threshold 5e-5); only the depth and the HWSS count are set here
     opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4}
+    overrides = {}
+    for kv in args.option:
+        k, _, v = kv.partition("=")
+        overrides[k] = float(v)
+    opts.update(overrides)
     pltpt = PLTPT(ctx, opts)
     pltpt.set_sample_pattern(PLTPT.HALTON, 16)  # the graph's GBufferRT jitter (samplePattern Halton, 16)
     pltpt.set_scene(scene)
@@ -692,7 +700,7 @@ def main():
                                "jitter; FP64/FP32 accumulation and auto-exposure tone mapping per frame",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
                    "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
-                   "segments_per_frame": seg_per_exec},
+                   "segments_per_frame": seg_per_exec, "option_overrides": overrides or None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "kernel_ms": kms,
                      "algorithmic_bytes_per_launch": kbytes,
