@@ -556,6 +556,30 @@ int plt_surrogate_train_step(plt_surrogate* s, uint32_t n, const float* X, const
  * mean/std) and the training box of the inputs: a sample with any input outside [x_lo, x_hi]
  * is out of distribution and is evaluated exactly. */
 int plt_surrogate_set_stats(plt_surrogate* s, const float* y_mean, const float* y_std, const float* x_lo, const float* x_hi);
+/* The training data pipeline on the device. From n exact samples (inputs X [n x 22], Mueller
+ * matrices M [n x 16] as plt_surrogate_generate writes them): the training targets T [n x 16]
+ * (ln M00 and asinh(Mk / M00), standardised by the valid rows' mean and standard deviation, the
+ * deviations divided by the output weights 4, 2, 2, 2, 1, ...), the valid rows' indices in row
+ * order in idx (the first *n_valid entries; a row is valid when M is finite and M00 > 1e-12), and
+ * the statistics and the valid inputs' box set as plt_surrogate_set_stats would. Waits for the
+ * stream. */
+int plt_surrogate_prepare(plt_surrogate* s, uint32_t n, const float* X, const float* M, float* T, uint32_t* idx, uint32_t* n_valid,
+                          plt_stream stream);
+/* The order plt_surrogate_train draws epoch `epoch` in for n samples and `seed`: out[pos] is the
+ * sample taken at position pos (a permutation of [0, n)). */
+int plt_surrogate_shuffle(uint32_t n, uint32_t seed, uint32_t epoch, uint32_t* out, plt_stream stream);
+/* The statistics set (host copies: target mean [16] and scale [16], input box lo [22], hi [22]). */
+int plt_surrogate_get_stats(const plt_surrogate* s, float* y_mean, float* y_std, float* x_lo, float* x_hi);
+/* Trains for `epochs` passes over the n_valid samples idx[0..n_valid) of (X, T) (from
+ * plt_surrogate_prepare) in batches of `batch` (a multiple of 32, at most max_batch; the last
+ * n_valid % batch of each epoch's order are left out): each epoch draws a fresh shuffle (a keyed
+ * bijection of [0, n_valid) computed on the device, seeded by `seed` and the epoch), Adam with the
+ * cosine-decayed learning rate lr (1 + cos(pi step / steps)) / 2 (at least 1e-6). With `graph` set,
+ * one training step is captured as a HIP graph and replayed (every step-dependent value lives in
+ * device state); without, the same launches run eagerly (identical results). Writes each epoch's
+ * mean batch loss to epoch_loss [epochs]; waits for the stream. */
+int plt_surrogate_train(plt_surrogate* s, uint32_t n_valid, const uint32_t* idx, const float* X, const float* T, uint32_t batch,
+                        uint32_t epochs, float lr, uint32_t seed, int graph, float* epoch_loss, plt_stream stream);
 
 /* Exact samples for training and judging (the material layer of the path tracer, FP32 with FP64
  * multilayer steps): per sample i a material (written to materials[i], binary16 fields as

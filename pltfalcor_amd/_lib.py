@@ -133,6 +133,13 @@ SIGNATURES = [
                                            C.POINTER(C.c_float), C.c_void_p]),
     ("plt_surrogate_set_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                           C.POINTER(C.c_float)]),
+    ("plt_surrogate_prepare", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_uint32), C.c_void_p]),
+    ("plt_surrogate_shuffle", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    ("plt_surrogate_get_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.POINTER(C.c_float)]),
+    ("plt_surrogate_train", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32,
+                                      C.c_float, C.c_uint32, C.c_int, C.POINTER(C.c_float), C.c_void_p]),
     ("plt_surrogate_generate", C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p]),
     ("plt_surrogate_eval", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

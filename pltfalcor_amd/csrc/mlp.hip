@@ -52,7 +52,14 @@ struct GemmArgs {
     int ldc, ldt, ldh;
     int k_per_split;  // K range of one split (multiple of 32)
     size_t split_stride;
+    // kEpiMask: per-tile column sums of the bf16 outputs (the bias gradient's partials) into
+    // bias_part[blockIdx.y * kBiasCols + bias_col + j] when set
+    float* bias_part;
+    int bias_col;
 };
+// Columns of the bias-gradient partials: b4 (16, from the loss kernel's 256-sample blocks), then
+// b3, b2, b1 (128 each, from the backward GEMMs' 64-row tiles).
+constexpr int kBiasCols = 16 + 3 * 128;
 
 constexpr int kTile = 64, kKStep = 32, kLdsRow = kKStep + 8;  // +8 bf16 per row breaks bank conflicts
 constexpr int kGemmThreads = 256;
@@ -60,6 +67,7 @@ constexpr int kGemmThreads = 256;
 __global__ void __launch_bounds__(kGemmThreads) gemm_nt_kernel(GemmArgs g) {
     __shared__ __bf16 As[kTile * kLdsRow];
     __shared__ __bf16 Bs[kTile * kLdsRow];
+    __shared__ float colsum[2][kTile];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
     const int kbeg = blockIdx.z * g.k_per_split;
@@ -93,6 +101,7 @@ __global__ void __launch_bounds__(kGemmThreads) gemm_nt_kernel(GemmArgs g) {
         __syncthreads();
     }
     // D layout of a 16 x 16 block: lane holds rows 4*(lane/16) + r (r = 0..3) of column lane%16
+    float cs[2] = {0.f, 0.f};  // this lane's share of its two columns' sums (kEpiMask)
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -119,6 +128,7 @@ __global__ void __launch_bounds__(kGemmThreads) gemm_nt_kernel(GemmArgs g) {
                         const __bf16 d = static_cast<__bf16>(h > 0.f ? v : 0.f);
                         if (g.Cb) g.Cb[size_t(i) * g.ldc + j] = d;
                         if (g.Ct) g.Ct[size_t(j) * g.ldt + i] = d;
+                        cs[b] += static_cast<float>(d);
                         break;
                     }
                     default:
@@ -126,6 +136,21 @@ __global__ void __launch_bounds__(kGemmThreads) gemm_nt_kernel(GemmArgs g) {
                         break;
                 }
             }
+    if (g.epi == kEpiMask && g.bias_part) {  // block-uniform
+        // the tile's column sums in a fixed order: lanes of one column (xor 16, 32), then the two
+        // waves that share the column range
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            cs[b] += __shfl_xor(cs[b], 16);
+            cs[b] += __shfl_xor(cs[b], 32);
+        }
+        if (lane < 16) {
+            colsum[wave >> 1][wc + lane] = cs[0];
+            colsum[wave >> 1][wc + 16 + lane] = cs[1];
+        }
+        __syncthreads();
+        if (t < kTile && n0 + t < g.N) g.bias_part[size_t(blockIdx.y) * kBiasCols + g.bias_col + n0 + t] = colsum[0][t] + colsum[1][t];
+    }
 }
 
 int gemm(const GemmArgs& g, int splits, hipStream_t st) {
@@ -286,13 +311,20 @@ __global__ void pack_inputs_kernel(int n, const float* __restrict__ X, __bf16* _
 
 // Mean-squared error over the 16 outputs: dL/dY = 2 (Y - T) / (n * 16), stored as bf16 in dY
 // [n x 32] (zero padded) and dYt [16 x ldt]; per-block partial sums of the loss in FP32.
+// The block's column sums of dY (the last layer's bias gradient) go to bias_part[blockIdx.x *
+// kBiasCols + f] in a fixed order (lanes of a wave by xor steps, then the four waves).
 __global__ void loss_grad_kernel(int n, const float* __restrict__ Y, const float* __restrict__ T, __bf16* __restrict__ dY,
-                                 __bf16* __restrict__ dYt, int ldt, float* __restrict__ loss_part) {
+                                 __bf16* __restrict__ dYt, int ldt, float* __restrict__ loss_part, float* __restrict__ bias_part) {
     __shared__ float red[kEw];
+    __shared__ float colsum[kEw / 64][kOut];
     const int i = blockIdx.x * kEw + threadIdx.x;
     float l = 0.f;
+    float gs[kOut];
+#pragma unroll
+    for (int f = 0; f < kOut; ++f) gs[f] = 0.f;
     if (i < n) {
         const float s = 2.f / (float(n) * float(kOut));
+#pragma unroll
         for (int f = 0; f < kOutPad; ++f) {
             float g = 0.f;
             if (f < kOut) {
@@ -302,53 +334,34 @@ __global__ void loss_grad_kernel(int n, const float* __restrict__ Y, const float
             }
             const __bf16 gb = static_cast<__bf16>(g);
             dY[size_t(i) * kOutPad + f] = gb;
-            if (f < kOut) dYt[size_t(f) * ldt + i] = gb;
+            if (f < kOut) {
+                dYt[size_t(f) * ldt + i] = gb;
+                gs[f] = static_cast<float>(gb);
+            }
         }
     }
     red[threadIdx.x] = l;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int f = 0; f < kOut; ++f) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) gs[f] += __shfl_xor(gs[f], o);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int f = 0; f < kOut; ++f) colsum[wave][f] = gs[f];
+    }
     __syncthreads();
     for (int s = kEw / 2; s > 0; s >>= 1) {
         if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
-}
-
-// out[e] = sum over splits of part[z * stride + e] (fixed order), for e < count.
-__global__ void reduce_splits_kernel(int count, int splits, size_t stride, const float* __restrict__ part, float* __restrict__ out) {
-    const int e = blockIdx.x * kEw + threadIdx.x;
-    if (e >= count) return;
-    float s = 0.f;
-    for (int z = 0; z < splits; ++z) s += part[z * stride + e];
-    out[e] = s;
-}
-
-// Bias gradient: out[j] = sum over the batch of Gt[j, 0..n) (one block per feature j).
-__global__ void bias_grad_kernel(int n, const __bf16* __restrict__ Gt, int ldt, float* __restrict__ out) {
-    __shared__ float red[kEw];
-    const int j = blockIdx.x;
-    float s = 0.f;
-    for (int i = threadIdx.x; i < n; i += kEw) s += static_cast<float>(Gt[size_t(j) * ldt + i]);
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = kEw / 2; k > 0; k >>= 1) {
-        if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
+    if (bias_part && threadIdx.x < kOut) {
+        float c = 0.f;
+        for (int w = 0; w < kEw / 64; ++w) c += colsum[w][threadIdx.x];
+        bias_part[size_t(blockIdx.x) * kBiasCols + threadIdx.x] = c;
     }
-    if (threadIdx.x == 0) out[j] = red[0];
-}
-
-// Adam (Kingma & Ba 2015) on the FP32 master parameters.
-__global__ void adam_kernel(int count, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
-                            float lr, float b1, float b2, float eps, float c1, float c2) {
-    const int e = blockIdx.x * kEw + threadIdx.x;
-    if (e >= count) return;
-    const float ge = g[e];
-    const float me = b1 * m[e] + (1.f - b1) * ge;
-    const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
-    m[e] = me;
-    v[e] = ve;
-    p[e] -= lr * (me / c1) / (sqrtf(ve / c2) + eps);
 }
 
 // bf16 operand copies of one layer: W [out x in_pad] and Wt [in x out_pad] from the master
@@ -378,6 +391,7 @@ constexpr size_t kOffW1 = 0, kOffB1 = kOffW1 + size_t(kHid) * kIn, kOffW2 = kOff
                  kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + size_t(kHid) * kHid, kOffW4 = kOffB3 + kHid,
                  kOffB4 = kOffW4 + size_t(kOut) * kHid;
 constexpr int kSplitTarget = 64;  // weight-gradient GEMM splits of the batch (K) dimension
+constexpr size_t kTrainStateWords = 16;
 
 template <class T>
 int dev_alloc(plt_surrogate* s, T** p, size_t count) {
@@ -428,19 +442,53 @@ int forward(plt_surrogate* s, int n, bool keep_t, hipStream_t st) {
     return gemm(g, 1, st);
 }
 
+// The weight gradients' split-K partials: region r of s->part holds splits x (out x in) floats of
+// layer 4, 3, 2, 1 in that order (layer 1 with its 32 padded inputs).
+constexpr int kGradCount[4] = {kOut * kHid, kHid * kHid, kHid * kHid, kHid * kInPad};
+constexpr int kGradTotal = kOut * kHid + 2 * kHid * kHid + kHid * kInPad;  // 38,912
+int split_len(int n) { return (n / kKStep + kSplitTarget - 1) / kSplitTarget * kKStep; }
+
 // dW[out x in] = sum over the batch of Gt[out, b] * Ht[in, b] (n a multiple of 32), as split-K
-// partials reduced in a fixed order.
-int weight_grad(plt_surrogate* s, int n, const __bf16* Gt, int out, const __bf16* Ht, int in, float* dst, hipStream_t st) {
+// partials into the layer's region of s->part; grad_reduce_kernel sums them.
+int weight_grad(plt_surrogate* s, int n, const __bf16* Gt, int out, const __bf16* Ht, int in, size_t region, hipStream_t st) {
     const int ld = int(s->max_batch);
-    const int per = (n / kKStep + kSplitTarget - 1) / kSplitTarget * kKStep;
+    const int per = split_len(n);
     const int splits = (n + per - 1) / per;
     GemmArgs g = nt(Gt, ld, Ht, ld, out, in, n);
-    g.epi = kEpiPartial; g.Cf = s->part; g.ldc = in; g.k_per_split = per; g.split_stride = size_t(out) * in;
-    int rc;
-    if ((rc = gemm(g, splits, st)) < 0) return rc;
-    const int count = out * in;
-    hipLaunchKernelGGL(reduce_splits_kernel, dim3((count + kEw - 1) / kEw), dim3(kEw), 0, st, count, splits, g.split_stride, s->part, dst);
-    return check_launch("reduce_splits_kernel");
+    g.epi = kEpiPartial; g.Cf = s->part + region * kSplitTarget; g.ldc = in; g.k_per_split = per; g.split_stride = size_t(out) * in;
+    return gemm(g, splits, st);
+}
+
+// Every gradient from its partials, in one launch: element e < kGradTotal of the weight regions
+// sums its `splits` partials (four interleaved accumulators, combined in a fixed order); the
+// kBiasCols elements after them are the biases (bias_reduce's sums). Layer 1 drops its padding.
+__global__ void grad_reduce_kernel(int splits, int rows_out, int rows_hid, const float* __restrict__ part,
+                                   const float* __restrict__ bias_part, float* __restrict__ grads) {
+    const int e = blockIdx.x * kEw + threadIdx.x;
+    if (e >= kGradTotal + kBiasCols) return;
+    if (e >= kGradTotal) {
+        const int c = e - kGradTotal;
+        const int rows = c < kOut ? rows_out : rows_hid;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < rows; ++r) acc[r & 3] += bias_part[size_t(r) * kBiasCols + c];
+        const size_t dst = c < kOut ? kOffB4 + c : c < kOut + kHid ? kOffB3 + (c - kOut) : c < kOut + 2 * kHid ? kOffB2 + (c - kOut - kHid)
+                                                                                           : kOffB1 + (c - kOut - 2 * kHid);
+        grads[dst] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+        return;
+    }
+    constexpr int c4 = kOut * kHid, c3 = kHid * kHid, c2 = kHid * kHid;
+    const int r = e < c4 ? 0 : e < c4 + c3 ? 1 : e < c4 + c3 + c2 ? 2 : 3;
+    const int first = r == 0 ? 0 : r == 1 ? c4 : r == 2 ? c4 + c3 : c4 + c3 + c2;
+    const int cnt = r == 0 ? c4 : r == 1 ? c3 : r == 2 ? c2 : kHid * kInPad;
+    const int local = e - first;
+    const size_t base = size_t(first) * kSplitTarget;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int z = 0; z < splits; ++z) acc[z & 3] += part[base + size_t(z) * cnt + local];
+    const float sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    if (r == 0) grads[kOffW4 + local] = sum;
+    else if (r == 1) grads[kOffW3 + local] = sum;
+    else if (r == 2) grads[kOffW2 + local] = sum;
+    else if (local % kInPad < kIn) grads[kOffW1 + (local / kInPad) * kIn + local % kInPad] = sum;
 }
 
 // Loss and gradients of n samples (X packed, T [n x 16] FP32 targets) into s->grads.
@@ -449,32 +497,33 @@ int loss_and_grads(plt_surrogate* s, int n, const float* T, float* loss_host, hi
     int rc;
     if ((rc = forward(s, n, true, st)) < 0) return rc;
     const int blocks = (n + kEw - 1) / kEw;
-    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks), dim3(kEw), 0, st, n, s->Y, T, s->dY, s->dYt, ld, s->loss_part);
+    hipLaunchKernelGGL(loss_grad_kernel, dim3(blocks), dim3(kEw), 0, st, n, s->Y, T, s->dY, s->dYt, ld, s->loss_part, s->bias_part);
     if ((rc = check_launch("loss_grad_kernel")) < 0) return rc;
-    // backward data: G3 = (dY W4) * [H3 > 0], G2 = (G3 W3) * [H2 > 0], G1 = (G2 W2) * [H1 > 0]
+    // backward data: G3 = (dY W4) * [H3 > 0], G2 = (G3 W3) * [H2 > 0], G1 = (G2 W2) * [H1 > 0]; each
+    // epilogue also writes its tiles' column sums (the bias gradients' partials)
     GemmArgs g = nt(s->dY, kOutPad, s->W4t, kOutPad, n, kHid, kOutPad);
     g.epi = kEpiMask; g.H = s->H3; g.ldh = kHid; g.Cb = s->G3; g.ldc = kHid; g.Ct = s->G3t; g.ldt = ld;
+    g.bias_part = s->bias_part; g.bias_col = kOut;
     if ((rc = gemm(g, 1, st)) < 0) return rc;
     g = nt(s->G3, kHid, s->W3t, kHid, n, kHid, kHid);
     g.epi = kEpiMask; g.H = s->H2; g.ldh = kHid; g.Cb = s->G2; g.ldc = kHid; g.Ct = s->G2t; g.ldt = ld;
+    g.bias_part = s->bias_part; g.bias_col = kOut + kHid;
     if ((rc = gemm(g, 1, st)) < 0) return rc;
     g = nt(s->G2, kHid, s->W2t, kHid, n, kHid, kHid);
     g.epi = kEpiMask; g.H = s->H1; g.ldh = kHid; g.Cb = nullptr; g.Ct = s->G1t; g.ldt = ld;
+    g.bias_part = s->bias_part; g.bias_col = kOut + 2 * kHid;
     if ((rc = gemm(g, 1, st)) < 0) return rc;
-    // weight gradients (the batch is the K dimension)
-    if ((rc = weight_grad(s, n, s->dYt, kOut, s->H3t, kHid, s->grads + kOffW4, st)) < 0) return rc;
-    if ((rc = weight_grad(s, n, s->G3t, kHid, s->H2t, kHid, s->grads + kOffW3, st)) < 0) return rc;
-    if ((rc = weight_grad(s, n, s->G2t, kHid, s->H1t, kHid, s->grads + kOffW2, st)) < 0) return rc;
-    // W1's gradient over the 22 real inputs: the product is [128 x 32]; keep columns < 22
-    if ((rc = weight_grad(s, n, s->G1t, kHid, s->Xt, kInPad, s->part + size_t(kSplitTarget + 1) * kHid * kHid, st)) < 0) return rc;
-    if (hipMemcpy2DAsync(s->grads + kOffW1, kIn * sizeof(float), s->part + size_t(kSplitTarget + 1) * kHid * kHid, kInPad * sizeof(float),
-                         kIn * sizeof(float), kHid, hipMemcpyDeviceToDevice, st) != hipSuccess)
-        return set_error(PLT_EHIP, "plt_surrogate: gradient copy failed");
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kOut), dim3(kEw), 0, st, n, s->dYt, ld, s->grads + kOffB4);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kHid), dim3(kEw), 0, st, n, s->G3t, ld, s->grads + kOffB3);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kHid), dim3(kEw), 0, st, n, s->G2t, ld, s->grads + kOffB2);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(kHid), dim3(kEw), 0, st, n, s->G1t, ld, s->grads + kOffB1);
-    if ((rc = check_launch("bias_grad_kernel")) < 0) return rc;
+    // weight gradients (the batch is the K dimension), each layer's partials in its own region
+    const size_t r3 = kGradCount[0], r2 = r3 + kGradCount[1], r1 = r2 + kGradCount[2];
+    if ((rc = weight_grad(s, n, s->dYt, kOut, s->H3t, kHid, 0, st)) < 0) return rc;
+    if ((rc = weight_grad(s, n, s->G3t, kHid, s->H2t, kHid, r3, st)) < 0) return rc;
+    if ((rc = weight_grad(s, n, s->G2t, kHid, s->H1t, kHid, r2, st)) < 0) return rc;
+    // W1's product is [128 x 32]; the reduction keeps the 22 real inputs
+    if ((rc = weight_grad(s, n, s->G1t, kHid, s->Xt, kInPad, r1, st)) < 0) return rc;
+    const int splits = (n + split_len(n) - 1) / split_len(n);
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((kGradTotal + kBiasCols + kEw - 1) / kEw), dim3(kEw), 0, st, splits, blocks,
+                       (n + kTile - 1) / kTile, s->part, s->bias_part, s->grads);
+    if ((rc = check_launch("grad_reduce_kernel")) < 0) return rc;
     if (loss_host) {
         std::vector<float> lp(static_cast<size_t>(blocks));
         if (hipMemcpyAsync(lp.data(), s->loss_part, lp.size() * sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -485,6 +534,290 @@ int loss_and_grads(plt_surrogate* s, int n, const float* T, float* loss_host, hi
         *loss_host = float(sum / (double(n) * kOut));
     }
     return PLT_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Training data pipeline and device-driven training (plt_surrogate_prepare / plt_surrogate_train).
+// ---------------------------------------------------------------------------------------
+// Loss weights per output, applied through the standardisation (an output scaled by 1/s weighs
+// s^2 in the mean-squared error): the radiance row (ln M00, then M01..M03) first.
+__constant__ float kOutputScale[kOut] = {4.f, 2.f, 2.f, 2.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+
+// The training targets of one exact Mueller matrix: ln M00 and asinh(Mk / M00), in double; a row
+// is valid when every entry is finite and M00 > 1e-12.
+__device__ bool targets_of(const float* __restrict__ Mrow, double* y) {
+    bool ok = Mrow[0] > 1e-12f;
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) ok = ok && isfinite(Mrow[k]);
+    const double m00 = ok ? double(Mrow[0]) : 1.0;
+    y[0] = log(m00);
+#pragma unroll
+    for (int k = 1; k < kOut; ++k) y[k] = asinh(double(Mrow[k]) / m00);
+    return ok;
+}
+
+// Pass 1, per block of kEw rows: valid count, column sums of the targets (double), and the box of
+// the valid rows' inputs. Sums run over the block's rows in order (deterministic).
+__global__ void __launch_bounds__(kEw) prep_sums_kernel(int n, const float* __restrict__ X, const float* __restrict__ M,
+                                                        double* __restrict__ psum, float* __restrict__ plo, float* __restrict__ phi,
+                                                        uint32_t* __restrict__ pcount) {
+    __shared__ double ys[kEw][kOut];
+    __shared__ uint8_t valid[kEw];
+    const int t = threadIdx.x, i = blockIdx.x * kEw + t;
+    double y[kOut];
+    bool ok = false;
+    if (i < n) ok = targets_of(M + size_t(i) * kOut, y);
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) ys[t][k] = ok ? y[k] : 0.0;
+    valid[t] = ok ? 1 : 0;
+    __syncthreads();
+    const int rows = min(kEw, n - int(blockIdx.x) * kEw);
+    if (t < kOut) {
+        double acc = 0.0;
+        for (int r = 0; r < rows; ++r) acc += ys[r][t];
+        psum[size_t(blockIdx.x) * kOut + t] = acc;
+    } else if (t >= 32 && t < 32 + kIn) {
+        const int f = t - 32;
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (int r = 0; r < rows; ++r)
+            if (valid[r]) {
+                const float v = X[(size_t(blockIdx.x) * kEw + r) * kIn + f];
+                lo = fminf(lo, v);
+                hi = fmaxf(hi, v);
+            }
+        plo[size_t(blockIdx.x) * kIn + f] = lo;
+        phi[size_t(blockIdx.x) * kIn + f] = hi;
+    } else if (t == 64) {
+        uint32_t c = 0;
+        for (int r = 0; r < rows; ++r) c += valid[r];
+        pcount[blockIdx.x] = c;
+    }
+}
+
+// One block: the means, the box (into stats), the total valid count, and each block's first slot in
+// the compacted index list (exclusive scan of the counts). Fixed-order sums.
+__global__ void __launch_bounds__(kEw) prep_mean_kernel(int blocks, const double* __restrict__ psum, const float* __restrict__ plo,
+                                                        const float* __restrict__ phi, uint32_t* __restrict__ pcount, double* __restrict__ dstat,
+                                                        float* __restrict__ stats, uint32_t* __restrict__ n_valid) {
+    const int t = threadIdx.x;
+    __shared__ uint32_t total;
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < blocks; ++b) {
+            const uint32_t c = pcount[b];
+            pcount[b] = run;
+            run += c;
+        }
+        total = run;
+        *n_valid = run;
+    }
+    __syncthreads();
+    if (t < kOut) {
+        double acc = 0.0;
+        for (int b = 0; b < blocks; ++b) acc += psum[size_t(b) * kOut + t];
+        const double mean = total ? acc / double(total) : 0.0;
+        dstat[t] = mean;
+        stats[t] = float(mean);
+    } else if (t >= 32 && t < 32 + kIn) {
+        const int f = t - 32;
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (int b = 0; b < blocks; ++b) {
+            lo = fminf(lo, plo[size_t(b) * kIn + f]);
+            hi = fmaxf(hi, phi[size_t(b) * kIn + f]);
+        }
+        stats[2 * kOut + f] = lo;
+        stats[2 * kOut + kIn + f] = hi;
+    }
+}
+
+// Pass 2: per block, the column sums of squared deviations from the mean (double) and the valid
+// rows' indices, compacted in row order.
+__global__ void __launch_bounds__(kEw) prep_dev_kernel(int n, const float* __restrict__ M, const double* __restrict__ dstat,
+                                                       const uint32_t* __restrict__ boffset, double* __restrict__ psum,
+                                                       uint32_t* __restrict__ idx) {
+    __shared__ double ys[kEw][kOut];
+    __shared__ uint32_t wcount[kEw / 64];
+    const int t = threadIdx.x, i = blockIdx.x * kEw + t;
+    double y[kOut];
+    bool ok = false;
+    if (i < n) ok = targets_of(M + size_t(i) * kOut, y);
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) {
+        const double d = ok ? y[k] - dstat[k] : 0.0;
+        ys[t][k] = d * d;
+    }
+    const uint64_t m = __ballot(ok);
+    const int lane = t & 63, wave = t >> 6;
+    if (lane == 0) wcount[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (ok) {
+        uint32_t before = uint32_t(__popcll(m & ((uint64_t(1) << lane) - 1u)));
+        for (int w = 0; w < wave; ++w) before += wcount[w];
+        idx[boffset[blockIdx.x] + before] = uint32_t(i);
+    }
+    const int rows = min(kEw, n - int(blockIdx.x) * kEw);
+    if (t < kOut) {
+        double acc = 0.0;
+        for (int r = 0; r < rows; ++r) acc += ys[r][t];
+        psum[size_t(blockIdx.x) * kOut + t] = acc;
+    }
+}
+
+// One block: the standard deviations (population), scaled by the output weights, into stats.
+__global__ void prep_std_kernel(int blocks, const double* __restrict__ psum, const uint32_t* __restrict__ n_valid,
+                                double* __restrict__ dstat, float* __restrict__ stats) {
+    const int t = threadIdx.x;
+    if (t >= kOut) return;
+    double acc = 0.0;
+    for (int b = 0; b < blocks; ++b) acc += psum[size_t(b) * kOut + t];
+    const uint32_t c = *n_valid;
+    double sd = c ? sqrt(acc / double(c)) : 1.0;
+    sd = (sd > 1e-6 ? sd : 1.0) / double(kOutputScale[t]);
+    dstat[kOut + t] = sd;
+    stats[kOut + t] = float(sd);
+}
+
+// Pass 3: the standardised targets T = (y - mean) / std of every row (rows that are not valid are
+// never drawn).
+__global__ void __launch_bounds__(kEw) prep_targets_kernel(int n, const float* __restrict__ M, const double* __restrict__ dstat,
+                                                           float* __restrict__ T) {
+    const int i = blockIdx.x * kEw + threadIdx.x;
+    if (i >= n) return;
+    double y[kOut];
+    (void)targets_of(M + size_t(i) * kOut, y);
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) T[size_t(i) * kOut + k] = float((y[k] - dstat[k]) / dstat[kOut + k]);
+}
+
+// Device-side state of plt_surrogate_train: every launch of a training step reads what it needs
+// from here, so one captured step replays as every step.
+struct TrainState {
+    uint32_t step, steps_per_epoch, steps_total, batch, n_valid, seed, adam_t;
+    float lr0;
+};
+static_assert(sizeof(TrainState) <= kTrainStateWords * 4, "train state size");
+
+__host__ __device__ inline uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+// A bijection of [0, 2^(2 half)): a four-round balanced Feistel network keyed by `key`.
+__device__ uint32_t feistel(uint32_t x, int half, uint32_t key) {
+    const uint32_t mask = (1u << half) - 1u;
+    uint32_t l = x >> half, r = x & mask;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t t = l ^ (mix32(r ^ mix32(key + k)) & mask);
+        l = r;
+        r = t;
+    }
+    return (l << half) | r;
+}
+// The epoch's shuffle of [0, n), n > 2^(2 half - 2): cycle-walking the Feistel bijection (the walk
+// from x < n ends: the bijection's cycle through x returns to x; it takes under four steps on
+// average).
+__device__ uint32_t shuffle_index(uint32_t x, uint32_t n, int half, uint32_t key) {
+    uint32_t y = feistel(x, half, key);
+    while (y >= n) y = feistel(y, half, key);
+    return y;
+}
+
+__host__ __device__ inline uint32_t epoch_key(uint32_t seed, uint32_t epoch) { return mix32(seed * 0x9E3779B9u + epoch); }
+int shuffle_half(uint32_t n) {
+    int bits = 2;
+    while (bits < 32 && (uint64_t(1) << bits) < n) ++bits;
+    return (bits + (bits & 1)) / 2;
+}
+
+// The epoch's order itself (plt_surrogate_shuffle).
+__global__ void shuffle_kernel(uint32_t n, int half, uint32_t key, uint32_t* __restrict__ out) {
+    const uint32_t x = blockIdx.x * kEw + threadIdx.x;
+    if (x < n) out[x] = shuffle_index(x, n, half, key);
+}
+
+// One training batch: row i is the valid sample idx[shuffle(pos)] of this epoch's order, packed
+// as the bf16 inputs (both layouts) and the FP32 targets.
+__global__ void __launch_bounds__(kEw) gather_pack_kernel(const TrainState* __restrict__ st, const uint32_t* __restrict__ idx,
+                                                          const float* __restrict__ X, const float* __restrict__ T, __bf16* __restrict__ Xb,
+                                                          __bf16* __restrict__ Xt, int ldt, float* __restrict__ Tb, int half) {
+    const TrainState s = *st;
+    const uint32_t i = blockIdx.x * kEw + threadIdx.x;
+    if (i >= s.batch) return;
+    const uint32_t epoch = s.step / s.steps_per_epoch;
+    const uint32_t pos = (s.step % s.steps_per_epoch) * s.batch + i;
+    const uint32_t j = idx[shuffle_index(pos, s.n_valid, half, epoch_key(s.seed, epoch))];
+#pragma unroll
+    for (int f = 0; f < kInPad; ++f) {
+        const __bf16 v = static_cast<__bf16>(f < kIn ? X[size_t(j) * kIn + f] : 0.f);
+        Xb[size_t(i) * kInPad + f] = v;
+        Xt[size_t(f) * ldt + i] = v;
+    }
+#pragma unroll
+    for (int f = 0; f < kOut; ++f) Tb[size_t(i) * kOut + f] = T[size_t(j) * kOut + f];
+}
+
+// The bf16 operand copies of the four weight matrices (W [out x in_pad], Wt [in x out_pad]).
+struct WeightCopies {
+    __bf16 *W1, *W1t, *W2, *W2t, *W3, *W3t, *W4, *W4t;
+};
+
+// Adam on the FP32 master parameters fused with the bf16 copies of each updated weight (the
+// padding of the copies stays zero). The learning rate and the bias corrections come from the
+// arguments, or with `st` from the device state: the cosine-decayed rate lr0 (1 + cos(pi step /
+// steps)) / 2 (at least 1e-6) and the corrections of Adam step adam_t + 1.
+__global__ void adam_cast_kernel(int count, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                 float* __restrict__ v, const TrainState* __restrict__ st, float lr, float c1, float c2, float b1, float b2,
+                                 float eps, WeightCopies w) {
+    const int e = blockIdx.x * kEw + threadIdx.x;
+    if (e >= count) return;
+    if (st) {
+        const TrainState s = *st;
+        const double tt = double(s.adam_t) + 1.0;
+        c1 = float(1.0 - pow(double(b1), tt));
+        c2 = float(1.0 - pow(double(b2), tt));
+        const double cs = cos(3.14159265358979323846 * double(s.step) / double(max(1u, s.steps_total)));
+        lr = float(fmax(double(s.lr0) * 0.5 * (1.0 + cs), 1e-6));
+    }
+    const float ge = g[e];
+    const float me = b1 * m[e] + (1.f - b1) * ge;
+    const float ve = b2 * v[e] + (1.f - b2) * ge * ge;
+    m[e] = me;
+    v[e] = ve;
+    const float pe = p[e] - lr * (me / c1) / (sqrtf(ve / c2) + eps);
+    p[e] = pe;
+    const __bf16 h = static_cast<__bf16>(pe);
+    if (size_t(e) < kOffB1) {
+        const int o = e / kIn, i = e % kIn;
+        w.W1[o * kInPad + i] = h;
+        w.W1t[i * kHid + o] = h;
+    } else if (size_t(e) >= kOffW2 && size_t(e) < kOffB2) {
+        const int l = int(e - kOffW2), o = l / kHid, i = l % kHid;
+        w.W2[o * kHid + i] = h;
+        w.W2t[i * kHid + o] = h;
+    } else if (size_t(e) >= kOffW3 && size_t(e) < kOffB3) {
+        const int l = int(e - kOffW3), o = l / kHid, i = l % kHid;
+        w.W3[o * kHid + i] = h;
+        w.W3t[i * kHid + o] = h;
+    } else if (size_t(e) >= kOffW4 && size_t(e) < kOffB4) {
+        const int l = int(e - kOffW4), o = l / kHid, i = l % kHid;
+        w.W4[o * kHid + i] = h;
+        w.W4t[i * kOutPad + o] = h;
+    }
+}
+WeightCopies weight_copies(const plt_surrogate* s) { return WeightCopies{s->W1, s->W1t, s->W2, s->W2t, s->W3, s->W3t, s->W4, s->W4t}; }
+
+// End of a step: its mean loss (the loss kernel's block partials, summed in order) and the counters.
+__global__ void step_end_kernel(TrainState* __restrict__ st, const float* __restrict__ loss_part, int blocks, float* __restrict__ loss_hist) {
+    if (threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int b = 0; b < blocks; ++b) acc += double(loss_part[b]);
+    loss_hist[st->step] = float(acc / (double(st->batch) * kOut));
+    st->step += 1u;
+    st->adam_t += 1u;
 }
 
 int pack(plt_surrogate* s, int n, const float* X, bool with_t, hipStream_t st) {
@@ -540,8 +873,10 @@ int plt_surrogate_create(plt_context* ctx, uint32_t max_batch, uint32_t seed, pl
     A(&s->dY, B * kOutPad); A(&s->dYt, B * kOut);
     A(&s->G3, B * kHid); A(&s->G3t, B * kHid); A(&s->G2, B * kHid); A(&s->G2t, B * kHid); A(&s->G1t, B * kHid);
     A(&s->Y, B * kOut);
-    A(&s->part, size_t(kSplitTarget + 2) * kHid * kHid);
+    A(&s->part, size_t(kSplitTarget) * kGradTotal);
     A(&s->loss_part, (B + kEw - 1) / kEw);
+    A(&s->bias_part, (B / kTile + 1) * size_t(kBiasCols));
+    A(&s->train_state, kTrainStateWords);
     A(&s->stats, 2 * kOut + 2 * kIn);
     A(&s->counter, 1);
     if (rc != PLT_OK) { plt_surrogate_destroy(s); return rc; }
@@ -628,10 +963,133 @@ int plt_surrogate_train_step(plt_surrogate* s, uint32_t n, const float* X, const
     ++s->step;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - std::pow(b1, float(s->step)), c2 = 1.f - std::pow(b2, float(s->step));
-    hipLaunchKernelGGL(adam_kernel, dim3((kParams + kEw - 1) / kEw), dim3(kEw), 0, st, kParams, s->params, s->grads, s->adam_m, s->adam_v,
-                       lr, b1, b2, 1e-8f, c1, c2);
-    if ((rc = check_launch("adam_kernel")) < 0) return rc;
-    return cast_all(s, st);
+    hipLaunchKernelGGL(adam_cast_kernel, dim3((kParams + kEw - 1) / kEw), dim3(kEw), 0, st, kParams, s->params, s->grads, s->adam_m,
+                       s->adam_v, nullptr, lr, c1, c2, b1, b2, 1e-8f, weight_copies(s));
+    return check_launch("adam_cast_kernel");
+}
+
+int plt_surrogate_prepare(plt_surrogate* s, uint32_t n, const float* X, const float* M, float* T, uint32_t* idx, uint32_t* n_valid,
+                          plt_stream stream) {
+    if (!s || n == 0 || !X || !M || !T || !idx || !n_valid || n > (1u << 30))
+        return set_error(PLT_EINVAL, "plt_surrogate_prepare: bad arguments");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const int blocks = int((n + kEw - 1) / kEw);
+    // scratch of the reductions: per-block sums, boxes and counts, the double statistics, the count
+    const size_t bytes = size_t(blocks) * (kOut * 8 + 2 * kIn * 4 + 4) + 2 * kOut * 8 + 64;
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return set_error(PLT_ENOMEM, "plt_surrogate_prepare: scratch allocation failed");
+    char* b = static_cast<char*>(q);
+    double* psum = reinterpret_cast<double*>(b);
+    double* dstat = psum + size_t(blocks) * kOut;
+    float* plo = reinterpret_cast<float*>(dstat + 2 * kOut);
+    float* phi = plo + size_t(blocks) * kIn;
+    uint32_t* pcount = reinterpret_cast<uint32_t*>(phi + size_t(blocks) * kIn);
+    uint32_t* nv = pcount + blocks;
+    int rc = PLT_OK;
+    hipLaunchKernelGGL(prep_sums_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), X, M, psum, plo, phi, pcount);
+    hipLaunchKernelGGL(prep_mean_kernel, dim3(1), dim3(kEw), 0, st, blocks, psum, plo, phi, pcount, dstat, s->stats, nv);
+    hipLaunchKernelGGL(prep_dev_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), M, dstat, pcount, psum, idx);
+    hipLaunchKernelGGL(prep_std_kernel, dim3(1), dim3(64), 0, st, blocks, psum, nv, dstat, s->stats);
+    hipLaunchKernelGGL(prep_targets_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), M, dstat, T);
+    rc = check_launch("plt_surrogate_prepare kernels");
+    if (rc == PLT_OK && (hipMemcpyAsync(n_valid, nv, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+                         hipStreamSynchronize(st) != hipSuccess))
+        rc = set_error(PLT_EHIP, "plt_surrogate_prepare: read-back failed");
+    (void)hipFree(q);
+    if (rc == PLT_OK) s->has_stats = *n_valid > 0;
+    return rc;
+}
+
+int plt_surrogate_shuffle(uint32_t n, uint32_t seed, uint32_t epoch, uint32_t* out, plt_stream stream) {
+    if (n == 0 || !out) return set_error(PLT_EINVAL, "plt_surrogate_shuffle: bad arguments");
+    hipLaunchKernelGGL(shuffle_kernel, dim3((n + kEw - 1) / kEw), dim3(kEw), 0, static_cast<hipStream_t>(stream), n, shuffle_half(n),
+                       epoch_key(seed, epoch), out);
+    return check_launch("shuffle_kernel");
+}
+
+int plt_surrogate_get_stats(const plt_surrogate* s, float* y_mean, float* y_std, float* x_lo, float* x_hi) {
+    if (!s || !y_mean || !y_std || !x_lo || !x_hi) return set_error(PLT_EINVAL, "plt_surrogate_get_stats: bad arguments");
+    if (!s->has_stats) return set_error(PLT_ESTATE, "plt_surrogate_get_stats: no statistics set");
+    float h[2 * kOut + 2 * kIn];
+    if (hipMemcpy(h, s->stats, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(PLT_EHIP, "plt_surrogate_get_stats: read-back failed");
+    std::memcpy(y_mean, h, kOut * sizeof(float));
+    std::memcpy(y_std, h + kOut, kOut * sizeof(float));
+    std::memcpy(x_lo, h + 2 * kOut, kIn * sizeof(float));
+    std::memcpy(x_hi, h + 2 * kOut + kIn, kIn * sizeof(float));
+    return PLT_OK;
+}
+
+int plt_surrogate_train(plt_surrogate* s, uint32_t n_valid, const uint32_t* idx, const float* X, const float* T, uint32_t batch,
+                        uint32_t epochs, float lr, uint32_t seed, int graph, float* epoch_loss, plt_stream stream) {
+    if (!s || !idx || !X || !T || !epoch_loss || batch == 0 || batch % 32 || batch > s->max_batch || epochs == 0 || n_valid < batch ||
+        !(lr > 0.f))
+        return set_error(PLT_EINVAL, "plt_surrogate_train: bad arguments (batch a multiple of 32, at most max_batch and n_valid)");
+    const uint32_t spe = n_valid / batch;
+    const uint64_t total = uint64_t(epochs) * spe;
+    if (total >= (uint64_t(1) << 31)) return set_error(PLT_EINVAL, "plt_surrogate_train: too many steps");
+    const int half = shuffle_half(n_valid);
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const TrainState ts{0u, spe, uint32_t(total), batch, n_valid, seed, s->step, lr};
+    // the step's own buffers: the batch's targets and the per-step losses
+    void* q = nullptr;
+    if (hipMalloc(&q, size_t(batch) * kOut * 4 + size_t(total) * 4) != hipSuccess)
+        return set_error(PLT_ENOMEM, "plt_surrogate_train: buffer allocation failed");
+    float* Tb = static_cast<float*>(q);
+    float* loss_hist = Tb + size_t(batch) * kOut;
+    const int lblocks = int((batch + kEw - 1) / kEw);
+    auto* state = reinterpret_cast<TrainState*>(s->train_state);
+    int rc = PLT_OK;
+    if (hipMemcpyAsync(state, &ts, sizeof(ts), hipMemcpyHostToDevice, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        rc = set_error(PLT_EHIP, "plt_surrogate_train: state upload failed");
+    auto step = [&](hipStream_t qs) -> int {
+        hipLaunchKernelGGL(gather_pack_kernel, dim3(lblocks), dim3(kEw), 0, qs, state, idx, X, T, s->Xb, s->Xt, int(s->max_batch), Tb, half);
+        int r;
+        if ((r = check_launch("gather_pack_kernel")) < 0) return r;
+        if ((r = loss_and_grads(s, int(batch), Tb, nullptr, qs)) < 0) return r;
+        hipLaunchKernelGGL(adam_cast_kernel, dim3((kParams + kEw - 1) / kEw), dim3(kEw), 0, qs, kParams, s->params, s->grads, s->adam_m,
+                           s->adam_v, state, 0.f, 1.f, 1.f, 0.9f, 0.999f, 1e-8f, weight_copies(s));
+        if ((r = check_launch("adam_cast_kernel")) < 0) return r;
+        hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, qs, state, s->loss_part, lblocks, loss_hist);
+        return check_launch("step_end_kernel");
+    };
+    if (rc == PLT_OK && graph) {
+        // one step captured and replayed: every step-dependent value is in the device state
+        hipStream_t cs = nullptr;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = set_error(PLT_EHIP, "plt_surrogate_train: capture failed to start");
+        } else {
+            rc = step(cs);
+            if (hipStreamEndCapture(cs, &g) != hipSuccess || !g) rc = rc < 0 ? rc : set_error(PLT_EHIP, "plt_surrogate_train: capture failed");
+            if (rc == PLT_OK && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess)
+                rc = set_error(PLT_EHIP, "plt_surrogate_train: graph instantiation failed");
+            for (uint64_t k = 0; rc == PLT_OK && k < total; ++k)
+                if (hipGraphLaunch(exec, st) != hipSuccess) rc = set_error(PLT_EHIP, "plt_surrogate_train: graph launch failed");
+        }
+        if (hipStreamSynchronize(st) != hipSuccess && rc == PLT_OK) rc = set_error(PLT_EHIP, "plt_surrogate_train: synchronize failed");
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (g) (void)hipGraphDestroy(g);
+        if (cs) (void)hipStreamDestroy(cs);
+    } else {
+        for (uint64_t k = 0; rc == PLT_OK && k < total; ++k) rc = step(st);
+    }
+    std::vector<float> lh(total);
+    if (rc == PLT_OK && (hipMemcpyAsync(lh.data(), loss_hist, total * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                         hipStreamSynchronize(st) != hipSuccess))
+        rc = set_error(PLT_EHIP, "plt_surrogate_train: loss read-back failed");
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(q);
+    if (rc < 0) return rc;
+    for (uint32_t e = 0; e < epochs; ++e) {
+        double acc = 0.0;
+        for (uint32_t k = 0; k < spe; ++k) acc += lh[size_t(e) * spe + k];
+        epoch_loss[e] = float(acc / spe);
+    }
+    s->step += uint32_t(total);
+    return PLT_OK;
 }
 
 // Raw GEMM (C = A * Bt^T, bf16 operands, FP32 output) for tests of the matrix-core kernel.

@@ -17,6 +17,8 @@ struct plt_surrogate {
     __bf16 *Xb = nullptr, *Xt = nullptr, *H1 = nullptr, *H1t = nullptr, *H2 = nullptr, *H2t = nullptr, *H3 = nullptr, *H3t = nullptr;
     __bf16 *dY = nullptr, *dYt = nullptr, *G3 = nullptr, *G3t = nullptr, *G2 = nullptr, *G2t = nullptr, *G1t = nullptr;
     float *Y = nullptr, *part = nullptr, *loss_part = nullptr;
+    float* bias_part = nullptr;       // bias-gradient partials per batch tile
+    uint32_t* train_state = nullptr;  // device-side training step state (plt_surrogate_train)
     // target statistics and the inputs' training box (plt_surrogate_set_stats): 16 + 16 + 22 + 22
     float* stats = nullptr;
     bool has_stats = false;

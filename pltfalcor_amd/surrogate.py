@@ -115,6 +115,12 @@ class Surrogate:
                                                      C.byref(loss) if want_loss else None, None))
         return float(loss.value) if want_loss else None
 
+    def get_stats(self):
+        """The statistics set on the device: (y_mean [16], y_std [16], x_lo [22], x_hi [22])."""
+        arrs = [np.zeros(N_OUT, np.float32), np.zeros(N_OUT, np.float32), np.zeros(N_IN, np.float32), np.zeros(N_IN, np.float32)]
+        _lib.check(self.lib.plt_surrogate_get_stats(self.handle, *[_fp(a) for a in arrs]))
+        return arrs
+
     def set_stats(self, y_mean, y_std, x_lo, x_hi):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (y_mean, y_std, x_lo, x_hi)]
         _lib.check(self.lib.plt_surrogate_set_stats(self.handle, *[_fp(a) for a in arrs]))
@@ -144,47 +150,34 @@ def fit_stats(X, M):
     return mean, std, X.min(0), X.max(0)
 
 
-def train(sur: Surrogate, samples: ExactSamples, epochs: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0,
-          log=None):
-    """Standardises the targets of `samples` (valid M00 > 0 only), sets the statistics, and runs
-    `epochs` passes of Adam with a cosine-decayed learning rate. Returns the per-epoch mean loss
-    (sampled every 16 batches)."""
+def prepare(sur: Surrogate, samples: ExactSamples):
+    """The training set on the device (plt_surrogate_prepare): standardised targets T [n, 16], the
+    valid rows' indices (the first n_valid of idx), and the statistics set on `sur` (also kept as
+    sur.stats). Returns (T, idx, n_valid)."""
     import torch
-    M, X = samples.M.double(), samples.X
-    ok = torch.isfinite(M).all(1) & (M[:, 0] > 1e-12)
-    m00 = torch.where(ok, M[:, 0], torch.ones_like(M[:, 0]))
-    y = torch.asinh(M / m00[:, None])
-    y[:, 0] = torch.log(m00)
-    yv = y[ok]
-    mean, std = yv.mean(0), yv.std(0, unbiased=False)
-    std = torch.where(std > 1e-6, std, torch.ones_like(std)) / torch.tensor(OUTPUT_SCALE, device=std.device)
-    Xv = X[ok]
-    sur.set_stats(mean.cpu().numpy(), std.cpu().numpy(), Xv.min(0).values.cpu().numpy(), Xv.max(0).values.cpu().numpy())
-    T = ((y - mean) / std).float()
-    del y, yv, Xv, M
-    idx = torch.nonzero(ok).flatten()
-    gen = torch.Generator(device=idx.device).manual_seed(seed)
-    n = (idx.numel() // batch) * batch
-    steps_total = epochs * (n // batch)
-    step = 0
-    losses = []
-    for ep in range(epochs):
-        perm = idx[torch.randperm(idx.numel(), generator=gen, device=idx.device)[:n]]
-        ep_loss = []
-        for b in range(0, n, batch):
-            sel = perm[b:b + batch]
-            xb = X.index_select(0, sel).contiguous()
-            tb = T.index_select(0, sel).contiguous()
-            lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / max(1, steps_total)))
-            want = (b // batch) % 16 == 0
-            l = sur.train_step(xb, tb, max(lr_t, 1e-6), want_loss=want)
-            if want:
-                ep_loss.append(l)
-            step += 1
-        losses.append(float(np.mean(ep_loss)))
-        if log:
-            log(f"epoch {ep}: loss {losses[-1]:.5f}")
-    return losses
+    n = samples.n
+    T = torch.empty((n, N_OUT), dtype=torch.float32, device=samples.X.device)
+    idx = torch.empty(n, dtype=torch.int32, device=samples.X.device)
+    nv = C.c_uint32(0)
+    _lib.check(sur.lib.plt_surrogate_prepare(sur.handle, n, _p(samples.X), _p(samples.M), _p(T), _p(idx), C.byref(nv), None))
+    sur.stats = sur.get_stats()
+    return T, idx, int(nv.value)
 
 
-__all__ = ["Surrogate", "ExactSamples", "train", "fit_stats", "mueller_to_targets", "targets_to_mueller", "FAMILIES"]
+def train(sur: Surrogate, samples: ExactSamples, epochs: int, batch: int = 16384, lr: float = 2e-3, seed: int = 0,
+          log=None, graph: bool = True):
+    """Standardises the targets of `samples` (valid M00 > 0 only) and sets the statistics
+    (`prepare`), then runs `epochs` passes of Adam with a cosine-decayed learning rate in the
+    library (`plt_surrogate_train`: shuffle, gather and every step on the device, one step captured
+    as a HIP graph and replayed when `graph`). Returns the per-epoch mean batch loss."""
+    T, idx, nv = prepare(sur, samples)
+    losses = np.zeros(epochs, np.float32)
+    _lib.check(sur.lib.plt_surrogate_train(sur.handle, nv, _p(idx), _p(samples.X), _p(T), batch, epochs, lr, seed,
+                                           1 if graph else 0, _fp(losses), None))
+    if log:
+        for ep, l in enumerate(losses):
+            log(f"epoch {ep}: loss {l:.5f}")
+    return [float(x) for x in losses]
+
+
+__all__ = ["Surrogate", "ExactSamples", "train", "prepare", "fit_stats", "mueller_to_targets", "targets_to_mueller", "FAMILIES"]

@@ -325,3 +325,61 @@ def test_gpu_surrogate_network_in_render_against_exact(gpu):
     print(f"network render: mean |difference| / mean {err:.4f}; image means exact {a.mean():.5f} network {b.mean():.5f}")
     assert err < 0.5
     s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_prepare_matches_host_pipeline(gpu):
+    """plt_surrogate_prepare against the same pipeline on the host (float64): the valid rows (finite,
+    M00 > 1e-12) in row order, the targets' mean and (population) standard deviation divided by the
+    output weights, the valid inputs' box, and the standardised targets."""
+    from pltfalcor_amd.surrogate import OUTPUT_SCALE, ExactSamples, Surrogate, mueller_to_targets, prepare
+    ctx, _ = _ctx_with_drude()
+    S = ExactSamples(ctx, 1 << 16, seed=31)
+    s = Surrogate(ctx, max_batch=1024, seed=1)
+    T, idx, nv = prepare(s, S)
+    M = S.M.cpu().numpy().astype(np.float64)
+    X = S.X.cpu().numpy()
+    ok = np.isfinite(M).all(1) & (M[:, 0] > 1e-12)
+    assert nv == ok.sum() and nv > 0.5 * len(ok)
+    assert np.array_equal(idx.cpu().numpy()[:nv], np.nonzero(ok)[0])
+    y = mueller_to_targets(M[ok])
+    mean, sd = y.mean(0), y.std(0)
+    sd = np.where(sd > 1e-6, sd, 1.0) / OUTPUT_SCALE
+    ym, ys, lo, hi = s.stats
+    assert np.allclose(ym, mean, rtol=1e-6, atol=1e-6) and np.allclose(ys, sd, rtol=1e-6)
+    assert np.array_equal(lo, X[ok].min(0)) and np.array_equal(hi, X[ok].max(0))
+    Tg = T.cpu().numpy()[ok]
+    assert np.allclose(Tg, (y - mean) / sd, rtol=1e-5, atol=1e-5)
+    s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_surrogate_graph_training_equals_eager(gpu):
+    """plt_surrogate_train with one step captured as a graph and replayed gives the parameters and
+    losses of the same steps launched eagerly, bit for bit (every step-dependent value lives in
+    device state), and the epoch order is a permutation that changes with the epoch."""
+    import ctypes as C
+    import torch
+    from pltfalcor_amd import _lib
+    from pltfalcor_amd.surrogate import ExactSamples, Surrogate, train
+    ctx, _ = _ctx_with_drude()
+    S = ExactSamples(ctx, 1 << 15, seed=32)
+    runs = []
+    for graph in (True, False):
+        s = Surrogate(ctx, max_batch=2048, seed=5)
+        losses = train(s, S, epochs=3, batch=2048, lr=2e-3, seed=7, graph=graph)
+        runs.append((s.get_params(), losses))
+        s.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert runs[0][1][-1] < runs[0][1][0]
+    lib = _lib.load()
+    n = 30001
+    orders = []
+    for epoch in (0, 1):
+        out = torch.empty(n, dtype=torch.int32, device="cuda")
+        _lib.check(lib.plt_surrogate_shuffle(n, 7, epoch, C.c_void_p(out.data_ptr()), None))
+        torch.cuda.synchronize()
+        o = out.cpu().numpy()
+        assert np.array_equal(np.sort(o), np.arange(n))
+        orders.append(o)
+    assert (orders[0] != orders[1]).mean() > 0.9
