@@ -163,15 +163,37 @@ int gemm(const GemmArgs& g, int splits, hipStream_t st) {
 
 // ---------------------------------------------------------------------------------------
 // Fused inference: all four layers in one persistent kernel. A workgroup of four waves loads
-// the bf16 weights once into registers -- wave w holds the B fragments of output columns
-// 32w..32w+31 of layers 1-3 (40 VGPRs... 72 with layer 1) and all of layer 4 (16 VGPRs) -- and
-// then walks 64-sample tiles: each layer reads its input rows from LDS (A fragments), multiplies
-// by its register-resident columns, and writes ReLU(acc + bias) back to LDS for the next layer.
+// the bf16 weights once into registers -- wave w holds the fragments of output features
+// 32w..32w+31 of layers 1-3 and all of layer 4 -- and then walks 64-sample tiles. Each product
+// is computed transposed, out[feature x sample] = W x in^T: the weights are the MFMA's A operand
+// (registers), the tile's activations the B operand (LDS rows of 8 consecutive features), and the
+// bias is the accumulator's initial value. The D layout then gives each lane four consecutive
+// features of one sample, written to the next layer's LDS rows as one 8-byte store, and the
+// output layer's four consecutive outputs of a sample are one Mueller row.
 // ---------------------------------------------------------------------------------------
 constexpr int kFTile = 64, kFThreads = 256;
 constexpr int kXRow = kInPad + 8, kHRow = kHid + 8;  // padded LDS rows (bank spread, 16 B aligned)
 
 __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// ReLU of an MFMA result as one integer max on its bits: negative floats (sign bit set, -0
+// included) read as negative integers and become +0; fmaxf would add a NaN canonicalisation.
+__device__ __forceinline__ float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
+// sinh for the decode: (e^y - e^-y) / 2 from one hardware exponential where |y| > 1/2, the Taylor
+// series through y^9 below (relative error under 1e-8 there, so no cancellation).
+__device__ __forceinline__ float sinh_fast(float y) {
+    const float a = fabsf(y);
+    if (a > 0.5f) {
+        const float e = __expf(a);
+        return copysignf(0.5f * (e - 1.f / e), y);
+    }
+    const float y2 = y * y;
+    return y * (1.f + y2 * (1.f / 6.f + y2 * (1.f / 120.f + y2 * (1.f / 5040.f + y2 * (1.f / 362880.f)))));
+}
 
 // With Sout: instead of writing the raw outputs, each sample inside the training box
 // (stats[32..54) / [54..76)) has its Mueller matrix decoded (M00 = exp(y0), Mk = M00 sinh(yk)
@@ -191,8 +213,8 @@ __global__ void __launch_bounds__(kFThreads) mlp_infer_kernel(int n, const float
     __shared__ __bf16 hb[kFTile * kHRow];
     __shared__ int outside[kFTile];  // sample outside the training box (apply mode)
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
-    // register-resident weights: B fragment (col block cb, k block kb) = W[col][32 kb + fk .. +7]
+    const int fr = lane & 15, fg = lane >> 4, fk = 8 * fg;
+    // register-resident A fragments: (feature block cb, k block kb) = W[feature][32 kb + fk .. +7]
     bf16x8 w1[2], w2[2][4], w3[2][4], w4[4];
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
@@ -206,90 +228,123 @@ __global__ void __launch_bounds__(kFThreads) mlp_infer_kernel(int n, const float
     }
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) w4[kb] = *reinterpret_cast<const bf16x8*>(W4 + size_t(fr) * kHid + 32 * kb + fk);
+    // the biases of the lane's D rows: features 32w + 16cb + 4fg + r, outputs 4fg + r
     const float* b1 = params + kHid * kIn;
     const float* b2 = b1 + kHid + kHid * kHid;
     const float* b3 = b2 + kHid + kHid * kHid;
     const float* b4 = b3 + kHid + kOut * kHid;
+    f32x4 c1[2], c2[2], c3[2], c4;
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int f = 32 * w + 16 * cb + 4 * fg + r;
+            c1[cb][r] = b1[f];
+            c2[cb][r] = b2[f];
+            c3[cb][r] = b3[f];
+        }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) c4[r] = b4[4 * fg + r];
+    // staging: thread t always loads input features 2p, 2p + 1 (p = t % 16) of rows t / 16 + 16 j
+    // (j < 4) as one 8-byte load, so its box bounds are loaded once; the pair is one 4-byte LDS store
+    const int sp = t & 15, sr = t >> 4;
+    const bool inPair = 2 * sp < kIn;  // kIn is even: a pair is all in or all padding
+    float blo[2] = {0.f, 0.f}, bhi[2] = {0.f, 0.f};
+    if (Sout && inPair)
+        for (int c = 0; c < 2; ++c) {
+            blo[c] = stats[2 * kOut + 2 * sp + c];
+            bhi[c] = stats[2 * kOut + kIn + 2 * sp + c];
+        }
     const int ntiles = (n + kFTile - 1) / kFTile;
+    // the thread's four input pairs of a tile, loaded back to back (the row index is clamped so every
+    // load is in bounds; rows past n and padding features are zeroed when staged); the next
+    // tile's are fetched while this one is computed
+    const int lp = inPair ? sp : 0;
+    float2 xv[kFTile / 16];
+    auto fetch = [&](int tile) {
+        const int r0 = tile * kFTile;
+#pragma unroll
+        for (int j = 0; j < kFTile / 16; ++j)
+            xv[j] = *reinterpret_cast<const float2*>(X + size_t(min(r0 + sr + 16 * j, n - 1)) * kIn + 2 * lp);
+    };
+    if (int(blockIdx.x) < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int r0 = tile * kFTile;
-        // stage the inputs as bf16 (zero padded rows and features); flag samples outside the box
-        if (t < kFTile) outside[t] = 0;
-        __syncthreads();
-        for (int e = t; e < kFTile * kInPad; e += kFThreads) {
-            const int r = e / kInPad, f = e % kInPad;
-            const float v = (r0 + r < n && f < kIn) ? X[size_t(r0 + r) * kIn + f] : 0.f;
-            xs[r * kXRow + f] = static_cast<__bf16>(v);
-            if (Sout && f < kIn && !(v >= stats[2 * kOut + f] && v <= stats[2 * kOut + kIn + f])) outside[r] = 1;
+#pragma unroll
+        for (int j = 0; j < kFTile / 16; ++j) {
+            const int r = sr + 16 * j;
+            const bool live = r0 + r < n && inPair;
+            const float v0 = live ? xv[j].x : 0.f, v1 = live ? xv[j].y : 0.f;
+            bf16x2 pv;
+            pv[0] = static_cast<__bf16>(v0);
+            pv[1] = static_cast<__bf16>(v1);
+            *reinterpret_cast<bf16x2*>(xs + r * kXRow + 2 * sp) = pv;
+            if (Sout) {
+                // a wave stages four rows (16 lanes each): a row is outside when any lane's pair is
+                const bool out = live && !(v0 >= blo[0] && v0 <= bhi[0] && v1 >= blo[1] && v1 <= bhi[1]);
+                const uint64_t m = __ballot(out);
+                if ((lane & 15) == 0) outside[r] = ((m >> (lane & 48)) & 0xFFFFull) != 0 ? 1 : 0;
+            }
         }
         __syncthreads();
-        // hidden layers: wave w computes columns 32w..32w+31 of all 64 rows
-        auto hidden = [&](const __bf16* in, int in_row, int kblocks, const bf16x8 (*wf)[4], const bf16x8* wf1, const float* bias,
+        if (tile + int(gridDim.x) < ntiles) fetch(tile + gridDim.x);
+        // hidden layers: wave w computes features 32w..32w+31 of all 64 samples
+        auto hidden = [&](const __bf16* in, int in_row, int kblocks, const bf16x8 (*wf)[4], const bf16x8* wf1, const f32x4* bias,
                           __bf16* out) {
-            f32x4 acc[4][2];
+            f32x4 acc[2][4];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-                for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int sb = 0; sb < 4; ++sb) acc[cb][sb] = bias[cb];
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+            for (int sb = 0; sb < 4; ++sb)
 #pragma unroll
                 for (int kb = 0; kb < 4; ++kb) {
                     if (kb >= kblocks) break;
-                    const bf16x8 a = *reinterpret_cast<const bf16x8*>(in + (16 * rb + fr) * in_row + 32 * kb + fk);
+                    const bf16x8 bfrag = *reinterpret_cast<const bf16x8*>(in + (16 * sb + fr) * in_row + 32 * kb + fk);
 #pragma unroll
-                    for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = mfma(a, wf ? wf[cb][kb] : wf1[cb], acc[rb][cb]);
+                    for (int cb = 0; cb < 2; ++cb) acc[cb][sb] = mfma(wf ? wf[cb][kb] : wf1[cb], bfrag, acc[cb][sb]);
                 }
 #pragma unroll
-            for (int cb = 0; cb < 2; ++cb) {
-                const int col = 32 * w + 16 * cb + fr;
-                const float bv = bias[col];
+            for (int cb = 0; cb < 2; ++cb)
 #pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
+                for (int sb = 0; sb < 4; ++sb) {
+                    bf16x4 h;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        out[(16 * rb + 4 * (lane >> 4) + r) * kHRow + col] = static_cast<__bf16>(fmaxf(acc[rb][cb][r] + bv, 0.f));
-            }
+                    for (int r = 0; r < 4; ++r) h[r] = static_cast<__bf16>(relu_bits(acc[cb][sb][r]));
+                    *reinterpret_cast<bf16x4*>(out + (16 * sb + fr) * kHRow + 32 * w + 16 * cb + 4 * fg) = h;
+                }
         };
-        hidden(xs, kXRow, 1, nullptr, w1, b1, ha);
+        hidden(xs, kXRow, 1, nullptr, w1, c1, ha);
         __syncthreads();
-        hidden(ha, kHRow, 4, w2, nullptr, b2, hb);
+        hidden(ha, kHRow, 4, w2, nullptr, c2, hb);
         __syncthreads();
-        hidden(hb, kHRow, 4, w3, nullptr, b3, ha);
+        hidden(hb, kHRow, 4, w3, nullptr, c3, ha);
         __syncthreads();
-        // output layer: wave w computes rows 16w..16w+15, all 16 outputs
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        // output layer: wave w computes samples 16w..16w+15, all 16 outputs; lane (fr, fg) holds
+        // outputs 4fg..4fg+3 of sample 16w + fr: row fg of its Mueller matrix
+        f32x4 acc = c4;
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) acc = mfma(*reinterpret_cast<const bf16x8*>(ha + (16 * w + fr) * kHRow + 32 * kb + fk), w4[kb], acc);
+        for (int kb = 0; kb < 4; ++kb) acc = mfma(w4[kb], *reinterpret_cast<const bf16x8*>(ha + (16 * w + fr) * kHRow + 32 * kb + fk), acc);
+        const int srow = 16 * w + fr, row = r0 + srow;
         if (!Sout) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + 16 * w + 4 * (lane >> 4) + r;
-                if (row < n) Y[size_t(row) * kOut + fr] = acc[r] + b4[fr];
-            }
+            if (row < n) *reinterpret_cast<f32x4*>(Y + size_t(row) * kOut + 4 * fg) = acc;
         } else {
-            // gather each sample's 16 outputs in one lane through LDS (the input tile is no longer needed)
-            float* ys = reinterpret_cast<float*>(xs);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) ys[(16 * w + 4 * (lane >> 4) + r) * kOut + fr] = acc[r] + b4[fr];
-            __syncthreads();
-            // thread t: Stokes component r = t % 4 of sample t / 4
-            const int lr = t >> 2, r = t & 3, row = r0 + lr;
-            if (row < n && !outside[lr]) {
-                const float* yr = ys + lr * kOut;
-                const float m00 = expf(yr[0] * stats[kOut] + stats[0]);
+            // M00 = exp(y0) from the lane holding output 0 of the same sample (fg = 0)
+            const float m00 = __shfl(__expf(acc[0] * stats[kOut] + stats[0]), fr);
+            if (row < n && !outside[srow]) {
                 const float* S = Sin + size_t(row) * 4;
                 float acc4 = 0.f;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const int k = 4 * r + c;
-                    const float v = k == 0 ? 1.f : sinhf(yr[k] * stats[kOut + k] + stats[k]);
+                    const int k = 4 * fg + c;
+                    const float v = k == 0 ? 1.f : sinh_fast(acc[c] * stats[kOut + k] + stats[k]);
                     acc4 += v * S[c];
                 }
-                Sout[size_t(row) * 4 + r] = m00 * acc4;
+                Sout[size_t(row) * 4 + fg] = m00 * acc4;
             }
         }
-        __syncthreads();  // the next tile overwrites xs and ha
+        __syncthreads();  // the next tile overwrites xs, ha and outside
     }
 }
 
