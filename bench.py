@@ -79,6 +79,9 @@ def parse_args():
     ap.add_argument("--solve", choices=("megakernel", "split", "autotune"), default="autotune",
                     help="solve-pass schedule; autotune (default) times the candidate launch configurations on "
                          "the workload before the warm-up and keeps the fastest")
+    ap.add_argument("--train-epochs", type=int, default=6,
+                    help="training epochs of the surrogate (--workload surrogate / c4); 0 keeps the untrained network "
+                         "with its statistics set (timing and counters of the inference kernel without the training)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="override a PLTPT option of the path workload (repeatable), e.g. --option doMNEE=0 to see "
                          "what the manifold walks cost in the per-kernel times; the line then names the overrides")
@@ -181,11 +184,16 @@ def train_surrogate(ctx, n_train: int, epochs: int):
     gen_s = time.perf_counter() - t0
     s = Surrogate(ctx, max_batch=16384, seed=3)
     t0 = time.perf_counter()
-    losses = train(s, tr, epochs=epochs, batch=16384, lr=2e-3)
+    if epochs > 0:
+        losses = train(s, tr, epochs=epochs, batch=16384, lr=2e-3)
+    else:  # statistics only: the untrained network
+        from pltfalcor_amd.surrogate import prepare
+        prepare(s, tr)
+        losses = [float("nan")]
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del tr
-    return s, {"samples": n_train, "epochs": epochs, "seconds": train_s, "samples_per_s": epochs * n_train / train_s,
+    return s, {"samples": n_train, "epochs": epochs, "seconds": train_s, "samples_per_s": epochs * n_train / train_s if epochs else None,
                "final_loss": losses[-1], "exact_sample_generation_s": gen_s}
 
 
@@ -525,7 +533,7 @@ def main():
     if args.workload in ("surrogate", "c4"):
         from pltfalcor_amd.bsdf import default_context
         sctx = default_context(0)
-        trained, training = train_surrogate(sctx, 1 << 24, 6)
+        trained, training = train_surrogate(sctx, 1 << 24, args.train_epochs)
     if args.workload == "c4":
         r = c4_run(sctx, trained, training)
         trained.close()
@@ -545,9 +553,9 @@ def main():
                "warmup": 1, "ms_per_step": (1 << 22) / r["value"] * 1e3, "higher_is_better": True, "scaling": "weak",
                "vs_baseline": None, "dtype": "bf16 (FP32 accumulation)", "data": "synthetic",
                "config": {"workload": "learned BSDF surrogate: MLP 22-128-128-128-16 ReLU trained on 2^24 exact "
-                                      "samples (6 epochs, batch 16384, Adam) of the scattering lobes of the coated "
-                                      "opaque dielectric, coated conductor and grated conductor; evaluated on 2^22 "
-                                      "held-out queries with random partially polarised incident Stokes vectors",
+                                      f"samples ({args.train_epochs} epochs, batch 16384, Adam) of the scattering lobes "
+                                      "of the coated opaque dielectric, coated conductor and grated conductor; evaluated "
+                                      "on 2^22 held-out queries with random partially polarised incident Stokes vectors",
                           "queries": 1 << 22},
                "roofline": r["roofline"], "cpu_baseline": r.get("cpu_baseline"),
                "surrogate": {k: r[k] for k in ("exact_value", "surrogate_over_exact", "error_vs_exact", "training")}}
