@@ -514,36 +514,59 @@ int weight_grad(plt_surrogate* s, int n, const __bf16* Gt, int out, const __bf16
     return gemm(g, splits, st);
 }
 
-// Every gradient from its partials, in one launch: element e < kGradTotal of the weight regions
-// sums its `splits` partials (four interleaved accumulators, combined in a fixed order); the
-// kBiasCols elements after them are the biases (bias_reduce's sums). Layer 1 drops its padding.
-__global__ void grad_reduce_kernel(int splits, int rows_out, int rows_hid, const float* __restrict__ part,
-                                   const float* __restrict__ bias_part, float* __restrict__ grads) {
-    const int e = blockIdx.x * kEw + threadIdx.x;
-    if (e >= kGradTotal + kBiasCols) return;
-    if (e >= kGradTotal) {
-        const int c = e - kGradTotal;
-        const int rows = c < kOut ? rows_out : rows_hid;
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int r = 0; r < rows; ++r) acc[r & 3] += bias_part[size_t(r) * kBiasCols + c];
+// Every gradient from its partials, in one launch. A block takes 64 elements of the
+// [weights..., biases] list with four threads each: thread group g sums the element's partials
+// z = g, g + 4, ... (all loads issued before the adds), and the four sums are combined in order.
+// Weight elements sum their `splits` split-K partials; the kBiasCols bias elements sum their
+// `rows_out` (b4) or `rows_hid` (b3, b2, b1) tile partials. Layer 1 drops its padding.
+constexpr int kReduceLanes = 64, kReduceMax = 16;  // elements per block; partials per thread at most
+__global__ void __launch_bounds__(256) grad_reduce_kernel(int splits, int rows_out, int rows_hid, const float* __restrict__ part,
+                                                          const float* __restrict__ bias_part, float* __restrict__ grads) {
+    __shared__ float sums[4][kReduceLanes];
+    const int el = threadIdx.x & (kReduceLanes - 1), g = threadIdx.x >> 6;
+    const int e = blockIdx.x * kReduceLanes + el;
+    const bool live = e < kGradTotal + kBiasCols;
+    const bool bias = e >= kGradTotal;
+    int count = splits, cnt = 0, local = 0, c = 0;
+    size_t base = 0;
+    if (live && bias) {
+        c = e - kGradTotal;
+        count = c < kOut ? rows_out : rows_hid;
+    } else if (live) {
+        constexpr int c4 = kOut * kHid, c3 = kHid * kHid, c2 = kHid * kHid;
+        const int r = e < c4 ? 0 : e < c4 + c3 ? 1 : e < c4 + c3 + c2 ? 2 : 3;
+        const int first = r == 0 ? 0 : r == 1 ? c4 : r == 2 ? c4 + c3 : c4 + c3 + c2;
+        cnt = r == 0 ? c4 : r == 1 ? c3 : r == 2 ? c2 : kHid * kInPad;
+        local = e - first;
+        base = size_t(first) * kSplitTarget;
+    }
+    float acc = 0.f;
+    if (live) {
+        // up to kReduceMax partials per thread in one batch of loads (weight partials: splits <= 64)
+        for (int z0 = g; z0 < count; z0 += 4 * kReduceMax) {
+            float v[kReduceMax];
+#pragma unroll
+            for (int q = 0; q < kReduceMax; ++q) {
+                const int z = z0 + 4 * q;
+                v[q] = z < count ? (bias ? bias_part[size_t(z) * kBiasCols + c] : part[base + size_t(z) * cnt + local]) : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < kReduceMax; ++q) acc += v[q];
+        }
+    }
+    sums[g][el] = acc;
+    __syncthreads();
+    if (g != 0 || !live) return;
+    const float sum = (sums[0][el] + sums[1][el]) + (sums[2][el] + sums[3][el]);
+    if (bias) {
         const size_t dst = c < kOut ? kOffB4 + c : c < kOut + kHid ? kOffB3 + (c - kOut) : c < kOut + 2 * kHid ? kOffB2 + (c - kOut - kHid)
                                                                                            : kOffB1 + (c - kOut - 2 * kHid);
-        grads[dst] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-        return;
+        grads[dst] = sum;
+    } else if (cnt == kHid * kInPad) {
+        if (local % kInPad < kIn) grads[kOffW1 + (local / kInPad) * kIn + local % kInPad] = sum;
+    } else {
+        grads[(cnt == kOut * kHid ? kOffW4 : e < kOut * kHid + kHid * kHid ? kOffW3 : kOffW2) + local] = sum;
     }
-    constexpr int c4 = kOut * kHid, c3 = kHid * kHid, c2 = kHid * kHid;
-    const int r = e < c4 ? 0 : e < c4 + c3 ? 1 : e < c4 + c3 + c2 ? 2 : 3;
-    const int first = r == 0 ? 0 : r == 1 ? c4 : r == 2 ? c4 + c3 : c4 + c3 + c2;
-    const int cnt = r == 0 ? c4 : r == 1 ? c3 : r == 2 ? c2 : kHid * kInPad;
-    const int local = e - first;
-    const size_t base = size_t(first) * kSplitTarget;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    for (int z = 0; z < splits; ++z) acc[z & 3] += part[base + size_t(z) * cnt + local];
-    const float sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-    if (r == 0) grads[kOffW4 + local] = sum;
-    else if (r == 1) grads[kOffW3 + local] = sum;
-    else if (r == 2) grads[kOffW2 + local] = sum;
-    else if (local % kInPad < kIn) grads[kOffW1 + (local / kInPad) * kIn + local % kInPad] = sum;
 }
 
 // Loss and gradients of n samples (X packed, T [n x 16] FP32 targets) into s->grads.
@@ -576,7 +599,7 @@ int loss_and_grads(plt_surrogate* s, int n, const float* T, float* loss_host, hi
     // W1's product is [128 x 32]; the reduction keeps the 22 real inputs
     if ((rc = weight_grad(s, n, s->G1t, kHid, s->Xt, kInPad, r1, st)) < 0) return rc;
     const int splits = (n + split_len(n) - 1) / split_len(n);
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3((kGradTotal + kBiasCols + kEw - 1) / kEw), dim3(kEw), 0, st, splits, blocks,
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((kGradTotal + kBiasCols + kReduceLanes - 1) / kReduceLanes), dim3(256), 0, st, splits, blocks,
                        (n + kTile - 1) / kTile, s->part, s->bias_part, s->grads);
     if ((rc = check_launch("grad_reduce_kernel")) < 0) return rc;
     if (loss_host) {
@@ -649,36 +672,77 @@ __global__ void __launch_bounds__(kEw) prep_sums_kernel(int n, const float* __re
     }
 }
 
-// One block: the means, the box (into stats), the total valid count, and each block's first slot in
-// the compacted index list (exclusive scan of the counts). Fixed-order sums.
-__global__ void __launch_bounds__(kEw) prep_mean_kernel(int blocks, const double* __restrict__ psum, const float* __restrict__ plo,
-                                                        const float* __restrict__ phi, uint32_t* __restrict__ pcount, double* __restrict__ dstat,
-                                                        float* __restrict__ stats, uint32_t* __restrict__ n_valid) {
+// One block of 1024 threads: the means, the box (into stats), the total valid count, and each
+// block's first slot in the compacted index list. Every sum splits the blocks into contiguous
+// chunks, one per thread, and combines the chunk results in chunk order (deterministic).
+constexpr int kPrepThreads = 1024;
+__global__ void __launch_bounds__(kPrepThreads) prep_mean_kernel(int blocks, const double* __restrict__ psum, const float* __restrict__ plo,
+                                                                 const float* __restrict__ phi, uint32_t* __restrict__ pcount,
+                                                                 double* __restrict__ dstat, float* __restrict__ stats,
+                                                                 uint32_t* __restrict__ n_valid) {
+    __shared__ uint32_t chunk_count[kPrepThreads];
+    __shared__ double col[kPrepThreads / kOut][kOut];
+    __shared__ float lo_s[kPrepThreads / 32][kIn], hi_s[kPrepThreads / 32][kIn];
     const int t = threadIdx.x;
+    // counts: chunk t of the blocks, then an exclusive scan over the chunks
+    const int per = (blocks + kPrepThreads - 1) / kPrepThreads;
+    const int b0 = min(blocks, t * per), b1 = min(blocks, b0 + per);
+    uint32_t c = 0;
+    for (int b = b0; b < b1; ++b) c += pcount[b];
+    chunk_count[t] = c;
+    // column sums: column t % 16 over chunk t / 16
+    {
+        const int cc = t % kOut, q = t / kOut, nq = kPrepThreads / kOut, pq = (blocks + nq - 1) / nq;
+        double acc = 0.0;
+        for (int b = q * pq; b < min(blocks, (q + 1) * pq); ++b) acc += psum[size_t(b) * kOut + cc];
+        col[q][cc] = acc;
+    }
+    // box: feature t % 32 (< 22) over chunk t / 32
+    {
+        const int f = t % 32, q = t / 32, nq = kPrepThreads / 32, pq = (blocks + nq - 1) / nq;
+        if (f < kIn) {
+            float lo = __builtin_inff(), hi = -__builtin_inff();
+            for (int b = q * pq; b < min(blocks, (q + 1) * pq); ++b) {
+                lo = fminf(lo, plo[size_t(b) * kIn + f]);
+                hi = fmaxf(hi, phi[size_t(b) * kIn + f]);
+            }
+            lo_s[q][f] = lo;
+            hi_s[q][f] = hi;
+        }
+    }
+    __syncthreads();
     __shared__ uint32_t total;
     if (t == 0) {
         uint32_t run = 0;
-        for (int b = 0; b < blocks; ++b) {
-            const uint32_t c = pcount[b];
-            pcount[b] = run;
-            run += c;
+        for (int q = 0; q < kPrepThreads; ++q) {
+            const uint32_t x = chunk_count[q];
+            chunk_count[q] = run;
+            run += x;
         }
         total = run;
         *n_valid = run;
     }
     __syncthreads();
+    {
+        uint32_t run = chunk_count[t];
+        for (int b = b0; b < b1; ++b) {
+            const uint32_t x = pcount[b];
+            pcount[b] = run;
+            run += x;
+        }
+    }
     if (t < kOut) {
         double acc = 0.0;
-        for (int b = 0; b < blocks; ++b) acc += psum[size_t(b) * kOut + t];
+        for (int q = 0; q < kPrepThreads / kOut; ++q) acc += col[q][t];
         const double mean = total ? acc / double(total) : 0.0;
         dstat[t] = mean;
         stats[t] = float(mean);
     } else if (t >= 32 && t < 32 + kIn) {
         const int f = t - 32;
         float lo = __builtin_inff(), hi = -__builtin_inff();
-        for (int b = 0; b < blocks; ++b) {
-            lo = fminf(lo, plo[size_t(b) * kIn + f]);
-            hi = fmaxf(hi, phi[size_t(b) * kIn + f]);
+        for (int q = 0; q < kPrepThreads / 32; ++q) {
+            lo = fminf(lo, lo_s[q][f]);
+            hi = fmaxf(hi, hi_s[q][f]);
         }
         stats[2 * kOut + f] = lo;
         stats[2 * kOut + kIn + f] = hi;
@@ -718,15 +782,22 @@ __global__ void __launch_bounds__(kEw) prep_dev_kernel(int n, const float* __res
     }
 }
 
-// One block: the standard deviations (population), scaled by the output weights, into stats.
-__global__ void prep_std_kernel(int blocks, const double* __restrict__ psum, const uint32_t* __restrict__ n_valid,
-                                double* __restrict__ dstat, float* __restrict__ stats) {
+// One block of 1024 threads: the standard deviations (population), scaled by the output weights,
+// into stats (chunked sums combined in chunk order).
+__global__ void __launch_bounds__(kPrepThreads) prep_std_kernel(int blocks, const double* __restrict__ psum, const uint32_t* __restrict__ n_valid,
+                                                                double* __restrict__ dstat, float* __restrict__ stats) {
+    __shared__ double col[kPrepThreads / kOut][kOut];
     const int t = threadIdx.x;
-    if (t >= kOut) return;
+    const int cc = t % kOut, q = t / kOut, nq = kPrepThreads / kOut, pq = (blocks + nq - 1) / nq;
     double acc = 0.0;
-    for (int b = 0; b < blocks; ++b) acc += psum[size_t(b) * kOut + t];
+    for (int b = q * pq; b < min(blocks, (q + 1) * pq); ++b) acc += psum[size_t(b) * kOut + cc];
+    col[q][cc] = acc;
+    __syncthreads();
+    if (t >= kOut) return;
+    double sum = 0.0;
+    for (int k = 0; k < nq; ++k) sum += col[k][t];
     const uint32_t c = *n_valid;
-    double sd = c ? sqrt(acc / double(c)) : 1.0;
+    double sd = c ? sqrt(sum / double(c)) : 1.0;
     sd = (sd > 1e-6 ? sd : 1.0) / double(kOutputScale[t]);
     dstat[kOut + t] = sd;
     stats[kOut + t] = float(sd);
@@ -1042,9 +1113,9 @@ int plt_surrogate_prepare(plt_surrogate* s, uint32_t n, const float* X, const fl
     uint32_t* nv = pcount + blocks;
     int rc = PLT_OK;
     hipLaunchKernelGGL(prep_sums_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), X, M, psum, plo, phi, pcount);
-    hipLaunchKernelGGL(prep_mean_kernel, dim3(1), dim3(kEw), 0, st, blocks, psum, plo, phi, pcount, dstat, s->stats, nv);
+    hipLaunchKernelGGL(prep_mean_kernel, dim3(1), dim3(kPrepThreads), 0, st, blocks, psum, plo, phi, pcount, dstat, s->stats, nv);
     hipLaunchKernelGGL(prep_dev_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), M, dstat, pcount, psum, idx);
-    hipLaunchKernelGGL(prep_std_kernel, dim3(1), dim3(64), 0, st, blocks, psum, nv, dstat, s->stats);
+    hipLaunchKernelGGL(prep_std_kernel, dim3(1), dim3(kPrepThreads), 0, st, blocks, psum, nv, dstat, s->stats);
     hipLaunchKernelGGL(prep_targets_kernel, dim3(blocks), dim3(kEw), 0, st, int(n), M, dstat, T);
     rc = check_launch("plt_surrogate_prepare kernels");
     if (rc == PLT_OK && (hipMemcpyAsync(n_valid, nv, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
