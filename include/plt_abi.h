@@ -476,11 +476,12 @@ int plt_pass_get_launch_config(const plt_pass* pass, plt_launch_config* config);
 int plt_pass_set_graph(plt_pass* pass, int enable);
 /* Autotuner: renders `frames` frames (from the current frame counter, outputs in `data`) with the
  * current configuration and with each candidate -- the sample megakernel at 64, 128 and 256 threads
- * per block, the wavefront sample schedule at 4, 8, 16 and 32 blocks per CU; then, with the fastest
- * sample pass, the solve megakernel and the split solve pass at 4, 8 and 16 events per path; then
- * the fastest of these replayed from graphs -- keeps the fastest, and restores the frame counter
- * and the jitter pattern. Writes the chosen configuration and the mean frame time of the starting
- * and the chosen one (ms). */
+ * per block, the wavefront sample schedule at 4, 8, 16 and 32 blocks per CU, compared by the sample
+ * pass's own time; then, with the fastest sample pass, the solve megakernel and the split solve
+ * pass at 4, 8 and 16 events per path; then the fastest of these replayed from graphs, compared by
+ * the frame time -- keeps the fastest (never one slower than the starting configuration as
+ * measured), and restores the frame counter and the jitter pattern. Writes the chosen configuration
+ * and the mean frame time of the starting and the chosen one (ms). */
 int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t frames, plt_stream stream,
                       plt_launch_config* best, double* default_ms, double* best_ms);
 

@@ -816,17 +816,20 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
         cands.push_back({PLT_SCHEDULE_WAVEFRONT, start.sample_block, g, 0u, start.solve_schedule, start.solve_events_per_path,
                          start.solve_hits_per_path});
     const hipStream_t st = static_cast<hipStream_t>(stream);
-    auto time_config = [&](const plt_launch_config& c, double& ms) -> int {
+    // mean frame time of a configuration, and of its sample pass alone (the pass's kernel timing)
+    auto time_config = [&](const plt_launch_config& c, double& ms, double* sample_ms = nullptr) -> int {
         int rc = plt_pass_set_launch_config(p, &c);
         if (rc < 0) return rc;
         p->frame = frame0;
         p->pattern = pattern0;
+        p->profiling = false;
         if ((rc = plt_pass_execute(p, rd, stream)) < 0) return rc;  // warm-up (and capture)
         hipEvent_t e0, e1;
         if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return set_error(PLT_EHIP, "plt_pass_autotune: hipEventCreate failed");
         p->frame = frame0;
         p->pattern = pattern0;
-        p->profiling = false;
+        p->profiling = sample_ms != nullptr;
+        p->events_used = 0;
         (void)hipEventRecord(e0, st);
         for (uint32_t f = 0; f < frames && rc >= 0; ++f) rc = plt_pass_execute(p, rd, stream);
         (void)hipEventRecord(e1, st);
@@ -836,21 +839,34 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         ms = double(t) / frames;
+        if (rc >= 0 && sample_ms) {
+            double smp = 0.0, solve = 0.0;
+            uint32_t ex = 0;
+            rc = plt_pass_get_kernel_times(p, &smp, &solve, &ex);
+            *sample_ms = ex ? smp / ex : 0.0;
+        }
+        p->profiling = false;
         return rc;
     };
-    double dms = 0.0, bms = 1e300;
-    plt_launch_config bc = start;
-    int rc = time_config(start, dms);
+    // the sample pass first, by its own time (the solve pass's run-to-run spread would hide its
+    // differences), then the solve pass and graph replay by the frame time; the choice is never
+    // slower than the starting configuration as measured
+    double dms = 0.0, bms = 0.0, dsm = 0.0, bsm = 0.0;
+    plt_launch_config bs = start;
+    int rc = time_config(start, dms, &dsm);
+    bsm = dsm;
     for (const auto& c : cands) {
         if (rc < 0) break;
-        double ms = 0.0;
-        rc = time_config(c, ms);
-        if (rc >= 0 && ms < bms) { bms = ms; bc = c; }
+        double ms = 0.0, sm = 0.0;
+        rc = time_config(c, ms, &sm);
+        if (rc >= 0 && sm < bsm) { bsm = sm; bs = c; }
     }
+    bms = dms;
+    plt_launch_config bc = start;
     // the solve pass's schedule and event-area size, with the fastest sample pass
     if (rc >= 0) {
         std::vector<plt_launch_config> solve;
-        plt_launch_config c = bc;
+        plt_launch_config c = bs;
         c.solve_schedule = PLT_SOLVE_MEGAKERNEL;
         solve.push_back(c);
         for (uint32_t e : {4u, 8u, 16u}) {
