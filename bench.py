@@ -82,6 +82,9 @@ def parse_args():
     ap.add_argument("--train-epochs", type=int, default=6,
                     help="training epochs of the surrogate (--workload surrogate / c4); 0 keeps the untrained network "
                          "with its statistics set (timing and counters of the inference kernel without the training)")
+    ap.add_argument("--bvh", choices=("host", "device"), default="host",
+                    help="the scene's BVH: the host builder's (binned SAH, made at scene creation) or rebuilt on "
+                         "the GPU (PLOC)")
     ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
                     help="override a PLTPT option of the path workload (repeatable), e.g. --option doMNEE=0 to see "
                          "what the manifold walks cost in the per-kernel times; the line then names the overrides")
@@ -571,6 +574,11 @@ def main():
     t_build = time.perf_counter()
     scene = Scene(ctx, aspect=WIDTH / HEIGHT, **PATH_2)
     t_build = time.perf_counter() - t_build
+    bvh = {"host": scene.bvh_info()}
+    if args.bvh == "device":
+        scene.build_bvh("device")
+        bvh["device"] = scene.bvh_info()
+    bvh["used"] = args.bvh
     # the reference's PLTPathTracer graph creates PLTPT with its defaults (manifold NEE on, 2 occluders,
     # 60 Newton iterations, threshold 5e-5); only the depth and the HWSS count are set here
     opts = {"maxBounces": MAX_BOUNCES, "HWSS": 4}
@@ -708,7 +716,7 @@ def main():
                                "jitter; FP64/FP32 accumulation and auto-exposure tone mapping per frame",
                    "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_bounces": MAX_BOUNCES,
                    "triangles": int(scene.arrays.n_triangles), "scene_build_s": round(t_build, 3),
-                   "segments_per_frame": seg_per_exec, "option_overrides": overrides or None},
+                   "segments_per_frame": seg_per_exec, "option_overrides": overrides or None, "bvh": bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kname, "kernel_ms": kms,
                      "algorithmic_bytes_per_launch": kbytes,

@@ -248,8 +248,9 @@ typedef struct plt_camera {
 
 /* ---------------------------------------------------------------------------------------
  * Scenes. A scene is triangle meshes (world space, one material per mesh), spectral profiles,
- * materials and a camera, with a software BVH built natively on the host (binned SAH) and kept
- * in HBM. Replaces the reference's DXR BLAS/TLAS build (Scene/Scene.cpp:3092, :3624).
+ * materials and a camera, with a software BVH kept in HBM, built on the host (binned SAH) or on
+ * the GPU (plt_scene_build_bvh). Replaces the reference's DXR BLAS/TLAS build (Scene/Scene.cpp:3092,
+ * :3624).
  * ------------------------------------------------------------------------------------- */
 typedef struct plt_scene plt_scene;
 
@@ -270,7 +271,8 @@ typedef struct plt_procedural_scene_desc {
     float aspect;
 } plt_procedural_scene_desc;
 
-/* ctx == NULL builds a host-only scene (arrays and BVH; plt_scene_trace then returns PLT_ESTATE). */
+/* ctx == NULL builds a host-only scene (arrays and BVH; plt_scene_trace then returns PLT_ESTATE).
+ * The BVH is the host builder's (plt_scene_build_bvh rebuilds it on the device). */
 int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_desc* desc, plt_scene** out);
 
 /* A scene from caller arrays: what the reference's SceneBuilder hands the renderer for triangle
@@ -309,6 +311,38 @@ typedef struct plt_scene_arrays {
     uint32_t bvh_max_depth;          /* levels of the 4-wide BVH (root = 1) */
 } plt_scene_arrays;
 int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out);
+
+/* BVH builders. The reference builds its acceleration structures on the device
+ * (Scene/Scene.cpp:3092-3400 buildBlas, :3624-3700 buildTlas); both builders here write the same
+ * 4-wide node and leaf-order triangle layout, and closest hits do not depend on the tree, so a
+ * scene renders bit-identically with either (only the speed differs).
+ *   PLT_BVH_HOST_SAH: binned SAH on the host's cores (what plt_scene_create runs);
+ *   PLT_BVH_DEVICE_PLOC: on the scene's GPU, parallel locally-ordered clustering of the triangles
+ *     sorted along a Morton curve (Meister and Bittner 2018), collapsed to 4-wide nodes with leaves
+ *     of at most four triangles. */
+typedef enum plt_bvh_builder { PLT_BVH_HOST_SAH = 0, PLT_BVH_DEVICE_PLOC = 1 } plt_bvh_builder;
+typedef struct plt_bvh_info {
+    uint32_t builder;     /* the builder whose tree the scene holds */
+    uint32_t n_nodes;     /* 4-wide nodes */
+    uint32_t max_depth;   /* levels of 4-wide nodes (root = 1) */
+    uint32_t iterations;  /* clustering iterations (device builder; 0 for the host's) */
+    float build_ms;       /* wall-clock time of the build (the device builder's with its allocations) */
+    uint32_t pad[3];
+} plt_bvh_info;
+/* Rebuilds the tree of a scene created with a context and replaces the one it holds (after waiting
+ * for the device: passes bound to the scene read the tree at their next execute). The device
+ * builder runs on `stream` and synchronizes it. PLT_EINVAL when the device tree would need more
+ * levels than the traversal stack holds (33); the scene then keeps its tree. */
+int plt_scene_build_bvh(plt_scene* scene, int builder, plt_stream stream, plt_bvh_info* info);
+/* plt_scene_create with a choice of builder: PLT_BVH_DEVICE_PLOC builds no host tree (it falls
+ * back to the host builder if the device tree would be too deep; info->builder says which ran).
+ * ctx must not be NULL for the device builder. info may be NULL. */
+int plt_scene_create_ex(plt_context* ctx, const plt_scene_desc* desc, int builder, plt_scene** out, plt_bvh_info* info);
+/* The tree a scene holds: info, and (arrays not NULL) device copies to host memory of its
+ * n_nodes x 128 B nodes and n_triangles x 48 B leaf-order triangle records (v0 with the triangle
+ * id's bits in w, v1, v2). Sizes smaller than these return PLT_EINVAL. */
+int plt_scene_get_bvh(const plt_scene* scene, plt_bvh_info* info, void* nodes, size_t nodes_bytes, void* tris,
+                      size_t tris_bytes);
 
 /* Analytic light sources: the reference's LightData (Scene/Lights/LightData.slang:36-66), with
  * the fields its light classes set (Scene/Lights/Light.cpp:115-400): position, direction, spot

@@ -80,6 +80,15 @@ class SceneArrays(C.Structure):
                 ("camera", Camera), ("bvh_max_depth", C.c_uint32)]
 
 
+class BvhInfo(C.Structure):
+    """plt_bvh_info: the builder whose tree a scene holds, its size, depth and build time."""
+    _fields_ = [("builder", C.c_uint32), ("n_nodes", C.c_uint32), ("max_depth", C.c_uint32),
+                ("iterations", C.c_uint32), ("build_ms", C.c_float), ("pad", C.c_uint32 * 3)]
+
+
+BVH_HOST_SAH, BVH_DEVICE_PLOC = 0, 1
+
+
 class Light(C.Structure):
     """plt_light: the reference's LightData (Scene/Lights/LightData.slang:47-66)."""
     _fields_ = [("pos", C.c_float * 3), ("type", C.c_uint32), ("dir", C.c_float * 3), ("opening_angle", C.c_float),
@@ -148,6 +157,10 @@ SIGNATURES = [
     ("plt_material_sample_batch", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                             C.c_void_p, C.c_void_p]),
     ("plt_scene_create", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]),
+    ("plt_scene_create_ex", C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_int, C.POINTER(C.c_void_p),
+                                      C.POINTER(BvhInfo)]),
+    ("plt_scene_build_bvh", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(BvhInfo)]),
+    ("plt_scene_get_bvh", C.c_int, [C.c_void_p, C.POINTER(BvhInfo), C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t]),
     ("plt_scene_destroy", C.c_int, [C.c_void_p]),
     ("plt_scene_get_arrays", C.c_int, [C.c_void_p, C.POINTER(SceneArrays)]),
     ("plt_scene_trace", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),

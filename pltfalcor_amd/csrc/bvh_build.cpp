@@ -11,6 +11,7 @@
 // flattened depth-first so that a parent and its first child are adjacent.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <future>
 #include <limits>
@@ -203,6 +204,7 @@ int32_t flatten(const BNode* b, std::vector<BvhNode>& out, int depth, int& max_d
 }  // namespace
 
 HostBvh build_bvh(const std::vector<float>& pos, const std::vector<uint32_t>& idx, int max_leaf) {
+    const auto t0 = std::chrono::steady_clock::now();
     HostBvh out;
     const uint32_t nt = uint32_t(idx.size() / 3);
     if (nt == 0) return out;
@@ -219,6 +221,7 @@ HostBvh build_bvh(const std::vector<float>& pos, const std::vector<uint32_t>& id
         flatten(root.get(), out.nodes, 1, out.max_depth);
     }
     out.tri_order = std::move(b.order);
+    out.build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return out;
 }
 

@@ -97,7 +97,7 @@ int gen_profile(const float* smp, size_t ns, float lo, float hi, int compute_icd
 extern "C" {
 
 const char* plt_last_error(void) { return plt::g_last_error.c_str(); }
-int plt_abi_version(void) { return 102; }
+int plt_abi_version(void) { return 103; }
 
 int plt_context_create(int device, plt_context** out) {
     if (!out) return set_error(PLT_EINVAL, "plt_context_create: out is NULL");

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <new>
 #include <string>
 #include <vector>
@@ -54,12 +55,24 @@ __global__ void __launch_bounds__(kTraceBlock) trace_kernel(TraceScene sc, uint3
 }
 }  // namespace
 
-int upload_scene(const HostScene& hs, DeviceSceneBuffers& d) {
+// Replaces the owned allocation `old` (if any) by `now` in the scene's list.
+void replace_owned(DeviceSceneBuffers& d, void* old, void* now) {
+    for (void*& p : d.owned)
+        if (old && p == old) {
+            (void)hipFree(p);
+            p = now;
+            return;
+        }
+    d.owned.push_back(now);
+}
+
+// Uploads the host tree: nodes as float4 x 8 and triangles in leaf order as (v0, id), (v1, 0),
+// (v2, 0), replacing the tree the scene holds.
+int upload_bvh(const HostScene& hs, DeviceSceneBuffers& d) {
     // a BVH4 of L levels pushes at most 3 (L - 1) entries; the builder's depth bound keeps L <= 32
     if (3 * (hs.bvh.max_depth - 1) > kTraversalStackDepth + kTraversalSpill)
         return set_error(PLT_EINVAL, "BVH depth " + std::to_string(hs.bvh.max_depth) + " exceeds the traversal stack (" +
                                          std::to_string(kTraversalStackDepth + kTraversalSpill) + " entries)");
-    // Nodes as float4 x 8; triangles in leaf order as (v0, id), (v1, 0), (v2, 0).
     std::vector<float4> nodes(hs.bvh.nodes.size() * 8);
     std::memcpy(nodes.data(), hs.bvh.nodes.data(), hs.bvh.nodes.size() * sizeof(BvhNode));
     const size_t nt = hs.bvh.tri_order.size();
@@ -73,17 +86,72 @@ int upload_scene(const HostScene& hs, DeviceSceneBuffers& d) {
             tris[3 * i + v] = make_float4(p[0], p[1], p[2], w);
         }
     }
+    std::vector<void*> fresh;
+    float4 *dn = nullptr, *dt = nullptr;
     int rc;
-    if ((rc = upload(nodes, &d.nodes, d.owned)) < 0) return rc;
-    if ((rc = upload(tris, &d.tris, d.owned)) < 0) return rc;
+    if ((rc = upload(nodes, &dn, fresh)) < 0 || (rc = upload(tris, &dt, fresh)) < 0) {
+        for (void* p : fresh) (void)hipFree(p);
+        return rc;
+    }
+    replace_owned(d, d.nodes, dn);
+    replace_owned(d, d.tris, dt);
+    d.nodes = dn;
+    d.tris = dt;
+    d.n_tris = uint32_t(nt);
+    d.n_nodes = uint32_t(hs.bvh.nodes.size());
+    return PLT_OK;
+}
+
+// Uploads the geometry and shading arrays (everything but the tree).
+int upload_arrays(const HostScene& hs, DeviceSceneBuffers& d) {
+    int rc;
     if ((rc = upload(hs.positions, &d.positions, d.owned)) < 0) return rc;
     if ((rc = upload(hs.normals, &d.normals, d.owned)) < 0) return rc;
     if ((rc = upload(hs.indices, &d.indices, d.owned)) < 0) return rc;
     if ((rc = upload(hs.tri_mesh, &d.tri_mesh, d.owned)) < 0) return rc;
     if ((rc = upload(hs.mesh_material, &d.mesh_material, d.owned)) < 0) return rc;
     if ((rc = upload(hs.profiles, &d.profiles, d.owned)) < 0) return rc;
-    d.n_tris = uint32_t(nt);
-    d.n_nodes = uint32_t(hs.bvh.nodes.size());
+    d.n_tris = uint32_t(hs.indices.size() / 3);
+    return PLT_OK;
+}
+
+int upload_scene(const HostScene& hs, DeviceSceneBuffers& d) {
+    int rc;
+    if ((rc = upload_arrays(hs, d)) < 0) return rc;
+    return upload_bvh(hs, d);
+}
+
+void host_bvh_info(const HostScene& hs, plt_bvh_info& info) {
+    info = plt_bvh_info{};
+    info.builder = PLT_BVH_HOST_SAH;
+    info.n_nodes = uint32_t(hs.bvh.nodes.size());
+    info.max_depth = uint32_t(hs.bvh.max_depth);
+    info.build_ms = hs.bvh.build_ms;
+}
+
+// Builds the scene's tree on the device and installs it; the scene keeps its old tree on failure.
+int device_bvh(plt_scene* s, hipStream_t st) {
+    DeviceBvh b;
+    const int rc = build_bvh_device(s->dev, st, b);
+    if (rc < 0) return rc;
+    if (hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(b.nodes);
+        (void)hipFree(b.tris);
+        return set_error(PLT_EHIP, "device BVH build: device synchronization failed");
+    }
+    replace_owned(s->dev, s->dev.nodes, b.nodes);
+    replace_owned(s->dev, s->dev.tris, b.tris);
+    s->dev.nodes = b.nodes;
+    s->dev.tris = b.tris;
+    s->dev.n_nodes = b.n_nodes;
+    // the host tree no longer describes the device's
+    s->host.bvh = HostBvh{};
+    s->bvh = plt_bvh_info{};
+    s->bvh.builder = PLT_BVH_DEVICE_PLOC;
+    s->bvh.n_nodes = b.n_nodes;
+    s->bvh.max_depth = uint32_t(b.max_depth);
+    s->bvh.iterations = uint32_t(b.iterations);
+    s->bvh.build_ms = b.ms;
     return PLT_OK;
 }
 
@@ -105,6 +173,7 @@ int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_des
         delete s;
         return set_error(PLT_EINVAL, "plt_scene_create_procedural: invalid description");
     }
+    plt::host_bvh_info(s->host, s->bvh);
     if (!ctx) {  // host-only scene: arrays and BVH, nothing on the device
         *out = s;
         return PLT_OK;
@@ -120,7 +189,14 @@ int plt_scene_create_procedural(plt_context* ctx, const plt_procedural_scene_des
 }
 
 int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out) {
-    if (!d || !out) return set_error(PLT_EINVAL, "plt_scene_create: bad arguments");
+    return plt_scene_create_ex(ctx, d, PLT_BVH_HOST_SAH, out, nullptr);
+}
+
+int plt_scene_create_ex(plt_context* ctx, const plt_scene_desc* d, int builder, plt_scene** out, plt_bvh_info* info) {
+    if (!d || !out || (builder != PLT_BVH_HOST_SAH && builder != PLT_BVH_DEVICE_PLOC))
+        return set_error(PLT_EINVAL, "plt_scene_create: bad arguments");
+    if (builder == PLT_BVH_DEVICE_PLOC && !ctx)
+        return set_error(PLT_EINVAL, "plt_scene_create: the device BVH builder needs a context");
     if (d->n_vertices == 0 || d->n_triangles == 0 || d->n_meshes == 0 || d->n_materials == 0 || !d->positions ||
         !d->normals || !d->indices || !d->tri_mesh || !d->mesh_material || !d->materials || (d->n_profiles && !d->profiles))
         return set_error(PLT_EINVAL, "plt_scene_create: empty scene or missing array");
@@ -158,18 +234,88 @@ int plt_scene_create(plt_context* ctx, const plt_scene_desc* d, plt_scene** out)
     h.materials.assign(d->materials, d->materials + d->n_materials);
     if (d->n_profiles) h.profiles.assign(d->profiles, d->profiles + d->n_profiles);
     h.camera = d->camera;
-    h.bvh = plt::build_bvh(h.positions, h.indices, 4);
-    if (!ctx) {
-        *out = s;
-        return PLT_OK;
+    int rc = PLT_OK;
+    if (builder == PLT_BVH_DEVICE_PLOC) {
+        rc = plt::upload_arrays(h, s->dev);
+        if (rc >= 0) rc = plt::device_bvh(s, nullptr);
+        if (rc == PLT_EINVAL) {  // too deep for the traversal stack: the host builder bounds the depth
+            h.bvh = plt::build_bvh(h.positions, h.indices, 4);
+            plt::host_bvh_info(h, s->bvh);
+            rc = plt::upload_bvh(h, s->dev);
+        }
+    } else {
+        h.bvh = plt::build_bvh(h.positions, h.indices, 4);
+        plt::host_bvh_info(h, s->bvh);
+        if (ctx) rc = plt::upload_scene(h, s->dev);
     }
-    const int rc = plt::upload_scene(s->host, s->dev);
     if (rc < 0) {
         plt::free_scene(s->dev);
         delete s;
         return rc;
     }
+    if (info) *info = s->bvh;
     *out = s;
+    return PLT_OK;
+}
+
+int plt_scene_build_bvh(plt_scene* scene, int builder, plt_stream stream, plt_bvh_info* info) {
+    if (!scene || (builder != PLT_BVH_HOST_SAH && builder != PLT_BVH_DEVICE_PLOC))
+        return set_error(PLT_EINVAL, "plt_scene_build_bvh: bad arguments");
+    plt::HostScene& h = scene->host;
+    if (builder == PLT_BVH_DEVICE_PLOC) {
+        if (!scene->ctx) return set_error(PLT_ESTATE, "plt_scene_build_bvh: host-only scene (created without a context)");
+        const int rc = plt::device_bvh(scene, static_cast<hipStream_t>(stream));
+        if (rc < 0) return rc;
+    } else {
+        plt::HostBvh b = plt::build_bvh(h.positions, h.indices, 4);
+        if (scene->ctx) {
+            if (hipDeviceSynchronize() != hipSuccess) return set_error(PLT_EHIP, "plt_scene_build_bvh: device synchronization failed");
+            std::swap(h.bvh, b);
+            const int rc = plt::upload_bvh(h, scene->dev);
+            if (rc < 0) {
+                std::swap(h.bvh, b);
+                return rc;
+            }
+        } else {
+            std::swap(h.bvh, b);
+        }
+        plt::host_bvh_info(h, scene->bvh);
+    }
+    if (info) *info = scene->bvh;
+    return PLT_OK;
+}
+
+int plt_scene_get_bvh(const plt_scene* scene, plt_bvh_info* info, void* nodes, size_t nodes_bytes, void* tris,
+                      size_t tris_bytes) {
+    if (!scene) return set_error(PLT_EINVAL, "plt_scene_get_bvh: NULL scene");
+    const size_t nb = size_t(scene->bvh.n_nodes) * sizeof(plt::BvhNode);
+    const size_t nt = scene->host.indices.size() / 3;
+    const size_t tb = nt * 3 * sizeof(float4);
+    if ((nodes && nodes_bytes < nb) || (tris && tris_bytes < tb))
+        return set_error(PLT_EINVAL, "plt_scene_get_bvh: output smaller than the tree");
+    if (info) *info = scene->bvh;
+    if (scene->ctx) {
+        if ((nodes && hipMemcpy(nodes, scene->dev.nodes, nb, hipMemcpyDeviceToHost) != hipSuccess) ||
+            (tris && hipMemcpy(tris, scene->dev.tris, tb, hipMemcpyDeviceToHost) != hipSuccess))
+            return set_error(PLT_EHIP, "plt_scene_get_bvh: hipMemcpy failed");
+        return PLT_OK;
+    }
+    // host-only scene: the host tree, in the device layout
+    const plt::HostScene& h = scene->host;
+    if (nodes) std::memcpy(nodes, h.bvh.nodes.data(), nb);
+    if (tris) {
+        float* out = static_cast<float*>(tris);
+        for (size_t i = 0; i < h.bvh.tri_order.size(); ++i) {
+            const uint32_t t = h.bvh.tri_order[i];
+            for (int v = 0; v < 3; ++v) {
+                const float* p = &h.positions[3 * size_t(h.indices[3 * size_t(t) + v])];
+                float* o = out + 12 * i + 4 * v;
+                o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+                o[3] = 0.f;
+                if (v == 0) std::memcpy(&o[3], &t, 4);
+            }
+        }
+    }
     return PLT_OK;
 }
 
@@ -211,8 +357,8 @@ int plt_scene_get_arrays(const plt_scene* scene, plt_scene_arrays* out) {
     out->n_meshes = uint32_t(h.mesh_first_tri.size());
     out->n_materials = uint32_t(h.materials.size());
     out->n_profiles = uint32_t(h.profiles.size());
-    out->n_bvh_nodes = uint32_t(h.bvh.nodes.size());
-    out->bvh_max_depth = uint32_t(h.bvh.max_depth);
+    out->n_bvh_nodes = scene->bvh.n_nodes;
+    out->bvh_max_depth = scene->bvh.max_depth;
     out->positions = h.positions.data();
     out->normals = h.normals.data();
     out->indices = h.indices.data();

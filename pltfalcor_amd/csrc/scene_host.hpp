@@ -42,6 +42,7 @@ struct HostBvh {
     std::vector<BvhNode> nodes;
     std::vector<uint32_t> tri_order;  // triangle ids in leaf order
     int max_depth = 0;                // BVH4 levels (root = 1)
+    float build_ms = 0.f;             // wall-clock time of the build
 };
 
 // Builds a binned-SAH BVH over triangles (positions as 3 floats per vertex, indices 3 per tri).

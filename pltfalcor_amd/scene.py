@@ -62,6 +62,12 @@ def c4_scene(ctx, aspect: float):
                              np.concatenate(tri_mesh), np.concatenate(mesh_mat), bytes(mats), a["profiles"], cam)
 
 
+def _builder(name: str) -> int:
+    if name not in ("host", "device"):
+        raise ValueError("BVH builder must be 'host' or 'device'")
+    return _lib.BVH_HOST_SAH if name == "host" else _lib.BVH_DEVICE_PLOC
+
+
 class Scene:
     """A scene on one GPU (or host-only when ctx is None): the procedural workload scene, or
     `Scene.from_arrays` for caller geometry, materials and spectral profiles."""
@@ -84,8 +90,9 @@ class Scene:
 
     @classmethod
     def from_arrays(cls, ctx, positions, normals, indices, tri_mesh, mesh_material, materials: bytes,
-                    profiles: bytes, camera):
-        """materials / profiles: packed plt_material / plt_spectral_profile records; camera: Camera."""
+                    profiles: bytes, camera, bvh: str = "host"):
+        """materials / profiles: packed plt_material / plt_spectral_profile records; camera: Camera.
+        bvh: "host" (binned SAH on the CPU) or "device" (built on the GPU; needs ctx)."""
         lib = _lib.load()
         pos = np.ascontiguousarray(positions, np.float32).reshape(-1)
         nrm = np.ascontiguousarray(normals, np.float32).reshape(-1)
@@ -103,8 +110,46 @@ class Scene:
                            C.cast(mats, C.POINTER(Material)) if mats is not None else None,
                            C.cast(profs, C.POINTER(_lib.SpectralProfile)) if profs is not None else None, camera)
         h = C.c_void_p()
-        _lib.check(lib.plt_scene_create(ctx.handle if ctx is not None else None, C.byref(d), C.byref(h)))
+        _lib.check(lib.plt_scene_create_ex(ctx.handle if ctx is not None else None, C.byref(d), _builder(bvh),
+                                           C.byref(h), None))
         return cls(ctx, _handle=h)
+
+    def build_bvh(self, builder: str = "device", stream=None) -> dict:
+        """Rebuilds the scene's BVH with the host ("host") or device ("device") builder and returns
+        bvh_info(). A device tree deeper than the traversal stack holds raises (PLT_EINVAL) and the
+        scene keeps its tree."""
+        if stream is None and self.ctx is not None and torch.cuda.is_available():
+            stream = torch.cuda.current_stream()
+        info = _lib.BvhInfo()
+        _lib.check(self.lib.plt_scene_build_bvh(self.handle, _builder(builder),
+                                                C.c_void_p(stream.cuda_stream) if stream is not None else None,
+                                                C.byref(info)))
+        self._refresh_arrays()
+        return self.bvh_info()
+
+    def bvh_info(self) -> dict:
+        info = _lib.BvhInfo()
+        _lib.check(self.lib.plt_scene_get_bvh(self.handle, C.byref(info), None, 0, None, 0))
+        return dict(builder="device" if info.builder == _lib.BVH_DEVICE_PLOC else "host", n_nodes=info.n_nodes,
+                    max_depth=info.max_depth, iterations=info.iterations, build_ms=info.build_ms)
+
+    def bvh_arrays(self):
+        """Host copies of the tree: nodes as a [n_nodes, 32] float32 array (rows lo x, hi x, lo y, hi y,
+        lo z, hi z of the four children, then the four child references and padding as int32 bits)
+        and the leaf-order triangle records as [n_triangles, 3, 4] float32 (v0 with the triangle id's
+        bits in w, v1, v2)."""
+        info = _lib.BvhInfo()
+        _lib.check(self.lib.plt_scene_get_bvh(self.handle, C.byref(info), None, 0, None, 0))
+        nodes = np.zeros((info.n_nodes, 32), np.float32)
+        tris = np.zeros((self.arrays.n_triangles, 3, 4), np.float32)
+        _lib.check(self.lib.plt_scene_get_bvh(self.handle, None, C.c_void_p(nodes.ctypes.data), nodes.nbytes,
+                                              C.c_void_p(tris.ctypes.data), tris.nbytes))
+        return nodes, tris
+
+    def _refresh_arrays(self):
+        a = SceneArrays()
+        _lib.check(self.lib.plt_scene_get_arrays(self.handle, C.byref(a)))
+        self.arrays = a
 
     def set_lights(self, lights):
         """Analytic lights (pltfalcor_amd.lights objects or packed plt_light bytes); bind the scene to
