@@ -72,7 +72,7 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("path", "bsdf", "surrogate", "c4"), default="path")
+    ap.add_argument("--workload", choices=("path", "bsdf", "surrogate", "c4", "bvh"), default="path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="skip the BSDF-batch secondary measurement")
@@ -508,6 +508,58 @@ def path_cpu_baseline(scene_np, budget_s: float, opts):
                       "BVH prebuilt)"}
 
 
+def bvh_run(ctx, steps: int):
+    """The path workload scene's BVH (5,242,884 triangles): the host builder's binned SAH (made at
+    scene creation) against the device builder (PLOC + 4-wide collapse), timed over `steps` rebuilds
+    after one warm-up build, and 2^20 camera rays cast against each tree (hits must be identical)."""
+    import numpy as np
+    import torch
+    from pltfalcor_amd.scene import PATH_2, Scene
+    scene = Scene(ctx, aspect=WIDTH / HEIGHT, **PATH_2)
+    host = scene.bvh_info()
+    n = 1 << 20
+    rng = np.random.default_rng(5)
+    cam = np.array([0, 5, 16], np.float32)
+    tgt = np.stack([rng.uniform(-10, 10, n), rng.uniform(0, 2, n), rng.uniform(-10, 10, n)], 1).astype(np.float32)
+    d = tgt - cam
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = torch.from_numpy(np.ascontiguousarray(np.concatenate(
+        [np.tile(cam, (n, 1)).T, d.T, np.zeros((1, n), np.float32), np.full((1, n), 3.4e38, np.float32)]),
+        dtype=np.float32)).cuda()
+
+    def cast():
+        hits = scene.trace(rays)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(10):
+            scene.trace(rays)
+        torch.cuda.synchronize()
+        return hits.cpu().numpy(), (time.perf_counter() - t) / 10 * 1e3
+
+    host_hits, host_cast = cast()
+    scene.build_bvh("device")
+    wall, build = [], []
+    for _ in range(max(1, steps)):
+        t = time.perf_counter()
+        build.append(scene.build_bvh("device")["build_ms"])
+        wall.append((time.perf_counter() - t) * 1e3)
+    dev = scene.bvh_info()
+    dev_hits, dev_cast = cast()
+    if not np.array_equal(host_hits, dev_hits):
+        raise RuntimeError("the device tree's hits differ from the host tree's")
+    ms = float(np.mean(build))
+    scene.close()
+    tris = scene_triangles(PATH_2)
+    return {"value": tris / (ms * 1e-3), "triangles": tris, "ms": ms, "wall_ms": float(np.mean(wall)),
+            "host_build_ms": host["build_ms"], "host": host, "device": dev,
+            "camera_rays_cast_ms": {"host_tree": host_cast, "device_tree": dev_cast}}
+
+
+def scene_triangles(desc) -> int:
+    # ground quad (2), 20 * 4^s per icosphere, the light quad (2)
+    return 4 + desc["sphere_count"] * 20 * 4 ** desc["subdivision"]
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -530,6 +582,20 @@ def main():
             out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = bsdf_cpu_baseline(batch.queries.cpu().numpy(), args.cpu_seconds)
+        print(json.dumps(out), flush=True)
+        return
+
+    if args.workload == "bvh":
+        from pltfalcor_amd.bsdf import default_context
+        r = bvh_run(default_context(0), args.steps)
+        out = {"metric": METRIC, "value": r["value"], "unit": "triangles/s (BVH build)", "n_gpus": 1,
+               "steps": args.steps, "warmup": 1, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": "BVH build of the path workload scene (seed 1, 256 icospheres subdiv 5, "
+                                      "5,242,884 triangles): device PLOC clustering + 4-wide collapse, against the "
+                                      "host binned-SAH build",
+                          "triangles": r["triangles"]},
+               "roofline": None, "cpu_baseline": None, "bvh": r}
         print(json.dumps(out), flush=True)
         return
 

@@ -8,7 +8,7 @@
 //
 // Stages, on one stream (the host reads one word back per clustering iteration and per tree
 // level, to size the next launch):
-//   1. triangle boxes, and the centroid bounds as order-preserving integer min/max atomics;
+//   1. triangle boxes, and the centroid bounds (block partials, then one reducing block);
 //   2. Morton keys of the centroids (21 bits per axis) and a radix sort of (key, triangle) pairs
 //      (rocPRIM; stable, so equal keys stay in triangle order and the build is deterministic);
 //   3. clustering: every cluster picks, among the clusters at most kRadius places away along the
@@ -19,7 +19,8 @@
 //   4. collapse, one launch per level: a 4-wide node starts from its binary node and replaces its
 //      inner child of largest area by that child's two children until it has four; a child of at
 //      most kLeafMax triangles becomes a leaf, whose triangles it writes in leaf order (3 x float4,
-//      the triangle id in the first record's w), and the others become the next level's nodes.
+//      the triangle id in the first record's w), and the others become the next level's nodes;
+//   5. a depth-first renumbering of the nodes (subtree sizes bottom-up, positions top-down).
 // Every merge and every level sees only data written by earlier launches, so the build needs no
 // device-wide fences between workgroups.
 #include <hip/hip_runtime.h>
@@ -37,8 +38,9 @@ namespace plt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kRadius = 16;               // clustering search radius (places along the curve)
+constexpr int kRadius = 8;                // clustering search radius (places along the curve)
 constexpr int kFinishClusters = 1024;     // the single-workgroup finish takes this many clusters
+constexpr uint32_t kBoxBlocks = 2048;     // blocks of the triangle-box pass (grid-stride beyond)
 constexpr uint32_t kLeafMax = 4;          // triangles per leaf at most
 // Clustering iterations above kFinishClusters clusters at most (each merges at least one pair;
 // geometry that merges only a few per iteration, such as sizes growing geometrically, would also
@@ -46,15 +48,6 @@ constexpr uint32_t kLeafMax = 4;          // triangles per leaf at most
 constexpr int kMaxIterations = 4096;
 // Levels of 4-wide nodes the traversal stack holds (three entries pushed per level).
 constexpr int kMaxLevels = 1 + (kTraversalStackDepth + kTraversalSpill) / 3;
-
-// Order-preserving map of floats to unsigned integers (for integer min / max atomics).
-__device__ __forceinline__ uint32_t ord_bits(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-__device__ __forceinline__ float unord_bits(uint32_t u) {
-    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
-}
 
 // Spreads the low 21 bits of v to every third bit.
 __device__ __forceinline__ uint64_t spread3(uint32_t v) {
@@ -88,19 +81,14 @@ __device__ __forceinline__ float box_area(float4 a, float4 A) {
     return dx * dy + dy * dz + dz * dx;
 }
 
-__global__ void bounds_init_kernel(uint32_t* __restrict__ cb) {
-    if (threadIdx.x < 3) cb[threadIdx.x] = 0xFFFFFFFFu;
-    else if (threadIdx.x < 6) cb[threadIdx.x] = 0u;
-}
-
-// Triangle boxes (xyz; w unused) and the bounds of their centroids.
+// Triangle boxes (xyz; w unused) and, per block, the bounds of their centroids (a grid-stride loop
+// over the triangles, then a block reduction: one partial per block, no atomics).
 __global__ void __launch_bounds__(kBlock) tri_box_kernel(uint32_t n, const float* __restrict__ pos,
                                                          const uint32_t* __restrict__ idx, float4* __restrict__ tlo,
-                                                         float4* __restrict__ thi, uint32_t* __restrict__ cb) {
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+                                                         float4* __restrict__ thi, float* __restrict__ partial) {
     constexpr float kInf = __builtin_inff();
     float cmin[3] = {kInf, kInf, kInf}, cmax[3] = {-kInf, -kInf, -kInf};
-    if (t < n) {
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < n; t += gridDim.x * kBlock) {
         float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
@@ -114,8 +102,13 @@ __global__ void __launch_bounds__(kBlock) tri_box_kernel(uint32_t n, const float
         tlo[t] = make_float4(lo[0], lo[1], lo[2], 0.f);
         thi[t] = make_float4(hi[0], hi[1], hi[2], 0.f);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) cmin[a] = cmax[a] = 0.5f * (lo[a] + hi[a]);
+        for (int a = 0; a < 3; ++a) {
+            const float c = 0.5f * (lo[a] + hi[a]);
+            cmin[a] = fminf(cmin[a], c);
+            cmax[a] = fmaxf(cmax[a], c);
+        }
     }
+    __shared__ float red[kBlock / 64][6];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         for (int o = 32; o > 0; o >>= 1) {
@@ -123,18 +116,58 @@ __global__ void __launch_bounds__(kBlock) tri_box_kernel(uint32_t n, const float
             cmax[a] = fmaxf(cmax[a], __shfl_xor(cmax[a], o));
         }
     }
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            atomicMin(cb + a, ord_bits(cmin[a]));
-            atomicMax(cb + 3 + a, ord_bits(cmax[a]));
+            red[wave][a] = cmin[a];
+            red[wave][3 + a] = cmax[a];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float v = red[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w) v = threadIdx.x < 3 ? fminf(v, red[w][threadIdx.x]) : fmaxf(v, red[w][threadIdx.x]);
+        partial[6 * blockIdx.x + threadIdx.x] = v;
+    }
+}
+
+// The centroid bounds from the blocks' partials (one block): cb = (min xyz, max xyz).
+__global__ void __launch_bounds__(kBlock) bounds_kernel(uint32_t blocks, const float* __restrict__ partial,
+                                                        float* __restrict__ cb) {
+    __shared__ float red[kBlock / 64][6];
+    constexpr float kInf = __builtin_inff();
+    float v[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = fminf(v[a], partial[6 * b + a]);
+            v[3 + a] = fmaxf(v[3 + a], partial[6 * b + 3 + a]);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        for (int o = 32; o > 0; o >>= 1) {
+            v[a] = fminf(v[a], __shfl_xor(v[a], o));
+            v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], o));
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int a = 0; a < 6; ++a) red[wave][a] = v[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float r = red[0][threadIdx.x];
+        for (int w = 1; w < kBlock / 64; ++w) r = threadIdx.x < 3 ? fminf(r, red[w][threadIdx.x]) : fmaxf(r, red[w][threadIdx.x]);
+        cb[threadIdx.x] = r;
     }
 }
 
 // 63-bit Morton key of each centroid in the centroid bounds; the value is the triangle id.
 __global__ void __launch_bounds__(kBlock) morton_kernel(uint32_t n, const float4* __restrict__ tlo,
-                                                        const float4* __restrict__ thi, const uint32_t* __restrict__ cb,
+                                                        const float4* __restrict__ thi, const float* __restrict__ cb,
                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= n) return;
@@ -143,7 +176,7 @@ __global__ void __launch_bounds__(kBlock) morton_kernel(uint32_t n, const float4
     uint32_t q[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float b0 = unord_bits(cb[a]), b1 = unord_bits(cb[3 + a]);
+        const float b0 = cb[a], b1 = cb[3 + a];
         const float ext = b1 - b0;
         const float s = ext > 0.f ? 2097151.f / ext : 0.f;
         q[a] = uint32_t(fminf(fmaxf((c[a] - b0) * s, 0.f), 2097151.f));
@@ -423,6 +456,59 @@ __global__ void __launch_bounds__(kBlock) collapse_kernel(uint32_t m, const uint
     q[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Depth-first layout of the collapsed tree (a node's first inner child right after it, each subtree
+// contiguous), computed level by level: the collapse numbers nodes breadth-first in the order its
+// threads append them, which scatters a subtree over a whole level. Inner child references are > 0
+// (the root, 0, is nobody's child), leaf references < 0 and empty slots 0.
+// Nodes in each subtree (itself included), from the deepest level up.
+__global__ void __launch_bounds__(kBlock) subtree_size_kernel(uint32_t first, uint32_t count,
+                                                              const float4* __restrict__ nodes,
+                                                              uint32_t* __restrict__ size) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const float4 cf = nodes[8 * size_t(first + k) + 6];
+    const int32_t r[4] = {__float_as_int(cf.x), __float_as_int(cf.y), __float_as_int(cf.z), __float_as_int(cf.w)};
+    uint32_t s = 1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (r[c] > 0) s += size[r[c]];
+    size[first + k] = s;
+}
+// Depth-first positions of the next level's nodes from this level's.
+__global__ void __launch_bounds__(kBlock) dfs_index_kernel(uint32_t first, uint32_t count, const float4* __restrict__ nodes,
+                                                           const uint32_t* __restrict__ size, uint32_t* __restrict__ at) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const float4 cf = nodes[8 * size_t(first + k) + 6];
+    const int32_t r[4] = {__float_as_int(cf.x), __float_as_int(cf.y), __float_as_int(cf.z), __float_as_int(cf.w)};
+    uint32_t q = at[first + k] + 1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        if (r[c] > 0) {
+            at[r[c]] = q;
+            q += size[r[c]];
+        }
+}
+__global__ void __launch_bounds__(kBlock) permute_kernel(uint32_t n_nodes, const float4* __restrict__ nodes,
+                                                         const uint32_t* __restrict__ at, float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n_nodes) return;
+    const float4* q = nodes + 8 * size_t(i);
+    float4* o = out + 8 * size_t(at[i]);
+#pragma unroll
+    for (int row = 0; row < 8; ++row) {
+        float4 v = q[row];
+        if (row == 6) {
+            const int32_t r[4] = {__float_as_int(v.x), __float_as_int(v.y), __float_as_int(v.z), __float_as_int(v.w)};
+            v = make_float4(__int_as_float(r[0] > 0 ? int32_t(at[r[0]]) : r[0]),
+                            __int_as_float(r[1] > 0 ? int32_t(at[r[1]]) : r[1]),
+                            __int_as_float(r[2] > 0 ? int32_t(at[r[2]]) : r[2]),
+                            __int_as_float(r[3] > 0 ? int32_t(at[r[3]]) : r[3]));
+        }
+        o[row] = v;
+    }
+}
+
 // Device allocations of one build, released together unless kept.
 struct Allocs {
     std::vector<void*> ptrs;
@@ -460,7 +546,10 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
     const auto t0 = std::chrono::steady_clock::now();
     Allocs A;
     const size_t nn2 = 2 * size_t(n);
-    uint32_t* cb = A.get<uint32_t>(8);
+    const uint32_t box_blocks = grid_of(n) < kBoxBlocks ? grid_of(n) : kBoxBlocks;
+    uint32_t* words = A.get<uint32_t>(8);  // [6] root, [7] next level's width
+    float* cb = A.get<float>(6);
+    float* partial = A.get<float>(6 * size_t(box_blocks));
     float4* tlo = A.get<float4>(n);  // triangle boxes, later the second cluster list
     float4* thi = A.get<float4>(n);
     uint64_t* bufA = A.get<uint64_t>(n);  // keys, then flags, then collapse items
@@ -475,7 +564,7 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
     uint32_t* nn = A.get<uint32_t>(n);
     float4* nodes = A.get<float4>(8 * size_t(n));
     float4* tris = A.get<float4>(3 * size_t(n));
-    if (!cb || !tlo || !thi || !bufA || !bufB || !vals || !ids || !nlo || !nhi || !ncnt || !clo || !chi || !nn || !nodes || !tris)
+    if (!words || !cb || !partial || !tlo || !thi || !bufA || !bufB || !vals || !ids || !nlo || !nhi || !ncnt || !clo || !chi || !nn || !nodes || !tris)
         return set_error(PLT_ENOMEM, "device BVH build: hipMalloc failed");
     size_t sort_bytes = 0, scan_bytes = 0;
     if (rocprim::radix_sort_pairs(nullptr, sort_bytes, bufA, bufB, vals, ids, n, 0, 63, st) != hipSuccess ||
@@ -485,8 +574,8 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
     if (!tmp) return set_error(PLT_ENOMEM, "device BVH build: hipMalloc failed");
     const size_t tmp_bytes = sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
 
-    hipLaunchKernelGGL(bounds_init_kernel, dim3(1), dim3(64), 0, st, cb);
-    hipLaunchKernelGGL(tri_box_kernel, dim3(grid_of(n)), dim3(kBlock), 0, st, n, d.positions, d.indices, tlo, thi, cb);
+    hipLaunchKernelGGL(tri_box_kernel, dim3(box_blocks), dim3(kBlock), 0, st, n, d.positions, d.indices, tlo, thi, partial);
+    hipLaunchKernelGGL(bounds_kernel, dim3(1), dim3(kBlock), 0, st, box_blocks, partial, cb);
     hipLaunchKernelGGL(morton_kernel, dim3(grid_of(n)), dim3(kBlock), 0, st, n, tlo, thi, cb, bufA, vals);
     size_t sb = tmp_bytes;
     if (rocprim::radix_sort_pairs(tmp, sb, bufA, bufB, vals, ids, n, 0, 63, st) != hipSuccess) return hip_fail("radix sort");
@@ -498,7 +587,7 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
     float4 *c_lo = clo, *c_hi = chi, *c_lo2 = tlo, *c_hi2 = thi;
     uint64_t* flags = bufA;
     uint64_t* scan = bufB;
-    uint32_t* root_word = cb + 6;
+    uint32_t* root_word = words + 6;
     int iterations = 0;
     while (m > uint32_t(kFinishClusters)) {
         hipLaunchKernelGGL(nn_kernel, dim3(grid_of(m)), dim3(kBlock), 0, st, m, c_lo, c_hi, nn);
@@ -536,11 +625,12 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
     // collapse, level by level
     uint2* items = reinterpret_cast<uint2*>(bufA);
     uint2* next = reinterpret_cast<uint2*>(bufB);
-    uint32_t* next_count = cb + 7;
+    uint32_t* next_count = words + 7;
     const uint2 first = make_uint2(root, 0u);
     if (hipMemcpyAsync(items, &first, sizeof(first), hipMemcpyHostToDevice, st) != hipSuccess) return hip_fail("collapse");
     uint32_t level_base = 0, width = 1;
     int levels = 0;
+    std::vector<uint32_t> level_first, level_width;
     while (width > 0) {
         if (levels == kMaxLevels)
             return set_error(PLT_EINVAL, "device BVH build: the tree needs more than " + std::to_string(kMaxLevels) +
@@ -552,17 +642,28 @@ int build_bvh_device(const DeviceSceneBuffers& d, hipStream_t st, DeviceBvh& out
         if (hipGetLastError() != hipSuccess || hipMemcpyAsync(&w2, next_count, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return hip_fail("collapse");
+        level_first.push_back(level_base);
+        level_width.push_back(width);
         level_base += width;
         width = w2;
         std::swap(items, next);
         ++levels;
     }
-    // the nodes into an allocation of their exact size
+    // depth-first layout, into an allocation of the exact size (the clustering's counts and
+    // neighbour lists are free again: sizes in ncnt, positions in nn, both at least n_nodes long)
     float4* exact = A.get<float4>(8 * size_t(level_base));
     if (!exact) return set_error(PLT_ENOMEM, "device BVH build: hipMalloc failed");
-    if (hipMemcpyAsync(exact, nodes, size_t(level_base) * sizeof(BvhNode), hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return hip_fail("node copy");
+    uint32_t* size = ncnt;
+    uint32_t* at = nn;
+    for (int l = levels - 1; l >= 0; --l)
+        hipLaunchKernelGGL(subtree_size_kernel, dim3(grid_of(level_width[l])), dim3(kBlock), 0, st, level_first[l],
+                           level_width[l], nodes, size);
+    if (hipMemsetAsync(at, 0, 4, st) != hipSuccess) return hip_fail("layout");
+    for (int l = 0; l + 1 < levels; ++l)
+        hipLaunchKernelGGL(dfs_index_kernel, dim3(grid_of(level_width[l])), dim3(kBlock), 0, st, level_first[l],
+                           level_width[l], nodes, size, at);
+    hipLaunchKernelGGL(permute_kernel, dim3(grid_of(level_base)), dim3(kBlock), 0, st, level_base, nodes, at, exact);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return hip_fail("layout");
     A.keep(exact);
     A.keep(tris);
     out.nodes = exact;
