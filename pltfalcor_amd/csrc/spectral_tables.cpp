@@ -85,7 +85,8 @@ int plt_spectrum_load_spd(const char* path, uint32_t capacity, float* wavelength
         return set_error(PLT_EINVAL, "plt_spectrum_load_spd: capacity below the table's " + std::to_string(wl.size()) +
                                          " entries");
     }
-    if (!wl.empty() && (!wavelengths || !values)) return set_error(PLT_EINVAL, "plt_spectrum_load_spd: NULL output");
+    if (wl.empty()) return PLT_OK;  // a table without entries: nothing to copy
+    if (!wavelengths || !values) return set_error(PLT_EINVAL, "plt_spectrum_load_spd: NULL output");
     std::memcpy(wavelengths, wl.data(), wl.size() * sizeof(float));
     std::memcpy(values, val.data(), val.size() * sizeof(float));
     return PLT_OK;
