@@ -3,7 +3,7 @@
 // independent of the tree (watertight intersection with a deterministic tie-break), so the tree
 // only affects speed.
 //
-// Build: per-triangle boxes and centroids; recursive 32-bin SAH over all three axes, with an
+// Build: per-triangle boxes and centroids; recursive 128-bin SAH over all three axes, with an
 // object-median split wherever the SAH split would leave a child that cannot be finished within
 // kMaxBvh2Depth levels; subtrees of more than 32k triangles are built on their own threads
 // (disjoint index ranges, partitioned in place). The binary tree is then collapsed into 4-wide
@@ -22,7 +22,7 @@
 namespace plt {
 namespace {
 
-constexpr int kBins = 32;
+constexpr int kBins = 128;
 constexpr uint32_t kParallelThreshold = 32768;
 
 struct BNode {
