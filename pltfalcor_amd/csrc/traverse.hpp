@@ -158,11 +158,16 @@ __device__ __forceinline__ Hit traverse_rays(const TraceScene& sc, const Ray& r,
             uint32_t k2 = e2 != kInf ? ((__float_as_uint(e2) & ~3u) | 2u) : 0xFFFFFFFFu;
             uint32_t k3 = e3 != kInf ? ((__float_as_uint(e3) & ~3u) | 3u) : 0xFFFFFFFFu;
             sort4(k0, k1, k2, k3);
+            // The child references are used only when a box is hit; left to itself the compiler
+            // issues their load there, after the box tests, which costs every node visit a second
+            // dependent memory round trip. Requiring them in registers here keeps the load with
+            // the boxes' (the empty asm emits nothing).
+            asm volatile("" : : "v"(cf.x), "v"(cf.y), "v"(cf.z), "v"(cf.w));
             if (k0 != 0xFFFFFFFFu) {
                 const int c0 = __float_as_int(cf.x), c1 = __float_as_int(cf.y), c2 = __float_as_int(cf.z), c3 = __float_as_int(cf.w);
-                auto child = [&](uint32_t k) {
-                    const uint32_t slot = k & 3u;
-                    return slot == 0u ? c0 : (slot == 1u ? c1 : (slot == 2u ? c2 : c3));
+                auto child = [&](uint32_t k) {  // selects, no branches
+                    const int lo = (k & 1u) ? c1 : c0, hi = (k & 1u) ? c3 : c2;
+                    return (k & 2u) ? hi : lo;
                 };
                 // the farther hit children on the stack, farthest first; the nearest next
                 if (k3 != 0xFFFFFFFFu) push(child(k3));
