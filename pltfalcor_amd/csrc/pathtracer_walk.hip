@@ -1,6 +1,9 @@
 // The split solve pass's deferred manifold walks: the persistent walk kernel over the frame's walk
-// queue, sized to fill every CU (16 one-wave blocks per CU: four waves per SIMD, which the
-// kernel's 10 KB of LDS per block allows exactly), and the clear of the queue's counters.
+// queue, sized to fill every CU (12 one-wave blocks per CU: three waves per SIMD, each with up to
+// 168 VGPRs), and the clear of the queue's counters. The kernel's time is the latency chain of its
+// longest walks, so it trades occupancy for registers: at four waves per SIMD (128 VGPRs) it
+// spilled, and every Newton iteration paid the spill reloads' round trips (C2 frame: 3.85 ms at
+// four waves, 3.48 ms at three, 3.59 ms at two).
 #include "pathtracer_split.hpp"
 
 namespace plt {
@@ -29,7 +32,7 @@ PLT_DEV V3 hit_pos(const SolveBuffers& sb, const uint32_t* h) {
 // MNEEMaxIterations line-search casts, one visibility cast and, between casts, at most two passes
 // over its vertices (a pass that casts nothing either meets the threshold at every vertex, which
 // ends the walk, or follows a pass that cast).
-__global__ void __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
+__global__ void __launch_bounds__(kWalkBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
 solve_walk_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
     __shared__ int stack[kTraversalStackDepth * kWalkBlock];
     __shared__ float betaS[kMaxMsOccluders * kWalkBlock];  // per-vertex step sizes, [vertex][lane]
@@ -201,7 +204,7 @@ int launch_solve_walks(const PtScene& sc, const PtFrame& fr, const SolveBuffers&
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         return set_error(PLT_EHIP, "solve_walk_kernel: device query failed");
-    hipLaunchKernelGGL(solve_walk_kernel, dim3(uint32_t(cus) * 16u), dim3(kWalkBlock), 0, st, sc, fr, sb);
+    hipLaunchKernelGGL(solve_walk_kernel, dim3(uint32_t(cus) * 12u), dim3(kWalkBlock), 0, st, sc, fr, sb);
     return check_launch("solve_walk_kernel");
 }
 
