@@ -98,6 +98,20 @@ def test_checker_accepts_the_host_tree_and_finds_planted_mistakes():
         check_tree(nodes, badt, pos, idx)
 
 
+def test_host_rebuild_is_deterministic():
+    """plt_scene_build_bvh with the host builder on a host-only scene rebuilds the same tree (the
+    binned SAH is deterministic, its parallel subtrees included) and reports it."""
+    from pltfalcor_amd.scene import Scene
+    s = Scene(None, seed=4, sphere_count=40, subdivision=3)  # > 32k triangles: parallel subtrees
+    nodes0, tris0 = s.bvh_arrays()
+    info = s.build_bvh("host")
+    nodes1, tris1 = s.bvh_arrays()
+    assert info["builder"] == "host" and info["n_nodes"] == len(nodes1) == len(nodes0)
+    assert np.array_equal(nodes0.view(np.uint32), nodes1.view(np.uint32)) and np.array_equal(tris0, tris1)
+    a = s.numpy()
+    assert check_tree(nodes1, tris1, a["positions"], a["indices"]) == info["max_depth"]
+
+
 def test_device_builder_needs_a_context():
     from pltfalcor_amd import _lib
     from pltfalcor_amd.scene import Scene
