@@ -12,6 +12,7 @@
 #include <map>
 #include <random>
 #include <string>
+#include <new>
 #include <vector>
 #include "pathtracer.hpp"
 #include "plt_internal.hpp"
@@ -170,6 +171,7 @@ struct plt_pass {
     float* d_tile_draws = nullptr;  // NEEUsePerTileSG draws (tiles x (maxBounces + 1))
     size_t tile_draws_cap = 0;
     size_t last_words = 0;  // record words written by the last execute
+    uint32_t last_width = 0, last_height = 0;  // its frame size (records are in path order)
     float cam[12] = {};
     float view_rot[9] = {};
     // kernel timing: kEventsPerExecute events per execute: before the sample pass, after it, after the
@@ -778,6 +780,8 @@ int plt_pass_execute(plt_pass* p, const plt_render_data* rd, plt_stream stream) 
     if (ev[5]) (void)hipEventRecord(ev[5], st);
     ++p->frame;  // PLTPT.cpp:498
     p->last_words = words;
+    p->last_width = rd->width;
+    p->last_height = rd->height;
     return PLT_OK;
 }
 
@@ -785,8 +789,22 @@ int plt_pass_copy_bounce_records(plt_pass* p, void* dst, size_t bytes) {
     if (!p || !dst) return set_error(PLT_EINVAL, "plt_pass_copy_bounce_records: bad arguments");
     if (!p->d_records || p->last_words == 0) return set_error(PLT_ESTATE, "plt_pass_copy_bounce_records: nothing rendered yet");
     if (bytes < p->last_words * 4) return set_error(PLT_EINVAL, "plt_pass_copy_bounce_records: destination too small");
-    if (hipMemcpy(dst, p->d_records, p->last_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<uint32_t> paths;
+    try {
+        paths.resize(p->last_words);
+    } catch (const std::bad_alloc&) {
+        return set_error(PLT_ENOMEM, "plt_pass_copy_bounce_records: out of host memory");
+    }
+    if (hipMemcpy(paths.data(), p->d_records, p->last_words * 4, hipMemcpyDeviceToHost) != hipSuccess)
         return set_error(PLT_EHIP, "plt_pass_copy_bounce_records: hipMemcpy failed");
+    // path order (8 x 8 pixel blocks) to the row-major pixel order of the layout
+    const uint32_t w = p->last_width, h = p->last_height;
+    const size_t n = size_t(w) * h, planes = p->last_words / n;
+    uint32_t* out = static_cast<uint32_t*>(dst);
+    std::vector<uint32_t> pixel(n);
+    for (uint32_t g = 0; g < uint32_t(n); ++g) pixel[g] = plt::path_pixel(w, h, g);
+    for (size_t k = 0; k < planes; ++k)
+        for (size_t g = 0; g < n; ++g) out[k * n + pixel[g]] = paths[k * n + g];
     return PLT_OK;
 }
 

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     __shared__ int stack[kTraversalStackDepth * B];
     const uint32_t li = blockIdx.x * B + threadIdx.x;
     if (li >= fr.n_paths) return;
-    const uint32_t pid = fr.path_base + li;
+    const uint32_t pid = path_pixel(fr.width, fr.height, fr.path_base + li);
     const uint32_t px = pid % fr.width, py = pid / fr.width;
     int* stk = stack + threadIdx.x;
     Rng sg = Rng::make(px, py, fr.frame, fr.sample_generator == 0);
@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(kWfBlock) wf_start_kernel(PtScene sc, PtFrame 
     const uint32_t li = blockIdx.x * kWfBlock + threadIdx.x;
     if (li == 0) w.counters[kWfCount0] = fr.n_paths;
     if (li >= fr.n_paths) return;
-    const uint32_t pid = fr.path_base + li;
+    const uint32_t pid = path_pixel(fr.width, fr.height, fr.path_base + li);
     const uint32_t px = pid % fr.width, py = pid / fr.width;
     Rng sg = Rng::make(px, py, fr.frame, fr.sample_generator == 0);
     const PathState ps = start_path(sc, fr, px, py, pid, sg);
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(kPtBlock) __attribute__((amdgpu_waves_per_eu(4
             load_path(w, li, fr.sample_generator == 0, ps, sg);
             const float4 hv = w.hit[li];
             const Hit h{__float_as_uint(hv.x), hv.y, hv.z, hv.w};
-            cont = sample_vertex(sc, fr, h, li, fr.path_base + li, ps, sg) && ps.len < fr.max_bounces + 1;
+            cont = sample_vertex(sc, fr, h, li, path_pixel(fr.width, fr.height, fr.path_base + li), ps, sg) && ps.len < fr.max_bounces + 1;
             if (cont) store_path(w, li, ps, sg);
         }
         // append the surviving paths to the next queue, one atomic per wave

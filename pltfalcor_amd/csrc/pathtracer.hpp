@@ -67,7 +67,7 @@ struct PtScene {
 
 struct PtFrame {
     uint32_t width, height, frame, max_bounces, hwss, options;
-    uint32_t path_base, n_paths;  // paths [path_base, path_base + n_paths) of this launch (row-major pixels)
+    uint32_t path_base, n_paths;  // paths [path_base, path_base + n_paths) of this launch (pixels: path_pixel)
     float sourcing_area, max_omega;
     // manifold NEE: occluder limit, Newton iteration limit, constraint threshold (Manifolds.slangh:42-54)
     uint32_t mnee_max_occluders, mnee_max_iterations;
@@ -86,6 +86,21 @@ struct PtFrame {
     // solves only the paths whose region overflowed its event buffer (NULL = every path)
     const uint32_t* solve_overflow;
 };
+
+// The row-major pixel of a frame's path g (g = path_base + li). Paths run in 8-row bands, each band
+// column by column (a column of the band's 8 rows, fewer in a last, shorter band), so the 64
+// consecutive paths of a wave, and of a split-pass region, are an 8 x 8 pixel block: their camera
+// rays, hits and shadow rays are coherent, which shortens the wave's traversals. Bounce records
+// and the split pass's regions stay in path order; colour, albedo, normal and the random streams
+// are the pixel's.
+__host__ __device__ inline uint32_t path_pixel(uint32_t width, uint32_t height, uint32_t g) {
+    const uint32_t bandPaths = 8u * width;
+    const uint32_t band = g / bandPaths, r = g - band * bandPaths;
+    const uint32_t rows = height - band * 8u < 8u ? height - band * 8u : 8u;
+    const uint32_t col = r / rows, row = r - col * rows;
+    return (band * 8u + row) * width + col;
+}
+
 
 // Split solve pass. The light-sampling kernel (one 64-lane wave per region of 64 consecutive paths)
 // replays each path's vertices with the path's random stream -- light samples, shadow rays and

@@ -774,7 +774,7 @@ solve_kernel(PtScene sc, PtFrame fr) {
         const uint32_t lane = li & 63u;
         if (((rg[lane < 32u ? kRegOverflowLo : kRegOverflowHi] >> (lane & 31u)) & 1u) == 0u) return;
     }
-    const uint32_t pid = fr.path_base + li;
+    const uint32_t pid = path_pixel(fr.width, fr.height, fr.path_base + li);
     const uint32_t px = pid % fr.width, py = pid / fr.width;
     const size_t n = fr.n_paths;
     int* stk = stack + threadIdx.x;

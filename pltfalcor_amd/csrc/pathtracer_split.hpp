@@ -95,7 +95,7 @@ solve_sample_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
         return true;
     };
     if (li < fr.n_paths) {
-        const uint32_t pid = fr.path_base + li;
+        const uint32_t pid = path_pixel(fr.width, fr.height, fr.path_base + li);
         const uint32_t px = pid % fr.width, py = pid / fr.width;
         const size_t n = fr.n_paths;
         int* stk = stack + lane;
@@ -742,7 +742,7 @@ solve_unwind_kernel(PtScene sc, PtFrame fr, SolveBuffers sb) {
     }
     if (Mode == kUnwindQuery || stage) return;
     const uint32_t li = region * kPtBlock + lane;
-    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sum.r, sum.g, sum.b, 1.f);
+    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[path_pixel(fr.width, fr.height, fr.path_base + li)] = make_float4(sum.r, sum.g, sum.b, 1.f);
 }
 
 // Staged unwind, after the walk kernel: the contributions of the walks that succeeded, one lane per
@@ -800,7 +800,7 @@ __global__ void __launch_bounds__(kPtBlock) solve_sum_kernel(PtFrame fr, SolveBu
         __syncthreads();
     }
     const uint32_t li = region * kPtBlock + lane;
-    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[fr.path_base + li] = make_float4(sum.r, sum.g, sum.b, 1.f);
+    if (li < fr.n_paths && !((ovf >> lane) & 1u)) fr.color[path_pixel(fr.width, fr.height, fr.path_base + li)] = make_float4(sum.r, sum.g, sum.b, 1.f);
 }
 
 // Clears the surrogate's query counters (a kernel, so a captured graph replays it).
