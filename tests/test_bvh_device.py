@@ -159,7 +159,7 @@ def _trace_both(s, pos, idx, rays, brute=True):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n", [1, 3, 4, 5, 17, 1024, 1025, 3000])
 def test_gpu_device_tree_small_soups(gpu, n):
-    """Triangle counts around the leaf size and the single-workgroup finish (1024 clusters): the
+    """Triangle counts around the leaf size and the host-built top (from 1024 clusters): the
     device tree is well formed and casts rays bit-exactly like brute force."""
     from pltfalcor_amd.bsdf import Context
     print(f"device BVH soup n={n}", flush=True)
