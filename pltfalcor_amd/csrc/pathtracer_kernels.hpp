@@ -396,17 +396,18 @@ struct Rec {
     uint32_t lobe, flags;
     float contrib, lastPdf;
 };
+// Non-temporal reads (see ev_get in pathtracer_split.hpp).
 PLT_DEV Rec read_rec(const uint32_t* rec, size_t n, size_t i, int p) {
     const uint32_t* b = rec + size_t(p) * kRecWords * n + i;
     Rec r;
-    r.tri = b[kRecTri * n];
-    r.b1 = __uint_as_float(b[kRecB1 * n]);
-    r.b2 = __uint_as_float(b[kRecB2 * n]);
-    r.wi = oct_decode(__uint_as_float(b[kRecWiX * n]), __uint_as_float(b[kRecWiY * n]));
-    r.lobe = b[kRecLobe * n];
-    r.flags = b[kRecFlags * n];
-    r.contrib = __uint_as_float(b[kRecContrib * n]);
-    r.lastPdf = __uint_as_float(b[kRecLastPdf * n]);
+    r.tri = __builtin_nontemporal_load(b + kRecTri * n);
+    r.b1 = __uint_as_float(__builtin_nontemporal_load(b + kRecB1 * n));
+    r.b2 = __uint_as_float(__builtin_nontemporal_load(b + kRecB2 * n));
+    r.wi = oct_decode(__uint_as_float(__builtin_nontemporal_load(b + kRecWiX * n)), __uint_as_float(__builtin_nontemporal_load(b + kRecWiY * n)));
+    r.lobe = __builtin_nontemporal_load(b + kRecLobe * n);
+    r.flags = __builtin_nontemporal_load(b + kRecFlags * n);
+    r.contrib = __uint_as_float(__builtin_nontemporal_load(b + kRecContrib * n));
+    r.lastPdf = __uint_as_float(__builtin_nontemporal_load(b + kRecLastPdf * n));
     return r;
 }
 PLT_DEV void write_rec(uint32_t* rec, size_t n, size_t i, int p, uint32_t tri, float b1, float b2, V3 wi, uint32_t lobe,
