@@ -684,13 +684,13 @@ def main():
 
     # launch configuration: the autotuner (before the warm-up, outside the timed region) times the
     # sample-pass schedules and block sizes, the solve megakernel against the split solve pass at
-    # several event-area sizes, and graph replay, on two frames of this workload
+    # several event-area sizes, and graph replay, on four frames of this workload
     tune = None
     if args.solve == "split":
         pltpt.set_launch_config(solve_schedule=PLTPT.SOLVE_SPLIT)
     elif args.solve == "autotune":
         t0 = time.perf_counter()
-        best, d_ms, b_ms = pltpt.autotune(WIDTH, HEIGHT, color, albedo, normal, frames=2)
+        best, d_ms, b_ms = pltpt.autotune(WIDTH, HEIGHT, color, albedo, normal, frames=4)
         tune = {"chosen": best, "default_ms_per_frame": d_ms, "best_ms_per_frame": b_ms,
                 "seconds": round(time.perf_counter() - t0, 3)}
     launch = pltpt.launch_config()

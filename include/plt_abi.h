@@ -512,8 +512,9 @@ int plt_pass_set_graph(plt_pass* pass, int enable);
  * current configuration and with each candidate -- the sample megakernel at 64, 128 and 256 threads
  * per block, the wavefront sample schedule at 4, 8, 16 and 32 blocks per CU, compared by the sample
  * pass's own time; then, with the fastest sample pass, the solve megakernel and the split solve
- * pass at 4, 8 and 16 events per path; then the fastest of these replayed from graphs, compared by
- * the frame time -- keeps the fastest (never one slower than the starting configuration as
+ * pass at 4, 8 and 16 events per path; then the fastest of these replayed from graphs (taken only
+ * when more than 1 % faster), compared by the frame time -- keeps the fastest (never one slower
+ * than the starting configuration as
  * measured), and restores the frame counter and the jitter pattern. Writes the chosen configuration
  * and the mean frame time of the starting and the chosen one (ms). */
 int plt_pass_autotune(plt_pass* pass, const plt_render_data* data, uint32_t frames, plt_stream stream,

@@ -901,11 +901,14 @@ int plt_pass_autotune(plt_pass* p, const plt_render_data* rd, uint32_t frames, p
         }
     }
     if (rc >= 0) {  // graph replay of the best launch sequence
+        // Replay saves the launch overhead of about ten launches per frame, well inside the frame
+        // time's run-to-run spread, so it is taken only when it is more than 1 % faster: otherwise
+        // timing noise alone would decide it.
         plt_launch_config g = bc;
         g.graph = 1u;
         double ms = 0.0;
         rc = time_config(g, ms);
-        if (rc >= 0 && ms < bms) { bms = ms; bc = g; }
+        if (rc >= 0 && ms < 0.99 * bms) { bms = ms; bc = g; }
     }
     p->frame = frame0;
     p->pattern = pattern0;
