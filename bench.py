@@ -177,7 +177,7 @@ MLP_FLOP_PER_QUERY = 2 * (22 * 128 + 128 * 128 + 128 * 128 + 128 * 16)  # 75,264
 
 
 def train_surrogate(ctx, n_train: int, epochs: int):
-    """The surrogate trained on n_train exact samples (C3): (surrogate, training record)."""
+    """The surrogate trained on n_train exact samples: (surrogate, training record)."""
     import torch
     from pltfalcor_amd.surrogate import ExactSamples, Surrogate, train
     torch.cuda.synchronize()

@@ -17,7 +17,7 @@
 //      (kept, created) flags compacts the list in curve order and numbers the new nodes. The last
 //      kTopClusters clusters are joined top-down on the host by a full-sweep SAH (greedy clustering
 //      makes its last merges between large clusters, where a split that sees the whole set does
-//      better: the C2 render on the device tree gained 3.6 %);
+//      better: the 1080p render of the 5.24M-triangle scene on the device tree gained 3.6 %);
 //   4. collapse, one launch per level: a 4-wide node starts from its binary node and replaces its
 //      inner child of largest area by that child's two children until it has four; a child of at
 //      most kLeafMax triangles becomes a leaf, whose triangles it writes in leaf order (3 x float4,

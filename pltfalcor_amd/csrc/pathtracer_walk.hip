@@ -2,7 +2,7 @@
 // queue, sized to fill every CU (12 one-wave blocks per CU: three waves per SIMD, each with up to
 // 168 VGPRs), and the clear of the queue's counters. The kernel's time is the latency chain of its
 // longest walks, so it trades occupancy for registers: at four waves per SIMD (128 VGPRs) it
-// spilled, and every Newton iteration paid the spill reloads' round trips (C2 frame: 3.85 ms at
+// spilled, and every Newton iteration paid the spill reloads' round trips (1080p frame, 5.24M triangles: 3.85 ms at
 // four waves, 3.48 ms at three, 3.59 ms at two).
 #include "pathtracer_split.hpp"
 

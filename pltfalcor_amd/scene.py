@@ -25,7 +25,7 @@ C4_SLAB = dict(size=3.6, thickness=0.005, center_y=2.6, ior=1.5)
 
 
 def c4_scene(ctx, aspect: float):
-    """The C2 procedural scene (PATH_2) with the grating plate and the glass slab of C4 added."""
+    """The 1080p workload's procedural scene (PATH_2) with the grating plate and the glass slab added."""
     from . import materials as M
     from .meshes import create_cube, create_quad
     base = Scene(None, aspect=aspect, **PATH_2)
